@@ -1,0 +1,165 @@
+/*
+ * skwarka_rt — MI355X (gfx950) closest-hit / occlusion traversal of Skwarka's ternary triangle
+ * tree and the photon-map kd-tree gather, behind a plain C ABI.
+ *
+ * Every entry point returns an skw_status; skw_last_error() gives the thread-local message of the
+ * last failure. Handles are immutable after creation; calls on different streams may overlap.
+ * Device pointers ("d_" prefix) must be HIP device (or managed) memory; "stream" is a hipStream_t
+ * (NULL = the null stream). Work is stream-ordered: outputs are ready when the stream is.
+ *
+ * Reference interfaces replaced (paths relative to the Skwarka source tree):
+ *   Source/RayLibs/Raytracer/Core/TriangleAccelerator.h:41-62   TriangleAccelerator ctor / Intersect /
+ *                                                              IntersectTest / GetWorldBounds
+ *   Source/RayLibs/Raytracer/Core/Scene.h:77,85,140-148         Scene::Intersect / Scene::IntersectTest
+ *   Source/RayLibs/Raytracer/Core/KDTree.h:45-119               KDTree<T> ctor / GetNearestPoints /
+ *                                                              GetNearestPoint
+ *   Source/RayLibs/Raytracer/LTEIntegrators/PhotonLTEIntegrator/PhotonLTEIntegrator.cpp:127-169, 579-636
+ *                                                              _LookupPhotonIrradiance / _LookupCausticRadiance
+ *   Source/RayLibs/Raytracer/Core/CoreUtils.h:139-149           CoreUtils::GetNextMinT (hit record fields)
+ */
+#ifndef SKWARKA_RT_H
+#define SKWARKA_RT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SKW_OK = 0,
+  SKW_INVALID_ARG = 1,
+  SKW_OUT_OF_MEMORY = 2,
+  SKW_HIP_ERROR = 3,
+  SKW_UNSUPPORTED = 4,
+  SKW_NO_DEVICE = 5
+} skw_status;
+
+/* Ray as the reference's Ray (Source/RayLibs/Math/Ray.h:31-60): origin, direction (normalised by
+ * the caller), [tmin, tmax]. 64 bytes. */
+typedef struct {
+  double o[3];
+  double d[3];
+  double tmin, tmax;
+} skw_ray;
+
+/* Closest hit: primitive index (scene order), triangle index inside the primitive's mesh (after
+ * the mesh drops triangles with out-of-range or repeated vertex indices), ray parameter t, and the
+ * barycentrics b1, b2 of vertices 1 and 2 computed in mesh space. A miss has
+ * prim = tri = 0xFFFFFFFF, t = +inf, b1 = b2 = 0. 32 bytes. */
+typedef struct {
+  uint32_t prim, tri;
+  double t, b1, b2;
+} skw_hit;
+
+/* Hit geometry (TriangleAccelerator.cpp:322-345): world point, geometric and shading normals, and
+ * the mesh-space cross/dot values GetNextMinT uses to keep a spawned ray off its own triangle. */
+typedef struct {
+  double p[3], ng[3], ns[3], cross[3], dot;
+} skw_hit_geom;
+
+/* Scene description. Meshes hold float vertices and uint32 vertex-index triples; a primitive
+ * places a mesh in the world through a mesh_to_world matrix and its inverse (row-major 4x4
+ * doubles, 32 per primitive; NULL = identity for all). A mesh shared by several primitives is
+ * instanced (one level), as in TriangleAccelerator.cpp:54-102. */
+#define SKW_MESH_INVERT_NORMALS 1u /* TriangleMesh m_invert_normals */
+#define SKW_MESH_SHADING_NORMALS 2u /* TriangleMesh m_use_shading_normals: interpolate `normals` */
+typedef struct {
+  uint32_t n_meshes;
+  const uint64_t *mesh_vert_offset; /* n_meshes + 1 */
+  const float *verts;               /* 3 per vertex */
+  const float *normals;             /* 3 per vertex shading normals, or NULL */
+  const uint64_t *mesh_tri_offset;  /* n_meshes + 1 */
+  const uint32_t *tris;             /* 3 per triangle, mesh-local vertex indices */
+  const uint8_t *mesh_flags;        /* per mesh SKW_MESH_* bits, or NULL (all 0) */
+  uint32_t n_prims;
+  const uint32_t *prim_mesh;        /* n_prims */
+  const double *prim_xform;         /* 32 per prim (matrix, inverse), or NULL */
+} skw_scene_desc;
+
+typedef struct skw_scene skw_scene;
+typedef struct skw_photon_map skw_photon_map;
+
+/* Scene flags. */
+#define SKW_SCENE_HOST_ONLY 1u /* build the tree on the host only (no device upload) */
+
+/* TriangleAccelerator::TriangleAccelerator (TriangleAccelerator.cpp:22-147): builds the ternary
+ * tree on the host (same split search, partition and limits) and uploads the flattened arrays to
+ * the current HIP device. */
+skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scene **out);
+skw_status skw_scene_destroy(skw_scene *scene);
+
+/* TriangleAccelerator::GetWorldBounds (TriangleAccelerator.cpp:149-152): min xyz, max xyz. */
+skw_status skw_scene_bounds(const skw_scene *scene, double out_minmax[6]);
+
+/* Counts: [0] nodes, [1] triangles, [2] max depth, [3] instances, [4] leaves, [5] stack depth the
+ * traversal kernels reserve. */
+skw_status skw_scene_info(const skw_scene *scene, uint64_t out[6]);
+
+/* Host copy of the flattened tree in depth-first pre-order (children left, middle, right): per
+ * node bbox (6 floats) and meta (flags, child mask, tri begin, tri end, inst begin, inst end);
+ * per triangle position its (prim, tri) ids (prim = 0xFFFFFFFF for instanced meshes). Any output
+ * may be NULL. Returns the node count through n_nodes. */
+skw_status skw_scene_export_tree(const skw_scene *scene, float *bbox, uint64_t *meta, uint32_t *tri_prim,
+                                 uint32_t *tri_tri, uint64_t *n_nodes);
+
+/* Scene::Intersect / TriangleAccelerator::Intersect + _NodeIntersect (TriangleAccelerator.cpp:154-352),
+ * batched: one hit record per ray. */
+skw_status skw_closest_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
+                           void *stream);
+
+/* skw_closest_hit that also records, per ray, the node boxes tested (d_pops) and the triangles
+ * tested (d_tests), in the traversal order of _NodeIntersect; used for the roofline's algorithmic
+ * byte count and to check visit order. Either counter array may be NULL. */
+skw_status skw_closest_hit_stats(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
+                                 uint32_t *d_pops, uint32_t *d_tests, void *stream);
+
+/* Scene::IntersectTest / _NodeIntersectTest (TriangleAccelerator.cpp:354-468): 1 if occluded. */
+skw_status skw_any_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, uint8_t *d_occluded,
+                       void *stream);
+
+/* Hit geometry for closest-hit records (TriangleAccelerator.cpp:309-345, TriangleMesh.cpp:439-500).
+ * Misses get zeros. */
+skw_status skw_hit_geometry(const skw_scene *scene, const skw_ray *d_rays, const skw_hit *d_hits, uint64_t n,
+                            skw_hit_geom *d_geom, void *stream);
+
+/* Photon map: KDTree<Photon> (KDTree.h:391-473; Photon layout PhotonInternalTypes.h:35-59).
+ * dir / normal are CompressedDirection ids. Built on the host (libstdc++ nth_element median split),
+ * uploaded in tree order; results refer to the caller's photon indices. */
+skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
+                                 const uint16_t *normal, uint32_t flags, skw_photon_map **out);
+skw_status skw_photon_map_destroy(skw_photon_map *map);
+
+/* Tree-order permutation and packed nodes (split float, flags/right-child word) on the host. */
+skw_status skw_photon_map_export(const skw_photon_map *map, uint32_t *perm, float *split, uint32_t *word);
+
+/* PhotonLTEIntegrator::_LookupPhotonIrradiance (PhotonLTEIntegrator.cpp:127-169): for each query point
+ * and unit normal (3 doubles each), up to k nearest photons inside radius r that pass PhotonFilter
+ * (PhotonInternalTypes.h:113-142), then the disc estimate split by side. Optional d_ids / d_d2
+ * (k per query) receive the photon set in the bounded max-heap's final order, d_count the set size.
+ * d_radiance (optional, 3 floats) gets rho/pi * E_ext for a Lambertian surface of albedo rho. */
+skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                 uint64_t nq, uint32_t k, double r, uint64_t paths, double rho, float *d_e_ext,
+                                 float *d_e_int, float *d_radiance, uint32_t *d_ids, double *d_d2,
+                                 uint32_t *d_count, void *stream);
+
+/* KDTree::GetNearestPoints with or without PhotonFilter (filter = 0 none, 1 PhotonFilter at the
+ * query normal). d_normals may be NULL when filter = 0. */
+skw_status skw_photon_knn(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                          uint32_t k, double r, int filter, uint32_t *d_ids, double *d_d2, uint32_t *d_count,
+                          void *stream);
+
+/* KDTree::GetNearestPoint (KDTree.h:484-503) with the same filter choice; 0xFFFFFFFF = none. */
+skw_status skw_photon_nearest(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                              uint64_t nq, double r, int filter, uint32_t *d_id, void *stream);
+
+/* Thread-local text of the last error. */
+const char *skw_last_error(void);
+
+/* Number of HIP devices visible (0 when there is none). */
+int skw_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

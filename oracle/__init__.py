@@ -1,0 +1,285 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of the CPU restatement of Skwarka's ray/photon hot path (oracle/src, built into
+oracle/_build/liborc.so by oracle/Makefile). Only tests/, __graft_entry__.smoke() and the CPU
+baseline leg of bench.py import this module; it is the checker, never the measured product.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("tri", "<u4"), ("t", "<f8"), ("b1", "<f8"), ("b2", "<f8")])
+GEOM_DTYPE = np.dtype([("p", "<f8", (3,)), ("ng", "<f8", (3,)), ("ns", "<f8", (3,)),
+                       ("cross", "<f8", (3,)), ("dot", "<f8")])
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_lib = None
+
+
+def build():
+    res = subprocess.run(["make", "-C", _HERE, "-s"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "orc_accel_create": ([_U32, _P, _P, _P, _P, _P, _P, _U32, _P, _P], _P),
+        "orc_accel_destroy": ([_P], None),
+        "orc_accel_info": ([_P, _P], None),
+        "orc_accel_bounds": ([_P, _P], None),
+        "orc_accel_dump": ([_P, _P, _P, _P, _P], _U64),
+        "orc_closest_hit": ([_P, _U64, _P, _P, _P, _P, _P, ctypes.c_int], None),
+        "orc_any_hit": ([_P, _U64, _P, _P, ctypes.c_int], None),
+        "orc_brute_closest": ([_P, _U64, _P, _P, ctypes.c_int], None),
+        "orc_next_min_t": ([_P, _U32, _P, _P], ctypes.c_double),
+        "orc_kdtree_create_d": ([_U64, _P], _P),
+        "orc_kdtree_create_f": ([_U64, _P], _P),
+        "orc_kdtree_destroy": ([_P], None),
+        "orc_kdtree_dump": ([_P, _P, _P, _P], None),
+        "orc_kdtree_knn": ([_P, _U64, _P, _P, _P, _U32, ctypes.c_int, _P, _P, _P, ctypes.c_int], None),
+        "orc_kdtree_nearest": ([_P, _U64, _P, _P, ctypes.c_int, _P, ctypes.c_int], None),
+        "orc_photon_map_create": ([_U64, _P, _P, _P, _P], _P),
+        "orc_photon_map_destroy": ([_P], None),
+        "orc_photon_map_dump": ([_P, _P, _P, _P], None),
+        "orc_photon_irradiance": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, _P, _P, _P, _P, _P, ctypes.c_int],
+                                  None),
+        "orc_photon_caustic": ([_P, _U64, _P, _P, _P, _U32, ctypes.c_double, _U64, ctypes.c_double, _P,
+                                ctypes.c_int], None),
+        "orc_photon_cone": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, ctypes.c_double, _P, ctypes.c_int],
+                            None),
+        "orc_cdir_encode": ([ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_uint16),
+        "orc_cdir_table": ([_P], None),
+        "orc_mt19937": ([_U32, _U64, _P], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+class Accel:
+    """TriangleAccelerator restatement over the same mesh/primitive description as the product."""
+
+    def __init__(self, meshes, prims=None):
+        from skwarka_amd import mesh_arrays
+        L = lib()
+        if prims is None:
+            prims = [(i, None) for i in range(len(meshes))]
+        a = mesh_arrays(meshes)
+        self._keep = a
+        pm = np.array([p[0] for p in prims], np.uint32)
+        xf = None
+        if any(p[1] is not None for p in prims):
+            xf = np.zeros((len(prims), 32), np.float64)
+            for i, (_, m) in enumerate(prims):
+                m = np.eye(4) if m is None else np.asarray(m, np.float64)
+                xf[i, :16] = m.reshape(-1)
+                xf[i, 16:] = np.linalg.inv(m).reshape(-1)
+        self._pm, self._xf = pm, xf
+        self.h = L.orc_accel_create(len(meshes), _ptr(a["vert_off"]), _ptr(a["verts"]), _ptr(a["normals"]),
+                                    _ptr(a["tri_off"]), _ptr(a["tris"]), _ptr(a["flags"]), len(prims), _ptr(pm),
+                                    _ptr(xf))
+
+    def __del__(self):
+        try:
+            lib().orc_accel_destroy(self.h)
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(5, np.uint64)
+        lib().orc_accel_info(self.h, out.ctypes.data)
+        return dict(zip(["nodes", "triangles", "max_depth", "instances", "leaves"], [int(x) for x in out]))
+
+    def bounds(self):
+        out = np.zeros(6, np.float64)
+        lib().orc_accel_bounds(self.h, out.ctypes.data)
+        return out
+
+    def dump(self):
+        info = self.info()
+        n = info["nodes"]
+        bbox = np.zeros((n, 6), np.float32)
+        meta = np.zeros((n, 6), np.uint64)
+        tp = np.zeros(info["triangles"], np.uint32)
+        tt = np.zeros(info["triangles"], np.uint32)
+        k = lib().orc_accel_dump(self.h, bbox.ctypes.data, meta.ctypes.data, tp.ctypes.data, tt.ctypes.data)
+        return bbox[:k], meta[:k], tp, tt
+
+    def closest_hit(self, rays, geom=False, counters=False, nthreads=None):
+        rays = np.ascontiguousarray(rays)
+        n = len(rays)
+        hits = np.zeros(n, HIT_DTYPE)
+        g = np.zeros(n, GEOM_DTYPE) if geom else None
+        pops = np.zeros(n, np.uint32) if counters else None
+        tests = np.zeros(n, np.uint32) if counters else None
+        lib().orc_closest_hit(self.h, n, rays.ctypes.data, hits.ctypes.data, _ptr(g), _ptr(pops), _ptr(tests),
+                              nthreads or threads())
+        out = [hits]
+        if geom:
+            out.append(g)
+        if counters:
+            out += [pops, tests]
+        return out[0] if len(out) == 1 else tuple(out)
+
+    def any_hit(self, rays, nthreads=None):
+        rays = np.ascontiguousarray(rays)
+        out = np.zeros(len(rays), np.uint8)
+        lib().orc_any_hit(self.h, len(rays), rays.ctypes.data, out.ctypes.data, nthreads or threads())
+        return out
+
+    def brute_closest(self, rays, nthreads=None):
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        lib().orc_brute_closest(self.h, len(rays), rays.ctypes.data, hits.ctypes.data, nthreads or threads())
+        return hits
+
+    def next_min_t(self, prim, geom_rec, direction):
+        g = np.ascontiguousarray(np.array(geom_rec, GEOM_DTYPE).reshape(1))
+        d = np.ascontiguousarray(np.asarray(direction, np.float64))
+        return lib().orc_next_min_t(self.h, int(prim), g.ctypes.data, d.ctypes.data)
+
+
+class KDTree:
+    """KDTree<Point3D_d> (double points) or KDTree<Point3D_f> (float points)."""
+
+    def __init__(self, pts):
+        pts = np.ascontiguousarray(pts)
+        self.n = len(pts)
+        self._pts = pts
+        if pts.dtype == np.float64:
+            self.h = lib().orc_kdtree_create_d(self.n, pts.ctypes.data)
+        else:
+            pts = np.ascontiguousarray(pts, np.float32)
+            self._pts = pts
+            self.h = lib().orc_kdtree_create_f(self.n, pts.ctypes.data)
+
+    def __del__(self):
+        try:
+            lib().orc_kdtree_destroy(self.h)
+        except Exception:
+            pass
+
+    def dump(self):
+        perm = np.zeros(self.n, np.uint32)
+        split = np.zeros(self.n, np.float32)
+        word = np.zeros(self.n, np.uint32)
+        lib().orc_kdtree_dump(self.h, perm.ctypes.data, split.ctypes.data, word.ctypes.data)
+        return perm, split, word
+
+    def knn(self, q, radius, k, filt=0, nthreads=None):
+        q = np.ascontiguousarray(q, np.float64)
+        nq = len(q)
+        radius = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float64), (nq,)))
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
+        kmax = int(k.max()) if nq else 0
+        ids = np.zeros((nq, max(kmax, 1)), np.uint32)
+        d2 = np.zeros((nq, max(kmax, 1)), np.float64)
+        cnt = np.zeros(nq, np.uint32)
+        lib().orc_kdtree_knn(self.h, nq, q.ctypes.data, radius.ctypes.data, k.ctypes.data, max(kmax, 1), filt,
+                             ids.ctypes.data, d2.ctypes.data, cnt.ctypes.data, nthreads or threads())
+        return ids, d2, cnt
+
+    def nearest(self, q, radius, filt=0, nthreads=None):
+        q = np.ascontiguousarray(q, np.float64)
+        nq = len(q)
+        radius = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float64), (nq,)))
+        out = np.zeros(nq, np.uint32)
+        lib().orc_kdtree_nearest(self.h, nq, q.ctypes.data, radius.ctypes.data, filt, out.ctypes.data,
+                                 nthreads or threads())
+        return out
+
+
+class PhotonMap:
+    def __init__(self, pos, weight, dir_ids, normal_ids):
+        self._pos = np.ascontiguousarray(pos, np.float32)
+        self._w = np.ascontiguousarray(weight, np.float32)
+        self._d = np.ascontiguousarray(dir_ids, np.uint16)
+        self._n = np.ascontiguousarray(normal_ids, np.uint16)
+        self.n = len(self._pos)
+        self.h = lib().orc_photon_map_create(self.n, self._pos.ctypes.data, self._w.ctypes.data, self._d.ctypes.data,
+                                             self._n.ctypes.data)
+
+    def __del__(self):
+        try:
+            lib().orc_photon_map_destroy(self.h)
+        except Exception:
+            pass
+
+    def dump(self):
+        perm = np.zeros(self.n, np.uint32)
+        split = np.zeros(self.n, np.float32)
+        word = np.zeros(self.n, np.uint32)
+        lib().orc_photon_map_dump(self.h, perm.ctypes.data, split.ctypes.data, word.ctypes.data)
+        return perm, split, word
+
+    def irradiance(self, p, n, k, r, paths=1, with_sets=True, nthreads=None):
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        nq = len(p)
+        e_ext = np.zeros((nq, 3), np.float32)
+        e_int = np.zeros((nq, 3), np.float32)
+        ids = np.zeros((nq, k), np.uint32) if with_sets else None
+        d2 = np.zeros((nq, k), np.float64) if with_sets else None
+        cnt = np.zeros(nq, np.uint32)
+        lib().orc_photon_irradiance(self.h, nq, p.ctypes.data, n.ctypes.data, k, r, paths, e_ext.ctypes.data,
+                                    e_int.ctypes.data, _ptr(ids), _ptr(d2), cnt.ctypes.data, nthreads or threads())
+        return e_ext, e_int, ids, d2, cnt
+
+    def caustic(self, p, n, wo, k, r, paths, rho, nthreads=None):
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        wo = np.ascontiguousarray(wo, np.float64)
+        out = np.zeros((len(p), 3), np.float64)
+        lib().orc_photon_caustic(self.h, len(p), p.ctypes.data, n.ctypes.data, wo.ctypes.data, k, r, paths, rho,
+                                 out.ctypes.data, nthreads or threads())
+        return out
+
+    def cone(self, p, n, k, r, paths, kc, nthreads=None):
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        out = np.zeros((len(p), 3), np.float32)
+        lib().orc_photon_cone(self.h, len(p), p.ctypes.data, n.ctypes.data, k, r, paths, kc, out.ctypes.data,
+                              nthreads or threads())
+        return out
+
+
+def cdir_encode(x, y, z):
+    return int(lib().orc_cdir_encode(x, y, z))
+
+
+def cdir_table():
+    out = np.zeros((65536, 3), np.float64)
+    lib().orc_cdir_table(out.ctypes.data)
+    return out
+
+
+def mt19937(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_mt19937(seed, n, out.ctypes.data)
+    return out

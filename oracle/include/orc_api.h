@@ -1,0 +1,82 @@
+/* ORACLE — TEST INFRASTRUCTURE ONLY.
+ * C interface of the CPU restatement of Skwarka's ray/photon hot path. Only the tests, smoke()
+ * and the CPU-baseline leg of bench.py load it (through ctypes); the product library never does.
+ * Each function names the reference code it restates in its source file. */
+#ifndef ORC_API_H
+#define ORC_API_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_accel orc_accel;
+typedef struct orc_kdtree orc_kdtree;
+
+/* Closest-hit record: ids after mesh cleaning, t from traversal, b1/b2 in mesh space. */
+typedef struct {
+  uint32_t prim, tri;
+  double t, b1, b2;
+} orc_hit;
+
+/* World-space hit geometry plus the mesh-space values GetNextMinT needs. */
+typedef struct {
+  double p[3], ng[3], ns[3], cross[3], dot;
+} orc_hit_geom;
+
+orc_accel *orc_accel_create(uint32_t n_meshes, const uint64_t *mesh_vert_off, const float *verts,
+                            const float *normals, const uint64_t *mesh_tri_off, const uint32_t *tris,
+                            const uint8_t *mesh_flags, uint32_t n_prims, const uint32_t *prim_mesh,
+                            const double *prim_xf);
+void orc_accel_destroy(orc_accel *h);
+void orc_accel_info(const orc_accel *h, uint64_t *out5);
+void orc_accel_bounds(const orc_accel *h, double *out6);
+uint64_t orc_accel_dump(const orc_accel *h, float *bbox, uint64_t *meta, uint32_t *tri_prim, uint32_t *tri_tri);
+void orc_closest_hit(const orc_accel *h, uint64_t n, const double *rays, orc_hit *hits, orc_hit_geom *geom,
+                     uint32_t *pops, uint32_t *tests, int nthreads);
+void orc_any_hit(const orc_accel *h, uint64_t n, const double *rays, uint8_t *out, int nthreads);
+void orc_brute_closest(const orc_accel *h, uint64_t n, const double *rays, orc_hit *hits, int nthreads);
+double orc_next_min_t(const orc_accel *h, uint32_t prim, const orc_hit_geom *g, const double *dir3);
+
+/* Generic point kd-tree (KDTree<Point3D_d> / KDTree<Point3D_f>) for the reference's own tests. */
+orc_kdtree *orc_kdtree_create_d(uint64_t n, const double *pts);
+orc_kdtree *orc_kdtree_create_f(uint64_t n, const float *pts);
+void orc_kdtree_destroy(orc_kdtree *h);
+/* Tree-order permutation: perm[i] = original index of the point at tree position i; nodes as
+ * the reference packs them (split float, 32-bit flags/right-child word). */
+void orc_kdtree_dump(const orc_kdtree *h, uint32_t *perm, float *split, uint32_t *word);
+/* filter: 0 = none, 1 = the reference test's CustomPointFilter (odd (int)(x+y+z)). Outputs
+ * original point ids and d^2 in the processor's final array order; returns per-query count. */
+void orc_kdtree_knn(const orc_kdtree *h, uint64_t nq, const double *q, const double *radius, const uint32_t *k,
+                    uint32_t kmax, int filter, uint32_t *ids, double *d2, uint32_t *count, int nthreads);
+void orc_kdtree_nearest(const orc_kdtree *h, uint64_t nq, const double *q, const double *radius, int filter,
+                        uint32_t *id, int nthreads);
+
+/* Photon map: KDTree<Photon> with PhotonFilter and the density estimators. */
+typedef struct orc_photon_map orc_photon_map;
+orc_photon_map *orc_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
+                                      const uint16_t *normal);
+void orc_photon_map_destroy(orc_photon_map *h);
+void orc_photon_map_dump(const orc_photon_map *h, uint32_t *perm, float *split, uint32_t *word);
+/* _LookupPhotonIrradiance: k nearest with PhotonFilter inside r; E_ext/E_int float RGB. */
+void orc_photon_irradiance(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
+                           double r, uint64_t paths, float *e_ext, float *e_int, uint32_t *ids, double *d2,
+                           uint32_t *count, int nthreads);
+/* _LookupCausticRadiance with a Lambertian BRDF of albedo rho (reflection side only). */
+void orc_photon_caustic(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, const double *wo,
+                        uint32_t k, double r, uint64_t paths, double rho, double *radiance, int nthreads);
+/* Cone-filtered irradiance (Jensen: w = 1 - d/(kc*r_max), normalised by 1 - 2/(3 kc)). */
+void orc_photon_cone(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
+                     double r, uint64_t paths, double kc, float *e_ext, int nthreads);
+
+/* CompressedDirection codec (Source/RayLibs/Math/CompressedDirection.{h,cpp}). */
+uint16_t orc_cdir_encode(double x, double y, double z);
+void orc_cdir_table(double *out_65536x3);
+
+/* Raw 32-bit outputs of std::mt19937 (boost::mt19937 is the same engine). */
+void orc_mt19937(uint32_t seed, uint64_t n, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

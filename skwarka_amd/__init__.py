@@ -1,0 +1,261 @@
+"""Python access to libskwarka_rt (include/skwarka_rt.h) through ctypes.
+
+The library is the product: host tree builders, gfx950 kernels and the C ABI. This module only
+loads it, describes its record layouts as numpy dtypes and wraps the handles. Device buffers are
+passed as integer device addresses (for example torch CUDA tensors' data_ptr()). There is no CPU
+path here: loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libskwarka_rt.so")
+
+RAY_DTYPE = np.dtype([("o", "<f8", (3,)), ("d", "<f8", (3,)), ("tmin", "<f8"), ("tmax", "<f8")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("tri", "<u4"), ("t", "<f8"), ("b1", "<f8"), ("b2", "<f8")])
+GEOM_DTYPE = np.dtype([("p", "<f8", (3,)), ("ng", "<f8", (3,)), ("ns", "<f8", (3,)),
+                       ("cross", "<f8", (3,)), ("dot", "<f8")])
+assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 32 and GEOM_DTYPE.itemsize == 104
+
+MISS = 0xFFFFFFFF
+SCENE_HOST_ONLY = 1
+
+# Every symbol include/skwarka_rt.h declares.
+EXPORTS = [
+    "skw_scene_create", "skw_scene_destroy", "skw_scene_bounds", "skw_scene_info",
+    "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
+    "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
+    "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_last_error",
+    "skw_device_count",
+]
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_meshes", _U32), ("mesh_vert_offset", _P), ("verts", _P), ("normals", _P),
+        ("mesh_tri_offset", _P), ("tris", _P), ("mesh_flags", _P),
+        ("n_prims", _U32), ("prim_mesh", _P), ("prim_xform", _P),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libskwarka_rt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libskwarka_rt.so is not built (run __graft_entry__.build() or "
+                           "python skwarka_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "skw_scene_create": ([ctypes.POINTER(SceneDesc), _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_scene_destroy": ([_P], ctypes.c_int),
+        "skw_scene_bounds": ([_P, _P], ctypes.c_int),
+        "skw_scene_info": ([_P, _P], ctypes.c_int),
+        "skw_scene_export_tree": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_closest_hit": ([_P, _P, _U64, _P, _P], ctypes.c_int),
+        "skw_closest_hit_stats": ([_P, _P, _U64, _P, _P, _P, _P], ctypes.c_int),
+        "skw_any_hit": ([_P, _P, _U64, _P, _P], ctypes.c_int),
+        "skw_hit_geometry": ([_P, _P, _P, _U64, _P, _P], ctypes.c_int),
+        "skw_photon_map_create": ([_U64, _P, _P, _P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_photon_map_destroy": ([_P], ctypes.c_int),
+        "skw_photon_map_export": ([_P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_irradiance": ([_P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double,
+                                   _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_knn": ([_P, _P, _P, _U64, _U32, ctypes.c_double, ctypes.c_int, _P, _P, _P, _P],
+                           ctypes.c_int),
+        "skw_photon_nearest": ([_P, _P, _P, _U64, ctypes.c_double, ctypes.c_int, _P, _P], ctypes.c_int),
+        "skw_last_error": ([], ctypes.c_char_p),
+        "skw_device_count": ([], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+class SkwError(RuntimeError):
+    pass
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = lib().skw_last_error().decode(errors="replace")
+        raise SkwError("%s failed (status %d): %s" % (what, status, msg))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+MESH_INVERT_NORMALS = 1
+MESH_SHADING_NORMALS = 2
+
+
+def mesh_arrays(meshes):
+    """Flattens a list of mesh dicts into the arrays of skw_scene_desc (also used for the oracle)."""
+    vo = np.zeros(len(meshes) + 1, np.uint64)
+    to = np.zeros(len(meshes) + 1, np.uint64)
+    for i, m in enumerate(meshes):
+        vo[i + 1] = vo[i] + len(m["verts"])
+        to[i + 1] = to[i] + len(m["tris"])
+    verts = np.ascontiguousarray(np.concatenate([np.asarray(m["verts"], np.float32).reshape(-1, 3) for m in meshes])
+                                 if meshes else np.zeros((0, 3), np.float32))
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(m["tris"], np.uint32).reshape(-1, 3) for m in meshes])
+                                if meshes else np.zeros((0, 3), np.uint32))
+    flags = np.zeros(len(meshes), np.uint8)
+    normals = None
+    if any(m.get("normals") is not None for m in meshes):
+        parts = []
+        for i, m in enumerate(meshes):
+            if m.get("normals") is not None:
+                parts.append(np.asarray(m["normals"], np.float32).reshape(-1, 3))
+                flags[i] |= MESH_SHADING_NORMALS
+            else:
+                parts.append(np.zeros((len(m["verts"]), 3), np.float32))
+        normals = np.ascontiguousarray(np.concatenate(parts))
+    for i, m in enumerate(meshes):
+        if m.get("invert_normals"):
+            flags[i] |= MESH_INVERT_NORMALS
+    return {"vert_off": vo, "tri_off": to, "verts": verts, "tris": tris, "normals": normals, "flags": flags}
+
+
+class Scene:
+    """A scene handle: the ternary triangle tree of TriangleAccelerator, built on the host and
+    (unless host_only) uploaded to the current HIP device.
+
+    meshes: list of dicts with 'verts' (V,3) float32, 'tris' (T,3) uint32, optional 'normals'
+            (V,3) float32 (shading normals, used when present) and 'invert_normals' bool.
+    prims:  list of (mesh index, 4x4 mesh_to_world or None). None for all prims = identity.
+    """
+
+    def __init__(self, meshes, prims=None, host_only=False):
+        L = lib()
+        if prims is None:
+            prims = [(i, None) for i in range(len(meshes))]
+        arrays = mesh_arrays(meshes)
+        vo, to = arrays["vert_off"], arrays["tri_off"]
+        self._verts, self._tris, self._normals, self._flags = (arrays["verts"], arrays["tris"], arrays["normals"],
+                                                               arrays["flags"])
+        self._vo, self._to = vo, to
+        self._pm = np.array([p[0] for p in prims], np.uint32)
+        self._xf = None
+        if any(p[1] is not None for p in prims):
+            xf = np.zeros((len(prims), 32), np.float64)
+            for i, (_, m) in enumerate(prims):
+                m = np.eye(4) if m is None else np.asarray(m, np.float64)
+                xf[i, :16] = m.reshape(-1)
+                xf[i, 16:] = np.linalg.inv(m).reshape(-1)
+            self._xf = xf
+        desc = SceneDesc(len(meshes), _ptr(vo), _ptr(self._verts), _ptr(self._normals), _ptr(to),
+                         _ptr(self._tris), _ptr(self._flags), len(prims), _ptr(self._pm), _ptr(self._xf))
+        h = _P()
+        check(L.skw_scene_create(ctypes.byref(desc), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
+              "skw_scene_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().skw_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(6, np.uint64)
+        check(lib().skw_scene_info(self.handle, out.ctypes.data), "skw_scene_info")
+        keys = ["nodes", "triangles", "max_depth", "instances", "leaves", "stack_need"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def bounds(self):
+        out = np.zeros(6, np.float64)
+        check(lib().skw_scene_bounds(self.handle, out.ctypes.data), "skw_scene_bounds")
+        return out
+
+    def export_tree(self):
+        info = self.info()
+        n = info["nodes"]
+        bbox = np.zeros((n, 6), np.float32)
+        meta = np.zeros((n, 6), np.uint64)
+        tp = np.zeros(info["triangles"], np.uint32)
+        tt = np.zeros(info["triangles"], np.uint32)
+        k = _U64()
+        check(lib().skw_scene_export_tree(self.handle, bbox.ctypes.data, meta.ctypes.data, tp.ctypes.data,
+                                          tt.ctypes.data, ctypes.byref(k)), "skw_scene_export_tree")
+        return bbox[:k.value], meta[:k.value], tp, tt
+
+    def closest_hit(self, d_rays, n, d_hits, stream=0):
+        check(lib().skw_closest_hit(self.handle, d_rays, n, d_hits, stream or None), "skw_closest_hit")
+
+    def closest_hit_stats(self, d_rays, n, d_hits, d_pops, d_tests, stream=0):
+        check(lib().skw_closest_hit_stats(self.handle, d_rays, n, d_hits, d_pops, d_tests, stream or None),
+              "skw_closest_hit_stats")
+
+    def any_hit(self, d_rays, n, d_occ, stream=0):
+        check(lib().skw_any_hit(self.handle, d_rays, n, d_occ, stream or None), "skw_any_hit")
+
+    def hit_geometry(self, d_rays, d_hits, n, d_geom, stream=0):
+        check(lib().skw_hit_geometry(self.handle, d_rays, d_hits, n, d_geom, stream or None), "skw_hit_geometry")
+
+
+class PhotonMap:
+    """KDTree<Photon>: pos (N,3) float32, weight (N,3) float32, dir/normal CompressedDirection ids."""
+
+    def __init__(self, pos, weight, dir_ids, normal_ids, host_only=False):
+        L = lib()
+        self._pos = np.ascontiguousarray(pos, np.float32)
+        self._w = np.ascontiguousarray(weight, np.float32)
+        self._d = np.ascontiguousarray(dir_ids, np.uint16)
+        self._n = np.ascontiguousarray(normal_ids, np.uint16)
+        self.n = len(self._pos)
+        h = _P()
+        check(L.skw_photon_map_create(self.n, _ptr(self._pos), _ptr(self._w), _ptr(self._d), _ptr(self._n),
+                                      SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)), "skw_photon_map_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().skw_photon_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self):
+        perm = np.zeros(self.n, np.uint32)
+        split = np.zeros(self.n, np.float32)
+        word = np.zeros(self.n, np.uint32)
+        check(lib().skw_photon_map_export(self.handle, perm.ctypes.data, split.ctypes.data, word.ctypes.data),
+              "skw_photon_map_export")
+        return perm, split, word
+
+    def irradiance(self, d_p, d_n, nq, k, r, paths, rho, d_eext, d_eint, d_rad=None, d_ids=None, d_d2=None,
+                   d_count=None, stream=0):
+        check(lib().skw_photon_irradiance(self.handle, d_p, d_n, nq, k, r, paths, rho, d_eext, d_eint, d_rad, d_ids,
+                                          d_d2, d_count, stream or None), "skw_photon_irradiance")
+
+    def knn(self, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream=0):
+        check(lib().skw_photon_knn(self.handle, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream or None),
+              "skw_photon_knn")
+
+    def nearest(self, d_p, d_n, nq, r, filt, d_id, stream=0):
+        check(lib().skw_photon_nearest(self.handle, d_p, d_n, nq, r, filt, d_id, stream or None),
+              "skw_photon_nearest")

@@ -1,0 +1,446 @@
+// The C ABI of skwarka_rt (include/skwarka_rt.h): handle ownership, device uploads, argument
+// checks, error reporting, and kernel launches. There is no CPU fallback: every query runs on the
+// HIP device, and a call without a device fails with SKW_NO_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/skwarka_rt.h"
+#include "host/builders.hpp"
+#include "kernels/kernels.hpp"
+
+using namespace skw;
+
+namespace {
+
+thread_local std::string g_err;
+
+skw_status fail(skw_status s, const std::string &msg) {
+  g_err = msg;
+  return s;
+}
+
+skw_status hip_fail(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? SKW_OUT_OF_MEMORY : SKW_HIP_ERROR,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SKW_HIP(call, what)                     \
+  do {                                          \
+    hipError_t e_ = (call);                     \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+  *dst = nullptr;
+  if (src.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int env_int(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return std::atoi(v);
+}
+
+// Launch-wide ray counters, one per stream so that calls on different streams never share one.
+struct CounterPool {
+  std::mutex mu;
+  std::map<void *, uint64_t *> by_stream;
+  ~CounterPool() {
+    for (auto &kv : by_stream) (void)hipFree(kv.second);
+  }
+  hipError_t get(void *stream, uint64_t **out) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = by_stream.find(stream);
+    if (it != by_stream.end()) {
+      *out = it->second;
+      return hipSuccess;
+    }
+    uint64_t *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, sizeof(uint64_t));
+    if (e != hipSuccess) return e;
+    by_stream[stream] = p;
+    *out = p;
+    return hipSuccess;
+  }
+};
+
+}  // namespace
+
+struct skw_scene {
+  TreeLayout host;
+  bool on_device = false;
+  bool has_inst = false;
+  bool needs_fix = false;  // some primitive's traversal barycentrics are not mesh-space values
+  int device = 0;
+  int cus = 256;
+  DevNode *d_nodes = nullptr;
+  DevTri *d_tris = nullptr;
+  DevInst *d_insts = nullptr;
+  uint32_t *d_inst_range = nullptr;
+  DevPrim *d_prims = nullptr;
+  DevMesh *d_meshes = nullptr;
+  float *d_mesh_verts = nullptr;
+  float *d_mesh_normals = nullptr;
+  uint32_t *d_mesh_tris = nullptr;
+  CounterPool counters;
+
+  ~skw_scene() {
+    void *ptrs[] = {d_nodes, d_tris, d_insts, d_inst_range, d_prims, d_meshes, d_mesh_verts, d_mesh_normals, d_mesh_tris};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+struct skw_photon_map {
+  KdLayout host;
+  bool on_device = false;
+  int device = 0;
+  DevKdNode *d_nodes = nullptr;
+  DevPhoton *d_photons = nullptr;
+  uint32_t *d_perm = nullptr;
+  double *d_cdir = nullptr;
+  ~skw_photon_map() {
+    void *ptrs[] = {d_nodes, d_photons, d_perm, d_cdir};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+const char *skw_last_error(void) { return g_err.c_str(); }
+
+int skw_device_count(void) { return device_count(); }
+
+skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scene **out) {
+  if (!desc || !out) return fail(SKW_INVALID_ARG, "desc and out must not be NULL");
+  *out = nullptr;
+  bool host_only = (flags & SKW_SCENE_HOST_ONLY) != 0;
+  if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
+  std::unique_ptr<skw_scene> s(new skw_scene());
+  std::string err;
+  if (!build_triangle_tree(*desc, s->host, err)) return fail(SKW_INVALID_ARG, err);
+  s->has_inst = !s->host.insts.empty();
+  std::vector<uint32_t> users(s->host.meshes.size(), 0);
+  for (const HostPrim &p : s->host.prims) users[p.mesh]++;
+  std::vector<DevPrim> prims(s->host.prims.size());
+  for (size_t i = 0; i < prims.size(); ++i) {
+    const HostPrim &hp = s->host.prims[i];
+    std::memcpy(prims[i].m2w, hp.m2w, sizeof(hp.m2w));
+    std::memcpy(prims[i].w2m, hp.w2m, sizeof(hp.w2m));
+    prims[i].mesh = hp.mesh;
+    // Traversal barycentrics equal the mesh-space ones for identity transforms and for
+    // instanced meshes (whose subtrees are walked in mesh space).
+    prims[i].identity = (hp.identity || users[hp.mesh] > 1) ? 1u : 0u;
+    prims[i].pad[0] = prims[i].pad[1] = 0;
+    if (!prims[i].identity) s->needs_fix = true;
+  }
+  if (host_only) {
+    *out = s.release();
+    return SKW_OK;
+  }
+  SKW_HIP(hipGetDevice(&s->device), "hipGetDevice");
+  SKW_HIP(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device), "hipDeviceGetAttribute");
+  std::vector<DevMesh> meshes(s->host.meshes.size());
+  std::vector<float> mv, mn;
+  std::vector<uint32_t> mt;
+  bool any_normals = false;
+  for (const HostMesh &m : s->host.meshes) any_normals |= !m.normals.empty();
+  for (size_t i = 0; i < meshes.size(); ++i) {
+    const HostMesh &m = s->host.meshes[i];
+    meshes[i].vert_off = mv.size() / 3;
+    meshes[i].tri_off = mt.size() / 3;
+    meshes[i].has_normals = m.normals.empty() ? 0u : 1u;
+    meshes[i].invert_normals = m.invert_normals ? 1u : 0u;
+    mv.insert(mv.end(), m.verts.begin(), m.verts.end());
+    if (any_normals) {
+      if (m.normals.empty())
+        mn.insert(mn.end(), m.verts.size(), 0.0f);
+      else
+        mn.insert(mn.end(), m.normals.begin(), m.normals.end());
+    }
+    mt.insert(mt.end(), m.tris.begin(), m.tris.end());
+  }
+  SKW_HIP(upload(&s->d_nodes, s->host.nodes), "upload nodes");
+  SKW_HIP(upload(&s->d_tris, s->host.tris), "upload triangles");
+  SKW_HIP(upload(&s->d_insts, s->host.insts), "upload instances");
+  SKW_HIP(upload(&s->d_inst_range, s->host.inst_range), "upload instance ranges");
+  SKW_HIP(upload(&s->d_prims, prims), "upload primitives");
+  SKW_HIP(upload(&s->d_meshes, meshes), "upload meshes");
+  SKW_HIP(upload(&s->d_mesh_verts, mv), "upload mesh vertices");
+  SKW_HIP(upload(&s->d_mesh_normals, mn), "upload mesh normals");
+  SKW_HIP(upload(&s->d_mesh_tris, mt), "upload mesh triangles");
+  s->on_device = true;
+  *out = s.release();
+  return SKW_OK;
+}
+
+skw_status skw_scene_destroy(skw_scene *scene) {
+  delete scene;
+  return SKW_OK;
+}
+
+skw_status skw_scene_bounds(const skw_scene *scene, double out[6]) {
+  if (!scene || !out) return fail(SKW_INVALID_ARG, "scene and out must not be NULL");
+  for (int i = 0; i < 6; ++i) out[i] = scene->host.bounds[i];
+  return SKW_OK;
+}
+
+skw_status skw_scene_info(const skw_scene *scene, uint64_t out[6]) {
+  if (!scene || !out) return fail(SKW_INVALID_ARG, "scene and out must not be NULL");
+  out[0] = scene->host.nodes.size();
+  out[1] = scene->host.tris.size();
+  out[2] = scene->host.max_depth;
+  out[3] = scene->host.insts.size();
+  out[4] = scene->host.leaves;
+  out[5] = scene->host.stack_need;
+  return SKW_OK;
+}
+
+skw_status skw_scene_export_tree(const skw_scene *scene, float *bbox, uint64_t *meta, uint32_t *tri_prim,
+                                 uint32_t *tri_tri, uint64_t *n_nodes) {
+  if (!scene) return fail(SKW_INVALID_ARG, "scene must not be NULL");
+  if ((tri_prim == nullptr) != (tri_tri == nullptr)) return fail(SKW_INVALID_ARG, "tri_prim and tri_tri go together");
+  uint64_t k = export_tree(scene->host, bbox, meta, tri_prim, tri_tri);
+  if (n_nodes) *n_nodes = k;
+  return SKW_OK;
+}
+
+static skw_status trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
+                        uint8_t *d_occ, void *stream, bool any, uint32_t *d_pops = nullptr,
+                        uint32_t *d_tests = nullptr) {
+  if (!scene) return fail(SKW_INVALID_ARG, "scene must not be NULL");
+  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
+  if (n == 0) return SKW_OK;
+  if (!d_rays || (!any && !d_hits) || (any && !d_occ)) return fail(SKW_INVALID_ARG, "ray/result pointers must not be NULL");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  skw_scene *s = const_cast<skw_scene *>(scene);
+  const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
+  uint64_t waves = (n + 63) / 64;
+  uint64_t cap = (uint64_t)s->cus * (uint64_t)(waves_per_cu > 0 ? waves_per_cu : 16);
+  uint32_t grid = (uint32_t)(waves < cap ? waves : cap);
+  TraceArgs a;
+  a.nodes = s->d_nodes;
+  a.tris = s->d_tris;
+  a.insts = s->d_insts;
+  a.inst_range = s->d_inst_range;
+  a.root = s->host.root;
+  a.rays = d_rays;
+  a.hits = d_hits;
+  a.occluded = d_occ;
+  a.n = n;
+  a.gstack = nullptr;
+  a.pops = d_pops;
+  a.tests = d_tests;
+  const bool stats = d_pops != nullptr || d_tests != nullptr;
+  SKW_HIP(s->counters.get(stream, &a.counter), "counter allocation");
+  uint32_t need = s->host.stack_need;
+  if (need > 128) {
+    SKW_HIP(hipMallocAsync((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
+  }
+  SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
+  hipError_t e = any ? launch_any_hit(a, s->has_inst, need, grid, st)
+                     : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
+  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
+  if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
+  if (!any && s->needs_fix) {
+    GeomArgs g;
+    g.prims = s->d_prims;
+    g.meshes = s->d_meshes;
+    g.mesh_verts = s->d_mesh_verts;
+    g.mesh_normals = s->d_mesh_normals;
+    g.mesh_tris = s->d_mesh_tris;
+    g.rays = d_rays;
+    g.hits = d_hits;
+    g.geom = nullptr;
+    g.n = n;
+    SKW_HIP(launch_fix_bary(g, st), "barycentric fix-up launch");
+  }
+  return SKW_OK;
+}
+
+skw_status skw_closest_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits, void *stream) {
+  return trace(scene, d_rays, n, d_hits, nullptr, stream, false);
+}
+
+skw_status skw_closest_hit_stats(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
+                                 uint32_t *d_pops, uint32_t *d_tests, void *stream) {
+  return trace(scene, d_rays, n, d_hits, nullptr, stream, false, d_pops, d_tests);
+}
+
+skw_status skw_any_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, uint8_t *d_occluded, void *stream) {
+  return trace(scene, d_rays, n, nullptr, d_occluded, stream, true);
+}
+
+skw_status skw_hit_geometry(const skw_scene *scene, const skw_ray *d_rays, const skw_hit *d_hits, uint64_t n,
+                            skw_hit_geom *d_geom, void *stream) {
+  if (!scene) return fail(SKW_INVALID_ARG, "scene must not be NULL");
+  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
+  if (n == 0) return SKW_OK;
+  if (!d_rays || !d_hits || !d_geom) return fail(SKW_INVALID_ARG, "pointers must not be NULL");
+  GeomArgs g;
+  g.prims = scene->d_prims;
+  g.meshes = scene->d_meshes;
+  g.mesh_verts = scene->d_mesh_verts;
+  g.mesh_normals = scene->d_mesh_normals;
+  g.mesh_tris = scene->d_mesh_tris;
+  g.rays = d_rays;
+  g.hits = const_cast<skw_hit *>(d_hits);
+  g.geom = d_geom;
+  g.n = n;
+  SKW_HIP(launch_hit_geometry(g, reinterpret_cast<hipStream_t>(stream)), "hit_geometry launch");
+  return SKW_OK;
+}
+
+skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
+                                 const uint16_t *normal, uint32_t flags, skw_photon_map **out) {
+  if (!out) return fail(SKW_INVALID_ARG, "out must not be NULL");
+  *out = nullptr;
+  if (n > 0 && !pos) return fail(SKW_INVALID_ARG, "pos must not be NULL");
+  if (n >= (uint64_t)kNoChild) return fail(SKW_INVALID_ARG, "too many photons for the 29-bit child index");
+  bool host_only = (flags & SKW_SCENE_HOST_ONLY) != 0;
+  if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
+  std::unique_ptr<skw_photon_map> m(new skw_photon_map());
+  build_photon_tree(n, pos, weight, dir, normal, m->host);
+  if (!host_only) {
+    SKW_HIP(hipGetDevice(&m->device), "hipGetDevice");
+    SKW_HIP(upload(&m->d_nodes, m->host.nodes), "upload kd nodes");
+    SKW_HIP(upload(&m->d_photons, m->host.photons), "upload photons");
+    SKW_HIP(upload(&m->d_perm, m->host.perm), "upload permutation");
+    std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
+    SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
+    m->on_device = true;
+  }
+  *out = m.release();
+  return SKW_OK;
+}
+
+skw_status skw_photon_map_destroy(skw_photon_map *map) {
+  delete map;
+  return SKW_OK;
+}
+
+skw_status skw_photon_map_export(const skw_photon_map *map, uint32_t *perm, float *split, uint32_t *word) {
+  if (!map) return fail(SKW_INVALID_ARG, "map must not be NULL");
+  const KdLayout &h = map->host;
+  for (size_t i = 0; i < h.nodes.size(); ++i) {
+    if (perm) perm[i] = h.perm[i];
+    uint32_t w = h.nodes[i].word;
+    if (word) word[i] = w;
+    if (split) {
+      uint32_t axis = w & 3u;
+      split[i] = axis == 3u ? 0.0f : h.nodes[i].p[axis];
+    }
+  }
+  return SKW_OK;
+}
+
+static skw_status gather(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                         uint32_t k, double r, uint64_t paths, double rho, int filter, int mode, float *e_ext,
+                         float *e_int, float *radiance, uint32_t *ids, double *d2, uint32_t *count,
+                         uint32_t *nearest, void *stream) {
+  if (!map) return fail(SKW_INVALID_ARG, "map must not be NULL");
+  if (!map->on_device) return fail(SKW_UNSUPPORTED, "photon map was created host-only");
+  if (!(r >= 0.0)) return fail(SKW_INVALID_ARG, "radius must be >= 0");
+  if (filter != 0 && filter != 1) return fail(SKW_INVALID_ARG, "filter must be 0 or 1");
+  if (nq == 0) return SKW_OK;
+  if (!d_points) return fail(SKW_INVALID_ARG, "points must not be NULL");
+  if (filter == 1 && !d_normals) return fail(SKW_INVALID_ARG, "PhotonFilter needs normals");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  GatherArgs a;
+  a.nodes = map->d_nodes;
+  a.photons = map->d_photons;
+  a.perm = map->d_perm;
+  a.cdir = map->d_cdir;
+  a.n_nodes = map->host.nodes.size();
+  a.points = d_points;
+  a.normals = d_normals;
+  a.nq = nq;
+  a.k = k;
+  a.r = r;
+  a.paths = (double)paths;
+  a.rho = rho;
+  a.sin_t = std::sqrt(1.0 - 0.87 * 0.87);
+  a.filter = filter;
+  a.heap_pos = nullptr;
+  a.heap_d2 = nullptr;
+  a.q_stride = 1;
+  a.j_stride = nq;
+  a.ids = ids;
+  a.d2 = d2;
+  a.count = count;
+  a.e_ext = e_ext;
+  a.e_int = e_int;
+  a.radiance = radiance;
+  a.nearest = nearest;
+  a.stack_need = map->host.max_depth + 1;
+  a.gstack = nullptr;
+  if (mode == 0 && k > 0) {
+    size_t elems = (size_t)nq * k;
+    SKW_HIP(hipMallocAsync((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
+    hipError_t e = hipMallocAsync((void **)&a.heap_d2, elems * sizeof(double), st);
+    if (e != hipSuccess) {
+      (void)hipFreeAsync(a.heap_pos, st);
+      return hip_fail(e, "heap allocation");
+    }
+  }
+  if (a.stack_need > 64) {
+    hipError_t e = hipMallocAsync((void **)&a.gstack, (size_t)nq * a.stack_need * sizeof(uint32_t), st);
+    if (e != hipSuccess) {
+      if (a.heap_pos) (void)hipFreeAsync(a.heap_pos, st);
+      if (a.heap_d2) (void)hipFreeAsync(a.heap_d2, st);
+      return hip_fail(e, "stack allocation");
+    }
+  }
+  hipError_t e = launch_gather(a, mode, st);
+  if (a.heap_pos) (void)hipFreeAsync(a.heap_pos, st);
+  if (a.heap_d2) (void)hipFreeAsync(a.heap_d2, st);
+  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
+  if (e != hipSuccess) return hip_fail(e, "gather launch");
+  return SKW_OK;
+}
+
+skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                 uint64_t nq, uint32_t k, double r, uint64_t paths, double rho, float *d_e_ext,
+                                 float *d_e_int, float *d_radiance, uint32_t *d_ids, double *d_d2,
+                                 uint32_t *d_count, void *stream) {
+  if (nq > 0 && (!d_e_ext || !d_e_int)) return fail(SKW_INVALID_ARG, "e_ext and e_int must not be NULL");
+  if (nq > 0 && !d_normals) return fail(SKW_INVALID_ARG, "normals must not be NULL");
+  return gather(map, d_points, d_normals, nq, k, r, paths, rho, 1, 0, d_e_ext, d_e_int, d_radiance, d_ids, d_d2,
+                d_count, nullptr, stream);
+}
+
+skw_status skw_photon_knn(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                          uint32_t k, double r, int filter, uint32_t *d_ids, double *d_d2, uint32_t *d_count,
+                          void *stream) {
+  if (nq > 0 && (!d_ids || !d_count)) return fail(SKW_INVALID_ARG, "ids and count must not be NULL");
+  return gather(map, d_points, d_normals, nq, k, r, 1, 0.0, filter, 0, nullptr, nullptr, nullptr, d_ids, d_d2,
+                d_count, nullptr, stream);
+}
+
+skw_status skw_photon_nearest(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                              uint64_t nq, double r, int filter, uint32_t *d_id, void *stream) {
+  if (nq > 0 && !d_id) return fail(SKW_INVALID_ARG, "id must not be NULL");
+  return gather(map, d_points, d_normals, nq, 0, r, 1, 0.0, filter, 1, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, d_id, stream);
+}
+
+}  // extern "C"

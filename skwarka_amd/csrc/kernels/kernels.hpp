@@ -1,0 +1,79 @@
+// Launch interfaces of the gfx950 kernels (host side of the kernel translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../skw_layout.hpp"
+#include "../../../include/skwarka_rt.h"
+
+namespace skw {
+
+struct TraceArgs {
+  const DevNode *nodes;
+  const DevTri *tris;
+  const DevInst *insts;
+  const uint32_t *inst_range;
+  uint32_t root;
+  const skw_ray *rays;
+  skw_hit *hits;
+  uint8_t *occluded;
+  uint64_t n;
+  uint64_t *counter;  // set to gridDim*blockDim before the trace kernel starts
+  uint32_t *gstack;   // used only when the tree is deeper than the LDS stack
+  uint32_t *pops;     // statistics variant only: node boxes tested per ray
+  uint32_t *tests;    // statistics variant only: triangles tested per ray
+};
+
+// Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).
+hipError_t launch_closest_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
+                              hipStream_t stream);
+hipError_t launch_any_hit(const TraceArgs &args, bool inst, uint32_t stack_need, uint32_t grid, hipStream_t stream);
+hipError_t launch_set_counter(uint64_t *counter, uint64_t value, hipStream_t stream);
+
+struct GeomArgs {
+  const DevPrim *prims;
+  const DevMesh *meshes;
+  const float *mesh_verts;
+  const float *mesh_normals;
+  const uint32_t *mesh_tris;
+  const skw_ray *rays;
+  skw_hit *hits;           // fix_bary: b1/b2 rewritten for primitives whose traversal values are not mesh-space
+  skw_hit_geom *geom;      // hit_geometry output
+  uint64_t n;
+};
+hipError_t launch_fix_bary(const GeomArgs &args, hipStream_t stream);
+hipError_t launch_hit_geometry(const GeomArgs &args, hipStream_t stream);
+
+struct GatherArgs {
+  const DevKdNode *nodes;
+  const DevPhoton *photons;
+  const uint32_t *perm;
+  const double *cdir;  // 65536 x 3
+  uint64_t n_nodes;
+  const double *points;
+  const double *normals;
+  uint64_t nq;
+  uint32_t k;
+  double r;
+  double paths;
+  double rho;
+  double sin_t;         // sqrt(1 - 0.87^2), computed on the host
+  int filter;           // 0 none, 1 PhotonFilter
+  // heap storage: element (q, j) at q*q_stride + j*j_stride
+  uint32_t *heap_pos;
+  double *heap_d2;
+  uint64_t q_stride, j_stride;
+  // outputs (any may be null)
+  uint32_t *ids;        // k per query, caller ids (row-major)
+  double *d2;           // k per query
+  uint32_t *count;
+  float *e_ext, *e_int, *radiance;
+  uint32_t *nearest;    // nearest mode
+  uint32_t stack_need;
+  uint32_t *gstack;
+};
+// mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
+hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
+
+}  // namespace skw

@@ -1,0 +1,122 @@
+"""The product's host builders (in libskwarka_rt.so, host-only mode: no device needed) produce
+the same trees as the oracle's restatement of the reference builders:
+  TriangleAccelerator ctor / Node::Node / _DetermineBestSplit (TriangleAccelerator.cpp:22-147, 505-729)
+  KDTree<Photon>::_Build (KDTree.h:427-473)
+and the C ABI exports every symbol include/skwarka_rt.h declares."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import skwarka_amd as sk
+from skwarka_amd import workloads as wl
+from tests import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sk.lib()
+    header = open(os.path.join(ROOT, "include", "skwarka_rt.h")).read()
+    declared = set(re.findall(r"\b(skw_[a-z_0-9]+)\s*\(", header))
+    assert declared, "no declarations found"
+    assert declared == set(sk.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_no_device_is_reported_not_faked():
+    L = sk.lib()
+    if L.skw_device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(sk.SkwError, match="no HIP device"):
+        sk.Scene([wl.box(1.0)])
+
+
+def _compare_trees(meshes, prims=None):
+    acc = oracle.Accel(meshes, prims)
+    sc = sk.Scene(meshes, prims, host_only=True)
+    b1, m1, tp1, tt1 = acc.dump()
+    b2, m2, tp2, tt2 = sc.export_tree()
+    assert len(b1) == len(b2)
+    assert np.array_equal(b1.view(np.uint32), b2.view(np.uint32))
+    assert np.array_equal(m1, m2)
+    assert np.array_equal(tp1, tp2) and np.array_equal(tt1, tt2)
+    assert np.array_equal(acc.bounds(), sc.bounds())
+    oi, si = acc.info(), sc.info()
+    for key in ("nodes", "triangles", "max_depth", "instances", "leaves"):
+        assert oi[key] == si[key], key
+    return sc
+
+
+def test_tree_c0_scene():
+    sc = _compare_trees(wl.scene_c0())
+    assert sc.info()["triangles"] == 81920 + 12
+
+
+def test_tree_reference_fixture():
+    _compare_trees(scenes.reference_spheres())
+
+
+def test_tree_transformed_and_instanced():
+    inst_meshes, inst_prims, flat = scenes.instanced_tetrahedra(200)
+    _compare_trees(inst_meshes, inst_prims)
+    separate = [scenes.tetra_mesh() for _ in inst_prims]
+    _compare_trees(separate, [(i, m) for i, (_, m) in enumerate(inst_prims)])
+    # mixed: instanced tetrahedra plus plain spheres
+    mixed = inst_meshes + scenes.small_spheres(5, n=3, subdiv=2)
+    prims = inst_prims + [(1, None), (2, None), (3, None), (4, None)]
+    _compare_trees(mixed, prims)
+
+
+def test_tree_degenerate_and_coplanar():
+    # Many coplanar axis-aligned triangles (zero node extent on an axis: the NaN bin case) and
+    # degenerate index triples.
+    g = np.linspace(-1, 1, 9)
+    x, y = np.meshgrid(g, g)
+    v = np.stack([x.ravel(), y.ravel(), np.zeros(x.size)], 1).astype(np.float32)
+    tris = []
+    for i in range(8):
+        for j in range(8):
+            a = i * 9 + j
+            tris += [(a, a + 1, a + 10), (a, a + 10, a + 9)]
+    tris += [(0, 0, 1), (5, 6, 9999)]
+    plane = {"verts": v, "tris": np.array(tris, np.uint32)}
+    _compare_trees([plane, wl.icosphere(2, 0.3, (0, 0, 0.5))])
+
+
+def test_tree_empty_scene():
+    _compare_trees([{"verts": np.zeros((0, 3), np.float32), "tris": np.zeros((0, 3), np.uint32)}])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1000, 50000])
+def test_photon_tree_matches_oracle(n):
+    mesh = wl.merge(wl.scene_c0())
+    ph = wl.photons_on_mesh(mesh, n, seed=n)
+    om = oracle.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
+    pm = sk.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"], host_only=True)
+    p1, s1, w1 = om.dump()
+    p2, s2, w2 = pm.export()
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(w1, w2)
+    leaf = (w1 & 3) == 3
+    assert np.array_equal(s1[~leaf].view(np.uint32), s2[~leaf].view(np.uint32))
+
+
+def test_photon_tree_with_duplicates():
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(-1, 1, (5000, 3)).astype(np.float32)
+    pos = np.concatenate([pos, pos[:500], pos[:100]])
+    z = np.zeros(len(pos), np.uint16)
+    w = np.ones((len(pos), 3), np.float32)
+    om = oracle.PhotonMap(pos, w, z, z)
+    pm = sk.PhotonMap(pos, w, z, z, host_only=True)
+    assert np.array_equal(om.dump()[0], pm.export()[0])
+    assert np.array_equal(om.dump()[2], pm.export()[2])
+
+
+def test_bad_descriptions_are_rejected():
+    with pytest.raises(sk.SkwError):
+        sk.Scene([wl.box(1.0)], prims=[(5, None)], host_only=True)
