@@ -222,9 +222,56 @@ __device__ __forceinline__ void load_ray(const skw_ray *__restrict__ rays, uint6
   tmax = d.y;
 }
 
+__device__ __forceinline__ bool zero_signed_or_nonfinite(double x) {
+  return (unsigned long long)__double_as_longlong(x) == 0x8000000000000000ull || !isfinite(x);
+}
+
+// Intersect() recomputes b1/b2 after moving the ray to mesh space with the primitive's
+// world-to-mesh matrix (TriangleAccelerator.cpp:311-322). For the identity matrix that product,
+// (1*x + 0*y) + 0*z (+ 0 for points), returns x unchanged except that -0.0 becomes +0.0 and a
+// non-finite component in the row poisons it. Rays with such components get their barycentrics
+// recomputed here with exactly that arithmetic; all other rays keep the traversal values, which
+// are then identical.
+__device__ __forceinline__ void identity_bary(const DevTri *__restrict__ tris, uint32_t pos, const RayState &r,
+                                              double &b1, double &b2) {
+  const double one = 1.0, zero = 0.0;
+  double ox = one * r.o0 + zero * r.o1 + zero * r.o2 + zero;
+  double oy = zero * r.o0 + one * r.o1 + zero * r.o2 + zero;
+  double oz = zero * r.o0 + zero * r.o1 + one * r.o2 + zero;
+  double w = zero * r.o0 + zero * r.o1 + zero * r.o2 + one;
+  if (w != 1.0) {
+    double iw = 1.0 / w;
+    ox = ox * iw;
+    oy = oy * iw;
+    oz = oz * iw;
+  }
+  double dx = one * r.d0 + zero * r.d1 + zero * r.d2;
+  double dy = zero * r.d0 + one * r.d1 + zero * r.d2;
+  double dz = zero * r.d0 + zero * r.d1 + one * r.d2;
+  const float4 *p = reinterpret_cast<const float4 *>(tris + pos);
+  float4 q0 = p[0], q1 = p[1], q2 = p[2];
+  double v0x = q0.x, v0y = q0.y, v0z = q0.z;
+  double e1x = (double)q0.w - v0x, e1y = (double)q1.x - v0y, e1z = (double)q1.y - v0z;
+  double e2x = (double)q1.z - v0x, e2y = (double)q1.w - v0y, e2z = (double)q2.x - v0z;
+  double s1x = dy * e2z - dz * e2y;
+  double s1y = dz * e2x - dx * e2z;
+  double s1z = dx * e2y - dy * e2x;
+  double divisor = s1x * e1x + s1y * e1y + s1z * e1z;
+  b1 = 0.0;
+  b2 = 0.0;
+  if (divisor == 0.0) return;  // TriangleMesh.cpp:421
+  double inv = 1.0 / divisor;
+  double ddx = ox - v0x, ddy = oy - v0y, ddz = oz - v0z;
+  b1 = (ddx * s1x + ddy * s1y + ddz * s1z) * inv;
+  double s2x = ddy * e1z - ddz * e1y;
+  double s2y = ddz * e1x - ddx * e1z;
+  double s2z = ddx * e1y - ddy * e1x;
+  b2 = (dx * s2x + dy * s2y + dz * s2z) * inv;
+}
+
 template <bool ANY>
 __device__ __forceinline__ void store_result(const TraceArgs &args, uint64_t idx, bool hit, const Best &best,
-                                             double tmax) {
+                                             double tmax, const RayState &r) {
   if (ANY) {
     args.occluded[idx] = hit ? 1 : 0;
     return;
@@ -243,6 +290,10 @@ __device__ __forceinline__ void store_result(const TraceArgs &args, uint64_t idx
     t = tmax;
     b1 = best.b1;
     b2 = best.b2;
+    if (!best.inst && (zero_signed_or_nonfinite(r.o0) || zero_signed_or_nonfinite(r.o1) ||
+                       zero_signed_or_nonfinite(r.o2) || zero_signed_or_nonfinite(r.d0) ||
+                       zero_signed_or_nonfinite(r.d1) || zero_signed_or_nonfinite(r.d2)))
+      identity_bary(args.tris, best.pos, r, b1, b2);
   }
   uint4 w0;
   w0.x = prim;
@@ -304,7 +355,7 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
     // One step of the walk.
     if (!have) {
       if (sp == 0) {
-        store_result<ANY>(args, idx, found, best, tmax);
+        store_result<ANY>(args, idx, found, best, tmax, r);
         if (STATS) {
           if (args.pops) args.pops[idx] = pops;
           if (args.tests) args.tests[idx] = tests;
