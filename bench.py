@@ -3,16 +3,20 @@
 
 A step is one pass of the hot path over one batch of synthetic input, resident in HBM before the
 timed region starts:
-  c0 (default line): closest hit of 512x512 FP64 primary rays (pinhole at (0,0,-2.9), 60 deg x-FOV,
+  c2 (the line's value): one direct-lighting frame of the seeded 2,026,260-triangle room (200
+      displaced icospheres, 4 quad area lights, matte and plastic materials) at 1024x1024, 16 spp,
+      4 light + 4 BSDF samples per hit: camera rays, closest hit, shading, occlusion rays, BSDF
+      rays, MIS resolve -> Mrays/s over every ray cast (primary + occlusion + BSDF).
+  c0 (component): closest hit of 512x512 FP64 primary rays (pinhole at (0,0,-2.9), 60 deg x-FOV,
       pixel centres) against an 81,920-triangle icosphere (r=1, +-2% radial noise) inside the
       12-triangle box [-3,3]^3 -> Mrays/s.
-  c1 (reported beside it): photon gather, 1,000,000 photons on the c0 scene, 262,143 gather points
-      at the c0 primary hits, k=100, r=0.05, PhotonFilter, disc estimate + Lambertian rho=0.5
-      radiance -> M gathers/s.
+  c1 (component): photon gather, 1,000,000 photons on the c0 scene, 262,143 gather points at the
+      c0 primary hits, k=100, r=0.05, PhotonFilter, disc estimate + Lambertian rho=0.5 radiance
+      -> M gathers/s.
 Timing: W untimed warm-up steps, then exactly K steps between HIP events recorded on the stream
 the kernels run on, with device synchronisation on both sides.
 The CPU baseline is the oracle (CPU restatement of the reference path, OpenMP) timed on this host
-on the same workload (1 warm-up, median of 5), on rank 0 at N=1 only.
+on a bounded sample of the same workload, on rank 0 at N=1 only.
 """
 import argparse
 import json
@@ -30,7 +34,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 64, 32, 32, 36
-KD_NODE_BYTES, CODE_BYTES, PHOTON_BYTES = 16, 4, 16
 
 
 def log(*a):
@@ -75,14 +78,42 @@ def time_steps(torch, fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps
 
 
-def bench_c0(torch, sk, wl, args, stream):
+def dev_bytes(torch, arr):
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).reshape(-1).copy()).to("cuda")
+
+
+def bench_c2(torch, sk, wl, steps, warmup, stream):
+    s = wl.scene_room()
+    t0 = time.time()
+    scene = sk.Scene(s["meshes"])
+    build_s = time.time() - t0
+    dls = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    w = h = 1024
+    spp = 16
+    cam, cam15 = sk.camera(*s["camera"], w, h)
+    d_rgb = torch.empty(w * h * 12, dtype=torch.uint8, device="cuda")
+    sptr = stream.cuda_stream
+    counts = {}
+
+    def step():
+        counts["rays"] = dls.render(cam, w, h, spp, d_rgb.data_ptr(), 4, 4, 1234, sptr)
+
+    ms = time_steps(torch, step, steps, warmup, stream)
+    rays = counts["rays"]
+    total = int(rays.sum())
+    rgb = d_rgb.cpu().numpy().view(np.float32)
+    return {"ms": ms, "value": total / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays], "build_s": build_s,
+            "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean())}
+
+
+def bench_c0(torch, sk, wl, steps, warmup, stream):
     meshes = wl.scene_c0()
     rays = wl.rays_c0()
     n = len(rays)
     t0 = time.time()
     scene = sk.Scene(meshes)
     build_s = time.time() - t0
-    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).to("cuda")
+    d_rays = dev_bytes(torch, rays)
     d_hits = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
     d_pops = torch.empty(n * 4, dtype=torch.uint8, device="cuda")
     d_tests = torch.empty(n * 4, dtype=torch.uint8, device="cuda")
@@ -92,41 +123,34 @@ def bench_c0(torch, sk, wl, args, stream):
     torch.cuda.synchronize()
     pops = d_pops.cpu().numpy().view(np.uint32).astype(np.int64)
     tests = d_tests.cpu().numpy().view(np.uint32).astype(np.int64)
-    ms = time_steps(torch, lambda: scene.closest_hit(d_rays.data_ptr(), n, d_hits.data_ptr(), sptr),
-                    args.steps, args.warmup, stream)
+    ms = time_steps(torch, lambda: scene.closest_hit(d_rays.data_ptr(), n, d_hits.data_ptr(), sptr), steps, warmup,
+                    stream)
     alg_bytes = n * (RAY_BYTES + HIT_BYTES) + NODE_BYTES * int(pops.sum()) + TRI_BYTES * int(tests.sum())
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
     hits = d_hits.cpu().numpy().view(sk.HIT_DTYPE)
-    return {
-        "n": n, "ms": ms, "value": n / (ms * 1e-3) / 1e6, "build_s": build_s,
-        "mean_pops": float(pops.mean()), "mean_tests": float(tests.mean()), "alg_bytes": alg_bytes,
-        "achieved_gbs": achieved, "hits": int((hits["prim"] != sk.MISS).sum()),
-        "info": scene.info(), "scene": meshes, "rays": rays, "hit_records": hits,
-    }
+    return {"n": n, "ms": ms, "value": n / (ms * 1e-3) / 1e6, "build_s": build_s, "mean_pops": float(pops.mean()),
+            "mean_tests": float(tests.mean()), "alg_bytes": alg_bytes,
+            "achieved_gbs": alg_bytes / (ms * 1e-3) / 1e9, "info": scene.info(), "scene": meshes, "rays": rays,
+            "hit_records": hits, "scene_handle": scene, "d_rays": d_rays, "d_hits": d_hits}
 
 
-def bench_c1(torch, sk, wl, args, stream, c0):
+def bench_c1(torch, sk, wl, steps, warmup, stream, c0):
     mesh = wl.merge(c0["scene"])
     ph = wl.photons_on_mesh(mesh, 1_000_000, seed=1)
     t0 = time.time()
     pm = sk.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
     build_s = time.time() - t0
     # gather points: c0 primary hits and their geometric normals, from the product's hit_geometry
-    rays, hits = c0["rays"], c0["hit_records"]
-    keep = hits["prim"] != sk.MISS
-    scene = sk.Scene(c0["scene"])
-    n = len(rays)
-    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).to("cuda")
-    d_hits = torch.from_numpy(hits.view(np.uint8).copy()).to("cuda")
+    n = len(c0["rays"])
+    keep = c0["hit_records"]["prim"] != sk.MISS
     d_geom = torch.empty(n * 104, dtype=torch.uint8, device="cuda")
-    scene.hit_geometry(d_rays.data_ptr(), d_hits.data_ptr(), n, d_geom.data_ptr(), stream.cuda_stream)
+    c0["scene_handle"].hit_geometry(c0["d_rays"].data_ptr(), c0["d_hits"].data_ptr(), n, d_geom.data_ptr(),
+                                    stream.cuda_stream)
     torch.cuda.synchronize()
     geom = d_geom.cpu().numpy().view(sk.GEOM_DTYPE)[keep]
     p = np.ascontiguousarray(geom["p"])
     nrm = np.ascontiguousarray(geom["ng"])
     nq, k, r = len(p), 100, 0.05
-    dp = torch.from_numpy(p.view(np.uint8).reshape(-1).copy()).to("cuda")
-    dn = torch.from_numpy(nrm.view(np.uint8).reshape(-1).copy()).to("cuda")
+    dp, dn = dev_bytes(torch, p), dev_bytes(torch, nrm)
     ee = torch.empty(nq * 12, dtype=torch.uint8, device="cuda")
     ei = torch.empty(nq * 12, dtype=torch.uint8, device="cuda")
     rad = torch.empty(nq * 12, dtype=torch.uint8, device="cuda")
@@ -137,14 +161,31 @@ def bench_c1(torch, sk, wl, args, stream, c0):
         pm.irradiance(dp.data_ptr(), dn.data_ptr(), nq, k, r, 1, 0.5, ee.data_ptr(), ei.data_ptr(), rad.data_ptr(),
                       None, None, cnt.data_ptr(), sptr)
 
-    ms = time_steps(torch, step, max(3, args.steps // 2), 2, stream)
+    ms = time_steps(torch, step, steps, warmup, stream)
     counts = cnt.cpu().numpy().view(np.uint32)
     return {"nq": nq, "ms": ms, "value": nq / (ms * 1e-3) / 1e6, "build_s": build_s,
             "mean_found": float(counts.mean()), "photons": ph, "p": p, "n": nrm}
 
 
-def cpu_baseline_c0(c0, threads):
+def cpu_baseline_c2(c2, threads, rows):
     import oracle  # CPU restatement of the reference path: the baseline, never the product
+    s = c2["spec"]
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    w, spp = 1024, 16
+    # rows around the image centre: a band that sees spheres, walls and lights
+    r0 = 512 - rows // 2
+    dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + 1) * w, nthreads=threads)  # warm-up
+    t = time.perf_counter()
+    _, cnt = dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + rows) * w, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": float(cnt.sum()) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "image rows %d-%d of the c2 frame (%d pixels x 16 spp, %d rays), one timed run (%s)"
+                      % (r0, r0 + rows - 1, rows * w, int(cnt.sum()), cpu_info())}
+
+
+def cpu_baseline_c0(c0, threads):
+    import oracle
     acc = oracle.Accel(c0["scene"])
     rays = c0["rays"]
     acc.closest_hit(rays, nthreads=threads)  # warm-up
@@ -155,7 +196,7 @@ def cpu_baseline_c0(c0, threads):
         times.append(time.perf_counter() - t)
     med = statistics.median(times)
     return {"value": len(rays) / med / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "all %d c0 primary rays per run; 1 warm-up, median of 5 runs (%s)" % (len(rays), cpu_info())}
+            "sample": "all %d c0 primary rays per run; 1 warm-up, median of 5 runs" % len(rays)}
 
 
 def cpu_baseline_c1(c1, threads, sample):
@@ -178,14 +219,29 @@ def cpu_baseline_c1(c1, threads, sample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--only", choices=["c0", "c1", "c2"], default=None,
+                    help="run one workload (profiling runs); the printed line still names c2's metric only "
+                         "when c2 runs")
+    ap.add_argument("--cpu-rows", type=int, default=64, help="c2 image rows in the CPU baseline sample")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (3 times): the reference "
+                         "point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the scene is replicated per "
-              "GPU and rays shard by image tile, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "GPU and rays shard by image tile); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -196,52 +252,70 @@ def main():
     sk.lib()
     stream = torch.cuda.current_stream()
     threads = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1)
+    run = (lambda w: args.only is None or args.only == w)
 
-    log("c0: closest hit, 512x512 rays, 81,932 triangles")
-    c0 = bench_c0(torch, sk, wl, args, stream)
-    log("c0: %.3f ms/step, %.1f Mrays/s" % (c0["ms"], c0["value"]))
-    c1 = None
-    if not args.no_c1:
+    c0 = c1 = c2 = None
+    if run("c0") or run("c1"):
+        log("c0: closest hit, 512x512 rays, 81,932 triangles")
+        c0 = bench_c0(torch, sk, wl, 20, 3, stream)
+        log("c0: %.3f ms/step, %.1f Mrays/s" % (c0["ms"], c0["value"]))
+    if run("c1"):
         log("c1: photon gather, 1M photons, k=100, r=0.05")
-        c1 = bench_c1(torch, sk, wl, args, stream, c0)
+        c1 = bench_c1(torch, sk, wl, 10, 2, stream, c0)
         log("c1: %.3f ms/step, %.2f M gathers/s" % (c1["ms"], c1["value"]))
-    cpu = None
-    cpu1 = None
+    if run("c2"):
+        log("c2: direct lighting, 2,026,260 triangles, 1024x1024 x 16 spp")
+        c2 = bench_c2(torch, sk, wl, args.steps, args.warmup, stream)
+        log("c2: %.1f ms/frame, %.1f Mrays/s, rays %s" % (c2["ms"], c2["value"], c2["rays"]))
+    cpu0 = cpu1 = cpu2 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
-        cpu = cpu_baseline_c0(c0, threads)
+        if c2 is not None:
+            cpu2 = cpu_baseline_c2(c2, threads, args.cpu_rows)
+        if c0 is not None:
+            cpu0 = cpu_baseline_c0(c0, threads)
         if c1 is not None:
             cpu1 = cpu_baseline_c1(c1, threads, 16384)
-    traffic = pmc_traffic("closest_hit_c0_pmc")
-    line = {
-        "metric": "Mrays/s closest-hit",
-        "value": round(c0["value"], 3),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(c0["ms"], 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "c0: 81,920-tri icosphere (r=1, 2% radial noise) + 12-tri box [-3,3]^3, "
-                               "512x512 FP64 primary rays, 1 spp, closest hit",
-                   "rays_per_step": c0["n"], "triangles": c0["info"]["triangles"], "nodes": c0["info"]["nodes"]},
-        "roofline": {"bound": "hbm", "achieved": round(c0["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(c0["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "bytes_per_ray": round(c0["alg_bytes"] / c0["n"], 2), "mean_node_tests": round(c0["mean_pops"], 3),
-                     "mean_tri_tests": round(c0["mean_tests"], 3)},
-        "cpu_baseline": cpu,
-        "components": {},
-    }
-    if c1 is not None:
-        line["components"]["c1_photon_gather"] = {
-            "metric": "M photon-gathers/s", "value": round(c1["value"], 4), "ms_per_step": round(c1["ms"], 4),
-            "gathers_per_step": c1["nq"], "mean_photons_found": round(c1["mean_found"], 3),
-            "cpu_baseline": cpu1,
+    comps = {}
+    if c0 is not None:
+        comps["c0_closest_hit"] = {
+            "metric": "Mrays/s closest-hit", "value": round(c0["value"], 3), "ms_per_step": round(c0["ms"], 5),
+            "rays_per_step": c0["n"], "cpu_baseline": cpu0,
+            "roofline": {"bound": "hbm", "achieved": round(c0["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(c0["achieved_gbs"] / HBM_PEAK_GBS, 5),
+                         "traffic": pmc_traffic("closest_hit_c0_pmc"),
+                         "bytes_per_ray": round(c0["alg_bytes"] / c0["n"], 2),
+                         "mean_node_tests": round(c0["mean_pops"], 3), "mean_tri_tests": round(c0["mean_tests"], 3)},
         }
+    if c1 is not None:
+        comps["c1_photon_gather"] = {
+            "metric": "M photon-gathers/s", "value": round(c1["value"], 4), "ms_per_step": round(c1["ms"], 4),
+            "gathers_per_step": c1["nq"], "mean_photons_found": round(c1["mean_found"], 3), "cpu_baseline": cpu1,
+        }
+    if c2 is not None:
+        line = {
+            "metric": "Mrays/s (direct lighting, all rays: primary + occlusion + BSDF)",
+            "value": round(c2["value"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(c2["ms"], 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "c2: seeded room, 2,026,260 triangles (200 displaced icospheres, 4 quad area "
+                                   "lights, matte + plastic), 1024x1024, 16 spp, 4 light + 4 BSDF samples per hit, "
+                                   "direct lighting, f32 radiance out",
+                       "rays_per_frame": c2["rays"], "triangles": c2["info"]["triangles"],
+                       "nodes": c2["info"]["nodes"]},
+            # The frame's dominant kernel is the traversal; its roofline is measured on c0 where the
+            # per-ray node/triangle counts are exact (components.c0_closest_hit.roofline).
+            "roofline": comps.get("c0_closest_hit", {}).get("roofline"),
+            "cpu_baseline": cpu2,
+            "components": comps,
+        }
+    else:
+        c = comps.get("c0_closest_hit") or comps.get("c1_photon_gather")
+        line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
+                "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": args.only}, "roofline": c.get("roofline"), "cpu_baseline": c["cpu_baseline"],
+                "components": comps}
     print(json.dumps(line), flush=True)
 
 

@@ -153,6 +153,53 @@ skw_status skw_photon_knn(const skw_photon_map *map, const double *d_points, con
 skw_status skw_photon_nearest(const skw_photon_map *map, const double *d_points, const double *d_normals,
                               uint64_t nq, double r, int filter, uint32_t *d_id, void *stream);
 
+/* ---------------------------------------------------------------- direct lighting
+ * DirectLightingLTEIntegrator / DirectLightingIntegrator over a scene (Raytracer/Core/
+ * DirectLightingIntegrator.cpp:54-366, LTEIntegrators/DirectLightingLTEIntegrator.cpp:33-60): camera ray,
+ * emitted light of area-light surfaces, light sampling with IrradianceLightsSamplingStrategy and
+ * occlusion rays, BSDF sampling with closest-hit rays, power-heuristic MIS. Materials are
+ * MatteMaterial (sigma 0: Lambertian) and PlasticMaterial (Lambertian + Blinn microfacet with
+ * FresnelDielectric(1.5, 1)); lights are DiffuseAreaLightSource on primitives whose meshes are
+ * in world space. Runs as a wavefront of kernels; all rays use the scene's traversal kernels. */
+typedef struct skw_dl_scene skw_dl_scene;
+
+typedef struct {
+  const uint8_t *mat_type;         /* per prim: 0 matte, 1 plastic */
+  const double *kd;                /* 3 per prim, clamped to [0,1] */
+  const double *ks;                /* 3 per prim, clamped to [0,1] (plastic) */
+  const double *roughness;         /* per prim (plastic), clamped to [0.001, 1] */
+  uint32_t n_lights;
+  const uint32_t *light_prim;      /* prim index of each area light */
+  const double *light_radiance;    /* 3 per light */
+} skw_dl_desc;
+
+/* Pinhole camera (PerspectiveCamera::GenerateRay with lens radius 0): origin, camera-to-world
+ * rotation (row-major), tan of half the x field of view, and y tan = x tan * height / width. */
+typedef struct {
+  double origin[3];
+  double rot[9];
+  double x_tan, y_tan;
+} skw_camera;
+
+/* Sample stream: dimension d of sample s of pixel p is splitmix64-derived
+ * u = (mix(seed ^ mix((p*spp + s)*64 + d)) >> 11) * 2^-53; dims 0-1 pixel jitter, 2.. light
+ * choice, then light position pairs, then BSDF component, then BSDF direction pairs. */
+typedef struct {
+  uint32_t width, height, spp;
+  uint32_t light_samples, bsdf_samples; /* per hit; at most 8 each */
+  uint64_t seed;
+} skw_dl_params;
+
+skw_status skw_dl_scene_create(const skw_scene *scene, const skw_dl_desc *desc, skw_dl_scene **out);
+skw_status skw_dl_scene_destroy(skw_dl_scene *dl);
+
+/* Renders the image: d_rgb = width*height*3 floats (mean radiance per pixel over spp samples;
+ * NaN, infinite or negative-luminance samples count as black, SamplerBasedRenderer.cpp:458-478).
+ * rays_out (host, optional, 3 values): primary, occlusion and BSDF rays cast. Work is enqueued
+ * on `stream`; the call returns after the frame has been enqueued and the ray counts read. */
+skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                         const skw_dl_params *params, float *d_rgb, uint64_t *rays_out, void *stream);
+
 /* Thread-local text of the last error. */
 const char *skw_last_error(void);
 

@@ -65,6 +65,10 @@ def lib():
         "orc_cdir_encode": ([ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_uint16),
         "orc_cdir_table": ([_P], None),
         "orc_mt19937": ([_U32, _U64, _P], None),
+        "orc_dl_scene_create": ([_P, _P, _P, _P, _P, _U32, _P, _P], _P),
+        "orc_dl_scene_destroy": ([_P], None),
+        "orc_dl_render": ([_P, _P, _P, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P, _P, _P, ctypes.c_int],
+                          None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -267,6 +271,41 @@ class PhotonMap:
         lib().orc_photon_cone(self.h, len(p), p.ctypes.data, n.ctypes.data, k, r, paths, kc, out.ctypes.data,
                               nthreads or threads())
         return out
+
+
+class DLScene:
+    """Direct-lighting restatement over an Accel (materials / lights as skwarka_amd.DLScene)."""
+
+    def __init__(self, accel, mat_type, kd, ks, roughness, light_prim, light_radiance):
+        self.accel = accel
+        self._keep = [np.ascontiguousarray(mat_type, np.uint8), np.ascontiguousarray(kd, np.float64),
+                      np.ascontiguousarray(ks, np.float64), np.ascontiguousarray(roughness, np.float64),
+                      np.ascontiguousarray(light_prim, np.uint32), np.ascontiguousarray(light_radiance, np.float64)]
+        k = self._keep
+        self.h = lib().orc_dl_scene_create(accel.h, k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data,
+                                           k[3].ctypes.data, len(k[4]), k[4].ctypes.data, k[5].ctypes.data)
+
+    def __del__(self):
+        try:
+            lib().orc_dl_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def render(self, cam15, width, height, spp, light_samples=4, bsdf_samples=4, seed=0, p0=0, p1=None,
+               primary=False, nthreads=None):
+        """Returns (rgb (P,3) float32 for pixels [p0, p1), ray counts [primary, shadow, bsdf],
+        and optionally the primary (prim, tri) per sample)."""
+        p1 = width * height if p1 is None else p1
+        cam = np.ascontiguousarray(cam15, np.float64)
+        rgb = np.zeros((p1 - p0, 3), np.float32)
+        cnt = np.zeros(3, np.uint64)
+        pp = np.zeros((p1 - p0) * spp, np.uint32) if primary else None
+        pt = np.zeros((p1 - p0) * spp, np.uint32) if primary else None
+        lib().orc_dl_render(self.accel.h, self.h, cam.ctypes.data, width, height, spp, light_samples, bsdf_samples,
+                            seed, p0, p1, rgb.ctypes.data, cnt.ctypes.data, _ptr(pp), _ptr(pt), nthreads or threads())
+        if primary:
+            return rgb, cnt, pp, pt
+        return rgb, cnt
 
 
 def cdir_encode(x, y, z):

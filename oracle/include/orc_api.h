@@ -37,6 +37,27 @@ void orc_closest_hit(const orc_accel *h, uint64_t n, const double *rays, orc_hit
 void orc_any_hit(const orc_accel *h, uint64_t n, const double *rays, uint8_t *out, int nthreads);
 void orc_brute_closest(const orc_accel *h, uint64_t n, const double *rays, orc_hit *hits, int nthreads);
 double orc_next_min_t(const orc_accel *h, uint32_t prim, const orc_hit_geom *g, const double *dir3);
+int orc_accel_intersect_one(const orc_accel *h, const double *ray, orc_hit *hit, orc_hit_geom *geom);
+uint32_t orc_accel_prim_count(const orc_accel *h);
+uint32_t orc_accel_mesh_tri_count(const orc_accel *h, uint32_t prim);
+int orc_accel_mesh_inverted(const orc_accel *h, uint32_t prim);
+void orc_accel_mesh_tri(const orc_accel *h, uint32_t prim, uint32_t tri, float *v9);
+
+/* Direct-lighting render (DirectLightingIntegrator with matte / plastic materials and diffuse
+ * area lights). mat_type: 0 matte (Lambertian), 1 plastic (Lambertian + Blinn microfacet with
+ * FresnelDielectric(1.5, 1)); kd, ks 3 per prim; roughness per prim. Lights: prim index (its mesh
+ * in world space) and RGB radiance. */
+typedef struct orc_dl_scene orc_dl_scene;
+orc_dl_scene *orc_dl_scene_create(const orc_accel *accel, const uint8_t *mat_type, const double *kd, const double *ks,
+                                  const double *roughness, uint32_t n_lights, const uint32_t *light_prim,
+                                  const double *light_radiance);
+void orc_dl_scene_destroy(orc_dl_scene *h);
+/* cam: origin[3], camera-to-world rotation[9] (row-major), x_tan, y_tan. Pixels [p0, p1);
+ * rgb 3 floats per pixel of the range; counters: primary, shadow, BSDF rays cast. */
+void orc_dl_render(const orc_accel *accel, const orc_dl_scene *dls, const double *cam, uint32_t width, uint32_t height,
+                   uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0,
+                   uint64_t p1, float *rgb, uint64_t *counters, uint32_t *primary_prim, uint32_t *primary_tri,
+                   int nthreads);
 
 /* Generic point kd-tree (KDTree<Point3D_d> / KDTree<Point3D_f>) for the reference's own tests. */
 orc_kdtree *orc_kdtree_create_d(uint64_t n, const double *pts);

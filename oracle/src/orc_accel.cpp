@@ -694,4 +694,34 @@ double orc_next_min_t(const orc_accel *h, uint32_t prim, const orc_hit_geom *g, 
   return a->next_min_t(prim, *g, V3{dir3[0], dir3[1], dir3[2]});
 }
 
+int orc_accel_intersect_one(const orc_accel *h, const double *ray, orc_hit *hit, orc_hit_geom *geom) {
+  const Accel *a = reinterpret_cast<const Accel *>(h);
+  return a->intersect(load_ray(ray), hit, geom, nullptr, nullptr) ? 1 : 0;
+}
+
+uint32_t orc_accel_prim_count(const orc_accel *h) {
+  return (uint32_t) reinterpret_cast<const Accel *>(h)->prim_mesh.size();
+}
+
+uint32_t orc_accel_mesh_tri_count(const orc_accel *h, uint32_t prim) {
+  const Accel *a = reinterpret_cast<const Accel *>(h);
+  return (uint32_t)(a->meshes[a->prim_mesh[prim]].tris.size() / 3);
+}
+
+int orc_accel_mesh_inverted(const orc_accel *h, uint32_t prim) {
+  const Accel *a = reinterpret_cast<const Accel *>(h);
+  return a->meshes[a->prim_mesh[prim]].invert_normals ? 1 : 0;
+}
+
+void orc_accel_mesh_tri(const orc_accel *h, uint32_t prim, uint32_t tri, float *v9) {
+  const Accel *a = reinterpret_cast<const Accel *>(h);
+  const Mesh &m = a->meshes[a->prim_mesh[prim]];
+  for (int k = 0; k < 3; ++k) {
+    F3 p = m.verts[m.tris[3 * tri + k]];
+    v9[3 * k] = p.x;
+    v9[3 * k + 1] = p.y;
+    v9[3 * k + 2] = p.z;
+  }
+}
+
 }  // extern "C"

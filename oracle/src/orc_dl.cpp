@@ -1,0 +1,578 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see orc_math.hpp). Never linked into the product library.
+//
+// CPU restatement of Skwarka's direct-lighting render for scenes of matte / plastic surfaces lit
+// by diffuse area lights, one camera sample at a time:
+//   LTEIntegrator::Radiance                     Source/RayLibs/Raytracer/Core/LTEIntegrator.cpp:28-56
+//   DirectLightingLTEIntegrator::_SurfaceRadiance  Raytracer/LTEIntegrators/DirectLightingLTEIntegrator.cpp:33-60
+//   DirectLightingIntegrator::ComputeDirectLighting / _SampleLights / _SampleBSDF
+//                                               Raytracer/Core/DirectLightingIntegrator.cpp:54-366
+//   BSDF::Evaluate / PDF / Sample, frame        Raytracer/Core/BSDF.cpp:22-200, BSDF.h:182-244
+//   BxDF::Sample / PDF (cosine hemisphere)      Raytracer/Core/BxDF.cpp:20-41; Lambertian.cpp:18-31
+//   Microfacet<FresnelDielectric, Blinn>        Raytracer/BxDFs/Microfacet.h:89-152
+//   BlinnDistribution                           Raytracer/MicrofacetDistributions/BlinnDistribution.h:70-110
+//   FresnelDielectric                           Raytracer/Core/Fresnel.h:92-124
+//   MatteMaterial (sigma = 0) / PlasticMaterial Raytracer/Materials/MatteMaterial.cpp:29-48, PlasticMaterial.cpp:33-52
+//   DiffuseAreaLightSource                      Raytracer/LightSources/DiffuseAreaLightSource.cpp:20-174
+//   IrradianceLightsSamplingStrategy            Raytracer/LightsSamplingStrategies/IrradianceLightsSamplingStrategy.cpp:19-129
+//   SubtendedSolidAngle, CoordinateSystem, BinarySearchCDF  Source/RayLibs/Math/MathRoutines.h:192-215, 298-390
+//   ConcentricDiskSampling, CosineHemisphereSampling, UniformTriangleSampling, PowerHeuristic
+//                                               Source/RayLibs/Math/SamplingRoutines.h:197-325, 449-455
+//   TriangleMesh::GetTriangleNormal, Triangle3D::GetArea   TriangleMesh.h:219-232, Math/Triangle3D.h:82-86
+//   PerspectiveCamera::GenerateRay              Raytracer/Cameras/PerspectiveCamera.cpp:36-64
+//   NaN / negative sample clamp                 Raytracer/Renderers/SamplerBasedRenderer.cpp:459-478
+// Spectrum luminance weights are the Y row of the reference's spectrum-to-XYZ matrix
+// (SpectrumRoutines.cpp:20-41 summed over its CIE Y table for the bands 360-494, 495-574, 575-830 nm).
+// Samples come from a counter-based stream (splitmix64 of seed, sample index and dimension) in
+// place of the reference's samplers; the product reproduces the same stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../include/orc_api.h"
+#include "orc_math.hpp"
+
+namespace orc {
+
+// Spectrum_d luminance: Y row of m_Spectrum_to_XYZ (SpectrumRoutines.cpp:55-62).
+static const double LUM_W[3] = {0x1.b38293da589a6p-5, 0x1.2b993cc221566p-1, 0x1.725d3400723f6p-2};
+static inline double luminance(const double s[3]) {
+  double r = 0.0;
+  for (int j = 0; j < 3; ++j) r += LUM_W[j] * s[j];
+  return r;
+}
+
+static inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// Uniform double in [0, 1) for (seed, sample index, dimension).
+static inline double rnd(uint64_t seed, uint64_t sample, uint32_t dim) {
+  uint64_t h = mix64(seed ^ mix64(sample * 64ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+// Camera and material description: see orc_api.h.
+struct Light {
+  uint32_t prim;
+  double radiance[3];
+  std::vector<double> cdf;       // normalised area CDF over the light mesh's triangles
+  std::vector<V3> normal;        // float GetTriangleNormal, as double
+  std::vector<V3> v0, v1, v2;    // float vertices as double
+  double area = 0.0;
+  BBoxD bbox;
+  double intensity = 0.0;
+};
+
+struct DLScene {
+  std::vector<uint8_t> mat;
+  std::vector<double> kd, ks, rough;
+  std::vector<Light> lights;
+  std::vector<int> light_of_prim;
+};
+
+// Triangle3D<float>::GetArea: float cross product, float length, (float)0.5 * length.
+static float tri_area_f(F3 a, F3 b, F3 c) {
+  float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+  float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+  float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+  return 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+// TriangleMesh::GetTriangleNormal: float cross product normalised in float (Vector3D<float>).
+static V3 tri_normal_f(F3 a, F3 b, F3 c, bool invert) {
+  float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+  float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+  float cx, cy, cz;
+  if (invert) {
+    cx = e2y * e1z - e2z * e1y;
+    cy = e2z * e1x - e2x * e1z;
+    cz = e2x * e1y - e2y * e1x;
+  } else {
+    cx = e1y * e2z - e1z * e2y;
+    cy = e1z * e2x - e1x * e2z;
+    cz = e1x * e2y - e1y * e2x;
+  }
+  float l2 = cx * cx + cy * cy + cz * cz;
+  if (l2 > 0.0f) {
+    double inv = 1.0 / (double)std::sqrt(l2);
+    cx = (float)(cx * inv);
+    cy = (float)(cy * inv);
+    cz = (float)(cz * inv);
+  }
+  return V3{cx, cy, cz};
+}
+
+// MathRoutines::_SubtendedSolidAngle / SubtendedSolidAngle
+static double solid_angle3(V3 d1, V3 d2, V3 d3) {
+  double det = dot(d1, cross(d2, d3));
+  double denom = 1 + dot(d1, d2) + dot(d1, d3) + dot(d2, d3);
+  if (denom > 0.0) return std::fabs(std::atan(det / denom) * 2.0);
+  if (denom < 0.0) return 2.0 * (PI_ - std::fabs(std::atan(det / denom)));
+  return 2.0 * PI_;
+}
+static double subtended_solid_angle(V3 p, const BBoxD &b) {
+  V3 dirs[8];
+  for (int i = 0; i < 8; ++i) {
+    V3 v{(i & 1) == 0 ? b.mn[0] : b.mx[0], (i & 2) == 0 ? b.mn[1] : b.mx[1], (i & 4) == 0 ? b.mn[2] : b.mx[2]};
+    V3 d = sub(v, p);
+    normalize(d);
+    dirs[i] = d;
+  }
+  bool inside = true;
+  double sum = 0.0;
+  if (p.x < b.mn[0] - DBL_EPS_) {
+    sum += solid_angle3(dirs[0], dirs[2], dirs[6]) + solid_angle3(dirs[0], dirs[4], dirs[6]);
+    inside = false;
+  } else if (p.x > b.mx[0] + DBL_EPS_) {
+    sum += solid_angle3(dirs[1], dirs[3], dirs[7]) + solid_angle3(dirs[1], dirs[5], dirs[7]);
+    inside = false;
+  }
+  if (p.y < b.mn[1] - DBL_EPS_) {
+    sum += solid_angle3(dirs[0], dirs[1], dirs[5]) + solid_angle3(dirs[0], dirs[4], dirs[5]);
+    inside = false;
+  } else if (p.y > b.mx[1] + DBL_EPS_) {
+    sum += solid_angle3(dirs[2], dirs[3], dirs[7]) + solid_angle3(dirs[2], dirs[6], dirs[7]);
+    inside = false;
+  }
+  if (p.z < b.mn[2] - DBL_EPS_) {
+    sum += solid_angle3(dirs[0], dirs[1], dirs[3]) + solid_angle3(dirs[0], dirs[2], dirs[3]);
+    inside = false;
+  } else if (p.z > b.mx[2] + DBL_EPS_) {
+    sum += solid_angle3(dirs[4], dirs[5], dirs[7]) + solid_angle3(dirs[4], dirs[6], dirs[7]);
+    inside = false;
+  }
+  return inside ? 4.0 * PI_ : sum;
+}
+
+static bool bbox_inside(const BBoxD &b, V3 p) {
+  return p.x >= b.mn[0] && p.y >= b.mn[1] && p.z >= b.mn[2] && p.x <= b.mx[0] && p.y <= b.mx[1] && p.z <= b.mx[2];
+}
+
+// IrradianceLightsSamplingStrategy::GetLightsCDF(point, normal) for area lights only.
+static void lights_cdf(const DLScene &s, V3 p, V3 n, double *cdf) {
+  size_t nl = s.lights.size();
+  for (size_t i = 0; i < nl; ++i) {
+    const Light &L = s.lights[i];
+    if (bbox_inside(L.bbox, p)) {
+      cdf[i] = PI_ * L.intensity;
+    } else {
+      double avg = 0.0;
+      for (int j = 0; j < 8; ++j) {
+        V3 v{(j & 1) ? L.bbox.mn[0] : L.bbox.mx[0], (j & 2) ? L.bbox.mn[1] : L.bbox.mx[1],
+             (j & 4) ? L.bbox.mn[2] : L.bbox.mx[2]};
+        V3 d = sub(v, p);
+        normalize(d);
+        avg += smax(0.0, dot(d, n));
+      }
+      avg *= 0.125;
+      if (avg > 0.0) {
+        if (avg < 0.1) avg = 0.1;
+        cdf[i] = subtended_solid_angle(p, L.bbox) * L.intensity * avg;
+      } else {
+        cdf[i] = 0.0;
+      }
+    }
+  }
+  for (size_t i = 1; i < nl; ++i) cdf[i] += cdf[i - 1];
+  if (cdf[nl - 1] > 0.0) {
+    double inv = 1.0 / cdf[nl - 1];
+    for (size_t i = 0; i < nl; ++i) cdf[i] *= inv;
+  } else {
+    double inv = 1.0 / (double)nl;
+    for (size_t i = 0; i < nl; ++i) cdf[i] = ((double)i + 1.0) * inv;
+  }
+}
+
+// MathRoutines::BinarySearchCDF
+static size_t search_cdf(const double *b, size_t n, double u, double *pdf) {
+  size_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    size_t mid = lo + (hi - lo) / 2;
+    if (b[mid] <= u) lo = mid + 1;
+    else hi = mid;
+  }
+  if (pdf) *pdf = lo == 0 ? b[0] : b[lo] - b[lo - 1];
+  return lo;
+}
+
+// SamplingRoutines::ConcentricDiskSampling
+static void concentric_disk(double u0, double u1, double &x, double &y) {
+  double s0 = 2.0 * u0 - 1.0, s1 = 2.0 * u1 - 1.0;
+  if (s0 == 0.0 && s1 == 0.0) {
+    x = y = 0.0;
+    return;
+  }
+  double r, theta;
+  if (s0 >= -s1) {
+    if (s0 > s1) {
+      r = s0;
+      theta = s1 > 0.0 ? s1 / r : 8.0 + s1 / r;
+    } else {
+      r = s1;
+      theta = 2.0 - s0 / r;
+    }
+  } else {
+    if (s0 <= s1) {
+      r = -s0;
+      theta = 4.0 - s1 / r;
+    } else {
+      r = -s1;
+      theta = 6.0 + s0 / r;
+    }
+  }
+  theta *= PI_ / 4.0;
+  x = r * std::cos(theta);
+  y = r * std::sin(theta);
+}
+
+// FresnelDielectric(1.5, 1.0)
+static double fresnel_dielectric(double c) {
+  double ni = 1.5, no = 1.0;
+  c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+  if (c < 0.0) std::swap(ni, no);
+  double sin_t = no / ni * std::sqrt(smax(0.0, 1.0 - c * c));
+  if (sin_t >= 1.0) return 1.0;
+  double cos_t = std::sqrt(smax(0.0, 1.0 - sin_t * sin_t));
+  c = std::fabs(c);
+  double rpar = ((ni * c) - (no * cos_t)) / ((ni * c) + (no * cos_t));
+  double rper = ((no * c) - (ni * cos_t)) / ((no * c) + (ni * cos_t));
+  return (rpar * rpar + rper * rper) * 0.5;
+}
+
+struct Bsdf {
+  V3 n, ng, e1, e2;
+  bool plastic;
+  double kd[3], ks[3];
+  double e;  // Blinn exponent
+
+  V3 to_local(V3 v) const { return V3{dot(v, e1), dot(v, e2), dot(v, n)}; }
+  V3 to_world(V3 v) const { return add(add(mul(e1, v.x), mul(e2, v.y)), mul(n, v.z)); }
+
+  double blinn_d(V3 h) const { return (e + 2.0) * 0.15915494309189533577 * std::pow(std::fabs(h.z), e); }
+  double blinn_pdf(V3 wo, V3 wi) const {
+    V3 h = add(wo, wi);
+    normalize(h);
+    double ct = std::fabs(h.z);
+    return ((e + 1.0) * std::pow(ct, e)) / (2.0 * PI_ * 4.0 * std::fabs(dot(wo, h)));
+  }
+  // Microfacet::Evaluate(incident, exitant), scalar part (multiplied by ks per channel)
+  bool micro_eval(V3 wo, V3 wi, double &scalar, double &fres) const {
+    double ci = std::fabs(wo.z), co = std::fabs(wi.z);
+    if (ci < DBL_EPS_ || co < DBL_EPS_) return false;
+    V3 h = add(wo, wi);
+    if (std::fabs(h.x) < DBL_EPS_ && std::fabs(h.y) < DBL_EPS_ && std::fabs(h.z) < DBL_EPS_) return false;
+    normalize(h);
+    fres = fresnel_dielectric(dot(wi, h));
+    double ch = std::fabs(h.z);
+    double tmp = 2.0 * ch / std::fabs(dot(wo, h));
+    double g = smin(1.0, smin(ci * tmp, co * tmp));
+    scalar = blinn_d(h) * g / (4.0 * ci * co);
+    return true;
+  }
+  double micro_pdf(V3 wo, V3 wi) const {
+    if (wo.z * wi.z < 0.0) return 0.0;
+    return blinn_pdf(wo, wi);
+  }
+  static double lambert_pdf(V3 wo, V3 wi) {
+    bool same = wo.z * wi.z > 0.0;
+    return same ? std::fabs(wi.z) * INV_PI_ : 0.0;
+  }
+  // Sum of the reflection BxDFs (all components here are reflective), local vectors.
+  void eval_reflect_local(V3 wo, V3 wi, double f[3]) const {
+    for (int c = 0; c < 3; ++c) f[c] = kd[c] * INV_PI_;
+    if (plastic) {
+      double sc, fr;
+      if (micro_eval(wo, wi, sc, fr))
+        for (int c = 0; c < 3; ++c) f[c] += (ks[c] * fr) * sc;
+    }
+  }
+  // BSDF::Evaluate(incident, exitant)
+  void evaluate(V3 wo_w, V3 wi_w, double f[3]) const {
+    V3 wo = to_local(wo_w), wi = to_local(wi_w);
+    if (dot(wo_w, ng) * dot(wi_w, ng) > 0) {
+      eval_reflect_local(wo, wi, f);
+    } else {
+      f[0] = f[1] = f[2] = 0.0;
+    }
+  }
+  // BSDF::PDF(incident, exitant) over all components
+  double pdf(V3 wo_w, V3 wi_w) const {
+    V3 wo = to_local(wo_w), wi = to_local(wi_w);
+    double p = lambert_pdf(wo, wi);
+    if (!plastic) return p;
+    p += micro_pdf(wo, wi);
+    return p / 2.0;
+  }
+  // BSDF::Sample with non-specular flags.
+  bool sample(V3 wo_w, double u0, double u1, double uc, V3 &wi_w, double &pdf_out, double f[3]) const {
+    size_t num = plastic ? 2 : 1;
+    size_t comp = smin((size_t)(uc * (double)num), num - 1);
+    V3 wo = to_local(wo_w);
+    V3 wi;
+    double p = 0.0;
+    if (comp == 0) {
+      double dx, dy;
+      concentric_disk(u0, u1, dx, dy);
+      double z = std::sqrt(smax(0.0, 1.0 - dx * dx - dy * dy));
+      wi = V3{dx, dy, z};
+      if (wo.z < 0.0) wi.z *= -1.0;
+      p = lambert_pdf(wo, wi);
+    } else {
+      double ct = std::pow(u0, 1.0 / (e + 1.0));
+      double st = std::sqrt(smax(0.0, 1.0 - ct * ct));
+      double phi = u1 * 2.0 * PI_;
+      V3 h{st * std::cos(phi), st * std::sin(phi), ct};
+      if (wo.z * h.z < 0.0) h.z *= -1.0;
+      wi = sub(mul(h, 2.0 * dot(wo, h)), wo);
+      p = ((e + 1.0) * std::pow(ct, e)) / (2.0 * PI_ * 4.0 * std::fabs(dot(wo, h)));
+      if (wo.z * wi.z < 0.0) p = 0.0;
+    }
+    wi_w = to_world(wi);
+    if (num > 1) {
+      p += comp == 0 ? micro_pdf(wo, wi) : lambert_pdf(wo, wi);
+      p /= (double)num;
+    }
+    pdf_out = p;
+    if (dot(wo_w, ng) * dot(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
+    else f[0] = f[1] = f[2] = 0.0;
+    return true;
+  }
+};
+
+static inline double power_heuristic(double n1, double p1, double n2, double p2) {
+  double f = n1 * p1, g = n2 * p2;
+  return (f * f) / (f * f + g * g);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+struct orc_dl_scene {
+  DLScene s;
+};
+
+extern "C" {
+
+orc_dl_scene *orc_dl_scene_create(const orc_accel *accel, const uint8_t *mat_type, const double *kd, const double *ks,
+                                  const double *roughness, uint32_t n_lights, const uint32_t *light_prim,
+                                  const double *light_radiance) {
+  (void)accel;
+  orc_dl_scene *h = new orc_dl_scene();
+  DLScene &s = h->s;
+  uint32_t np = orc_accel_prim_count(accel);
+  s.mat.assign(mat_type, mat_type + np);
+  s.kd.assign(kd, kd + 3 * (size_t)np);
+  s.ks.assign(ks, ks + 3 * (size_t)np);
+  s.rough.assign(roughness, roughness + np);
+  s.light_of_prim.assign(np, -1);
+  for (uint32_t l = 0; l < n_lights; ++l) {
+    Light L;
+    L.prim = light_prim[l];
+    for (int c = 0; c < 3; ++c) L.radiance[c] = light_radiance[3 * l + c];
+    s.light_of_prim[L.prim] = (int)l;
+    uint32_t nt = orc_accel_mesh_tri_count(accel, L.prim);
+    bool inv = orc_accel_mesh_inverted(accel, L.prim) != 0;
+    L.cdf.assign(nt, 0.0);
+    BBoxF bf;
+    for (uint32_t t = 0; t < nt; ++t) {
+      float v[9];
+      orc_accel_mesh_tri(accel, L.prim, t, v);
+      F3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+      L.cdf[t] = tri_area_f(a, b, c);
+      if (t > 0) L.cdf[t] += L.cdf[t - 1];
+      L.normal.push_back(tri_normal_f(a, b, c, inv));
+      L.v0.push_back(to_d(a));
+      L.v1.push_back(to_d(b));
+      L.v2.push_back(to_d(c));
+      bf.unite(a);
+      bf.unite(b);
+      bf.unite(c);
+    }
+    if (nt > 0) {
+      L.area = L.cdf.back();
+      if (L.area > 0.0) {
+        double inv_area = 1.0 / L.area;
+        for (double &x : L.cdf) x *= inv_area;
+      }
+    }
+    L.bbox.unite(bf);
+    // Power = (pi * area) * radiance; intensity = Luminance(power) / (pi * bbox area)
+    double power[3];
+    for (int c = 0; c < 3; ++c) power[c] = (PI_ * L.area) * L.radiance[c];
+    L.intensity = luminance(power) / (PI_ * L.bbox.area());
+    s.lights.push_back(L);
+  }
+  return h;
+}
+
+void orc_dl_scene_destroy(orc_dl_scene *h) { delete h; }
+
+// Renders pixels [p0, p1) of a width x height image at spp samples each.
+// cam: origin (3), camera-to-world rotation (9, row-major), x_tan, y_tan. Output rgb (float, 3 per pixel of
+// the range), and counters: [0] primary rays, [1] shadow rays, [2] BSDF rays.
+void orc_dl_render(const orc_accel *accel, const orc_dl_scene *dls, const double *cam, uint32_t width, uint32_t height,
+                   uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0,
+                   uint64_t p1, float *rgb, uint64_t *counters, uint32_t *primary_prim, uint32_t *primary_tri,
+                   int nthreads) {
+  const DLScene &s = dls->s;
+  const size_t nl = s.lights.size();
+  uint64_t c_prim = 0, c_shadow = 0, c_bsdf = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : c_prim, c_shadow, c_bsdf)
+  for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix) {
+    uint32_t px = (uint32_t)(pix % width), py = (uint32_t)(pix / width);
+    double acc[3] = {0, 0, 0};
+    std::vector<double> cdf(nl > 0 ? nl : 1);
+    for (uint32_t si = 0; si < spp; ++si) {
+      uint64_t sidx = (uint64_t)pix * spp + si;
+      // PerspectiveCamera::GenerateRay with lens radius 0
+      double ix = (double)px + rnd(seed, sidx, 0), iy = (double)py + rnd(seed, sidx, 1);
+      double cx = 2.0 * cam[12] * (ix - width * 0.5) / width;
+      double cy = 2.0 * cam[13] * (iy - height * 0.5) / height;
+      V3 d{cam[3] * cx + cam[4] * cy + cam[5], cam[6] * cx + cam[7] * cy + cam[8], cam[9] * cx + cam[10] * cy + cam[11]};
+      normalize(d);
+      double ray[8] = {cam[0], cam[1], cam[2], d.x, d.y, d.z, 0.0, DBL_INF_};
+      orc_hit h;
+      orc_hit_geom g;
+      orc_accel_intersect_one(accel, ray, &h, &g);
+      ++c_prim;
+      if (primary_prim) {
+        primary_prim[(pix - p0) * spp + si] = h.prim;
+        primary_tri[(pix - p0) * spp + si] = h.tri;
+      }
+      double L[3] = {0, 0, 0};
+      double cdl[3] = {0, 0, 0};
+      if (h.prim != 0xffffffffu) {
+        V3 wo = mul(d, -1.0);
+        V3 p{g.p[0], g.p[1], g.p[2]}, ng{g.ng[0], g.ng[1], g.ng[2]}, ns{g.ns[0], g.ns[1], g.ns[2]};
+        int li = s.light_of_prim[h.prim];
+        if (li >= 0 && dot(wo, ng) > 0.0)
+          for (int c = 0; c < 3; ++c) L[c] = s.lights[li].radiance[c];
+        // BSDF frame (BSDF.h:182-192) with the DG tangent of a mesh without UVs or tangents
+        // (TriangleMesh.cpp:520-540): CoordinateSystem(ns), then made orthogonal to ns.
+        V3 t;
+        if (std::fabs(ns.x) > std::fabs(ns.y)) {
+          double il = 1.0 / std::sqrt(ns.x * ns.x + ns.z * ns.z);
+          t = V3{-ns.z * il, 0.0, ns.x * il};
+        } else {
+          double il = 1.0 / std::sqrt(ns.y * ns.y + ns.z * ns.z);
+          t = V3{0.0, ns.z * il, -ns.y * il};
+        }
+        V3 t2 = cross(ns, t);
+        V3 tangent;
+        if (len_sqr(t2) > DBL_EPS_) {
+          tangent = cross(t2, ns);
+          normalize(tangent);
+        } else {
+          tangent = t;
+        }
+        Bsdf b;
+        b.n = ns;
+        b.ng = ng;
+        b.e1 = tangent;
+        b.e2 = cross(ns, tangent);
+        uint32_t pr = h.prim;
+        b.plastic = s.mat[pr] == 1;
+        for (int c = 0; c < 3; ++c) {
+          b.kd[c] = smin(1.0, smax(0.0, s.kd[3 * pr + c]));
+          b.ks[c] = smin(1.0, smax(0.0, s.ks[3 * pr + c]));
+        }
+        double rough = s.rough[pr] < 0.001 ? 0.001 : (s.rough[pr] > 1.0 ? 1.0 : s.rough[pr]);
+        double ex = 1.0 / rough;
+        b.e = ex < 0.0 ? 0.0 : (ex > 1000.0 ? 1000.0 : ex);
+        if (nl > 0) {
+          V3 nrm = dot(wo, ns) >= 0.0 ? ns : mul(ns, -1.0);
+          lights_cdf(s, p, nrm, cdf.data());
+          double dl[3] = {0, 0, 0};
+          // _SampleLights (area lights only)
+          for (uint32_t k = 0; k < n_light_samples; ++k) {
+            double uc = rnd(seed, sidx, 2 + k);
+            double u0 = rnd(seed, sidx, 6 + 2 * k), u1 = rnd(seed, sidx, 7 + 2 * k);
+            double comp_pdf;
+            size_t l = search_cdf(cdf.data(), nl, uc, &comp_pdf);
+            const Light &Lt = s.lights[l];
+            double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
+            if (Lt.cdf.empty()) continue;
+            size_t ti = search_cdf(Lt.cdf.data(), Lt.cdf.size(), tri_u, nullptr);
+            double tmp = std::sqrt(u0);
+            double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
+            V3 sp = add(add(mul(Lt.v0[ti], bb1), mul(Lt.v1[ti], bb2)), mul(Lt.v2[ti], 1.0 - bb1 - bb2));
+            V3 lv = sub(sp, p);
+            double len = std::sqrt(len_sqr(lv));
+            V3 wi = mul(lv, 1.0 / len);
+            double ac = std::fabs(dot(wi, Lt.normal[ti]));
+            double lpdf = ac > DBL_EPS_ ? (len * len) / (Lt.area * ac) : 0.0;
+            bool lit = dot(Lt.normal[ti], wi) < 0.0;
+            double le[3] = {lit ? Lt.radiance[0] : 0.0, lit ? Lt.radiance[1] : 0.0, lit ? Lt.radiance[2] : 0.0};
+            if (!(lpdf > 0.0) || (le[0] == 0.0 && le[1] == 0.0 && le[2] == 0.0)) continue;
+            double f[3];
+            b.evaluate(wo, wi, f);
+            double lf[3] = {le[0] * f[0], le[1] * f[1], le[2] * f[2]};
+            if (lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0) continue;
+            double mint = orc_next_min_t(accel, pr, &g, &wi.x);
+            double sray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, len - 1e-4};
+            uint8_t occ = 0;
+            orc_any_hit(accel, 1, sray, &occ, 1);
+            ++c_shadow;
+            if (occ) continue;
+            double bp = b.pdf(wo, wi);
+            double w = power_heuristic((double)n_light_samples, lpdf * comp_pdf, (double)n_bsdf_samples, bp);
+            w *= std::fabs(dot(wi, ns)) / (lpdf * comp_pdf);
+            for (int c = 0; c < 3; ++c) dl[c] += lf[c] * w;
+          }
+          // ComputeDirectLighting = (0 + _SampleLights) + _SampleBSDF; each divides by its sample
+          // count through Spectrum::operator/ (multiplication by the reciprocal).
+          for (int c = 0; c < 3; ++c) cdl[c] = dl[c] * (1.0 / (double)n_light_samples);
+          // _SampleBSDF
+          double db[3] = {0, 0, 0};
+          for (uint32_t k = 0; k < n_bsdf_samples; ++k) {
+            double uc = rnd(seed, sidx, 14 + k);
+            double u0 = rnd(seed, sidx, 18 + 2 * k), u1 = rnd(seed, sidx, 19 + 2 * k);
+            V3 wi;
+            double bp, f[3];
+            b.sample(wo, u0, u1, uc, wi, bp, f);
+            if (!(bp > 0.0) || (f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) continue;
+            double mint = orc_next_min_t(accel, pr, &g, &wi.x);
+            double bray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, DBL_INF_};
+            orc_hit h2;
+            orc_hit_geom g2;
+            orc_accel_intersect_one(accel, bray, &h2, &g2);
+            ++c_bsdf;
+            if (h2.prim == 0xffffffffu) continue;
+            int l2 = s.light_of_prim[h2.prim];
+            if (l2 < 0) continue;
+            V3 ng2{g2.ng[0], g2.ng[1], g2.ng[2]};
+            if (!(dot(mul(wi, -1.0), ng2) > 0.0)) continue;
+            const Light &Lt = s.lights[l2];
+            if (Lt.radiance[0] == 0.0 && Lt.radiance[1] == 0.0 && Lt.radiance[2] == 0.0) continue;
+            V3 ln = Lt.normal[h2.tri];
+            double dist = h2.t - mint;
+            double ac = std::fabs(dot(wi, ln));
+            double lpdf = ac > DBL_EPS_ ? (dist * dist) / (Lt.area * ac) : 0.0;
+            double cp = l2 == 0 ? cdf[0] : cdf[l2] - cdf[l2 - 1];
+            double w = power_heuristic((double)n_bsdf_samples, bp, (double)n_light_samples, lpdf * cp);
+            w *= std::fabs(dot(wi, ns)) / bp;
+            for (int c = 0; c < 3; ++c) db[c] += f[c] * Lt.radiance[c] * w;
+          }
+          for (int c = 0; c < 3; ++c) cdl[c] += db[c] * (1.0 / (double)n_bsdf_samples);
+        }
+        for (int c = 0; c < 3; ++c) L[c] = L[c] + cdl[c];
+      }
+      // SamplerBasedRenderer.cpp:458-478: NaN, negative-luminance and infinite samples become black
+      bool bad = std::isnan(L[0]) || std::isnan(L[1]) || std::isnan(L[2]) || luminance(L) < -DBL_EPS_ ||
+                 std::isinf(L[0]) || std::isinf(L[1]) || std::isinf(L[2]);
+      if (!bad)
+        for (int c = 0; c < 3; ++c) acc[c] += L[c];
+    }
+    for (int c = 0; c < 3; ++c) rgb[3 * (pix - p0) + c] = (float)(acc[c] / (double)spp);
+  }
+  if (counters) {
+    counters[0] = c_prim;
+    counters[1] = c_shadow;
+    counters[2] = c_bsdf;
+  }
+}
+
+}  // extern "C"

@@ -28,7 +28,7 @@ EXPORTS = [
     "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
     "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_last_error",
-    "skw_device_count",
+    "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
 ]
 
 _P = ctypes.c_void_p
@@ -42,6 +42,21 @@ class SceneDesc(ctypes.Structure):
         ("mesh_tri_offset", _P), ("tris", _P), ("mesh_flags", _P),
         ("n_prims", _U32), ("prim_mesh", _P), ("prim_xform", _P),
     ]
+
+
+class DLDesc(ctypes.Structure):
+    _fields_ = [("mat_type", _P), ("kd", _P), ("ks", _P), ("roughness", _P), ("n_lights", _U32),
+                ("light_prim", _P), ("light_radiance", _P)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_double * 3), ("rot", ctypes.c_double * 9), ("x_tan", ctypes.c_double),
+                ("y_tan", ctypes.c_double)]
+
+
+class DLParams(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("light_samples", _U32),
+                ("bsdf_samples", _U32), ("seed", _U64)]
 
 
 _lib = None
@@ -76,6 +91,9 @@ def lib():
         "skw_photon_nearest": ([_P, _P, _P, _U64, ctypes.c_double, ctypes.c_int, _P, _P], ctypes.c_int),
         "skw_last_error": ([], ctypes.c_char_p),
         "skw_device_count": ([], ctypes.c_int),
+        "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_dl_scene_destroy": ([_P], ctypes.c_int),
+        "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -259,3 +277,62 @@ class PhotonMap:
     def nearest(self, d_p, d_n, nq, r, filt, d_id, stream=0):
         check(lib().skw_photon_nearest(self.handle, d_p, d_n, nq, r, filt, d_id, stream or None),
               "skw_photon_nearest")
+
+
+def camera(origin, target_dir, up, x_fov_deg, width, height):
+    """skw_camera for a pinhole looking along target_dir (look-at frame of Transform.cpp:138-165,
+    tangents of PerspectiveCamera.cpp:20-34). Returns (Camera, cam15) where cam15 is the flat
+    layout the oracle takes: origin, rotation (row-major), x_tan, y_tan."""
+    d = np.asarray(target_dir, np.float64)
+    d = d * (1.0 / np.sqrt(np.dot(d, d)))
+    right = np.cross(d, np.asarray(up, np.float64))
+    right = right * (1.0 / np.sqrt(np.dot(right, right)))
+    new_up = np.cross(right, d)
+    rot = np.stack([right, -new_up, d], 1)
+    x_tan = float(np.tan(np.deg2rad(x_fov_deg) / 2.0))
+    y_tan = x_tan * float(height) / float(width)
+    c = Camera()
+    for i in range(3):
+        c.origin[i] = float(origin[i])
+    for i in range(9):
+        c.rot[i] = float(rot.reshape(-1)[i])
+    c.x_tan, c.y_tan = x_tan, y_tan
+    cam15 = np.concatenate([np.asarray(origin, np.float64), rot.reshape(-1), [x_tan, y_tan]])
+    return c, cam15
+
+
+class DLScene:
+    """Materials and area lights for skw_dl_render over a Scene."""
+
+    def __init__(self, scene, mat_type, kd, ks, roughness, light_prim, light_radiance):
+        L = lib()
+        self._mt = np.ascontiguousarray(mat_type, np.uint8)
+        self._kd = np.ascontiguousarray(kd, np.float64)
+        self._ks = np.ascontiguousarray(ks, np.float64)
+        self._r = np.ascontiguousarray(roughness, np.float64)
+        self._lp = np.ascontiguousarray(light_prim, np.uint32)
+        self._lr = np.ascontiguousarray(light_radiance, np.float64)
+        desc = DLDesc(_ptr(self._mt), _ptr(self._kd), _ptr(self._ks), _ptr(self._r), len(self._lp), _ptr(self._lp),
+                      _ptr(self._lr))
+        h = _P()
+        check(L.skw_dl_scene_create(scene.handle, ctypes.byref(desc), ctypes.byref(h)), "skw_dl_scene_create")
+        self.handle = h
+        self.scene = scene
+
+    def close(self):
+        if self.handle:
+            lib().skw_dl_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0, stream=0):
+        prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
+        rays = np.zeros(3, np.uint64)
+        check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
+                                  rays.ctypes.data, stream or None), "skw_dl_render")
+        return rays

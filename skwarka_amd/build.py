@@ -17,10 +17,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [
     "abi.cpp",
+    "abi_dl.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",
     "kernels/gather.hip",
+    "kernels/dl.hip",
 ]
 
 COMMON = [

@@ -12,45 +12,32 @@
 #include <string>
 #include <vector>
 
-#include "../../include/skwarka_rt.h"
-#include "host/builders.hpp"
-#include "kernels/kernels.hpp"
+#include "internal.hpp"
 
 using namespace skw;
 
-namespace {
+namespace skw {
 
+namespace {
 thread_local std::string g_err;
+}
 
 skw_status fail(skw_status s, const std::string &msg) {
   g_err = msg;
   return s;
 }
 
+const char *last_error() { return g_err.c_str(); }
+
 skw_status hip_fail(hipError_t e, const char *what) {
   return fail(e == hipErrorOutOfMemory ? SKW_OUT_OF_MEMORY : SKW_HIP_ERROR,
               std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define SKW_HIP(call, what)                     \
-  do {                                          \
-    hipError_t e_ = (call);                     \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 int device_count() {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
-}
-
-template <class T>
-hipError_t upload(T **dst, const std::vector<T> &src) {
-  *dst = nullptr;
-  if (src.empty()) return hipSuccess;
-  hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(T));
-  if (e != hipSuccess) return e;
-  return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
 int env_int(const char *name, int dflt) {
@@ -59,74 +46,100 @@ int env_int(const char *name, int dflt) {
   return std::atoi(v);
 }
 
-// Launch-wide ray counters, one per stream so that calls on different streams never share one.
-struct CounterPool {
-  std::mutex mu;
-  std::map<void *, uint64_t *> by_stream;
-  ~CounterPool() {
-    for (auto &kv : by_stream) (void)hipFree(kv.second);
-  }
-  hipError_t get(void *stream, uint64_t **out) {
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = by_stream.find(stream);
-    if (it != by_stream.end()) {
-      *out = it->second;
-      return hipSuccess;
-    }
-    uint64_t *p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, sizeof(uint64_t));
-    if (e != hipSuccess) return e;
-    by_stream[stream] = p;
-    *out = p;
+CounterPool::~CounterPool() {
+  for (auto &kv : by_stream) (void)hipFree(kv.second);
+}
+
+hipError_t CounterPool::get(void *stream, uint64_t **out) {
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = by_stream.find(stream);
+  if (it != by_stream.end()) {
+    *out = it->second;
     return hipSuccess;
   }
-};
+  uint64_t *p = nullptr;
+  hipError_t e = hipMalloc((void **)&p, sizeof(uint64_t));
+  if (e != hipSuccess) return e;
+  by_stream[stream] = p;
+  *out = p;
+  return hipSuccess;
+}
 
-}  // namespace
-
-struct skw_scene {
-  TreeLayout host;
-  bool on_device = false;
-  bool has_inst = false;
-  bool needs_fix = false;  // some primitive's traversal barycentrics are not mesh-space values
-  int device = 0;
-  int cus = 256;
-  DevNode *d_nodes = nullptr;
-  DevTri *d_tris = nullptr;
-  DevInst *d_insts = nullptr;
-  uint32_t *d_inst_range = nullptr;
-  DevPrim *d_prims = nullptr;
-  DevMesh *d_meshes = nullptr;
-  float *d_mesh_verts = nullptr;
-  float *d_mesh_normals = nullptr;
-  uint32_t *d_mesh_tris = nullptr;
-  CounterPool counters;
-
-  ~skw_scene() {
-    void *ptrs[] = {d_nodes, d_tris, d_insts, d_inst_range, d_prims, d_meshes, d_mesh_verts, d_mesh_normals, d_mesh_tris};
-    for (void *p : ptrs)
-      if (p) (void)hipFree(p);
+skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev,
+                         skw_hit *d_hits, uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops,
+                         uint32_t *d_tests, bool fix) {
+  skw_scene *s = const_cast<skw_scene *>(scene);
+  const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
+  uint64_t waves = (n + 63) / 64;
+  uint64_t cap = (uint64_t)s->cus * (uint64_t)(waves_per_cu > 0 ? waves_per_cu : 16);
+  uint32_t grid = (uint32_t)(waves < cap ? waves : cap);
+  if (grid == 0) grid = 1;
+  TraceArgs a;
+  a.nodes = s->d_nodes;
+  a.tris = s->d_tris;
+  a.insts = s->d_insts;
+  a.inst_range = s->d_inst_range;
+  a.root = s->host.root;
+  a.rays = d_rays;
+  a.hits = d_hits;
+  a.occluded = d_occ;
+  a.n = n;
+  a.n_dev = n_dev;
+  a.gstack = nullptr;
+  a.pops = d_pops;
+  a.tests = d_tests;
+  const bool stats = d_pops != nullptr || d_tests != nullptr;
+  SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
+  uint32_t need = s->host.stack_need;
+  if (need > 128) {
+    SKW_HIP(hipMallocAsync((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
   }
-};
-
-struct skw_photon_map {
-  KdLayout host;
-  bool on_device = false;
-  int device = 0;
-  DevKdNode *d_nodes = nullptr;
-  DevPhoton *d_photons = nullptr;
-  uint32_t *d_perm = nullptr;
-  double *d_cdir = nullptr;
-  ~skw_photon_map() {
-    void *ptrs[] = {d_nodes, d_photons, d_perm, d_cdir};
-    for (void *p : ptrs)
-      if (p) (void)hipFree(p);
+  SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
+  hipError_t e = any ? launch_any_hit(a, s->has_inst, need, grid, st)
+                     : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
+  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
+  if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
+  if (!any && fix && s->needs_fix) {
+    GeomArgs g = s->geom_args();
+    g.rays = d_rays;
+    g.hits = d_hits;
+    g.n = n;
+    SKW_HIP(launch_fix_bary(g, st), "barycentric fix-up launch");
   }
-};
+  return SKW_OK;
+}
+
+}  // namespace skw
+
+skw_scene::~skw_scene() {
+  void *ptrs[] = {d_nodes, d_tris, d_insts, d_inst_range, d_prims, d_meshes, d_mesh_verts, d_mesh_normals, d_mesh_tris};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+GeomArgs skw_scene::geom_args() const {
+  GeomArgs g;
+  g.prims = d_prims;
+  g.meshes = d_meshes;
+  g.mesh_verts = d_mesh_verts;
+  g.mesh_normals = d_mesh_normals;
+  g.mesh_tris = d_mesh_tris;
+  g.rays = nullptr;
+  g.hits = nullptr;
+  g.geom = nullptr;
+  g.n = 0;
+  return g;
+}
+
+skw_photon_map::~skw_photon_map() {
+  void *ptrs[] = {d_nodes, d_photons, d_perm, d_cdir};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+}
 
 extern "C" {
 
-const char *skw_last_error(void) { return g_err.c_str(); }
+const char *skw_last_error(void) { return skw::last_error(); }
 
 int skw_device_count(void) { return device_count(); }
 
@@ -231,50 +244,8 @@ static skw_status trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t 
   if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
   if (n == 0) return SKW_OK;
   if (!d_rays || (!any && !d_hits) || (any && !d_occ)) return fail(SKW_INVALID_ARG, "ray/result pointers must not be NULL");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  skw_scene *s = const_cast<skw_scene *>(scene);
-  const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
-  uint64_t waves = (n + 63) / 64;
-  uint64_t cap = (uint64_t)s->cus * (uint64_t)(waves_per_cu > 0 ? waves_per_cu : 16);
-  uint32_t grid = (uint32_t)(waves < cap ? waves : cap);
-  TraceArgs a;
-  a.nodes = s->d_nodes;
-  a.tris = s->d_tris;
-  a.insts = s->d_insts;
-  a.inst_range = s->d_inst_range;
-  a.root = s->host.root;
-  a.rays = d_rays;
-  a.hits = d_hits;
-  a.occluded = d_occ;
-  a.n = n;
-  a.gstack = nullptr;
-  a.pops = d_pops;
-  a.tests = d_tests;
-  const bool stats = d_pops != nullptr || d_tests != nullptr;
-  SKW_HIP(s->counters.get(stream, &a.counter), "counter allocation");
-  uint32_t need = s->host.stack_need;
-  if (need > 128) {
-    SKW_HIP(hipMallocAsync((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
-  }
-  SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
-  hipError_t e = any ? launch_any_hit(a, s->has_inst, need, grid, st)
-                     : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
-  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
-  if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
-  if (!any && s->needs_fix) {
-    GeomArgs g;
-    g.prims = s->d_prims;
-    g.meshes = s->d_meshes;
-    g.mesh_verts = s->d_mesh_verts;
-    g.mesh_normals = s->d_mesh_normals;
-    g.mesh_tris = s->d_mesh_tris;
-    g.rays = d_rays;
-    g.hits = d_hits;
-    g.geom = nullptr;
-    g.n = n;
-    SKW_HIP(launch_fix_bary(g, st), "barycentric fix-up launch");
-  }
-  return SKW_OK;
+  return enqueue_trace(scene, d_rays, n, nullptr, d_hits, d_occ, reinterpret_cast<hipStream_t>(stream), any, d_pops,
+                       d_tests, true);
 }
 
 skw_status skw_closest_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits, void *stream) {

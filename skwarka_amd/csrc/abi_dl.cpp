@@ -1,0 +1,256 @@
+// C ABI of the direct-lighting render (skw_dl_*): host tables for lights and materials and the
+// per-frame wavefront (camera -> closest hit -> shade -> occlusion + BSDF rays -> resolve).
+//
+// Light tables follow DiffuseAreaLightSource's constructor (Raytracer/LightSources/
+// DiffuseAreaLightSource.cpp:20-55): per-triangle float areas accumulated into a double CDF and
+// normalised, float triangle normals (TriangleMesh::GetTriangleNormal, TriangleMesh.h:219-232);
+// intensities follow IrradianceLightsSamplingStrategy's constructor
+// (IrradianceLightsSamplingStrategy.cpp:19-31): luminance of Power() = (pi * area) * radiance over
+// pi times the area of the light mesh's bounding box. Material parameters are clamped as
+// MatteMaterial / PlasticMaterial do (MatteMaterial.cpp:33-34, PlasticMaterial.cpp:37-45).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace skw;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr uint32_t kMaxLights = 16;
+const double kLumW[3] = {0x1.b38293da589a6p-5, 0x1.2b993cc221566p-1, 0x1.725d3400723f6p-2};
+
+double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
+}  // namespace
+
+struct skw_dl_scene {
+  std::vector<DevMat> mats;
+  std::vector<DevLight> lights;
+  std::vector<DevLightTri> ltris;
+  DevMat *d_mats = nullptr;
+  DevLight *d_lights = nullptr;
+  DevLightTri *d_ltris = nullptr;
+  ~skw_dl_scene() {
+    void *ptrs[] = {d_mats, d_lights, d_ltris};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+skw_status skw_dl_scene_create(const skw_scene *scene, const skw_dl_desc *desc, skw_dl_scene **out) {
+  if (!scene || !desc || !out) return fail(SKW_INVALID_ARG, "scene, desc and out must not be NULL");
+  *out = nullptr;
+  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
+  const size_t np = scene->host.prims.size();
+  if (np > 0 && (!desc->mat_type || !desc->kd || !desc->ks || !desc->roughness))
+    return fail(SKW_INVALID_ARG, "material arrays must not be NULL");
+  if (desc->n_lights > kMaxLights) return fail(SKW_UNSUPPORTED, "at most 16 area lights");
+  if (desc->n_lights > 0 && (!desc->light_prim || !desc->light_radiance))
+    return fail(SKW_INVALID_ARG, "light arrays must not be NULL");
+  std::unique_ptr<skw_dl_scene> d(new skw_dl_scene());
+  d->mats.resize(np);
+  for (size_t p = 0; p < np; ++p) {
+    DevMat &m = d->mats[p];
+    if (desc->mat_type[p] > 1) return fail(SKW_INVALID_ARG, "material type must be 0 (matte) or 1 (plastic)");
+    m.type = desc->mat_type[p];
+    for (int c = 0; c < 3; ++c) {
+      m.kd[c] = clamp01(desc->kd[3 * p + c]);
+      m.ks[c] = clamp01(desc->ks[3 * p + c]);
+    }
+    double r = desc->roughness[p];
+    r = r < 0.001 ? 0.001 : (r > 1.0 ? 1.0 : r);
+    double e = 1.0 / r;  // BlinnDistribution(1/roughness), exponent clamped to [0, 1000]
+    m.exponent = e < 0.0 ? 0.0 : (e > 1000.0 ? 1000.0 : e);
+    m.light = kMiss;
+  }
+  for (uint32_t l = 0; l < desc->n_lights; ++l) {
+    uint32_t prim = desc->light_prim[l];
+    if (prim >= np) return fail(SKW_INVALID_ARG, "light prim out of range");
+    const HostPrim &hp = scene->host.prims[prim];
+    if (!hp.identity) return fail(SKW_UNSUPPORTED, "area light primitives must have identity transforms");
+    if (d->mats[prim].light != kMiss) return fail(SKW_INVALID_ARG, "a primitive carries two lights");
+    d->mats[prim].light = l;
+    const HostMesh &mesh = scene->host.meshes[hp.mesh];
+    DevLight L;
+    for (int c = 0; c < 3; ++c) L.radiance[c] = desc->light_radiance[3 * l + c];
+    L.tri_off = (uint32_t)d->ltris.size();
+    L.tri_count = (uint32_t)(mesh.tris.size() / 3);
+    float fmn[3] = {INFINITY, INFINITY, INFINITY}, fmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double acc = 0.0;
+    size_t first = d->ltris.size();
+    for (uint32_t t = 0; t < L.tri_count; ++t) {
+      const float *v[3];
+      for (int k = 0; k < 3; ++k) v[k] = &mesh.verts[3 * mesh.tris[3 * t + k]];
+      // Triangle3D<float>::GetArea: float cross product and length
+      float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+      float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+      float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+      float area = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+      acc = (t == 0) ? (double)area : (double)area + acc;
+      // GetTriangleNormal: float cross of (v1-v0, v2-v0) or (v2-v0, v1-v0), Normalized() in float
+      float nx, ny, nz;
+      if (mesh.invert_normals) {
+        nx = e2[1] * e1[2] - e2[2] * e1[1];
+        ny = e2[2] * e1[0] - e2[0] * e1[2];
+        nz = e2[0] * e1[1] - e2[1] * e1[0];
+      } else {
+        nx = cx;
+        ny = cy;
+        nz = cz;
+      }
+      float l2 = nx * nx + ny * ny + nz * nz;
+      if (l2 > 0.0f) {
+        double inv = 1.0 / (double)std::sqrt(l2);
+        nx = (float)(nx * inv);
+        ny = (float)(ny * inv);
+        nz = (float)(nz * inv);
+      }
+      DevLightTri T;
+      T.cdf = acc;
+      T.n[0] = nx;
+      T.n[1] = ny;
+      T.n[2] = nz;
+      for (int c = 0; c < 3; ++c) {
+        T.v0[c] = v[0][c];
+        T.v1[c] = v[1][c];
+        T.v2[c] = v[2][c];
+        for (int k = 0; k < 3; ++k) {
+          fmn[c] = (v[k][c] < fmn[c]) ? v[k][c] : fmn[c];
+          fmx[c] = (fmx[c] < v[k][c]) ? v[k][c] : fmx[c];
+        }
+      }
+      d->ltris.push_back(T);
+    }
+    L.area = L.tri_count > 0 ? acc : 0.0;
+    if (L.area > 0.0) {
+      double inv_area = 1.0 / L.area;
+      for (size_t t = first; t < d->ltris.size(); ++t) d->ltris[t].cdf *= inv_area;
+    }
+    for (int c = 0; c < 3; ++c) {
+      L.bmin[c] = fmn[c];
+      L.bmax[c] = fmx[c];
+    }
+    double x = std::fabs(L.bmax[0] - L.bmin[0]), y = std::fabs(L.bmax[1] - L.bmin[1]), z = std::fabs(L.bmax[2] - L.bmin[2]);
+    double bbox_area = 2.0 * (x * y + x * z + y * z);
+    double lum = 0.0;
+    for (int c = 0; c < 3; ++c) lum += kLumW[c] * ((kPi * L.area) * L.radiance[c]);
+    L.intensity = lum / (kPi * bbox_area);
+    d->lights.push_back(L);
+  }
+  SKW_HIP(upload(&d->d_mats, d->mats), "upload materials");
+  SKW_HIP(upload(&d->d_lights, d->lights), "upload lights");
+  SKW_HIP(upload(&d->d_ltris, d->ltris), "upload light triangles");
+  *out = d.release();
+  return SKW_OK;
+}
+
+skw_status skw_dl_scene_destroy(skw_dl_scene *dl) {
+  delete dl;
+  return SKW_OK;
+}
+
+skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                         const skw_dl_params *prm, float *d_rgb, uint64_t *rays_out, void *stream) {
+  if (!scene || !dl || !cam || !prm || !d_rgb) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
+  if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(SKW_INVALID_ARG, "empty image");
+  if (prm->light_samples == 0 || prm->light_samples > 8 || prm->bsdf_samples == 0 || prm->bsdf_samples > 8)
+    return fail(SKW_INVALID_ARG, "light_samples and bsdf_samples must be in [1, 8]");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const uint64_t npix = (uint64_t)prm->width * prm->height;
+  const uint64_t batch = npix;
+  const uint32_t nls = prm->light_samples, nbs = prm->bsdf_samples, nl = (uint32_t)dl->lights.size();
+  DlArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.geom = scene->geom_args();
+  a.mats = dl->d_mats;
+  a.lights = dl->d_lights;
+  a.ltris = dl->d_ltris;
+  a.n_lights = nl;
+  for (int i = 0; i < 3; ++i) a.cam_o[i] = cam->origin[i];
+  for (int i = 0; i < 9; ++i) a.cam_r[i] = cam->rot[i];
+  a.x_tan = cam->x_tan;
+  a.y_tan = cam->y_tan;
+  a.width = prm->width;
+  a.height = prm->height;
+  a.spp = prm->spp;
+  a.nls = nls;
+  a.nbs = nbs;
+  a.seed = prm->seed;
+  a.pix0 = 0;
+  a.batch = batch;
+  // One device block holds every per-frame buffer.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  size_t o_prim_rays = take(batch * sizeof(skw_ray)), o_prim_hits = take(batch * sizeof(skw_hit));
+  size_t o_le = take(batch * 3 * sizeof(double)), o_cdf = take(batch * (nl ? nl : 1) * sizeof(double));
+  size_t o_ns = take(batch * 3 * sizeof(double));
+  size_t o_srays = take(batch * nls * sizeof(skw_ray)), o_sidx = take(batch * nls * sizeof(int32_t));
+  size_t o_sc = take(batch * nls * 3 * sizeof(double)), o_occ = take(batch * nls);
+  size_t o_brays = take(batch * nbs * sizeof(skw_ray)), o_bidx = take(batch * nbs * sizeof(int32_t));
+  size_t o_brec = take(batch * nbs * 6 * sizeof(double)), o_bhits = take(batch * nbs * sizeof(skw_hit));
+  size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(3 * sizeof(uint64_t));
+  size_t o_film = take(npix * 3 * sizeof(double));
+  char *base = nullptr;
+  SKW_HIP(hipMallocAsync((void **)&base, off, st), "frame buffer allocation");
+  a.prim_rays = (skw_ray *)(base + o_prim_rays);
+  a.prim_hits = (skw_hit *)(base + o_prim_hits);
+  a.le = (double *)(base + o_le);
+  a.cdf = (double *)(base + o_cdf);
+  a.ns = (double *)(base + o_ns);
+  a.shadow_rays = (skw_ray *)(base + o_srays);
+  a.shadow_idx = (int32_t *)(base + o_sidx);
+  a.shadow_c = (double *)(base + o_sc);
+  a.occluded = (uint8_t *)(base + o_occ);
+  a.bsdf_rays = (skw_ray *)(base + o_brays);
+  a.bsdf_idx = (int32_t *)(base + o_bidx);
+  a.bsdf_rec = (double *)(base + o_brec);
+  a.bsdf_hits = (skw_hit *)(base + o_bhits);
+  a.counts = (uint64_t *)(base + o_counts);
+  a.totals = (uint64_t *)(base + o_totals);
+  a.film = (double *)(base + o_film);
+  skw_status rs = SKW_OK;
+  hipError_t e = launch_dl_clear(a, st);
+  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
+    a.pass = s;
+    e = launch_dl_camera(a, st);
+    if (e != hipSuccess) break;
+    rs = enqueue_trace(scene, a.prim_rays, batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr, false);
+    if (rs != SKW_OK) break;
+    e = launch_dl_shade(a, st);
+    if (e != hipSuccess) break;
+    rs = enqueue_trace(scene, a.shadow_rays, batch * nls, a.counts, nullptr, a.occluded, st, true, nullptr, nullptr,
+                       false);
+    if (rs != SKW_OK) break;
+    rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
+                       nullptr, false);
+    if (rs != SKW_OK) break;
+    e = launch_dl_resolve(a, st);
+  }
+  if (e == hipSuccess && rs == SKW_OK) e = launch_dl_finish(a, d_rgb, st);
+  uint64_t totals[3] = {0, 0, 0};
+  if (e == hipSuccess && rs == SKW_OK && rays_out) {
+    e = hipMemcpyAsync(totals, a.totals, sizeof(totals), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  (void)hipFreeAsync(base, st);
+  if (rs != SKW_OK) return rs;
+  if (e != hipSuccess) return hip_fail(e, "direct lighting frame");
+  if (rays_out)
+    for (int i = 0; i < 3; ++i) rays_out[i] = totals[i];
+  return SKW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Internal declarations shared by the C ABI translation units: handle structs, error helpers and
+// the trace enqueue used by every caller of the traversal kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/skwarka_rt.h"
+#include "host/builders.hpp"
+#include "kernels/kernels.hpp"
+
+namespace skw {
+
+skw_status fail(skw_status s, const std::string &msg);
+const char *last_error();
+skw_status hip_fail(hipError_t e, const char *what);
+int device_count();
+int env_int(const char *name, int dflt);
+
+#define SKW_HIP(call, what)                            \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return skw::hip_fail(e_, what); \
+  } while (0)
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+  *dst = nullptr;
+  if (src.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Launch-wide ray counters, one per stream so that calls on different streams never share one.
+struct CounterPool {
+  std::mutex mu;
+  std::map<void *, uint64_t *> by_stream;
+  ~CounterPool();
+  hipError_t get(void *stream, uint64_t **out);
+};
+
+}  // namespace skw
+
+struct skw_scene {
+  skw::TreeLayout host;
+  bool on_device = false;
+  bool has_inst = false;
+  bool needs_fix = false;  // some primitive's traversal barycentrics are not mesh-space values
+  int device = 0;
+  int cus = 256;
+  skw::DevNode *d_nodes = nullptr;
+  skw::DevTri *d_tris = nullptr;
+  skw::DevInst *d_insts = nullptr;
+  uint32_t *d_inst_range = nullptr;
+  skw::DevPrim *d_prims = nullptr;
+  skw::DevMesh *d_meshes = nullptr;
+  float *d_mesh_verts = nullptr;
+  float *d_mesh_normals = nullptr;
+  uint32_t *d_mesh_tris = nullptr;
+  skw::CounterPool counters;
+  ~skw_scene();
+  skw::GeomArgs geom_args() const;
+};
+
+struct skw_photon_map {
+  skw::KdLayout host;
+  bool on_device = false;
+  int device = 0;
+  skw::DevKdNode *d_nodes = nullptr;
+  skw::DevPhoton *d_photons = nullptr;
+  uint32_t *d_perm = nullptr;
+  double *d_cdir = nullptr;
+  ~skw_photon_map();
+};
+
+namespace skw {
+// Enqueues closest-hit (any = false) or occlusion (any = true) for n rays; when n_dev is set the
+// count is read on the device and n only bounds the launch size. fix: rewrite b1/b2 of hits on
+// transformed primitives in mesh space.
+skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev, skw_hit *d_hits,
+                         uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops, uint32_t *d_tests, bool fix);
+}  // namespace skw

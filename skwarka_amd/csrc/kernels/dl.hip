@@ -1,0 +1,581 @@
+// Direct-lighting wavefront on gfx950: camera rays -> closest hit -> shade (emission, light and
+// BSDF sampling, ray emission) -> occlusion / closest-hit of the emitted rays -> resolve (MIS
+// weights of BSDF rays that reach a light, per-sample sum, film accumulation).
+//
+// Per-sample arithmetic follows:
+//   DirectLightingIntegrator::ComputeDirectLighting / _SampleLights / _SampleBSDF
+//       (Source/RayLibs/Raytracer/Core/DirectLightingIntegrator.cpp:54-366)
+//   BSDF::Evaluate / PDF / Sample and its shading frame (Raytracer/Core/BSDF.cpp:22-200, BSDF.h:182-244)
+//   BxDF::Sample / PDF, Lambertian (Raytracer/Core/BxDF.cpp:20-41, BxDFs/Lambertian.cpp:18-31)
+//   Microfacet<FresnelDielectric, BlinnDistribution> (BxDFs/Microfacet.h:89-152,
+//       MicrofacetDistributions/BlinnDistribution.h:70-110, Core/Fresnel.h:92-124)
+//   DiffuseAreaLightSource::SampleLighting / LightingPDF / Radiance (LightSources/DiffuseAreaLightSource.cpp:57-174)
+//   IrradianceLightsSamplingStrategy::GetLightsCDF (LightsSamplingStrategies/IrradianceLightsSamplingStrategy.cpp:54-129)
+//   MathRoutines::SubtendedSolidAngle / BinarySearchCDF; SamplingRoutines::ConcentricDiskSampling /
+//       PowerHeuristic (Source/RayLibs/Math/MathRoutines.h:192-215, 319-390; SamplingRoutines.h:197-325, 449-455)
+//   PerspectiveCamera::GenerateRay (Raytracer/Cameras/PerspectiveCamera.cpp:36-64)
+//   the NaN / negative / infinite sample rule (Raytracer/Renderers/SamplerBasedRenderer.cpp:458-478).
+// Rays go through the scene's own traversal kernels. Compiled with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "geom_device.hpp"
+#include "kernels.hpp"
+
+namespace skw {
+namespace {
+
+using namespace dev;
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInvPi = 0.31830988618379067154;
+constexpr double kInv2Pi = 0.15915494309189533577;
+constexpr double kEps = 2.220446049250313080847e-16;
+constexpr int kMaxLights = 16;
+// Y row of the reference's spectrum-to-XYZ matrix (SpectrumRoutines.cpp:20-62), i.e. its CIE Y
+// table summed over the 360-494, 495-574 and 575-830 nm bands and divided by the table's total.
+__constant__ double kLumW[3] = {0x1.b38293da589a6p-5, 0x1.2b993cc221566p-1, 0x1.725d3400723f6p-2};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double rnd(uint64_t seed, uint64_t sample, uint32_t dim) {
+  uint64_t h = mix64(seed ^ mix64(sample * 64ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// ------------------------------------------------------------------ lights
+
+__device__ double solid_angle3(D3 d1, D3 d2, D3 d3) {
+  double det = dot3(d1, cross3(d2, d3));
+  double denom = 1 + dot3(d1, d2) + dot3(d1, d3) + dot3(d2, d3);
+  if (denom > 0.0) return fabs(atan(det / denom) * 2.0);
+  if (denom < 0.0) return 2.0 * (kPi - fabs(atan(det / denom)));
+  return 2.0 * kPi;
+}
+
+__device__ double subtended_solid_angle(D3 p, const DevLight &L) {
+  D3 dirs[8];
+  for (int i = 0; i < 8; ++i) {
+    D3 v{(i & 1) == 0 ? L.bmin[0] : L.bmax[0], (i & 2) == 0 ? L.bmin[1] : L.bmax[1],
+         (i & 4) == 0 ? L.bmin[2] : L.bmax[2]};
+    D3 d = sub3(v, p);
+    normalize3(d);
+    dirs[i] = d;
+  }
+  bool inside = true;
+  double sum = 0.0;
+  if (p.x < L.bmin[0] - kEps) {
+    sum += solid_angle3(dirs[0], dirs[2], dirs[6]) + solid_angle3(dirs[0], dirs[4], dirs[6]);
+    inside = false;
+  } else if (p.x > L.bmax[0] + kEps) {
+    sum += solid_angle3(dirs[1], dirs[3], dirs[7]) + solid_angle3(dirs[1], dirs[5], dirs[7]);
+    inside = false;
+  }
+  if (p.y < L.bmin[1] - kEps) {
+    sum += solid_angle3(dirs[0], dirs[1], dirs[5]) + solid_angle3(dirs[0], dirs[4], dirs[5]);
+    inside = false;
+  } else if (p.y > L.bmax[1] + kEps) {
+    sum += solid_angle3(dirs[2], dirs[3], dirs[7]) + solid_angle3(dirs[2], dirs[6], dirs[7]);
+    inside = false;
+  }
+  if (p.z < L.bmin[2] - kEps) {
+    sum += solid_angle3(dirs[0], dirs[1], dirs[3]) + solid_angle3(dirs[0], dirs[2], dirs[3]);
+    inside = false;
+  } else if (p.z > L.bmax[2] + kEps) {
+    sum += solid_angle3(dirs[4], dirs[5], dirs[7]) + solid_angle3(dirs[4], dirs[6], dirs[7]);
+    inside = false;
+  }
+  return inside ? 4.0 * kPi : sum;
+}
+
+__device__ void lights_cdf(const DlArgs &a, D3 p, D3 n, double *cdf) {
+  const uint32_t nl = a.n_lights;
+  for (uint32_t i = 0; i < nl; ++i) {
+    const DevLight &L = a.lights[i];
+    bool inside = p.x >= L.bmin[0] && p.y >= L.bmin[1] && p.z >= L.bmin[2] && p.x <= L.bmax[0] && p.y <= L.bmax[1] &&
+                  p.z <= L.bmax[2];
+    if (inside) {
+      cdf[i] = kPi * L.intensity;
+    } else {
+      double avg = 0.0;
+      for (int j = 0; j < 8; ++j) {
+        D3 v{(j & 1) ? L.bmin[0] : L.bmax[0], (j & 2) ? L.bmin[1] : L.bmax[1], (j & 4) ? L.bmin[2] : L.bmax[2]};
+        D3 d = sub3(v, p);
+        normalize3(d);
+        avg += dmax(0.0, dot3(d, n));
+      }
+      avg *= 0.125;
+      if (avg > 0.0) {
+        if (avg < 0.1) avg = 0.1;
+        cdf[i] = subtended_solid_angle(p, L) * L.intensity * avg;
+      } else {
+        cdf[i] = 0.0;
+      }
+    }
+  }
+  for (uint32_t i = 1; i < nl; ++i) cdf[i] += cdf[i - 1];
+  if (cdf[nl - 1] > 0.0) {
+    double inv = 1.0 / cdf[nl - 1];
+    for (uint32_t i = 0; i < nl; ++i) cdf[i] *= inv;
+  } else {
+    double inv = 1.0 / (double)nl;
+    for (uint32_t i = 0; i < nl; ++i) cdf[i] = ((double)i + 1.0) * inv;
+  }
+}
+
+template <class Get>
+__device__ uint32_t search_cdf(Get get, uint32_t n, double u) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (get(mid) <= u) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------ BSDF
+
+__device__ double fresnel_dielectric(double c) {
+  double ni = 1.5, no = 1.0;
+  c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+  if (c < 0.0) {
+    double t = ni;
+    ni = no;
+    no = t;
+  }
+  double sin_t = no / ni * sqrt(dmax(0.0, 1.0 - c * c));
+  if (sin_t >= 1.0) return 1.0;
+  double cos_t = sqrt(dmax(0.0, 1.0 - sin_t * sin_t));
+  c = fabs(c);
+  double rpar = ((ni * c) - (no * cos_t)) / ((ni * c) + (no * cos_t));
+  double rper = ((no * c) - (ni * cos_t)) / ((no * c) + (ni * cos_t));
+  return (rpar * rpar + rper * rper) * 0.5;
+}
+
+struct Bsdf {
+  D3 n, ng, e1, e2;
+  bool plastic;
+  double kd[3], ks[3];
+  double e;
+
+  __device__ D3 to_local(D3 v) const { return D3{dot3(v, e1), dot3(v, e2), dot3(v, n)}; }
+  __device__ D3 to_world(D3 v) const { return add3(add3(mul3(e1, v.x), mul3(e2, v.y)), mul3(n, v.z)); }
+  __device__ double blinn_pdf(D3 wo, D3 wi) const {
+    D3 h = add3(wo, wi);
+    normalize3(h);
+    double ct = fabs(h.z);
+    return ((e + 1.0) * pow(ct, e)) / (2.0 * kPi * 4.0 * fabs(dot3(wo, h)));
+  }
+  __device__ bool micro_eval(D3 wo, D3 wi, double &scalar, double &fres) const {
+    double ci = fabs(wo.z), co = fabs(wi.z);
+    if (ci < kEps || co < kEps) return false;
+    D3 h = add3(wo, wi);
+    if (fabs(h.x) < kEps && fabs(h.y) < kEps && fabs(h.z) < kEps) return false;
+    normalize3(h);
+    fres = fresnel_dielectric(dot3(wi, h));
+    double ch = fabs(h.z);
+    double tmp = 2.0 * ch / fabs(dot3(wo, h));
+    double g = dmin(1.0, dmin(ci * tmp, co * tmp));
+    double d = (e + 2.0) * kInv2Pi * pow(ch, e);
+    scalar = d * g / (4.0 * ci * co);
+    return true;
+  }
+  __device__ double micro_pdf(D3 wo, D3 wi) const { return (wo.z * wi.z < 0.0) ? 0.0 : blinn_pdf(wo, wi); }
+  __device__ static double lambert_pdf(D3 wo, D3 wi) { return (wo.z * wi.z > 0.0) ? fabs(wi.z) * kInvPi : 0.0; }
+  __device__ void eval_reflect_local(D3 wo, D3 wi, double f[3]) const {
+    for (int c = 0; c < 3; ++c) f[c] = kd[c] * kInvPi;
+    if (plastic) {
+      double sc, fr;
+      if (micro_eval(wo, wi, sc, fr))
+        for (int c = 0; c < 3; ++c) f[c] += (ks[c] * fr) * sc;
+    }
+  }
+  __device__ void evaluate(D3 wo_w, D3 wi_w, double f[3]) const {
+    D3 wo = to_local(wo_w), wi = to_local(wi_w);
+    if (dot3(wo_w, ng) * dot3(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
+    else f[0] = f[1] = f[2] = 0.0;
+  }
+  __device__ double pdf(D3 wo_w, D3 wi_w) const {
+    D3 wo = to_local(wo_w), wi = to_local(wi_w);
+    double p = lambert_pdf(wo, wi);
+    if (!plastic) return p;
+    p += micro_pdf(wo, wi);
+    return p / 2.0;
+  }
+  __device__ void sample(D3 wo_w, double u0, double u1, double uc, D3 &wi_w, double &pdf_out, double f[3]) const {
+    const uint64_t num = plastic ? 2 : 1;
+    uint64_t comp = (uint64_t)(uc * (double)num);
+    if (comp > num - 1) comp = num - 1;
+    D3 wo = to_local(wo_w);
+    D3 wi;
+    double p;
+    if (comp == 0) {
+      // ConcentricDiskSampling + CosineHemisphereSampling
+      double s0 = 2.0 * u0 - 1.0, s1 = 2.0 * u1 - 1.0;
+      double dx = 0.0, dy = 0.0;
+      if (!(s0 == 0.0 && s1 == 0.0)) {
+        double r, th;
+        if (s0 >= -s1) {
+          if (s0 > s1) {
+            r = s0;
+            th = s1 > 0.0 ? s1 / r : 8.0 + s1 / r;
+          } else {
+            r = s1;
+            th = 2.0 - s0 / r;
+          }
+        } else {
+          if (s0 <= s1) {
+            r = -s0;
+            th = 4.0 - s1 / r;
+          } else {
+            r = -s1;
+            th = 6.0 + s0 / r;
+          }
+        }
+        th *= kPi / 4.0;
+        dx = r * cos(th);
+        dy = r * sin(th);
+      }
+      double z = sqrt(dmax(0.0, 1.0 - dx * dx - dy * dy));
+      wi = D3{dx, dy, z};
+      if (wo.z < 0.0) wi.z *= -1.0;
+      p = lambert_pdf(wo, wi);
+    } else {
+      double ct = pow(u0, 1.0 / (e + 1.0));
+      double st = sqrt(dmax(0.0, 1.0 - ct * ct));
+      double phi = u1 * 2.0 * kPi;
+      D3 h{st * cos(phi), st * sin(phi), ct};
+      if (wo.z * h.z < 0.0) h.z *= -1.0;
+      wi = sub3(mul3(h, 2.0 * dot3(wo, h)), wo);
+      p = ((e + 1.0) * pow(ct, e)) / (2.0 * kPi * 4.0 * fabs(dot3(wo, h)));
+      if (wo.z * wi.z < 0.0) p = 0.0;
+    }
+    wi_w = to_world(wi);
+    if (num > 1) {
+      p += comp == 0 ? micro_pdf(wo, wi) : lambert_pdf(wo, wi);
+      p /= (double)num;
+    }
+    pdf_out = p;
+    if (dot3(wo_w, ng) * dot3(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
+    else f[0] = f[1] = f[2] = 0.0;
+  }
+};
+
+__device__ __forceinline__ double power_heuristic(double n1, double p1, double n2, double p2) {
+  double f = n1 * p1, g = n2 * p2;
+  return (f * f) / (f * f + g * g);
+}
+
+__device__ __forceinline__ void put_ray(skw_ray *r, D3 o, D3 d, double tmin, double tmax) {
+  double2 *q = reinterpret_cast<double2 *>(r);
+  q[0] = make_double2(o.x, o.y);
+  q[1] = make_double2(o.z, d.x);
+  q[2] = make_double2(d.y, d.z);
+  q[3] = make_double2(tmin, tmax);
+}
+
+// Wave-aggregated append: lanes with `want` get consecutive slots of *counter.
+__device__ __forceinline__ int32_t append_slot(uint64_t *counter, bool want) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return -1;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0;
+  if ((int)lane == leader)
+    base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (!want) return -1;
+  return (int32_t)(base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    a.counts[0] = 0;
+    a.counts[1] = 0;
+  }
+  if (i >= a.batch) return;
+  const uint64_t pix = a.pix0 + i;
+  const uint32_t px = (uint32_t)(pix % a.width), py = (uint32_t)(pix / a.width);
+  const uint64_t sidx = pix * a.spp + a.pass;
+  double ix = (double)px + rnd(a.seed, sidx, 0), iy = (double)py + rnd(a.seed, sidx, 1);
+  double cx = 2.0 * a.x_tan * (ix - a.width * 0.5) / a.width;
+  double cy = 2.0 * a.y_tan * (iy - a.height * 0.5) / a.height;
+  D3 d{a.cam_r[0] * cx + a.cam_r[1] * cy + a.cam_r[2], a.cam_r[3] * cx + a.cam_r[4] * cy + a.cam_r[5],
+       a.cam_r[6] * cx + a.cam_r[7] * cy + a.cam_r[8]};
+  normalize3(d);
+  put_ray(a.prim_rays + i, D3{a.cam_o[0], a.cam_o[1], a.cam_o[2]}, d, 0.0, __builtin_huge_val());
+}
+
+__global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < a.batch;
+  bool hit = false;
+  skw_hit h;
+  h.prim = kMiss;
+  if (in) {
+    h = a.prim_hits[i];
+    hit = h.prim != kMiss;
+  }
+  const uint32_t nl = a.n_lights;
+  const uint64_t pix = a.pix0 + i;
+  const uint64_t sidx = pix * a.spp + a.pass;
+  D3 o{0, 0, 0}, d{0, 0, 1}, wo{0, 0, 0}, p{0, 0, 0};
+  HitGeom g;
+  Bsdf b;
+  double cdf[kMaxLights];
+  double le[3] = {0.0, 0.0, 0.0};
+  uint32_t prim = h.prim;
+  if (hit) {
+    const skw_ray &r = a.prim_rays[i];
+    o = D3{r.o[0], r.o[1], r.o[2]};
+    d = D3{r.d[0], r.d[1], r.d[2]};
+    g = hit_geometry(a.geom, o, d, prim, h.tri);
+    wo = mul3(d, -1.0);
+    p = g.p;
+    const DevMat &m = a.mats[prim];
+    if (m.light != kMiss && dot3(wo, g.ng) > 0.0)
+      for (int c = 0; c < 3; ++c) le[c] = a.lights[m.light].radiance[c];
+    // BSDF frame: DG tangent of a mesh without UVs or tangents (TriangleMesh.cpp:520-540)
+    D3 ns = g.ns, t;
+    if (fabs(ns.x) > fabs(ns.y)) {
+      double il = 1.0 / sqrt(ns.x * ns.x + ns.z * ns.z);
+      t = D3{-ns.z * il, 0.0, ns.x * il};
+    } else {
+      double il = 1.0 / sqrt(ns.y * ns.y + ns.z * ns.z);
+      t = D3{0.0, ns.z * il, -ns.y * il};
+    }
+    D3 t2 = cross3(ns, t);
+    D3 tangent;
+    if (len_sqr3(t2) > kEps) {
+      tangent = cross3(t2, ns);
+      normalize3(tangent);
+    } else {
+      tangent = t;
+    }
+    b.n = ns;
+    b.ng = g.ng;
+    b.e1 = tangent;
+    b.e2 = cross3(ns, tangent);
+    b.plastic = m.type == 1;
+    for (int c = 0; c < 3; ++c) {
+      b.kd[c] = m.kd[c];
+      b.ks[c] = m.ks[c];
+    }
+    b.e = m.exponent;
+    if (nl > 0) {
+      D3 nrm = dot3(wo, ns) >= 0.0 ? ns : mul3(ns, -1.0);
+      lights_cdf(a, p, nrm, cdf);
+      for (uint32_t l = 0; l < nl; ++l) a.cdf[i * nl + l] = cdf[l];
+    }
+    for (int c = 0; c < 3; ++c) a.ns[3 * i + c] = c == 0 ? ns.x : (c == 1 ? ns.y : ns.z);
+  }
+  if (in)
+    for (int c = 0; c < 3; ++c) a.le[3 * i + c] = le[c];
+  const bool shade = hit && nl > 0;
+  const DevPrim *pr = a.geom.prims + (hit ? prim : 0);
+  // _SampleLights: one occlusion ray per sample whose unoccluded contribution is not black.
+  for (uint32_t k = 0; k < a.nls; ++k) {
+    bool want = false;
+    D3 wi{0, 0, 0};
+    double lf[3] = {0, 0, 0};
+    double len = 0.0, mint = 0.0;
+    if (shade) {
+      double uc = rnd(a.seed, sidx, 2 + k);
+      double u0 = rnd(a.seed, sidx, 6 + 2 * k), u1 = rnd(a.seed, sidx, 7 + 2 * k);
+      uint32_t l = search_cdf([&](uint32_t j) { return cdf[j]; }, nl, uc);
+      double comp_pdf = l == 0 ? cdf[0] : cdf[l] - cdf[l - 1];
+      const DevLight &L = a.lights[l];
+      double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
+      if (L.tri_count > 0) {
+        const DevLightTri *lt = a.ltris + L.tri_off;
+        uint32_t ti = search_cdf([&](uint32_t j) { return lt[j].cdf; }, L.tri_count, tri_u);
+        const DevLightTri &T = lt[ti];
+        double tmp = sqrt(u0);
+        double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
+        D3 v0{T.v0[0], T.v0[1], T.v0[2]}, v1{T.v1[0], T.v1[1], T.v1[2]}, v2{T.v2[0], T.v2[1], T.v2[2]};
+        D3 sp = add3(add3(mul3(v0, bb1), mul3(v1, bb2)), mul3(v2, 1.0 - bb1 - bb2));
+        D3 lv = sub3(sp, p);
+        len = sqrt(len_sqr3(lv));
+        wi = mul3(lv, 1.0 / len);
+        D3 ln{T.n[0], T.n[1], T.n[2]};
+        double ac = fabs(dot3(wi, ln));
+        double lpdf = ac > kEps ? (len * len) / (L.area * ac) : 0.0;
+        bool lit = dot3(ln, wi) < 0.0;
+        double lr[3] = {lit ? L.radiance[0] : 0.0, lit ? L.radiance[1] : 0.0, lit ? L.radiance[2] : 0.0};
+        if (lpdf > 0.0 && !(lr[0] == 0.0 && lr[1] == 0.0 && lr[2] == 0.0)) {
+          double f[3];
+          b.evaluate(wo, wi, f);
+          for (int c = 0; c < 3; ++c) lf[c] = lr[c] * f[c];
+          if (!(lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0)) {
+            mint = next_min_t(*pr, g, wi);
+            double bp = b.pdf(wo, wi);
+            double w = power_heuristic((double)a.nls, lpdf * comp_pdf, (double)a.nbs, bp);
+            w *= fabs(dot3(wi, b.n)) / (lpdf * comp_pdf);
+            for (int c = 0; c < 3; ++c) lf[c] = lf[c] * w;
+            want = true;
+          }
+        }
+      }
+    }
+    int32_t slot = append_slot(a.counts, want);
+    if (in) a.shadow_idx[i * a.nls + k] = slot;
+    if (want) {
+      put_ray(a.shadow_rays + slot, p, wi, mint, len - 1e-4);
+      double *c3 = a.shadow_c + 3 * (i * a.nls + k);
+      c3[0] = lf[0];
+      c3[1] = lf[1];
+      c3[2] = lf[2];
+    }
+  }
+  // _SampleBSDF: one closest-hit ray per sample with a positive pdf and a non-black value.
+  for (uint32_t k = 0; k < a.nbs; ++k) {
+    bool want = false;
+    D3 wi{0, 0, 0};
+    double f[3] = {0, 0, 0}, bp = 0.0, mint = 0.0;
+    if (shade) {
+      double uc = rnd(a.seed, sidx, 14 + k);
+      double u0 = rnd(a.seed, sidx, 18 + 2 * k), u1 = rnd(a.seed, sidx, 19 + 2 * k);
+      b.sample(wo, u0, u1, uc, wi, bp, f);
+      if (bp > 0.0 && !(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
+        mint = next_min_t(*pr, g, wi);
+        want = true;
+      }
+    }
+    int32_t slot = append_slot(a.counts + 1, want);
+    if (in) a.bsdf_idx[i * a.nbs + k] = slot;
+    if (want) {
+      put_ray(a.bsdf_rays + slot, p, wi, mint, __builtin_huge_val());
+      double *rec = a.bsdf_rec + 6 * (i * a.nbs + k);
+      rec[0] = f[0];
+      rec[1] = f[1];
+      rec[2] = f[2];
+      rec[3] = bp;
+      rec[4] = fabs(dot3(wi, b.n)) / bp;
+      rec[5] = mint;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    a.totals[0] += a.batch;
+    a.totals[1] += a.counts[0];
+    a.totals[2] += a.counts[1];
+  }
+  if (i >= a.batch) return;
+  const skw_hit h = a.prim_hits[i];
+  double L[3] = {0.0, 0.0, 0.0};
+  if (h.prim != kMiss) {
+    for (int c = 0; c < 3; ++c) L[c] = a.le[3 * i + c];
+    double cdl[3] = {0.0, 0.0, 0.0};
+    const uint32_t nl = a.n_lights;
+    if (nl > 0) {
+      double dl[3] = {0.0, 0.0, 0.0};
+      for (uint32_t k = 0; k < a.nls; ++k) {
+        int32_t j = a.shadow_idx[i * a.nls + k];
+        if (j < 0 || a.occluded[j]) continue;
+        const double *c3 = a.shadow_c + 3 * (i * a.nls + k);
+        for (int c = 0; c < 3; ++c) dl[c] += c3[c];
+      }
+      for (int c = 0; c < 3; ++c) cdl[c] = dl[c] * (1.0 / (double)a.nls);
+      double db[3] = {0.0, 0.0, 0.0};
+      for (uint32_t k = 0; k < a.nbs; ++k) {
+        int32_t j = a.bsdf_idx[i * a.nbs + k];
+        if (j < 0) continue;
+        const skw_hit h2 = a.bsdf_hits[j];
+        if (h2.prim == kMiss) continue;
+        const DevMat &m2 = a.mats[h2.prim];
+        if (m2.light == kMiss) continue;
+        const skw_ray &r2 = a.bsdf_rays[j];
+        D3 wi{r2.d[0], r2.d[1], r2.d[2]};
+        // DiffuseAreaLightSource::Radiance tests the DG geometric normal of the hit.
+        HitGeom g2 = hit_geometry(a.geom, D3{r2.o[0], r2.o[1], r2.o[2]}, wi, h2.prim, h2.tri);
+        if (!(dot3(mul3(wi, -1.0), g2.ng) > 0.0)) continue;
+        const DevLight &Lt = a.lights[m2.light];
+        if (Lt.radiance[0] == 0.0 && Lt.radiance[1] == 0.0 && Lt.radiance[2] == 0.0) continue;
+        const DevLightTri &T = a.ltris[Lt.tri_off + h2.tri];
+        const double *rec = a.bsdf_rec + 6 * (i * a.nbs + k);
+        double dist = h2.t - rec[5];
+        double ac = fabs(dot3(wi, D3{T.n[0], T.n[1], T.n[2]}));
+        double lpdf = ac > kEps ? (dist * dist) / (Lt.area * ac) : 0.0;
+        const double *cdf = a.cdf + i * nl;
+        double cp = m2.light == 0 ? cdf[0] : cdf[m2.light] - cdf[m2.light - 1];
+        double w = power_heuristic((double)a.nbs, rec[3], (double)a.nls, lpdf * cp);
+        w *= rec[4];
+        for (int c = 0; c < 3; ++c) db[c] += rec[c] * Lt.radiance[c] * w;
+      }
+      for (int c = 0; c < 3; ++c) cdl[c] += db[c] * (1.0 / (double)a.nbs);
+    }
+    for (int c = 0; c < 3; ++c) L[c] = L[c] + cdl[c];
+  }
+  double lum = 0.0;
+  for (int c = 0; c < 3; ++c) lum += kLumW[c] * L[c];
+  bool bad = isnan(L[0]) || isnan(L[1]) || isnan(L[2]) || lum < -kEps || isinf(L[0]) || isinf(L[1]) || isinf(L[2]);
+  double *f = a.film + 3 * (a.pix0 + i);
+  if (!bad)
+    for (int c = 0; c < 3; ++c) f[c] += L[c];
+}
+
+// Clears the film and the ray totals (a kernel rather than a memset, so a captured frame replays
+// it faithfully).
+__global__ __launch_bounds__(256) void k_dl_clear(DlArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = (uint64_t)a.width * a.height * 3;
+  if (i < 3) a.totals[i] = 0;
+  if (i < n) a.film[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_dl_finish(DlArgs a, float *rgb) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.batch) return;
+  const uint64_t pix = a.pix0 + i;
+  for (int c = 0; c < 3; ++c) rgb[3 * pix + c] = (float)(a.film[3 * pix + c] / (double)a.spp);
+}
+
+}  // namespace
+
+hipError_t launch_dl_clear(const DlArgs &a, hipStream_t stream) {
+  uint64_t n = (uint64_t)a.width * a.height * 3;
+  uint32_t grid = (uint32_t)((n + 255) / 256);
+  hipLaunchKernelGGL(k_dl_clear, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  hipLaunchKernelGGL(k_dl_camera, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 63) / 64);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_shade, dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  hipLaunchKernelGGL(k_dl_resolve, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_finish, dim3(grid), dim3(256), 0, stream, a, rgb);
+  return hipGetLastError();
+}
+
+}  // namespace skw

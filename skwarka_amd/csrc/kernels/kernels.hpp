@@ -19,6 +19,7 @@ struct TraceArgs {
   skw_hit *hits;
   uint8_t *occluded;
   uint64_t n;
+  const uint64_t *n_dev;  // when set, the ray count is read here on the device (n is ignored)
   uint64_t *counter;  // set to gridDim*blockDim before the trace kernel starts
   uint32_t *gstack;   // used only when the tree is deeper than the LDS stack
   uint32_t *pops;     // statistics variant only: node boxes tested per ray
@@ -75,5 +76,63 @@ struct GatherArgs {
 };
 // mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
+
+// ------------------------------------------------------------------ direct lighting
+struct DevMat {
+  double kd[3];
+  double ks[3];
+  double exponent;  // Blinn exponent of a plastic material
+  uint32_t type;    // 0 matte, 1 plastic
+  uint32_t light;   // area light index of the primitive, 0xFFFFFFFF if none
+};
+
+struct DevLight {
+  double radiance[3];
+  double area;
+  double intensity;
+  double bmin[3], bmax[3];
+  uint32_t tri_off, tri_count;
+};
+
+struct DevLightTri {
+  double cdf;
+  double n[3];  // GetTriangleNormal (float math) as double
+  double v0[3], v1[3], v2[3];
+};
+
+struct DlArgs {
+  GeomArgs geom;
+  const DevMat *mats;
+  const DevLight *lights;
+  const DevLightTri *ltris;
+  uint32_t n_lights;
+  double cam_o[3], cam_r[9], x_tan, y_tan;
+  uint32_t width, height, spp, nls, nbs;
+  uint64_t seed;
+  uint32_t pass;      // sample index s of this pass
+  uint64_t pix0;      // first pixel of this batch
+  uint64_t batch;     // pixels in this batch
+  skw_ray *prim_rays;
+  skw_hit *prim_hits;
+  double *le;         // 3 per sample: emitted radiance seen by the camera ray
+  double *cdf;        // n_lights per sample
+  double *ns;         // 3 per sample: shading normal
+  skw_ray *shadow_rays;
+  int32_t *shadow_idx;   // nls per sample: ray index or -1
+  double *shadow_c;      // 3 per (sample, k): unoccluded contribution
+  skw_ray *bsdf_rays;
+  int32_t *bsdf_idx;     // nbs per sample
+  double *bsdf_rec;      // 6 per (sample, k): f rgb, pdf, |wi.ns|/pdf, min_t
+  uint64_t *counts;      // [0] shadow rays, [1] BSDF rays of this pass (device)
+  uint8_t *occluded;
+  skw_hit *bsdf_hits;
+  double *film;          // 3 per pixel, running sum over passes
+  uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device)
+};
+hipError_t launch_dl_clear(const DlArgs &a, hipStream_t stream);
+hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream);
+hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream);
+hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream);
+hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream);
 
 }  // namespace skw

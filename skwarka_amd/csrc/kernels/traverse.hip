@@ -315,13 +315,13 @@ __device__ __forceinline__ void store_result(const TraceArgs &args, uint64_t idx
 template <bool ANY, bool INST, bool STATS, class Stack>
 __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint64_t n = args.n;
+  const uint64_t n = args.n_dev ? *args.n_dev : args.n;
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool busy = false;
   RayState r;
   double tmax = 0.0;
   Best best;
-  uint32_t sp = 0, cur = 0;
+  uint32_t sp = 0, cur = 0, ca = 0, cb = 0;
   bool have = false, found = false;
   bool first = true;
   uint32_t pops = 0, tests = 0;
@@ -345,14 +345,20 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
       best.hit = false;
       best.inst = false;
       sp = 0;
-      cur = args.root;
-      have = true;
       found = false;
       busy = true;
       pops = 0;
       tests = 0;
+      // The root is tested like any popped node (TriangleAccelerator.cpp:165-192).
+      float4 lo, hi;
+      load_node(args.nodes, args.root, lo, hi);
+      if (STATS) ++pops;
+      have = !slab_cull(lo, hi, r, tmax);
+      cur = args.root;
+      ca = __float_as_uint(hi.z);
+      cb = __float_as_uint(hi.w);
     }
-    // One step of the walk.
+    // One step of the walk: visit one node whose box has been accepted.
     if (!have) {
       if (sp == 0) {
         store_result<ANY>(args, idx, found, best, tmax, r);
@@ -372,25 +378,50 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
       } else {
         --sp;
       }
+      // A pending sibling passed the tmax-independent tests when it was pushed; the box is
+      // tested again with the current tmax, as the reference tests it when it pops it.
+      float4 lo, hi;
+      load_node(args.nodes, cur, lo, hi);
+      if (slab_cull(lo, hi, r, tmax)) continue;
+      ca = __float_as_uint(hi.z);
+      cb = __float_as_uint(hi.w);
     }
     have = false;
-    float4 lo, hi;
-    load_node(args.nodes, cur, lo, hi);
-    if (STATS) ++pops;
-    if (slab_cull(lo, hi, r, tmax)) continue;
-    const uint32_t a = __float_as_uint(hi.z), b = __float_as_uint(hi.w);
+    const uint32_t a = ca, b = cb;
     const uint32_t kind = b & 3u;
     if (kind != 0) {
-      uint32_t nch = __builtin_popcount((b >> 2) & 7u);
-      double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
-      if (dax > 0.0) {
-        cur = a;
-        if (nch > 1) stk.set(sp++, ((a + 1) << 3) | (nch - 1));
-      } else {
-        cur = a + nch - 1;
-        if (nch > 1) stk.set(sp++, ((cur - 1) << 3) | 4u | (nch - 1));
+      // Test all children now, in walk order, with the current tmax. The first survivor is the
+      // node the reference pops next; culled children stay culled later (their tmax-free
+      // conditions do not change and tmax only shrinks); later survivors are pushed as one
+      // entry (they are consecutive in walk order) and re-tested when popped.
+      const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
+      const double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
+      const bool fwd = dax > 0.0;
+      float4 clo[3], chi[3];
+      uint32_t cidx[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        cidx[k] = fwd ? a + (uint32_t)k : a + nch - 1u - (uint32_t)k;
+        if ((uint32_t)k < nch) load_node(args.nodes, cidx[k], clo[k], chi[k]);
       }
+      uint32_t surv = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if ((uint32_t)k < nch && !slab_cull(clo[k], chi[k], r, tmax)) surv |= 1u << k;
+      if (STATS) pops += nch;
+      if (surv == 0) continue;
+      const uint32_t s0 = __builtin_ctz(surv);
+      cur = s0 == 0 ? cidx[0] : (s0 == 1 ? cidx[1] : cidx[2]);
+      const float4 hs = s0 == 0 ? chi[0] : (s0 == 1 ? chi[1] : chi[2]);
+      ca = __float_as_uint(hs.z);
+      cb = __float_as_uint(hs.w);
       have = true;
+      const uint32_t rest = surv & ~((2u << s0) - 1u);
+      if (rest) {
+        const uint32_t p = __builtin_ctz(rest);
+        const uint32_t start = p == 1 ? cidx[1] : cidx[2];
+        stk.set(sp++, (start << 3) | (fwd ? 0u : 4u) | (uint32_t)__builtin_popcount(rest));
+      }
       continue;
     }
     bool stop = false;

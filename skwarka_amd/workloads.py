@@ -311,3 +311,59 @@ def scene_c0(seed=0):
 
 def rays_c0(width=512, height=512):
     return camera_rays(width, height, (0.0, 0.0, -2.9), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)
+
+
+def scene_room(seed=2, n_spheres=200, subdiv_counts=(70, 111, 19), subdiv_levels=(5, 4, 3), room=4.0,
+               light_radiance=8.0):
+    """Seeded direct-lighting room: the cube [-room, room]^3 (inward normals, matte 0.5), n_spheres
+    displaced icospheres (radii log-uniform in [0.05, 0.5], +-2% radial noise, centres uniform in
+    the room), the largest subdiv_counts[0] spheres at subdiv_levels[0] and so on (70/111/19 at
+    5/4/3 gives 2,026,240 sphere triangles), and 4 quad area lights of 1 x 1 just under the ceiling
+    facing down. Primitive order: room, spheres, lights. Materials: prim p belongs to reflectance /
+    roughness group p % 64 (64 groups, gray reflectance U[0.1, 0.9], roughness U[0.01, 0.3]);
+    spheres with odd index are plastic (glossy reflectance 0.04), the rest matte; lights are black
+    matte. Light radiance is light_radiance times a per-light tint U[0.8, 1.2]^3."""
+    rng = np.random.default_rng(seed)
+    radii = np.exp(rng.uniform(np.log(0.05), np.log(0.5), n_spheres))
+    centers = rng.uniform(-room + 0.6, room - 0.6, (n_spheres, 3))
+    order = np.argsort(-radii, kind="stable")
+    level = np.empty(n_spheres, np.int64)
+    start = 0
+    for cnt, lev in zip(subdiv_counts, subdiv_levels):
+        level[order[start:start + cnt]] = lev
+        start += cnt
+    level[order[start:]] = subdiv_levels[-1]
+    meshes = [box(room)]
+    for i in range(n_spheres):
+        meshes.append(icosphere(int(level[i]), float(radii[i]), centers[i], noise=0.02, rng=rng))
+    y = room - 0.01
+    light_pos = [(-1.5, -1.5), (1.5, -1.5), (-1.5, 1.5), (1.5, 1.5)]
+    for (x, z) in light_pos:
+        # vertices ordered so that (v1 - v0) x (v2 - v0) points down (-y)
+        meshes.append(quad((x - 0.5, y, z - 0.5), (x + 0.5, y, z - 0.5), (x + 0.5, y, z + 0.5), (x - 0.5, y, z + 0.5)))
+    n_prims = len(meshes)
+    group_refl = rng.uniform(0.1, 0.9, 64)
+    group_rough = rng.uniform(0.01, 0.3, 64)
+    mat = np.zeros(n_prims, np.uint8)
+    kd = np.zeros((n_prims, 3))
+    ks = np.zeros((n_prims, 3))
+    rough = np.zeros(n_prims)
+    for p in range(n_prims - 4):
+        g = p % 64
+        kd[p] = group_refl[g]
+        rough[p] = group_rough[g]
+        if 1 <= p <= n_spheres and (p - 1) % 2 == 1:
+            mat[p] = 1
+            ks[p] = 0.04
+    kd[0] = 0.5
+    rough[n_prims - 4:] = 0.1
+    light_prim = np.arange(n_prims - 4, n_prims, dtype=np.uint32)
+    light_rad = light_radiance * rng.uniform(0.8, 1.2, (4, 3))
+    return {"meshes": meshes, "mat": mat, "kd": kd, "ks": ks, "rough": rough, "light_prim": light_prim,
+            "light_radiance": light_rad, "group_refl": group_refl, "group_rough": group_rough,
+            "camera": ((0.0, 0.0, -room + 0.1), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)}
+
+
+def scene_room_small(seed=5):
+    """A small room (20 spheres at subdivision 3 and 2) for parity tests."""
+    return scene_room(seed=seed, n_spheres=20, subdiv_counts=(8, 12), subdiv_levels=(3, 2))
