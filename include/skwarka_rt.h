@@ -200,6 +200,51 @@ skw_status skw_dl_scene_destroy(skw_dl_scene *dl);
 skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
                          const skw_dl_params *params, float *d_rgb, uint64_t *rays_out, void *stream);
 
+/* ---------------------------------------------------------------- batched scenes
+ * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in
+ * skw_dl_render, sample s of pixel p has index p*spp + s) are traced to their closest hit by one
+ * kernel that takes ray chunks from a device work queue (smallest scenes first). The frame
+ * (queue reset + trace) is captured once in a hipGraph and replayed. No counterpart in the
+ * reference, which renders one scene at a time; its per-ray semantics are those of
+ * skw_closest_hit. Scenes must not use instancing and need a traversal stack of at most 64. */
+typedef struct skw_batch skw_batch;
+
+typedef struct {
+  skw_camera cam;
+  uint32_t width, height, spp;
+  uint64_t seed;
+} skw_batch_view;
+
+/* Launch shape: waves per workgroup (1, 2 or 4), LDS stack entries per lane (32 or 64, at least
+ * the deepest scene's need), rays per queue chunk (a multiple of 64), resident waves per CU. */
+typedef struct {
+  uint32_t waves_per_block;
+  uint32_t stack_entries;
+  uint32_t chunk_rays;
+  uint32_t waves_per_cu;
+} skw_batch_config;
+
+skw_status skw_batch_create(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
+                            skw_batch **out);
+skw_status skw_batch_destroy(skw_batch *batch);
+/* Rebuilds the work queue for `cfg` and captures the frame graph. */
+skw_status skw_batch_configure(skw_batch *batch, const skw_batch_config *cfg);
+/* Times every valid launch shape (frames_per_config frames each, after one warm-up), keeps the
+ * fastest (configured and captured on return) and reports it. */
+skw_status skw_batch_autotune(skw_batch *batch, uint32_t frames_per_config, skw_batch_config *best,
+                              double *best_ms);
+/* Replays the captured frame on `stream`. */
+skw_status skw_batch_frame(skw_batch *batch, void *stream);
+/* Device pointer and count of a scene's hit records (valid after a frame). */
+skw_status skw_batch_hits(const skw_batch *batch, uint32_t scene, const skw_hit **d_hits, uint64_t *n);
+/* Completion time of each scene's last chunk, in nanoseconds after the frame started (waits for
+ * the last frame). */
+skw_status skw_batch_latency(skw_batch *batch, double *ns_per_scene);
+/* Writes scene `scene`'s camera rays (as the frame generates them) to d_rays; for checking. */
+skw_status skw_batch_rays(const skw_batch *batch, uint32_t scene, skw_ray *d_rays, void *stream);
+/* Total rays per frame. */
+skw_status skw_batch_ray_count(const skw_batch *batch, uint64_t *n);
+
 /* Thread-local text of the last error. */
 const char *skw_last_error(void);
 

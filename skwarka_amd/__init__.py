@@ -29,6 +29,8 @@ EXPORTS = [
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
     "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
+    "skw_batch_create", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
+    "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
 ]
 
 _P = ctypes.c_void_p
@@ -57,6 +59,17 @@ class Camera(ctypes.Structure):
 class DLParams(ctypes.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("light_samples", _U32),
                 ("bsdf_samples", _U32), ("seed", _U64)]
+
+
+class BatchView(ctypes.Structure):
+    _fields_ = [("cam", Camera), ("width", _U32), ("height", _U32), ("spp", _U32), ("seed", _U64)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("waves_per_block", _U32), ("stack_entries", _U32), ("chunk_rays", _U32), ("waves_per_cu", _U32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 _lib = None
@@ -94,6 +107,16 @@ def lib():
         "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
         "skw_dl_scene_destroy": ([_P], ctypes.c_int),
         "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
+        "skw_batch_create": ([_P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_batch_destroy": ([_P], ctypes.c_int),
+        "skw_batch_configure": ([_P, ctypes.POINTER(BatchConfig)], ctypes.c_int),
+        "skw_batch_autotune": ([_P, _U32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double)],
+                               ctypes.c_int),
+        "skw_batch_frame": ([_P, _P], ctypes.c_int),
+        "skw_batch_hits": ([_P, _U32, ctypes.POINTER(_P), ctypes.POINTER(_U64)], ctypes.c_int),
+        "skw_batch_latency": ([_P, _P], ctypes.c_int),
+        "skw_batch_rays": ([_P, _U32, _P, _P], ctypes.c_int),
+        "skw_batch_ray_count": ([_P, ctypes.POINTER(_U64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -336,3 +359,67 @@ class DLScene:
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
                                   rays.ctypes.data, stream or None), "skw_dl_render")
         return rays
+
+
+class Batch:
+    """Many scenes traced in one persistent launch from a device work queue (skw_batch_*).
+    views: list of (Camera, width, height, spp, seed), one per scene."""
+
+    def __init__(self, scenes, views):
+        L = lib()
+        self.scenes = list(scenes)
+        n = len(self.scenes)
+        hs = (_P * n)(*[s.handle for s in self.scenes])
+        vs = (BatchView * n)()
+        for i, (cam, w, h, spp, seed) in enumerate(views):
+            vs[i].cam = cam
+            vs[i].width, vs[i].height, vs[i].spp, vs[i].seed = w, h, spp, seed
+        self._hs, self._vs = hs, vs
+        b = _P()
+        check(L.skw_batch_create(hs, vs, n, ctypes.byref(b)), "skw_batch_create")
+        self.handle = b
+        self.n = n
+
+    def close(self):
+        if self.handle:
+            lib().skw_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, waves_per_block, stack_entries, chunk_rays, waves_per_cu):
+        cfg = BatchConfig(waves_per_block, stack_entries, chunk_rays, waves_per_cu)
+        check(lib().skw_batch_configure(self.handle, ctypes.byref(cfg)), "skw_batch_configure")
+
+    def autotune(self, frames=3):
+        cfg = BatchConfig()
+        ms = ctypes.c_double()
+        check(lib().skw_batch_autotune(self.handle, frames, ctypes.byref(cfg), ctypes.byref(ms)),
+              "skw_batch_autotune")
+        return cfg.as_dict(), ms.value
+
+    def frame(self, stream=0):
+        check(lib().skw_batch_frame(self.handle, stream or None), "skw_batch_frame")
+
+    def hits(self, scene):
+        p = _P()
+        n = _U64()
+        check(lib().skw_batch_hits(self.handle, scene, ctypes.byref(p), ctypes.byref(n)), "skw_batch_hits")
+        return p.value, n.value
+
+    def latency_ns(self):
+        out = np.zeros(self.n, np.float64)
+        check(lib().skw_batch_latency(self.handle, out.ctypes.data), "skw_batch_latency")
+        return out
+
+    def rays(self, scene, d_rays, stream=0):
+        check(lib().skw_batch_rays(self.handle, scene, d_rays, stream or None), "skw_batch_rays")
+
+    def ray_count(self):
+        n = _U64()
+        check(lib().skw_batch_ray_count(self.handle, ctypes.byref(n)), "skw_batch_ray_count")
+        return n.value

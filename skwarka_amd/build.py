@@ -18,11 +18,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [
     "abi.cpp",
     "abi_dl.cpp",
+    "abi_batch.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",
     "kernels/gather.hip",
     "kernels/dl.hip",
+    "kernels/batch.hip",
 ]
 
 COMMON = [

@@ -1,0 +1,280 @@
+// C ABI of the batched multi-scene form (skw_batch_*): scene table, work queue (smallest scenes
+// first), the frame captured in a hipGraph, and the launch-shape autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace skw;
+
+struct skw_batch {
+  std::vector<const skw_scene *> scenes;
+  std::vector<DevBatchScene> table;
+  uint64_t total_rays = 0;
+  uint32_t max_stack = 0;
+  int device = 0;
+  int cus = 256;
+  double wall_khz = 100000.0;
+  skw_batch_config cfg{1, 32, 256, 16};
+  DevBatchScene *d_table = nullptr;
+  DevChunk *d_chunks = nullptr;
+  uint64_t n_chunks = 0;
+  uint64_t *d_state = nullptr;  // queue head, frame start, then per scene: done, end
+  skw_hit *d_hits = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t last_stream = nullptr;
+
+  BatchArgs args() const {
+    BatchArgs a;
+    a.scenes = d_table;
+    a.chunks = d_chunks;
+    a.n_chunks = n_chunks;
+    a.queue_head = d_state;
+    a.frame_start = d_state + 1;
+    a.scene_done = d_state + 2;
+    a.scene_end = d_state + 2 + table.size();
+    a.n_scenes = (uint32_t)table.size();
+    a.hits = d_hits;
+    return a;
+  }
+  uint32_t grid() const {
+    uint64_t waves = (uint64_t)cus * cfg.waves_per_cu;
+    uint64_t blocks = (waves + cfg.waves_per_block - 1) / cfg.waves_per_block;
+    return (uint32_t)std::max<uint64_t>(1, blocks);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~skw_batch() {
+    drop_graph();
+    void *ptrs[] = {d_table, d_chunks, d_state, d_hits};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+bool valid_config(const skw_batch *b, const skw_batch_config &c) {
+  bool waves_ok = c.waves_per_block == 1 || c.waves_per_block == 2 || c.waves_per_block == 4;
+  bool stack_ok = (c.stack_entries == 32 || c.stack_entries == 64) && c.stack_entries >= b->max_stack;
+  return waves_ok && stack_ok && c.chunk_rays >= 64 && c.chunk_rays % 64 == 0 && c.waves_per_cu >= 1 &&
+         c.waves_per_cu <= 64;
+}
+
+// Chunks of cfg.chunk_rays rays, scenes ordered by ray count (then index), so small scenes finish
+// first.
+hipError_t build_queue(skw_batch *b) {
+  std::vector<uint32_t> order(b->table.size());
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return b->table[x].n_rays < b->table[y].n_rays; });
+  std::vector<DevChunk> chunks;
+  for (uint32_t s : order) {
+    for (uint64_t r = 0; r < b->table[s].n_rays; r += b->cfg.chunk_rays) {
+      DevChunk c;
+      c.scene = s;
+      c.pad = 0;
+      c.begin = r;
+      c.end = std::min<uint64_t>(r + b->cfg.chunk_rays, b->table[s].n_rays);
+      chunks.push_back(c);
+    }
+  }
+  if (b->d_chunks) (void)hipFree(b->d_chunks);
+  b->d_chunks = nullptr;
+  b->n_chunks = chunks.size();
+  return upload(&b->d_chunks, chunks);
+}
+
+hipError_t enqueue_frame(skw_batch *b, hipStream_t st) {
+  BatchArgs a = b->args();
+  hipError_t e = launch_batch_reset(a, st);
+  if (e != hipSuccess) return e;
+  return launch_batch_trace(a, b->cfg.waves_per_block, b->cfg.stack_entries, b->grid(), st);
+}
+
+hipError_t capture(skw_batch *b) {
+  b->drop_graph();
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t e2 = enqueue_frame(b, b->stream);
+  e = hipStreamEndCapture(b->stream, &b->graph);
+  if (e2 != hipSuccess) return e2;
+  if (e != hipSuccess) return e;
+  return hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+skw_status skw_batch_create(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
+                            skw_batch **out) {
+  if (!out || (n > 0 && (!scenes || !views))) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  *out = nullptr;
+  std::unique_ptr<skw_batch> b(new skw_batch());
+  SKW_HIP(hipGetDevice(&b->device), "hipGetDevice");
+  SKW_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device), "hipDeviceGetAttribute");
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
+    b->wall_khz = khz;
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const skw_scene *s = scenes[i];
+    if (!s || !s->on_device) return fail(SKW_INVALID_ARG, "every scene must be a device scene");
+    if (s->has_inst) return fail(SKW_UNSUPPORTED, "batched scenes cannot use instancing");
+    if (s->host.stack_need > 64) return fail(SKW_UNSUPPORTED, "a batched scene needs a stack deeper than 64");
+    const skw_batch_view &v = views[i];
+    if (v.width == 0 || v.height == 0 || v.spp == 0) return fail(SKW_INVALID_ARG, "empty view");
+    DevBatchScene t;
+    std::memset(&t, 0, sizeof(t));
+    t.nodes = s->d_nodes;
+    t.tris = s->d_tris;
+    t.root = s->host.root;
+    t.width = v.width;
+    t.height = v.height;
+    t.spp = v.spp;
+    for (int k = 0; k < 3; ++k) t.cam_o[k] = v.cam.origin[k];
+    for (int k = 0; k < 9; ++k) t.cam_r[k] = v.cam.rot[k];
+    t.x_tan = v.cam.x_tan;
+    t.y_tan = v.cam.y_tan;
+    t.seed = v.seed;
+    t.n_rays = (uint64_t)v.width * v.height * v.spp;
+    t.hit_offset = off;
+    off += t.n_rays;
+    b->table.push_back(t);
+    b->scenes.push_back(s);
+    b->max_stack = std::max(b->max_stack, s->host.stack_need);
+  }
+  b->total_rays = off;
+  b->cfg.stack_entries = b->max_stack <= 32 ? 32 : 64;
+  SKW_HIP(upload(&b->d_table, b->table), "upload scene table");
+  SKW_HIP(hipMalloc((void **)&b->d_state, (2 + 2 * (size_t)n) * sizeof(uint64_t)), "state allocation");
+  SKW_HIP(hipMalloc((void **)&b->d_hits, std::max<uint64_t>(1, off) * sizeof(skw_hit)), "hit allocation");
+  SKW_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream creation");
+  SKW_HIP(build_queue(b.get()), "queue upload");
+  SKW_HIP(capture(b.get()), "frame capture");
+  *out = b.release();
+  return SKW_OK;
+}
+
+skw_status skw_batch_destroy(skw_batch *batch) {
+  if (batch) (void)hipDeviceSynchronize();
+  delete batch;
+  return SKW_OK;
+}
+
+skw_status skw_batch_configure(skw_batch *b, const skw_batch_config *cfg) {
+  if (!b || !cfg) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  if (!valid_config(b, *cfg)) return fail(SKW_INVALID_ARG, "invalid launch shape for this batch");
+  SKW_HIP(hipDeviceSynchronize(), "synchronize");
+  b->cfg = *cfg;
+  SKW_HIP(build_queue(b), "queue upload");
+  SKW_HIP(capture(b), "frame capture");
+  return SKW_OK;
+}
+
+skw_status skw_batch_frame(skw_batch *b, void *stream) {
+  if (!b) return fail(SKW_INVALID_ARG, "batch must not be NULL");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SKW_HIP(hipGraphLaunch(b->exec, st), "graph launch");
+  b->last_stream = st;
+  return SKW_OK;
+}
+
+skw_status skw_batch_autotune(skw_batch *b, uint32_t frames, skw_batch_config *best, double *best_ms) {
+  if (!b) return fail(SKW_INVALID_ARG, "batch must not be NULL");
+  if (frames == 0) frames = 3;
+  const uint32_t waves_opts[] = {1, 2, 4};
+  const uint32_t stack_opts[] = {32, 64};
+  const uint32_t chunk_opts[] = {64, 256, 1024, 4096};
+  const uint32_t wpc_opts[] = {8, 16, 32};
+  skw_batch_config keep = b->cfg, win = b->cfg;
+  double win_ms = 1e300;
+  hipEvent_t e0, e1;
+  SKW_HIP(hipEventCreate(&e0), "event");
+  SKW_HIP(hipEventCreate(&e1), "event");
+  for (uint32_t w : waves_opts)
+    for (uint32_t s : stack_opts)
+      for (uint32_t c : chunk_opts)
+        for (uint32_t p : wpc_opts) {
+          skw_batch_config cfg{w, s, c, p};
+          if (!valid_config(b, cfg)) continue;
+          if (s == 64 && b->max_stack <= 32) continue;  // a deeper LDS stack only lowers occupancy
+          b->cfg = cfg;
+          if (build_queue(b) != hipSuccess) continue;
+          hipError_t e = enqueue_frame(b, b->stream);
+          e = e == hipSuccess ? hipEventRecord(e0, b->stream) : e;
+          for (uint32_t f = 0; f < frames && e == hipSuccess; ++f) e = enqueue_frame(b, b->stream);
+          e = e == hipSuccess ? hipEventRecord(e1, b->stream) : e;
+          e = e == hipSuccess ? hipEventSynchronize(e1) : e;
+          float ms = 0.0f;
+          if (e == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            double per = ms / frames;
+            if (per < win_ms) {
+              win_ms = per;
+              win = cfg;
+            }
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (win_ms == 1e300) {
+    b->cfg = keep;
+    (void)build_queue(b);
+    (void)capture(b);
+    return fail(SKW_HIP_ERROR, "no launch shape ran");
+  }
+  b->cfg = win;
+  SKW_HIP(build_queue(b), "queue upload");
+  SKW_HIP(capture(b), "frame capture");
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return SKW_OK;
+}
+
+skw_status skw_batch_hits(const skw_batch *b, uint32_t scene, const skw_hit **d_hits, uint64_t *n) {
+  if (!b || scene >= b->table.size()) return fail(SKW_INVALID_ARG, "bad batch or scene index");
+  if (d_hits) *d_hits = b->d_hits + b->table[scene].hit_offset;
+  if (n) *n = b->table[scene].n_rays;
+  return SKW_OK;
+}
+
+skw_status skw_batch_latency(skw_batch *b, double *ns) {
+  if (!b || !ns) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  SKW_HIP(hipDeviceSynchronize(), "synchronize");
+  std::vector<uint64_t> st(2 + 2 * b->table.size());
+  SKW_HIP(hipMemcpy(st.data(), b->d_state, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "state copy");
+  const size_t n = b->table.size();
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t end = st[2 + n + i];
+    ns[i] = end >= st[1] ? (double)(end - st[1]) * 1e6 / b->wall_khz : 0.0;
+  }
+  return SKW_OK;
+}
+
+skw_status skw_batch_rays(const skw_batch *b, uint32_t scene, skw_ray *d_rays, void *stream) {
+  if (!b || scene >= b->table.size() || !d_rays) return fail(SKW_INVALID_ARG, "bad arguments");
+  SKW_HIP(launch_batch_rays(b->d_table + scene, d_rays, b->table[scene].n_rays, reinterpret_cast<hipStream_t>(stream)),
+          "ray generation");
+  return SKW_OK;
+}
+
+skw_status skw_batch_ray_count(const skw_batch *b, uint64_t *n) {
+  if (!b || !n) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  *n = b->total_rays;
+  return SKW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+// Many scenes in one persistent launch (gfx950).
+//
+// Work is a queue of chunks (scene, ray range); a wave takes one chunk at a time with a single
+// atomicAdd on the queue head, and its lanes pull rays from the chunk as they finish (ballot +
+// prefix rank, as in the single-scene kernel), so a wave only waits for stragglers at chunk ends.
+// All lanes of a wave work on the same scene, so scene pointers stay wave-uniform. Camera rays are
+// generated in the kernel (no ray buffer): PerspectiveCamera::GenerateRay
+// (Raytracer/Cameras/PerspectiveCamera.cpp:36-64) at pixel + splitmix64 jitter, the same stream as
+// the direct-lighting camera. The walk is trace_device.hpp's (TriangleAccelerator::_NodeIntersect
+// semantics). When a scene's last chunk completes, the wave records the wall clock for it.
+#include <hip/hip_runtime.h>
+
+#include "geom_device.hpp"
+#include "kernels.hpp"
+#include "trace_device.hpp"
+
+namespace skw {
+namespace {
+
+using namespace trace;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double rnd(uint64_t seed, uint64_t sample, uint32_t dim) {
+  uint64_t h = mix64(seed ^ mix64(sample * 64ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ void camera_ray(const DevBatchScene &s, uint64_t idx, RayState &r) {
+  const uint64_t pix = idx / s.spp;
+  const uint32_t px = (uint32_t)(pix % s.width), py = (uint32_t)(pix / s.width);
+  double ix = (double)px + rnd(s.seed, idx, 0), iy = (double)py + rnd(s.seed, idx, 1);
+  double cx = 2.0 * s.x_tan * (ix - s.width * 0.5) / s.width;
+  double cy = 2.0 * s.y_tan * (iy - s.height * 0.5) / s.height;
+  dev::D3 d{s.cam_r[0] * cx + s.cam_r[1] * cy + s.cam_r[2], s.cam_r[3] * cx + s.cam_r[4] * cy + s.cam_r[5],
+            s.cam_r[6] * cx + s.cam_r[7] * cy + s.cam_r[8]};
+  dev::normalize3(d);
+  make_state(r, s.cam_o[0], s.cam_o[1], s.cam_o[2], d.x, d.y, d.z, 0.0);
+}
+
+__global__ void k_batch_reset(BatchArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *a.queue_head = 0;
+    *a.frame_start = wall_clock64();
+  }
+  if (i < a.n_scenes) {
+    a.scene_done[i] = 0;
+    a.scene_end[i] = 0;
+  }
+}
+
+template <int S, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
+  __shared__ uint32_t lds[S * 64 * WAVES];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  LdsStack<S> stk{lds + wave * (S * 64) + lane};
+  Walker<false, false, false, LdsStack<S>> w;
+  w.insts = nullptr;
+  w.inst_range = nullptr;
+  w.stk = stk;
+  bool busy = false;
+  uint64_t idx = 0, cursor = 0, end = 0, chunk_len = 0;
+  uint32_t sid = 0xFFFFFFFFu;
+  const DevBatchScene *sc = nullptr;
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    if (idle == ~0ull && cursor >= end) {
+      // The wave's chunk is done: account for it, then take the next one.
+      if (sid != 0xFFFFFFFFu && lane == 0) {
+        unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(a.scene_done + sid),
+                                           (unsigned long long)chunk_len);
+        if (old + chunk_len == sc->n_rays) a.scene_end[sid] = wall_clock64();
+      }
+      unsigned long long q = 0;
+      if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
+      q = __shfl(q, 0);
+      if (q >= a.n_chunks) break;
+      const DevChunk c = a.chunks[q];
+      sid = c.scene;
+      cursor = c.begin;
+      end = c.end;
+      chunk_len = c.end - c.begin;
+      sc = a.scenes + sid;
+      w.nodes = sc->nodes;
+      w.tris = sc->tris;
+      w.root = sc->root;
+    }
+    if (idle != 0 && cursor < end) {
+      const uint64_t avail = end - cursor;
+      const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
+      if (!busy && rank < avail) {
+        idx = cursor + rank;
+        RayState r;
+        camera_ray(*sc, idx, r);
+        w.start(r, __builtin_huge_val());
+        busy = true;
+      }
+      const uint64_t took = (uint64_t)__popcll(idle);
+      cursor += took < avail ? took : avail;
+    }
+    if (busy && w.step()) {
+      store_hit(a.hits + sc->hit_offset + idx, w.tris, w.found, w.best, w.tmax, w.r);
+      busy = false;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_rays(const DevBatchScene *s, skw_ray *rays, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  RayState r;
+  camera_ray(*s, i, r);
+  skw_ray &o = rays[i];
+  o.o[0] = r.o0;
+  o.o[1] = r.o1;
+  o.o[2] = r.o2;
+  o.d[0] = r.d0;
+  o.d[1] = r.d1;
+  o.d[2] = r.d2;
+  o.tmin = 0.0;
+  o.tmax = __builtin_huge_val();
+}
+
+}  // namespace
+
+hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream) {
+  uint32_t grid = (a.n_scenes + 255) / 256;
+  hipLaunchKernelGGL(k_batch_reset, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_trace(const BatchArgs &a, uint32_t waves, uint32_t stack, uint32_t grid, hipStream_t stream) {
+  dim3 g(grid);
+#define SKW_BATCH_LAUNCH(S, W) hipLaunchKernelGGL((k_batch_trace<S, W>), g, dim3(64 * (W)), 0, stream, a)
+  if (stack <= 32) {
+    if (waves == 1) SKW_BATCH_LAUNCH(32, 1);
+    else if (waves == 2) SKW_BATCH_LAUNCH(32, 2);
+    else SKW_BATCH_LAUNCH(32, 4);
+  } else {
+    if (waves == 1) SKW_BATCH_LAUNCH(64, 1);
+    else if (waves == 2) SKW_BATCH_LAUNCH(64, 2);
+    else SKW_BATCH_LAUNCH(64, 4);
+  }
+#undef SKW_BATCH_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_rays(const DevBatchScene *scene, skw_ray *rays, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint32_t grid = (uint32_t)((n + 255) / 256);
+  hipLaunchKernelGGL(k_batch_rays, dim3(grid), dim3(256), 0, stream, scene, rays, n);
+  return hipGetLastError();
+}
+
+}  // namespace skw

@@ -135,4 +135,38 @@ hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream);
 
+// ------------------------------------------------------------------ batched scenes
+struct DevBatchScene {
+  const DevNode *nodes;
+  const DevTri *tris;
+  uint32_t root;
+  uint32_t width, height, spp;
+  double cam_o[3], cam_r[9], x_tan, y_tan;
+  uint64_t seed;
+  uint64_t n_rays;      // width * height * spp
+  uint64_t hit_offset;  // first record of this scene in the hit buffer
+};
+
+struct DevChunk {
+  uint32_t scene;
+  uint32_t pad;
+  uint64_t begin, end;  // ray index range inside the scene
+};
+
+struct BatchArgs {
+  const DevBatchScene *scenes;
+  const DevChunk *chunks;
+  uint64_t n_chunks;
+  uint64_t *queue_head;
+  uint64_t *scene_done;  // rays finished per scene
+  uint64_t *scene_end;   // wall clock of each scene's completion
+  uint64_t *frame_start; // wall clock at frame start
+  uint32_t n_scenes;
+  skw_hit *hits;
+};
+hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream);
+// waves: 1, 2 or 4 per workgroup; stack: 32 or 64 entries per lane.
+hipError_t launch_batch_trace(const BatchArgs &a, uint32_t waves, uint32_t stack, uint32_t grid, hipStream_t stream);
+hipError_t launch_batch_rays(const DevBatchScene *scene, skw_ray *rays, uint64_t n, hipStream_t stream);
+
 }  // namespace skw

@@ -1,0 +1,445 @@
+// Device-side traversal of the ternary triangle tree, shared by the single-scene kernels
+// (traverse.hip) and the multi-scene batch kernel (batch.hip).
+//
+// Semantics follow TriangleAccelerator::_NodeIntersect / _NodeIntersectTest
+// (Source/RayLibs/Raytracer/Core/TriangleAccelerator.cpp:154-293, 354-462) bit for bit:
+//  * FP64 slab test of the float node box with the 12 comparisons of :188-192 (a NaN never culls);
+//  * children visited left, middle, right when dir[axis] > 0.0, else right, middle, left (:199-210),
+//    each tested against the ray's current tmax when it is reached;
+//  * FP64 Moller-Trumbore on float vertices with the +-DBL_EPSILON bounds of :256/:261; an accepted
+//    t (tmin <= t <= tmax) becomes the new tmax, so on an exact tie the later triangle wins (:265);
+//  * instances of a leaf are traversed before its triangles with the ray moved by the instance's
+//    world-to-mesh matrix (:215-233).
+// Every translation unit including this is compiled with -ffp-contract=off (no fused multiply-add,
+// as in the reference's SSE2 build); f64 division stays the correctly rounded IEEE sequence.
+//
+// Layout: nodes are 32 bytes, children of a node form one contiguous block, so a stack entry is
+// one 32-bit word: next sibling index (29 bits), walk direction (1 bit), siblings left (2 bits).
+// The stack lives in LDS, [depth][lane], so every lane hits its own bank at any depth.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../skw_layout.hpp"
+#include "../../../include/skwarka_rt.h"
+#include "kernels.hpp"
+
+namespace skw {
+namespace trace {
+
+
+constexpr double kEps = 2.220446049250313080847e-16;  // DBL_EPSILON (Constants.h: numeric_limits epsilon)
+constexpr int kWave = 64;
+
+struct RayState {
+  double o0, o1, o2, d0, d1, d2, tmin;
+  double i0, i1, i2;
+};
+
+struct Best {
+  uint32_t pos;   // tree position of the best plain triangle
+  uint32_t prim;  // for instance hits
+  uint32_t tri;
+  bool inst;
+  bool hit;
+  double b1, b2;
+};
+
+__device__ __forceinline__ void load_node(const DevNode *__restrict__ nodes, uint32_t idx, float4 &lo, float4 &hi) {
+  const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+  lo = p[0];
+  hi = p[1];
+}
+
+// true = culled. TriangleAccelerator.cpp:176-192.
+__device__ __forceinline__ bool slab_cull(const float4 &lo, const float4 &hi, const RayState &r, double tmax) {
+  double tn1 = ((double)lo.x - r.o0) * r.i0;
+  double tf1 = ((double)lo.w - r.o0) * r.i0;
+  bool s1 = tn1 > tf1;
+  double a1 = s1 ? tf1 : tn1, b1 = s1 ? tn1 : tf1;
+  double tn2 = ((double)lo.y - r.o1) * r.i1;
+  double tf2 = ((double)hi.x - r.o1) * r.i1;
+  bool s2 = tn2 > tf2;
+  double a2 = s2 ? tf2 : tn2, b2 = s2 ? tn2 : tf2;
+  double tn3 = ((double)lo.z - r.o2) * r.i2;
+  double tf3 = ((double)hi.y - r.o2) * r.i2;
+  bool s3 = tn3 > tf3;
+  double a3 = s3 ? tf3 : tn3, b3 = s3 ? tn3 : tf3;
+  return r.tmin > b1 || a1 > b2 || a2 > b1 || a3 > b1 || r.tmin > b2 || a1 > b3 || a2 > b3 || a3 > b2 ||
+         r.tmin > b3 || a1 > tmax || a2 > tmax || a3 > tmax;
+}
+
+// TriangleAccelerator.cpp:238-274. Returns true with t, b1, b2 when accepted in [tmin, tmax].
+__device__ __forceinline__ bool tri_test(const DevTri *__restrict__ tris, uint32_t pos, const RayState &r,
+                                         double tmax, double &t_out, double &b1_out, double &b2_out) {
+  const float4 *p = reinterpret_cast<const float4 *>(tris + pos);
+  float4 q0 = p[0], q1 = p[1], q2 = p[2];
+  double v0x = q0.x, v0y = q0.y, v0z = q0.z;
+  double e1x = (double)q0.w - v0x, e1y = (double)q1.x - v0y, e1z = (double)q1.y - v0z;
+  double e2x = (double)q1.z - v0x, e2y = (double)q1.w - v0y, e2z = (double)q2.x - v0z;
+  // s1 = dir ^ e2
+  double s1x = r.d1 * e2z - r.d2 * e2y;
+  double s1y = r.d2 * e2x - r.d0 * e2z;
+  double s1z = r.d0 * e2y - r.d1 * e2x;
+  double divisor = s1x * e1x + s1y * e1y + s1z * e1z;
+  double inv = 1.0 / divisor;
+  double dx = r.o0 - v0x, dy = r.o1 - v0y, dz = r.o2 - v0z;
+  double b1 = (dx * s1x + dy * s1y + dz * s1z) * inv;
+  if (b1 > -kEps && b1 < (1.0 + kEps)) {
+    // s2 = d ^ e1
+    double s2x = dy * e1z - dz * e1y;
+    double s2y = dz * e1x - dx * e1z;
+    double s2z = dx * e1y - dy * e1x;
+    double b2 = (r.d0 * s2x + r.d1 * s2y + r.d2 * s2z) * inv;
+    if (b2 > -kEps && b1 + b2 < (1.0 + kEps)) {
+      double t = (e2x * s2x + e2y * s2y + e2z * s2z) * inv;
+      if (t >= r.tmin && t <= tmax) {
+        t_out = t;
+        b1_out = b1;
+        b2_out = b2;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ void make_state(RayState &s, double o0, double o1, double o2, double d0, double d1,
+                                           double d2, double tmin) {
+  s.o0 = o0;
+  s.o1 = o1;
+  s.o2 = o2;
+  s.d0 = d0;
+  s.d1 = d1;
+  s.d2 = d2;
+  s.tmin = tmin;
+  s.i0 = 1.0 / d0;
+  s.i1 = 1.0 / d1;
+  s.i2 = 1.0 / d2;
+}
+
+// Ray moved into an instance's mesh space (Transform.h:223-300): origin as a point (with the w
+// division when w != 1), direction as a vector; tmin/tmax unchanged.
+__device__ __forceinline__ void instance_ray(const DevInst *__restrict__ inst, const RayState &r, RayState &t) {
+  const double *m = inst->w2m;
+  double xp = m[0] * r.o0 + m[1] * r.o1 + m[2] * r.o2 + m[3];
+  double yp = m[4] * r.o0 + m[5] * r.o1 + m[6] * r.o2 + m[7];
+  double zp = m[8] * r.o0 + m[9] * r.o1 + m[10] * r.o2 + m[11];
+  double w = m[12] * r.o0 + m[13] * r.o1 + m[14] * r.o2 + m[15];
+  if (w != 1.0) {
+    double iw = 1.0 / w;
+    xp = xp * iw;
+    yp = yp * iw;
+    zp = zp * iw;
+  }
+  double dx = m[0] * r.d0 + m[1] * r.d1 + m[2] * r.d2;
+  double dy = m[4] * r.d0 + m[5] * r.d1 + m[6] * r.d2;
+  double dz = m[8] * r.d0 + m[9] * r.d1 + m[10] * r.d2;
+  make_state(t, xp, yp, zp, dx, dy, dz, r.tmin);
+}
+
+// Stack of 32-bit entries for one lane. LDS variant: [depth][lane] inside the workgroup's array.
+template <int S>
+struct LdsStack {
+  uint32_t *base;  // &lds[lane]
+  __device__ __forceinline__ uint32_t get(uint32_t i) const { return base[i * kWave]; }
+  __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { base[i * kWave] = v; }
+};
+// Global-memory variant for trees deeper than the LDS stack: [depth][global lane].
+struct GlobalStack {
+  uint32_t *base;
+  uint64_t stride;
+  __device__ __forceinline__ uint32_t get(uint32_t i) const { return base[i * stride]; }
+  __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { base[i * stride] = v; }
+};
+
+// A complete walk of an instanced mesh's subtree (triangles only, one level of instancing as in
+// TriangleAccelerator.cpp:215-233) with the stack above `sp0`. ANY: stop at the first accepted
+// triangle. Updates tmax and best.pos / b1 / b2 on each accepted triangle.
+template <bool ANY, bool STATS, class Stack>
+__device__ bool walk_instance(const DevNode *__restrict__ nodes, const DevTri *__restrict__ tris, uint32_t start,
+                              const RayState &r, double &tmax, const Stack &stk, uint32_t sp0, Best &best,
+                              uint32_t &pops, uint32_t &tests) {
+  uint32_t sp = sp0;
+  uint32_t cur = start;
+  bool have = true;
+  bool found = false;
+  while (true) {
+    if (!have) {
+      if (sp == sp0) break;
+      uint32_t e = stk.get(sp - 1);
+      cur = e >> 3;
+      uint32_t left = e & 3u;
+      if (left > 1) {
+        uint32_t nxt = (e & 4u) ? cur - 1 : cur + 1;
+        stk.set(sp - 1, (nxt << 3) | (e & 4u) | (left - 1));
+      } else {
+        --sp;
+      }
+    }
+    have = false;
+    float4 lo, hi;
+    load_node(nodes, cur, lo, hi);
+    if (STATS) ++pops;
+    if (slab_cull(lo, hi, r, tmax)) continue;
+    uint32_t a = __float_as_uint(hi.z), b = __float_as_uint(hi.w);
+    uint32_t kind = b & 3u;
+    if (kind != 0) {
+      uint32_t nch = __builtin_popcount((b >> 2) & 7u);
+      double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
+      if (dax > 0.0) {
+        cur = a;
+        if (nch > 1) stk.set(sp++, ((a + 1) << 3) | (nch - 1));
+      } else {
+        cur = a + nch - 1;
+        if (nch > 1) stk.set(sp++, ((cur - 1) << 3) | 4u | (nch - 1));
+      }
+      have = true;
+      continue;
+    }
+    uint32_t cnt = b >> 3;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      double t, b1, b2;
+      if (STATS) ++tests;
+      if (tri_test(tris, a + k, r, tmax, t, b1, b2)) {
+        if (ANY) return true;
+        tmax = t;
+        found = true;
+        best.pos = a + k;
+        best.b1 = b1;
+        best.b2 = b2;
+      }
+    }
+  }
+  return found;
+}
+
+__device__ __forceinline__ void load_ray(const skw_ray *__restrict__ rays, uint64_t i, RayState &s, double &tmax) {
+  const double2 *p = reinterpret_cast<const double2 *>(rays + i);
+  double2 a = p[0], b = p[1], c = p[2], d = p[3];
+  make_state(s, a.x, a.y, b.x, b.y, c.x, c.y, d.x);
+  tmax = d.y;
+}
+
+__device__ __forceinline__ bool zero_signed_or_nonfinite(double x) {
+  return (unsigned long long)__double_as_longlong(x) == 0x8000000000000000ull || !isfinite(x);
+}
+
+// Intersect() recomputes b1/b2 after moving the ray to mesh space with the primitive's
+// world-to-mesh matrix (TriangleAccelerator.cpp:311-322). For the identity matrix that product,
+// (1*x + 0*y) + 0*z (+ 0 for points), returns x unchanged except that -0.0 becomes +0.0 and a
+// non-finite component in the row poisons it. Rays with such components get their barycentrics
+// recomputed here with exactly that arithmetic; all other rays keep the traversal values, which
+// are then identical.
+__device__ __forceinline__ void identity_bary(const DevTri *__restrict__ tris, uint32_t pos, const RayState &r,
+                                              double &b1, double &b2) {
+  const double one = 1.0, zero = 0.0;
+  double ox = one * r.o0 + zero * r.o1 + zero * r.o2 + zero;
+  double oy = zero * r.o0 + one * r.o1 + zero * r.o2 + zero;
+  double oz = zero * r.o0 + zero * r.o1 + one * r.o2 + zero;
+  double w = zero * r.o0 + zero * r.o1 + zero * r.o2 + one;
+  if (w != 1.0) {
+    double iw = 1.0 / w;
+    ox = ox * iw;
+    oy = oy * iw;
+    oz = oz * iw;
+  }
+  double dx = one * r.d0 + zero * r.d1 + zero * r.d2;
+  double dy = zero * r.d0 + one * r.d1 + zero * r.d2;
+  double dz = zero * r.d0 + zero * r.d1 + one * r.d2;
+  const float4 *p = reinterpret_cast<const float4 *>(tris + pos);
+  float4 q0 = p[0], q1 = p[1], q2 = p[2];
+  double v0x = q0.x, v0y = q0.y, v0z = q0.z;
+  double e1x = (double)q0.w - v0x, e1y = (double)q1.x - v0y, e1z = (double)q1.y - v0z;
+  double e2x = (double)q1.z - v0x, e2y = (double)q1.w - v0y, e2z = (double)q2.x - v0z;
+  double s1x = dy * e2z - dz * e2y;
+  double s1y = dz * e2x - dx * e2z;
+  double s1z = dx * e2y - dy * e2x;
+  double divisor = s1x * e1x + s1y * e1y + s1z * e1z;
+  b1 = 0.0;
+  b2 = 0.0;
+  if (divisor == 0.0) return;  // TriangleMesh.cpp:421
+  double inv = 1.0 / divisor;
+  double ddx = ox - v0x, ddy = oy - v0y, ddz = oz - v0z;
+  b1 = (ddx * s1x + ddy * s1y + ddz * s1z) * inv;
+  double s2x = ddy * e1z - ddz * e1y;
+  double s2y = ddz * e1x - ddx * e1z;
+  double s2z = ddx * e1y - ddy * e1x;
+  b2 = (dx * s2x + dy * s2y + dz * s2z) * inv;
+}
+
+
+// Closest hit record of a finished walk (skw_hit layout, two 16-byte stores).
+__device__ __forceinline__ void store_hit(skw_hit *out, const DevTri *__restrict__ tris, bool hit, const Best &best,
+                                          double tmax, const RayState &r) {
+  uint32_t prim = kMiss, tri = kMiss;
+  double t = __builtin_huge_val(), b1 = 0.0, b2 = 0.0;
+  if (hit) {
+    if (best.inst) {
+      prim = best.prim;
+      tri = best.tri;
+    } else {
+      const uint4 ids = reinterpret_cast<const uint4 *>(tris + best.pos)[2];
+      prim = ids.y;
+      tri = ids.z;
+    }
+    t = tmax;
+    b1 = best.b1;
+    b2 = best.b2;
+    if (!best.inst && (zero_signed_or_nonfinite(r.o0) || zero_signed_or_nonfinite(r.o1) ||
+                       zero_signed_or_nonfinite(r.o2) || zero_signed_or_nonfinite(r.d0) ||
+                       zero_signed_or_nonfinite(r.d1) || zero_signed_or_nonfinite(r.d2)))
+      identity_bary(tris, best.pos, r, b1, b2);
+  }
+  uint4 w0;
+  w0.x = prim;
+  w0.y = tri;
+  unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+  w0.z = (uint32_t)tb;
+  w0.w = (uint32_t)(tb >> 32);
+  uint4 *o = reinterpret_cast<uint4 *>(out);
+  o[0] = w0;
+  reinterpret_cast<double2 *>(o)[1] = make_double2(b1, b2);
+}
+
+// One lane's walk as a resumable state machine: start() tests the root; each step() visits one
+// node whose box was accepted (an internal node: all its children are tested at once with the
+// current tmax, the first survivor is entered and later survivors pushed as one entry; a leaf:
+// its instances and triangles) or pops the next pending sibling and re-tests it. step() returns
+// true when the walk has ended; found / best / tmax then hold the result.
+// Testing the children at visit time is exact: the first survivor is the node the reference pops
+// next, with the same tmax; a culled child stays culled at its later pop, since its tmax-free
+// conditions do not change and tmax only shrinks; later survivors are tested again when popped.
+template <bool ANY, bool INST, bool STATS, class Stack>
+struct Walker {
+  const DevNode *__restrict__ nodes;
+  const DevTri *__restrict__ tris;
+  const DevInst *__restrict__ insts;
+  const uint32_t *__restrict__ inst_range;
+  uint32_t root;
+  Stack stk;
+  RayState r;
+  double tmax;
+  Best best;
+  uint32_t sp, cur, ca, cb;
+  bool have, found;
+  uint32_t pops, tests;
+
+  __device__ __forceinline__ void start(const RayState &ray, double tmax0) {
+    r = ray;
+    tmax = tmax0;
+    best.hit = false;
+    best.inst = false;
+    sp = 0;
+    found = false;
+    pops = 0;
+    tests = 0;
+    // The root is tested like any popped node (TriangleAccelerator.cpp:165-192).
+    float4 lo, hi;
+    load_node(nodes, root, lo, hi);
+    if (STATS) ++pops;
+    have = !slab_cull(lo, hi, r, tmax);
+    cur = root;
+    ca = __float_as_uint(hi.z);
+    cb = __float_as_uint(hi.w);
+  }
+
+  __device__ __forceinline__ bool step() {
+    if (!have) {
+      if (sp == 0) return true;
+      uint32_t e = stk.get(sp - 1);
+      cur = e >> 3;
+      uint32_t left = e & 3u;
+      if (left > 1) {
+        uint32_t nxt = (e & 4u) ? cur - 1 : cur + 1;
+        stk.set(sp - 1, (nxt << 3) | (e & 4u) | (left - 1));
+      } else {
+        --sp;
+      }
+      float4 lo, hi;
+      load_node(nodes, cur, lo, hi);
+      if (slab_cull(lo, hi, r, tmax)) return false;
+      ca = __float_as_uint(hi.z);
+      cb = __float_as_uint(hi.w);
+    }
+    have = false;
+    const uint32_t a = ca, b = cb;
+    const uint32_t kind = b & 3u;
+    if (kind != 0) {
+      const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
+      const double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
+      const bool fwd = dax > 0.0;
+      float4 clo[3], chi[3];
+      uint32_t cidx[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        cidx[k] = fwd ? a + (uint32_t)k : a + nch - 1u - (uint32_t)k;
+        if ((uint32_t)k < nch) load_node(nodes, cidx[k], clo[k], chi[k]);
+      }
+      uint32_t surv = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if ((uint32_t)k < nch && !slab_cull(clo[k], chi[k], r, tmax)) surv |= 1u << k;
+      if (STATS) pops += nch;
+      if (surv == 0) return false;
+      const uint32_t s0 = __builtin_ctz(surv);
+      cur = s0 == 0 ? cidx[0] : (s0 == 1 ? cidx[1] : cidx[2]);
+      const float4 hs = s0 == 0 ? chi[0] : (s0 == 1 ? chi[1] : chi[2]);
+      ca = __float_as_uint(hs.z);
+      cb = __float_as_uint(hs.w);
+      have = true;
+      const uint32_t rest = surv & ~((2u << s0) - 1u);
+      if (rest) {
+        const uint32_t p = __builtin_ctz(rest);
+        const uint32_t start = p == 1 ? cidx[1] : cidx[2];
+        stk.set(sp++, (start << 3) | (fwd ? 0u : 4u) | (uint32_t)__builtin_popcount(rest));
+      }
+      return false;
+    }
+    bool stop = false;
+    if (INST && (b & 4u)) {
+      uint32_t ib = inst_range[2 * cur], ie = inst_range[2 * cur + 1];
+      for (uint32_t k = ib; k < ie && !stop; ++k) {
+        const DevInst *in = insts + k;
+        RayState tr;
+        instance_ray(in, r, tr);
+        Best nb;
+        if (walk_instance<ANY, STATS>(nodes, tris, in->root, tr, tmax, stk, sp, nb, pops, tests)) {
+          found = true;
+          if (ANY) {
+            stop = true;
+          } else {
+            best.hit = true;
+            best.inst = true;
+            best.prim = in->prim;
+            best.tri = reinterpret_cast<const uint4 *>(tris + nb.pos)[2].z;
+            best.b1 = nb.b1;
+            best.b2 = nb.b2;
+          }
+        }
+      }
+    }
+    const uint32_t cnt = b >> 3;
+    for (uint32_t k = 0; k < cnt && !stop; ++k) {
+      double t, b1, b2;
+      if (STATS) ++tests;
+      if (tri_test(tris, a + k, r, tmax, t, b1, b2)) {
+        found = true;
+        if (ANY) {
+          stop = true;
+        } else {
+          tmax = t;
+          best.hit = true;
+          best.inst = false;
+          best.pos = a + k;
+          best.b1 = b1;
+          best.b2 = b2;
+        }
+      }
+    }
+    if (ANY && stop) sp = 0;  // the walk ends at its first accepted hit
+    return false;
+  }
+};
+
+}  // namespace trace
+}  // namespace skw

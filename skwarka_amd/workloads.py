@@ -367,3 +367,27 @@ def scene_room(seed=2, n_spheres=200, subdiv_counts=(70, 111, 19), subdiv_levels
 def scene_room_small(seed=5):
     """A small room (20 spheres at subdivision 3 and 2) for parity tests."""
     return scene_room(seed=seed, n_spheres=20, subdiv_counts=(8, 12), subdiv_levels=(3, 2))
+
+
+def batch_scenes(n=64, seed=3, tri_lo=1e3, tri_hi=1e6, sizes=(128, 256, 512), spp=4):
+    """The batched workload: n seeded scenes whose triangle counts are log-uniform in
+    [tri_lo, tri_hi] (icospheres of subdivision 2-5 filling a unit-ish volume until the count is
+    reached, inside a 12-triangle box), image sizes drawn from `sizes` (square), spp samples.
+    Returns a list of (meshes, camera tuple, size, spp, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        target = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        meshes = [box(3.0)]
+        total = 12
+        while total < target:
+            left = target - total
+            sub = 5 if left > 20480 * 1.5 else (4 if left > 5120 * 1.5 else (3 if left > 1280 * 1.5 else 2))
+            r = float(np.exp(rng.uniform(np.log(0.1), np.log(0.8))))
+            c = rng.uniform(-2.2, 2.2, 3)
+            m = icosphere(sub, r, c, noise=0.02, rng=rng)
+            meshes.append(m)
+            total += len(m["tris"])
+        size = int(sizes[rng.integers(0, len(sizes))])
+        out.append((meshes, ((0.0, 0.0, -2.9), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0), size, spp, 1000 + i))
+    return out
