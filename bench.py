@@ -103,7 +103,7 @@ def bench_c2(torch, sk, wl, steps, warmup, stream):
     total = int(rays.sum())
     rgb = d_rgb.cpu().numpy().view(np.float32)
     return {"ms": ms, "value": total / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays], "build_s": build_s,
-            "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean())}
+            "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean()), "scene": scene}
 
 
 def bench_c0(torch, sk, wl, steps, warmup, stream):
@@ -167,6 +167,108 @@ def bench_c1(torch, sk, wl, steps, warmup, stream, c0):
             "mean_found": float(counts.mean()), "photons": ph, "p": p, "n": nrm}
 
 
+def bench_c3(torch, sk, wl, steps, warmup, stream):
+    specs = wl.batch_scenes()
+    t0 = time.time()
+    scenes, views = [], []
+    for meshes, camspec, size, spp, seed in specs:
+        scenes.append(sk.Scene(meshes))
+        cam, _ = sk.camera(*camspec, size, size)
+        views.append((cam, size, size, spp, seed))
+    build_s = time.time() - t0
+    batch = sk.Batch(scenes, views)
+    t0 = time.time()
+    cfg, tune_ms = batch.autotune(frames=1)
+    tune_s = time.time() - t0
+    n = batch.ray_count()
+    sptr = stream.cuda_stream
+    ms = time_steps(torch, lambda: batch.frame(sptr), steps, warmup, stream)
+    lat = batch.latency_ns() / 1e6  # completion of each scene in the last frame, ms after its start
+    tris = [int(s.info()["triangles"]) for s in scenes]
+    return {"ms": ms, "value": n / (ms * 1e-3) / 1e6, "rays": n, "config": cfg, "tune_ms": tune_ms,
+            "tune_s": tune_s, "build_s": build_s, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "n_scenes": len(scenes),
+            "tri_range": [min(tris), max(tris)], "tris_total": sum(tris), "specs": specs, "batch": batch,
+            "scenes": scenes}
+
+
+def bench_c4(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
+    s = spec if spec is not None else wl.scene_room()
+    if scene is None:
+        scene = sk.Scene(s["meshes"])
+    dls = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"],
+                     prim_group=s["prim_group"], n_groups=s["n_groups"])
+    w = h = 512
+    spp = 8
+    cam, cam15 = sk.camera(*s["camera"], w, h)
+    th0 = s["theta0"]
+    tt = wl.perturbed_theta(th0)
+    sptr = stream.cuda_stream
+    target = torch.empty((w * h, 3), dtype=torch.float64, device="cuda")
+    dls.render_replay(tt, tt, cam, w, h, spp, target.data_ptr(), 4, 4, 4321, sptr)
+    img = torch.empty_like(target)
+    adj = torch.empty_like(target)
+    grad = torch.empty(dls.n_params, dtype=torch.float64, device="cuda")
+    loss = torch.empty((), dtype=torch.float64, device="cuda")
+
+    def step():
+        # one gradient evaluation: forward image, L2 loss and its image gradient, adjoint replay
+        dls.render_replay(th0, th0, cam, w, h, spp, img.data_ptr(), 4, 4, 1234, sptr)
+        torch.sub(img, target, out=adj)
+        torch.sum(adj * adj, out=loss)
+        adj.mul_(2.0)
+        dls.adjoint(th0, cam, w, h, spp, adj.data_ptr(), grad.data_ptr(), 4, 4, 1234, sptr)
+
+    ms = time_steps(torch, step, steps, warmup, stream)
+    g = grad.cpu().numpy()
+    return {"ms": ms, "value": 1e3 / ms, "n_params": dls.n_params, "loss": float(loss.item()),
+            "grad_norm": float(np.linalg.norm(g)), "grad_nonzero": int(np.count_nonzero(g)), "spec": s,
+            "cam15": cam15, "theta0": th0, "target_theta": tt, "target": target, "img": img}
+
+
+def cpu_baseline_c3(c3, threads, n_sample):
+    import oracle
+    import torch
+    batch, scenes, specs = c3["batch"], c3["scenes"], c3["specs"]
+    order = sorted(range(len(scenes)), key=lambda i: c3["specs"][i][2] ** 2 * c3["specs"][i][3])
+    pick = [order[int(round(j * (len(order) - 1) / max(n_sample - 1, 1)))] for j in range(n_sample)]
+    rays_total, secs = 0, 0.0
+    for i in pick:
+        _, n = batch.hits(i)
+        d = torch.empty(n * 64, dtype=torch.uint8, device="cuda")
+        batch.rays(i, d.data_ptr())
+        torch.cuda.synchronize()
+        rays = d.cpu().numpy().view(np.float64).reshape(n, 8)
+        acc = oracle.Accel(specs[i][0])
+        t = time.perf_counter()
+        acc.closest_hit(rays, nthreads=threads)
+        secs += time.perf_counter() - t
+        rays_total += n
+    return {"value": rays_total / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "every camera ray of %d of the %d scenes (spread over frame sizes), %d rays, one timed run"
+                      % (len(pick), len(scenes), rays_total)}
+
+
+def cpu_baseline_c4(c4, threads, rows):
+    import oracle
+    s = c4["spec"]
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"],
+                         prim_group=s["prim_group"], n_groups=s["n_groups"])
+    w, spp = 512, 8
+    r0 = 256 - rows // 2
+    p0, p1 = r0 * w, (r0 + rows) * w
+    th0 = c4["theta0"]
+    target = c4["target"].cpu().numpy()[p0:p1]
+    t = time.perf_counter()
+    img = dls.render_replay(th0, th0, c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
+    dls.gradient(th0, 2.0 * (img - target), c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": (rows / 512.0) / dt, "unit": "gradients/s", "cores": threads, "kind": "port",
+            "sample": "image rows %d-%d of the 512x512 x 8 spp frame (forward image + FP64 forward-mode "
+                      "gradient of those rows), time scaled by 512/%d rows" % (r0, r0 + rows - 1, rows)}
+
+
 def cpu_baseline_c2(c2, threads, rows):
     import oracle  # CPU restatement of the reference path: the baseline, never the product
     s = c2["spec"]
@@ -222,7 +324,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["c0", "c1", "c2"], default=None,
+    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
     ap.add_argument("--cpu-rows", type=int, default=64, help="c2 image rows in the CPU baseline sample")
@@ -267,7 +369,17 @@ def main():
         log("c2: direct lighting, 2,026,260 triangles, 1024x1024 x 16 spp")
         c2 = bench_c2(torch, sk, wl, args.steps, args.warmup, stream)
         log("c2: %.1f ms/frame, %.1f Mrays/s, rays %s" % (c2["ms"], c2["value"], c2["rays"]))
-    cpu0 = cpu1 = cpu2 = None
+    c3 = c4 = None
+    if run("c3"):
+        log("c3: batched multi-scene closest hit, 64 scenes")
+        c3 = bench_c3(torch, sk, wl, 5, 1, stream)
+        log("c3: %.2f ms/frame, %.1f Mrays/s, p50 %.2f ms p99 %.2f ms, config %s" %
+            (c3["ms"], c3["value"], c3["p50_ms"], c3["p99_ms"], c3["config"]))
+    if run("c4"):
+        log("c4: direct-lighting gradient, 512x512 x 8 spp, 132 parameters")
+        c4 = bench_c4(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
+        log("c4: %.1f ms/gradient, %.2f gradients/s" % (c4["ms"], c4["value"]))
+    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
         if c2 is not None:
@@ -276,7 +388,31 @@ def main():
             cpu0 = cpu_baseline_c0(c0, threads)
         if c1 is not None:
             cpu1 = cpu_baseline_c1(c1, threads, 16384)
+        if c3 is not None:
+            cpu3 = cpu_baseline_c3(c3, threads, 4)
+        if c4 is not None:
+            cpu4 = cpu_baseline_c4(c4, threads, 16)
     comps = {}
+    if c3 is not None:
+        comps["c3_batch"] = {
+            "metric": "Mrays/s (64 scenes, one persistent launch per frame)", "value": round(c3["value"], 3),
+            "ms_per_step": round(c3["ms"], 4), "rays_per_step": c3["rays"],
+            "scene_latency_ms": {"p50": round(c3["p50_ms"], 4), "p99": round(c3["p99_ms"], 4),
+                                 "max": round(c3["max_ms"], 4)},
+            "autotuned": c3["config"], "autotune_s": round(c3["tune_s"], 2),
+            "workload": "%d scenes, %d-%d triangles (%d total), 128/256/512 square frames, 4 spp" %
+                        (c3["n_scenes"], c3["tri_range"][0], c3["tri_range"][1], c3["tris_total"]),
+            "cpu_baseline": cpu3,
+        }
+    if c4 is not None:
+        comps["c4_adjoint"] = {
+            "metric": "gradients/s (132-parameter direct-lighting gradient: forward + L2 loss + adjoint)",
+            "value": round(c4["value"], 4), "ms_per_step": round(c4["ms"], 3), "n_params": c4["n_params"],
+            "loss": c4["loss"], "grad_norm": c4["grad_norm"], "grad_nonzero": c4["grad_nonzero"],
+            "workload": "c2 scene at 512x512, 8 spp, 4 light + 4 BSDF samples; target = perturbed parameters "
+                        "(seed 4)",
+            "cpu_baseline": cpu4,
+        }
     if c0 is not None:
         comps["c0_closest_hit"] = {
             "metric": "Mrays/s closest-hit", "value": round(c0["value"], 3), "ms_per_step": round(c0["ms"], 5),
@@ -310,7 +446,8 @@ def main():
             "components": comps,
         }
     else:
-        c = comps.get("c0_closest_hit") or comps.get("c1_photon_gather")
+        c = (comps.get("c0_closest_hit") or comps.get("c1_photon_gather") or comps.get("c3_batch")
+             or comps.get("c4_adjoint"))
         line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
                 "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -171,7 +171,14 @@ typedef struct {
   uint32_t n_lights;
   const uint32_t *light_prim;      /* prim index of each area light */
   const double *light_radiance;    /* 3 per light */
+  /* Parameter groups for skw_dl_render_replay / skw_dl_adjoint (optional: n_groups 0 and
+   * prim_group NULL). A primitive in group g takes gray reflectance theta[g] (as kd) and roughness
+   * theta[G + g]; light l takes radiance theta[2G + l] * light_radiance[l]. */
+  uint32_t n_groups;               /* G, at most 64 */
+  const uint32_t *prim_group;      /* per prim: group index or SKW_NO_GROUP */
 } skw_dl_desc;
+
+#define SKW_NO_GROUP 0xFFFFFFFFu
 
 /* Pinhole camera (PerspectiveCamera::GenerateRay with lens radius 0): origin, camera-to-world
  * rotation (row-major), tan of half the x field of view, and y tan = x tan * height / width. */
@@ -199,6 +206,35 @@ skw_status skw_dl_scene_destroy(skw_dl_scene *dl);
  * on `stream`; the call returns after the frame has been enqueued and the ray counts read. */
 skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
                          const skw_dl_params *params, float *d_rgb, uint64_t *rays_out, void *stream);
+
+/* ---------------------------------------------------------------- differentiable direct lighting
+ * No counterpart in the reference. Parameters theta (host, 2G + n_lights doubles): reflectance
+ * per group, roughness per group, emission scale per light (see skw_dl_desc).
+ *
+ * The replayed estimator F(theta_eval; theta_sample) runs skw_dl_render's wavefront with every
+ * sampling decision, pdf, MIS weight and light-choice CDF taken at theta_sample and the BSDF
+ * values and emitted radiance taken at theta_eval, so F(theta; theta) is skw_dl_render at theta
+ * and F(., theta_sample) is smooth in theta_eval for a fixed set of paths. Its gradient at
+ * theta_eval = theta_sample is an unbiased estimate of the gradient of the expected image (the
+ * MIS weights sum to one over the strategies; these parameters move no geometry). */
+
+/* Number of parameters, 2G + n_lights. */
+skw_status skw_dl_param_count(const skw_dl_scene *dl, uint32_t *n);
+
+/* d_image = width*height*3 doubles: the replayed estimator's mean radiance per pixel. */
+skw_status skw_dl_render_replay(const skw_scene *scene, const skw_dl_scene *dl, const double *theta_eval,
+                                const double *theta_sample, const skw_camera *cam, const skw_dl_params *params,
+                                double *d_image, uint64_t *rays_out, void *stream);
+
+/* Adjoint of skw_dl_render_replay at theta_eval = theta_sample = theta: replays the same paths
+ * (recomputing them, nothing is kept from a forward call) and writes
+ *   d_grad[j] = sum over pixels and channels of d_adjoint[pixel, c] * dImage[pixel, c] / dtheta[j]
+ * (d_adjoint = width*height*3 doubles on the device, e.g. dLoss/dImage; d_grad = 2G + n_lights
+ * doubles on the device, overwritten). Accumulates with FP64 atomics (LDS per workgroup, then
+ * global), so the summation order, and the last bits, vary from call to call. */
+skw_status skw_dl_adjoint(const skw_scene *scene, const skw_dl_scene *dl, const double *theta,
+                          const skw_camera *cam, const skw_dl_params *params, const double *d_adjoint,
+                          double *d_grad, uint64_t *rays_out, void *stream);
 
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in

@@ -69,6 +69,12 @@ def lib():
         "orc_dl_scene_destroy": ([_P], None),
         "orc_dl_render": ([_P, _P, _P, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P, _P, _P, ctypes.c_int],
                           None),
+        "orc_dl_scene_create_groups": ([_P, _P, _P, _P, _P, _U32, _P, _P, _U32, _P], _P),
+        "orc_dl_param_count": ([_P], _U32),
+        "orc_dl_render_replay": ([_P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P,
+                                  ctypes.c_int], None),
+        "orc_dl_gradient": ([_P, _P, _P, _P, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P, _P,
+                             ctypes.c_int], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -276,14 +282,48 @@ class PhotonMap:
 class DLScene:
     """Direct-lighting restatement over an Accel (materials / lights as skwarka_amd.DLScene)."""
 
-    def __init__(self, accel, mat_type, kd, ks, roughness, light_prim, light_radiance):
+    def __init__(self, accel, mat_type, kd, ks, roughness, light_prim, light_radiance, prim_group=None,
+                 n_groups=0):
         self.accel = accel
         self._keep = [np.ascontiguousarray(mat_type, np.uint8), np.ascontiguousarray(kd, np.float64),
                       np.ascontiguousarray(ks, np.float64), np.ascontiguousarray(roughness, np.float64),
                       np.ascontiguousarray(light_prim, np.uint32), np.ascontiguousarray(light_radiance, np.float64)]
         k = self._keep
-        self.h = lib().orc_dl_scene_create(accel.h, k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data,
-                                           k[3].ctypes.data, len(k[4]), k[4].ctypes.data, k[5].ctypes.data)
+        grp = None if prim_group is None else np.ascontiguousarray(prim_group, np.uint32)
+        self._keep.append(grp)
+        self.h = lib().orc_dl_scene_create_groups(accel.h, k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data,
+                                                  k[3].ctypes.data, len(k[4]), k[4].ctypes.data, k[5].ctypes.data,
+                                                  n_groups if grp is not None else 0, _ptr(grp))
+        self.n_params = int(lib().orc_dl_param_count(self.h))
+
+    def render_replay(self, theta_eval, theta_sample, cam15, width, height, spp, light_samples=4, bsdf_samples=4,
+                      seed=0, p0=0, p1=None, nthreads=None):
+        """Replayed estimator F(theta_eval; theta_sample): (P, 3) float64 mean radiance."""
+        p1 = width * height if p1 is None else p1
+        cam = np.ascontiguousarray(cam15, np.float64)
+        te = np.ascontiguousarray(theta_eval, np.float64)
+        ts = np.ascontiguousarray(theta_sample, np.float64)
+        assert te.shape == (self.n_params,) and ts.shape == (self.n_params,)
+        img = np.zeros((p1 - p0, 3), np.float64)
+        lib().orc_dl_render_replay(self.accel.h, self.h, te.ctypes.data, ts.ctypes.data, cam.ctypes.data, width,
+                                   height, spp, light_samples, bsdf_samples, seed, p0, p1, img.ctypes.data,
+                                   nthreads or threads())
+        return img
+
+    def gradient(self, theta, adj, cam15, width, height, spp, light_samples=4, bsdf_samples=4, seed=0, p0=0,
+                 p1=None, nthreads=None):
+        """(grad, image): grad[j] = sum adj * dF/dtheta_j (FP64 forward-mode dual numbers)."""
+        p1 = width * height if p1 is None else p1
+        cam = np.ascontiguousarray(cam15, np.float64)
+        th = np.ascontiguousarray(theta, np.float64)
+        a = np.ascontiguousarray(adj, np.float64).reshape(p1 - p0, 3)
+        assert th.shape == (self.n_params,)
+        grad = np.zeros(self.n_params, np.float64)
+        img = np.zeros((p1 - p0, 3), np.float64)
+        lib().orc_dl_gradient(self.accel.h, self.h, th.ctypes.data, cam.ctypes.data, width, height, spp,
+                              light_samples, bsdf_samples, seed, p0, p1, a.ctypes.data, grad.ctypes.data,
+                              img.ctypes.data, nthreads or threads())
+        return grad, img
 
     def __del__(self):
         try:

@@ -58,6 +58,26 @@ void orc_dl_render(const orc_accel *accel, const orc_dl_scene *dls, const double
                    uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0,
                    uint64_t p1, float *rgb, uint64_t *counters, uint32_t *primary_prim, uint32_t *primary_tri,
                    int nthreads);
+/* Parameter groups: prim_group[p] in [0, n_groups) or 0xFFFFFFFF; theta = [reflectance G,
+ * roughness G, emission scale per light]. */
+orc_dl_scene *orc_dl_scene_create_groups(const orc_accel *accel, const uint8_t *mat_type, const double *kd,
+                                         const double *ks, const double *roughness, uint32_t n_lights,
+                                         const uint32_t *light_prim, const double *light_radiance, uint32_t n_groups,
+                                         const uint32_t *prim_group);
+uint32_t orc_dl_param_count(const orc_dl_scene *h);
+/* Replayed estimator F(theta_eval; theta_sample): sampling decisions, pdfs, MIS weights and the
+ * light CDF at theta_sample, BSDF values and emitted radiance at theta_eval. img: 3 doubles per
+ * pixel of [p0, p1). */
+void orc_dl_render_replay(const orc_accel *accel, const orc_dl_scene *dls, const double *theta_eval,
+                          const double *theta_sample, const double *cam, uint32_t width, uint32_t height, uint32_t spp,
+                          uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0, uint64_t p1,
+                          double *img, int nthreads);
+/* grad[j] = sum over pixels of [p0, p1) and channels of adj * dF/dtheta_j at theta (forward-mode
+ * dual numbers); img optional. */
+void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const double *theta, const double *cam,
+                     uint32_t width, uint32_t height, uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples,
+                     uint64_t seed, uint64_t p0, uint64_t p1, const double *adj, double *grad, double *img,
+                     int nthreads);
 
 /* Generic point kd-tree (KDTree<Point3D_d> / KDTree<Point3D_f>) for the reference's own tests. */
 orc_kdtree *orc_kdtree_create_d(uint64_t n, const double *pts);

@@ -71,7 +71,10 @@ struct DLScene {
   std::vector<double> kd, ks, rough;
   std::vector<Light> lights;
   std::vector<int> light_of_prim;
+  uint32_t n_groups = 0;
+  std::vector<uint32_t> group;  // per prim, NO_GROUP if none
 };
+static const uint32_t NO_GROUP = 0xFFFFFFFFu;
 
 // Triangle3D<float>::GetArea: float cross product, float length, (float)0.5 * length.
 static float tri_area_f(F3 a, F3 b, F3 c) {
@@ -151,12 +154,12 @@ static bool bbox_inside(const BBoxD &b, V3 p) {
 }
 
 // IrradianceLightsSamplingStrategy::GetLightsCDF(point, normal) for area lights only.
-static void lights_cdf(const DLScene &s, V3 p, V3 n, double *cdf) {
+static void lights_cdf(const DLScene &s, const double *intensity, V3 p, V3 n, double *cdf) {
   size_t nl = s.lights.size();
   for (size_t i = 0; i < nl; ++i) {
     const Light &L = s.lights[i];
     if (bbox_inside(L.bbox, p)) {
-      cdf[i] = PI_ * L.intensity;
+      cdf[i] = PI_ * intensity[i];
     } else {
       double avg = 0.0;
       for (int j = 0; j < 8; ++j) {
@@ -169,7 +172,7 @@ static void lights_cdf(const DLScene &s, V3 p, V3 n, double *cdf) {
       avg *= 0.125;
       if (avg > 0.0) {
         if (avg < 0.1) avg = 0.1;
-        cdf[i] = subtended_solid_angle(p, L.bbox) * L.intensity * avg;
+        cdf[i] = subtended_solid_angle(p, L.bbox) * intensity[i] * avg;
       } else {
         cdf[i] = 0.0;
       }
@@ -306,7 +309,9 @@ struct Bsdf {
     return p / 2.0;
   }
   // BSDF::Sample with non-specular flags.
-  bool sample(V3 wo_w, double u0, double u1, double uc, V3 &wi_w, double &pdf_out, double f[3]) const {
+  // wo_l / wi_l: the local pair f was evaluated with.
+  bool sample(V3 wo_w, double u0, double u1, double uc, V3 &wi_w, double &pdf_out, double f[3], V3 &wo_l,
+              V3 &wi_l) const {
     size_t num = plastic ? 2 : 1;
     size_t comp = smin((size_t)(uc * (double)num), num - 1);
     V3 wo = to_local(wo_w);
@@ -335,6 +340,8 @@ struct Bsdf {
       p /= (double)num;
     }
     pdf_out = p;
+    wo_l = wo;
+    wi_l = wi;
     if (dot(wo_w, ng) * dot(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
     else f[0] = f[1] = f[2] = 0.0;
     return true;
@@ -346,12 +353,345 @@ static inline double power_heuristic(double n1, double p1, double n2, double p2)
   return (f * f) / (f * f + g * g);
 }
 
+// ---------------------------------------------------------------- forward-mode derivatives
+// Dual number over the parameters one camera sample can depend on: [0] reflectance of the hit
+// primitive's group, [1] its roughness, [2 + l] the emission scale of light l. Values follow
+// exactly the double operations of the plain render, so value parts are bit-identical to it.
+constexpr int ND = 2 + 16;
+struct Dual {
+  double v = 0.0;
+  double d[ND] = {};
+  Dual() = default;
+  Dual(double x) : v(x) {}
+};
+static inline Dual operator+(const Dual &a, const Dual &b) {
+  Dual r(a.v + b.v);
+  for (int j = 0; j < ND; ++j) r.d[j] = a.d[j] + b.d[j];
+  return r;
+}
+static inline Dual operator+(const Dual &a, double b) {
+  Dual r = a;
+  r.v = a.v + b;
+  return r;
+}
+static inline Dual operator*(const Dual &a, const Dual &b) {
+  Dual r(a.v * b.v);
+  for (int j = 0; j < ND; ++j) r.d[j] = a.d[j] * b.v + a.v * b.d[j];
+  return r;
+}
+static inline Dual operator*(const Dual &a, double b) {
+  Dual r(a.v * b);
+  for (int j = 0; j < ND; ++j) r.d[j] = a.d[j] * b;
+  return r;
+}
+static inline Dual operator*(double a, const Dual &b) {
+  Dual r(a * b.v);
+  for (int j = 0; j < ND; ++j) r.d[j] = a * b.d[j];
+  return r;
+}
+static inline Dual operator/(const Dual &a, double b) {
+  Dual r(a.v / b);
+  for (int j = 0; j < ND; ++j) r.d[j] = a.d[j] / b;
+  return r;
+}
+static inline double value(double x) { return x; }
+static inline double value(const Dual &x) { return x.v; }
+// b^e for a constant base b >= 0 (d/de b^e = b^e ln b, taken as 0 where b^e is 0).
+static inline double pow_c(double b, double e) { return std::pow(b, e); }
+static inline Dual pow_c(double b, const Dual &e) {
+  Dual r(std::pow(b, e.v));
+  double k = r.v > 0.0 ? r.v * std::log(b) : 0.0;
+  for (int j = 0; j < ND; ++j) r.d[j] = k * e.d[j];
+  return r;
+}
+template <class T>
+static inline T seeded(double v, int dir, double dv);
+template <>
+inline double seeded<double>(double v, int, double) {
+  return v;
+}
+template <>
+inline Dual seeded<Dual>(double v, int dir, double dv) {
+  Dual r(v);
+  r.d[dir] = dv;
+  return r;
+}
+
+// Evaluation-side material: the values the estimator is differentiated through.
+template <class T>
+struct EvalMat {
+  T kd[3];
+  double ks[3];
+  T e;
+  bool plastic;
+};
+
+// Bsdf::eval_reflect_local with the evaluation material.
+template <class T>
+static void eval_local(const EvalMat<T> &m, V3 wo, V3 wi, T f[3]) {
+  for (int c = 0; c < 3; ++c) f[c] = m.kd[c] * INV_PI_;
+  if (!m.plastic) return;
+  double ci = std::fabs(wo.z), co = std::fabs(wi.z);
+  if (ci < DBL_EPS_ || co < DBL_EPS_) return;
+  V3 h = add(wo, wi);
+  if (std::fabs(h.x) < DBL_EPS_ && std::fabs(h.y) < DBL_EPS_ && std::fabs(h.z) < DBL_EPS_) return;
+  normalize(h);
+  double fres = fresnel_dielectric(dot(wi, h));
+  double ch = std::fabs(h.z);
+  double tmp = 2.0 * ch / std::fabs(dot(wo, h));
+  double g = smin(1.0, smin(ci * tmp, co * tmp));
+  T d = (m.e + 2.0) * 0.15915494309189533577 * pow_c(ch, m.e);
+  T sc = d * g / (4.0 * ci * co);
+  for (int c = 0; c < 3; ++c) f[c] = f[c] + (m.ks[c] * fres) * sc;
+}
+// Bsdf::evaluate (world vectors) with the evaluation material.
+template <class T>
+static void eval_world(const Bsdf &frame, const EvalMat<T> &m, V3 wo_w, V3 wi_w, T f[3]) {
+  V3 wo = frame.to_local(wo_w), wi = frame.to_local(wi_w);
+  if (dot(wo_w, frame.ng) * dot(wi_w, frame.ng) > 0) {
+    eval_local(m, wo, wi, f);
+  } else {
+    for (int c = 0; c < 3; ++c) f[c] = 0.0;
+  }
+}
+
+// Material and light values of one parameter vector. A grouped primitive takes gray kd =
+// clamp(theta[g], 0, 1) and exponent 1/clamp(theta[G+g], 0.001, 1) clamped to [0, 1000]; light l
+// takes radiance theta[2G+l] * its base radiance. Derivatives are 0 where a clamp holds.
+struct ParamView {
+  std::vector<double> kd, e, dkd, de_dr;  // kd: 3 per prim
+  std::vector<double> rad, intensity;     // rad: 3 per light
+};
+
+static void make_view(const DLScene &s, const double *theta, ParamView &v) {
+  const size_t np = s.mat.size(), nl = s.lights.size();
+  const uint32_t G = s.n_groups;
+  v.kd.assign(3 * np, 0.0);
+  v.e.assign(np, 0.0);
+  v.dkd.assign(np, 0.0);
+  v.de_dr.assign(np, 0.0);
+  for (size_t p = 0; p < np; ++p) {
+    for (int c = 0; c < 3; ++c) v.kd[3 * p + c] = smin(1.0, smax(0.0, s.kd[3 * p + c]));
+    double rough = s.rough[p] < 0.001 ? 0.001 : (s.rough[p] > 1.0 ? 1.0 : s.rough[p]);
+    double ex = 1.0 / rough;
+    v.e[p] = ex < 0.0 ? 0.0 : (ex > 1000.0 ? 1000.0 : ex);
+    if (theta && s.group[p] != NO_GROUP) {
+      const uint32_t g = s.group[p];
+      const double rho = theta[g];
+      const double kd = rho < 0.0 ? 0.0 : (rho > 1.0 ? 1.0 : rho);
+      for (int c = 0; c < 3; ++c) v.kd[3 * p + c] = kd;
+      v.dkd[p] = (rho > 0.0 && rho < 1.0) ? 1.0 : 0.0;
+      const double r = theta[G + g];
+      const double rc = r < 0.001 ? 0.001 : (r > 1.0 ? 1.0 : r);
+      const double e = 1.0 / rc;
+      v.e[p] = e < 0.0 ? 0.0 : (e > 1000.0 ? 1000.0 : e);
+      v.de_dr[p] = (r > 0.001 && r < 1.0) ? -1.0 / (r * r) : 0.0;
+    }
+  }
+  v.rad.assign(3 * nl, 0.0);
+  v.intensity.assign(nl, 0.0);
+  for (size_t l = 0; l < nl; ++l) {
+    const Light &L = s.lights[l];
+    if (!theta) {
+      for (int c = 0; c < 3; ++c) v.rad[3 * l + c] = L.radiance[c];
+      v.intensity[l] = L.intensity;
+      continue;
+    }
+    const double sc = theta[2 * G + l];
+    double power[3];
+    for (int c = 0; c < 3; ++c) {
+      v.rad[3 * l + c] = sc * L.radiance[c];
+      power[c] = (PI_ * L.area) * v.rad[3 * l + c];
+    }
+    v.intensity[l] = luminance(power) / (PI_ * L.bbox.area());
+  }
+}
+
+struct FrameSpec {
+  const orc_accel *accel;
+  const DLScene *s;
+  const double *cam;
+  uint32_t width, height, spp, nls, nbs;
+  uint64_t seed;
+  const ParamView *vs;  // sampling: BSDF sampling, pdfs, light CDF, control flow
+  const ParamView *ve;  // evaluation: BSDF values and emitted radiance
+};
+
+// One camera sample of LTEIntegrator::Radiance with direct lighting (the per-sample body of the
+// render), with values of type T taken from the evaluation view. Returns whether the sample
+// counts (the SamplerBasedRenderer rule) and the primitive the camera ray hit.
+template <class T>
+static bool dl_sample(const FrameSpec &F, uint64_t pix, uint32_t si, T L[3], uint32_t &hit_prim, uint32_t &hit_tri,
+                      uint64_t &c_shadow, uint64_t &c_bsdf) {
+  const DLScene &s = *F.s;
+  const ParamView &vs = *F.vs, &ve = *F.ve;
+  const double *cam = F.cam;
+  const size_t nl = s.lights.size();
+  const uint32_t px = (uint32_t)(pix % F.width), py = (uint32_t)(pix / F.width);
+  const uint64_t sidx = pix * F.spp + si;
+  // PerspectiveCamera::GenerateRay with lens radius 0
+  double ix = (double)px + rnd(F.seed, sidx, 0), iy = (double)py + rnd(F.seed, sidx, 1);
+  double cx = 2.0 * cam[12] * (ix - F.width * 0.5) / F.width;
+  double cy = 2.0 * cam[13] * (iy - F.height * 0.5) / F.height;
+  V3 d{cam[3] * cx + cam[4] * cy + cam[5], cam[6] * cx + cam[7] * cy + cam[8], cam[9] * cx + cam[10] * cy + cam[11]};
+  normalize(d);
+  double ray[8] = {cam[0], cam[1], cam[2], d.x, d.y, d.z, 0.0, DBL_INF_};
+  orc_hit h;
+  orc_hit_geom g;
+  orc_accel_intersect_one(F.accel, ray, &h, &g);
+  hit_prim = h.prim;
+  hit_tri = h.tri;
+  for (int c = 0; c < 3; ++c) L[c] = 0.0;
+  if (h.prim != 0xffffffffu) {
+    V3 wo = mul(d, -1.0);
+    V3 p{g.p[0], g.p[1], g.p[2]}, ng{g.ng[0], g.ng[1], g.ng[2]}, ns{g.ns[0], g.ns[1], g.ns[2]};
+    T Le[3] = {0.0, 0.0, 0.0};
+    int li = s.light_of_prim[h.prim];
+    if (li >= 0 && dot(wo, ng) > 0.0)
+      for (int c = 0; c < 3; ++c) Le[c] = seeded<T>(ve.rad[3 * li + c], 2 + li, s.lights[li].radiance[c]);
+    // BSDF frame (BSDF.h:182-192) with the DG tangent of a mesh without UVs or tangents
+    // (TriangleMesh.cpp:520-540): CoordinateSystem(ns), then made orthogonal to ns.
+    V3 t;
+    if (std::fabs(ns.x) > std::fabs(ns.y)) {
+      double il = 1.0 / std::sqrt(ns.x * ns.x + ns.z * ns.z);
+      t = V3{-ns.z * il, 0.0, ns.x * il};
+    } else {
+      double il = 1.0 / std::sqrt(ns.y * ns.y + ns.z * ns.z);
+      t = V3{0.0, ns.z * il, -ns.y * il};
+    }
+    V3 t2 = cross(ns, t);
+    V3 tangent;
+    if (len_sqr(t2) > DBL_EPS_) {
+      tangent = cross(t2, ns);
+      normalize(tangent);
+    } else {
+      tangent = t;
+    }
+    const uint32_t pr = h.prim;
+    Bsdf b;  // sampling BSDF
+    b.n = ns;
+    b.ng = ng;
+    b.e1 = tangent;
+    b.e2 = cross(ns, tangent);
+    b.plastic = s.mat[pr] == 1;
+    for (int c = 0; c < 3; ++c) {
+      b.kd[c] = vs.kd[3 * pr + c];
+      b.ks[c] = smin(1.0, smax(0.0, s.ks[3 * pr + c]));
+    }
+    b.e = vs.e[pr];
+    EvalMat<T> me;  // evaluation material
+    me.plastic = b.plastic;
+    for (int c = 0; c < 3; ++c) {
+      me.kd[c] = seeded<T>(ve.kd[3 * pr + c], 0, ve.dkd[pr]);
+      me.ks[c] = b.ks[c];
+    }
+    me.e = seeded<T>(ve.e[pr], 1, ve.de_dr[pr]);
+    auto rad_e = [&](size_t l, int c) { return seeded<T>(ve.rad[3 * l + c], 2 + (int)l, s.lights[l].radiance[c]); };
+    T cdl[3] = {0.0, 0.0, 0.0};
+    if (nl > 0) {
+      std::vector<double> cdf(nl);
+      V3 nrm = dot(wo, ns) >= 0.0 ? ns : mul(ns, -1.0);
+      lights_cdf(s, vs.intensity.data(), p, nrm, cdf.data());
+      T dl[3] = {0.0, 0.0, 0.0};
+      // _SampleLights (area lights only)
+      for (uint32_t k = 0; k < F.nls; ++k) {
+        double uc = rnd(F.seed, sidx, 2 + k);
+        double u0 = rnd(F.seed, sidx, 6 + 2 * k), u1 = rnd(F.seed, sidx, 7 + 2 * k);
+        double comp_pdf;
+        size_t l = search_cdf(cdf.data(), nl, uc, &comp_pdf);
+        const Light &Lt = s.lights[l];
+        double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
+        if (Lt.cdf.empty()) continue;
+        size_t ti = search_cdf(Lt.cdf.data(), Lt.cdf.size(), tri_u, nullptr);
+        double tmp = std::sqrt(u0);
+        double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
+        V3 sp = add(add(mul(Lt.v0[ti], bb1), mul(Lt.v1[ti], bb2)), mul(Lt.v2[ti], 1.0 - bb1 - bb2));
+        V3 lv = sub(sp, p);
+        double len = std::sqrt(len_sqr(lv));
+        V3 wi = mul(lv, 1.0 / len);
+        double ac = std::fabs(dot(wi, Lt.normal[ti]));
+        double lpdf = ac > DBL_EPS_ ? (len * len) / (Lt.area * ac) : 0.0;
+        bool lit = dot(Lt.normal[ti], wi) < 0.0;
+        const double *rs = &vs.rad[3 * l];
+        double le[3] = {lit ? rs[0] : 0.0, lit ? rs[1] : 0.0, lit ? rs[2] : 0.0};
+        if (!(lpdf > 0.0) || (le[0] == 0.0 && le[1] == 0.0 && le[2] == 0.0)) continue;
+        double f[3];
+        b.evaluate(wo, wi, f);
+        double lf[3] = {le[0] * f[0], le[1] * f[1], le[2] * f[2]};
+        if (lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0) continue;
+        double mint = orc_next_min_t(F.accel, pr, &g, &wi.x);
+        double sray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, len - 1e-4};
+        uint8_t occ = 0;
+        orc_any_hit(F.accel, 1, sray, &occ, 1);
+        ++c_shadow;
+        if (occ) continue;
+        double bp = b.pdf(wo, wi);
+        double w = power_heuristic((double)F.nls, lpdf * comp_pdf, (double)F.nbs, bp);
+        w *= std::fabs(dot(wi, ns)) / (lpdf * comp_pdf);
+        T fe[3];
+        eval_world(b, me, wo, wi, fe);
+        for (int c = 0; c < 3; ++c) {
+          T v = (lit ? rad_e(l, c) : T(0.0)) * fe[c];
+          dl[c] = dl[c] + v * w;
+        }
+      }
+      // ComputeDirectLighting = (0 + _SampleLights) + _SampleBSDF; each divides by its sample
+      // count through Spectrum::operator/ (multiplication by the reciprocal).
+      for (int c = 0; c < 3; ++c) cdl[c] = dl[c] * (1.0 / (double)F.nls);
+      // _SampleBSDF
+      T db[3] = {0.0, 0.0, 0.0};
+      for (uint32_t k = 0; k < F.nbs; ++k) {
+        double uc = rnd(F.seed, sidx, 14 + k);
+        double u0 = rnd(F.seed, sidx, 18 + 2 * k), u1 = rnd(F.seed, sidx, 19 + 2 * k);
+        V3 wi, wo_l, wi_l;
+        double bp, f[3];
+        b.sample(wo, u0, u1, uc, wi, bp, f, wo_l, wi_l);
+        if (!(bp > 0.0) || (f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) continue;
+        double mint = orc_next_min_t(F.accel, pr, &g, &wi.x);
+        double bray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, DBL_INF_};
+        orc_hit h2;
+        orc_hit_geom g2;
+        orc_accel_intersect_one(F.accel, bray, &h2, &g2);
+        ++c_bsdf;
+        if (h2.prim == 0xffffffffu) continue;
+        int l2 = s.light_of_prim[h2.prim];
+        if (l2 < 0) continue;
+        V3 ng2{g2.ng[0], g2.ng[1], g2.ng[2]};
+        if (!(dot(mul(wi, -1.0), ng2) > 0.0)) continue;
+        const Light &Lt = s.lights[l2];
+        const double *rs = &vs.rad[3 * l2];
+        if (rs[0] == 0.0 && rs[1] == 0.0 && rs[2] == 0.0) continue;
+        V3 ln = Lt.normal[h2.tri];
+        double dist = h2.t - mint;
+        double ac = std::fabs(dot(wi, ln));
+        double lpdf = ac > DBL_EPS_ ? (dist * dist) / (Lt.area * ac) : 0.0;
+        double cp = l2 == 0 ? cdf[0] : cdf[l2] - cdf[l2 - 1];
+        double w = power_heuristic((double)F.nbs, bp, (double)F.nls, lpdf * cp);
+        w *= std::fabs(dot(wi, ns)) / bp;
+        // f is non-black, so the pair passed the geometric-normal test: evaluate it again with the
+        // evaluation material.
+        T fe[3];
+        eval_local(me, wo_l, wi_l, fe);
+        for (int c = 0; c < 3; ++c) db[c] = db[c] + fe[c] * rad_e(l2, c) * w;
+      }
+      for (int c = 0; c < 3; ++c) cdl[c] = cdl[c] + db[c] * (1.0 / (double)F.nbs);
+    }
+    for (int c = 0; c < 3; ++c) L[c] = Le[c] + cdl[c];
+  }
+  // SamplerBasedRenderer.cpp:458-478: NaN, negative-luminance and infinite samples become black
+  double Lv[3] = {value(L[0]), value(L[1]), value(L[2])};
+  bool bad = std::isnan(Lv[0]) || std::isnan(Lv[1]) || std::isnan(Lv[2]) || luminance(Lv) < -DBL_EPS_ ||
+             std::isinf(Lv[0]) || std::isinf(Lv[1]) || std::isinf(Lv[2]);
+  return !bad;
+}
+
 }  // namespace orc
 
 using namespace orc;
 
 struct orc_dl_scene {
   DLScene s;
+  ParamView nominal;  // the scene's own materials
 };
 
 extern "C" {
@@ -359,7 +699,14 @@ extern "C" {
 orc_dl_scene *orc_dl_scene_create(const orc_accel *accel, const uint8_t *mat_type, const double *kd, const double *ks,
                                   const double *roughness, uint32_t n_lights, const uint32_t *light_prim,
                                   const double *light_radiance) {
-  (void)accel;
+  return orc_dl_scene_create_groups(accel, mat_type, kd, ks, roughness, n_lights, light_prim, light_radiance, 0,
+                                    nullptr);
+}
+
+orc_dl_scene *orc_dl_scene_create_groups(const orc_accel *accel, const uint8_t *mat_type, const double *kd,
+                                         const double *ks, const double *roughness, uint32_t n_lights,
+                                         const uint32_t *light_prim, const double *light_radiance, uint32_t n_groups,
+                                         const uint32_t *prim_group) {
   orc_dl_scene *h = new orc_dl_scene();
   DLScene &s = h->s;
   uint32_t np = orc_accel_prim_count(accel);
@@ -368,6 +715,9 @@ orc_dl_scene *orc_dl_scene_create(const orc_accel *accel, const uint8_t *mat_typ
   s.ks.assign(ks, ks + 3 * (size_t)np);
   s.rough.assign(roughness, roughness + np);
   s.light_of_prim.assign(np, -1);
+  s.n_groups = n_groups;
+  s.group.assign(np, NO_GROUP);
+  if (n_groups > 0 && prim_group) s.group.assign(prim_group, prim_group + np);
   for (uint32_t l = 0; l < n_lights; ++l) {
     Light L;
     L.prim = light_prim[l];
@@ -405,10 +755,13 @@ orc_dl_scene *orc_dl_scene_create(const orc_accel *accel, const uint8_t *mat_typ
     L.intensity = luminance(power) / (PI_ * L.bbox.area());
     s.lights.push_back(L);
   }
+  make_view(s, nullptr, h->nominal);
   return h;
 }
 
 void orc_dl_scene_destroy(orc_dl_scene *h) { delete h; }
+
+uint32_t orc_dl_param_count(const orc_dl_scene *h) { return 2 * h->s.n_groups + (uint32_t)h->s.lights.size(); }
 
 // Renders pixels [p0, p1) of a width x height image at spp samples each.
 // cam: origin (3), camera-to-world rotation (9, row-major), x_tan, y_tan. Output rgb (float, 3 per pixel of
@@ -417,153 +770,22 @@ void orc_dl_render(const orc_accel *accel, const orc_dl_scene *dls, const double
                    uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0,
                    uint64_t p1, float *rgb, uint64_t *counters, uint32_t *primary_prim, uint32_t *primary_tri,
                    int nthreads) {
-  const DLScene &s = dls->s;
-  const size_t nl = s.lights.size();
+  FrameSpec F{accel, &dls->s, cam, width, height, spp, n_light_samples, n_bsdf_samples, seed, &dls->nominal,
+              &dls->nominal};
   uint64_t c_prim = 0, c_shadow = 0, c_bsdf = 0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : c_prim, c_shadow, c_bsdf)
   for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix) {
-    uint32_t px = (uint32_t)(pix % width), py = (uint32_t)(pix / width);
     double acc[3] = {0, 0, 0};
-    std::vector<double> cdf(nl > 0 ? nl : 1);
     for (uint32_t si = 0; si < spp; ++si) {
-      uint64_t sidx = (uint64_t)pix * spp + si;
-      // PerspectiveCamera::GenerateRay with lens radius 0
-      double ix = (double)px + rnd(seed, sidx, 0), iy = (double)py + rnd(seed, sidx, 1);
-      double cx = 2.0 * cam[12] * (ix - width * 0.5) / width;
-      double cy = 2.0 * cam[13] * (iy - height * 0.5) / height;
-      V3 d{cam[3] * cx + cam[4] * cy + cam[5], cam[6] * cx + cam[7] * cy + cam[8], cam[9] * cx + cam[10] * cy + cam[11]};
-      normalize(d);
-      double ray[8] = {cam[0], cam[1], cam[2], d.x, d.y, d.z, 0.0, DBL_INF_};
-      orc_hit h;
-      orc_hit_geom g;
-      orc_accel_intersect_one(accel, ray, &h, &g);
+      double L[3];
+      uint32_t hp, ht;
+      bool good = dl_sample<double>(F, (uint64_t)pix, si, L, hp, ht, c_shadow, c_bsdf);
       ++c_prim;
       if (primary_prim) {
-        primary_prim[(pix - p0) * spp + si] = h.prim;
-        primary_tri[(pix - p0) * spp + si] = h.tri;
+        primary_prim[(pix - p0) * spp + si] = hp;
+        primary_tri[(pix - p0) * spp + si] = ht;
       }
-      double L[3] = {0, 0, 0};
-      double cdl[3] = {0, 0, 0};
-      if (h.prim != 0xffffffffu) {
-        V3 wo = mul(d, -1.0);
-        V3 p{g.p[0], g.p[1], g.p[2]}, ng{g.ng[0], g.ng[1], g.ng[2]}, ns{g.ns[0], g.ns[1], g.ns[2]};
-        int li = s.light_of_prim[h.prim];
-        if (li >= 0 && dot(wo, ng) > 0.0)
-          for (int c = 0; c < 3; ++c) L[c] = s.lights[li].radiance[c];
-        // BSDF frame (BSDF.h:182-192) with the DG tangent of a mesh without UVs or tangents
-        // (TriangleMesh.cpp:520-540): CoordinateSystem(ns), then made orthogonal to ns.
-        V3 t;
-        if (std::fabs(ns.x) > std::fabs(ns.y)) {
-          double il = 1.0 / std::sqrt(ns.x * ns.x + ns.z * ns.z);
-          t = V3{-ns.z * il, 0.0, ns.x * il};
-        } else {
-          double il = 1.0 / std::sqrt(ns.y * ns.y + ns.z * ns.z);
-          t = V3{0.0, ns.z * il, -ns.y * il};
-        }
-        V3 t2 = cross(ns, t);
-        V3 tangent;
-        if (len_sqr(t2) > DBL_EPS_) {
-          tangent = cross(t2, ns);
-          normalize(tangent);
-        } else {
-          tangent = t;
-        }
-        Bsdf b;
-        b.n = ns;
-        b.ng = ng;
-        b.e1 = tangent;
-        b.e2 = cross(ns, tangent);
-        uint32_t pr = h.prim;
-        b.plastic = s.mat[pr] == 1;
-        for (int c = 0; c < 3; ++c) {
-          b.kd[c] = smin(1.0, smax(0.0, s.kd[3 * pr + c]));
-          b.ks[c] = smin(1.0, smax(0.0, s.ks[3 * pr + c]));
-        }
-        double rough = s.rough[pr] < 0.001 ? 0.001 : (s.rough[pr] > 1.0 ? 1.0 : s.rough[pr]);
-        double ex = 1.0 / rough;
-        b.e = ex < 0.0 ? 0.0 : (ex > 1000.0 ? 1000.0 : ex);
-        if (nl > 0) {
-          V3 nrm = dot(wo, ns) >= 0.0 ? ns : mul(ns, -1.0);
-          lights_cdf(s, p, nrm, cdf.data());
-          double dl[3] = {0, 0, 0};
-          // _SampleLights (area lights only)
-          for (uint32_t k = 0; k < n_light_samples; ++k) {
-            double uc = rnd(seed, sidx, 2 + k);
-            double u0 = rnd(seed, sidx, 6 + 2 * k), u1 = rnd(seed, sidx, 7 + 2 * k);
-            double comp_pdf;
-            size_t l = search_cdf(cdf.data(), nl, uc, &comp_pdf);
-            const Light &Lt = s.lights[l];
-            double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
-            if (Lt.cdf.empty()) continue;
-            size_t ti = search_cdf(Lt.cdf.data(), Lt.cdf.size(), tri_u, nullptr);
-            double tmp = std::sqrt(u0);
-            double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
-            V3 sp = add(add(mul(Lt.v0[ti], bb1), mul(Lt.v1[ti], bb2)), mul(Lt.v2[ti], 1.0 - bb1 - bb2));
-            V3 lv = sub(sp, p);
-            double len = std::sqrt(len_sqr(lv));
-            V3 wi = mul(lv, 1.0 / len);
-            double ac = std::fabs(dot(wi, Lt.normal[ti]));
-            double lpdf = ac > DBL_EPS_ ? (len * len) / (Lt.area * ac) : 0.0;
-            bool lit = dot(Lt.normal[ti], wi) < 0.0;
-            double le[3] = {lit ? Lt.radiance[0] : 0.0, lit ? Lt.radiance[1] : 0.0, lit ? Lt.radiance[2] : 0.0};
-            if (!(lpdf > 0.0) || (le[0] == 0.0 && le[1] == 0.0 && le[2] == 0.0)) continue;
-            double f[3];
-            b.evaluate(wo, wi, f);
-            double lf[3] = {le[0] * f[0], le[1] * f[1], le[2] * f[2]};
-            if (lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0) continue;
-            double mint = orc_next_min_t(accel, pr, &g, &wi.x);
-            double sray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, len - 1e-4};
-            uint8_t occ = 0;
-            orc_any_hit(accel, 1, sray, &occ, 1);
-            ++c_shadow;
-            if (occ) continue;
-            double bp = b.pdf(wo, wi);
-            double w = power_heuristic((double)n_light_samples, lpdf * comp_pdf, (double)n_bsdf_samples, bp);
-            w *= std::fabs(dot(wi, ns)) / (lpdf * comp_pdf);
-            for (int c = 0; c < 3; ++c) dl[c] += lf[c] * w;
-          }
-          // ComputeDirectLighting = (0 + _SampleLights) + _SampleBSDF; each divides by its sample
-          // count through Spectrum::operator/ (multiplication by the reciprocal).
-          for (int c = 0; c < 3; ++c) cdl[c] = dl[c] * (1.0 / (double)n_light_samples);
-          // _SampleBSDF
-          double db[3] = {0, 0, 0};
-          for (uint32_t k = 0; k < n_bsdf_samples; ++k) {
-            double uc = rnd(seed, sidx, 14 + k);
-            double u0 = rnd(seed, sidx, 18 + 2 * k), u1 = rnd(seed, sidx, 19 + 2 * k);
-            V3 wi;
-            double bp, f[3];
-            b.sample(wo, u0, u1, uc, wi, bp, f);
-            if (!(bp > 0.0) || (f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) continue;
-            double mint = orc_next_min_t(accel, pr, &g, &wi.x);
-            double bray[8] = {p.x, p.y, p.z, wi.x, wi.y, wi.z, mint, DBL_INF_};
-            orc_hit h2;
-            orc_hit_geom g2;
-            orc_accel_intersect_one(accel, bray, &h2, &g2);
-            ++c_bsdf;
-            if (h2.prim == 0xffffffffu) continue;
-            int l2 = s.light_of_prim[h2.prim];
-            if (l2 < 0) continue;
-            V3 ng2{g2.ng[0], g2.ng[1], g2.ng[2]};
-            if (!(dot(mul(wi, -1.0), ng2) > 0.0)) continue;
-            const Light &Lt = s.lights[l2];
-            if (Lt.radiance[0] == 0.0 && Lt.radiance[1] == 0.0 && Lt.radiance[2] == 0.0) continue;
-            V3 ln = Lt.normal[h2.tri];
-            double dist = h2.t - mint;
-            double ac = std::fabs(dot(wi, ln));
-            double lpdf = ac > DBL_EPS_ ? (dist * dist) / (Lt.area * ac) : 0.0;
-            double cp = l2 == 0 ? cdf[0] : cdf[l2] - cdf[l2 - 1];
-            double w = power_heuristic((double)n_bsdf_samples, bp, (double)n_light_samples, lpdf * cp);
-            w *= std::fabs(dot(wi, ns)) / bp;
-            for (int c = 0; c < 3; ++c) db[c] += f[c] * Lt.radiance[c] * w;
-          }
-          for (int c = 0; c < 3; ++c) cdl[c] += db[c] * (1.0 / (double)n_bsdf_samples);
-        }
-        for (int c = 0; c < 3; ++c) L[c] = L[c] + cdl[c];
-      }
-      // SamplerBasedRenderer.cpp:458-478: NaN, negative-luminance and infinite samples become black
-      bool bad = std::isnan(L[0]) || std::isnan(L[1]) || std::isnan(L[2]) || luminance(L) < -DBL_EPS_ ||
-                 std::isinf(L[0]) || std::isinf(L[1]) || std::isinf(L[2]);
-      if (!bad)
+      if (good)
         for (int c = 0; c < 3; ++c) acc[c] += L[c];
     }
     for (int c = 0; c < 3; ++c) rgb[3 * (pix - p0) + c] = (float)(acc[c] / (double)spp);
@@ -573,6 +795,79 @@ void orc_dl_render(const orc_accel *accel, const orc_dl_scene *dls, const double
     counters[1] = c_shadow;
     counters[2] = c_bsdf;
   }
+}
+
+// The replayed estimator F(theta_eval; theta_sample) over pixels [p0, p1): mean radiance per pixel
+// in double.
+void orc_dl_render_replay(const orc_accel *accel, const orc_dl_scene *dls, const double *theta_eval,
+                          const double *theta_sample, const double *cam, uint32_t width, uint32_t height, uint32_t spp,
+                          uint32_t n_light_samples, uint32_t n_bsdf_samples, uint64_t seed, uint64_t p0, uint64_t p1,
+                          double *img, int nthreads) {
+  ParamView vs, ve;
+  make_view(dls->s, theta_sample, vs);
+  make_view(dls->s, theta_eval, ve);
+  FrameSpec F{accel, &dls->s, cam, width, height, spp, n_light_samples, n_bsdf_samples, seed, &vs, &ve};
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix) {
+    double acc[3] = {0, 0, 0};
+    uint64_t cs = 0, cb = 0;
+    for (uint32_t si = 0; si < spp; ++si) {
+      double L[3];
+      uint32_t hp, ht;
+      if (dl_sample<double>(F, (uint64_t)pix, si, L, hp, ht, cs, cb))
+        for (int c = 0; c < 3; ++c) acc[c] += L[c];
+    }
+    for (int c = 0; c < 3; ++c) img[3 * (pix - p0) + c] = acc[c] / (double)spp;
+  }
+}
+
+// Gradient of sum_{pixel, c} adj[pixel, c] * image[pixel, c] over pixels [p0, p1) with respect to
+// theta (theta_eval = theta_sample = theta), by forward-mode dual numbers per sample; img (optional)
+// receives the image.
+void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const double *theta, const double *cam,
+                     uint32_t width, uint32_t height, uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples,
+                     uint64_t seed, uint64_t p0, uint64_t p1, const double *adj, double *grad, double *img,
+                     int nthreads) {
+  const DLScene &s = dls->s;
+  ParamView v;
+  make_view(s, theta, v);
+  FrameSpec F{accel, &s, cam, width, height, spp, n_light_samples, n_bsdf_samples, seed, &v, &v};
+  const uint32_t G = s.n_groups, nl = (uint32_t)s.lights.size(), n_params = 2 * G + nl;
+  std::vector<double> total(n_params, 0.0);
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    std::vector<double> part(n_params, 0.0);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix) {
+      double acc[3] = {0, 0, 0};
+      uint64_t cs = 0, cb = 0;
+      const double *a = adj + 3 * (pix - p0);
+      for (uint32_t si = 0; si < spp; ++si) {
+        Dual L[3];
+        uint32_t hp, ht;
+        if (!dl_sample<Dual>(F, (uint64_t)pix, si, L, hp, ht, cs, cb)) continue;
+        for (int c = 0; c < 3; ++c) acc[c] += L[c].v;
+        if (hp == 0xffffffffu) continue;
+        const double inv_spp = 1.0 / (double)spp;
+        double dj[ND];
+        for (int j = 0; j < ND; ++j) {
+          dj[j] = 0.0;
+          for (int c = 0; c < 3; ++c) dj[j] += a[c] * L[c].d[j];
+          dj[j] *= inv_spp;
+        }
+        if (s.group[hp] != NO_GROUP) {
+          part[s.group[hp]] += dj[0];
+          part[G + s.group[hp]] += dj[1];
+        }
+        for (uint32_t l = 0; l < nl; ++l) part[2 * G + l] += dj[2 + l];
+      }
+      if (img)
+        for (int c = 0; c < 3; ++c) img[3 * (pix - p0) + c] = acc[c] / (double)spp;
+    }
+#pragma omp critical
+    for (uint32_t j = 0; j < n_params; ++j) total[j] += part[j];
+  }
+  for (uint32_t j = 0; j < n_params; ++j) grad[j] = total[j];
 }
 
 }  // extern "C"

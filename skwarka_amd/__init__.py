@@ -29,6 +29,7 @@ EXPORTS = [
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
     "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
+    "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
     "skw_batch_create", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
 ]
@@ -48,7 +49,7 @@ class SceneDesc(ctypes.Structure):
 
 class DLDesc(ctypes.Structure):
     _fields_ = [("mat_type", _P), ("kd", _P), ("ks", _P), ("roughness", _P), ("n_lights", _U32),
-                ("light_prim", _P), ("light_radiance", _P)]
+                ("light_prim", _P), ("light_radiance", _P), ("n_groups", _U32), ("prim_group", _P)]
 
 
 class Camera(ctypes.Structure):
@@ -106,6 +107,9 @@ def lib():
         "skw_device_count": ([], ctypes.c_int),
         "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
         "skw_dl_scene_destroy": ([_P], ctypes.c_int),
+        "skw_dl_param_count": ([_P, _P], ctypes.c_int),
+        "skw_dl_render_replay": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_dl_adjoint": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
         "skw_batch_create": ([_P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_batch_destroy": ([_P], ctypes.c_int),
@@ -327,7 +331,7 @@ def camera(origin, target_dir, up, x_fov_deg, width, height):
 class DLScene:
     """Materials and area lights for skw_dl_render over a Scene."""
 
-    def __init__(self, scene, mat_type, kd, ks, roughness, light_prim, light_radiance):
+    def __init__(self, scene, mat_type, kd, ks, roughness, light_prim, light_radiance, prim_group=None, n_groups=0):
         L = lib()
         self._mt = np.ascontiguousarray(mat_type, np.uint8)
         self._kd = np.ascontiguousarray(kd, np.float64)
@@ -335,12 +339,16 @@ class DLScene:
         self._r = np.ascontiguousarray(roughness, np.float64)
         self._lp = np.ascontiguousarray(light_prim, np.uint32)
         self._lr = np.ascontiguousarray(light_radiance, np.float64)
+        self._grp = None if prim_group is None else np.ascontiguousarray(prim_group, np.uint32)
         desc = DLDesc(_ptr(self._mt), _ptr(self._kd), _ptr(self._ks), _ptr(self._r), len(self._lp), _ptr(self._lp),
-                      _ptr(self._lr))
+                      _ptr(self._lr), n_groups if self._grp is not None else 0, _ptr(self._grp))
         h = _P()
         check(L.skw_dl_scene_create(scene.handle, ctypes.byref(desc), ctypes.byref(h)), "skw_dl_scene_create")
         self.handle = h
         self.scene = scene
+        n = ctypes.c_uint32()
+        check(L.skw_dl_param_count(h, ctypes.byref(n)), "skw_dl_param_count")
+        self.n_params = n.value
 
     def close(self):
         if self.handle:
@@ -358,6 +366,34 @@ class DLScene:
         rays = np.zeros(3, np.uint64)
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
                                   rays.ctypes.data, stream or None), "skw_dl_render")
+        return rays
+
+    def _theta(self, theta):
+        t = np.ascontiguousarray(theta, np.float64)
+        if t.shape != (self.n_params,):
+            raise ValueError("theta must hold %d values" % self.n_params)
+        return t
+
+    def render_replay(self, theta_eval, theta_sample, cam, width, height, spp, d_image, light_samples=4,
+                      bsdf_samples=4, seed=0, stream=0):
+        """Replayed estimator F(theta_eval; theta_sample) into d_image (width*height*3 doubles)."""
+        te, ts = self._theta(theta_eval), self._theta(theta_sample)
+        prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
+        rays = np.zeros(3, np.uint64)
+        check(lib().skw_dl_render_replay(self.scene.handle, self.handle, te.ctypes.data, ts.ctypes.data,
+                                         ctypes.byref(cam), ctypes.byref(prm), d_image, rays.ctypes.data,
+                                         stream or None), "skw_dl_render_replay")
+        return rays
+
+    def adjoint(self, theta, cam, width, height, spp, d_adjoint, d_grad, light_samples=4, bsdf_samples=4, seed=0,
+                stream=0):
+        """d_grad (n_params doubles) = sum d_adjoint * dImage/dtheta, replaying the paths at theta."""
+        th = self._theta(theta)
+        prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
+        rays = np.zeros(3, np.uint64)
+        check(lib().skw_dl_adjoint(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
+                                   ctypes.byref(prm), d_adjoint, d_grad, rays.ctypes.data, stream or None),
+              "skw_dl_adjoint")
         return rays
 
 

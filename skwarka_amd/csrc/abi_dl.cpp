@@ -33,15 +33,204 @@ struct skw_dl_scene {
   std::vector<DevMat> mats;
   std::vector<DevLight> lights;
   std::vector<DevLightTri> ltris;
+  std::vector<double> rad;  // 3 per light
+  uint32_t n_groups = 0;
   DevMat *d_mats = nullptr;
   DevLight *d_lights = nullptr;
   DevLightTri *d_ltris = nullptr;
+  double *d_rad = nullptr;
   ~skw_dl_scene() {
-    void *ptrs[] = {d_mats, d_lights, d_ltris};
+    void *ptrs[] = {d_mats, d_lights, d_ltris, d_rad};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
   }
 };
+
+namespace {
+
+// What a frame produces: a float image, a double image, or (adj set) a gradient.
+struct FrameOut {
+  float *rgb = nullptr;
+  double *img = nullptr;
+  const double *adj = nullptr;
+  double *grad = nullptr;
+};
+
+// Parameter tables of one call, placed at the front of the frame's device block.
+struct ParamPlan {
+  bool active = false;
+  bool alias = false;  // theta_eval equals theta_sample: one material table
+  const double *theta_s = nullptr, *theta_e = nullptr;
+};
+
+skw_status check_frame_args(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                            const skw_dl_params *prm) {
+  if (!scene || !dl || !cam || !prm) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
+  if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(SKW_INVALID_ARG, "empty image");
+  if (prm->light_samples == 0 || prm->light_samples > 8 || prm->bsdf_samples == 0 || prm->bsdf_samples > 8)
+    return fail(SKW_INVALID_ARG, "light_samples and bsdf_samples must be in [1, 8]");
+  return SKW_OK;
+}
+
+// The wavefront of one frame: per sample pass, camera -> closest hit -> shade -> occlusion and
+// BSDF rays (ray counts read on the device) -> resolve; then the image (or nothing, for the
+// adjoint, whose resolve accumulates the gradient).
+skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const ParamPlan &pp, const skw_camera *cam,
+                     const skw_dl_params *prm, const FrameOut &out, uint64_t *rays_out, hipStream_t st) {
+  const uint64_t npix = (uint64_t)prm->width * prm->height;
+  const uint64_t batch = npix;
+  const uint32_t nls = prm->light_samples, nbs = prm->bsdf_samples, nl = (uint32_t)dl->lights.size();
+  const uint32_t np = (uint32_t)dl->mats.size();
+  const bool adj = out.adj != nullptr;
+  DlArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.geom = scene->geom_args();
+  a.ltris = dl->d_ltris;
+  a.n_lights = nl;
+  a.n_groups = dl->n_groups;
+  for (int i = 0; i < 3; ++i) a.cam_o[i] = cam->origin[i];
+  for (int i = 0; i < 9; ++i) a.cam_r[i] = cam->rot[i];
+  a.x_tan = cam->x_tan;
+  a.y_tan = cam->y_tan;
+  a.width = prm->width;
+  a.height = prm->height;
+  a.spp = prm->spp;
+  a.nls = nls;
+  a.nbs = nbs;
+  a.seed = prm->seed;
+  a.pix0 = 0;
+  a.batch = batch;
+  // One device block holds every per-frame buffer.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  size_t o_mats_s = 0, o_mats_e = 0, o_lights = 0, o_rad_e = 0, o_rad_b = 0;
+  if (pp.active) {
+    o_mats_s = take(np * sizeof(DevMat));
+    if (!pp.alias) o_mats_e = take(np * sizeof(DevMat));
+    o_lights = take(nl * sizeof(DevLight));
+    o_rad_e = take(nl * 3 * sizeof(double));
+    o_rad_b = take(nl * 3 * sizeof(double));
+  }
+  size_t o_prim_rays = take(batch * sizeof(skw_ray)), o_prim_hits = take(batch * sizeof(skw_hit));
+  size_t o_le = take(batch * 3 * sizeof(double)), o_cdf = take(batch * (nl ? nl : 1) * sizeof(double));
+  size_t o_ns = take(batch * 3 * sizeof(double));
+  size_t o_srays = take(batch * nls * sizeof(skw_ray)), o_sidx = take(batch * nls * sizeof(int32_t));
+  size_t o_sc = take(batch * nls * 3 * sizeof(double)), o_occ = take(batch * nls);
+  size_t o_brays = take(batch * nbs * sizeof(skw_ray)), o_bidx = take(batch * nbs * sizeof(int32_t));
+  size_t o_brec = take(batch * nbs * 8 * sizeof(double)), o_bhits = take(batch * nbs * sizeof(skw_hit));
+  size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(3 * sizeof(uint64_t));
+  size_t o_film = take(npix * 3 * sizeof(double));
+  size_t o_saux = 0, o_caml = 0;
+  if (adj) {
+    o_saux = take(batch * nls * 6 * sizeof(double));
+    o_caml = take(batch * sizeof(int32_t));
+  }
+  char *base = nullptr;
+  SKW_HIP(hipMallocAsync((void **)&base, off, st), "frame buffer allocation");
+  hipError_t e = hipSuccess;
+  if (pp.active) {
+    DlParamArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.base_mats = dl->d_mats;
+    pa.base_lights = dl->d_lights;
+    pa.n_prims = np;
+    pa.n_lights = nl;
+    pa.n_groups = dl->n_groups;
+    pa.mats_s = (DevMat *)(base + o_mats_s);
+    pa.mats_e = pp.alias ? pa.mats_s : (DevMat *)(base + o_mats_e);
+    pa.lights_s = (DevLight *)(base + o_lights);
+    pa.rad_e = (double *)(base + o_rad_e);
+    pa.rad_base = (double *)(base + o_rad_b);
+    const uint32_t n_params = 2 * dl->n_groups + nl;
+    for (uint32_t j = 0; j < n_params; ++j) {
+      pa.theta_s[j] = pp.theta_s[j];
+      pa.theta_e[j] = pp.theta_e[j];
+    }
+    e = launch_dl_params(pa, st);
+    a.mats = pa.mats_s;
+    a.mats_e = pa.mats_e;
+    a.lights = pa.lights_s;
+    a.rad_e = pa.rad_e;
+    a.rad_base = pa.rad_base;
+  } else {
+    a.mats = dl->d_mats;
+    a.mats_e = dl->d_mats;
+    a.lights = dl->d_lights;
+    a.rad_e = dl->d_rad;
+    a.rad_base = dl->d_rad;
+  }
+  a.prim_rays = (skw_ray *)(base + o_prim_rays);
+  a.prim_hits = (skw_hit *)(base + o_prim_hits);
+  a.le = (double *)(base + o_le);
+  a.cdf = (double *)(base + o_cdf);
+  a.ns = (double *)(base + o_ns);
+  a.shadow_rays = (skw_ray *)(base + o_srays);
+  a.shadow_idx = (int32_t *)(base + o_sidx);
+  a.shadow_c = (double *)(base + o_sc);
+  a.occluded = (uint8_t *)(base + o_occ);
+  a.bsdf_rays = (skw_ray *)(base + o_brays);
+  a.bsdf_idx = (int32_t *)(base + o_bidx);
+  a.bsdf_rec = (double *)(base + o_brec);
+  a.bsdf_hits = (skw_hit *)(base + o_bhits);
+  a.counts = (uint64_t *)(base + o_counts);
+  a.totals = (uint64_t *)(base + o_totals);
+  a.film = (double *)(base + o_film);
+  if (adj) {
+    a.adj = out.adj;
+    a.grad = out.grad;
+    a.shadow_aux = (double *)(base + o_saux);
+    a.cam_light = (int32_t *)(base + o_caml);
+  }
+  skw_status rs = SKW_OK;
+  if (e == hipSuccess) e = launch_dl_clear(a, st);
+  if (e == hipSuccess && adj) e = launch_dl_clear_grad(out.grad, 2 * dl->n_groups + nl, st);
+  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
+    a.pass = s;
+    e = launch_dl_camera(a, st);
+    if (e != hipSuccess) break;
+    rs = enqueue_trace(scene, a.prim_rays, batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr, false);
+    if (rs != SKW_OK) break;
+    e = launch_dl_shade(a, st);
+    if (e != hipSuccess) break;
+    rs = enqueue_trace(scene, a.shadow_rays, batch * nls, a.counts, nullptr, a.occluded, st, true, nullptr, nullptr,
+                       false);
+    if (rs != SKW_OK) break;
+    rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
+                       nullptr, false);
+    if (rs != SKW_OK) break;
+    e = launch_dl_resolve(a, st);
+  }
+  if (e == hipSuccess && rs == SKW_OK) {
+    if (out.rgb) e = launch_dl_finish(a, out.rgb, st);
+    else if (out.img) e = launch_dl_finish_f64(a, out.img, st);
+  }
+  uint64_t totals[3] = {0, 0, 0};
+  if (e == hipSuccess && rs == SKW_OK && rays_out) {
+    e = hipMemcpyAsync(totals, a.totals, sizeof(totals), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  (void)hipFreeAsync(base, st);
+  if (rs != SKW_OK) return rs;
+  if (e != hipSuccess) return hip_fail(e, "direct lighting frame");
+  if (rays_out)
+    for (int i = 0; i < 3; ++i) rays_out[i] = totals[i];
+  return SKW_OK;
+}
+
+skw_status check_theta(const skw_dl_scene *dl, const double *theta) {
+  if (!theta) return fail(SKW_INVALID_ARG, "theta must not be NULL");
+  const uint32_t n = 2 * dl->n_groups + (uint32_t)dl->lights.size();
+  for (uint32_t j = 0; j < n; ++j)
+    if (!std::isfinite(theta[j])) return fail(SKW_INVALID_ARG, "theta holds a non-finite value");
+  return SKW_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -55,10 +244,21 @@ skw_status skw_dl_scene_create(const skw_scene *scene, const skw_dl_desc *desc, 
   if (desc->n_lights > kMaxLights) return fail(SKW_UNSUPPORTED, "at most 16 area lights");
   if (desc->n_lights > 0 && (!desc->light_prim || !desc->light_radiance))
     return fail(SKW_INVALID_ARG, "light arrays must not be NULL");
+  if (desc->n_groups > kDlMaxGroups) return fail(SKW_UNSUPPORTED, "at most 64 parameter groups");
+  if (desc->n_groups > 0 && np > 0 && !desc->prim_group)
+    return fail(SKW_INVALID_ARG, "prim_group must not be NULL when n_groups > 0");
   std::unique_ptr<skw_dl_scene> d(new skw_dl_scene());
+  d->n_groups = desc->n_groups;
   d->mats.resize(np);
   for (size_t p = 0; p < np; ++p) {
     DevMat &m = d->mats[p];
+    std::memset(&m, 0, sizeof(m));
+    m.group = kMiss;
+    if (desc->n_groups > 0) {
+      uint32_t g = desc->prim_group[p];
+      if (g != SKW_NO_GROUP && g >= desc->n_groups) return fail(SKW_INVALID_ARG, "prim_group out of range");
+      m.group = g == SKW_NO_GROUP ? kMiss : g;
+    }
     if (desc->mat_type[p] > 1) return fail(SKW_INVALID_ARG, "material type must be 0 (matte) or 1 (plastic)");
     m.type = desc->mat_type[p];
     for (int c = 0; c < 3; ++c) {
@@ -144,10 +344,12 @@ skw_status skw_dl_scene_create(const skw_scene *scene, const skw_dl_desc *desc, 
     for (int c = 0; c < 3; ++c) lum += kLumW[c] * ((kPi * L.area) * L.radiance[c]);
     L.intensity = lum / (kPi * bbox_area);
     d->lights.push_back(L);
+    for (int c = 0; c < 3; ++c) d->rad.push_back(L.radiance[c]);
   }
   SKW_HIP(upload(&d->d_mats, d->mats), "upload materials");
   SKW_HIP(upload(&d->d_lights, d->lights), "upload lights");
   SKW_HIP(upload(&d->d_ltris, d->ltris), "upload light triangles");
+  SKW_HIP(upload(&d->d_rad, d->rad), "upload light radiance");
   *out = d.release();
   return SKW_OK;
 }
@@ -159,98 +361,54 @@ skw_status skw_dl_scene_destroy(skw_dl_scene *dl) {
 
 skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
                          const skw_dl_params *prm, float *d_rgb, uint64_t *rays_out, void *stream) {
-  if (!scene || !dl || !cam || !prm || !d_rgb) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
-  if (!scene->on_device) return fail(SKW_UNSUPPORTED, "scene was created host-only");
-  if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(SKW_INVALID_ARG, "empty image");
-  if (prm->light_samples == 0 || prm->light_samples > 8 || prm->bsdf_samples == 0 || prm->bsdf_samples > 8)
-    return fail(SKW_INVALID_ARG, "light_samples and bsdf_samples must be in [1, 8]");
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const uint64_t npix = (uint64_t)prm->width * prm->height;
-  const uint64_t batch = npix;
-  const uint32_t nls = prm->light_samples, nbs = prm->bsdf_samples, nl = (uint32_t)dl->lights.size();
-  DlArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.geom = scene->geom_args();
-  a.mats = dl->d_mats;
-  a.lights = dl->d_lights;
-  a.ltris = dl->d_ltris;
-  a.n_lights = nl;
-  for (int i = 0; i < 3; ++i) a.cam_o[i] = cam->origin[i];
-  for (int i = 0; i < 9; ++i) a.cam_r[i] = cam->rot[i];
-  a.x_tan = cam->x_tan;
-  a.y_tan = cam->y_tan;
-  a.width = prm->width;
-  a.height = prm->height;
-  a.spp = prm->spp;
-  a.nls = nls;
-  a.nbs = nbs;
-  a.seed = prm->seed;
-  a.pix0 = 0;
-  a.batch = batch;
-  // One device block holds every per-frame buffer.
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off += (bytes + 255) & ~size_t(255);
-    return o;
-  };
-  size_t o_prim_rays = take(batch * sizeof(skw_ray)), o_prim_hits = take(batch * sizeof(skw_hit));
-  size_t o_le = take(batch * 3 * sizeof(double)), o_cdf = take(batch * (nl ? nl : 1) * sizeof(double));
-  size_t o_ns = take(batch * 3 * sizeof(double));
-  size_t o_srays = take(batch * nls * sizeof(skw_ray)), o_sidx = take(batch * nls * sizeof(int32_t));
-  size_t o_sc = take(batch * nls * 3 * sizeof(double)), o_occ = take(batch * nls);
-  size_t o_brays = take(batch * nbs * sizeof(skw_ray)), o_bidx = take(batch * nbs * sizeof(int32_t));
-  size_t o_brec = take(batch * nbs * 6 * sizeof(double)), o_bhits = take(batch * nbs * sizeof(skw_hit));
-  size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(3 * sizeof(uint64_t));
-  size_t o_film = take(npix * 3 * sizeof(double));
-  char *base = nullptr;
-  SKW_HIP(hipMallocAsync((void **)&base, off, st), "frame buffer allocation");
-  a.prim_rays = (skw_ray *)(base + o_prim_rays);
-  a.prim_hits = (skw_hit *)(base + o_prim_hits);
-  a.le = (double *)(base + o_le);
-  a.cdf = (double *)(base + o_cdf);
-  a.ns = (double *)(base + o_ns);
-  a.shadow_rays = (skw_ray *)(base + o_srays);
-  a.shadow_idx = (int32_t *)(base + o_sidx);
-  a.shadow_c = (double *)(base + o_sc);
-  a.occluded = (uint8_t *)(base + o_occ);
-  a.bsdf_rays = (skw_ray *)(base + o_brays);
-  a.bsdf_idx = (int32_t *)(base + o_bidx);
-  a.bsdf_rec = (double *)(base + o_brec);
-  a.bsdf_hits = (skw_hit *)(base + o_bhits);
-  a.counts = (uint64_t *)(base + o_counts);
-  a.totals = (uint64_t *)(base + o_totals);
-  a.film = (double *)(base + o_film);
-  skw_status rs = SKW_OK;
-  hipError_t e = launch_dl_clear(a, st);
-  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
-    a.pass = s;
-    e = launch_dl_camera(a, st);
-    if (e != hipSuccess) break;
-    rs = enqueue_trace(scene, a.prim_rays, batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr, false);
-    if (rs != SKW_OK) break;
-    e = launch_dl_shade(a, st);
-    if (e != hipSuccess) break;
-    rs = enqueue_trace(scene, a.shadow_rays, batch * nls, a.counts, nullptr, a.occluded, st, true, nullptr, nullptr,
-                       false);
-    if (rs != SKW_OK) break;
-    rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
-                       nullptr, false);
-    if (rs != SKW_OK) break;
-    e = launch_dl_resolve(a, st);
-  }
-  if (e == hipSuccess && rs == SKW_OK) e = launch_dl_finish(a, d_rgb, st);
-  uint64_t totals[3] = {0, 0, 0};
-  if (e == hipSuccess && rs == SKW_OK && rays_out) {
-    e = hipMemcpyAsync(totals, a.totals, sizeof(totals), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-  }
-  (void)hipFreeAsync(base, st);
-  if (rs != SKW_OK) return rs;
-  if (e != hipSuccess) return hip_fail(e, "direct lighting frame");
-  if (rays_out)
-    for (int i = 0; i < 3; ++i) rays_out[i] = totals[i];
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  if (!d_rgb) return fail(SKW_INVALID_ARG, "d_rgb must not be NULL");
+  FrameOut out;
+  out.rgb = d_rgb;
+  return run_frame(scene, dl, ParamPlan(), cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
+}
+
+skw_status skw_dl_param_count(const skw_dl_scene *dl, uint32_t *n) {
+  if (!dl || !n) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  *n = 2 * dl->n_groups + (uint32_t)dl->lights.size();
   return SKW_OK;
+}
+
+skw_status skw_dl_render_replay(const skw_scene *scene, const skw_dl_scene *dl, const double *theta_eval,
+                                const double *theta_sample, const skw_camera *cam, const skw_dl_params *prm,
+                                double *d_image, uint64_t *rays_out, void *stream) {
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  if (!d_image) return fail(SKW_INVALID_ARG, "d_image must not be NULL");
+  if ((s = check_theta(dl, theta_eval)) != SKW_OK || (s = check_theta(dl, theta_sample)) != SKW_OK) return s;
+  ParamPlan pp;
+  pp.active = true;
+  pp.theta_e = theta_eval;
+  pp.theta_s = theta_sample;
+  const size_t n = 2 * dl->n_groups + dl->lights.size();
+  pp.alias = n == 0 || std::memcmp(theta_eval, theta_sample, n * sizeof(double)) == 0;
+  FrameOut out;
+  out.img = d_image;
+  return run_frame(scene, dl, pp, cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
+}
+
+skw_status skw_dl_adjoint(const skw_scene *scene, const skw_dl_scene *dl, const double *theta,
+                          const skw_camera *cam, const skw_dl_params *prm, const double *d_adjoint,
+                          double *d_grad, uint64_t *rays_out, void *stream) {
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  if (!d_adjoint || !d_grad) return fail(SKW_INVALID_ARG, "d_adjoint and d_grad must not be NULL");
+  if ((s = check_theta(dl, theta)) != SKW_OK) return s;
+  ParamPlan pp;
+  pp.active = true;
+  pp.alias = true;
+  pp.theta_e = theta;
+  pp.theta_s = theta;
+  FrameOut out;
+  out.adj = d_adjoint;
+  out.grad = d_grad;
+  return run_frame(scene, dl, pp, cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

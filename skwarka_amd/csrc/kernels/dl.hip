@@ -173,7 +173,10 @@ struct Bsdf {
     double ct = fabs(h.z);
     return ((e + 1.0) * pow(ct, e)) / (2.0 * kPi * 4.0 * fabs(dot3(wo, h)));
   }
-  __device__ bool micro_eval(D3 wo, D3 wi, double &scalar, double &fres) const {
+  // Microfacet::Evaluate's scalar D*G/(4 cos_i cos_o) and Fresnel term; with DERIV also
+  // d scalar / d exponent, from d D / d e = D / (e + 2) + D ln(cos_h).
+  template <bool DERIV>
+  __device__ bool micro_eval(D3 wo, D3 wi, double &scalar, double &fres, double &dscalar) const {
     double ci = fabs(wo.z), co = fabs(wi.z);
     if (ci < kEps || co < kEps) return false;
     D3 h = add3(wo, wi);
@@ -183,24 +186,48 @@ struct Bsdf {
     double ch = fabs(h.z);
     double tmp = 2.0 * ch / fabs(dot3(wo, h));
     double g = dmin(1.0, dmin(ci * tmp, co * tmp));
-    double d = (e + 2.0) * kInv2Pi * pow(ch, e);
+    double pw = pow(ch, e);
+    double d = (e + 2.0) * kInv2Pi * pw;
     scalar = d * g / (4.0 * ci * co);
+    if (DERIV) {
+      double dd = kInv2Pi * pw * (1.0 + (e + 2.0) * log(ch));
+      dscalar = dd * g / (4.0 * ci * co);
+    }
     return true;
   }
   __device__ double micro_pdf(D3 wo, D3 wi) const { return (wo.z * wi.z < 0.0) ? 0.0 : blinn_pdf(wo, wi); }
   __device__ static double lambert_pdf(D3 wo, D3 wi) { return (wo.z * wi.z > 0.0) ? fabs(wi.z) * kInvPi : 0.0; }
-  __device__ void eval_reflect_local(D3 wo, D3 wi, double f[3]) const {
+  // Sum of the reflection BxDFs for local vectors. With DERIV, dfr = d f[c] / d roughness / ks[c]
+  // (the exponent is 1/roughness; de_dr carries its derivative, 0 where clamped).
+  template <bool DERIV>
+  __device__ void eval_reflect_local(D3 wo, D3 wi, double f[3], double de_dr, double &dfr) const {
     for (int c = 0; c < 3; ++c) f[c] = kd[c] * kInvPi;
+    if (DERIV) dfr = 0.0;
     if (plastic) {
-      double sc, fr;
-      if (micro_eval(wo, wi, sc, fr))
+      double sc, fr, dsc = 0.0;
+      if (micro_eval<DERIV>(wo, wi, sc, fr, dsc)) {
         for (int c = 0; c < 3; ++c) f[c] += (ks[c] * fr) * sc;
+        if (DERIV) dfr = fr * dsc * de_dr;
+      }
+    }
+  }
+  __device__ void eval_reflect_local(D3 wo, D3 wi, double f[3]) const {
+    double unused;
+    eval_reflect_local<false>(wo, wi, f, 0.0, unused);
+  }
+  template <bool DERIV>
+  __device__ void evaluate(D3 wo_w, D3 wi_w, double f[3], double de_dr, double &dfr) const {
+    D3 wo = to_local(wo_w), wi = to_local(wi_w);
+    if (dot3(wo_w, ng) * dot3(wi_w, ng) > 0) {
+      eval_reflect_local<DERIV>(wo, wi, f, de_dr, dfr);
+    } else {
+      f[0] = f[1] = f[2] = 0.0;
+      if (DERIV) dfr = 0.0;
     }
   }
   __device__ void evaluate(D3 wo_w, D3 wi_w, double f[3]) const {
-    D3 wo = to_local(wo_w), wi = to_local(wi_w);
-    if (dot3(wo_w, ng) * dot3(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
-    else f[0] = f[1] = f[2] = 0.0;
+    double unused;
+    evaluate<false>(wo_w, wi_w, f, 0.0, unused);
   }
   __device__ double pdf(D3 wo_w, D3 wi_w) const {
     D3 wo = to_local(wo_w), wi = to_local(wi_w);
@@ -209,7 +236,9 @@ struct Bsdf {
     p += micro_pdf(wo, wi);
     return p / 2.0;
   }
-  __device__ void sample(D3 wo_w, double u0, double u1, double uc, D3 &wi_w, double &pdf_out, double f[3]) const {
+  // BSDF::Sample; wo_l / wi_l return the local vectors f was evaluated with.
+  __device__ void sample(D3 wo_w, double u0, double u1, double uc, D3 &wi_w, double &pdf_out, double f[3], D3 &wo_l,
+                         D3 &wi_l) const {
     const uint64_t num = plastic ? 2 : 1;
     uint64_t comp = (uint64_t)(uc * (double)num);
     if (comp > num - 1) comp = num - 1;
@@ -263,6 +292,8 @@ struct Bsdf {
       p /= (double)num;
     }
     pdf_out = p;
+    wo_l = wo;
+    wi_l = wi;
     if (dot3(wo_w, ng) * dot3(wi_w, ng) > 0) eval_reflect_local(wo, wi, f);
     else f[0] = f[1] = f[2] = 0.0;
   }
@@ -316,6 +347,9 @@ __global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
   put_ray(a.prim_rays + i, D3{a.cam_o[0], a.cam_o[1], a.cam_o[2]}, d, 0.0, __builtin_huge_val());
 }
 
+// SPLIT: evaluation materials differ from sampling materials (replayed estimator).
+// ADJ: also record what the adjoint needs (d f / d roughness, weights, lights).
+template <bool SPLIT, bool ADJ>
 __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < a.batch;
@@ -331,9 +365,11 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
   const uint64_t sidx = pix * a.spp + a.pass;
   D3 o{0, 0, 0}, d{0, 0, 1}, wo{0, 0, 0}, p{0, 0, 0};
   HitGeom g;
-  Bsdf b;
+  Bsdf b, be;
+  double de_dr = 0.0;
   double cdf[kMaxLights];
   double le[3] = {0.0, 0.0, 0.0};
+  int32_t cam_light = -1;
   uint32_t prim = h.prim;
   if (hit) {
     const skw_ray &r = a.prim_rays[i];
@@ -343,8 +379,10 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
     wo = mul3(d, -1.0);
     p = g.p;
     const DevMat &m = a.mats[prim];
-    if (m.light != kMiss && dot3(wo, g.ng) > 0.0)
-      for (int c = 0; c < 3; ++c) le[c] = a.lights[m.light].radiance[c];
+    if (m.light != kMiss && dot3(wo, g.ng) > 0.0) {
+      for (int c = 0; c < 3; ++c) le[c] = a.rad_e[3 * m.light + c];
+      cam_light = (int32_t)m.light;
+    }
     // BSDF frame: DG tangent of a mesh without UVs or tangents (TriangleMesh.cpp:520-540)
     D3 ns = g.ns, t;
     if (fabs(ns.x) > fabs(ns.y)) {
@@ -372,6 +410,16 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
       b.ks[c] = m.ks[c];
     }
     b.e = m.exponent;
+    be = b;
+    if (SPLIT || ADJ) {
+      const DevMat &me = a.mats_e[prim];
+      for (int c = 0; c < 3; ++c) {
+        be.kd[c] = me.kd[c];
+        be.ks[c] = me.ks[c];
+      }
+      be.e = me.exponent;
+      de_dr = me.de_dr;
+    }
     if (nl > 0) {
       D3 nrm = dot3(wo, ns) >= 0.0 ? ns : mul3(ns, -1.0);
       lights_cdf(a, p, nrm, cdf);
@@ -379,8 +427,10 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
     }
     for (int c = 0; c < 3; ++c) a.ns[3 * i + c] = c == 0 ? ns.x : (c == 1 ? ns.y : ns.z);
   }
-  if (in)
+  if (in) {
     for (int c = 0; c < 3; ++c) a.le[3 * i + c] = le[c];
+    if (ADJ) a.cam_light[i] = cam_light;
+  }
   const bool shade = hit && nl > 0;
   const DevPrim *pr = a.geom.prims + (hit ? prim : 0);
   // _SampleLights: one occlusion ray per sample whose unoccluded contribution is not black.
@@ -389,10 +439,12 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
     D3 wi{0, 0, 0};
     double lf[3] = {0, 0, 0};
     double len = 0.0, mint = 0.0;
+    double fe[3] = {0, 0, 0}, w = 0.0, dfr = 0.0;
+    uint32_t l = 0;
     if (shade) {
       double uc = rnd(a.seed, sidx, 2 + k);
       double u0 = rnd(a.seed, sidx, 6 + 2 * k), u1 = rnd(a.seed, sidx, 7 + 2 * k);
-      uint32_t l = search_cdf([&](uint32_t j) { return cdf[j]; }, nl, uc);
+      l = search_cdf([&](uint32_t j) { return cdf[j]; }, nl, uc);
       double comp_pdf = l == 0 ? cdf[0] : cdf[l] - cdf[l - 1];
       const DevLight &L = a.lights[l];
       double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
@@ -414,13 +466,21 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
         double lr[3] = {lit ? L.radiance[0] : 0.0, lit ? L.radiance[1] : 0.0, lit ? L.radiance[2] : 0.0};
         if (lpdf > 0.0 && !(lr[0] == 0.0 && lr[1] == 0.0 && lr[2] == 0.0)) {
           double f[3];
-          b.evaluate(wo, wi, f);
+          if (ADJ) b.evaluate<true>(wo, wi, f, de_dr, dfr);
+          else b.evaluate(wo, wi, f);
           for (int c = 0; c < 3; ++c) lf[c] = lr[c] * f[c];
           if (!(lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0)) {
             mint = next_min_t(*pr, g, wi);
             double bp = b.pdf(wo, wi);
-            double w = power_heuristic((double)a.nls, lpdf * comp_pdf, (double)a.nbs, bp);
+            w = power_heuristic((double)a.nls, lpdf * comp_pdf, (double)a.nbs, bp);
             w *= fabs(dot3(wi, b.n)) / (lpdf * comp_pdf);
+            for (int c = 0; c < 3; ++c) fe[c] = f[c];
+            if (SPLIT) {
+              be.evaluate(wo, wi, fe);
+              for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * fe[c];
+            } else {
+              for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * f[c];
+            }
             for (int c = 0; c < 3; ++c) lf[c] = lf[c] * w;
             want = true;
           }
@@ -435,37 +495,59 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
       c3[0] = lf[0];
       c3[1] = lf[1];
       c3[2] = lf[2];
+      if (ADJ) {
+        double *x = a.shadow_aux + 6 * (i * a.nls + k);
+        x[0] = fe[0];
+        x[1] = fe[1];
+        x[2] = fe[2];
+        x[3] = w;
+        x[4] = dfr;
+        x[5] = (double)l;
+      }
     }
   }
   // _SampleBSDF: one closest-hit ray per sample with a positive pdf and a non-black value.
   for (uint32_t k = 0; k < a.nbs; ++k) {
     bool want = false;
     D3 wi{0, 0, 0};
-    double f[3] = {0, 0, 0}, bp = 0.0, mint = 0.0;
+    double f[3] = {0, 0, 0}, bp = 0.0, mint = 0.0, dfr = 0.0;
     if (shade) {
       double uc = rnd(a.seed, sidx, 14 + k);
       double u0 = rnd(a.seed, sidx, 18 + 2 * k), u1 = rnd(a.seed, sidx, 19 + 2 * k);
-      b.sample(wo, u0, u1, uc, wi, bp, f);
+      D3 wo_l, wi_l;
+      b.sample(wo, u0, u1, uc, wi, bp, f, wo_l, wi_l);
       if (bp > 0.0 && !(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
         mint = next_min_t(*pr, g, wi);
         want = true;
+        // f is non-black, so the sampled pair passed the geometric-normal test: re-evaluate
+        // the same local pair with the evaluation materials.
+        if (SPLIT) be.eval_reflect_local(wo_l, wi_l, f);
+        if (ADJ) be.eval_reflect_local<true>(wo_l, wi_l, f, de_dr, dfr);
       }
     }
     int32_t slot = append_slot(a.counts + 1, want);
     if (in) a.bsdf_idx[i * a.nbs + k] = slot;
     if (want) {
       put_ray(a.bsdf_rays + slot, p, wi, mint, __builtin_huge_val());
-      double *rec = a.bsdf_rec + 6 * (i * a.nbs + k);
+      double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
       rec[0] = f[0];
       rec[1] = f[1];
       rec[2] = f[2];
       rec[3] = bp;
       rec[4] = fabs(dot3(wi, b.n)) / bp;
       rec[5] = mint;
+      if (ADJ) rec[6] = dfr;
     }
   }
 }
 
+// Per-sample sum and film accumulation. ADJ: instead of the film, accumulates the sample's
+// contribution to d_grad: for L = Le + (1/nls) sum_k f_k * le_k * W_k + (1/nbs) sum_k f_k * le_k * W_k
+// with the weights W fixed, dL/d(emission l) = base radiance of l times the terms that use l,
+// dL/d(reflectance g) = (dkd / pi) * le * W summed, dL/d(roughness g) = ks * dfr * le * W summed,
+// where g is the group of the primitive the camera ray hit. The sample's dLoss/dL is
+// adj[pixel] / spp. Sums go to LDS (ds_add_f64), then one global FP64 atomic per parameter per block.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
@@ -473,8 +555,16 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
     a.totals[1] += a.counts[0];
     a.totals[2] += a.counts[1];
   }
-  if (i >= a.batch) return;
-  const skw_hit h = a.prim_hits[i];
+  __shared__ double gsum[ADJ ? kDlMaxParams : 1];
+  const uint32_t G = a.n_groups, n_params = 2 * a.n_groups + a.n_lights;
+  if (ADJ) {
+    for (uint32_t t = threadIdx.x; t < n_params; t += blockDim.x) gsum[t] = 0.0;
+    __syncthreads();
+  }
+  const bool in = i < a.batch;
+  skw_hit h;
+  h.prim = kMiss;
+  if (in) h = a.prim_hits[i];
   double L[3] = {0.0, 0.0, 0.0};
   if (h.prim != kMiss) {
     for (int c = 0; c < 3; ++c) L[c] = a.le[3 * i + c];
@@ -493,6 +583,8 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
       for (uint32_t k = 0; k < a.nbs; ++k) {
         int32_t j = a.bsdf_idx[i * a.nbs + k];
         if (j < 0) continue;
+        double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
+        if (ADJ) rec[7] = -1.0;  // no light reached
         const skw_hit h2 = a.bsdf_hits[j];
         if (h2.prim == kMiss) continue;
         const DevMat &m2 = a.mats[h2.prim];
@@ -505,7 +597,6 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
         const DevLight &Lt = a.lights[m2.light];
         if (Lt.radiance[0] == 0.0 && Lt.radiance[1] == 0.0 && Lt.radiance[2] == 0.0) continue;
         const DevLightTri &T = a.ltris[Lt.tri_off + h2.tri];
-        const double *rec = a.bsdf_rec + 6 * (i * a.nbs + k);
         double dist = h2.t - rec[5];
         double ac = fabs(dot3(wi, D3{T.n[0], T.n[1], T.n[2]}));
         double lpdf = ac > kEps ? (dist * dist) / (Lt.area * ac) : 0.0;
@@ -513,7 +604,9 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
         double cp = m2.light == 0 ? cdf[0] : cdf[m2.light] - cdf[m2.light - 1];
         double w = power_heuristic((double)a.nbs, rec[3], (double)a.nls, lpdf * cp);
         w *= rec[4];
-        for (int c = 0; c < 3; ++c) db[c] += rec[c] * Lt.radiance[c] * w;
+        const double *re = a.rad_e + 3 * m2.light;
+        for (int c = 0; c < 3; ++c) db[c] += rec[c] * re[c] * w;
+        if (ADJ) rec[7] = w;
       }
       for (int c = 0; c < 3; ++c) cdl[c] += db[c] * (1.0 / (double)a.nbs);
     }
@@ -522,9 +615,73 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
   double lum = 0.0;
   for (int c = 0; c < 3; ++c) lum += kLumW[c] * L[c];
   bool bad = isnan(L[0]) || isnan(L[1]) || isnan(L[2]) || lum < -kEps || isinf(L[0]) || isinf(L[1]) || isinf(L[2]);
-  double *f = a.film + 3 * (a.pix0 + i);
-  if (!bad)
-    for (int c = 0; c < 3; ++c) f[c] += L[c];
+  if (!ADJ) {
+    if (in && !bad) {
+      double *f = a.film + 3 * (a.pix0 + i);
+      for (int c = 0; c < 3; ++c) f[c] += L[c];
+    }
+    return;
+  }
+  if (in && h.prim != kMiss && !bad) {
+    const uint64_t pix = a.pix0 + i;
+    const double inv_spp = 1.0 / (double)a.spp;
+    double wa[3];
+    for (int c = 0; c < 3; ++c) wa[c] = a.adj[3 * pix + c] * inv_spp;
+    const DevMat &me = a.mats_e[h.prim];
+    double g_refl = 0.0, g_rough = 0.0;
+    const int32_t cl = a.cam_light[i];
+    if (cl >= 0) {
+      const double *rb = a.rad_base + 3 * cl;
+      atomicAdd(&gsum[2 * G + cl], wa[0] * rb[0] + wa[1] * rb[1] + wa[2] * rb[2]);
+    }
+    if (a.n_lights > 0) {
+      const double inv_nls = 1.0 / (double)a.nls, inv_nbs = 1.0 / (double)a.nbs;
+      for (uint32_t k = 0; k < a.nls; ++k) {
+        int32_t j = a.shadow_idx[i * a.nls + k];
+        if (j < 0 || a.occluded[j]) continue;
+        const double *x = a.shadow_aux + 6 * (i * a.nls + k);
+        const uint32_t l = (uint32_t)x[5];
+        const double W = x[3] * inv_nls;
+        const double *re = a.rad_e + 3 * l, *rb = a.rad_base + 3 * l;
+        double ge = 0.0, gr = 0.0, gs = 0.0;
+        for (int c = 0; c < 3; ++c) {
+          ge += wa[c] * x[c] * rb[c];
+          gr += wa[c] * re[c];
+          gs += wa[c] * re[c] * me.ks[c];
+        }
+        atomicAdd(&gsum[2 * G + l], ge * W);
+        g_refl += gr * W;
+        g_rough += gs * x[4] * W;
+      }
+      for (uint32_t k = 0; k < a.nbs; ++k) {
+        int32_t j = a.bsdf_idx[i * a.nbs + k];
+        if (j < 0) continue;
+        const double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
+        if (rec[7] < 0.0) continue;
+        const uint32_t l = a.mats[a.bsdf_hits[j].prim].light;
+        const double W = rec[7] * inv_nbs;
+        const double *re = a.rad_e + 3 * l, *rb = a.rad_base + 3 * l;
+        double ge = 0.0, gr = 0.0, gs = 0.0;
+        for (int c = 0; c < 3; ++c) {
+          ge += wa[c] * rec[c] * rb[c];
+          gr += wa[c] * re[c];
+          gs += wa[c] * re[c] * me.ks[c];
+        }
+        atomicAdd(&gsum[2 * G + l], ge * W);
+        g_refl += gr * W;
+        g_rough += gs * rec[6] * W;
+      }
+    }
+    if (me.group != kMiss) {
+      atomicAdd(&gsum[me.group], g_refl * me.dkd * kInvPi);
+      atomicAdd(&gsum[G + me.group], g_rough);
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n_params; t += blockDim.x) {
+    const double v = gsum[t];
+    if (v != 0.0) unsafeAtomicAdd(a.grad + t, v);
+  }
 }
 
 // Clears the film and the ray totals (a kernel rather than a memset, so a captured frame replays
@@ -541,6 +698,67 @@ __global__ __launch_bounds__(256) void k_dl_finish(DlArgs a, float *rgb) {
   if (i >= a.batch) return;
   const uint64_t pix = a.pix0 + i;
   for (int c = 0; c < 3; ++c) rgb[3 * pix + c] = (float)(a.film[3 * pix + c] / (double)a.spp);
+}
+
+__global__ __launch_bounds__(256) void k_dl_finish_f64(DlArgs a, double *img) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.batch) return;
+  const uint64_t pix = a.pix0 + i;
+  for (int c = 0; c < 3; ++c) img[3 * pix + c] = a.film[3 * pix + c] / (double)a.spp;
+}
+
+__global__ __launch_bounds__(256) void k_dl_clear_grad(double *grad, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) grad[i] = 0.0;
+}
+
+// Material of a grouped primitive at one parameter vector: gray kd = clamp(theta[g], 0, 1) and
+// Blinn exponent 1/clamp(theta[G+g], 0.001, 1) clamped to [0, 1000], as the scene's own materials
+// (MatteMaterial.cpp:33-34, PlasticMaterial.cpp:37-45); derivatives are 0 where a clamp holds.
+__device__ void param_material(const DevMat &base, const double *theta, uint32_t G, DevMat &o) {
+  o = base;
+  if (base.group == kMiss) return;
+  const double rho = theta[base.group];
+  const double kd = rho < 0.0 ? 0.0 : (rho > 1.0 ? 1.0 : rho);
+  for (int c = 0; c < 3; ++c) o.kd[c] = kd;
+  o.dkd = (rho > 0.0 && rho < 1.0) ? 1.0 : 0.0;
+  const double r = theta[G + base.group];
+  const double rc = r < 0.001 ? 0.001 : (r > 1.0 ? 1.0 : r);
+  const double e = 1.0 / rc;
+  o.exponent = e < 0.0 ? 0.0 : (e > 1000.0 ? 1000.0 : e);
+  o.de_dr = (r > 0.001 && r < 1.0) ? -1.0 / (r * r) : 0.0;
+}
+
+// Sampling and evaluation tables of one call. Light intensity as the scene's light table has it:
+// luminance of (pi * area) * radiance over pi times the light's bounding-box area
+// (IrradianceLightsSamplingStrategy.cpp:19-31).
+__global__ __launch_bounds__(256) void k_dl_params(DlParamArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n_prims) {
+    const DevMat base = a.base_mats[i];
+    DevMat m;
+    param_material(base, a.theta_s, a.n_groups, m);
+    a.mats_s[i] = m;
+    if (a.mats_e != a.mats_s) {
+      param_material(base, a.theta_e, a.n_groups, m);
+      a.mats_e[i] = m;
+    }
+  }
+  if (i < a.n_lights) {
+    DevLight L = a.base_lights[i];
+    const double ss = a.theta_s[2 * a.n_groups + i], se = a.theta_e[2 * a.n_groups + i];
+    for (int c = 0; c < 3; ++c) {
+      a.rad_base[3 * i + c] = L.radiance[c];
+      a.rad_e[3 * i + c] = se * L.radiance[c];
+      L.radiance[c] = ss * L.radiance[c];
+    }
+    double x = fabs(L.bmax[0] - L.bmin[0]), y = fabs(L.bmax[1] - L.bmin[1]), z = fabs(L.bmax[2] - L.bmin[2]);
+    double bbox_area = 2.0 * (x * y + x * z + y * z);
+    double lum = 0.0;
+    for (int c = 0; c < 3; ++c) lum += kLumW[c] * ((kPi * L.area) * L.radiance[c]);
+    L.intensity = lum / (kPi * bbox_area);
+    a.lights_s[i] = L;
+  }
 }
 
 }  // namespace
@@ -561,13 +779,36 @@ hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream) {
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 63) / 64);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dl_shade, dim3(grid), dim3(64), 0, stream, a);
+  if (a.adj) hipLaunchKernelGGL((k_dl_shade<false, true>), dim3(grid), dim3(64), 0, stream, a);
+  else if (a.mats_e != a.mats) hipLaunchKernelGGL((k_dl_shade<true, false>), dim3(grid), dim3(64), 0, stream, a);
+  else hipLaunchKernelGGL((k_dl_shade<false, false>), dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 255) / 256);
-  hipLaunchKernelGGL(k_dl_resolve, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  if (a.adj) hipLaunchKernelGGL((k_dl_resolve<true>), dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  else hipLaunchKernelGGL((k_dl_resolve<false>), dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_finish_f64(const DlArgs &a, double *img, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_finish_f64, dim3(grid), dim3(256), 0, stream, a, img);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_clear_grad(double *grad, uint32_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_dl_clear_grad, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, stream, grad, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_params(const DlParamArgs &a, hipStream_t stream) {
+  uint32_t n = a.n_prims > a.n_lights ? a.n_prims : a.n_lights;
+  uint32_t grid = (n + 255) / 256;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_params, dim3(grid), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -84,6 +84,12 @@ struct DevMat {
   double exponent;  // Blinn exponent of a plastic material
   uint32_t type;    // 0 matte, 1 plastic
   uint32_t light;   // area light index of the primitive, 0xFFFFFFFF if none
+  // parameter view (differentiable render): group, d kd / d reflectance (0 where clamped),
+  // d exponent / d roughness (0 where clamped)
+  uint32_t group;   // 0xFFFFFFFF if none
+  uint32_t pad;
+  double dkd;
+  double de_dr;
 };
 
 struct DevLight {
@@ -102,10 +108,14 @@ struct DevLightTri {
 
 struct DlArgs {
   GeomArgs geom;
-  const DevMat *mats;
-  const DevLight *lights;
+  const DevMat *mats;        // sampling materials: BSDF sampling, pdfs, control flow
+  const DevLight *lights;    // sampling lights: CDF intensities, radiance tests
   const DevLightTri *ltris;
+  const DevMat *mats_e;      // evaluation materials: BSDF values
+  const double *rad_e;       // 3 per light: evaluation radiance
+  const double *rad_base;    // 3 per light: radiance at emission scale 1 (adjoint)
   uint32_t n_lights;
+  uint32_t n_groups;
   double cam_o[3], cam_r[9], x_tan, y_tan;
   uint32_t width, height, spp, nls, nbs;
   uint64_t seed;
@@ -122,18 +132,42 @@ struct DlArgs {
   double *shadow_c;      // 3 per (sample, k): unoccluded contribution
   skw_ray *bsdf_rays;
   int32_t *bsdf_idx;     // nbs per sample
-  double *bsdf_rec;      // 6 per (sample, k): f rgb, pdf, |wi.ns|/pdf, min_t
+  double *bsdf_rec;      // 8 per (sample, k): f rgb, pdf, |wi.ns|/pdf, min_t, d f / d roughness, unused
   uint64_t *counts;      // [0] shadow rays, [1] BSDF rays of this pass (device)
   uint8_t *occluded;
   skw_hit *bsdf_hits;
   double *film;          // 3 per pixel, running sum over passes
   uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device)
+  // adjoint (when adj is set): dLoss/dImage per pixel, gradient output, per-sample records
+  const double *adj;     // 3 per pixel
+  double *grad;          // 2 * n_groups + n_lights
+  double *shadow_aux;    // 6 per (sample, k): f rgb (evaluation), weight, d f / d roughness, light
+  int32_t *cam_light;    // per sample: light seen by the camera ray, or -1
 };
+constexpr uint32_t kDlMaxGroups = 64;
+constexpr uint32_t kDlMaxParams = 2 * kDlMaxGroups + 16;
+
 hipError_t launch_dl_clear(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream);
+hipError_t launch_dl_finish_f64(const DlArgs &a, double *img, hipStream_t stream);
+hipError_t launch_dl_clear_grad(double *grad, uint32_t n, hipStream_t stream);
+
+// Materials and lights of one parameter vector, built on the device from the scene's base tables
+// (the vectors travel as kernel arguments, so no host buffer has to outlive the call).
+struct DlParamArgs {
+  const DevMat *base_mats;
+  const DevLight *base_lights;
+  uint32_t n_prims, n_lights, n_groups;
+  DevMat *mats_s, *mats_e;
+  DevLight *lights_s;
+  double *rad_e, *rad_base;
+  double theta_s[kDlMaxParams];
+  double theta_e[kDlMaxParams];
+};
+hipError_t launch_dl_params(const DlParamArgs &a, hipStream_t stream);
 
 // ------------------------------------------------------------------ batched scenes
 struct DevBatchScene {

@@ -18,6 +18,8 @@ types: float32 vertices, float64 rays, float32 photon powers, 16-bit CompressedD
 """
 import numpy as np
 
+NO_GROUP = 0xFFFFFFFF
+
 # ---------------------------------------------------------------- meshes
 
 
@@ -343,25 +345,40 @@ def scene_room(seed=2, n_spheres=200, subdiv_counts=(70, 111, 19), subdiv_levels
         meshes.append(quad((x - 0.5, y, z - 0.5), (x + 0.5, y, z - 0.5), (x + 0.5, y, z + 0.5), (x - 0.5, y, z + 0.5)))
     n_prims = len(meshes)
     group_refl = rng.uniform(0.1, 0.9, 64)
+    group_refl[0] = 0.5  # group 0 holds the room
     group_rough = rng.uniform(0.01, 0.3, 64)
     mat = np.zeros(n_prims, np.uint8)
     kd = np.zeros((n_prims, 3))
     ks = np.zeros((n_prims, 3))
     rough = np.zeros(n_prims)
+    prim_group = np.full(n_prims, NO_GROUP, np.uint32)
     for p in range(n_prims - 4):
         g = p % 64
+        prim_group[p] = g
         kd[p] = group_refl[g]
         rough[p] = group_rough[g]
         if 1 <= p <= n_spheres and (p - 1) % 2 == 1:
             mat[p] = 1
             ks[p] = 0.04
-    kd[0] = 0.5
     rough[n_prims - 4:] = 0.1
     light_prim = np.arange(n_prims - 4, n_prims, dtype=np.uint32)
     light_rad = light_radiance * rng.uniform(0.8, 1.2, (4, 3))
+    theta0 = np.concatenate([group_refl, group_rough, np.ones(4)])
     return {"meshes": meshes, "mat": mat, "kd": kd, "ks": ks, "rough": rough, "light_prim": light_prim,
             "light_radiance": light_rad, "group_refl": group_refl, "group_rough": group_rough,
+            "prim_group": prim_group, "n_groups": 64, "theta0": theta0,
             "camera": ((0.0, 0.0, -room + 0.1), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0)}
+
+
+def perturbed_theta(theta0, n_groups=64, seed=4):
+    """Target parameters of the material-fitting workload: reflectance + N(0, 0.05), roughness +
+    N(0, 0.02), emission scales x 1.2."""
+    rng = np.random.default_rng(seed)
+    t = np.array(theta0, np.float64)
+    t[:n_groups] += rng.normal(0.0, 0.05, n_groups)
+    t[n_groups:2 * n_groups] += rng.normal(0.0, 0.02, n_groups)
+    t[2 * n_groups:] *= 1.2
+    return t
 
 
 def scene_room_small(seed=5):

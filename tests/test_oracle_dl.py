@@ -78,3 +78,71 @@ def test_plastic_floor_estimators_agree():
     a, _ = dls.render(cam15, 1, 1, 8192, 4, 4, seed=6)
     b, _ = dls.render(cam15, 1, 1, 8192, 1, 1, seed=7)
     assert np.allclose(a, b, rtol=3e-2)
+
+
+# ---------------------------------------------------------------- differentiable render
+def _room(n=16):
+    s = wl.scene_room_small()
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"],
+                         prim_group=s["prim_group"], n_groups=s["n_groups"])
+    _, cam15 = sk.camera(*s["camera"], n, n)
+    return s, dls, cam15
+
+
+def test_replay_at_equal_parameters_is_the_render():
+    s, dls, cam15 = _room()
+    rgb, _ = dls.render(cam15, 16, 16, 2, 2, 2, seed=9)
+    img = dls.render_replay(s["theta0"], s["theta0"], cam15, 16, 16, 2, 2, 2, seed=9)
+    assert np.array_equal(img.astype(np.float32), rgb)
+    assert dls.n_params == 132
+
+
+def test_gradient_is_linear_in_emission_and_reflectance():
+    # F is linear in the emission scales jointly (each term carries one light's radiance) and
+    # affine in a group's reflectance inside (0, 1): the derivative is the one-light image, and
+    # equals the secant between two reflectances.
+    s, dls, cam15 = _room()
+    th = s["theta0"]
+    rng = np.random.default_rng(1)
+    adj = rng.normal(size=(256, 3))
+    g, img = dls.gradient(th, adj, cam15, 16, 16, 2, 2, 2, seed=9)
+    assert np.array_equal(img, dls.render_replay(th, th, cam15, 16, 16, 2, 2, 2, seed=9))
+    for l in range(4):
+        te = th.copy()
+        te[128:] = 0.0
+        te[128 + l] = 1.0
+        fl = dls.render_replay(te, th, cam15, 16, 16, 2, 2, 2, seed=9)
+        assert np.isclose(g[128 + l], np.sum(adj * fl), rtol=1e-10, atol=1e-12)
+    for grp in (0, 1, 2, 5):
+        lo, hi = th.copy(), th.copy()
+        lo[grp], hi[grp] = 0.2, 0.7
+        f_lo = dls.render_replay(lo, th, cam15, 16, 16, 2, 2, 2, seed=9)
+        f_hi = dls.render_replay(hi, th, cam15, 16, 16, 2, 2, 2, seed=9)
+        secant = np.sum(adj * (f_hi - f_lo)) / 0.5
+        assert np.isclose(g[grp], secant, rtol=1e-8, atol=1e-10 * max(1.0, abs(secant)))
+
+
+def test_gradient_matches_central_differences():
+    s, dls, cam15 = _room(12)
+    th = s["theta0"]
+    rng = np.random.default_rng(2)
+    target = rng.uniform(0.0, 0.5, (144, 3))
+    img0 = dls.render_replay(th, th, cam15, 12, 12, 2, 2, 2, seed=5)
+    g, _ = dls.gradient(th, 2.0 * (img0 - target), cam15, 12, 12, 2, 2, 2, seed=5)
+
+    def loss(te):
+        return np.sum((dls.render_replay(te, th, cam15, 12, 12, 2, 2, 2, seed=5) - target) ** 2)
+
+    h = 1e-6
+    fd = np.zeros_like(g)
+    for j in range(len(th)):
+        tp, tm = th.copy(), th.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (loss(tp) - loss(tm)) / (2 * h)
+    scale = np.max(np.abs(fd))
+    assert scale > 0
+    assert np.all(np.abs(g - fd) <= 1e-6 * np.abs(fd) + 1e-7 * scale), np.max(np.abs(g - fd))
+    # plastic groups (odd spheres) have roughness gradients; the room's group has a reflectance one
+    assert np.count_nonzero(g[64:128]) > 0 and g[0] != 0.0
