@@ -143,6 +143,22 @@ skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_poin
                                  float *d_e_int, float *d_radiance, uint32_t *d_ids, double *d_d2,
                                  uint32_t *d_count, void *stream);
 
+/* PhotonLTEIntegrator::_LookupCausticRadiance + _PhotonKernel (PhotonLTEIntegrator.cpp:579-636) for a
+ * Lambertian surface of albedo rho seen from direction d_wo (3 doubles per query, unit): the
+ * PhotonFilter'd k-nearest set weighted by Simpson's kernel with the running maximum of d^2 in the
+ * set's array order, / (paths * max d^2) with the n/(n-0.5) rule. d_radiance: 3 doubles per query. */
+skw_status skw_photon_caustic_radiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                       const double *d_wo, uint64_t nq, uint32_t k, double r, uint64_t paths,
+                                       double rho, double *d_radiance, void *stream);
+
+/* Cone-filtered irradiance over the same PhotonFilter'd k-nearest set (Jensen's cone filter,
+ * weight 1 - d/(kc * r_max), normalised by 1 - 2/(3 kc); photons from the normal's side only).
+ * No counterpart in the reference, whose estimators are the disc (_LookupPhotonIrradiance) and
+ * the Simpson kernel (_LookupCausticRadiance). d_e_ext: 3 floats per query. */
+skw_status skw_photon_cone_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                      uint64_t nq, uint32_t k, double r, uint64_t paths, double kc, float *d_e_ext,
+                                      void *stream);
+
 /* KDTree::GetNearestPoints with or without PhotonFilter (filter = 0 none, 1 PhotonFilter at the
  * query normal). d_normals may be NULL when filter = 0. */
 skw_status skw_photon_knn(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,

@@ -27,7 +27,8 @@ EXPORTS = [
     "skw_scene_create", "skw_scene_destroy", "skw_scene_bounds", "skw_scene_info",
     "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
-    "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_last_error",
+    "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
+    "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
     "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
     "skw_batch_create", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
@@ -103,6 +104,10 @@ def lib():
         "skw_photon_knn": ([_P, _P, _P, _U64, _U32, ctypes.c_double, ctypes.c_int, _P, _P, _P, _P],
                            ctypes.c_int),
         "skw_photon_nearest": ([_P, _P, _P, _U64, ctypes.c_double, ctypes.c_int, _P, _P], ctypes.c_int),
+        "skw_photon_cone_irradiance": ([_P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double, _P, _P],
+                                       ctypes.c_int),
+        "skw_photon_caustic_radiance": ([_P, _P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double, _P,
+                                         _P], ctypes.c_int),
         "skw_last_error": ([], ctypes.c_char_p),
         "skw_device_count": ([], ctypes.c_int),
         "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
@@ -300,6 +305,14 @@ class PhotonMap:
     def knn(self, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream=0):
         check(lib().skw_photon_knn(self.handle, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream or None),
               "skw_photon_knn")
+
+    def cone_irradiance(self, d_p, d_n, nq, k, r, paths, kc, d_e, stream=0):
+        check(lib().skw_photon_cone_irradiance(self.handle, d_p, d_n, nq, k, r, paths, kc, d_e, stream or None),
+              "skw_photon_cone_irradiance")
+
+    def caustic_radiance(self, d_p, d_n, d_wo, nq, k, r, paths, rho, d_rad, stream=0):
+        check(lib().skw_photon_caustic_radiance(self.handle, d_p, d_n, d_wo, nq, k, r, paths, rho, d_rad,
+                                                stream or None), "skw_photon_caustic_radiance")
 
     def nearest(self, d_p, d_n, nq, r, filt, d_id, stream=0):
         check(lib().skw_photon_nearest(self.handle, d_p, d_n, nq, r, filt, d_id, stream or None),

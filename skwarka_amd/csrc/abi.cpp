@@ -324,10 +324,18 @@ skw_status skw_photon_map_export(const skw_photon_map *map, uint32_t *perm, floa
   return SKW_OK;
 }
 
+// Estimator choice of a k-nearest gather (GatherArgs::estimate): 0 disc, 1 cone, 2 caustic.
+struct Estimate {
+  int kind = 0;
+  double kc = 0.0;
+  const double *wo = nullptr;
+  double *caustic = nullptr;
+};
+
 static skw_status gather(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                          uint32_t k, double r, uint64_t paths, double rho, int filter, int mode, float *e_ext,
                          float *e_int, float *radiance, uint32_t *ids, double *d2, uint32_t *count,
-                         uint32_t *nearest, void *stream) {
+                         uint32_t *nearest, void *stream, const Estimate &est = Estimate()) {
   if (!map) return fail(SKW_INVALID_ARG, "map must not be NULL");
   if (!map->on_device) return fail(SKW_UNSUPPORTED, "photon map was created host-only");
   if (!(r >= 0.0)) return fail(SKW_INVALID_ARG, "radius must be >= 0");
@@ -364,6 +372,10 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.nearest = nearest;
   a.stack_need = map->host.max_depth + 1;
   a.gstack = nullptr;
+  a.estimate = est.kind;
+  a.kc = est.kc;
+  a.wo = est.wo;
+  a.caustic = est.caustic;
   if (mode == 0 && k > 0) {
     size_t elems = (size_t)nq * k;
     SKW_HIP(hipMallocAsync((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
@@ -397,6 +409,33 @@ skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_poin
   if (nq > 0 && !d_normals) return fail(SKW_INVALID_ARG, "normals must not be NULL");
   return gather(map, d_points, d_normals, nq, k, r, paths, rho, 1, 0, d_e_ext, d_e_int, d_radiance, d_ids, d_d2,
                 d_count, nullptr, stream);
+}
+
+skw_status skw_photon_cone_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                      uint64_t nq, uint32_t k, double r, uint64_t paths, double kc, float *d_e_ext,
+                                      void *stream) {
+  if (nq > 0 && (!d_e_ext || !d_normals)) return fail(SKW_INVALID_ARG, "normals and e_ext must not be NULL");
+  if (!(kc >= 1.0)) return fail(SKW_INVALID_ARG, "cone constant kc must be >= 1");
+  if (k == 0) return fail(SKW_INVALID_ARG, "k must be > 0");
+  Estimate est;
+  est.kind = 1;
+  est.kc = kc;
+  return gather(map, d_points, d_normals, nq, k, r, paths, 0.0, 1, 0, d_e_ext, nullptr, nullptr, nullptr, nullptr,
+                nullptr, nullptr, stream, est);
+}
+
+skw_status skw_photon_caustic_radiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                       const double *d_wo, uint64_t nq, uint32_t k, double r, uint64_t paths,
+                                       double rho, double *d_radiance, void *stream) {
+  if (nq > 0 && (!d_radiance || !d_normals || !d_wo))
+    return fail(SKW_INVALID_ARG, "normals, wo and radiance must not be NULL");
+  if (k == 0) return fail(SKW_INVALID_ARG, "k must be > 0");
+  Estimate est;
+  est.kind = 2;
+  est.wo = d_wo;
+  est.caustic = d_radiance;
+  return gather(map, d_points, d_normals, nq, k, r, paths, rho, 1, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, nullptr, stream, est);
 }
 
 skw_status skw_photon_knn(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,

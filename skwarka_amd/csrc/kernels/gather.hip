@@ -201,6 +201,76 @@ struct GlobalStack {
   __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { base[i * stride] = v; }
 };
 
+// Cone-filtered irradiance over the set (Jensen's cone filter; not an estimator of the
+// reference): photon weight 1 - d / (kc * r_max), only photons arriving from the normal's side,
+// normalised by (1 - 2 / (3 kc)) * pi * paths * r_max^2, with r_max^2 = the set's largest d^2, or
+// r^2 when the set is not full.
+__device__ void cone_estimate(const GatherArgs &a, const Query &q, uint64_t i, const Heap &heap, uint32_t size) {
+  float *out = a.e_ext + 3 * i;
+  if (size == 0) {
+    out[0] = out[1] = out[2] = 0.0f;
+    return;
+  }
+  double mx = 0.0;
+  for (uint32_t j = 0; j < size; ++j) mx = (mx < heap.d(j)) ? heap.d(j) : mx;
+  const double rmax2 = (size < a.k || mx == 0.0) ? a.r * a.r : mx;
+  const double rmax = sqrt(rmax2);
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (uint32_t j = 0; j < size; ++j) {
+    float4 ph = reinterpret_cast<const float4 *>(a.photons)[heap.p(j)];
+    const double *dv = a.cdir + 3 * (size_t)(__float_as_uint(ph.w) & 0xFFFFu);
+    if (!(dv[0] * q.n0 + dv[1] * q.n1 + dv[2] * q.n2 > 0.0)) continue;
+    const double wgt = 1.0 - sqrt(heap.d(j)) / (a.kc * rmax);
+    e0 += wgt * (double)ph.x;
+    e1 += wgt * (double)ph.y;
+    e2 += wgt * (double)ph.z;
+  }
+  const double norm = (1.0 - 2.0 / (3.0 * a.kc)) * kPi * a.paths * rmax2;
+  out[0] = (float)(e0 / norm);
+  out[1] = (float)(e1 / norm);
+  out[2] = (float)(e2 / norm);
+}
+
+// PhotonLTEIntegrator::_LookupCausticRadiance / _PhotonKernel (PhotonLTEIntegrator.cpp:579-636) for
+// a Lambertian surface of albedo rho: Simpson kernel 3/pi (1 - d^2 / m)^2 with m the running
+// maximum of d^2 in the set's array order, BSDF value rho/pi when photon direction and wo lie on
+// the same side of the normal; then / (paths * max_d2) with the disc estimate's n / (n - 0.5) rule.
+__device__ void caustic_estimate(const GatherArgs &a, const Query &q, uint64_t i, const Heap &heap, uint32_t size) {
+  double *out = a.caustic + 3 * i;
+  if (size == 0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  const double f = a.rho * kInvPi;
+  const double *wv = a.wo + 3 * i;
+  const double wn = wv[0] * q.n0 + wv[1] * q.n1 + wv[2] * q.n2;
+  double mx = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  for (uint32_t j = 0; j < size; ++j) {
+    const uint32_t pos = heap.p(j);
+    float4 nd = reinterpret_cast<const float4 *>(a.nodes)[pos];
+    double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
+    double t = dx * dx + dy * dy + dz * dz;
+    if (t > mx) mx = t;
+    double tmp = 1.0 - t / mx;
+    double kernel = 3.0 * kInvPi * tmp * tmp;
+    float4 ph = reinterpret_cast<const float4 *>(a.photons)[pos];
+    const double *dv = a.cdir + 3 * (size_t)(__float_as_uint(ph.w) & 0xFFFFu);
+    double pn = dv[0] * q.n0 + dv[1] * q.n1 + dv[2] * q.n2;
+    double fv = (pn * wn > 0) ? f : 0.0;
+    r0 += fv * (double)ph.x * kernel;
+    r1 += fv * (double)ph.y * kernel;
+    r2 += fv * (double)ph.z * kernel;
+  }
+  if (size < a.k || mx == 0.0)
+    mx = a.r * a.r;
+  else
+    mx *= (double)size / ((double)size - 0.5);
+  const double inv = 1.0 / (a.paths * mx);
+  out[0] = r0 * inv;
+  out[1] = r1 * inv;
+  out[2] = r2 * inv;
+}
+
 template <int MODE, class Stack>
 __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
   Query q;
@@ -234,6 +304,14 @@ __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
       if (a.ids) a.ids[i * a.k + j] = a.perm[pos];
       if (a.d2) a.d2[i * a.k + j] = heap.d(j);
     }
+  }
+  if (a.estimate == 1) {
+    cone_estimate(a, q, i, heap, size);
+    return;
+  }
+  if (a.estimate == 2) {
+    caustic_estimate(a, q, i, heap, size);
+    return;
   }
   if (!a.e_ext) return;
   // PhotonLTEIntegrator.cpp:139-168

@@ -73,6 +73,12 @@ struct GatherArgs {
   uint32_t *nearest;    // nearest mode
   uint32_t stack_need;
   uint32_t *gstack;
+  // estimate over the k-nearest set: 0 disc (e_ext / e_int / radiance), 1 cone filter with
+  // constant kc (e_ext), 2 Simpson-kernel caustic radiance toward wo (caustic, 3 doubles)
+  int estimate;
+  double kc;
+  const double *wo;
+  double *caustic;
 };
 // mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);

@@ -119,3 +119,36 @@ def test_duplicates_small_and_empty_maps(torch_cuda):
         ree, rei, rids, rd2, rc = om1.irradiance(q[:100], qn[:100], 4, 2.0)
         check_sets(cnt, ids, d2, rc, rids, rd2)
         assert np.array_equal(ee, ree)
+
+
+def test_cone_and_caustic_estimates_match_oracle(torch_cuda, c1_data):
+    print("case: cone-filtered irradiance and caustic radiance, 1M photons", flush=True)
+    ph, om, p, n = c1_data
+    pm = sk.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
+    nq = len(p)
+    dp, dn = to_dev(torch_cuda, p), to_dev(torch_cuda, n)
+    e = empty_dev(torch_cuda, nq * 12)
+    for k, r, kc in ((100, 0.05, 1.0), (16, 0.2, 1.5)):
+        pm.cone_irradiance(dp.data_ptr(), dn.data_ptr(), nq, k, r, 2, kc, e.data_ptr())
+        torch_cuda.cuda.synchronize()
+        got = from_dev(e, np.float32, nq * 3).reshape(nq, 3)
+        ref = om.cone(p, n, k, r, 2, kc)
+        # float outputs of double sums over the same (bit-exact) sets; the weights take a sqrt,
+        # so the bar is the floating-point one: 1e-6 relative
+        assert np.all(np.abs(got - ref) <= 1e-6 * np.abs(ref)), np.abs(got - ref).max()
+        assert ref.max() > 0
+    # caustic radiance toward a direction on the normal's side for half the points, the other side
+    # for the rest
+    rng = np.random.default_rng(4)
+    wo = n + rng.normal(0, 0.3, n.shape)
+    wo /= np.linalg.norm(wo, axis=1)[:, None]
+    wo[1::2] *= -1.0
+    dwo = to_dev(torch_cuda, wo)
+    rad = empty_dev(torch_cuda, nq * 24)
+    for k, r in ((100, 0.05), (5, 0.1)):
+        pm.caustic_radiance(dp.data_ptr(), dn.data_ptr(), dwo.data_ptr(), nq, k, r, 3, 0.5, rad.data_ptr())
+        torch_cuda.cuda.synchronize()
+        got = from_dev(rad, np.float64, nq * 3).reshape(nq, 3)
+        ref = om.caustic(p, n, wo, k, r, 3, 0.5)
+        assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), np.abs(got - ref).max()
+        assert ref.max() > 0

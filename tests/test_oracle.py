@@ -356,3 +356,43 @@ def test_photon_irradiance_closed_form():
             if c:
                 assert np.allclose(e_ext[i], (ext * inv).astype(np.float32), rtol=1e-6, atol=0)
                 assert np.allclose(e_int[i], (itn * inv).astype(np.float32), rtol=1e-6, atol=0)
+
+
+def _uniform_plane_photons(n=200000, seed=8):
+    # photons uniformly on the square [-1,1]^2 of the plane z = 0, normal +z, arriving from +z,
+    # equal weights: irradiance E = n * w / 4 everywhere away from the edges
+    rng = np.random.default_rng(seed)
+    pos = np.c_[rng.uniform(-1, 1, (n, 2)), np.zeros(n)].astype(np.float32)
+    up = wl.cdir_encode(np.array([[0.0, 0.0, 1.0]]))[0]
+    codes = np.full(n, up, np.uint16)
+    w = np.full((n, 3), 2e-6, np.float32)
+    q = np.c_[rng.uniform(-0.7, 0.7, (600, 2)), np.zeros(600)]
+    qn = np.tile([0.0, 0.0, 1.0], (600, 1))
+    e = n * 2e-6 / 4.0
+    return oracle.PhotonMap(pos, w, codes, codes), q, qn, e
+
+
+def test_cone_filter_is_unbiased_on_uniform_density():
+    # Jensen's cone filter normalised by 1 - 2/(3 kc) integrates to one over the disc, so a uniform
+    # photon density gives back its irradiance (the cone variant has no reference counterpart).
+    pm, q, qn, e = _uniform_plane_photons()
+    disc = pm.irradiance(q, qn, 100, 0.2, with_sets=False)[0]
+    for kc in (1.0, 1.5):
+        cone = pm.cone(q, qn, 100, 0.2, 1, kc)
+        assert abs(cone.mean() / e - 1.0) < 0.03, (kc, cone.mean() / e)
+    assert abs(disc.mean() / e - 1.0) < 0.03
+    # a set that is not full uses r: a radius covering few photons
+    cone_small = pm.cone(q, qn, 100, 0.005, 1, 1.0)
+    assert np.all(np.isfinite(cone_small))
+
+
+def test_caustic_simpson_kernel_on_uniform_density():
+    # Simpson's kernel 3/pi (1 - d^2/R^2)^2 integrates to R^2 over the disc; with a full set the
+    # running maximum starts at the heap top (the farthest photon), so a uniform density gives
+    # (rho/pi) E (n - 0.5)/n for a Lambertian receiver seen from its side, and 0 from the other.
+    pm, q, qn, e = _uniform_plane_photons()
+    wo = np.tile([0.3, 0.0, np.sqrt(1 - 0.09)], (len(q), 1))
+    rad = pm.caustic(q, qn, wo, 100, 0.2, 1, 0.5)
+    expect = 0.5 / np.pi * e * (99.5 / 100.0)
+    assert abs(rad.mean() / expect - 1.0) < 0.03, rad.mean() / expect
+    assert np.all(pm.caustic(q, qn, -wo, 100, 0.2, 1, 0.5) == 0.0)
