@@ -33,7 +33,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 64, 32, 32, 36
+# Per-ray algorithmic bytes: ray in, hit out, 40 B per node box tested (the reference's node: float
+# bbox + child/range words) and 36 B per triangle tested (three float vertices).
+RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 64, 32, 40, 36
 
 
 def log(*a):
@@ -93,17 +95,32 @@ def bench_c2(torch, sk, wl, steps, warmup, stream):
     cam, cam15 = sk.camera(*s["camera"], w, h)
     d_rgb = torch.empty(w * h * 12, dtype=torch.uint8, device="cuda")
     sptr = stream.cuda_stream
-    counts = {}
+    profs = []
 
     def step():
-        counts["rays"] = dls.render(cam, w, h, spp, d_rgb.data_ptr(), 4, 4, 1234, sptr)
+        # the frame with HIP events at its stage boundaries (on the frame's stream)
+        profs.append(dls.render_profiled(cam, w, h, spp, d_rgb.data_ptr(), 4, 4, 1234, False, sptr))
 
     ms = time_steps(torch, step, steps, warmup, stream)
-    rays = counts["rays"]
+    timed = profs[warmup:]
+    rays = np.array(timed[-1]["rays"], np.int64)
     total = int(rays.sum())
+    stage_ms = {k: float(np.mean([p["stage_ms"][k] for p in timed])) for k in timed[-1]["stage_ms"]}
+    # one untimed frame of the counting walk: node boxes and triangles tested per ray class (the
+    # same rays as the timed frames: the sample stream is deterministic)
+    cnt = dls.render_profiled(cam, w, h, spp, d_rgb.data_ptr(), 4, 4, 1234, True, sptr)
+    assert cnt["rays"] == timed[-1]["rays"]
+    nodes, tris = np.array(cnt["node_tests"], np.int64), np.array(cnt["tri_tests"], np.int64)
+    out_bytes = np.array([HIT_BYTES, 1, HIT_BYTES], np.int64)  # occlusion rays write one byte
+    alg_bytes = int((rays * (RAY_BYTES + out_bytes) + NODE_BYTES * nodes + TRI_BYTES * tris).sum())
+    trace_ms = stage_ms["primary_trace"] + stage_ms["occlusion_trace"] + stage_ms["bsdf_trace"]
     rgb = d_rgb.cpu().numpy().view(np.float32)
     return {"ms": ms, "value": total / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays], "build_s": build_s,
-            "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean()), "scene": scene}
+            "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean()), "scene": scene,
+            "stage_ms": stage_ms, "trace_ms": trace_ms, "alg_bytes": alg_bytes,
+            "achieved_gbs": alg_bytes / (trace_ms * 1e-3) / 1e9,
+            "mean_node_tests": (nodes / np.maximum(rays, 1)).tolist(),
+            "mean_tri_tests": (tris / np.maximum(rays, 1)).tolist()}
 
 
 def bench_c0(torch, sk, wl, steps, warmup, stream):
@@ -215,7 +232,7 @@ def bench_c4(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
         # one gradient evaluation: forward image, L2 loss and its image gradient, adjoint replay
         dls.render_replay(th0, th0, cam, w, h, spp, img.data_ptr(), 4, 4, 1234, sptr)
         torch.sub(img, target, out=adj)
-        torch.sum(adj * adj, out=loss)
+        loss.copy_((adj * adj).sum())
         adj.mul_(2.0)
         dls.adjoint(th0, cam, w, h, spp, adj.data_ptr(), grad.data_ptr(), 4, 4, 1234, sptr)
 
@@ -439,9 +456,17 @@ def main():
                                    "direct lighting, f32 radiance out",
                        "rays_per_frame": c2["rays"], "triangles": c2["info"]["triangles"],
                        "nodes": c2["info"]["nodes"]},
-            # The frame's dominant kernel is the traversal; its roofline is measured on c0 where the
-            # per-ray node/triangle counts are exact (components.c0_closest_hit.roofline).
-            "roofline": comps.get("c0_closest_hit", {}).get("roofline"),
+            # The frame's dominant kernel is the traversal (primary, occlusion and BSDF launches):
+            # algorithmic bytes from the counting frame over the traversal stages' event time.
+            "roofline": {"bound": "hbm", "achieved": round(c2["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(c2["achieved_gbs"] / HBM_PEAK_GBS, 5),
+                         "traffic": pmc_traffic("trace_c2_pmc"),
+                         "kernel": "k_trace_lds (closest + any hit), all launches of a frame",
+                         "trace_ms_per_frame": round(c2["trace_ms"], 3),
+                         "alg_bytes_per_frame": c2["alg_bytes"],
+                         "mean_node_tests": [round(x, 3) for x in c2["mean_node_tests"]],
+                         "mean_tri_tests": [round(x, 3) for x in c2["mean_tri_tests"]]},
+            "stage_ms_per_frame": {k: round(v, 3) for k, v in c2["stage_ms"].items()},
             "cpu_baseline": cpu2,
             "components": comps,
         }

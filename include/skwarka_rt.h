@@ -223,6 +223,22 @@ skw_status skw_dl_scene_destroy(skw_dl_scene *dl);
 skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
                          const skw_dl_params *params, float *d_rgb, uint64_t *rays_out, void *stream);
 
+/* Per-frame profile of skw_dl_render's wavefront: HIP-event time of each stage summed over the
+ * sample passes, the rays cast per class, and (when counted) the node boxes and triangles the
+ * traversal tested per class. Classes: 0 primary, 1 occlusion, 2 BSDF. */
+typedef struct {
+  double stage_ms[6]; /* camera, primary trace, shade, occlusion trace, BSDF trace, resolve */
+  uint64_t rays[3];
+  uint64_t node_tests[3];
+  uint64_t tri_tests[3];
+} skw_dl_profile;
+
+/* skw_dl_render with a profile (no counterpart in the reference; for measurement). count_tests
+ * selects the traversal's counting variant (same walk, slower): use a separate frame for timing. */
+skw_status skw_dl_render_profiled(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                                  const skw_dl_params *params, float *d_rgb, int count_tests, skw_dl_profile *profile,
+                                  void *stream);
+
 /* ---------------------------------------------------------------- differentiable direct lighting
  * No counterpart in the reference. Parameters theta (host, 2G + n_lights doubles): reflectance
  * per group, roughness per group, emission scale per light (see skw_dl_desc).

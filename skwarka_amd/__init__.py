@@ -30,7 +30,7 @@ EXPORTS = [
     "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
     "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
-    "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
+    "skw_dl_render_profiled", "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
     "skw_batch_create", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
 ]
@@ -61,6 +61,17 @@ class Camera(ctypes.Structure):
 class DLParams(ctypes.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("light_samples", _U32),
                 ("bsdf_samples", _U32), ("seed", _U64)]
+
+
+class DLProfile(ctypes.Structure):
+    _fields_ = [("stage_ms", ctypes.c_double * 6), ("rays", _U64 * 3), ("node_tests", _U64 * 3),
+                ("tri_tests", _U64 * 3)]
+
+    STAGES = ("camera", "primary_trace", "shade", "occlusion_trace", "bsdf_trace", "resolve")
+
+    def as_dict(self):
+        return {"stage_ms": dict(zip(self.STAGES, list(self.stage_ms))), "rays": list(self.rays),
+                "node_tests": list(self.node_tests), "tri_tests": list(self.tri_tests)}
 
 
 class BatchView(ctypes.Structure):
@@ -113,6 +124,7 @@ def lib():
         "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
         "skw_dl_scene_destroy": ([_P], ctypes.c_int),
         "skw_dl_param_count": ([_P, _P], ctypes.c_int),
+        "skw_dl_render_profiled": ([_P, _P, _P, _P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
         "skw_dl_render_replay": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_dl_adjoint": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
@@ -380,6 +392,15 @@ class DLScene:
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
                                   rays.ctypes.data, stream or None), "skw_dl_render")
         return rays
+
+    def render_profiled(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0,
+                        count_tests=False, stream=0):
+        prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
+        prof = DLProfile()
+        check(lib().skw_dl_render_profiled(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm),
+                                           d_rgb, 1 if count_tests else 0, ctypes.byref(prof), stream or None),
+              "skw_dl_render_profiled")
+        return prof.as_dict()
 
     def _theta(self, theta):
         t = np.ascontiguousarray(theta, np.float64)

@@ -67,7 +67,7 @@ hipError_t CounterPool::get(void *stream, uint64_t **out) {
 
 skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev,
                          skw_hit *d_hits, uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops,
-                         uint32_t *d_tests, bool fix) {
+                         uint32_t *d_tests, bool fix, uint64_t *d_sums) {
   skw_scene *s = const_cast<skw_scene *>(scene);
   const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
   uint64_t waves = (n + 63) / 64;
@@ -88,14 +88,15 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.gstack = nullptr;
   a.pops = d_pops;
   a.tests = d_tests;
-  const bool stats = d_pops != nullptr || d_tests != nullptr;
+  a.sums = d_sums;
+  const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
   SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
   uint32_t need = s->host.stack_need;
   if (need > 128) {
     SKW_HIP(hipMallocAsync((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
   }
   SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
-  hipError_t e = any ? launch_any_hit(a, s->has_inst, need, grid, st)
+  hipError_t e = any ? launch_any_hit(a, s->has_inst, stats, need, grid, st)
                      : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
   if (a.gstack) (void)hipFreeAsync(a.gstack, st);
   if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");

@@ -76,8 +76,11 @@ skw_status check_frame_args(const skw_scene *scene, const skw_dl_scene *dl, cons
 // The wavefront of one frame: per sample pass, camera -> closest hit -> shade -> occlusion and
 // BSDF rays (ray counts read on the device) -> resolve; then the image (or nothing, for the
 // adjoint, whose resolve accumulates the gradient).
+// prof (optional): HIP-event time per stage summed over the passes; count: also count node and
+// triangle tests per ray class (the walk's counting variant).
 skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const ParamPlan &pp, const skw_camera *cam,
-                     const skw_dl_params *prm, const FrameOut &out, uint64_t *rays_out, hipStream_t st) {
+                     const skw_dl_params *prm, const FrameOut &out, uint64_t *rays_out, hipStream_t st,
+                     skw_dl_profile *prof = nullptr, bool count = false) {
   const uint64_t npix = (uint64_t)prm->width * prm->height;
   const uint64_t batch = npix;
   const uint32_t nls = prm->light_samples, nbs = prm->bsdf_samples, nl = (uint32_t)dl->lights.size();
@@ -123,7 +126,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   size_t o_sc = take(batch * nls * 3 * sizeof(double)), o_occ = take(batch * nls);
   size_t o_brays = take(batch * nbs * sizeof(skw_ray)), o_bidx = take(batch * nbs * sizeof(int32_t));
   size_t o_brec = take(batch * nbs * 8 * sizeof(double)), o_bhits = take(batch * nbs * sizeof(skw_hit));
-  size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(3 * sizeof(uint64_t));
+  size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(9 * sizeof(uint64_t));
   size_t o_film = take(npix * 3 * sizeof(double));
   size_t o_saux = 0, o_caml = 0;
   if (adj) {
@@ -187,33 +190,69 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     a.cam_light = (int32_t *)(base + o_caml);
   }
   skw_status rs = SKW_OK;
+  // stage boundaries of every pass: before camera, after camera / primary trace / shade /
+  // occlusion trace / BSDF trace / resolve
+  std::vector<hipEvent_t> ev;
+  if (prof) {
+    ev.assign(6 * (size_t)prm->spp + 1, nullptr);
+    for (auto &x : ev)
+      if (e == hipSuccess) e = hipEventCreate(&x);
+  }
+  size_t ev_next = 0;
+  auto mark = [&]() {
+    if (prof && e == hipSuccess) e = hipEventRecord(ev[ev_next++], st);
+  };
+  uint64_t *sums = count ? a.totals + 3 : nullptr;
   if (e == hipSuccess) e = launch_dl_clear(a, st);
   if (e == hipSuccess && adj) e = launch_dl_clear_grad(out.grad, 2 * dl->n_groups + nl, st);
+  mark();
   for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
     a.pass = s;
     e = launch_dl_camera(a, st);
     if (e != hipSuccess) break;
-    rs = enqueue_trace(scene, a.prim_rays, batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr, false);
+    mark();
+    rs = enqueue_trace(scene, a.prim_rays, batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr, false,
+                       sums);
     if (rs != SKW_OK) break;
+    mark();
     e = launch_dl_shade(a, st);
     if (e != hipSuccess) break;
+    mark();
     rs = enqueue_trace(scene, a.shadow_rays, batch * nls, a.counts, nullptr, a.occluded, st, true, nullptr, nullptr,
-                       false);
+                       false, sums ? sums + 2 : nullptr);
     if (rs != SKW_OK) break;
+    mark();
     rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
-                       nullptr, false);
+                       nullptr, false, sums ? sums + 4 : nullptr);
     if (rs != SKW_OK) break;
+    mark();
     e = launch_dl_resolve(a, st);
+    mark();
   }
   if (e == hipSuccess && rs == SKW_OK) {
     if (out.rgb) e = launch_dl_finish(a, out.rgb, st);
     else if (out.img) e = launch_dl_finish_f64(a, out.img, st);
   }
-  uint64_t totals[3] = {0, 0, 0};
-  if (e == hipSuccess && rs == SKW_OK && rays_out) {
+  uint64_t totals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess && rs == SKW_OK && (rays_out || prof)) {
     e = hipMemcpyAsync(totals, a.totals, sizeof(totals), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
+  if (prof && e == hipSuccess && rs == SKW_OK) {
+    std::memset(prof, 0, sizeof(*prof));
+    for (size_t i = 0; i + 1 < ev.size() && e == hipSuccess; ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      prof->stage_ms[i % 6] += ms;
+    }
+    for (int c = 0; c < 3; ++c) {
+      prof->rays[c] = totals[c];
+      prof->node_tests[c] = totals[3 + 2 * c];
+      prof->tri_tests[c] = totals[4 + 2 * c];
+    }
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
   (void)hipFreeAsync(base, st);
   if (rs != SKW_OK) return rs;
   if (e != hipSuccess) return hip_fail(e, "direct lighting frame");
@@ -367,6 +406,18 @@ skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const s
   FrameOut out;
   out.rgb = d_rgb;
   return run_frame(scene, dl, ParamPlan(), cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
+}
+
+skw_status skw_dl_render_profiled(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                                  const skw_dl_params *prm, float *d_rgb, int count_tests, skw_dl_profile *profile,
+                                  void *stream) {
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  if (!d_rgb || !profile) return fail(SKW_INVALID_ARG, "d_rgb and profile must not be NULL");
+  FrameOut out;
+  out.rgb = d_rgb;
+  return run_frame(scene, dl, ParamPlan(), cam, prm, out, nullptr, reinterpret_cast<hipStream_t>(stream), profile,
+                   count_tests != 0);
 }
 
 skw_status skw_dl_param_count(const skw_dl_scene *dl, uint32_t *n) {

@@ -81,7 +81,9 @@ struct skw_photon_map {
 namespace skw {
 // Enqueues closest-hit (any = false) or occlusion (any = true) for n rays; when n_dev is set the
 // count is read on the device and n only bounds the launch size. fix: rewrite b1/b2 of hits on
-// transformed primitives in mesh space.
+// transformed primitives in mesh space. d_sums (optional, device): [0] += node boxes tested,
+// [1] += triangles tested over all rays (selects the counting variant of the walk).
 skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev, skw_hit *d_hits,
-                         uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops, uint32_t *d_tests, bool fix);
+                         uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops, uint32_t *d_tests, bool fix,
+                         uint64_t *d_sums = nullptr);
 }  // namespace skw

@@ -689,7 +689,7 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
 __global__ __launch_bounds__(256) void k_dl_clear(DlArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t n = (uint64_t)a.width * a.height * 3;
-  if (i < 3) a.totals[i] = 0;
+  if (i < 9) a.totals[i] = 0;  // ray totals and the traversal test sums of a counting frame
   if (i < n) a.film[i] = 0.0;
 }
 

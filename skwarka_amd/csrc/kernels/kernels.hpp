@@ -24,12 +24,14 @@ struct TraceArgs {
   uint32_t *gstack;   // used only when the tree is deeper than the LDS stack
   uint32_t *pops;     // statistics variant only: node boxes tested per ray
   uint32_t *tests;    // statistics variant only: triangles tested per ray
+  uint64_t *sums;     // statistics variant only: [0] += node boxes, [1] += triangles over all rays
 };
 
 // Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).
 hipError_t launch_closest_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
                               hipStream_t stream);
-hipError_t launch_any_hit(const TraceArgs &args, bool inst, uint32_t stack_need, uint32_t grid, hipStream_t stream);
+hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
+                          hipStream_t stream);
 hipError_t launch_set_counter(uint64_t *counter, uint64_t value, hipStream_t stream);
 
 struct GeomArgs {
@@ -143,7 +145,8 @@ struct DlArgs {
   uint8_t *occluded;
   skw_hit *bsdf_hits;
   double *film;          // 3 per pixel, running sum over passes
-  uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device)
+  uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device); [3..8]
+                         // node / triangle tests of the three ray classes (counting frames)
   // adjoint (when adj is set): dLoss/dImage per pixel, gradient output, per-sample records
   const double *adj;     // 3 per pixel
   double *grad;          // 2 * n_groups + n_lights

@@ -25,6 +25,7 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool busy = false;
   bool first = true;
+  unsigned long long sum_pops = 0, sum_tests = 0;
   Walker<ANY, INST, STATS, Stack> w;
   w.nodes = args.nodes;
   w.tris = args.tris;
@@ -62,8 +63,21 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
       if (STATS) {
         if (args.pops) args.pops[idx] = w.pops;
         if (args.tests) args.tests[idx] = w.tests;
+        sum_pops += w.pops;
+        sum_tests += w.tests;
       }
       busy = false;
+    }
+  }
+  if (STATS && args.sums) {
+    // all lanes have left the loop: one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+      sum_pops += __shfl_xor(sum_pops, off);
+      sum_tests += __shfl_xor(sum_tests, off);
+    }
+    if (lane == 0) {
+      atomicAdd(reinterpret_cast<unsigned long long *>(args.sums), (unsigned long long)sum_pops);
+      atomicAdd(reinterpret_cast<unsigned long long *>(args.sums + 1), (unsigned long long)sum_tests);
     }
   }
 }
@@ -110,7 +124,11 @@ hipError_t launch_closest_hit(const TraceArgs &args, bool inst, bool stats, uint
               : launch_trace<false, false, false>(args, stack_need, grid, stream);
 }
 
-hipError_t launch_any_hit(const TraceArgs &args, bool inst, uint32_t stack_need, uint32_t grid, hipStream_t stream) {
+hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
+                          hipStream_t stream) {
+  if (stats)
+    return inst ? launch_trace<true, true, true>(args, stack_need, grid, stream)
+                : launch_trace<true, false, true>(args, stack_need, grid, stream);
   return inst ? launch_trace<true, true, false>(args, stack_need, grid, stream)
               : launch_trace<true, false, false>(args, stack_need, grid, stream);
 }

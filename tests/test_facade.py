@@ -76,12 +76,12 @@ def _case():
     rays = np.concatenate([wl.camera_rays(24, 24, (0.0, 0.0, -2.9), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0),
                            wl.make_rays(rng.uniform(-2.5, 2.5, (200, 3)), dirs)])
     mesh = wl.merge(meshes)
-    ph = wl.photons_on_mesh(mesh, 20000, seed=3)
+    ph = wl.photons_on_mesh(mesh, 80000, seed=3)
     acc = oracle.Accel(meshes, prims)
-    hits = acc.closest_hit(rays)
+    hits, geom = acc.closest_hit(rays, geom=True)
     keep = np.flatnonzero(hits["prim"] != sk.MISS)[:150]
-    q = rays["o"][keep] + rays["d"][keep] * hits["t"][keep][:, None]
-    qn = -rays["d"][keep]
+    q = geom["p"][keep].copy()
+    qn = geom["ng"][keep].copy()
     return meshes, prims, rays, ph, q, qn, acc
 
 
