@@ -294,6 +294,25 @@ typedef struct {
 
 skw_status skw_batch_create(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
                             skw_batch **out);
+
+/* A photon-gather job of a batch: skw_photon_irradiance's work (k nearest with PhotonFilter, disc
+ * estimate) for nq query points, written to the job's own outputs every frame. */
+typedef struct {
+  const skw_photon_map *map;
+  const double *d_points, *d_normals; /* 3 doubles per query, device */
+  uint64_t nq;
+  uint32_t k;
+  double r;
+  uint64_t paths;
+  double rho;
+  float *d_e_ext, *d_e_int;          /* 3 floats per query, device */
+  float *d_radiance;                 /* optional */
+} skw_batch_gather;
+
+/* A batch whose queue also holds photon-gather jobs: ray chunks and query chunks of all scenes and
+ * jobs are served by the same persistent launch (items ordered by size, smallest first). */
+skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
+                                         const skw_batch_gather *gathers, uint32_t n_gathers, skw_batch **out);
 skw_status skw_batch_destroy(skw_batch *batch);
 /* Rebuilds the work queue for `cfg` and captures the frame graph. */
 skw_status skw_batch_configure(skw_batch *batch, const skw_batch_config *cfg);
@@ -305,8 +324,8 @@ skw_status skw_batch_autotune(skw_batch *batch, uint32_t frames_per_config, skw_
 skw_status skw_batch_frame(skw_batch *batch, void *stream);
 /* Device pointer and count of a scene's hit records (valid after a frame). */
 skw_status skw_batch_hits(const skw_batch *batch, uint32_t scene, const skw_hit **d_hits, uint64_t *n);
-/* Completion time of each scene's last chunk, in nanoseconds after the frame started (waits for
- * the last frame). */
+/* Completion time of each item's last chunk (scenes, then gather jobs), in nanoseconds after the
+ * frame started (waits for the last frame). */
 skw_status skw_batch_latency(skw_batch *batch, double *ns_per_scene);
 /* Writes scene `scene`'s camera rays (as the frame generates them) to d_rays; for checking. */
 skw_status skw_batch_rays(const skw_batch *batch, uint32_t scene, skw_ray *d_rays, void *stream);

@@ -31,7 +31,7 @@ EXPORTS = [
     "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
     "skw_dl_render_profiled", "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
-    "skw_batch_create", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
+    "skw_batch_create", "skw_batch_create_with_gathers", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
 ]
 
@@ -76,6 +76,11 @@ class DLProfile(ctypes.Structure):
 
 class BatchView(ctypes.Structure):
     _fields_ = [("cam", Camera), ("width", _U32), ("height", _U32), ("spp", _U32), ("seed", _U64)]
+
+
+class BatchGather(ctypes.Structure):
+    _fields_ = [("map", _P), ("d_points", _P), ("d_normals", _P), ("nq", _U64), ("k", _U32), ("r", ctypes.c_double),
+                ("paths", _U64), ("rho", ctypes.c_double), ("d_e_ext", _P), ("d_e_int", _P), ("d_radiance", _P)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -129,6 +134,7 @@ def lib():
         "skw_dl_adjoint": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
         "skw_batch_create": ([_P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_batch_create_with_gathers": ([_P, _P, _U32, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_batch_destroy": ([_P], ctypes.c_int),
         "skw_batch_configure": ([_P, ctypes.POINTER(BatchConfig)], ctypes.c_int),
         "skw_batch_autotune": ([_P, _U32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_double)],
@@ -433,22 +439,27 @@ class DLScene:
 
 class Batch:
     """Many scenes traced in one persistent launch from a device work queue (skw_batch_*).
-    views: list of (Camera, width, height, spp, seed), one per scene."""
+    views: list of (Camera, width, height, spp, seed), one per scene. gathers (optional): list of
+    (PhotonMap, d_points, d_normals, nq, k, r, paths, rho, d_e_ext, d_e_int, d_radiance) jobs."""
 
-    def __init__(self, scenes, views):
+    def __init__(self, scenes, views, gathers=()):
         L = lib()
         self.scenes = list(scenes)
-        n = len(self.scenes)
-        hs = (_P * n)(*[s.handle for s in self.scenes])
-        vs = (BatchView * n)()
+        self.maps = [g[0] for g in gathers]
+        n, m = len(self.scenes), len(gathers)
+        hs = (_P * max(n, 1))(*[s.handle for s in self.scenes])
+        vs = (BatchView * max(n, 1))()
         for i, (cam, w, h, spp, seed) in enumerate(views):
             vs[i].cam = cam
             vs[i].width, vs[i].height, vs[i].spp, vs[i].seed = w, h, spp, seed
-        self._hs, self._vs = hs, vs
+        gs = (BatchGather * max(m, 1))()
+        for j, (pm, dp, dn, nq, k, r, paths, rho, ee, ei, rad) in enumerate(gathers):
+            gs[j] = BatchGather(pm.handle, dp, dn, nq, k, r, paths, rho, ee, ei, rad)
+        self._hs, self._vs, self._gs = hs, vs, gs
         b = _P()
-        check(L.skw_batch_create(hs, vs, n, ctypes.byref(b)), "skw_batch_create")
+        check(L.skw_batch_create_with_gathers(hs, vs, n, gs, m, ctypes.byref(b)), "skw_batch_create_with_gathers")
         self.handle = b
-        self.n = n
+        self.n = n + m
 
     def close(self):
         if self.handle:

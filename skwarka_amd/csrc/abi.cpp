@@ -110,6 +110,30 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   return SKW_OK;
 }
 
+GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                       uint32_t k, double r, uint64_t paths, double rho, int filter) {
+  GatherArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nodes = map->d_nodes;
+  a.photons = map->d_photons;
+  a.perm = map->d_perm;
+  a.cdir = map->d_cdir;
+  a.n_nodes = map->host.nodes.size();
+  a.points = d_points;
+  a.normals = d_normals;
+  a.nq = nq;
+  a.k = k;
+  a.r = r;
+  a.paths = (double)paths;
+  a.rho = rho;
+  a.sin_t = std::sqrt(1.0 - 0.87 * 0.87);  // PhotonFilter's sin threshold for cos 0.87
+  a.filter = filter;
+  a.q_stride = 1;
+  a.j_stride = nq;
+  a.stack_need = map->host.max_depth + 1;
+  return a;
+}
+
 }  // namespace skw
 
 skw_scene::~skw_scene() {
@@ -345,25 +369,7 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   if (!d_points) return fail(SKW_INVALID_ARG, "points must not be NULL");
   if (filter == 1 && !d_normals) return fail(SKW_INVALID_ARG, "PhotonFilter needs normals");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  GatherArgs a;
-  a.nodes = map->d_nodes;
-  a.photons = map->d_photons;
-  a.perm = map->d_perm;
-  a.cdir = map->d_cdir;
-  a.n_nodes = map->host.nodes.size();
-  a.points = d_points;
-  a.normals = d_normals;
-  a.nq = nq;
-  a.k = k;
-  a.r = r;
-  a.paths = (double)paths;
-  a.rho = rho;
-  a.sin_t = std::sqrt(1.0 - 0.87 * 0.87);
-  a.filter = filter;
-  a.heap_pos = nullptr;
-  a.heap_d2 = nullptr;
-  a.q_stride = 1;
-  a.j_stride = nq;
+  GatherArgs a = gather_args(map, d_points, d_normals, nq, k, r, paths, rho, filter);
   a.ids = ids;
   a.d2 = d2;
   a.count = count;
@@ -371,8 +377,6 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.e_int = e_int;
   a.radiance = radiance;
   a.nearest = nearest;
-  a.stack_need = map->host.max_depth + 1;
-  a.gstack = nullptr;
   a.estimate = est.kind;
   a.kc = est.kc;
   a.wo = est.wo;

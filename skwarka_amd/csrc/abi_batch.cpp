@@ -15,8 +15,12 @@ using namespace skw;
 struct skw_batch {
   std::vector<const skw_scene *> scenes;
   std::vector<DevBatchScene> table;
+  std::vector<GatherArgs> jobs;   // photon-gather jobs, heap storage attached
+  std::vector<void *> job_mem;    // per job: heap slots (nq * k positions, then nq * k d^2)
+  GatherArgs *d_jobs = nullptr;
   uint64_t total_rays = 0;
-  uint32_t max_stack = 0;
+  uint64_t total_queries = 0;
+  uint32_t max_stack = 0;          // deepest walk or lookup stack of any item
   int device = 0;
   int cus = 256;
   double wall_khz = 100000.0;
@@ -31,16 +35,19 @@ struct skw_batch {
   hipGraphExec_t exec = nullptr;
   hipStream_t last_stream = nullptr;
 
+  size_t n_items() const { return table.size() + jobs.size(); }
   BatchArgs args() const {
     BatchArgs a;
     a.scenes = d_table;
+    a.jobs = d_jobs;
     a.chunks = d_chunks;
     a.n_chunks = n_chunks;
     a.queue_head = d_state;
     a.frame_start = d_state + 1;
-    a.scene_done = d_state + 2;
-    a.scene_end = d_state + 2 + table.size();
+    a.item_done = d_state + 2;
+    a.item_end = d_state + 2 + n_items();
     a.n_scenes = (uint32_t)table.size();
+    a.n_jobs = (uint32_t)jobs.size();
     a.hits = d_hits;
     return a;
   }
@@ -57,8 +64,10 @@ struct skw_batch {
   }
   ~skw_batch() {
     drop_graph();
-    void *ptrs[] = {d_table, d_chunks, d_state, d_hits};
+    void *ptrs[] = {d_table, d_chunks, d_state, d_hits, d_jobs};
     for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    for (void *p : job_mem)
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -73,21 +82,23 @@ bool valid_config(const skw_batch *b, const skw_batch_config &c) {
          c.waves_per_cu <= 64;
 }
 
-// Chunks of cfg.chunk_rays rays, scenes ordered by ray count (then index), so small scenes finish
-// first.
+// Chunks of cfg.chunk_rays rays (or gather queries), items ordered by their size (then index), so
+// small scenes and jobs finish first.
 hipError_t build_queue(skw_batch *b) {
-  std::vector<uint32_t> order(b->table.size());
+  const size_t ns = b->table.size();
+  auto units = [&](size_t item) { return item < ns ? b->table[item].n_rays : b->jobs[item - ns].nq; };
+  std::vector<uint32_t> order(b->n_items());
   std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](uint32_t x, uint32_t y) { return b->table[x].n_rays < b->table[y].n_rays; });
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return units(x) < units(y); });
   std::vector<DevChunk> chunks;
   for (uint32_t s : order) {
-    for (uint64_t r = 0; r < b->table[s].n_rays; r += b->cfg.chunk_rays) {
+    const uint64_t n = units(s);
+    for (uint64_t r = 0; r < n; r += b->cfg.chunk_rays) {
       DevChunk c;
-      c.scene = s;
+      c.item = s;
       c.pad = 0;
       c.begin = r;
-      c.end = std::min<uint64_t>(r + b->cfg.chunk_rays, b->table[s].n_rays);
+      c.end = std::min<uint64_t>(r + b->cfg.chunk_rays, n);
       chunks.push_back(c);
     }
   }
@@ -121,7 +132,14 @@ extern "C" {
 
 skw_status skw_batch_create(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
                             skw_batch **out) {
-  if (!out || (n > 0 && (!scenes || !views))) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  return skw_batch_create_with_gathers(scenes, views, n, nullptr, 0, out);
+}
+
+skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const skw_batch_view *views, uint32_t n,
+                                         const skw_batch_gather *gathers, uint32_t n_gathers, skw_batch **out) {
+  if (!out || (n > 0 && (!scenes || !views)) || (n_gathers > 0 && !gathers))
+    return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  if (n == 0 && n_gathers == 0) return fail(SKW_INVALID_ARG, "empty batch");
   *out = nullptr;
   std::unique_ptr<skw_batch> b(new skw_batch());
   SKW_HIP(hipGetDevice(&b->device), "hipGetDevice");
@@ -158,9 +176,32 @@ skw_status skw_batch_create(const skw_scene *const *scenes, const skw_batch_view
     b->max_stack = std::max(b->max_stack, s->host.stack_need);
   }
   b->total_rays = off;
+  for (uint32_t j = 0; j < n_gathers; ++j) {
+    const skw_batch_gather &g = gathers[j];
+    if (!g.map || !g.map->on_device) return fail(SKW_INVALID_ARG, "every gather needs a device photon map");
+    if (g.nq == 0 || g.k == 0) return fail(SKW_INVALID_ARG, "empty gather job");
+    if (!g.d_points || !g.d_normals || !g.d_e_ext || !g.d_e_int)
+      return fail(SKW_INVALID_ARG, "gather points, normals, e_ext and e_int must not be NULL");
+    if (!(g.r >= 0.0)) return fail(SKW_INVALID_ARG, "radius must be >= 0");
+    if (g.map->host.max_depth + 1 > 64) return fail(SKW_UNSUPPORTED, "a photon tree needs a stack deeper than 64");
+    GatherArgs a = gather_args(g.map, g.d_points, g.d_normals, g.nq, g.k, g.r, g.paths, g.rho, 1);
+    a.e_ext = g.d_e_ext;
+    a.e_int = g.d_e_int;
+    a.radiance = g.d_radiance;
+    const size_t slots = (size_t)g.nq * g.k;
+    void *mem = nullptr;
+    SKW_HIP(hipMalloc(&mem, slots * (sizeof(uint32_t) + sizeof(double)) + 256), "gather heap allocation");
+    b->job_mem.push_back(mem);
+    a.heap_d2 = static_cast<double *>(mem);
+    a.heap_pos = reinterpret_cast<uint32_t *>(static_cast<char *>(mem) + slots * sizeof(double));
+    b->jobs.push_back(a);
+    b->total_queries += g.nq;
+    b->max_stack = std::max(b->max_stack, a.stack_need);
+  }
   b->cfg.stack_entries = b->max_stack <= 32 ? 32 : 64;
   SKW_HIP(upload(&b->d_table, b->table), "upload scene table");
-  SKW_HIP(hipMalloc((void **)&b->d_state, (2 + 2 * (size_t)n) * sizeof(uint64_t)), "state allocation");
+  SKW_HIP(upload(&b->d_jobs, b->jobs), "upload gather jobs");
+  SKW_HIP(hipMalloc((void **)&b->d_state, (2 + 2 * b->n_items()) * sizeof(uint64_t)), "state allocation");
   SKW_HIP(hipMalloc((void **)&b->d_hits, std::max<uint64_t>(1, off) * sizeof(skw_hit)), "hit allocation");
   SKW_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream creation");
   SKW_HIP(build_queue(b.get()), "queue upload");
@@ -254,9 +295,9 @@ skw_status skw_batch_hits(const skw_batch *b, uint32_t scene, const skw_hit **d_
 skw_status skw_batch_latency(skw_batch *b, double *ns) {
   if (!b || !ns) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
   SKW_HIP(hipDeviceSynchronize(), "synchronize");
-  std::vector<uint64_t> st(2 + 2 * b->table.size());
+  std::vector<uint64_t> st(2 + 2 * b->n_items());
   SKW_HIP(hipMemcpy(st.data(), b->d_state, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "state copy");
-  const size_t n = b->table.size();
+  const size_t n = b->n_items();
   for (size_t i = 0; i < n; ++i) {
     uint64_t end = st[2 + n + i];
     ns[i] = end >= st[1] ? (double)(end - st[1]) * 1e6 / b->wall_khz : 0.0;

@@ -86,4 +86,9 @@ namespace skw {
 skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev, skw_hit *d_hits,
                          uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops, uint32_t *d_tests, bool fix,
                          uint64_t *d_sums = nullptr);
+
+// Gather arguments for one photon map and query set, outputs unset and no heap or stack storage
+// attached (heap: nq * k slots, slot-major; stack: only when the tree is deeper than 64).
+GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                       uint32_t k, double r, uint64_t paths, double rho, int filter);
 }  // namespace skw

@@ -7,9 +7,12 @@
 // generated in the kernel (no ray buffer): PerspectiveCamera::GenerateRay
 // (Raytracer/Cameras/PerspectiveCamera.cpp:36-64) at pixel + splitmix64 jitter, the same stream as
 // the direct-lighting camera. The walk is trace_device.hpp's (TriangleAccelerator::_NodeIntersect
-// semantics). When a scene's last chunk completes, the wave records the wall clock for it.
+// semantics). Photon-gather jobs (k nearest + disc estimate, gather_device.hpp) share the queue:
+// their chunks are query ranges. When an item's last chunk completes, the wave records the wall
+// clock for it.
 #include <hip/hip_runtime.h>
 
+#include "gather_device.hpp"
 #include "geom_device.hpp"
 #include "kernels.hpp"
 #include "trace_device.hpp"
@@ -48,9 +51,19 @@ __global__ void k_batch_reset(BatchArgs a) {
     *a.queue_head = 0;
     *a.frame_start = wall_clock64();
   }
-  if (i < a.n_scenes) {
-    a.scene_done[i] = 0;
-    a.scene_end[i] = 0;
+  if (i < a.n_scenes + a.n_jobs) {
+    a.item_done[i] = 0;
+    a.item_end[i] = 0;
+  }
+}
+
+// Marks len units of item done; the wave that completes the item records the wall clock.
+__device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint64_t len, uint64_t total,
+                                        uint32_t lane) {
+  if (lane == 0) {
+    unsigned long long old =
+        atomicAdd(reinterpret_cast<unsigned long long *>(a.item_done + item), (unsigned long long)len);
+    if (old + len == total) a.item_end[item] = wall_clock64();
   }
 }
 
@@ -71,17 +84,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     const uint64_t idle = __ballot(!busy);
     if (idle == ~0ull && cursor >= end) {
       // The wave's chunk is done: account for it, then take the next one.
-      if (sid != 0xFFFFFFFFu && lane == 0) {
-        unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(a.scene_done + sid),
-                                           (unsigned long long)chunk_len);
-        if (old + chunk_len == sc->n_rays) a.scene_end[sid] = wall_clock64();
-      }
+      if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lane);
+      sid = 0xFFFFFFFFu;
       unsigned long long q = 0;
       if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
       q = __shfl(q, 0);
       if (q >= a.n_chunks) break;
       const DevChunk c = a.chunks[q];
-      sid = c.scene;
+      if (c.item >= a.n_scenes) {
+        // A photon-gather chunk: the wave's lanes take its queries 64 at a time, the lookup stack
+        // in the same LDS rows the ray walk uses (a lane does one kind of work at a time).
+        const GatherArgs &ga = a.jobs[c.item - a.n_scenes];
+        gath::LdsStack<S> gstk{stk.base};
+        for (uint64_t qi = c.begin + lane; qi < c.end; qi += 64) gath::gather_one<0>(ga, qi, gstk);
+        account(a, c.item, c.end - c.begin, ga.nq, lane);
+        cursor = end = 0;
+        continue;
+      }
+      sid = c.item;
       cursor = c.begin;
       end = c.end;
       chunk_len = c.end - c.begin;
@@ -129,7 +149,7 @@ __global__ __launch_bounds__(256) void k_batch_rays(const DevBatchScene *s, skw_
 }  // namespace
 
 hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream) {
-  uint32_t grid = (a.n_scenes + 255) / 256;
+  uint32_t grid = (a.n_scenes + a.n_jobs + 255) / 256;
   hipLaunchKernelGGL(k_batch_reset, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
   return hipGetLastError();
 }

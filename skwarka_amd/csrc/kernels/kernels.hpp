@@ -190,21 +190,25 @@ struct DevBatchScene {
   uint64_t hit_offset;  // first record of this scene in the hit buffer
 };
 
+// A queue chunk: a ray range of a scene, or (item >= n_scenes) a query range of gather job
+// item - n_scenes.
 struct DevChunk {
-  uint32_t scene;
+  uint32_t item;
   uint32_t pad;
-  uint64_t begin, end;  // ray index range inside the scene
+  uint64_t begin, end;
 };
 
 struct BatchArgs {
   const DevBatchScene *scenes;
+  const GatherArgs *jobs;  // photon gathers (disc estimate), one record per job
   const DevChunk *chunks;
   uint64_t n_chunks;
   uint64_t *queue_head;
-  uint64_t *scene_done;  // rays finished per scene
-  uint64_t *scene_end;   // wall clock of each scene's completion
+  uint64_t *item_done;   // rays / queries finished per item (scenes, then jobs)
+  uint64_t *item_end;    // wall clock of each item's completion
   uint64_t *frame_start; // wall clock at frame start
   uint32_t n_scenes;
+  uint32_t n_jobs;
   skw_hit *hits;
 };
 hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream);

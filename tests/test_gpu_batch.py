@@ -8,7 +8,7 @@ import pytest
 
 import skwarka_amd as sk
 from skwarka_amd import workloads as wl
-from tests.conftest import empty_dev, from_dev
+from tests.conftest import empty_dev, from_dev, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +85,47 @@ def test_batch_rejects_bad_shapes(torch_cuda):
     for shape in ((3, 32, 256, 16), (1, 16, 256, 16), (1, 32, 100, 16)):
         with pytest.raises(sk.SkwError):
             batch.configure(*shape)
+
+
+def test_batch_with_photon_gathers(torch_cuda):
+    print("case: batch of 3 scenes + 2 photon-gather jobs vs skw_photon_irradiance", flush=True)
+    torch = torch_cuda
+    scenes, views = _small_batch(3)
+    jobs, refs = [], []
+    for seed, (nph, nq, k, r) in enumerate(((200000, 20000, 100, 0.05), (30000, 3000, 8, 0.2))):
+        mesh = wl.merge(wl.scene_c0())
+        ph = wl.photons_on_mesh(mesh, nph, seed=seed + 1)
+        pm = sk.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
+        rng = np.random.default_rng(seed)
+        sel = rng.choice(len(ph["pos"]), nq, replace=False)
+        p = ph["pos"][sel].astype(np.float64)
+        n = wl.cdir_table()[ph["normal"][sel]]
+        dp, dn = to_dev(torch, p), to_dev(torch, n)
+        outs = [empty_dev(torch, nq * 12) for _ in range(3)]
+        jobs.append((pm, dp.data_ptr(), dn.data_ptr(), nq, k, r, 2, 0.5, outs[0].data_ptr(), outs[1].data_ptr(),
+                     outs[2].data_ptr()))
+        ref = [empty_dev(torch, nq * 12) for _ in range(3)]
+        pm.irradiance(dp.data_ptr(), dn.data_ptr(), nq, k, r, 2, 0.5, ref[0].data_ptr(), ref[1].data_ptr(),
+                      ref[2].data_ptr())
+        refs.append((outs, ref, (dp, dn), nq))
+    batch = sk.Batch(scenes, views, jobs)
+    torch.cuda.synchronize()
+    for shape in ((1, 32, 256, 16), (4, 64, 4096, 8)):
+        batch.configure(*shape)
+        for _ in range(2):
+            batch.frame()
+            torch.cuda.synchronize()
+            for outs, ref, _, nq in refs:
+                for o, r in zip(outs, ref):
+                    assert np.array_equal(from_dev(o, np.uint32, nq * 3), from_dev(r, np.uint32, nq * 3))
+            for i, sc in enumerate(scenes):
+                got = _batch_hits(torch, batch, i)
+                d_rays = empty_dev(torch, len(got) * 64)
+                batch.rays(i, d_rays.data_ptr())
+                d_hits = empty_dev(torch, len(got) * 32)
+                sc.closest_hit(d_rays.data_ptr(), len(got), d_hits.data_ptr())
+                torch.cuda.synchronize()
+                assert np.array_equal(got.view(np.uint8), from_dev(d_hits, sk.HIT_DTYPE, len(got)).view(np.uint8))
+    lat = batch.latency_ns()
+    assert len(lat) == 5 and np.all(lat > 0)
+    assert from_dev(refs[0][1][0], np.float32, 100).max() > 0
