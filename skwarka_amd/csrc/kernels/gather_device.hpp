@@ -122,6 +122,13 @@ struct Query {
   double sin_t;
 };
 
+// Query coordinate on a split axis, selected as a value (a conditional over the struct's members
+// would select their addresses and move the query into scratch memory).
+__device__ __forceinline__ double axis_coord(const Query &q, uint32_t axis) {
+  const double x = q.q0, y = q.q1, z = q.q2;
+  return axis == 0 ? x : (axis == 1 ? y : z);
+}
+
 // PhotonFilter::operator() (PhotonInternalTypes.h:128-134); to = photon - point, already formed
 // by the distance computation with the same subtraction.
 __device__ __forceinline__ bool photon_filter(const GatherArgs &a, const Query &q, uint32_t pos, double dx, double dy,
@@ -151,7 +158,7 @@ __device__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, cons
         float4 nd = reinterpret_cast<const float4 *>(a.nodes)[par];
         uint32_t w = __float_as_uint(nd.w);
         uint32_t axis = w & 3u;
-        double qa = axis == 0 ? q.q0 : (axis == 1 ? q.q1 : q.q2);
+        double qa = axis_coord(q, axis);
         double split = axis == 0 ? (double)nd.x : (axis == 1 ? (double)nd.y : (double)nd.z);
         double pd2 = (qa - split) * (qa - split);
         if (pd2 < max_d2) {
@@ -190,7 +197,7 @@ __device__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, cons
     uint32_t w = __float_as_uint(nd.w);
     uint32_t axis = w & 3u;
     if (axis == 3u) continue;  // leaf
-    double qa = axis == 0 ? q.q0 : (axis == 1 ? q.q1 : q.q2);
+    double qa = axis_coord(q, axis);
     double split = axis == 0 ? (double)nd.x : (axis == 1 ? (double)nd.y : (double)nd.z);
     bool has_left = (w & 4u) != 0;
     uint32_t right = w >> 3;

@@ -44,6 +44,13 @@ struct Best {
   double b1, b2;
 };
 
+// Bit k set when direction component k is > 0.0: the child order rule of :199 as integer bits,
+// so the split axis selects a bit instead of a double (a selected double would push the ray into
+// scratch memory).
+__device__ __forceinline__ uint32_t dir_mask(const RayState &r) {
+  return (r.d0 > 0.0 ? 1u : 0u) | (r.d1 > 0.0 ? 2u : 0u) | (r.d2 > 0.0 ? 4u : 0u);
+}
+
 __device__ __forceinline__ void load_node(const DevNode *__restrict__ nodes, uint32_t idx, float4 &lo, float4 &hi) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
   lo = p[0];
@@ -163,6 +170,7 @@ __device__ bool walk_instance(const DevNode *__restrict__ nodes, const DevTri *_
   uint32_t cur = start;
   bool have = true;
   bool found = false;
+  const uint32_t dpos = dir_mask(r);
   while (true) {
     if (!have) {
       if (sp == sp0) break;
@@ -185,8 +193,7 @@ __device__ bool walk_instance(const DevNode *__restrict__ nodes, const DevTri *_
     uint32_t kind = b & 3u;
     if (kind != 0) {
       uint32_t nch = __builtin_popcount((b >> 2) & 7u);
-      double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
-      if (dax > 0.0) {
+      if ((dpos >> (kind - 1u)) & 1u) {
         cur = a;
         if (nch > 1) stk.set(sp++, ((a + 1) << 3) | (nch - 1));
       } else {
@@ -320,13 +327,14 @@ struct Walker {
   RayState r;
   double tmax;
   Best best;
-  uint32_t sp, cur, ca, cb;
+  uint32_t sp, cur, ca, cb, dpos;
   bool have, found;
   uint32_t pops, tests;
 
   __device__ __forceinline__ void start(const RayState &ray, double tmax0) {
     r = ray;
     tmax = tmax0;
+    dpos = dir_mask(ray);
     best.hit = false;
     best.inst = false;
     sp = 0;
@@ -366,31 +374,31 @@ struct Walker {
     const uint32_t kind = b & 3u;
     if (kind != 0) {
       const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
-      const double dax = kind == 1 ? r.d0 : (kind == 2 ? r.d1 : r.d2);
-      const bool fwd = dax > 0.0;
-      float4 clo[3], chi[3];
-      uint32_t cidx[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        cidx[k] = fwd ? a + (uint32_t)k : a + nch - 1u - (uint32_t)k;
-        if ((uint32_t)k < nch) load_node(nodes, cidx[k], clo[k], chi[k]);
-      }
-      uint32_t surv = 0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if ((uint32_t)k < nch && !slab_cull(clo[k], chi[k], r, tmax)) surv |= 1u << k;
+      const bool fwd = (dpos >> (kind - 1u)) & 1u;
+      // Children in visiting order (scalars, not arrays: an indexed array would live in scratch).
+      const uint32_t c0 = fwd ? a : a + nch - 1u;
+      const uint32_t c1 = fwd ? a + 1u : a + nch - 2u;
+      const uint32_t c2 = a + (fwd ? 2u : 0u);
+      float4 lo0, hi0, lo1 = make_float4(0, 0, 0, 0), hi1 = lo1, lo2 = lo1, hi2 = lo1;
+      load_node(nodes, c0, lo0, hi0);
+      if (nch > 1) load_node(nodes, c1, lo1, hi1);
+      if (nch > 2) load_node(nodes, c2, lo2, hi2);
+      uint32_t surv = slab_cull(lo0, hi0, r, tmax) ? 0u : 1u;
+      if (nch > 1 && !slab_cull(lo1, hi1, r, tmax)) surv |= 2u;
+      if (nch > 2 && !slab_cull(lo2, hi2, r, tmax)) surv |= 4u;
       if (STATS) pops += nch;
       if (surv == 0) return false;
       const uint32_t s0 = __builtin_ctz(surv);
-      cur = s0 == 0 ? cidx[0] : (s0 == 1 ? cidx[1] : cidx[2]);
-      const float4 hs = s0 == 0 ? chi[0] : (s0 == 1 ? chi[1] : chi[2]);
-      ca = __float_as_uint(hs.z);
-      cb = __float_as_uint(hs.w);
+      cur = s0 == 0 ? c0 : (s0 == 1 ? c1 : c2);
+      const float hz = s0 == 0 ? hi0.z : (s0 == 1 ? hi1.z : hi2.z);
+      const float hw = s0 == 0 ? hi0.w : (s0 == 1 ? hi1.w : hi2.w);
+      ca = __float_as_uint(hz);
+      cb = __float_as_uint(hw);
       have = true;
       const uint32_t rest = surv & ~((2u << s0) - 1u);
       if (rest) {
         const uint32_t p = __builtin_ctz(rest);
-        const uint32_t start = p == 1 ? cidx[1] : cidx[2];
+        const uint32_t start = p == 1 ? c1 : c2;
         stk.set(sp++, (start << 3) | (fwd ? 0u : 4u) | (uint32_t)__builtin_popcount(rest));
       }
       return false;

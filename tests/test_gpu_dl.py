@@ -67,3 +67,35 @@ def test_dl_rejects_bad_parameters(torch_cuda):
         gdl.render(cam, 8, 8, 1, d_rgb.data_ptr(), 0, 4)
     with pytest.raises(sk.SkwError):
         gdl.render(cam, 0, 8, 1, d_rgb.data_ptr())
+
+
+def test_profiled_frame_is_the_render_and_counts_the_walk(torch_cuda):
+    print("case: profiled direct-lighting frame, counting walk", flush=True)
+    torch = torch_cuda
+    s = wl.scene_room_small()
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    w, h, spp, seed = 40, 32, 2, 21
+    cam, _ = sk.camera(*s["camera"], w, h)
+    a, b, c = (empty_dev(torch, w * h * 12) for _ in range(3))
+    rays = gdl.render(cam, w, h, spp, a.data_ptr(), 3, 2, seed)
+    p1 = gdl.render_profiled(cam, w, h, spp, b.data_ptr(), 3, 2, seed, False)
+    p2 = gdl.render_profiled(cam, w, h, spp, c.data_ptr(), 3, 2, seed, True)
+    torch.cuda.synchronize()
+    img = from_dev(a, np.uint32, w * h * 3)
+    assert np.array_equal(img, from_dev(b, np.uint32, w * h * 3))
+    assert np.array_equal(img, from_dev(c, np.uint32, w * h * 3))
+    assert p1["rays"] == p2["rays"] == [int(x) for x in rays]
+    assert all(v > 0 for v in p1["stage_ms"].values())
+    assert p1["node_tests"] == [0, 0, 0] and all(v > 0 for v in p2["node_tests"])
+    # the primary rays are the camera rays the batch form generates for the same view: the
+    # counting walk's totals equal the per-ray counts of skw_closest_hit_stats over them
+    batch = sk.Batch([scene], [(cam, w, h, spp, seed)])
+    n = w * h * spp
+    d_rays, d_hits = empty_dev(torch, n * 64), empty_dev(torch, n * 32)
+    d_pops, d_tests = empty_dev(torch, n * 4), empty_dev(torch, n * 4)
+    batch.rays(0, d_rays.data_ptr())
+    scene.closest_hit_stats(d_rays.data_ptr(), n, d_hits.data_ptr(), d_pops.data_ptr(), d_tests.data_ptr())
+    torch.cuda.synchronize()
+    assert p2["node_tests"][0] == int(from_dev(d_pops, np.uint32, n).astype(np.int64).sum())
+    assert p2["tri_tests"][0] == int(from_dev(d_tests, np.uint32, n).astype(np.int64).sum())
