@@ -268,6 +268,20 @@ skw_status skw_dl_adjoint(const skw_scene *scene, const skw_dl_scene *dl, const 
                           const skw_camera *cam, const skw_dl_params *params, const double *d_adjoint,
                           double *d_grad, uint64_t *rays_out, void *stream);
 
+/* Store-instead-of-recompute form of the adjoint. skw_dl_render_tape renders
+ * skw_dl_render_replay(theta, theta) into d_image and keeps each sample's derivatives on the device
+ * (a tape of (6 + 3 n_lights) doubles + 4 bytes per sample, e.g. 310 MB for 512x512 x 8 spp with
+ * 4 lights); skw_dl_tape_adjoint then contracts the tape with any dLoss/dImage (one pass over the
+ * tape, no rays), writing the same gradient skw_dl_adjoint computes. When the tape would exceed
+ * max_tape_bytes or device memory, skw_dl_render_tape returns SKW_OUT_OF_MEMORY and the caller
+ * recomputes with skw_dl_adjoint instead. */
+typedef struct skw_dl_tape skw_dl_tape;
+skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, const double *theta,
+                              const skw_camera *cam, const skw_dl_params *params, double *d_image,
+                              uint64_t max_tape_bytes, skw_dl_tape **tape, uint64_t *rays_out, void *stream);
+skw_status skw_dl_tape_adjoint(const skw_dl_tape *tape, const double *d_adjoint, double *d_grad, void *stream);
+skw_status skw_dl_tape_destroy(skw_dl_tape *tape);
+
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in
  * skw_dl_render, sample s of pixel p has index p*spp + s) are traced to their closest hit by one

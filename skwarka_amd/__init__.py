@@ -31,6 +31,7 @@ EXPORTS = [
     "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
     "skw_dl_render_profiled", "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
+    "skw_dl_render_tape", "skw_dl_tape_adjoint", "skw_dl_tape_destroy",
     "skw_batch_create", "skw_batch_create_with_gathers", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
 ]
@@ -132,6 +133,9 @@ def lib():
         "skw_dl_render_profiled": ([_P, _P, _P, _P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
         "skw_dl_render_replay": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_dl_adjoint": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_dl_render_tape": ([_P, _P, _P, _P, _P, _P, _U64, ctypes.POINTER(_P), _P, _P], ctypes.c_int),
+        "skw_dl_tape_adjoint": ([_P, _P, _P, _P], ctypes.c_int),
+        "skw_dl_tape_destroy": ([_P], ctypes.c_int),
         "skw_dl_render": ([_P, _P, ctypes.POINTER(Camera), ctypes.POINTER(DLParams), _P, _P, _P], ctypes.c_int),
         "skw_batch_create": ([_P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_batch_create_with_gathers": ([_P, _P, _U32, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
@@ -425,6 +429,18 @@ class DLScene:
                                          stream or None), "skw_dl_render_replay")
         return rays
 
+    def render_tape(self, theta, cam, width, height, spp, d_image, max_tape_bytes=1 << 34, light_samples=4,
+                    bsdf_samples=4, seed=0, stream=0):
+        """Replayed render at theta into d_image, keeping per-sample derivatives: returns a DLTape."""
+        th = self._theta(theta)
+        prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
+        rays = np.zeros(3, np.uint64)
+        t = _P()
+        check(lib().skw_dl_render_tape(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
+                                       ctypes.byref(prm), d_image, max_tape_bytes, ctypes.byref(t), rays.ctypes.data,
+                                       stream or None), "skw_dl_render_tape")
+        return DLTape(t, rays)
+
     def adjoint(self, theta, cam, width, height, spp, d_adjoint, d_grad, light_samples=4, bsdf_samples=4, seed=0,
                 stream=0):
         """d_grad (n_params doubles) = sum d_adjoint * dImage/dtheta, replaying the paths at theta."""
@@ -435,6 +451,28 @@ class DLScene:
                                    ctypes.byref(prm), d_adjoint, d_grad, rays.ctypes.data, stream or None),
               "skw_dl_adjoint")
         return rays
+
+
+class DLTape:
+    """Per-sample derivatives of one replayed frame (skw_dl_render_tape)."""
+
+    def __init__(self, handle, rays):
+        self.handle = handle
+        self.rays = rays
+
+    def adjoint(self, d_adjoint, d_grad, stream=0):
+        check(lib().skw_dl_tape_adjoint(self.handle, d_adjoint, d_grad, stream or None), "skw_dl_tape_adjoint")
+
+    def close(self):
+        if self.handle:
+            lib().skw_dl_tape_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -46,14 +46,28 @@ struct skw_dl_scene {
   }
 };
 
+// Per-sample derivatives of one replayed frame (see skw_dl_render_tape).
+struct skw_dl_tape {
+  double *d_j = nullptr;
+  uint32_t *d_group = nullptr;
+  uint32_t stride = 0, n_groups = 0, n_lights = 0, spp = 0, width = 0, height = 0;
+  uint64_t batch = 0, n_samples = 0;
+  ~skw_dl_tape() {
+    if (d_j) (void)hipFree(d_j);
+    if (d_group) (void)hipFree(d_group);
+  }
+};
+
 namespace {
 
-// What a frame produces: a float image, a double image, or (adj set) a gradient.
+// What a frame produces: a float image, a double image, (adj set) a gradient, or (tape set) a
+// double image and the tape.
 struct FrameOut {
   float *rgb = nullptr;
   double *img = nullptr;
   const double *adj = nullptr;
   double *grad = nullptr;
+  skw_dl_tape *tape = nullptr;
 };
 
 // Parameter tables of one call, placed at the front of the frame's device block.
@@ -129,7 +143,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(9 * sizeof(uint64_t));
   size_t o_film = take(npix * 3 * sizeof(double));
   size_t o_saux = 0, o_caml = 0;
-  if (adj) {
+  if (adj || out.tape) {
     o_saux = take(batch * nls * 6 * sizeof(double));
     o_caml = take(batch * sizeof(int32_t));
   }
@@ -183,11 +197,16 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   a.counts = (uint64_t *)(base + o_counts);
   a.totals = (uint64_t *)(base + o_totals);
   a.film = (double *)(base + o_film);
-  if (adj) {
+  if (adj || out.tape) {
     a.adj = out.adj;
     a.grad = out.grad;
     a.shadow_aux = (double *)(base + o_saux);
     a.cam_light = (int32_t *)(base + o_caml);
+  }
+  if (out.tape) {
+    a.tape_j = out.tape->d_j;
+    a.tape_group = out.tape->d_group;
+    a.tape_stride = out.tape->stride;
   }
   skw_status rs = SKW_OK;
   // stage boundaries of every pass: before camera, after camera / primary trace / shade /
@@ -460,6 +479,72 @@ skw_status skw_dl_adjoint(const skw_scene *scene, const skw_dl_scene *dl, const 
   out.adj = d_adjoint;
   out.grad = d_grad;
   return run_frame(scene, dl, pp, cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
+}
+
+skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, const double *theta,
+                              const skw_camera *cam, const skw_dl_params *prm, double *d_image,
+                              uint64_t max_tape_bytes, skw_dl_tape **tape, uint64_t *rays_out, void *stream) {
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  if (!d_image || !tape) return fail(SKW_INVALID_ARG, "d_image and tape must not be NULL");
+  *tape = nullptr;
+  if ((s = check_theta(dl, theta)) != SKW_OK) return s;
+  std::unique_ptr<skw_dl_tape> t(new skw_dl_tape());
+  t->n_groups = dl->n_groups;
+  t->n_lights = (uint32_t)dl->lights.size();
+  t->stride = 6 + 3 * t->n_lights;
+  t->spp = prm->spp;
+  t->width = prm->width;
+  t->height = prm->height;
+  t->batch = (uint64_t)prm->width * prm->height;
+  t->n_samples = t->batch * prm->spp;
+  const uint64_t bytes = t->n_samples * (t->stride * sizeof(double) + sizeof(uint32_t));
+  if (bytes > max_tape_bytes) return fail(SKW_OUT_OF_MEMORY, "the tape would exceed max_tape_bytes; use skw_dl_adjoint");
+  hipError_t e = hipMalloc((void **)&t->d_j, t->n_samples * t->stride * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&t->d_group, t->n_samples * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SKW_OUT_OF_MEMORY, "no device memory for the tape; use skw_dl_adjoint");
+  }
+  ParamPlan pp;
+  pp.active = true;
+  pp.alias = true;
+  pp.theta_e = theta;
+  pp.theta_s = theta;
+  FrameOut out;
+  out.img = d_image;
+  out.tape = t.get();
+  s = run_frame(scene, dl, pp, cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
+  if (s != SKW_OK) return s;
+  *tape = t.release();
+  return SKW_OK;
+}
+
+skw_status skw_dl_tape_adjoint(const skw_dl_tape *t, const double *d_adjoint, double *d_grad, void *stream) {
+  if (!t || !d_adjoint || !d_grad) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SKW_HIP(launch_dl_clear_grad(d_grad, 2 * t->n_groups + t->n_lights, st), "gradient clear");
+  DlTapeArgs a;
+  a.tape_j = t->d_j;
+  a.tape_group = t->d_group;
+  a.stride = t->stride;
+  a.n_groups = t->n_groups;
+  a.n_lights = t->n_lights;
+  a.spp = t->spp;
+  a.width = t->width;
+  a.height = t->height;
+  a.batch = t->batch;
+  a.n_samples = t->n_samples;
+  a.adj = d_adjoint;
+  a.grad = d_grad;
+  SKW_HIP(launch_dl_tape_adjoint(a, st), "tape contraction");
+  return SKW_OK;
+}
+
+skw_status skw_dl_tape_destroy(skw_dl_tape *t) {
+  if (t) (void)hipDeviceSynchronize();
+  delete t;
+  return SKW_OK;
 }
 
 }  // extern "C"

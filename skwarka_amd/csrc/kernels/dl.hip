@@ -190,7 +190,7 @@ struct Bsdf {
     double d = (e + 2.0) * kInv2Pi * pw;
     scalar = d * g / (4.0 * ci * co);
     if (DERIV) {
-      double dd = kInv2Pi * pw * (1.0 + (e + 2.0) * log(ch));
+      double dd = pw > 0.0 ? kInv2Pi * pw * (1.0 + (e + 2.0) * log(ch)) : 0.0;  // D = 0 at cos_h = 0
       dscalar = dd * g / (4.0 * ci * co);
     }
     return true;
@@ -326,6 +326,20 @@ __device__ __forceinline__ int32_t append_slot(uint64_t *counter, bool want) {
   return (int32_t)(base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// Pixel of sample slot i of a pass: 8x8 pixel tiles in row order, so the 64 camera rays of a wave
+// (and the occlusion / BSDF rays compacted from them in slot order) start from one tile. The
+// sample stream depends only on the pixel, so the mapping changes no value. Images whose sides are
+// not multiples of 8 use row order.
+__device__ __forceinline__ uint64_t pixel_of(uint32_t width, uint32_t height, uint64_t s) {
+  if ((width & 7u) != 0 || (height & 7u) != 0) return s;
+  const uint64_t tile = s >> 6, in = s & 63u, tiles_x = width >> 3;
+  const uint64_t x = (tile % tiles_x) * 8 + (in & 7u), y = (tile / tiles_x) * 8 + (in >> 3);
+  return y * width + x;
+}
+__device__ __forceinline__ uint64_t pixel_of(const DlArgs &a, uint64_t i) {
+  return pixel_of(a.width, a.height, a.pix0 + i);
+}
+
 // ------------------------------------------------------------------ kernels
 
 __global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
     a.counts[1] = 0;
   }
   if (i >= a.batch) return;
-  const uint64_t pix = a.pix0 + i;
+  const uint64_t pix = pixel_of(a, i);
   const uint32_t px = (uint32_t)(pix % a.width), py = (uint32_t)(pix / a.width);
   const uint64_t sidx = pix * a.spp + a.pass;
   double ix = (double)px + rnd(a.seed, sidx, 0), iy = (double)py + rnd(a.seed, sidx, 1);
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
     hit = h.prim != kMiss;
   }
   const uint32_t nl = a.n_lights;
-  const uint64_t pix = a.pix0 + i;
+  const uint64_t pix = pixel_of(a, i);
   const uint64_t sidx = pix * a.spp + a.pass;
   D3 o{0, 0, 0}, d{0, 0, 1}, wo{0, 0, 0}, p{0, 0, 0};
   HitGeom g;
@@ -547,8 +561,12 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
 // dL/d(reflectance g) = (dkd / pi) * le * W summed, dL/d(roughness g) = ks * dfr * le * W summed,
 // where g is the group of the primitive the camera ray hit. The sample's dLoss/dL is
 // adj[pixel] / spp. Sums go to LDS (ds_add_f64), then one global FP64 atomic per parameter per block.
-template <bool ADJ>
+// MODE 0: film. MODE 1 (ADJ): the contraction above. MODE 2 (TAPE): film, and the sample's
+// derivatives themselves (dL[c] per reflectance, roughness and each emission scale) into its
+// tape row, for a later contraction with any dLoss/dImage.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
+  constexpr bool ADJ = MODE == 1, TAPE = MODE == 2;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     a.totals[0] += a.batch;
@@ -584,7 +602,7 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
         int32_t j = a.bsdf_idx[i * a.nbs + k];
         if (j < 0) continue;
         double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
-        if (ADJ) rec[7] = -1.0;  // no light reached
+        if (ADJ || TAPE) rec[7] = -1.0;  // no light reached
         const skw_hit h2 = a.bsdf_hits[j];
         if (h2.prim == kMiss) continue;
         const DevMat &m2 = a.mats[h2.prim];
@@ -606,7 +624,7 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
         w *= rec[4];
         const double *re = a.rad_e + 3 * m2.light;
         for (int c = 0; c < 3; ++c) db[c] += rec[c] * re[c] * w;
-        if (ADJ) rec[7] = w;
+        if (ADJ || TAPE) rec[7] = w;
       }
       for (int c = 0; c < 3; ++c) cdl[c] += db[c] * (1.0 / (double)a.nbs);
     }
@@ -617,13 +635,62 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
   bool bad = isnan(L[0]) || isnan(L[1]) || isnan(L[2]) || lum < -kEps || isinf(L[0]) || isinf(L[1]) || isinf(L[2]);
   if (!ADJ) {
     if (in && !bad) {
-      double *f = a.film + 3 * (a.pix0 + i);
+      double *f = a.film + 3 * pixel_of(a, i);
       for (int c = 0; c < 3; ++c) f[c] += L[c];
+    }
+    if (TAPE && in) {
+      // row: [0..2] reflectance, [3..5] roughness, [6 + 3l + c] emission scale of light l
+      double *J = a.tape_j + (a.pass * a.batch + i) * (uint64_t)a.tape_stride;
+      for (uint32_t t = 0; t < a.tape_stride; ++t) J[t] = 0.0;
+      uint32_t grp = kMiss;
+      if (h.prim != kMiss && !bad) {
+        const DevMat &me = a.mats_e[h.prim];
+        grp = me.group;
+        double jr[3] = {0.0, 0.0, 0.0}, js[3] = {0.0, 0.0, 0.0};
+        const int32_t cl = a.cam_light[i];
+        if (cl >= 0)
+          for (int c = 0; c < 3; ++c) J[6 + 3 * cl + c] += a.rad_base[3 * cl + c];
+        if (a.n_lights > 0) {
+          const double inv_nls = 1.0 / (double)a.nls, inv_nbs = 1.0 / (double)a.nbs;
+          for (uint32_t k = 0; k < a.nls; ++k) {
+            int32_t j = a.shadow_idx[i * a.nls + k];
+            if (j < 0 || a.occluded[j]) continue;
+            const double *x = a.shadow_aux + 6 * (i * a.nls + k);
+            const uint32_t l = (uint32_t)x[5];
+            const double W = x[3] * inv_nls;
+            const double *re = a.rad_e + 3 * l, *rb = a.rad_base + 3 * l;
+            for (int c = 0; c < 3; ++c) {
+              J[6 + 3 * l + c] += x[c] * rb[c] * W;
+              jr[c] += re[c] * W;
+              js[c] += re[c] * me.ks[c] * x[4] * W;
+            }
+          }
+          for (uint32_t k = 0; k < a.nbs; ++k) {
+            int32_t j = a.bsdf_idx[i * a.nbs + k];
+            if (j < 0) continue;
+            const double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
+            if (rec[7] < 0.0) continue;
+            const uint32_t l = a.mats[a.bsdf_hits[j].prim].light;
+            const double W = rec[7] * inv_nbs;
+            const double *re = a.rad_e + 3 * l, *rb = a.rad_base + 3 * l;
+            for (int c = 0; c < 3; ++c) {
+              J[6 + 3 * l + c] += rec[c] * rb[c] * W;
+              jr[c] += re[c] * W;
+              js[c] += re[c] * me.ks[c] * rec[6] * W;
+            }
+          }
+        }
+        for (int c = 0; c < 3; ++c) {
+          J[c] = jr[c] * me.dkd * kInvPi;
+          J[3 + c] = js[c];
+        }
+      }
+      a.tape_group[a.pass * a.batch + i] = grp;
     }
     return;
   }
   if (in && h.prim != kMiss && !bad) {
-    const uint64_t pix = a.pix0 + i;
+    const uint64_t pix = pixel_of(a, i);
     const double inv_spp = 1.0 / (double)a.spp;
     double wa[3];
     for (int c = 0; c < 3; ++c) wa[c] = a.adj[3 * pix + c] * inv_spp;
@@ -681,6 +748,38 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
   for (uint32_t t = threadIdx.x; t < n_params; t += blockDim.x) {
     const double v = gsum[t];
     if (v != 0.0) unsafeAtomicAdd(a.grad + t, v);
+  }
+}
+
+// Contraction of a tape with dLoss/dImage: grad[j] += sum over samples and channels of
+// adj[pixel] / spp * row[j-part]; grid-stride, LDS sums, one global atomic per parameter per block.
+__global__ __launch_bounds__(256) void k_dl_tape_adjoint(DlTapeArgs t) {
+  __shared__ double gsum[kDlMaxParams];
+  const uint32_t G = t.n_groups, nl = t.n_lights, n_params = 2 * G + nl;
+  for (uint32_t p = threadIdx.x; p < n_params; p += blockDim.x) gsum[p] = 0.0;
+  __syncthreads();
+  const double inv_spp = 1.0 / (double)t.spp;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < t.n_samples;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t pix = pixel_of(t.width, t.height, s % t.batch);
+    double wa[3];
+    for (int c = 0; c < 3; ++c) wa[c] = t.adj[3 * pix + c] * inv_spp;
+    const double *J = t.tape_j + s * (uint64_t)t.stride;
+    const uint32_t grp = t.tape_group[s];
+    if (grp != kMiss) {
+      atomicAdd(&gsum[grp], wa[0] * J[0] + wa[1] * J[1] + wa[2] * J[2]);
+      atomicAdd(&gsum[G + grp], wa[0] * J[3] + wa[1] * J[4] + wa[2] * J[5]);
+    }
+    for (uint32_t l = 0; l < nl; ++l) {
+      const double *e = J + 6 + 3 * l;
+      const double v = wa[0] * e[0] + wa[1] * e[1] + wa[2] * e[2];
+      if (v != 0.0) atomicAdd(&gsum[2 * G + l], v);
+    }
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < n_params; p += blockDim.x) {
+    const double v = gsum[p];
+    if (v != 0.0) unsafeAtomicAdd(t.grad + p, v);
   }
 }
 
@@ -779,7 +878,7 @@ hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream) {
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 63) / 64);
   if (grid == 0) return hipSuccess;
-  if (a.adj) hipLaunchKernelGGL((k_dl_shade<false, true>), dim3(grid), dim3(64), 0, stream, a);
+  if (a.adj || a.tape_j) hipLaunchKernelGGL((k_dl_shade<false, true>), dim3(grid), dim3(64), 0, stream, a);
   else if (a.mats_e != a.mats) hipLaunchKernelGGL((k_dl_shade<true, false>), dim3(grid), dim3(64), 0, stream, a);
   else hipLaunchKernelGGL((k_dl_shade<false, false>), dim3(grid), dim3(64), 0, stream, a);
   return hipGetLastError();
@@ -787,8 +886,17 @@ hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
 
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 255) / 256);
-  if (a.adj) hipLaunchKernelGGL((k_dl_resolve<true>), dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
-  else hipLaunchKernelGGL((k_dl_resolve<false>), dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, a);
+  dim3 g(grid > 0 ? grid : 1);
+  if (a.adj) hipLaunchKernelGGL((k_dl_resolve<1>), g, dim3(256), 0, stream, a);
+  else if (a.tape_j) hipLaunchKernelGGL((k_dl_resolve<2>), g, dim3(256), 0, stream, a);
+  else hipLaunchKernelGGL((k_dl_resolve<0>), g, dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_tape_adjoint(const DlTapeArgs &t, hipStream_t stream) {
+  uint64_t blocks = (t.n_samples + 255) / 256;
+  uint32_t grid = (uint32_t)(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+  hipLaunchKernelGGL(k_dl_tape_adjoint, dim3(grid), dim3(256), 0, stream, t);
   return hipGetLastError();
 }
 

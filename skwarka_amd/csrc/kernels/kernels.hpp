@@ -152,7 +152,22 @@ struct DlArgs {
   double *grad;          // 2 * n_groups + n_lights
   double *shadow_aux;    // 6 per (sample, k): f rgb (evaluation), weight, d f / d roughness, light
   int32_t *cam_light;    // per sample: light seen by the camera ray, or -1
+  // tape (when tape_j is set): per sample of the frame (pass * batch + slot) a row of tape_stride
+  // derivatives and the hit primitive's group
+  double *tape_j;
+  uint32_t *tape_group;
+  uint32_t tape_stride;
 };
+
+struct DlTapeArgs {
+  const double *tape_j;
+  const uint32_t *tape_group;
+  uint32_t stride, n_groups, n_lights, spp, width, height;
+  uint64_t batch, n_samples;
+  const double *adj;
+  double *grad;
+};
+hipError_t launch_dl_tape_adjoint(const DlTapeArgs &t, hipStream_t stream);
 constexpr uint32_t kDlMaxGroups = 64;
 constexpr uint32_t kDlMaxParams = 2 * kDlMaxGroups + 16;
 

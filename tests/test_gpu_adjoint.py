@@ -141,3 +141,49 @@ def test_adjoint_repeats_and_rejects_bad_theta(torch_cuda):
         gdl.adjoint(bad, cam, W, H, SPP, adj.data_ptr(), g1.data_ptr(), NLS, NBS, SEED)
     with pytest.raises(ValueError):
         gdl.adjoint(s["theta0"][:10], cam, W, H, SPP, adj.data_ptr(), g1.data_ptr(), NLS, NBS, SEED)
+
+
+def test_tape_adjoint_equals_recomputed_adjoint(torch_cuda):
+    print("case: stored-derivative (tape) adjoint vs recomputed adjoint, autograd through the tape", flush=True)
+    torch = torch_cuda
+    s, scene, gdl, cam, cam15 = _setup(torch)
+    th0 = s["theta0"]
+    img = torch.empty((W * H, 3), dtype=torch.float64, device="cuda")
+    tape = gdl.render_tape(th0, cam, W, H, SPP, img.data_ptr(), 1 << 30, NLS, NBS, SEED)
+    torch.cuda.synchronize()
+    assert torch.equal(img, _replay(torch, gdl, cam, th0, th0))
+    rng = np.random.default_rng(7)
+    adj = torch.from_numpy(rng.normal(size=(W * H, 3))).cuda()
+    g_tape = torch.empty(gdl.n_params, dtype=torch.float64, device="cuda")
+    g_rep = torch.empty_like(g_tape)
+    for _ in range(2):  # the tape serves any number of adjoints
+        tape.adjoint(adj.data_ptr(), g_tape.data_ptr())
+        gdl.adjoint(th0, cam, W, H, SPP, adj.data_ptr(), g_rep.data_ptr(), NLS, NBS, SEED)
+        torch.cuda.synchronize()
+        a, b = g_tape.cpu().numpy(), g_rep.cpu().numpy()
+        assert np.all(np.abs(a - b) <= 1e-10 * np.abs(b).max()), np.abs(a - b).max()
+    tape.close()
+    # a tape larger than the budget is refused, so the caller recomputes instead
+    with pytest.raises(sk.SkwError, match="max_tape_bytes"):
+        gdl.render_tape(th0, cam, W, H, SPP, img.data_ptr(), 1024, NLS, NBS, SEED)
+
+    class TapedDirectLighting(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta):
+            out = torch.empty((W * H, 3), dtype=torch.float64, device="cuda")
+            ctx.tape = gdl.render_tape(theta.detach().cpu().numpy(), cam, W, H, SPP, out.data_ptr(), 1 << 30, NLS,
+                                       NBS, SEED)
+            return out
+
+        @staticmethod
+        def backward(ctx, grad_img):
+            grad = torch.empty(gdl.n_params, dtype=torch.float64, device="cuda")
+            ctx.tape.adjoint(grad_img.contiguous().data_ptr(), grad.data_ptr())
+            torch.cuda.synchronize()
+            ctx.tape.close()
+            return grad
+
+    theta = torch.tensor(th0, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = (TapedDirectLighting.apply(theta) * adj).sum()
+    loss.backward()
+    assert np.all(np.abs(theta.grad.cpu().numpy() - b) <= 1e-10 * np.abs(b).max())
