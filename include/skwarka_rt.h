@@ -274,7 +274,8 @@ skw_status skw_dl_adjoint(const skw_scene *scene, const skw_dl_scene *dl, const 
  * 4 lights); skw_dl_tape_adjoint then contracts the tape with any dLoss/dImage (one pass over the
  * tape, no rays), writing the same gradient skw_dl_adjoint computes. When the tape would exceed
  * max_tape_bytes or device memory, skw_dl_render_tape returns SKW_OUT_OF_MEMORY and the caller
- * recomputes with skw_dl_adjoint instead. */
+ * recomputes with skw_dl_adjoint instead. *tape: NULL to allocate a new tape, or a tape of the same
+ * frame shape to overwrite (no allocation). */
 typedef struct skw_dl_tape skw_dl_tape;
 skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, const double *theta,
                               const skw_camera *cam, const skw_dl_params *params, double *d_image,

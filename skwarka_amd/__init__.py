@@ -430,15 +430,19 @@ class DLScene:
         return rays
 
     def render_tape(self, theta, cam, width, height, spp, d_image, max_tape_bytes=1 << 34, light_samples=4,
-                    bsdf_samples=4, seed=0, stream=0):
-        """Replayed render at theta into d_image, keeping per-sample derivatives: returns a DLTape."""
+                    bsdf_samples=4, seed=0, stream=0, tape=None):
+        """Replayed render at theta into d_image, keeping per-sample derivatives: returns a DLTape
+        (tape: an earlier DLTape of the same frame shape to overwrite)."""
         th = self._theta(theta)
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
         rays = np.zeros(3, np.uint64)
-        t = _P()
+        t = _P(tape.handle if tape is not None else None)
         check(lib().skw_dl_render_tape(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
                                        ctypes.byref(prm), d_image, max_tape_bytes, ctypes.byref(t), rays.ctypes.data,
                                        stream or None), "skw_dl_render_tape")
+        if tape is not None:
+            tape.rays = rays
+            return tape
         return DLTape(t, rays)
 
     def adjoint(self, theta, cam, width, height, spp, d_adjoint, d_grad, light_samples=4, bsdf_samples=4, seed=0,

@@ -487,24 +487,35 @@ skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, co
   skw_status s = check_frame_args(scene, dl, cam, prm);
   if (s != SKW_OK) return s;
   if (!d_image || !tape) return fail(SKW_INVALID_ARG, "d_image and tape must not be NULL");
-  *tape = nullptr;
   if ((s = check_theta(dl, theta)) != SKW_OK) return s;
-  std::unique_ptr<skw_dl_tape> t(new skw_dl_tape());
-  t->n_groups = dl->n_groups;
-  t->n_lights = (uint32_t)dl->lights.size();
-  t->stride = 6 + 3 * t->n_lights;
-  t->spp = prm->spp;
-  t->width = prm->width;
-  t->height = prm->height;
-  t->batch = (uint64_t)prm->width * prm->height;
-  t->n_samples = t->batch * prm->spp;
-  const uint64_t bytes = t->n_samples * (t->stride * sizeof(double) + sizeof(uint32_t));
-  if (bytes > max_tape_bytes) return fail(SKW_OUT_OF_MEMORY, "the tape would exceed max_tape_bytes; use skw_dl_adjoint");
-  hipError_t e = hipMalloc((void **)&t->d_j, t->n_samples * t->stride * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void **)&t->d_group, t->n_samples * sizeof(uint32_t));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(SKW_OUT_OF_MEMORY, "no device memory for the tape; use skw_dl_adjoint");
+  const uint32_t n_lights = (uint32_t)dl->lights.size();
+  const uint64_t batch = (uint64_t)prm->width * prm->height;
+  skw_dl_tape *old = *tape;
+  if (old && (old->n_groups != dl->n_groups || old->n_lights != n_lights || old->spp != prm->spp ||
+              old->width != prm->width || old->height != prm->height))
+    return fail(SKW_INVALID_ARG, "the tape to reuse has another frame shape");
+  std::unique_ptr<skw_dl_tape> fresh;
+  skw_dl_tape *t = old;
+  if (!t) {
+    fresh.reset(new skw_dl_tape());
+    t = fresh.get();
+    t->n_groups = dl->n_groups;
+    t->n_lights = n_lights;
+    t->stride = 6 + 3 * n_lights;
+    t->spp = prm->spp;
+    t->width = prm->width;
+    t->height = prm->height;
+    t->batch = batch;
+    t->n_samples = batch * prm->spp;
+    const uint64_t bytes = t->n_samples * (t->stride * sizeof(double) + sizeof(uint32_t));
+    if (bytes > max_tape_bytes)
+      return fail(SKW_OUT_OF_MEMORY, "the tape would exceed max_tape_bytes; use skw_dl_adjoint");
+    hipError_t e = hipMalloc((void **)&t->d_j, t->n_samples * t->stride * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&t->d_group, t->n_samples * sizeof(uint32_t));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SKW_OUT_OF_MEMORY, "no device memory for the tape; use skw_dl_adjoint");
+    }
   }
   ParamPlan pp;
   pp.active = true;
@@ -513,10 +524,10 @@ skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, co
   pp.theta_s = theta;
   FrameOut out;
   out.img = d_image;
-  out.tape = t.get();
+  out.tape = t;
   s = run_frame(scene, dl, pp, cam, prm, out, rays_out, reinterpret_cast<hipStream_t>(stream));
   if (s != SKW_OK) return s;
-  *tape = t.release();
+  if (fresh) *tape = fresh.release();
   return SKW_OK;
 }
 

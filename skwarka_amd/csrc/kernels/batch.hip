@@ -7,12 +7,9 @@
 // generated in the kernel (no ray buffer): PerspectiveCamera::GenerateRay
 // (Raytracer/Cameras/PerspectiveCamera.cpp:36-64) at pixel + splitmix64 jitter, the same stream as
 // the direct-lighting camera. The walk is trace_device.hpp's (TriangleAccelerator::_NodeIntersect
-// semantics). Photon-gather jobs (k nearest + disc estimate, gather_device.hpp) share the queue:
-// their chunks are query ranges. When an item's last chunk completes, the wave records the wall
-// clock for it.
+// semantics). When a scene's last chunk completes, the wave records the wall clock for it.
 #include <hip/hip_runtime.h>
 
-#include "gather_device.hpp"
 #include "geom_device.hpp"
 #include "kernels.hpp"
 #include "trace_device.hpp"
@@ -67,39 +64,45 @@ __device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint6
   }
 }
 
-// The queries of one gather chunk, 64 at a time.
-template <int S>
-__device__ __forceinline__ void gather_chunk(const GatherArgs &ga, uint64_t begin, uint64_t end, uint32_t lane,
-                                             uint32_t *lds_row) {
-  gath::LdsStack<S> gstk{lds_row};
-  for (uint64_t qi = begin + lane; qi < end; qi += 64) gath::gather_one<0>(ga, qi, gstk);
-}
-
-// The rays of one scene chunk: lanes refill from the chunk as their walks end (ballot + prefix
-// rank), so the wave waits for stragglers only at the chunk's end.
-template <int S>
-__device__ __forceinline__ void ray_chunk(const BatchArgs &a, const DevBatchScene &sc, uint64_t begin, uint64_t end,
-                                          uint32_t lane, const LdsStack<S> &stk) {
+template <int S, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
+  __shared__ uint32_t lds[S * 64 * WAVES];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  LdsStack<S> stk{lds + wave * (S * 64) + lane};
   Walker<false, false, false, LdsStack<S>> w;
-  w.nodes = sc.nodes;
-  w.tris = sc.tris;
   w.insts = nullptr;
   w.inst_range = nullptr;
-  w.root = sc.root;
   w.stk = stk;
-  skw_hit *hits = a.hits + sc.hit_offset;
   bool busy = false;
-  uint64_t idx = 0, cursor = begin;
+  uint64_t idx = 0, cursor = 0, end = 0, chunk_len = 0;
+  uint32_t sid = 0xFFFFFFFFu;
+  const DevBatchScene *sc = nullptr;
   while (true) {
     const uint64_t idle = __ballot(!busy);
-    if (idle == ~0ull && cursor >= end) break;
+    if (idle == ~0ull && cursor >= end) {
+      // The wave's chunk is done: account for it, then take the next one.
+      if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lane);
+      unsigned long long q = 0;
+      if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
+      q = __shfl(q, 0);
+      if (q >= a.n_chunks) break;
+      const DevChunk c = a.chunks[q];
+      sid = c.item;
+      cursor = c.begin;
+      end = c.end;
+      chunk_len = c.end - c.begin;
+      sc = a.scenes + sid;
+      w.nodes = sc->nodes;
+      w.tris = sc->tris;
+      w.root = sc->root;
+    }
     if (idle != 0 && cursor < end) {
       const uint64_t avail = end - cursor;
       const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
       if (!busy && rank < avail) {
         idx = cursor + rank;
         RayState r;
-        camera_ray(sc, idx, r);
+        camera_ray(*sc, idx, r);
         w.start(r, __builtin_huge_val());
         busy = true;
       }
@@ -107,34 +110,8 @@ __device__ __forceinline__ void ray_chunk(const BatchArgs &a, const DevBatchScen
       cursor += took < avail ? took : avail;
     }
     if (busy && w.step()) {
-      store_hit(hits + idx, w.tris, w.found, w.best, w.tmax, w.r);
+      store_hit(a.hits + sc->hit_offset + idx, w.tris, w.found, w.best, w.tmax, w.r);
       busy = false;
-    }
-  }
-}
-
-// GATHER: the queue holds gather chunks too (a separate instance, since the gather's registers
-// would lower the occupancy of ray-only batches).
-template <int S, int WAVES, bool GATHER>
-__global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
-  __shared__ uint32_t lds[S * 64 * WAVES];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const LdsStack<S> stk{lds + wave * (S * 64) + lane};
-  while (true) {
-    unsigned long long q = 0;
-    if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
-    q = __shfl(q, 0);
-    if (q >= a.n_chunks) break;
-    const DevChunk c = a.chunks[q];
-    if (GATHER && c.item >= a.n_scenes) {
-      // A photon-gather chunk; its lookup stack uses the same LDS rows as the ray walk.
-      const GatherArgs &ga = a.jobs[c.item - a.n_scenes];
-      gather_chunk<S>(ga, c.begin, c.end, lane, stk.base);
-      account(a, c.item, c.end - c.begin, ga.nq, lane);
-    } else {
-      const DevBatchScene &sc = a.scenes[c.item];
-      ray_chunk<S>(a, sc, c.begin, c.end, lane, stk);
-      account(a, c.item, c.end - c.begin, sc.n_rays, lane);
     }
   }
 }
@@ -165,13 +142,7 @@ hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream) {
 
 hipError_t launch_batch_trace(const BatchArgs &a, uint32_t waves, uint32_t stack, uint32_t grid, hipStream_t stream) {
   dim3 g(grid);
-#define SKW_BATCH_LAUNCH(S, W)                                                                  \
-  do {                                                                                          \
-    if (a.n_jobs > 0)                                                                           \
-      hipLaunchKernelGGL((k_batch_trace<S, W, true>), g, dim3(64 * (W)), 0, stream, a);         \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_batch_trace<S, W, false>), g, dim3(64 * (W)), 0, stream, a);        \
-  } while (0)
+#define SKW_BATCH_LAUNCH(S, W) hipLaunchKernelGGL((k_batch_trace<S, W>), g, dim3(64 * (W)), 0, stream, a)
   if (stack <= 32) {
     if (waves == 1) SKW_BATCH_LAUNCH(32, 1);
     else if (waves == 2) SKW_BATCH_LAUNCH(32, 2);
