@@ -436,7 +436,7 @@ class DLScene:
         th = self._theta(theta)
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
         rays = np.zeros(3, np.uint64)
-        t = _P(tape.handle if tape is not None else None)
+        t = _P(tape.handle.value if tape is not None else None)
         check(lib().skw_dl_render_tape(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
                                        ctypes.byref(prm), d_image, max_tape_bytes, ctypes.byref(t), rays.ctypes.data,
                                        stream or None), "skw_dl_render_tape")
