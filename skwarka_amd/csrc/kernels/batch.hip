@@ -92,9 +92,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
       end = c.end;
       chunk_len = c.end - c.begin;
       sc = a.scenes + sid;
-      w.nodes = sc->nodes;
+      w.set_root(sc->nodes, sc->root);
       w.tris = sc->tris;
-      w.root = sc->root;
     }
     if (idle != 0 && cursor < end) {
       const uint64_t avail = end - cursor;

@@ -57,6 +57,11 @@ __device__ __forceinline__ void load_node(const DevNode *__restrict__ nodes, uin
   hi = p[1];
 }
 
+// Makes a loaded node word live at this point, so the compiler issues the whole 32-byte node load
+// at once instead of sinking the child words' load into the branch that reads them (which would
+// add a second dependent memory round trip to the step). Emits no instruction.
+__device__ __forceinline__ void keep_live(const float4 &v) { asm volatile("" ::"v"(v.z), "v"(v.w)); }
+
 // true = culled. TriangleAccelerator.cpp:176-192.
 __device__ __forceinline__ bool slab_cull(const float4 &lo, const float4 &hi, const RayState &r, double tmax) {
   double tn1 = ((double)lo.x - r.o0) * r.i0;
@@ -75,11 +80,26 @@ __device__ __forceinline__ bool slab_cull(const float4 &lo, const float4 &hi, co
          r.tmin > b3 || a1 > tmax || a2 > tmax || a3 > tmax;
 }
 
-// TriangleAccelerator.cpp:238-274. Returns true with t, b1, b2 when accepted in [tmin, tmax].
-__device__ __forceinline__ bool tri_test(const DevTri *__restrict__ tris, uint32_t pos, const RayState &r,
-                                         double tmax, double &t_out, double &b1_out, double &b2_out) {
+// A triangle's nine float vertex coordinates (the first 36 bytes of its 48-byte record).
+struct TriVerts {
+  float4 q0, q1;
+  float q2x;
+};
+
+__device__ __forceinline__ TriVerts load_tri(const DevTri *__restrict__ tris, uint32_t pos) {
   const float4 *p = reinterpret_cast<const float4 *>(tris + pos);
-  float4 q0 = p[0], q1 = p[1], q2 = p[2];
+  TriVerts v;
+  v.q0 = p[0];
+  v.q1 = p[1];
+  v.q2x = reinterpret_cast<const float *>(p + 2)[0];
+  return v;
+}
+
+// TriangleAccelerator.cpp:238-274. Returns true with t, b1, b2 when accepted in [tmin, tmax].
+__device__ __forceinline__ bool tri_test_verts(const TriVerts &tv, const RayState &r, double tmax, double &t_out,
+                                               double &b1_out, double &b2_out) {
+  const float4 q0 = tv.q0, q1 = tv.q1;
+  const float4 q2 = make_float4(tv.q2x, 0.f, 0.f, 0.f);
   double v0x = q0.x, v0y = q0.y, v0z = q0.z;
   double e1x = (double)q0.w - v0x, e1y = (double)q1.x - v0y, e1z = (double)q1.y - v0z;
   double e2x = (double)q1.z - v0x, e2y = (double)q1.w - v0y, e2z = (double)q2.x - v0z;
@@ -108,6 +128,11 @@ __device__ __forceinline__ bool tri_test(const DevTri *__restrict__ tris, uint32
     }
   }
   return false;
+}
+
+__device__ __forceinline__ bool tri_test(const DevTri *__restrict__ tris, uint32_t pos, const RayState &r,
+                                         double tmax, double &t_out, double &b1_out, double &b2_out) {
+  return tri_test_verts(load_tri(tris, pos), r, tmax, t_out, b1_out, b2_out);
 }
 
 __device__ __forceinline__ void make_state(RayState &s, double o0, double o1, double o2, double d0, double d1,
@@ -323,6 +348,7 @@ struct Walker {
   const DevInst *__restrict__ insts;
   const uint32_t *__restrict__ inst_range;
   uint32_t root;
+  float4 root_lo, root_hi;  // the root's box and child words, loaded once per tree by set_root()
   Stack stk;
   RayState r;
   double tmax;
@@ -330,6 +356,12 @@ struct Walker {
   uint32_t sp, cur, ca, cb, dpos;
   bool have, found;
   uint32_t pops, tests;
+
+  __device__ __forceinline__ void set_root(const DevNode *__restrict__ tree_nodes, uint32_t root_idx) {
+    nodes = tree_nodes;
+    root = root_idx;
+    load_node(tree_nodes, root_idx, root_lo, root_hi);
+  }
 
   __device__ __forceinline__ void start(const RayState &ray, double tmax0) {
     r = ray;
@@ -342,13 +374,11 @@ struct Walker {
     pops = 0;
     tests = 0;
     // The root is tested like any popped node (TriangleAccelerator.cpp:165-192).
-    float4 lo, hi;
-    load_node(nodes, root, lo, hi);
     if (STATS) ++pops;
-    have = !slab_cull(lo, hi, r, tmax);
+    have = !slab_cull(root_lo, root_hi, r, tmax);
     cur = root;
-    ca = __float_as_uint(hi.z);
-    cb = __float_as_uint(hi.w);
+    ca = __float_as_uint(root_hi.z);
+    cb = __float_as_uint(root_hi.w);
   }
 
   __device__ __forceinline__ bool step() {
@@ -365,6 +395,7 @@ struct Walker {
       }
       float4 lo, hi;
       load_node(nodes, cur, lo, hi);
+      keep_live(hi);
       if (slab_cull(lo, hi, r, tmax)) return false;
       ca = __float_as_uint(hi.z);
       cb = __float_as_uint(hi.w);
@@ -376,16 +407,22 @@ struct Walker {
       const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
       const bool fwd = (dpos >> (kind - 1u)) & 1u;
       // Children in visiting order (scalars, not arrays: an indexed array would live in scratch).
+      // Absent children alias c0, so the three loads are issued together and wait once.
       const uint32_t c0 = fwd ? a : a + nch - 1u;
-      const uint32_t c1 = fwd ? a + 1u : a + nch - 2u;
-      const uint32_t c2 = a + (fwd ? 2u : 0u);
-      float4 lo0, hi0, lo1 = make_float4(0, 0, 0, 0), hi1 = lo1, lo2 = lo1, hi2 = lo1;
+      const uint32_t c1 = nch > 1 ? (fwd ? a + 1u : a + nch - 2u) : c0;
+      const uint32_t c2 = nch > 2 ? a + (fwd ? 2u : 0u) : c0;
+      float4 lo0, hi0, lo1, hi1, lo2, hi2;
       load_node(nodes, c0, lo0, hi0);
-      if (nch > 1) load_node(nodes, c1, lo1, hi1);
-      if (nch > 2) load_node(nodes, c2, lo2, hi2);
-      uint32_t surv = slab_cull(lo0, hi0, r, tmax) ? 0u : 1u;
-      if (nch > 1 && !slab_cull(lo1, hi1, r, tmax)) surv |= 2u;
-      if (nch > 2 && !slab_cull(lo2, hi2, r, tmax)) surv |= 4u;
+      load_node(nodes, c1, lo1, hi1);
+      load_node(nodes, c2, lo2, hi2);
+      keep_live(hi0);
+      keep_live(hi1);
+      keep_live(hi2);
+      // All three tests are evaluated (no short circuit), so no load is sunk into a branch.
+      const uint32_t k0 = slab_cull(lo0, hi0, r, tmax) ? 0u : 1u;
+      const uint32_t k1 = slab_cull(lo1, hi1, r, tmax) ? 0u : 2u;
+      const uint32_t k2 = slab_cull(lo2, hi2, r, tmax) ? 0u : 4u;
+      const uint32_t surv = (k0 | k1 | k2) & ((1u << nch) - 1u);
       if (STATS) pops += nch;
       if (surv == 0) return false;
       const uint32_t s0 = __builtin_ctz(surv);
@@ -426,11 +463,17 @@ struct Walker {
         }
       }
     }
+    // The leaf's triangles are contiguous: the next one's vertices are loaded while the current
+    // one is tested (the last load re-reads the current triangle instead of running past the leaf).
     const uint32_t cnt = b >> 3;
+    TriVerts nv;
+    if (cnt) nv = load_tri(tris, a);  // an empty leaf's triangle index may be past the array
     for (uint32_t k = 0; k < cnt && !stop; ++k) {
       double t, b1, b2;
       if (STATS) ++tests;
-      if (tri_test(tris, a + k, r, tmax, t, b1, b2)) {
+      const TriVerts tv = nv;
+      nv = load_tri(tris, a + (k + 1 < cnt ? k + 1 : k));
+      if (tri_test_verts(tv, r, tmax, t, b1, b2)) {
         found = true;
         if (ANY) {
           stop = true;

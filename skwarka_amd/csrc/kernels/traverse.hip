@@ -3,10 +3,11 @@
 //
 // Scheduling: one ray per lane, 64-lane workgroups, a persistent grid sized to the chip. Every
 // iteration each lane makes one step of its own walk; a lane whose walk has ended stores its
-// result and takes the next unclaimed ray: the wave's idle lanes are counted with a ballot, one
-// lane does a single atomicAdd on the launch counter, and each idle lane gets base + (its rank
-// among idle lanes). The first gridDim*blockDim rays are handed out statically; the counter starts
-// at that value. Rays are 64-byte records read as four 16-byte loads.
+// result and takes the next unstarted ray of its wave's pool: the wave's idle lanes are counted
+// with a ballot and each gets pool + (its rank among idle lanes). When the pool runs short, one
+// lane claims the next kClaim rays with a single atomicAdd on the launch counter. The first
+// gridDim*blockDim rays are handed out statically; the counter starts at that value. Rays are
+// 64-byte records read as four 16-byte loads.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -16,6 +17,9 @@ namespace skw {
 namespace {
 
 using namespace trace;
+
+// Rays a wave claims from the launch counter at once (one atomic per kClaim refills).
+constexpr uint64_t kClaim = 64;
 
 // STATS: count node boxes and triangles tested per ray (the oracle counts the same events).
 template <bool ANY, bool INST, bool STATS, class Stack>
@@ -27,23 +31,34 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
   bool first = true;
   unsigned long long sum_pops = 0, sum_tests = 0;
   Walker<ANY, INST, STATS, Stack> w;
-  w.nodes = args.nodes;
+  w.set_root(args.nodes, args.root);
   w.tris = args.tris;
   w.insts = args.insts;
   w.inst_range = args.inst_range;
-  w.root = args.root;
   w.stk = stk;
+  // The wave's pool of claimed, unstarted rays [pool, pool_end): the same value in every lane.
+  uint64_t pool = 0, pool_end = 0;
   while (true) {
     if (!first) {
       const uint64_t need = __ballot(!busy);
       if (need) {
-        const int leader = __ffsll((unsigned long long)need) - 1;
-        unsigned long long base = 0;
-        if ((int)lane == leader)
-          base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter),
-                           (unsigned long long)__popcll(need));
-        base = __shfl(base, leader);
-        if (!busy) idx = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        const uint64_t n_idle = (uint64_t)__popcll(need);
+        const uint64_t rank = (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        const uint64_t avail = pool_end - pool;
+        if (avail >= n_idle) {
+          if (!busy) idx = pool + rank;
+          pool += n_idle;
+        } else {
+          // Idle lanes take the rest of the pool, then the start of a newly claimed chunk.
+          const int leader = __ffsll((unsigned long long)need) - 1;
+          unsigned long long base = 0;
+          if ((int)lane == leader)
+            base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter), (unsigned long long)kClaim);
+          base = __shfl(base, leader);
+          if (!busy) idx = rank < avail ? pool + rank : base + (rank - avail);
+          pool = base + (n_idle - avail);
+          pool_end = base + kClaim;
+        }
       }
     }
     first = false;
