@@ -396,11 +396,14 @@ class DLScene:
         except Exception:
             pass
 
-    def render(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0, stream=0):
+    def render(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0, stream=0,
+               count_rays=True):
+        """Renders into d_rgb. With count_rays the call waits for the frame and returns the
+        primary / shadow / BSDF ray counts; without, it only enqueues the frame and returns None."""
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
-        rays = np.zeros(3, np.uint64)
+        rays = np.zeros(3, np.uint64) if count_rays else None
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
-                                  rays.ctypes.data, stream or None), "skw_dl_render")
+                                  rays.ctypes.data if count_rays else None, stream or None), "skw_dl_render")
         return rays
 
     def render_profiled(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0,
@@ -419,41 +422,45 @@ class DLScene:
         return t
 
     def render_replay(self, theta_eval, theta_sample, cam, width, height, spp, d_image, light_samples=4,
-                      bsdf_samples=4, seed=0, stream=0):
-        """Replayed estimator F(theta_eval; theta_sample) into d_image (width*height*3 doubles)."""
+                      bsdf_samples=4, seed=0, stream=0, count_rays=True):
+        """Replayed estimator F(theta_eval; theta_sample) into d_image (width*height*3 doubles).
+        count_rays: as in render()."""
         te, ts = self._theta(theta_eval), self._theta(theta_sample)
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
-        rays = np.zeros(3, np.uint64)
+        rays = np.zeros(3, np.uint64) if count_rays else None
         check(lib().skw_dl_render_replay(self.scene.handle, self.handle, te.ctypes.data, ts.ctypes.data,
-                                         ctypes.byref(cam), ctypes.byref(prm), d_image, rays.ctypes.data,
-                                         stream or None), "skw_dl_render_replay")
+                                         ctypes.byref(cam), ctypes.byref(prm), d_image,
+                                         rays.ctypes.data if count_rays else None, stream or None),
+              "skw_dl_render_replay")
         return rays
 
     def render_tape(self, theta, cam, width, height, spp, d_image, max_tape_bytes=1 << 34, light_samples=4,
-                    bsdf_samples=4, seed=0, stream=0, tape=None):
+                    bsdf_samples=4, seed=0, stream=0, tape=None, count_rays=True):
         """Replayed render at theta into d_image, keeping per-sample derivatives: returns a DLTape
-        (tape: an earlier DLTape of the same frame shape to overwrite)."""
+        (tape: an earlier DLTape of the same frame shape to overwrite). count_rays: as in render()."""
         th = self._theta(theta)
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
-        rays = np.zeros(3, np.uint64)
+        rays = np.zeros(3, np.uint64) if count_rays else None
         t = _P(tape.handle.value if tape is not None else None)
         check(lib().skw_dl_render_tape(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
-                                       ctypes.byref(prm), d_image, max_tape_bytes, ctypes.byref(t), rays.ctypes.data,
-                                       stream or None), "skw_dl_render_tape")
+                                       ctypes.byref(prm), d_image, max_tape_bytes, ctypes.byref(t),
+                                       rays.ctypes.data if count_rays else None, stream or None),
+              "skw_dl_render_tape")
         if tape is not None:
             tape.rays = rays
             return tape
         return DLTape(t, rays)
 
     def adjoint(self, theta, cam, width, height, spp, d_adjoint, d_grad, light_samples=4, bsdf_samples=4, seed=0,
-                stream=0):
-        """d_grad (n_params doubles) = sum d_adjoint * dImage/dtheta, replaying the paths at theta."""
+                stream=0, count_rays=True):
+        """d_grad (n_params doubles) = sum d_adjoint * dImage/dtheta, replaying the paths at theta.
+        count_rays: as in render()."""
         th = self._theta(theta)
         prm = DLParams(width, height, spp, light_samples, bsdf_samples, seed)
-        rays = np.zeros(3, np.uint64)
+        rays = np.zeros(3, np.uint64) if count_rays else None
         check(lib().skw_dl_adjoint(self.scene.handle, self.handle, th.ctypes.data, ctypes.byref(cam),
-                                   ctypes.byref(prm), d_adjoint, d_grad, rays.ctypes.data, stream or None),
-              "skw_dl_adjoint")
+                                   ctypes.byref(prm), d_adjoint, d_grad, rays.ctypes.data if count_rays else None,
+                                   stream or None), "skw_dl_adjoint")
         return rays
 
 

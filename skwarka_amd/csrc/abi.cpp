@@ -46,6 +46,52 @@ int env_int(const char *name, int dflt) {
   return std::atoi(v);
 }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pools[kMaxDevices] = {};
+}  // namespace
+
+hipError_t pool_alloc(void **p, size_t bytes, hipStream_t stream) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  hipMemPool_t pool = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pools[dev]) {
+      hipMemPoolProps props;
+      std::memset(&props, 0, sizeof(props));
+      props.allocType = hipMemAllocationTypePinned;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      hipMemPool_t created = nullptr;
+      e = hipMemPoolCreate(&created, &props);
+      if (e != hipSuccess) return e;
+      uint64_t keep = UINT64_MAX;
+      e = hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) {
+        (void)hipMemPoolDestroy(created);
+        return e;
+      }
+      g_pools[dev] = created;
+    }
+    pool = g_pools[dev];
+  }
+  return hipMallocFromPoolAsync(p, bytes, pool, stream);
+}
+
+void pool_free(void *p, hipStream_t stream) {
+  if (p) (void)hipFreeAsync(p, stream);
+}
+
+static int clamp_int(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Lanes a trace wave gathers before it takes new rays / tests leaves (see traverse.hip).
+constexpr int kTraceRefillMin = 8;
+constexpr int kTraceLeafMin = 16;
+
 CounterPool::~CounterPool() {
   for (auto &kv : by_stream) (void)hipFree(kv.second);
 }
@@ -89,16 +135,18 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.pops = d_pops;
   a.tests = d_tests;
   a.sums = d_sums;
+  a.refill_min = (uint32_t)clamp_int(env_int("SKW_TRACE_REFILL_MIN", kTraceRefillMin), 1, 64);
+  a.leaf_min = (uint32_t)clamp_int(env_int("SKW_TRACE_LEAF_MIN", kTraceLeafMin), 1, 64);
   const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
   SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
   uint32_t need = s->host.stack_need;
   if (need > 128) {
-    SKW_HIP(hipMallocAsync((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
+    SKW_HIP(pool_alloc((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
   }
   SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
   hipError_t e = any ? launch_any_hit(a, s->has_inst, stats, need, grid, st)
                      : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
-  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
+  pool_free(a.gstack, st);
   if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
   if (!any && fix && s->needs_fix) {
     GeomArgs g = s->geom_args();
@@ -383,25 +431,25 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.caustic = est.caustic;
   if (mode == 0 && k > 0) {
     size_t elems = (size_t)nq * k;
-    SKW_HIP(hipMallocAsync((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
-    hipError_t e = hipMallocAsync((void **)&a.heap_d2, elems * sizeof(double), st);
+    SKW_HIP(pool_alloc((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
+    hipError_t e = pool_alloc((void **)&a.heap_d2, elems * sizeof(double), st);
     if (e != hipSuccess) {
-      (void)hipFreeAsync(a.heap_pos, st);
+      pool_free(a.heap_pos, st);
       return hip_fail(e, "heap allocation");
     }
   }
   if (a.stack_need > 64) {
-    hipError_t e = hipMallocAsync((void **)&a.gstack, (size_t)nq * a.stack_need * sizeof(uint32_t), st);
+    hipError_t e = pool_alloc((void **)&a.gstack, (size_t)nq * a.stack_need * sizeof(uint32_t), st);
     if (e != hipSuccess) {
-      if (a.heap_pos) (void)hipFreeAsync(a.heap_pos, st);
-      if (a.heap_d2) (void)hipFreeAsync(a.heap_d2, st);
+      pool_free(a.heap_pos, st);
+      pool_free(a.heap_d2, st);
       return hip_fail(e, "stack allocation");
     }
   }
   hipError_t e = launch_gather(a, mode, st);
-  if (a.heap_pos) (void)hipFreeAsync(a.heap_pos, st);
-  if (a.heap_d2) (void)hipFreeAsync(a.heap_d2, st);
-  if (a.gstack) (void)hipFreeAsync(a.gstack, st);
+  pool_free(a.heap_pos, st);
+  pool_free(a.heap_d2, st);
+  pool_free(a.gstack, st);
   if (e != hipSuccess) return hip_fail(e, "gather launch");
   return SKW_OK;
 }

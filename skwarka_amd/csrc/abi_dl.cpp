@@ -148,7 +148,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     o_caml = take(batch * sizeof(int32_t));
   }
   char *base = nullptr;
-  SKW_HIP(hipMallocAsync((void **)&base, off, st), "frame buffer allocation");
+  SKW_HIP(pool_alloc((void **)&base, off, st), "frame buffer allocation");
   hipError_t e = hipSuccess;
   if (pp.active) {
     DlParamArgs pa;
@@ -272,7 +272,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
-  (void)hipFreeAsync(base, st);
+  pool_free(base, st);
   if (rs != SKW_OK) return rs;
   if (e != hipSuccess) return hip_fail(e, "direct lighting frame");
   if (rays_out)

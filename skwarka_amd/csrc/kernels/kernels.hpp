@@ -25,6 +25,8 @@ struct TraceArgs {
   uint32_t *pops;     // statistics variant only: node boxes tested per ray
   uint32_t *tests;    // statistics variant only: triangles tested per ray
   uint64_t *sums;     // statistics variant only: [0] += node boxes, [1] += triangles over all rays
+  uint32_t refill_min;  // idle lanes a wave gathers before taking new rays (1..64)
+  uint32_t leaf_min;    // lanes holding a leaf a wave gathers before testing leaves (1..64)
 };
 
 // Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).

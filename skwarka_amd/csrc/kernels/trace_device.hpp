@@ -95,6 +95,27 @@ __device__ __forceinline__ TriVerts load_tri(const DevTri *__restrict__ tris, ui
   return v;
 }
 
+// The same decision as slab_cull for a ray whose slab products cannot be NaN. The products
+// (coord - o) * inv are NaN only for 0 * inf (a zero or tiny direction component with the origin
+// on the slab plane) or a non-finite origin; for a ray with finite origin and finite inverse
+// direction every product is a number (or +-inf), so the swap of :183-185 is min / max and the 12
+// comparisons of :188-192 are max(tmin, a1, a2, a3) > min(b1, b2, b3) or max(a1, a2, a3) > tmax
+// (the pairs a_k > b_k they add are false since a_k <= b_k, and tmin > tmax is not one of them).
+__device__ __forceinline__ bool slab_cull_nan_free(const float4 &lo, const float4 &hi, const RayState &r,
+                                                   double tmax) {
+  const double tn1 = ((double)lo.x - r.o0) * r.i0, tf1 = ((double)lo.w - r.o0) * r.i0;
+  const double tn2 = ((double)lo.y - r.o1) * r.i1, tf2 = ((double)hi.x - r.o1) * r.i1;
+  const double tn3 = ((double)lo.z - r.o2) * r.i2, tf3 = ((double)hi.y - r.o2) * r.i2;
+  const double amax = fmax(fmax(fmin(tn1, tf1), fmin(tn2, tf2)), fmin(tn3, tf3));
+  const double bmin = fmin(fmin(fmax(tn1, tf1), fmax(tn2, tf2)), fmax(tn3, tf3));
+  return fmax(amax, r.tmin) > bmin || amax > tmax;
+}
+
+__device__ __forceinline__ bool ray_nan_free(const RayState &r) {
+  return isfinite(r.o0) && isfinite(r.o1) && isfinite(r.o2) && isfinite(r.i0) && isfinite(r.i1) &&
+         isfinite(r.i2);
+}
+
 // TriangleAccelerator.cpp:238-274. Returns true with t, b1, b2 when accepted in [tmin, tmax].
 __device__ __forceinline__ bool tri_test_verts(const TriVerts &tv, const RayState &r, double tmax, double &t_out,
                                                double &b1_out, double &b2_out) {
@@ -333,14 +354,17 @@ __device__ __forceinline__ void store_hit(skw_hit *out, const DevTri *__restrict
   reinterpret_cast<double2 *>(o)[1] = make_double2(b1, b2);
 }
 
-// One lane's walk as a resumable state machine: start() tests the root; each step() visits one
-// node whose box was accepted (an internal node: all its children are tested at once with the
-// current tmax, the first survivor is entered and later survivors pushed as one entry; a leaf:
-// its instances and triangles) or pops the next pending sibling and re-tests it. step() returns
-// true when the walk has ended; found / best / tmax then hold the result.
+// One lane's walk as a resumable state machine. start() tests the root. advance() makes one move
+// between leaves: with no current node it pops the next pending sibling and re-tests its box; an
+// internal current node has all its children tested at once with the current tmax, the first
+// survivor is entered and later survivors pushed as one entry. When the current node is an
+// accepted leaf, advance() stops there (at_leaf) until leaf() has tested its instances and
+// triangles; the caller chooses when, so that lanes of a wave test leaves together. advance()
+// returns true when the walk has ended; found / best / tmax then hold the result.
 // Testing the children at visit time is exact: the first survivor is the node the reference pops
 // next, with the same tmax; a culled child stays culled at its later pop, since its tmax-free
 // conditions do not change and tmax only shrinks; later survivors are tested again when popped.
+// Holding a lane at a leaf changes no value: each lane's own sequence of tests is unchanged.
 template <bool ANY, bool INST, bool STATS, class Stack>
 struct Walker {
   const DevNode *__restrict__ nodes;
@@ -354,7 +378,8 @@ struct Walker {
   double tmax;
   Best best;
   uint32_t sp, cur, ca, cb, dpos;
-  bool have, found;
+  bool have, found, at_leaf;
+  bool nan_free;  // slab products of this ray can never be NaN (see slab_cull_nan_free)
   uint32_t pops, tests;
 
   __device__ __forceinline__ void set_root(const DevNode *__restrict__ tree_nodes, uint32_t root_idx) {
@@ -367,6 +392,7 @@ struct Walker {
     r = ray;
     tmax = tmax0;
     dpos = dir_mask(ray);
+    nan_free = ray_nan_free(ray);
     best.hit = false;
     best.inst = false;
     sp = 0;
@@ -379,9 +405,17 @@ struct Walker {
     cur = root;
     ca = __float_as_uint(root_hi.z);
     cb = __float_as_uint(root_hi.w);
+    at_leaf = have && (cb & 3u) == 0;
   }
 
-  __device__ __forceinline__ bool step() {
+  // FAST: every busy lane of the wave has nan_free set (a wave-uniform choice of the slab test).
+  template <bool FAST>
+  __device__ __forceinline__ bool cull(const float4 &lo, const float4 &hi) const {
+    return FAST ? slab_cull_nan_free(lo, hi, r, tmax) : slab_cull(lo, hi, r, tmax);
+  }
+
+  template <bool FAST>
+  __device__ __forceinline__ bool advance() {
     if (!have) {
       if (sp == 0) return true;
       uint32_t e = stk.get(sp - 1);
@@ -396,50 +430,61 @@ struct Walker {
       float4 lo, hi;
       load_node(nodes, cur, lo, hi);
       keep_live(hi);
-      if (slab_cull(lo, hi, r, tmax)) return false;
+      if (cull<FAST>(lo, hi)) return false;
       ca = __float_as_uint(hi.z);
       cb = __float_as_uint(hi.w);
+      if ((cb & 3u) == 0) {
+        have = true;
+        at_leaf = true;
+        return false;
+      }
     }
     have = false;
     const uint32_t a = ca, b = cb;
     const uint32_t kind = b & 3u;
-    if (kind != 0) {
-      const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
-      const bool fwd = (dpos >> (kind - 1u)) & 1u;
-      // Children in visiting order (scalars, not arrays: an indexed array would live in scratch).
-      // Absent children alias c0, so the three loads are issued together and wait once.
-      const uint32_t c0 = fwd ? a : a + nch - 1u;
-      const uint32_t c1 = nch > 1 ? (fwd ? a + 1u : a + nch - 2u) : c0;
-      const uint32_t c2 = nch > 2 ? a + (fwd ? 2u : 0u) : c0;
-      float4 lo0, hi0, lo1, hi1, lo2, hi2;
-      load_node(nodes, c0, lo0, hi0);
-      load_node(nodes, c1, lo1, hi1);
-      load_node(nodes, c2, lo2, hi2);
-      keep_live(hi0);
-      keep_live(hi1);
-      keep_live(hi2);
-      // All three tests are evaluated (no short circuit), so no load is sunk into a branch.
-      const uint32_t k0 = slab_cull(lo0, hi0, r, tmax) ? 0u : 1u;
-      const uint32_t k1 = slab_cull(lo1, hi1, r, tmax) ? 0u : 2u;
-      const uint32_t k2 = slab_cull(lo2, hi2, r, tmax) ? 0u : 4u;
-      const uint32_t surv = (k0 | k1 | k2) & ((1u << nch) - 1u);
-      if (STATS) pops += nch;
-      if (surv == 0) return false;
-      const uint32_t s0 = __builtin_ctz(surv);
-      cur = s0 == 0 ? c0 : (s0 == 1 ? c1 : c2);
-      const float hz = s0 == 0 ? hi0.z : (s0 == 1 ? hi1.z : hi2.z);
-      const float hw = s0 == 0 ? hi0.w : (s0 == 1 ? hi1.w : hi2.w);
-      ca = __float_as_uint(hz);
-      cb = __float_as_uint(hw);
-      have = true;
-      const uint32_t rest = surv & ~((2u << s0) - 1u);
-      if (rest) {
-        const uint32_t p = __builtin_ctz(rest);
-        const uint32_t start = p == 1 ? c1 : c2;
-        stk.set(sp++, (start << 3) | (fwd ? 0u : 4u) | (uint32_t)__builtin_popcount(rest));
-      }
-      return false;
+    const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
+    const bool fwd = (dpos >> (kind - 1u)) & 1u;
+    // Children in visiting order (scalars, not arrays: an indexed array would live in scratch).
+    // Absent children alias c0, so the three loads are issued together and wait once.
+    const uint32_t c0 = fwd ? a : a + nch - 1u;
+    const uint32_t c1 = nch > 1 ? (fwd ? a + 1u : a + nch - 2u) : c0;
+    const uint32_t c2 = nch > 2 ? a + (fwd ? 2u : 0u) : c0;
+    float4 lo0, hi0, lo1, hi1, lo2, hi2;
+    load_node(nodes, c0, lo0, hi0);
+    load_node(nodes, c1, lo1, hi1);
+    load_node(nodes, c2, lo2, hi2);
+    keep_live(hi0);
+    keep_live(hi1);
+    keep_live(hi2);
+    // All three tests are evaluated (no short circuit), so no load is sunk into a branch.
+    const uint32_t k0 = cull<FAST>(lo0, hi0) ? 0u : 1u;
+    const uint32_t k1 = cull<FAST>(lo1, hi1) ? 0u : 2u;
+    const uint32_t k2 = cull<FAST>(lo2, hi2) ? 0u : 4u;
+    const uint32_t surv = (k0 | k1 | k2) & ((1u << nch) - 1u);
+    if (STATS) pops += nch;
+    if (surv == 0) return false;
+    const uint32_t s0 = __builtin_ctz(surv);
+    cur = s0 == 0 ? c0 : (s0 == 1 ? c1 : c2);
+    const float hz = s0 == 0 ? hi0.z : (s0 == 1 ? hi1.z : hi2.z);
+    const float hw = s0 == 0 ? hi0.w : (s0 == 1 ? hi1.w : hi2.w);
+    ca = __float_as_uint(hz);
+    cb = __float_as_uint(hw);
+    have = true;
+    at_leaf = (cb & 3u) == 0;
+    const uint32_t rest = surv & ~((2u << s0) - 1u);
+    if (rest) {
+      const uint32_t p = __builtin_ctz(rest);
+      const uint32_t start = p == 1 ? c1 : c2;
+      stk.set(sp++, (start << 3) | (fwd ? 0u : 4u) | (uint32_t)__builtin_popcount(rest));
     }
+    return false;
+  }
+
+  // Tests the instances and triangles of the accepted leaf the lane is holding.
+  __device__ __forceinline__ void leaf() {
+    at_leaf = false;
+    have = false;
+    const uint32_t a = ca, b = cb;
     bool stop = false;
     if (INST && (b & 4u)) {
       uint32_t ib = inst_range[2 * cur], ie = inst_range[2 * cur + 1];
@@ -488,7 +533,15 @@ struct Walker {
       }
     }
     if (ANY && stop) sp = 0;  // the walk ends at its first accepted hit
-    return false;
+  }
+
+  // One move of either kind (for callers that do not group leaves); true when the walk has ended.
+  __device__ __forceinline__ bool step() {
+    if (at_leaf) {
+      leaf();
+      return false;
+    }
+    return advance<false>();
   }
 };
 

@@ -1,13 +1,16 @@
 // Single-scene closest-hit and occlusion kernels on gfx950 (the walk itself is in
 // trace_device.hpp).
 //
-// Scheduling: one ray per lane, 64-lane workgroups, a persistent grid sized to the chip. Every
-// iteration each lane makes one step of its own walk; a lane whose walk has ended stores its
-// result and takes the next unstarted ray of its wave's pool: the wave's idle lanes are counted
-// with a ballot and each gets pool + (its rank among idle lanes). When the pool runs short, one
-// lane claims the next kClaim rays with a single atomicAdd on the launch counter. The first
-// gridDim*blockDim rays are handed out statically; the counter starts at that value. Rays are
-// 64-byte records read as four 16-byte loads.
+// Scheduling: one ray per lane, 64-lane workgroups, a persistent grid sized to the chip. A wave
+// iteration has three phases, each run only for the lanes that need it: refill (idle lanes load
+// a new ray and test the root), leaf (lanes holding an accepted leaf test its triangles) and
+// advance (every other busy lane pops or descends one internal node). Refill and leaf phases are
+// the expensive, rarely needed ones, so a wave defers them until refill_min / leaf_min lanes wait
+// for them (or no lane could move otherwise): each then runs for many lanes at once instead of
+// for one or two lanes per iteration. Leaves are no longer deferred once the rays run out. Idle lanes take the next unstarted rays of the wave's pool
+// (ballot + rank); when the pool runs short, one lane claims the next kClaim rays with a single
+// atomicAdd on the launch counter. The first gridDim*blockDim rays are handed out statically; the
+// counter starts at that value. Rays are 64-byte records read as four 16-byte loads.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -38,50 +41,72 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
   w.stk = stk;
   // The wave's pool of claimed, unstarted rays [pool, pool_end): the same value in every lane.
   uint64_t pool = 0, pool_end = 0;
+  bool drained = false;  // some lane found no ray left (the same value in every lane)
   while (true) {
-    if (!first) {
-      const uint64_t need = __ballot(!busy);
-      if (need) {
-        const uint64_t n_idle = (uint64_t)__popcll(need);
-        const uint64_t rank = (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-        const uint64_t avail = pool_end - pool;
-        if (avail >= n_idle) {
-          if (!busy) idx = pool + rank;
-          pool += n_idle;
-        } else {
-          // Idle lanes take the rest of the pool, then the start of a newly claimed chunk.
-          const int leader = __ffsll((unsigned long long)need) - 1;
-          unsigned long long base = 0;
-          if ((int)lane == leader)
-            base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter), (unsigned long long)kClaim);
-          base = __shfl(base, leader);
-          if (!busy) idx = rank < avail ? pool + rank : base + (rank - avail);
-          pool = base + (n_idle - avail);
-          pool_end = base + kClaim;
+    // Refill: idle lanes take new rays once at least refill_min of them wait (or nothing else can
+    // move), so the ray load and root test run for many lanes at a time.
+    const uint64_t idle = __ballot(!busy);
+    if (idle) {
+      const uint64_t others = __ballot(busy);
+      if (first || (uint64_t)__popcll(idle) >= args.refill_min || others == 0) {
+        if (!first) {
+          const uint64_t n_idle = (uint64_t)__popcll(idle);
+          const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
+          const uint64_t avail = pool_end - pool;
+          if (avail >= n_idle) {
+            if (!busy) idx = pool + rank;
+            pool += n_idle;
+          } else {
+            // Idle lanes take the rest of the pool, then the start of a newly claimed chunk.
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader)
+              base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter), (unsigned long long)kClaim);
+            base = __shfl(base, leader);
+            if (!busy) idx = rank < avail ? pool + rank : base + (rank - avail);
+            pool = base + (n_idle - avail);
+            pool_end = base + kClaim;
+          }
+        }
+        // Once the launch's rays run out, deferring leaves only delays the last walks.
+        drained = drained || __ballot(!busy && idx >= n) != 0;
+        if (!busy) {
+          if (idx >= n) break;
+          RayState r;
+          double tmax;
+          load_ray(args.rays, idx, r, tmax);
+          w.start(r, tmax);
+          busy = true;
         }
       }
     }
     first = false;
-    if (!busy) {
-      if (idx >= n) break;
-      RayState r;
-      double tmax;
-      load_ray(args.rays, idx, r, tmax);
-      w.start(r, tmax);
-      busy = true;
-    }
-    if (w.step()) {
-      if (ANY)
-        args.occluded[idx] = w.found ? 1 : 0;
-      else
-        store_hit(args.hits + idx, args.tris, w.found, w.best, w.tmax, w.r);
-      if (STATS) {
-        if (args.pops) args.pops[idx] = w.pops;
-        if (args.tests) args.tests[idx] = w.tests;
-        sum_pops += w.pops;
-        sum_tests += w.tests;
+    // Leaves: lanes holding an accepted leaf test it together once at least leaf_min of them
+    // hold one (or no other lane can move).
+    const uint64_t held = __ballot(busy && w.at_leaf);
+    if (held) {
+      const uint64_t moving = __ballot(busy && !w.at_leaf);
+      if ((uint64_t)__popcll(held) >= args.leaf_min || moving == 0 || drained) {
+        if (busy && w.at_leaf) w.leaf();
       }
-      busy = false;
+    }
+    // Between leaves: one move per lane; the NaN-free slab test when every busy lane allows it.
+    const bool nan_free = __ballot(busy && !w.nan_free) == 0;
+    if (busy && !w.at_leaf) {
+      const bool done = nan_free ? w.template advance<true>() : w.template advance<false>();
+      if (done) {
+        if (ANY)
+          args.occluded[idx] = w.found ? 1 : 0;
+        else
+          store_hit(args.hits + idx, args.tris, w.found, w.best, w.tmax, w.r);
+        if (STATS) {
+          if (args.pops) args.pops[idx] = w.pops;
+          if (args.tests) args.tests[idx] = w.tests;
+          sum_pops += w.pops;
+          sum_tests += w.tests;
+        }
+        busy = false;
+      }
     }
   }
   if (STATS && args.sums) {
