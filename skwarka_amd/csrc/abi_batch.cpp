@@ -141,7 +141,6 @@ skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const s
     return fail(SKW_INVALID_ARG, "arguments must not be NULL");
   if (n == 0 && n_gathers == 0) return fail(SKW_INVALID_ARG, "empty batch");
   *out = nullptr;
-  if (n_gathers > 0) return fail(SKW_UNSUPPORTED, "photon-gather jobs in the batched launch are switched off");
   std::unique_ptr<skw_batch> b(new skw_batch());
   SKW_HIP(hipGetDevice(&b->device), "hipGetDevice");
   SKW_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device), "hipDeviceGetAttribute");

@@ -7,9 +7,13 @@
 // generated in the kernel (no ray buffer): PerspectiveCamera::GenerateRay
 // (Raytracer/Cameras/PerspectiveCamera.cpp:36-64) at pixel + splitmix64 jitter, the same stream as
 // the direct-lighting camera. The walk is trace_device.hpp's (TriangleAccelerator::_NodeIntersect
-// semantics). When a scene's last chunk completes, the wave records the wall clock for it.
+// semantics). Photon-gather jobs (k nearest + disc estimate, gather_device.hpp) can share the
+// queue: their chunks are query ranges, which a wave takes like a ray chunk and serves 64 queries
+// at a time before it takes its next chunk. When an item's last chunk completes, the wave records
+// the wall clock for it.
 #include <hip/hip_runtime.h>
 
+#include "gather_device.hpp"
 #include "geom_device.hpp"
 #include "kernels.hpp"
 #include "trace_device.hpp"
@@ -64,7 +68,9 @@ __device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint6
   }
 }
 
-template <int S, int WAVES>
+// GATHER: the queue may hold photon-gather chunks (an instance of its own, so the gather's
+// registers do not lower the occupancy of ray-only batches).
+template <int S, int WAVES, bool GATHER>
 __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
   __shared__ uint32_t lds[S * 64 * WAVES];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -82,11 +88,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     if (idle == ~0ull && cursor >= end) {
       // The wave's chunk is done: account for it, then take the next one.
       if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lane);
+      if (GATHER) sid = 0xFFFFFFFFu;  // a gather chunk taken next must not account this scene again
       unsigned long long q = 0;
       if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
       q = __shfl(q, 0);
       if (q >= a.n_chunks) break;
       const DevChunk c = a.chunks[q];
+      if (GATHER) {
+        const uint32_t item = __builtin_amdgcn_readfirstlane(c.item);
+        if (item >= a.n_scenes) {
+          // A photon-gather chunk: its queries 64 at a time, the lookup stack in the LDS rows the
+          // ray walk uses (no lane of the wave is walking a ray now).
+          const GatherArgs &ga = a.jobs[item - a.n_scenes];
+          const gath::LdsStack<S> gstk{stk.base};
+          for (uint64_t qi = c.begin + lane; qi < c.end; qi += 64) gath::gather_one<0>(ga, qi, gstk);
+          account(a, item, c.end - c.begin, ga.nq, lane);
+          cursor = 0;
+          end = 0;
+          continue;
+        }
+      }
       sid = c.item;
       cursor = c.begin;
       end = c.end;
@@ -141,7 +162,13 @@ hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream) {
 
 hipError_t launch_batch_trace(const BatchArgs &a, uint32_t waves, uint32_t stack, uint32_t grid, hipStream_t stream) {
   dim3 g(grid);
-#define SKW_BATCH_LAUNCH(S, W) hipLaunchKernelGGL((k_batch_trace<S, W>), g, dim3(64 * (W)), 0, stream, a)
+#define SKW_BATCH_LAUNCH(S, W)                                                           \
+  do {                                                                                   \
+    if (a.n_jobs > 0)                                                                    \
+      hipLaunchKernelGGL((k_batch_trace<S, W, true>), g, dim3(64 * (W)), 0, stream, a);  \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_batch_trace<S, W, false>), g, dim3(64 * (W)), 0, stream, a); \
+  } while (0)
   if (stack <= 32) {
     if (waves == 1) SKW_BATCH_LAUNCH(32, 1);
     else if (waves == 2) SKW_BATCH_LAUNCH(32, 2);

@@ -87,8 +87,6 @@ def test_batch_rejects_bad_shapes(torch_cuda):
             batch.configure(*shape)
 
 
-@pytest.mark.skip(reason="photon-gather jobs in the batched launch are switched off: a GPU run of the batch "
-                         "kernel that served them hung (in a ray-only test) and its cause is not shown yet")
 def test_batch_with_photon_gathers(torch_cuda):
     print("case: batch of 3 scenes + 2 photon-gather jobs vs skw_photon_irradiance", flush=True)
     torch = torch_cuda
