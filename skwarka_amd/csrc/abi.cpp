@@ -91,6 +91,7 @@ static int clamp_int(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi 
 // Lanes a trace wave gathers before it takes new rays / tests leaves (see traverse.hip).
 constexpr int kTraceRefillMin = 8;
 constexpr int kTraceLeafMin = 16;
+constexpr int kTraceClaim = 64;  // rays per claim on the launch counter
 
 CounterPool::~CounterPool() {
   for (auto &kv : by_stream) (void)hipFree(kv.second);
@@ -137,6 +138,7 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.sums = d_sums;
   a.refill_min = (uint32_t)clamp_int(env_int("SKW_TRACE_REFILL_MIN", kTraceRefillMin), 1, 64);
   a.leaf_min = (uint32_t)clamp_int(env_int("SKW_TRACE_LEAF_MIN", kTraceLeafMin), 1, 64);
+  a.claim = (uint32_t)clamp_int(env_int("SKW_TRACE_CLAIM", kTraceClaim), 64, 4096);
   const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
   SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
   uint32_t need = s->host.stack_need;

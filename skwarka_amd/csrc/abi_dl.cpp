@@ -135,7 +135,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   }
   size_t o_prim_rays = take(batch * sizeof(skw_ray)), o_prim_hits = take(batch * sizeof(skw_hit));
   size_t o_le = take(batch * 3 * sizeof(double)), o_cdf = take(batch * (nl ? nl : 1) * sizeof(double));
-  size_t o_ns = take(batch * 3 * sizeof(double));
+  size_t o_ns = take(batch * 3 * sizeof(double)), o_point = take(batch * 24 * sizeof(double));
   size_t o_srays = take(batch * nls * sizeof(skw_ray)), o_sidx = take(batch * nls * sizeof(int32_t));
   size_t o_sc = take(batch * nls * 3 * sizeof(double)), o_occ = take(batch * nls);
   size_t o_brays = take(batch * nbs * sizeof(skw_ray)), o_bidx = take(batch * nbs * sizeof(int32_t));
@@ -186,6 +186,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   a.le = (double *)(base + o_le);
   a.cdf = (double *)(base + o_cdf);
   a.ns = (double *)(base + o_ns);
+  a.point = (double *)(base + o_point);
   a.shadow_rays = (skw_ray *)(base + o_srays);
   a.shadow_idx = (int32_t *)(base + o_sidx);
   a.shadow_c = (double *)(base + o_sc);

@@ -312,18 +312,28 @@ __device__ __forceinline__ void put_ray(skw_ray *r, D3 o, D3 d, double tmin, dou
   q[3] = make_double2(tmin, tmax);
 }
 
-// Wave-aggregated append: lanes with `want` get consecutive slots of *counter.
-__device__ __forceinline__ int32_t append_slot(uint64_t *counter, bool want) {
-  const uint32_t lane = threadIdx.x & 63u;
+// Block-aggregated append: lanes with `want` get consecutive slots of *counter, one atomicAdd
+// per block of kSampleBlock threads. A single device-wide counter takes about one atomic per
+// 11 ns, so per-wave appends would bound the sample kernels; per block they cost 8x less. Every
+// thread of the block must call this (it has two barriers).
+constexpr uint32_t kSampleBlock = 512;
+__device__ __forceinline__ int32_t append_slot_block(uint64_t *counter, bool want) {
+  __shared__ uint32_t wave_count[kSampleBlock / 64];
+  __shared__ unsigned long long base_s;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t mask = __ballot(want);
-  if (mask == 0) return -1;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  unsigned long long base = 0;
-  if ((int)lane == leader)
-    base = atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
+  if (lane == 0) wave_count[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) total += wave_count[w];
+    base_s = total ? atomicAdd(reinterpret_cast<unsigned long long *>(counter), (unsigned long long)total) : 0ull;
+  }
+  __syncthreads();
+  uint64_t off = base_s;
+  for (uint32_t w = 0; w < wave; ++w) off += wave_count[w];
   if (!want) return -1;
-  return (int32_t)(base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull)));
+  return (int32_t)(off + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
 // Pixel of sample slot i of a pass: 8x8 pixel tiles in row order, so the 64 camera rays of a wave
@@ -361,44 +371,39 @@ __global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
   put_ray(a.prim_rays + i, D3{a.cam_o[0], a.cam_o[1], a.cam_o[2]}, d, 0.0, __builtin_huge_val());
 }
 
-// SPLIT: evaluation materials differ from sampling materials (replayed estimator).
-// ADJ: also record what the adjoint needs (d f / d roughness, weights, lights).
-template <bool SPLIT, bool ADJ>
-__global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
+// Shading is split in three kernels so that each keeps few registers (more resident waves hide
+// the dependent loads): k_dl_point (one thread per camera sample: hit geometry, emitted radiance,
+// the light CDF, the BSDF frame), then k_dl_light and k_dl_bsdf (one thread per (sample, k):
+// light sample k / BSDF sample k of _SampleLights / _SampleBSDF, and the ray it casts). The
+// shading point travels through a per-sample record of kPointDoubles doubles:
+//   p[0..2] wo[3..5] ns[6..8] ng[9..11] e1[12..14] e2[15..17] cross[18..20] dot[21] prim bits[22].
+constexpr uint32_t kPointDoubles = 24;
+
+__global__ __launch_bounds__(64) void k_dl_point(DlArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < a.batch;
-  bool hit = false;
-  skw_hit h;
-  h.prim = kMiss;
-  if (in) {
-    h = a.prim_hits[i];
-    hit = h.prim != kMiss;
-  }
+  if (i >= a.batch) return;
+  const skw_hit h = a.prim_hits[i];
+  const bool hit = h.prim != kMiss;
   const uint32_t nl = a.n_lights;
-  const uint64_t pix = pixel_of(a, i);
-  const uint64_t sidx = pix * a.spp + a.pass;
-  D3 o{0, 0, 0}, d{0, 0, 1}, wo{0, 0, 0}, p{0, 0, 0};
-  HitGeom g;
-  Bsdf b, be;
-  double de_dr = 0.0;
-  double cdf[kMaxLights];
   double le[3] = {0.0, 0.0, 0.0};
   int32_t cam_light = -1;
-  uint32_t prim = h.prim;
+  double rec[kPointDoubles];
+  for (uint32_t j = 0; j < kPointDoubles; ++j) rec[j] = 0.0;
+  rec[22] = __longlong_as_double((long long)kMiss);
   if (hit) {
+    const uint32_t prim = h.prim;
     const skw_ray &r = a.prim_rays[i];
-    o = D3{r.o[0], r.o[1], r.o[2]};
-    d = D3{r.d[0], r.d[1], r.d[2]};
-    g = hit_geometry(a.geom, o, d, prim, h.tri);
-    wo = mul3(d, -1.0);
-    p = g.p;
+    const D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
+    const HitGeom g = hit_geometry(a.geom, o, d, prim, h.tri);
+    const D3 wo = mul3(d, -1.0);
     const DevMat &m = a.mats[prim];
     if (m.light != kMiss && dot3(wo, g.ng) > 0.0) {
       for (int c = 0; c < 3; ++c) le[c] = a.rad_e[3 * m.light + c];
       cam_light = (int32_t)m.light;
     }
     // BSDF frame: DG tangent of a mesh without UVs or tangents (TriangleMesh.cpp:520-540)
-    D3 ns = g.ns, t;
+    const D3 ns = g.ns;
+    D3 t;
     if (fabs(ns.x) > fabs(ns.y)) {
       double il = 1.0 / sqrt(ns.x * ns.x + ns.z * ns.z);
       t = D3{-ns.z * il, 0.0, ns.x * il};
@@ -414,144 +419,224 @@ __global__ __launch_bounds__(64) void k_dl_shade(DlArgs a) {
     } else {
       tangent = t;
     }
-    b.n = ns;
-    b.ng = g.ng;
-    b.e1 = tangent;
-    b.e2 = cross3(ns, tangent);
-    b.plastic = m.type == 1;
-    for (int c = 0; c < 3; ++c) {
-      b.kd[c] = m.kd[c];
-      b.ks[c] = m.ks[c];
-    }
-    b.e = m.exponent;
-    be = b;
-    if (SPLIT || ADJ) {
-      const DevMat &me = a.mats_e[prim];
-      for (int c = 0; c < 3; ++c) {
-        be.kd[c] = me.kd[c];
-        be.ks[c] = me.ks[c];
-      }
-      be.e = me.exponent;
-      de_dr = me.de_dr;
-    }
+    const D3 e2 = cross3(ns, tangent);
     if (nl > 0) {
       D3 nrm = dot3(wo, ns) >= 0.0 ? ns : mul3(ns, -1.0);
-      lights_cdf(a, p, nrm, cdf);
+      double cdf[kMaxLights];
+      lights_cdf(a, g.p, nrm, cdf);
       for (uint32_t l = 0; l < nl; ++l) a.cdf[i * nl + l] = cdf[l];
     }
-    for (int c = 0; c < 3; ++c) a.ns[3 * i + c] = c == 0 ? ns.x : (c == 1 ? ns.y : ns.z);
+    const D3 vs[7] = {g.p, wo, ns, g.ng, tangent, e2, g.cross};
+    for (int v = 0; v < 7; ++v) {
+      rec[3 * v] = vs[v].x;
+      rec[3 * v + 1] = vs[v].y;
+      rec[3 * v + 2] = vs[v].z;
+    }
+    rec[21] = g.dot;
+    rec[22] = __longlong_as_double((long long)prim);
+    a.ns[3 * i] = ns.x;
+    a.ns[3 * i + 1] = ns.y;
+    a.ns[3 * i + 2] = ns.z;
   }
-  if (in) {
-    for (int c = 0; c < 3; ++c) a.le[3 * i + c] = le[c];
-    if (ADJ) a.cam_light[i] = cam_light;
+  double2 *out = reinterpret_cast<double2 *>(a.point + i * kPointDoubles);
+  for (uint32_t j = 0; j < kPointDoubles / 2; ++j) out[j] = make_double2(rec[2 * j], rec[2 * j + 1]);
+  for (int c = 0; c < 3; ++c) a.le[3 * i + c] = le[c];
+  if (a.cam_light) a.cam_light[i] = cam_light;
+}
+
+// The shading point of sample i: position, outgoing direction, the BSDF with the sampling
+// material (b) and the evaluation material (be, when SPLIT or ADJ), the primitive, m_cross / m_dot.
+struct Point {
+  D3 p, wo;
+  HitGeom g;
+  uint32_t prim;
+};
+
+__device__ __forceinline__ bool load_point(const DlArgs &a, uint64_t i, Point &pt) {
+  const double2 *q = reinterpret_cast<const double2 *>(a.point + i * kPointDoubles);
+  double v[kPointDoubles];
+  for (uint32_t j = 0; j < kPointDoubles / 2; ++j) {
+    const double2 x = q[j];
+    v[2 * j] = x.x;
+    v[2 * j + 1] = x.y;
   }
-  const bool shade = hit && nl > 0;
-  const DevPrim *pr = a.geom.prims + (hit ? prim : 0);
-  // _SampleLights: one occlusion ray per sample whose unoccluded contribution is not black.
-  for (uint32_t k = 0; k < a.nls; ++k) {
-    bool want = false;
-    D3 wi{0, 0, 0};
-    double lf[3] = {0, 0, 0};
-    double len = 0.0, mint = 0.0;
-    double fe[3] = {0, 0, 0}, w = 0.0, dfr = 0.0;
-    uint32_t l = 0;
-    if (shade) {
-      double uc = rnd(a.seed, sidx, 2 + k);
-      double u0 = rnd(a.seed, sidx, 6 + 2 * k), u1 = rnd(a.seed, sidx, 7 + 2 * k);
-      l = search_cdf([&](uint32_t j) { return cdf[j]; }, nl, uc);
-      double comp_pdf = l == 0 ? cdf[0] : cdf[l] - cdf[l - 1];
-      const DevLight &L = a.lights[l];
-      double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
-      if (L.tri_count > 0) {
-        const DevLightTri *lt = a.ltris + L.tri_off;
-        uint32_t ti = search_cdf([&](uint32_t j) { return lt[j].cdf; }, L.tri_count, tri_u);
-        const DevLightTri &T = lt[ti];
-        double tmp = sqrt(u0);
-        double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
-        D3 v0{T.v0[0], T.v0[1], T.v0[2]}, v1{T.v1[0], T.v1[1], T.v1[2]}, v2{T.v2[0], T.v2[1], T.v2[2]};
-        D3 sp = add3(add3(mul3(v0, bb1), mul3(v1, bb2)), mul3(v2, 1.0 - bb1 - bb2));
-        D3 lv = sub3(sp, p);
-        len = sqrt(len_sqr3(lv));
-        wi = mul3(lv, 1.0 / len);
-        D3 ln{T.n[0], T.n[1], T.n[2]};
-        double ac = fabs(dot3(wi, ln));
-        double lpdf = ac > kEps ? (len * len) / (L.area * ac) : 0.0;
-        bool lit = dot3(ln, wi) < 0.0;
-        double lr[3] = {lit ? L.radiance[0] : 0.0, lit ? L.radiance[1] : 0.0, lit ? L.radiance[2] : 0.0};
-        if (lpdf > 0.0 && !(lr[0] == 0.0 && lr[1] == 0.0 && lr[2] == 0.0)) {
-          double f[3];
-          if (ADJ) b.evaluate<true>(wo, wi, f, de_dr, dfr);
-          else b.evaluate(wo, wi, f);
-          for (int c = 0; c < 3; ++c) lf[c] = lr[c] * f[c];
-          if (!(lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0)) {
-            mint = next_min_t(*pr, g, wi);
-            double bp = b.pdf(wo, wi);
-            w = power_heuristic((double)a.nls, lpdf * comp_pdf, (double)a.nbs, bp);
-            w *= fabs(dot3(wi, b.n)) / (lpdf * comp_pdf);
-            for (int c = 0; c < 3; ++c) fe[c] = f[c];
-            if (SPLIT) {
-              be.evaluate(wo, wi, fe);
-              for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * fe[c];
-            } else {
-              for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * f[c];
+  pt.prim = (uint32_t)__double_as_longlong(v[22]);
+  pt.p = D3{v[0], v[1], v[2]};
+  pt.wo = D3{v[3], v[4], v[5]};
+  pt.g.p = pt.p;
+  pt.g.ns = D3{v[6], v[7], v[8]};
+  pt.g.ng = D3{v[9], v[10], v[11]};
+  pt.g.cross = D3{v[18], v[19], v[20]};
+  pt.g.dot = v[21];
+  return pt.prim != kMiss;
+}
+
+__device__ __forceinline__ void point_bsdf(const DlArgs &a, uint64_t i, uint32_t prim, Bsdf &b) {
+  const double2 *q = reinterpret_cast<const double2 *>(a.point + i * kPointDoubles);
+  const double2 n01 = q[3], n2g0 = q[4], g12 = q[5], t01 = q[6], t2u0 = q[7], u12 = q[8];
+  b.n = D3{n01.x, n01.y, n2g0.x};
+  b.ng = D3{n2g0.y, g12.x, g12.y};
+  b.e1 = D3{t01.x, t01.y, t2u0.x};
+  b.e2 = D3{t2u0.y, u12.x, u12.y};
+  const DevMat &m = a.mats[prim];
+  b.plastic = m.type == 1;
+  for (int c = 0; c < 3; ++c) {
+    b.kd[c] = m.kd[c];
+    b.ks[c] = m.ks[c];
+  }
+  b.e = m.exponent;
+}
+
+// _SampleLights, light sample k of camera sample i: one occlusion ray when the unoccluded
+// contribution is not black. SPLIT: evaluation materials differ from sampling materials
+// (replayed estimator). ADJ: also record what the adjoint needs (d f / d roughness, weight, light).
+template <bool SPLIT, bool ADJ>
+__global__ __launch_bounds__(kSampleBlock) void k_dl_light(DlArgs a) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nls = a.nls, nl = a.n_lights;
+  const uint64_t i = t / nls;
+  const uint32_t k = (uint32_t)(t - i * nls);
+  const bool in = i < a.batch;
+  Point pt;
+  const bool shade = in && nl > 0 && load_point(a, i, pt);
+  bool want = false;
+  D3 wi{0, 0, 0};
+  double lf[3] = {0, 0, 0};
+  double len = 0.0, mint = 0.0;
+  double fe[3] = {0, 0, 0}, w = 0.0, dfr = 0.0;
+  uint32_t l = 0;
+  if (shade) {
+    Bsdf b;
+    point_bsdf(a, i, pt.prim, b);
+    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    const double *cdf = a.cdf + i * nl;
+    double uc = rnd(a.seed, sidx, 2 + k);
+    double u0 = rnd(a.seed, sidx, 6 + 2 * k), u1 = rnd(a.seed, sidx, 7 + 2 * k);
+    l = search_cdf([&](uint32_t j) { return cdf[j]; }, nl, uc);
+    double comp_pdf = l == 0 ? cdf[0] : cdf[l] - cdf[l - 1];
+    const DevLight &L = a.lights[l];
+    double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
+    if (L.tri_count > 0) {
+      const DevLightTri *lt = a.ltris + L.tri_off;
+      uint32_t ti = search_cdf([&](uint32_t j) { return lt[j].cdf; }, L.tri_count, tri_u);
+      const DevLightTri &T = lt[ti];
+      double tmp = sqrt(u0);
+      double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
+      D3 v0{T.v0[0], T.v0[1], T.v0[2]}, v1{T.v1[0], T.v1[1], T.v1[2]}, v2{T.v2[0], T.v2[1], T.v2[2]};
+      D3 sp = add3(add3(mul3(v0, bb1), mul3(v1, bb2)), mul3(v2, 1.0 - bb1 - bb2));
+      D3 lv = sub3(sp, pt.p);
+      len = sqrt(len_sqr3(lv));
+      wi = mul3(lv, 1.0 / len);
+      D3 ln{T.n[0], T.n[1], T.n[2]};
+      double ac = fabs(dot3(wi, ln));
+      double lpdf = ac > kEps ? (len * len) / (L.area * ac) : 0.0;
+      bool lit = dot3(ln, wi) < 0.0;
+      double lr[3] = {lit ? L.radiance[0] : 0.0, lit ? L.radiance[1] : 0.0, lit ? L.radiance[2] : 0.0};
+      if (lpdf > 0.0 && !(lr[0] == 0.0 && lr[1] == 0.0 && lr[2] == 0.0)) {
+        double f[3];
+        // ADJ runs at one parameter vector (sampling and evaluation materials agree); d exponent /
+        // d roughness comes from the evaluation table.
+        if (ADJ) b.evaluate<true>(pt.wo, wi, f, a.mats_e[pt.prim].de_dr, dfr);
+        else b.evaluate(pt.wo, wi, f);
+        for (int c = 0; c < 3; ++c) lf[c] = lr[c] * f[c];
+        if (!(lf[0] == 0.0 && lf[1] == 0.0 && lf[2] == 0.0)) {
+          mint = next_min_t(a.geom.prims[pt.prim], pt.g, wi);
+          double bp = b.pdf(pt.wo, wi);
+          w = power_heuristic((double)a.nls, lpdf * comp_pdf, (double)a.nbs, bp);
+          w *= fabs(dot3(wi, b.n)) / (lpdf * comp_pdf);
+          for (int c = 0; c < 3; ++c) fe[c] = f[c];
+          if (SPLIT) {
+            Bsdf be = b;
+            const DevMat &me = a.mats_e[pt.prim];
+            for (int c = 0; c < 3; ++c) {
+              be.kd[c] = me.kd[c];
+              be.ks[c] = me.ks[c];
             }
-            for (int c = 0; c < 3; ++c) lf[c] = lf[c] * w;
-            want = true;
+            be.e = me.exponent;
+            be.evaluate(pt.wo, wi, fe);
+            for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * fe[c];
+          } else {
+            for (int c = 0; c < 3; ++c) lf[c] = (lit ? a.rad_e[3 * l + c] : 0.0) * f[c];
           }
+          for (int c = 0; c < 3; ++c) lf[c] = lf[c] * w;
+          want = true;
         }
       }
     }
-    int32_t slot = append_slot(a.counts, want);
-    if (in) a.shadow_idx[i * a.nls + k] = slot;
-    if (want) {
-      put_ray(a.shadow_rays + slot, p, wi, mint, len - 1e-4);
-      double *c3 = a.shadow_c + 3 * (i * a.nls + k);
-      c3[0] = lf[0];
-      c3[1] = lf[1];
-      c3[2] = lf[2];
-      if (ADJ) {
-        double *x = a.shadow_aux + 6 * (i * a.nls + k);
-        x[0] = fe[0];
-        x[1] = fe[1];
-        x[2] = fe[2];
-        x[3] = w;
-        x[4] = dfr;
-        x[5] = (double)l;
+  }
+  const int32_t slot = append_slot_block(a.counts, want);
+  if (in) a.shadow_idx[i * nls + k] = slot;
+  if (want) {
+    put_ray(a.shadow_rays + slot, pt.p, wi, mint, len - 1e-4);
+    double *c3 = a.shadow_c + 3 * (i * nls + k);
+    c3[0] = lf[0];
+    c3[1] = lf[1];
+    c3[2] = lf[2];
+    if (ADJ) {
+      double *x = a.shadow_aux + 6 * (i * nls + k);
+      x[0] = fe[0];
+      x[1] = fe[1];
+      x[2] = fe[2];
+      x[3] = w;
+      x[4] = dfr;
+      x[5] = (double)l;
+    }
+  }
+}
+
+// _SampleBSDF, BSDF sample k of camera sample i: one closest-hit ray when the pdf is positive and
+// the value not black.
+template <bool SPLIT, bool ADJ>
+__global__ __launch_bounds__(kSampleBlock) void k_dl_bsdf(DlArgs a) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nbs = a.nbs;
+  const uint64_t i = t / nbs;
+  const uint32_t k = (uint32_t)(t - i * nbs);
+  const bool in = i < a.batch;
+  Point pt;
+  const bool shade = in && a.n_lights > 0 && load_point(a, i, pt);
+  bool want = false;
+  D3 wi{0, 0, 0};
+  double f[3] = {0, 0, 0}, bp = 0.0, mint = 0.0, dfr = 0.0, cosw = 0.0;
+  if (shade) {
+    Bsdf b;
+    point_bsdf(a, i, pt.prim, b);
+    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    double uc = rnd(a.seed, sidx, 14 + k);
+    double u0 = rnd(a.seed, sidx, 18 + 2 * k), u1 = rnd(a.seed, sidx, 19 + 2 * k);
+    D3 wo_l, wi_l;
+    b.sample(pt.wo, u0, u1, uc, wi, bp, f, wo_l, wi_l);
+    if (bp > 0.0 && !(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
+      mint = next_min_t(a.geom.prims[pt.prim], pt.g, wi);
+      cosw = fabs(dot3(wi, b.n)) / bp;
+      want = true;
+      // f is non-black, so the sampled pair passed the geometric-normal test: re-evaluate
+      // the same local pair with the evaluation materials.
+      if (SPLIT || ADJ) {
+        Bsdf be = b;
+        const DevMat &me = a.mats_e[pt.prim];
+        for (int c = 0; c < 3; ++c) {
+          be.kd[c] = me.kd[c];
+          be.ks[c] = me.ks[c];
+        }
+        be.e = me.exponent;
+        if (SPLIT) be.eval_reflect_local(wo_l, wi_l, f);
+        if (ADJ) be.eval_reflect_local<true>(wo_l, wi_l, f, me.de_dr, dfr);
       }
     }
   }
-  // _SampleBSDF: one closest-hit ray per sample with a positive pdf and a non-black value.
-  for (uint32_t k = 0; k < a.nbs; ++k) {
-    bool want = false;
-    D3 wi{0, 0, 0};
-    double f[3] = {0, 0, 0}, bp = 0.0, mint = 0.0, dfr = 0.0;
-    if (shade) {
-      double uc = rnd(a.seed, sidx, 14 + k);
-      double u0 = rnd(a.seed, sidx, 18 + 2 * k), u1 = rnd(a.seed, sidx, 19 + 2 * k);
-      D3 wo_l, wi_l;
-      b.sample(wo, u0, u1, uc, wi, bp, f, wo_l, wi_l);
-      if (bp > 0.0 && !(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
-        mint = next_min_t(*pr, g, wi);
-        want = true;
-        // f is non-black, so the sampled pair passed the geometric-normal test: re-evaluate
-        // the same local pair with the evaluation materials.
-        if (SPLIT) be.eval_reflect_local(wo_l, wi_l, f);
-        if (ADJ) be.eval_reflect_local<true>(wo_l, wi_l, f, de_dr, dfr);
-      }
-    }
-    int32_t slot = append_slot(a.counts + 1, want);
-    if (in) a.bsdf_idx[i * a.nbs + k] = slot;
-    if (want) {
-      put_ray(a.bsdf_rays + slot, p, wi, mint, __builtin_huge_val());
-      double *rec = a.bsdf_rec + 8 * (i * a.nbs + k);
-      rec[0] = f[0];
-      rec[1] = f[1];
-      rec[2] = f[2];
-      rec[3] = bp;
-      rec[4] = fabs(dot3(wi, b.n)) / bp;
-      rec[5] = mint;
-      if (ADJ) rec[6] = dfr;
-    }
+  const int32_t slot = append_slot_block(a.counts + 1, want);
+  if (in) a.bsdf_idx[i * nbs + k] = slot;
+  if (want) {
+    put_ray(a.bsdf_rays + slot, pt.p, wi, mint, __builtin_huge_val());
+    double *rec = a.bsdf_rec + 8 * (i * nbs + k);
+    rec[0] = f[0];
+    rec[1] = f[1];
+    rec[2] = f[2];
+    rec[3] = bp;
+    rec[4] = cosw;
+    rec[5] = mint;
+    if (ADJ) rec[6] = dfr;
   }
 }
 
@@ -875,12 +960,20 @@ hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <bool SPLIT, bool ADJ>
+static void launch_dl_samples(const DlArgs &a, hipStream_t stream) {
+  const uint64_t nl = a.batch * a.nls, nb = a.batch * a.nbs, B = kSampleBlock;
+  if (nl) hipLaunchKernelGGL((k_dl_light<SPLIT, ADJ>), dim3((uint32_t)((nl + B - 1) / B)), dim3(B), 0, stream, a);
+  if (nb) hipLaunchKernelGGL((k_dl_bsdf<SPLIT, ADJ>), dim3((uint32_t)((nb + B - 1) / B)), dim3(B), 0, stream, a);
+}
+
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 63) / 64);
   if (grid == 0) return hipSuccess;
-  if (a.adj || a.tape_j) hipLaunchKernelGGL((k_dl_shade<false, true>), dim3(grid), dim3(64), 0, stream, a);
-  else if (a.mats_e != a.mats) hipLaunchKernelGGL((k_dl_shade<true, false>), dim3(grid), dim3(64), 0, stream, a);
-  else hipLaunchKernelGGL((k_dl_shade<false, false>), dim3(grid), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL(k_dl_point, dim3(grid), dim3(64), 0, stream, a);
+  if (a.adj || a.tape_j) launch_dl_samples<false, true>(a, stream);
+  else if (a.mats_e != a.mats) launch_dl_samples<true, false>(a, stream);
+  else launch_dl_samples<false, false>(a, stream);
   return hipGetLastError();
 }
 

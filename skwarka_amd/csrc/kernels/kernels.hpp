@@ -27,6 +27,7 @@ struct TraceArgs {
   uint64_t *sums;     // statistics variant only: [0] += node boxes, [1] += triangles over all rays
   uint32_t refill_min;  // idle lanes a wave gathers before taking new rays (1..64)
   uint32_t leaf_min;    // lanes holding a leaf a wave gathers before testing leaves (1..64)
+  uint32_t claim;       // rays a wave claims from the counter at once (64..4096)
 };
 
 // Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).
@@ -137,6 +138,7 @@ struct DlArgs {
   double *le;         // 3 per sample: emitted radiance seen by the camera ray
   double *cdf;        // n_lights per sample
   double *ns;         // 3 per sample: shading normal
+  double *point;      // 24 per sample: the shading point record of k_dl_point (dl.hip)
   skw_ray *shadow_rays;
   int32_t *shadow_idx;   // nls per sample: ray index or -1
   double *shadow_c;      // 3 per (sample, k): unoccluded contribution

@@ -7,10 +7,12 @@
 // advance (every other busy lane pops or descends one internal node). Refill and leaf phases are
 // the expensive, rarely needed ones, so a wave defers them until refill_min / leaf_min lanes wait
 // for them (or no lane could move otherwise): each then runs for many lanes at once instead of
-// for one or two lanes per iteration. Leaves are no longer deferred once the rays run out. Idle lanes take the next unstarted rays of the wave's pool
-// (ballot + rank); when the pool runs short, one lane claims the next kClaim rays with a single
-// atomicAdd on the launch counter. The first gridDim*blockDim rays are handed out statically; the
-// counter starts at that value. Rays are 64-byte records read as four 16-byte loads.
+// for one or two lanes per iteration. Leaves are no longer deferred once the rays run out.
+// Idle lanes take the next unstarted rays of the wave's pool (ballot + rank); when the pool runs
+// short, one lane claims the next `claim` rays (at least 64) with a single atomicAdd on the launch
+// counter. One device-wide counter serves about one atomic per 11 ns, so claims are kept rare.
+// The first gridDim*blockDim rays are handed out statically; the counter starts at that value.
+// Rays are 64-byte records read as four 16-byte loads.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -20,9 +22,6 @@ namespace skw {
 namespace {
 
 using namespace trace;
-
-// Rays a wave claims from the launch counter at once (one atomic per kClaim refills).
-constexpr uint64_t kClaim = 64;
 
 // STATS: count node boxes and triangles tested per ray (the oracle counts the same events).
 template <bool ANY, bool INST, bool STATS, class Stack>
@@ -61,11 +60,11 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
             unsigned long long base = 0;
             if ((int)lane == leader)
-              base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter), (unsigned long long)kClaim);
+              base = atomicAdd(reinterpret_cast<unsigned long long *>(args.counter), (unsigned long long)args.claim);
             base = __shfl(base, leader);
             if (!busy) idx = rank < avail ? pool + rank : base + (rank - avail);
             pool = base + (n_idle - avail);
-            pool_end = base + kClaim;
+            pool_end = base + args.claim;
           }
         }
         // Once the launch's rays run out, deferring leaves only delays the last walks.
