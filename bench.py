@@ -308,15 +308,16 @@ def cpu_baseline_c2(c2, threads, rows):
     acc = oracle.Accel(s["meshes"])
     dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
     w, spp = 1024, 16
-    # rows around the image centre: a band that sees spheres, walls and lights
+    # rows around the image centre (all 1024 rows by default: the whole frame)
+    rows = max(1, min(rows, 1024))
     r0 = 512 - rows // 2
-    dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + 1) * w, nthreads=threads)  # warm-up
+    dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, 512 * w, 513 * w, nthreads=threads)  # warm-up
     t = time.perf_counter()
     _, cnt = dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + rows) * w, nthreads=threads)
     dt = time.perf_counter() - t
     return {"value": float(cnt.sum()) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "image rows %d-%d of the c2 frame (%d pixels x 16 spp, %d rays), one timed run (%s)"
-                      % (r0, r0 + rows - 1, rows * w, int(cnt.sum()), cpu_info())}
+            "sample": "image rows %d-%d of the c2 frame (%d pixels x 16 spp, %d rays), one timed run of %.1f s (%s)"
+                      % (r0, r0 + rows - 1, rows * w, int(cnt.sum()), dt, cpu_info())}
 
 
 def cpu_baseline_c0(c0, threads):
@@ -360,7 +361,8 @@ def main():
     ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
-    ap.add_argument("--cpu-rows", type=int, default=64, help="c2 image rows in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=1024,
+                    help="c2 image rows in the CPU baseline sample (1024: the whole frame)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (3 times): the reference "
                          "point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -420,11 +422,11 @@ def main():
         if c0 is not None:
             cpu0 = cpu_baseline_c0(c0, threads)
         if c1 is not None:
-            cpu1 = cpu_baseline_c1(c1, threads, 16384)
+            cpu1 = cpu_baseline_c1(c1, threads, 1 << 18)
         if c3 is not None:
-            cpu3 = cpu_baseline_c3(c3, threads, 4)
+            cpu3 = cpu_baseline_c3(c3, threads, 16)
         if c4 is not None:
-            cpu4 = cpu_baseline_c4(c4, threads, 16)
+            cpu4 = cpu_baseline_c4(c4, threads, 64)
     comps = {}
     if c3 is not None:
         comps["c3_batch"] = {

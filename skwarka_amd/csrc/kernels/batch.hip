@@ -58,6 +58,11 @@ __global__ void k_batch_reset(BatchArgs a) {
   }
 }
 
+// Lanes a wave gathers before it starts new rays / tests leaves (as the single-scene kernel's
+// defaults).
+constexpr uint32_t kRefillMin = 8;
+constexpr uint32_t kLeafMin = 16;
+
 // Marks len units of item done; the wave that completes the item records the wall clock.
 __device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint64_t len, uint64_t total,
                                         uint32_t lane) {
@@ -116,7 +121,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
       w.set_root(sc->nodes, sc->root);
       w.tris = sc->tris;
     }
-    if (idle != 0 && cursor < end) {
+    // Refill, leaf and advance phases as in the single-scene kernel (traverse.hip): refills and
+    // leaf tests wait until kRefillMin / kLeafMin lanes need them, unless nothing else can move
+    // or the chunk has no rays left to start.
+    if (idle != 0 && cursor < end && (idle == ~0ull || (uint32_t)__popcll(idle) >= kRefillMin)) {
       const uint64_t avail = end - cursor;
       const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
       if (!busy && rank < avail) {
@@ -129,9 +137,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
       const uint64_t took = (uint64_t)__popcll(idle);
       cursor += took < avail ? took : avail;
     }
-    if (busy && w.step()) {
-      store_hit(a.hits + sc->hit_offset + idx, w.tris, w.found, w.best, w.tmax, w.r);
-      busy = false;
+    const uint64_t held = __ballot(busy && w.at_leaf);
+    if (held) {
+      const uint64_t moving = __ballot(busy && !w.at_leaf);
+      if ((uint32_t)__popcll(held) >= kLeafMin || moving == 0 || cursor >= end) {
+        if (busy && w.at_leaf) w.leaf();
+      }
+    }
+    const bool nan_free = __ballot(busy && !w.nan_free) == 0;
+    if (busy && !w.at_leaf) {
+      const bool done = nan_free ? w.template advance<true>() : w.template advance<false>();
+      if (done) {
+        store_hit(a.hits + sc->hit_offset + idx, w.tris, w.found, w.best, w.tmax, w.r);
+        busy = false;
+      }
     }
   }
 }
