@@ -166,6 +166,7 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
   std::memset(&a, 0, sizeof(a));
   a.nodes = map->d_nodes;
   a.photons = map->d_photons;
+  a.psplit = map->d_psplit;
   a.perm = map->d_perm;
   a.cdir = map->d_cdir;
   a.n_nodes = map->host.nodes.size();
@@ -178,8 +179,8 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
   a.rho = rho;
   a.sin_t = std::sqrt(1.0 - 0.87 * 0.87);  // PhotonFilter's sin threshold for cos 0.87
   a.filter = filter;
-  a.q_stride = 1;
-  a.j_stride = nq;
+  a.q_stride = k;  // a query's heap slots are contiguous
+  a.j_stride = 1;
   a.stack_need = map->host.max_depth + 1;
   return a;
 }
@@ -207,7 +208,7 @@ GeomArgs skw_scene::geom_args() const {
 }
 
 skw_photon_map::~skw_photon_map() {
-  void *ptrs[] = {d_nodes, d_photons, d_perm, d_cdir};
+  void *ptrs[] = {d_nodes, d_photons, d_psplit, d_perm, d_cdir};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -370,6 +371,7 @@ skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weig
     SKW_HIP(hipGetDevice(&m->device), "hipGetDevice");
     SKW_HIP(upload(&m->d_nodes, m->host.nodes), "upload kd nodes");
     SKW_HIP(upload(&m->d_photons, m->host.photons), "upload photons");
+    SKW_HIP(upload(&m->d_psplit, m->host.psplit), "upload parent splits");
     SKW_HIP(upload(&m->d_perm, m->host.perm), "upload permutation");
     std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
     SKW_HIP(upload(&m->d_cdir, table), "upload direction table");

@@ -675,6 +675,14 @@ void build_photon_tree(uint64_t n, const float *pos, const float *weight, const 
     for (int c = 0; c < 3; ++c) ph.w[c] = weight ? weight[3 * (size_t)it.id + c] : 0.0f;
     ph.codes = (uint32_t)(dir ? dir[it.id] : 0) | ((uint32_t)(normal ? normal[it.id] : 0) << 16);
   }
+  out.psplit.assign(n, 0.0f);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t w = out.nodes[i].word, axis = w & 3u;
+    if (axis == 3u) continue;
+    const float sv = out.nodes[i].p[axis];
+    if (w & 4u) out.psplit[i + 1] = sv;
+    if ((w >> 3) < n) out.psplit[w >> 3] = sv;
+  }
 }
 
 const double *cdir_table() {

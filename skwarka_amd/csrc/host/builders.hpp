@@ -49,6 +49,7 @@ struct KdLayout {
   std::vector<DevKdNode> nodes;
   std::vector<DevPhoton> photons;
   std::vector<uint32_t> perm;  // tree position -> caller's photon index
+  std::vector<float> psplit;   // tree position -> the parent's split coordinate (0 for the root)
   uint32_t max_depth = 0;
 };
 

@@ -79,6 +79,7 @@ struct skw_photon_map {
   int device = 0;
   skw::DevKdNode *d_nodes = nullptr;
   skw::DevPhoton *d_photons = nullptr;
+  float *d_psplit = nullptr;
   uint32_t *d_perm = nullptr;
   double *d_cdir = nullptr;
   ~skw_photon_map();

@@ -1,6 +1,7 @@
-// Photon-map gather kernels on gfx950: one query per lane, lookup stack in LDS (global memory for
-// trees deeper than 64), heap slots in device memory laid out slot-major so the lanes of a wave
-// touch consecutive addresses. The lookup, heap and estimates are gather_device.hpp's.
+// Photon-map gather kernels on gfx950: one query per lane (all of a launch's queries start at
+// once, so no query waits behind another), the pending far sides of the lookup in LDS (global
+// memory for trees deeper than 64), heap slots per query contiguous in device memory. The lookup,
+// heap and estimates are gather_device.hpp's.
 #include <hip/hip_runtime.h>
 
 #include "gather_device.hpp"
@@ -30,7 +31,7 @@ __global__ __launch_bounds__(64) void k_gather_global(GatherArgs a) {
 
 template <int MODE>
 hipError_t launch_mode(const GatherArgs &a, hipStream_t stream) {
-  uint32_t grid = (uint32_t)((a.nq + kWave - 1) / kWave);
+  const uint32_t grid = (uint32_t)((a.nq + kWave - 1) / kWave);
   if (a.stack_need <= 32)
     hipLaunchKernelGGL((k_gather_lds<MODE, 32>), dim3(grid), dim3(kWave), 0, stream, a);
   else if (a.stack_need <= 64)

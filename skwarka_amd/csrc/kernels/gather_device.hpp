@@ -59,7 +59,7 @@ __device__ __forceinline__ bool np_less(double da, uint32_t pa, double db, uint3
 }
 
 // libstdc++ __push_heap
-__device__ void push_heap_impl(const Heap &h, int64_t hole, int64_t top, uint32_t vp, double vd) {
+__device__ void push_heap_impl(Heap h, int64_t hole, int64_t top, uint32_t vp, double vd) {
   int64_t parent = (hole - 1) / 2;
   while (hole > top && np_less(h.d(parent), h.p(parent), vd, vp)) {
     h.set(hole, h.p(parent), h.d(parent));
@@ -70,7 +70,7 @@ __device__ void push_heap_impl(const Heap &h, int64_t hole, int64_t top, uint32_
 }
 
 // libstdc++ __adjust_heap
-__device__ void adjust_heap(const Heap &h, int64_t hole, int64_t len, uint32_t vp, double vd) {
+__device__ void adjust_heap(Heap h, int64_t hole, int64_t len, uint32_t vp, double vd) {
   const int64_t top = hole;
   int64_t child = hole;
   while (child < (len - 1) / 2) {
@@ -87,7 +87,7 @@ __device__ void adjust_heap(const Heap &h, int64_t hole, int64_t len, uint32_t v
   push_heap_impl(h, hole, top, vp, vd);
 }
 
-__device__ void make_heap(const Heap &h, int64_t len) {
+__device__ void make_heap(Heap h, int64_t len) {
   if (len < 2) return;
   int64_t parent = (len - 2) / 2;
   while (true) {
@@ -100,7 +100,7 @@ __device__ void make_heap(const Heap &h, int64_t len) {
 }
 
 // pop_heap(first, first + len): moves the top to len - 1 and re-heaps [0, len - 1).
-__device__ void pop_heap(const Heap &h, int64_t len) {
+__device__ void pop_heap(Heap h, int64_t len) {
   if (len > 1) {
     int64_t last = len - 1;
     uint32_t vp = h.p(last);
@@ -110,7 +110,7 @@ __device__ void pop_heap(const Heap &h, int64_t len) {
   }
 }
 
-__device__ void push_heap(const Heap &h, int64_t len) {
+__device__ void push_heap(Heap h, int64_t len) {
   uint32_t vp = h.p(len - 1);
   double vd = h.d(len - 1);
   push_heap_impl(h, len - 1, 0, vp, vd);
@@ -130,85 +130,149 @@ __device__ __forceinline__ double axis_coord(const Query &q, uint32_t axis) {
 }
 
 // PhotonFilter::operator() (PhotonInternalTypes.h:128-134); to = photon - point, already formed
-// by the distance computation with the same subtraction.
-__device__ __forceinline__ bool photon_filter(const GatherArgs &a, const Query &q, uint32_t pos, double dx, double dy,
-                                              double dz) {
-  uint32_t code = a.photons[pos].codes >> 16;
-  const double *nv = a.cdir + 3 * (size_t)code;
+// by the distance computation with the same subtraction; codes = the photon's packed direction
+// and normal ids.
+__device__ __forceinline__ bool photon_filter(const GatherArgs &a, const Query &q, uint32_t codes, double dx,
+                                              double dy, double dz) {
+  const double *nv = a.cdir + 3 * (size_t)(codes >> 16);
   double c = nv[0] * q.n0 + nv[1] * q.n1 + nv[2] * q.n2;
   if (!(c > kCosThreshold)) return false;
   double along = dx * q.n0 + dy * q.n1 + dz * q.n2;
   return fabs(along) < q.sin_t * sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-template <int MODE, class Stack>
-__device__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, const Stack &stk, uint32_t &size,
-                       int64_t &nearest, const Heap &heap) {
-  if (a.n_nodes == 0) return;
+// Adds a photon's weight to the exitant (ext) or the incoming-side sums (PhotonLTEIntegrator.cpp:
+// 150-153). Both sums are updated with value selects (the other one gains +0.0, which leaves a
+// sum of non-negative weights unchanged bit for bit); an if/else over the sums would let the
+// compiler select their addresses and move them to scratch memory.
+__device__ __forceinline__ void add_split(bool ext, const float4 &ph, double &e0, double &e1, double &e2, double &i0,
+                                          double &i1, double &i2) {
+  const double x = ph.x, y = ph.y, z = ph.z;
+  e0 += ext ? x : 0.0;
+  e1 += ext ? y : 0.0;
+  e2 += ext ? z : 0.0;
+  i0 += ext ? 0.0 : x;
+  i1 += ext ? 0.0 : y;
+  i2 += ext ? 0.0 : z;
+}
+
+// The disc estimate's sums (PhotonLTEIntegrator.cpp:139-153) kept while the set is filled: until
+// the set holds k photons it is an array in insertion order, which is the order the estimate
+// sums it in, so adding each photon as it is inserted gives the same sums bit for bit. Once the
+// set is full (make_heap reorders it), `full` is set and the sums are taken from the final array.
+struct DiscSums {
+  double ext0, ext1, ext2, in0, in1, in2, mx;
+  bool full;
+};
+
+// Visits node `cur` of a lookup (KDTree.h:545-575): tests its photon (strict d^2 < max_d2, then
+// the PhotonFilter, then the bounded set), then picks the children. Returns true when the node
+// has a near child (set in `cur`); `far` / `axis` / `split` describe a far child to visit later
+// when `far_exists`.
+template <int MODE, bool SUMS>
+__device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint32_t &cur, double &max_d2,
+                                      uint32_t &size, int64_t &nearest, const Heap &heap, DiscSums &sums,
+                                      bool &far_exists, uint32_t &far, uint32_t &axis, float &split) {
   const uint32_t k = a.k;
+  // The node and its photon record are independent loads, issued together.
+  const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
+  const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
+  double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
+  double d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 < max_d2) {
+    const uint32_t codes = __float_as_uint(ph.w);
+    bool pass = a.filter == 0 || photon_filter(a, q, codes, dx, dy, dz);
+    if (pass) {
+      if (MODE == 1) {
+        nearest = cur;
+        max_d2 = d2;
+      } else if (size < k) {
+        heap.set(size, cur, d2);
+        ++size;
+        if (SUMS) {
+          if (d2 > sums.mx) sums.mx = d2;
+          const double *dv = a.cdir + 3 * (size_t)(codes & 0xFFFFu);
+          const bool ext = dv[0] * q.n0 + dv[1] * q.n1 + dv[2] * q.n2 > 0.0;
+          add_split(ext, ph, sums.ext0, sums.ext1, sums.ext2, sums.in0, sums.in1, sums.in2);
+        }
+        if (size == k) {
+          if (SUMS) sums.full = true;
+          make_heap(heap, size);
+          max_d2 = heap.d(0);
+        }
+      } else {
+        pop_heap(heap, size);
+        heap.set(k - 1, cur, d2);
+        push_heap(heap, size);
+        max_d2 = heap.d(0);
+      }
+    }
+  }
+  const uint32_t w = __float_as_uint(nd.w);
+  axis = w & 3u;
+  far_exists = false;
+  if (axis == 3u) return false;  // leaf
+  const double qa = axis_coord(q, axis);
+  // a value select (a conditional over the members would select their addresses: scratch)
+  const float sx = nd.x, sy = nd.y, sz = nd.z;
+  split = axis == 0 ? (float)sx : (axis == 1 ? (float)sy : (float)sz);
+  const bool has_left = (w & 4u) != 0;
+  const uint32_t right = w >> 3;
+  const bool has_right = right < a.n_nodes;
+  const bool left_near = qa <= (double)split;
+  const bool near_exists = left_near ? has_left : has_right;
+  far_exists = left_near ? has_right : has_left;
+  far = left_near ? right : cur + 1;
+  if (near_exists) cur = left_near ? cur + 1 : right;
+  return near_exists;
+}
+
+// The whole lookup of one query. The stack holds pending far sides as (far child << 2) | axis.
+// A far side is pushed only when its plane is inside the radius at push time, and re-tested when
+// popped only if the radius has shrunk since the lookup began (the plane coordinate then comes
+// from psplit, the parent's split per node): the radius only shrinks, so a side that fails at
+// push time fails at pop time too, and while the radius is unchanged the push-time test stands.
+// This visits the nodes the recursion visits (KDTree.h:558-575), in its order, without
+// re-reading the parent node at each pop.
+template <int MODE, bool SUMS, class Stack>
+__device__ __forceinline__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, const Stack &stk,
+                                       uint32_t &size, int64_t &nearest, const Heap &heap, DiscSums &sums) {
+  if (a.n_nodes == 0) return;
+  const double r2 = max_d2;
   uint32_t sp = 0;
   uint32_t cur = 0;
   bool have = true;
   while (true) {
     if (!have) {
-      // Pop the deepest pending far side; visit it if the plane is still inside the radius.
       bool next = false;
       while (sp > 0) {
-        uint32_t par = stk.get(--sp);
-        float4 nd = reinterpret_cast<const float4 *>(a.nodes)[par];
-        uint32_t w = __float_as_uint(nd.w);
-        uint32_t axis = w & 3u;
-        double qa = axis_coord(q, axis);
-        double split = axis == 0 ? (double)nd.x : (axis == 1 ? (double)nd.y : (double)nd.z);
-        double pd2 = (qa - split) * (qa - split);
+        const uint32_t e = stk.get(--sp);
+        const uint32_t far = e >> 2;
+        if (max_d2 == r2) {
+          cur = far;
+          next = true;
+          break;
+        }
+        const double qa = axis_coord(q, e & 3u);
+        const double sd = (double)a.psplit[far];
+        const double pd2 = (qa - sd) * (qa - sd);
         if (pd2 < max_d2) {
-          cur = (qa <= split) ? (w >> 3) : par + 1;
+          cur = far;
           next = true;
           break;
         }
       }
       if (!next) break;
     }
-    have = false;
-    float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
-    double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
-    double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 < max_d2) {
-      bool pass = a.filter == 0 || photon_filter(a, q, cur, dx, dy, dz);
-      if (pass) {
-        if (MODE == 1) {
-          nearest = cur;
-          max_d2 = d2;
-        } else if (size < k) {
-          heap.set(size, cur, d2);
-          ++size;
-          if (size == k) {
-            make_heap(heap, size);
-            max_d2 = heap.d(0);
-          }
-        } else {
-          pop_heap(heap, size);
-          heap.set(k - 1, cur, d2);
-          push_heap(heap, size);
-          max_d2 = heap.d(0);
-        }
-      }
-    }
-    uint32_t w = __float_as_uint(nd.w);
-    uint32_t axis = w & 3u;
-    if (axis == 3u) continue;  // leaf
-    double qa = axis_coord(q, axis);
-    double split = axis == 0 ? (double)nd.x : (axis == 1 ? (double)nd.y : (double)nd.z);
-    bool has_left = (w & 4u) != 0;
-    uint32_t right = w >> 3;
-    bool has_right = right < a.n_nodes;
-    bool left_near = qa <= split;
-    bool near_exists = left_near ? has_left : has_right;
-    bool far_exists = left_near ? has_right : has_left;
-    if (far_exists) stk.set(sp++, cur);
-    if (near_exists) {
-      cur = left_near ? cur + 1 : right;
-      have = true;
+    bool far_exists;
+    uint32_t far, axis;
+    float split;
+    have = visit<MODE, SUMS>(a, q, cur, max_d2, size, nearest, heap, sums, far_exists, far, axis, split);
+    if (far_exists) {
+      const double qa = axis_coord(q, axis);
+      const double sd = (double)split;
+      const double pd2 = (qa - sd) * (qa - sd);
+      if (pd2 < max_d2) stk.set(sp++, (far << 2) | axis);
     }
   }
 }
@@ -217,7 +281,7 @@ __device__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, cons
 // reference): photon weight 1 - d / (kc * r_max), only photons arriving from the normal's side,
 // normalised by (1 - 2 / (3 kc)) * pi * paths * r_max^2, with r_max^2 = the set's largest d^2, or
 // r^2 when the set is not full.
-__device__ void cone_estimate(const GatherArgs &a, const Query &q, uint64_t i, const Heap &heap, uint32_t size) {
+__device__ __forceinline__ void cone_estimate(const GatherArgs &a, const Query &q, uint64_t i, Heap heap, uint32_t size) {
   float *out = a.e_ext + 3 * i;
   if (size == 0) {
     out[0] = out[1] = out[2] = 0.0f;
@@ -247,7 +311,7 @@ __device__ void cone_estimate(const GatherArgs &a, const Query &q, uint64_t i, c
 // a Lambertian surface of albedo rho: Simpson kernel 3/pi (1 - d^2 / m)^2 with m the running
 // maximum of d^2 in the set's array order, BSDF value rho/pi when photon direction and wo lie on
 // the same side of the normal; then / (paths * max_d2) with the disc estimate's n / (n - 0.5) rule.
-__device__ void caustic_estimate(const GatherArgs &a, const Query &q, uint64_t i, const Heap &heap, uint32_t size) {
+__device__ __forceinline__ void caustic_estimate(const GatherArgs &a, const Query &q, uint64_t i, Heap heap, uint32_t size) {
   double *out = a.caustic + 3 * i;
   if (size == 0) {
     out[0] = out[1] = out[2] = 0.0;
@@ -283,8 +347,7 @@ __device__ void caustic_estimate(const GatherArgs &a, const Query &q, uint64_t i
   out[2] = r2 * inv;
 }
 
-template <int MODE, class Stack>
-__device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
+__device__ __forceinline__ Query load_query(const GatherArgs &a, uint64_t i) {
   Query q;
   const double *pp = a.points + 3 * i;
   q.q0 = pp[0];
@@ -299,16 +362,24 @@ __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
     q.n0 = q.n1 = q.n2 = 0.0;
   }
   q.sin_t = a.sin_t;
-  double max_d2 = a.r * a.r;
-  uint32_t size = 0;
-  int64_t nearest = -1;
-  Heap heap{a.heap_pos + i * a.q_stride, a.heap_d2 + i * a.q_stride, a.j_stride};
+  return q;
+}
+
+__device__ __forceinline__ Heap query_heap(const GatherArgs &a, uint64_t i) {
+  return Heap{a.heap_pos + i * a.q_stride, a.heap_d2 + i * a.q_stride, a.j_stride};
+}
+
+// Outputs of query i once its lookup has ended: the nearest photon (MODE 1), or the set's size,
+// ids, d^2 and the requested estimate (MODE 0).
+// (Values, not references: a call that is not inlined would otherwise keep the caller's walk
+// state in scratch memory.)
+template <int MODE>
+__device__ __forceinline__ void finish_query(const GatherArgs &a, const Query q, uint64_t i, const Heap heap,
+                                             uint32_t size, int64_t nearest, const DiscSums sums) {
   if (MODE == 1) {
-    lookup<1>(a, q, max_d2, stk, size, nearest, heap);
     a.nearest[i] = nearest < 0 ? kMiss : a.perm[nearest];
     return;
   }
-  if (a.k > 0) lookup<0>(a, q, max_d2, stk, size, nearest, heap);
   if (a.count) a.count[i] = size;
   if (a.ids || a.d2) {
     for (uint32_t j = 0; j < size; ++j) {
@@ -327,9 +398,10 @@ __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
   }
   if (!a.e_ext) return;
   // PhotonLTEIntegrator.cpp:139-168
-  double ext0 = 0.0, ext1 = 0.0, ext2 = 0.0, in0 = 0.0, in1 = 0.0, in2 = 0.0;
-  double mx = 0.0;
-  for (uint32_t j = 0; j < size; ++j) {
+  double ext0 = sums.ext0, ext1 = sums.ext1, ext2 = sums.ext2, in0 = sums.in0, in1 = sums.in1, in2 = sums.in2;
+  double mx = sums.mx;
+  if (sums.full) ext0 = ext1 = ext2 = in0 = in1 = in2 = mx = 0.0;
+  for (uint32_t j = 0; sums.full && j < size; ++j) {
     uint32_t pos = heap.p(j);
     double t = heap.d(j);
     if (t > mx) mx = t;
@@ -337,15 +409,7 @@ __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
     uint32_t dcode = __float_as_uint(ph.w) & 0xFFFFu;
     const double *dv = a.cdir + 3 * (size_t)dcode;
     double c = dv[0] * q.n0 + dv[1] * q.n1 + dv[2] * q.n2;
-    if (c > 0.0) {
-      ext0 += (double)ph.x;
-      ext1 += (double)ph.y;
-      ext2 += (double)ph.z;
-    } else {
-      in0 += (double)ph.x;
-      in1 += (double)ph.y;
-      in2 += (double)ph.z;
-    }
+    add_split(c > 0.0, ph, ext0, ext1, ext2, in0, in1, in2);
   }
   float e[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (size > 0) {
@@ -368,6 +432,28 @@ __device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
   }
 }
 
+// The sums are kept during the lookup only for the disc estimate.
+__device__ __forceinline__ bool disc_sums(const GatherArgs &a) { return a.estimate == 0 && a.e_ext != nullptr; }
+
+// One query from start to end (one lane, no refill).
+template <int MODE, class Stack>
+__device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
+  const Query q = load_query(a, i);
+  double max_d2 = a.r * a.r;
+  uint32_t size = 0;
+  int64_t nearest = -1;
+  const Heap heap = query_heap(a, i);
+  DiscSums sums{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false};
+  if (MODE == 1) {
+    lookup<1, false>(a, q, max_d2, stk, size, nearest, heap, sums);
+  } else if (a.k > 0) {
+    if (disc_sums(a))
+      lookup<0, true>(a, q, max_d2, stk, size, nearest, heap, sums);
+    else
+      lookup<0, false>(a, q, max_d2, stk, size, nearest, heap, sums);
+  }
+  finish_query<MODE>(a, q, i, heap, size, nearest, sums);
+}
 
 }  // namespace
 }  // namespace gath

@@ -54,6 +54,7 @@ hipError_t launch_hit_geometry(const GeomArgs &args, hipStream_t stream);
 struct GatherArgs {
   const DevKdNode *nodes;
   const DevPhoton *photons;
+  const float *psplit;  // per node: its parent's split coordinate (0 for the root)
   const uint32_t *perm;
   const double *cdir;  // 65536 x 3
   uint64_t n_nodes;
