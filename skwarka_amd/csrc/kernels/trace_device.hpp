@@ -430,7 +430,9 @@ struct Walker {
       float4 lo, hi;
       load_node(nodes, cur, lo, hi);
       keep_live(hi);
-      if (cull<FAST>(lo, hi)) return false;
+      // An occlusion walk never changes tmax (it ends at its first hit), so a sibling that passed
+      // its test when pushed passes it again now: only the closest-hit walk re-tests.
+      if (!ANY && cull<FAST>(lo, hi)) return false;
       ca = __float_as_uint(hi.z);
       cb = __float_as_uint(hi.w);
       if ((cb & 3u) == 0) {
