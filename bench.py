@@ -54,16 +54,22 @@ def cpu_info():
     return model
 
 
-def pmc_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary (profiles/<name>.json)."""
+def pmc_summary(name):
+    """The committed rocprofv3 --pmc summary profiles/<name>.json (HBM bytes per launch, L2 hit
+    rate, VALU figures), or {} when it is absent."""
     path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f)
     except Exception:
-        return None
+        return {}
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary (profiles/<name>.json)."""
+    return pmc_summary(name).get("hbm_bytes_per_launch")
 
 
 def time_steps(torch, fn, steps, warmup, stream):
@@ -482,9 +488,15 @@ def main():
             "roofline": {"bound": "hbm", "achieved": round(c2["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(c2["achieved_gbs"] / HBM_PEAK_GBS, 5),
                          "traffic": pmc_traffic("trace_c2_pmc"),
-                         "kernel": "k_trace_lds (closest + any hit), all launches of a frame",
+                         "kernel": "k_trace_lds (closest + any hit), the 48 trace launches of a frame",
+                         "launches_per_frame": 48,
+                         "alg_bytes_per_launch": c2["alg_bytes"] // 48,
+                         "avg_launch_ms": round(c2["trace_ms"] / 48, 4),
                          "trace_ms_per_frame": round(c2["trace_ms"], 3),
                          "alg_bytes_per_frame": c2["alg_bytes"],
+                         "l2_hit_rate": pmc_summary("trace_c2_pmc").get("l2_hit_rate"),
+                         "valu_issue_utilisation": (pmc_summary("trace_c2_pmc").get("closest_hit") or {}).get(
+                             "valu_issue_utilisation"),
                          "mean_node_tests": [round(x, 3) for x in c2["mean_node_tests"]],
                          "mean_tri_tests": [round(x, 3) for x in c2["mean_tri_tests"]]},
             "stage_ms_per_frame": {k: round(v, 3) for k, v in c2["stage_ms"].items()},
