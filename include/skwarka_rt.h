@@ -283,6 +283,60 @@ skw_status skw_dl_render_tape(const skw_scene *scene, const skw_dl_scene *dl, co
 skw_status skw_dl_tape_adjoint(const skw_dl_tape *tape, const double *d_adjoint, double *d_grad, void *stream);
 skw_status skw_dl_tape_destroy(skw_dl_tape *tape);
 
+/* ---------------------------------------------------------------- photon final gather
+ * The photon integrator's indirect term: an irradiance photon map built from the direct, indirect
+ * and caustic photon maps, and the final gather that reads it through gather rays. */
+
+/* PhotonLTEIntegrator::_ConstructIrradiancePhotonMap (PhotonLTEIntegrator.cpp:171-245): every 10th
+ * photon of the indirect map in tree order becomes an irradiance photon (position and normal of that
+ * photon) whose external / internal irradiance is the sum, in float, of the caustic, direct and
+ * indirect disc estimates at it (_LookupPhotonIrradiance with k = 100 and the radius holding 100
+ * photons of each map on average over scene_area; IrradiancePhotonProcess, PhotonInternalTypes.h:
+ * 269-288); the irradiance photons get a kd-tree, and the lookup distance is the 0.1 % percentile
+ * of the distances from every 11th indirect photon to its nearest irradiance photon (infinite when
+ * that distance is 0). The estimates and lookups run on the device; the tree is built on the host.
+ * caustic may be NULL; direct and indirect are required (device maps). scene_area: the summed
+ * areas of the scene's meshes (TriangleMesh::GetArea per primitive). */
+typedef struct skw_irradiance_map skw_irradiance_map;
+typedef struct {
+  const skw_photon_map *caustic, *direct, *indirect;
+  uint64_t caustic_paths, direct_paths, indirect_paths;
+  double scene_area;
+} skw_irradiance_map_desc;
+skw_status skw_irradiance_map_create(const skw_irradiance_map_desc *desc, skw_irradiance_map **out, void *stream);
+/* An irradiance photon map from given photons (KDTree<IrradiancePhoton> over them): positions (3
+ * floats), normal ids (CompressedDirection), external and internal irradiance (3 floats each), and
+ * the lookup distance. Host arrays. */
+skw_status skw_irradiance_map_create_from(uint64_t n, const float *pos, const uint16_t *normal, const float *e_ext,
+                                          const float *e_int, double max_lookup_dist, skw_irradiance_map **out);
+skw_status skw_irradiance_map_destroy(skw_irradiance_map *map);
+/* Irradiance photons and lookup distance (host outputs, any may be NULL; arrays in creation order). */
+skw_status skw_irradiance_map_info(const skw_irradiance_map *map, uint64_t *n, double *max_lookup_dist);
+skw_status skw_irradiance_map_export(const skw_irradiance_map *map, float *pos, uint16_t *normal, float *e_ext,
+                                     float *e_int);
+
+/* PhotonLTEIntegrator::_FinalGather (PhotonLTEIntegrator.cpp:387-577) at the camera hit of every
+ * sample of a frame (the direct-lighting camera and sample stream, so the shading points are those
+ * of skw_dl_render with the same width, height, spp and seed): the 32 nearest indirect photons
+ * with PhotonFilter, gather_samples BSDF samples and gather_samples samples of 10-degree cones
+ * around those photons' directions with power-heuristic weights, one closest-hit gather ray each,
+ * and at each gather hit the nearest irradiance photon (IrradiancePhotonFilter, within the map's
+ * lookup distance) times the hit surface's total reflection scattering / pi. gather_samples is
+ * rounded up to a square as the reference does without a Sample; its stratified 1D / 2D
+ * sequences are jittered from a counter-based stream of their own. indirect may be NULL (no
+ * photon-direction samples). Outputs (device, any may be NULL): d_rgb = mean over spp per pixel
+ * (3 floats), d_radiance = every sample's value (3 doubles, sample pixel * spp + s). rays_out
+ * (host, optional, 2 values): camera rays, gather rays. */
+typedef struct {
+  uint32_t width, height, spp;
+  uint32_t gather_samples;
+  uint64_t seed;
+} skw_fg_params;
+skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, const skw_irradiance_map *irr,
+                               const skw_photon_map *indirect, double scene_area, const skw_camera *cam,
+                               const skw_fg_params *params, float *d_rgb, double *d_radiance, uint64_t *rays_out,
+                               void *stream);
+
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in
  * skw_dl_render, sample s of pixel p has index p*spp + s) are traced to their closest hit by one

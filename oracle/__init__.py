@@ -58,6 +58,17 @@ def lib():
         "orc_photon_map_dump": ([_P, _P, _P, _P], None),
         "orc_photon_irradiance": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, _P, _P, _P, _P, _P, ctypes.c_int],
                                   None),
+        "orc_photon_nearest": ([_P, _U64, _P, _P, ctypes.c_double, _P, ctypes.c_int], None),
+        "orc_photon_map_points": ([_P, _P, _P, _P], None),
+        "orc_photon_map_size": ([_P], _U64),
+        "orc_irr_map_create": ([_P, _P, _P, _U64, _U64, _U64, ctypes.c_double, ctypes.c_int], _P),
+        "orc_irr_map_create_from": ([_U64, _P, _P, _P, _P, ctypes.c_double], _P),
+        "orc_irr_map_destroy": ([_P], None),
+        "orc_irr_map_info": ([_P, _P, _P], None),
+        "orc_irr_map_export": ([_P, _P, _P, _P, _P], None),
+        "orc_irr_map_nearest": ([_P, _U64, _P, _P, ctypes.c_double, _P, ctypes.c_int], None),
+        "orc_fg_render": ([_P, _P, _P, _P, ctypes.c_double, _P, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P,
+                           ctypes.c_int], None),
         "orc_photon_caustic": ([_P, _U64, _P, _P, _P, _U32, ctypes.c_double, _U64, ctypes.c_double, _P,
                                 ctypes.c_int], None),
         "orc_photon_cone": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, ctypes.c_double, _P, ctypes.c_int],
@@ -270,6 +281,23 @@ class PhotonMap:
                                  out.ctypes.data, nthreads or threads())
         return out
 
+    def nearest(self, p, n, r, nthreads=None):
+        """GetNearestPoint with PhotonFilter within r: original index or 0xFFFFFFFF."""
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        out = np.zeros(len(p), np.uint32)
+        lib().orc_photon_nearest(self.h, len(p), p.ctypes.data, n.ctypes.data, r, out.ctypes.data,
+                                 nthreads or threads())
+        return out
+
+    def points(self):
+        """Tree order (GetAllPoints): positions, dir ids, normal ids."""
+        pos = np.zeros((self.n, 3), np.float32)
+        d = np.zeros(self.n, np.uint16)
+        nn = np.zeros(self.n, np.uint16)
+        lib().orc_photon_map_points(self.h, pos.ctypes.data, d.ctypes.data, nn.ctypes.data)
+        return pos, d, nn
+
     def cone(self, p, n, k, r, paths, kc, nthreads=None):
         p = np.ascontiguousarray(p, np.float64)
         n = np.ascontiguousarray(n, np.float64)
@@ -277,6 +305,63 @@ class PhotonMap:
         lib().orc_photon_cone(self.h, len(p), p.ctypes.data, n.ctypes.data, k, r, paths, kc, out.ctypes.data,
                               nthreads or threads())
         return out
+
+
+class IrradianceMap:
+    """Irradiance photon map (_ConstructIrradiancePhotonMap) built from direct / indirect / caustic
+    PhotonMaps, or (explicit=...) from given photons: (pos, normal ids, e_ext, e_int, max_dist)."""
+
+    def __init__(self, direct=None, indirect=None, caustic=None, paths=(1, 1, 1), scene_area=1.0, explicit=None,
+                 nthreads=None):
+        self._keep = (direct, indirect, caustic)
+        if explicit is not None:
+            pos, nid, ee, ei, md = explicit
+            self._arr = [np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(nid, np.uint16),
+                         np.ascontiguousarray(ee, np.float32), np.ascontiguousarray(ei, np.float32)]
+            self.h = lib().orc_irr_map_create_from(len(self._arr[1]), *[a.ctypes.data for a in self._arr], md)
+        else:
+            cp, dp, ip = paths
+            self.h = lib().orc_irr_map_create(caustic.h if caustic else None, direct.h, indirect.h, cp, dp, ip,
+                                              scene_area, nthreads or threads())
+        n = ctypes.c_uint64()
+        md = ctypes.c_double()
+        lib().orc_irr_map_info(self.h, ctypes.byref(n), ctypes.byref(md))
+        self.n, self.max_dist = n.value, md.value
+
+    def __del__(self):
+        try:
+            lib().orc_irr_map_destroy(self.h)
+        except Exception:
+            pass
+
+    def export(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        nid = np.zeros(self.n, np.uint16)
+        ee = np.zeros((self.n, 3), np.float32)
+        ei = np.zeros((self.n, 3), np.float32)
+        lib().orc_irr_map_export(self.h, pos.ctypes.data, nid.ctypes.data, ee.ctypes.data, ei.ctypes.data)
+        return pos, nid, ee, ei
+
+    def nearest(self, p, n, r, nthreads=None):
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        out = np.zeros(len(p), np.uint32)
+        lib().orc_irr_map_nearest(self.h, len(p), p.ctypes.data, n.ctypes.data, r, out.ctypes.data,
+                                  nthreads or threads())
+        return out
+
+
+def fg_render(accel, dls, irr, indirect, scene_area, cam, width, height, spp, gather_samples, seed, p0=0, p1=None,
+              nthreads=None):
+    """Final-gather radiance (3 doubles) of every sample of pixels [p0, p1); returns (radiance, gather rays)."""
+    p1 = width * height if p1 is None else p1
+    out = np.zeros(((p1 - p0) * spp, 3), np.float64)
+    rays = ctypes.c_uint64()
+    cam = np.ascontiguousarray(cam, np.float64)
+    lib().orc_fg_render(accel.h, dls.h, irr.h, indirect.h if indirect is not None else None, scene_area,
+                        cam.ctypes.data, width, height, spp, gather_samples, seed, p0, p1, out.ctypes.data,
+                        ctypes.byref(rays), nthreads or threads())
+    return out, rays.value
 
 
 class DLScene:

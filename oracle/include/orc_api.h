@@ -79,6 +79,16 @@ void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const doub
                      uint64_t seed, uint64_t p0, uint64_t p1, const double *adj, double *grad, double *img,
                      int nthreads);
 
+typedef struct orc_irr_map orc_irr_map;
+typedef struct orc_photon_map orc_photon_map;
+/* PhotonLTEIntegrator::_FinalGather (PhotonLTEIntegrator.cpp:387-577) at the camera hit of every sample
+ * of pixels [p0, p1) (the direct-lighting camera stream): radiance = 3 doubles per sample, sample
+ * (pix - p0) * spp + s. gather_samples is rounded up to a square. indirect may be NULL. */
+void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                   uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1,
+                   double *radiance, uint64_t *gather_rays, int nthreads);
+
 /* Generic point kd-tree (KDTree<Point3D_d> / KDTree<Point3D_f>) for the reference's own tests. */
 orc_kdtree *orc_kdtree_create_d(uint64_t n, const double *pts);
 orc_kdtree *orc_kdtree_create_f(uint64_t n, const float *pts);
@@ -94,7 +104,6 @@ void orc_kdtree_nearest(const orc_kdtree *h, uint64_t nq, const double *q, const
                         uint32_t *id, int nthreads);
 
 /* Photon map: KDTree<Photon> with PhotonFilter and the density estimators. */
-typedef struct orc_photon_map orc_photon_map;
 orc_photon_map *orc_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
                                       const uint16_t *normal);
 void orc_photon_map_destroy(orc_photon_map *h);
@@ -109,6 +118,29 @@ void orc_photon_caustic(const orc_photon_map *h, uint64_t nq, const double *p, c
 /* Cone-filtered irradiance (Jensen: w = 1 - d/(kc*r_max), normalised by 1 - 2/(3 kc)). */
 void orc_photon_cone(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
                      double r, uint64_t paths, double kc, float *e_ext, int nthreads);
+
+/* GetNearestPoint with PhotonFilter within r: original index or 0xFFFFFFFF. */
+void orc_photon_nearest(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, double r, uint32_t *id,
+                        int nthreads);
+/* The map's photons in tree order (GetAllPoints): positions, dir and normal codes (either may be NULL). */
+void orc_photon_map_points(const orc_photon_map *h, float *pos, uint16_t *dir, uint16_t *normal);
+uint64_t orc_photon_map_size(const orc_photon_map *h);
+
+/* Irradiance photon map (_ConstructIrradiancePhotonMap, PhotonLTEIntegrator.cpp:171-245): every 10th
+ * indirect photon (tree order) with E_ext / E_int = caustic + direct + indirect disc estimates (k = 100,
+ * radii from the scene area), a kd-tree over them, and the lookup distance covering 99.9 % of the
+ * indirect photons. caustic may be NULL. */
+orc_irr_map *orc_irr_map_create(const orc_photon_map *caustic, const orc_photon_map *direct,
+                                const orc_photon_map *indirect, uint64_t caustic_paths, uint64_t direct_paths,
+                                uint64_t indirect_paths, double scene_area, int nthreads);
+orc_irr_map *orc_irr_map_create_from(uint64_t n, const float *pos, const uint16_t *normal, const float *e_ext,
+                                     const float *e_int, double max_dist);
+void orc_irr_map_destroy(orc_irr_map *m);
+void orc_irr_map_info(const orc_irr_map *m, uint64_t *n, double *max_dist);
+/* Creation order: positions (3 floats), normal codes, E_ext and E_int (3 floats each). */
+void orc_irr_map_export(const orc_irr_map *m, float *pos, uint16_t *normal, float *e_ext, float *e_int);
+void orc_irr_map_nearest(const orc_irr_map *m, uint64_t nq, const double *p, const double *n, double r, uint32_t *id,
+                         int nthreads);
 
 /* CompressedDirection codec (Source/RayLibs/Math/CompressedDirection.{h,cpp}). */
 uint16_t orc_cdir_encode(double x, double y, double z);

@@ -25,6 +25,7 @@
 // Samples come from a counter-based stream (splitmix64 of seed, sample index and dimension) in
 // place of the reference's samplers; the product reproduces the same stream.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -685,6 +686,244 @@ static bool dl_sample(const FrameSpec &F, uint64_t pix, uint32_t si, T L[3], uin
   return !bad;
 }
 
+// ------------------------------------------------------------------ final gather
+
+// Counter-based stream of the final gather's stratified samples (a stream of its own: dims up to
+// 4096 per camera sample).
+static inline double fg_rnd(uint64_t seed, uint64_t sample, uint32_t dim) {
+  uint64_t h = mix64((seed ^ 0x6A09E667F3BCC909ull) ^ mix64(sample * 4096ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+static const double FG_COS_GATHER = 0.9848;  // PhotonLTEIntegrator.cpp:396 (10 degrees)
+
+// MathRoutines::CoordinateSystem (MathRoutines.h:298-313)
+static void coordinate_system(V3 e1, V3 &e2, V3 &e3) {
+  if (std::fabs(e1.x) > std::fabs(e1.y)) {
+    double inv = 1.0 / std::sqrt(e1.x * e1.x + e1.z * e1.z);
+    e2 = V3{-e1.z * inv, 0.0, e1.x * inv};
+  } else {
+    double inv = 1.0 / std::sqrt(e1.y * e1.y + e1.z * e1.z);
+    e2 = V3{0.0, e1.z * inv, -e1.y * inv};
+  }
+  e3 = cross(e1, e2);
+}
+
+// BSDF::TotalScattering(incident, 5x5 stratified samples, BSDF_ALL_REFLECTION) of a matte /
+// plastic surface (BSDF.cpp:209-224): Lambertian::TotalScattering = R (Lambertian.cpp:32-36) plus,
+// for plastic, BxDF::TotalScattering of the microfacet lobe (BxDF.cpp:43-68) with
+// Microfacet::Sample (Microfacet.h:112-128) / BlinnDistribution::Sample (BlinnDistribution.h:81-98).
+// BSDF::TotalScattering hands the BxDFs the world-space incident vector (BSDF.cpp:213-219), which is
+// kept here: the lobe is integrated around the world z axis.
+static void total_scattering(const Bsdf &b, V3 incident, double ts[3]) {
+  double r[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 3; ++c) r[c] = r[c] + b.kd[c];
+  if (b.plastic) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int y = 0; y < 5; ++y)
+      for (int x = 0; x < 5; ++x) {
+        const double u0 = (x + 0.5) * (1.0 / 5), u1 = (y + 0.5) * (1.0 / 5);
+        const double ct = std::pow(u0, 1.0 / (b.e + 1.0));
+        const double st = std::sqrt(smax(0.0, 1.0 - ct * ct));
+        const double phi = u1 * 2.0 * PI_;
+        V3 h{st * std::cos(phi), st * std::sin(phi), ct};
+        if (incident.z * h.z < 0.0) h.z *= -1.0;
+        V3 ex = sub(mul(h, 2.0 * dot(incident, h)), incident);
+        double pdf = ((b.e + 1.0) * std::pow(ct, b.e)) / (2.0 * PI_ * 4.0 * std::fabs(dot(incident, h)));
+        double f[3] = {0.0, 0.0, 0.0};
+        if (incident.z * ex.z < 0.0) {
+          pdf = 0.0;
+        } else {
+          double sc, fr;
+          if (b.micro_eval(incident, ex, sc, fr))
+            for (int c = 0; c < 3; ++c) f[c] = (b.ks[c] * fr) * sc;
+        }
+        if (pdf > 0.0) {
+          const double w = std::fabs(ex.z) / pdf;
+          for (int c = 0; c < 3; ++c) m[c] += w * f[c];
+        }
+      }
+    for (int c = 0; c < 3; ++c) {
+      m[c] = m[c] * (1.0 / 25.0);
+      m[c] = smin(1.0, smax(0.0, m[c]));
+      r[c] = r[c] + m[c];
+    }
+  }
+  for (int c = 0; c < 3; ++c) ts[c] = smin(1.0, smax(0.0, r[c]));
+}
+
+struct FgSpec {
+  const orc_accel *accel;
+  const DLScene *s;
+  const ParamView *v;
+  const orc_irr_map *irr;
+  const orc_photon_map *indirect;
+  const std::vector<uint16_t> *dir_by_id;  // indirect photons' incident-direction codes by caller index
+  const double *cdir;                      // 65536 x 3 decode table
+  const std::vector<float> *irr_ext, *irr_int;
+  double irr_max_dist, lookup_r;
+  const double *cam;
+  uint32_t width, height, spp, gs, gs_sqrt;
+  uint64_t seed;
+};
+
+static void make_bsdf(const DLScene &s, const ParamView &v, uint32_t pr, V3 ng, V3 ns, Bsdf &b) {
+  V3 t;
+  if (std::fabs(ns.x) > std::fabs(ns.y)) {
+    double il = 1.0 / std::sqrt(ns.x * ns.x + ns.z * ns.z);
+    t = V3{-ns.z * il, 0.0, ns.x * il};
+  } else {
+    double il = 1.0 / std::sqrt(ns.y * ns.y + ns.z * ns.z);
+    t = V3{0.0, ns.z * il, -ns.y * il};
+  }
+  V3 t2 = cross(ns, t);
+  V3 tangent;
+  if (len_sqr(t2) > DBL_EPS_) {
+    tangent = cross(t2, ns);
+    normalize(tangent);
+  } else {
+    tangent = t;
+  }
+  b.n = ns;
+  b.ng = ng;
+  b.e1 = tangent;
+  b.e2 = cross(ns, tangent);
+  b.plastic = s.mat[pr] == 1;
+  for (int c = 0; c < 3; ++c) {
+    b.kd[c] = v.kd[3 * pr + c];
+    b.ks[c] = smin(1.0, smax(0.0, s.ks[3 * pr + c]));
+  }
+  b.e = v.e[pr];
+}
+
+// PhotonLTEIntegrator::_FinalGather (PhotonLTEIntegrator.cpp:387-577) at the camera hit of one
+// sample; the stratified sequences come from fg_rnd. Returns the gather rays cast.
+static uint64_t fg_sample(const FgSpec &F, uint64_t pix, uint32_t si, double L[3]) {
+  const DLScene &s = *F.s;
+  const double *cam = F.cam;
+  const uint32_t px = (uint32_t)(pix % F.width), py = (uint32_t)(pix / F.width);
+  const uint64_t sidx = pix * F.spp + si;
+  double ix = (double)px + rnd(F.seed, sidx, 0), iy = (double)py + rnd(F.seed, sidx, 1);
+  double cx = 2.0 * cam[12] * (ix - F.width * 0.5) / F.width;
+  double cy = 2.0 * cam[13] * (iy - F.height * 0.5) / F.height;
+  V3 d{cam[3] * cx + cam[4] * cy + cam[5], cam[6] * cx + cam[7] * cy + cam[8], cam[9] * cx + cam[10] * cy + cam[11]};
+  normalize(d);
+  double ray[8] = {cam[0], cam[1], cam[2], d.x, d.y, d.z, 0.0, DBL_INF_};
+  orc_hit h;
+  orc_hit_geom g;
+  orc_accel_intersect_one(F.accel, ray, &h, &g);
+  for (int c = 0; c < 3; ++c) L[c] = 0.0;
+  if (h.prim == 0xffffffffu || F.gs == 0) return 0;
+  const V3 wo = mul(d, -1.0);
+  const V3 p{g.p[0], g.p[1], g.p[2]}, ng{g.ng[0], g.ng[1], g.ng[2]}, ns{g.ns[0], g.ns[1], g.ns[2]};
+  Bsdf b;
+  make_bsdf(s, *F.v, h.prim, ng, ns, b);
+  const uint32_t gs = F.gs, gq = F.gs_sqrt;
+  const double inv_n = 1.0 / gs, inv_q = 1.0 / gq;
+  // StratifiedSampling1D / 2D with jitter (SamplingRoutines.h:327-373): BSDF 1D [0, gs), BSDF 2D
+  // [gs, 3 gs), direction 1D [3 gs, 4 gs), direction 2D [4 gs, 6 gs).
+  auto s1 = [&](uint32_t base, uint32_t i) { return (i + fg_rnd(F.seed, sidx, base + i)) * inv_n; };
+  auto s2 = [&](uint32_t base, uint32_t i, double &u0, double &u1) {
+    const uint32_t x = i % gq, y = i / gq;
+    u0 = (x + fg_rnd(F.seed, sidx, base + 2 * i)) * inv_q;
+    u1 = (y + fg_rnd(F.seed, sidx, base + 2 * i + 1)) * inv_q;
+  };
+  // the 32 nearest indirect photons with PhotonFilter (array order)
+  uint32_t found = 0;
+  uint32_t ids[32];
+  if (F.indirect && orc_photon_map_size(F.indirect) > 0) {
+    double pp[3] = {p.x, p.y, p.z}, nn[3] = {ng.x, ng.y, ng.z};
+    float e0[3], e1[3];
+    double d2[32];
+    orc_photon_irradiance(F.indirect, 1, pp, nn, 32, F.lookup_r, 1, e0, e1, ids, d2, &found, 1);
+  }
+  const double inv_found = found > 0 ? 1.0 / found : 0.0;
+  V3 pdir[32];
+  for (uint32_t j = 0; j < found; ++j) {
+    const double *dv = F.cdir + 3 * (size_t)(*F.dir_by_id)[ids[j]];
+    pdir[j] = V3{dv[0], dv[1], dv[2]};
+  }
+  const double cone_pdf = 1.0 / (2.0 * PI_ * (1.0 - FG_COS_GATHER));
+  auto photon_pdf_of = [&](V3 ex) {
+    double pp = 0.0;
+    for (uint32_t j = 0; j < found; ++j)
+      if (dot(pdir[j], ex) > FG_COS_GATHER) pp += cone_pdf;
+    return pp * inv_found;
+  };
+  std::vector<V3> gdir;
+  std::vector<std::array<double, 3>> gw;
+  for (uint32_t i = 0; i < gs; ++i) {
+    const double uc = s1(0, i);
+    double u0, u1;
+    s2(gs, i, u0, u1);
+    V3 ex, wo_l, wi_l;
+    double bp, f[3];
+    b.sample(wo, u0, u1, uc, ex, bp, f, wo_l, wi_l);
+    if (bp > 0.0 && !(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
+      const double ppdf = photon_pdf_of(ex);
+      const double w = power_heuristic((double)gs, bp, (double)gs, ppdf);
+      const double k = w * std::fabs(dot(ex, ns)) / bp;
+      gdir.push_back(ex);
+      gw.push_back({f[0] * k, f[1] * k, f[2] * k});
+    }
+  }
+  for (uint32_t i = 0; i < gs && found > 0; ++i) {
+    const size_t si2 = (size_t)(s1(3 * gs, i) * found);
+    double u0, u1;
+    s2(4 * gs, i, u0, u1);
+    V3 e2, e3;
+    coordinate_system(pdir[si2], e2, e3);
+    const double ct = (1.0 - u0) + u0 * FG_COS_GATHER;
+    const double st = std::sqrt(1.0 - ct * ct);
+    const double phi = u1 * 2.0 * PI_;
+    const V3 loc{st * std::cos(phi), st * std::sin(phi), ct};
+    const V3 ex = add(add(mul(e2, loc.x), mul(e3, loc.y)), mul(pdir[si2], loc.z));
+    double f[3];
+    b.evaluate(wo, ex, f);
+    if (!(f[0] == 0.0 && f[1] == 0.0 && f[2] == 0.0)) {
+      const double bp = b.pdf(wo, ex);
+      const double ppdf = photon_pdf_of(ex);
+      const double w = power_heuristic((double)gs, ppdf, (double)gs, bp);
+      const double k = w * std::fabs(dot(ex, ns)) / ppdf;
+      gdir.push_back(ex);
+      gw.push_back({f[0] * k, f[1] * k, f[2] * k});
+    }
+  }
+  double rad[3] = {0.0, 0.0, 0.0};
+  for (size_t i = 0; i < gdir.size(); ++i) {
+    const V3 dir = gdir[i];
+    const double mint = orc_next_min_t(F.accel, h.prim, &g, &dir.x);
+    double gr[8] = {p.x, p.y, p.z, dir.x, dir.y, dir.z, mint, DBL_INF_};
+    orc_hit h2;
+    orc_hit_geom g2;
+    orc_accel_intersect_one(F.accel, gr, &h2, &g2);
+    if (h2.prim == 0xffffffffu) continue;
+    const V3 gd = mul(dir, -1.0);
+    const V3 ng2{g2.ng[0], g2.ng[1], g2.ng[2]}, ns2{g2.ns[0], g2.ns[1], g2.ns[2]};
+    double qp[3] = {g2.p[0], g2.p[1], g2.p[2]}, qn[3] = {ng2.x, ng2.y, ng2.z};
+    uint32_t id = 0xffffffffu;
+    orc_irr_map_nearest(F.irr, 1, qp, qn, F.irr_max_dist, &id, 1);
+    if (id == 0xffffffffu) continue;
+    Bsdf b2;
+    make_bsdf(s, *F.v, h2.prim, ng2, ns2, b2);
+    double ts[3];
+    total_scattering(b2, gd, ts);
+    const bool ext = dot(gd, ng2) > 0.0;
+    const float *es = ext ? &(*F.irr_ext)[3 * id] : &(*F.irr_int)[3 * id];
+    const float *eo = ext ? &(*F.irr_int)[3 * id] : &(*F.irr_ext)[3 * id];
+    for (int c = 0; c < 3; ++c) {
+      double t = 0.0;
+      t = t + ts[c] * (double)es[c];
+      t = t + 0.0 * (double)eo[c];  // no transmission lobe: TotalScattering of BSDF_ALL_TRANSMISSION is 0
+      t = t * INV_PI_;
+      rad[c] = rad[c] + (t * 1.0) * gw[i][c];
+    }
+  }
+  const double inv = 1.0 / (double)gs;
+  for (int c = 0; c < 3; ++c) L[c] = rad[c] * inv;
+  return gdir.size();
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -868,6 +1107,59 @@ void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const doub
     for (uint32_t j = 0; j < n_params; ++j) total[j] += part[j];
   }
   for (uint32_t j = 0; j < n_params; ++j) grad[j] = total[j];
+}
+
+void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                   uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1,
+                   double *radiance, uint64_t *gather_rays, int nthreads) {
+  std::vector<double> table(3 * 65536);
+  orc_cdir_table(table.data());
+  std::vector<uint16_t> dir_by_id;
+  uint64_t n_ind = indirect ? orc_photon_map_size(indirect) : 0;
+  if (n_ind > 0) {
+    std::vector<float> pos(3 * n_ind);
+    std::vector<uint16_t> dir(n_ind);
+    std::vector<uint32_t> perm(n_ind);
+    orc_photon_map_points(indirect, pos.data(), dir.data(), nullptr);
+    orc_photon_map_dump(indirect, perm.data(), nullptr, nullptr);
+    dir_by_id.assign(n_ind, 0);
+    for (uint64_t i = 0; i < n_ind; ++i) dir_by_id[perm[i]] = dir[i];
+  }
+  uint64_t n_irr = 0;
+  double irr_max = 0.0;
+  orc_irr_map_info(irr, &n_irr, &irr_max);
+  std::vector<float> e_ext(3 * n_irr), e_int(3 * n_irr);
+  orc_irr_map_export(irr, nullptr, nullptr, e_ext.data(), e_int.data());
+  FgSpec F;
+  F.accel = accel;
+  F.s = &dls->s;
+  F.v = &dls->nominal;
+  F.irr = irr;
+  F.indirect = indirect;
+  F.dir_by_id = &dir_by_id;
+  F.cdir = table.data();
+  F.irr_ext = &e_ext;
+  F.irr_int = &e_int;
+  F.irr_max_dist = irr_max;
+  // PhotonLTEIntegrator.cpp:460-462: the radius holding 32 indirect photons on average
+  F.lookup_r = n_ind > 0 ? std::sqrt((scene_area / (double)n_ind) * 32 * INV_PI_) : 0.0;
+  F.cam = cam;
+  F.width = width;
+  F.height = height;
+  F.spp = spp;
+  F.gs_sqrt = (uint32_t)std::ceil(std::sqrt((double)gather_samples));
+  F.gs = F.gs_sqrt * F.gs_sqrt;
+  F.seed = seed;
+  uint64_t rays = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : rays)
+  for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix)
+    for (uint32_t si = 0; si < spp; ++si) {
+      double L[3];
+      rays += fg_sample(F, (uint64_t)pix, si, L);
+      for (int c = 0; c < 3; ++c) radiance[3 * (((uint64_t)pix - p0) * spp + si) + c] = L[c];
+    }
+  if (gather_rays) *gather_rays = rays;
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@
 // order (and so the summation order of the estimators) is that library's.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <random>
 #include <vector>
 
@@ -233,6 +234,16 @@ static const double MAX_NORMAL_DEVIATION_COS = 0.87;
 struct PhotonMap {
   KD<Photon> tree;
   explicit PhotonMap(std::vector<Photon> &&v) : tree(std::move(v)) {}
+
+  // GetNearestPoint with PhotonFilter (KDTree.h:484-503, NearestPointProc :293-323); the
+  // IrradiancePhotonFilter of PhotonInternalTypes.h:153-168 makes the same test. Tree position or -1.
+  int64_t nearest(V3 p, V3 n, double r) const {
+    if (tree.pts.empty()) return -1;
+    NearestProc<PhotonFilter> proc{-1, PhotonFilter(&tree, p, n, MAX_NORMAL_DEVIATION_COS)};
+    double max_d2 = r * r;
+    tree.lookup(0, p, proc, max_d2);
+    return proc.best;
+  }
 
   // Returns photons found; arr holds them in the processor's final order.
   size_t knn(V3 p, V3 n, uint32_t k, double r, NP *arr) const {
@@ -497,6 +508,162 @@ void orc_photon_cone(const orc_photon_map *h, uint64_t nq, const double *p, cons
       for (int c = 0; c < 3; ++c) e_ext[3 * i + c] = (float)(ext[c] / norm);
     }
   }
+}
+
+void orc_photon_nearest(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, double r, uint32_t *id,
+                        int nthreads) {
+  const PhotonMap &m = *h->m;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < (int64_t)nq; ++i) {
+    V3 qp{p[3 * i], p[3 * i + 1], p[3 * i + 2]}, qn{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+    const int64_t b = m.nearest(qp, qn, r);
+    id[i] = b < 0 ? 0xffffffffu : m.tree.pts[(size_t)b].id;
+  }
+}
+
+void orc_photon_map_points(const orc_photon_map *h, float *pos, uint16_t *dir, uint16_t *normal) {
+  const PhotonMap &m = *h->m;
+  for (size_t i = 0; i < m.tree.pts.size(); ++i) {
+    const Photon &ph = m.tree.pts[i];
+    for (int c = 0; c < 3; ++c) pos[3 * i + c] = ph.p[c];
+    if (dir) dir[i] = ph.dir;
+    if (normal) normal[i] = ph.normal;
+  }
+}
+
+uint64_t orc_photon_map_size(const orc_photon_map *h) { return h->m->tree.pts.size(); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ irradiance photon map
+
+struct orc_irr_map {
+  orc_photon_map *map = nullptr;  // the irradiance photons (weight = external irradiance)
+  std::vector<float> pos, e_ext, e_int;
+  std::vector<uint16_t> normal;
+  double max_dist = DBL_INF_;
+  ~orc_irr_map() {
+    if (map) orc_photon_map_destroy(map);
+  }
+};
+
+static void irr_build_tree(orc_irr_map *m) {
+  const uint64_t n = m->normal.size();
+  std::vector<uint16_t> dir(n, 0);
+  m->map = orc_photon_map_create(n, m->pos.data(), m->e_ext.data(), dir.data(), m->normal.data());
+}
+
+extern "C" {
+
+// PhotonLTEIntegrator::_ConstructIrradiancePhotonMap (PhotonLTEIntegrator.cpp:171-245) with the
+// per-photon estimate of IrradiancePhotonProcess (PhotonInternalTypes.h:269-288).
+orc_irr_map *orc_irr_map_create(const orc_photon_map *caustic, const orc_photon_map *direct,
+                                const orc_photon_map *indirect, uint64_t caustic_paths, uint64_t direct_paths,
+                                uint64_t indirect_paths, double scene_area, int nthreads) {
+  if (!indirect || !direct) return nullptr;
+  cdir_init();
+  const std::vector<Photon> &ind = indirect->m->tree.pts;  // GetAllPoints: tree order
+  orc_irr_map *m = new orc_irr_map();
+  for (size_t i = 0; i < ind.size(); i += 10) {
+    for (int c = 0; c < 3; ++c) m->pos.push_back(ind[i].p[c]);
+    m->normal.push_back(ind[i].normal);
+  }
+  const uint64_t n = m->normal.size();
+  m->e_ext.assign(3 * n, 0.0f);
+  m->e_int.assign(3 * n, 0.0f);
+  if (n == 0) return m;  // the reference leaves the map null: no lookups find anything
+  const size_t nd = direct->m->tree.pts.size(), ni = ind.size();
+  const double direct_area = nd == 0 ? 0.0 : scene_area / (double)nd;
+  const double r_direct = std::sqrt(direct_area * 100.0 * INV_PI_);
+  const double indirect_area = ni == 0 ? 0.0 : scene_area / (double)ni;
+  const double r_indirect = std::sqrt(indirect_area * 100.0 * INV_PI_);
+  const double r_caustic = std::max(r_direct, r_indirect);
+  std::vector<double> pts(3 * n), nrm(3 * n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double *dv = g_cdir[m->normal[i]];
+    for (int c = 0; c < 3; ++c) {
+      pts[3 * i + c] = (double)m->pos[3 * i + c];
+      nrm[3 * i + c] = dv[c];
+    }
+  }
+  std::vector<float> ce(3 * n, 0.0f), ci(3 * n, 0.0f), de(3 * n), di(3 * n), ie(3 * n), ii(3 * n);
+  if (caustic)
+    orc_photon_irradiance(caustic, n, pts.data(), nrm.data(), 100, r_caustic, caustic_paths, ce.data(), ci.data(),
+                          nullptr, nullptr, nullptr, nthreads);
+  orc_photon_irradiance(direct, n, pts.data(), nrm.data(), 100, r_direct, direct_paths, de.data(), di.data(), nullptr,
+                        nullptr, nullptr, nthreads);
+  orc_photon_irradiance(indirect, n, pts.data(), nrm.data(), 100, r_indirect, indirect_paths, ie.data(), ii.data(),
+                        nullptr, nullptr, nullptr, nthreads);
+  for (uint64_t j = 0; j < 3 * n; ++j) {
+    m->e_ext[j] = (ce[j] + de[j]) + ie[j];
+    m->e_int[j] = (ci[j] + di[j]) + ii[j];
+  }
+  irr_build_tree(m);
+  // The 0.1 % percentile of the nearest-irradiance-photon distances of every 11th indirect photon.
+  const size_t step = 11, nq = (ni + step - 1) / step;
+  std::vector<double> qp(3 * nq), qn(3 * nq), sq(nq, 0.0);
+  for (size_t i = 0; i < nq; ++i) {
+    const Photon &ph = ind[i * step];
+    const double *dv = g_cdir[ph.normal];
+    for (int c = 0; c < 3; ++c) {
+      qp[3 * i + c] = (double)ph.p[c];
+      qn[3 * i + c] = dv[c];
+    }
+  }
+  std::vector<uint32_t> id(nq);
+  orc_photon_nearest(m->map, nq, qp.data(), qn.data(), DBL_INF_, id.data(), nthreads);
+  for (size_t i = 0; i < nq; ++i) {
+    if (id[i] == 0xffffffffu) continue;
+    V3 a{qp[3 * i], qp[3 * i + 1], qp[3 * i + 2]};
+    V3 b{(double)m->pos[3 * id[i]], (double)m->pos[3 * id[i] + 1], (double)m->pos[3 * id[i] + 2]};
+    sq[i] = len_sqr(sub(a, b));
+  }
+  const size_t index = sq.size() <= 1 ? 0 : std::max(sq.size() / 1000, (size_t)1);
+  if (!sq.empty()) {
+    std::nth_element(sq.begin(), sq.begin() + index, sq.end(), std::greater<double>());
+    m->max_dist = sq[index] == 0.0 ? DBL_INF_ : std::sqrt(sq[index]);
+  }
+  return m;
+}
+
+// An irradiance photon map from explicit photons (creation order = caller order).
+orc_irr_map *orc_irr_map_create_from(uint64_t n, const float *pos, const uint16_t *normal, const float *e_ext,
+                                     const float *e_int, double max_dist) {
+  cdir_init();
+  orc_irr_map *m = new orc_irr_map();
+  m->pos.assign(pos, pos + 3 * n);
+  m->normal.assign(normal, normal + n);
+  m->e_ext.assign(e_ext, e_ext + 3 * n);
+  m->e_int.assign(e_int, e_int + 3 * n);
+  m->max_dist = max_dist;
+  irr_build_tree(m);
+  return m;
+}
+
+void orc_irr_map_destroy(orc_irr_map *m) { delete m; }
+
+void orc_irr_map_info(const orc_irr_map *m, uint64_t *n, double *max_dist) {
+  if (n) *n = m->normal.size();
+  if (max_dist) *max_dist = m->max_dist;
+}
+
+void orc_irr_map_export(const orc_irr_map *m, float *pos, uint16_t *normal, float *e_ext, float *e_int) {
+  const size_t n = m->normal.size();
+  if (pos) std::copy(m->pos.begin(), m->pos.begin() + 3 * n, pos);
+  if (normal) std::copy(m->normal.begin(), m->normal.end(), normal);
+  if (e_ext) std::copy(m->e_ext.begin(), m->e_ext.end(), e_ext);
+  if (e_int) std::copy(m->e_int.begin(), m->e_int.end(), e_int);
+}
+
+// The nearest irradiance photon of each query with IrradiancePhotonFilter within r (creation index
+// or 0xFFFFFFFF).
+void orc_irr_map_nearest(const orc_irr_map *m, uint64_t nq, const double *p, const double *n, double r, uint32_t *id,
+                         int nthreads) {
+  if (!m->map) {
+    for (uint64_t i = 0; i < nq; ++i) id[i] = 0xffffffffu;
+    return;
+  }
+  orc_photon_nearest(m->map, nq, p, n, r, id, nthreads);
 }
 
 uint16_t orc_cdir_encode(double x, double y, double z) { return cdir_encode(V3{x, y, z}); }

@@ -34,6 +34,8 @@ EXPORTS = [
     "skw_dl_render_tape", "skw_dl_tape_adjoint", "skw_dl_tape_destroy",
     "skw_batch_create", "skw_batch_create_with_gathers", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
+    "skw_irradiance_map_create", "skw_irradiance_map_create_from", "skw_irradiance_map_destroy",
+    "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
 ]
 
 _P = ctypes.c_void_p
@@ -73,6 +75,15 @@ class DLProfile(ctypes.Structure):
     def as_dict(self):
         return {"stage_ms": dict(zip(self.STAGES, list(self.stage_ms))), "rays": list(self.rays),
                 "node_tests": list(self.node_tests), "tri_tests": list(self.tri_tests)}
+
+
+class IrrMapDesc(ctypes.Structure):
+    _fields_ = [("caustic", _P), ("direct", _P), ("indirect", _P), ("caustic_paths", _U64),
+                ("direct_paths", _U64), ("indirect_paths", _U64), ("scene_area", ctypes.c_double)]
+
+
+class FGParams(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("gather_samples", _U32), ("seed", _U64)]
 
 
 class BatchView(ctypes.Structure):
@@ -148,6 +159,14 @@ def lib():
         "skw_batch_latency": ([_P, _P], ctypes.c_int),
         "skw_batch_rays": ([_P, _U32, _P, _P], ctypes.c_int),
         "skw_batch_ray_count": ([_P, ctypes.POINTER(_U64)], ctypes.c_int),
+        "skw_irradiance_map_create": ([ctypes.POINTER(IrrMapDesc), ctypes.POINTER(_P), _P], ctypes.c_int),
+        "skw_irradiance_map_create_from": ([_U64, _P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(_P)],
+                                           ctypes.c_int),
+        "skw_irradiance_map_destroy": ([_P], ctypes.c_int),
+        "skw_irradiance_map_info": ([_P, _P, _P], ctypes.c_int),
+        "skw_irradiance_map_export": ([_P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
+                                 _P, _P, _P, _P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -341,6 +360,60 @@ class PhotonMap:
               "skw_photon_nearest")
 
 
+class IrradianceMap:
+    """The irradiance photon map of the photon integrator's final gather.
+
+    IrradianceMap(direct=, indirect=, caustic=None, *_paths=, scene_area=) estimates it on the
+    device from device PhotonMaps; IrradianceMap(explicit=(pos, normal_ids, e_ext, e_int,
+    max_lookup_dist)) takes given irradiance photons."""
+
+    def __init__(self, direct=None, indirect=None, caustic=None, direct_paths=1, indirect_paths=1, caustic_paths=1,
+                 scene_area=0.0, explicit=None, stream=0):
+        L = lib()
+        h = _P()
+        self._keep = (direct, indirect, caustic)
+        if explicit is not None:
+            pos, nid, ee, ei, md = explicit
+            pos = np.ascontiguousarray(pos, np.float32)
+            nid = np.ascontiguousarray(nid, np.uint16)
+            ee = np.ascontiguousarray(ee, np.float32)
+            ei = np.ascontiguousarray(ei, np.float32)
+            check(L.skw_irradiance_map_create_from(len(nid), _ptr(pos), _ptr(nid), _ptr(ee), _ptr(ei), md,
+                                                   ctypes.byref(h)), "skw_irradiance_map_create_from")
+        else:
+            desc = IrrMapDesc(caustic.handle if caustic is not None else None, direct.handle, indirect.handle,
+                              caustic_paths, direct_paths, indirect_paths, scene_area)
+            check(L.skw_irradiance_map_create(ctypes.byref(desc), ctypes.byref(h), stream or None),
+                  "skw_irradiance_map_create")
+        self.handle = h
+        n = ctypes.c_uint64()
+        md = ctypes.c_double()
+        check(L.skw_irradiance_map_info(h, ctypes.byref(n), ctypes.byref(md)), "skw_irradiance_map_info")
+        self.n = n.value
+        self.max_lookup_dist = md.value
+
+    def close(self):
+        if self.handle:
+            lib().skw_irradiance_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self):
+        """(pos (n,3) float32, normal ids (n,) uint16, e_ext (n,3) float32, e_int (n,3) float32)."""
+        pos = np.zeros((self.n, 3), np.float32)
+        nid = np.zeros(self.n, np.uint16)
+        ee = np.zeros((self.n, 3), np.float32)
+        ei = np.zeros((self.n, 3), np.float32)
+        check(lib().skw_irradiance_map_export(self.handle, pos.ctypes.data, nid.ctypes.data, ee.ctypes.data,
+                                              ei.ctypes.data), "skw_irradiance_map_export")
+        return pos, nid, ee, ei
+
+
 def camera(origin, target_dir, up, x_fov_deg, width, height):
     """skw_camera for a pinhole looking along target_dir (look-at frame of Transform.cpp:138-165,
     tangents of PerspectiveCamera.cpp:20-34). Returns (Camera, cam15) where cam15 is the flat
@@ -404,6 +477,19 @@ class DLScene:
         rays = np.zeros(3, np.uint64) if count_rays else None
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
                                   rays.ctypes.data if count_rays else None, stream or None), "skw_dl_render")
+        return rays
+
+    def final_gather(self, irr, indirect, scene_area, cam, width, height, spp, gather_samples, d_rgb=None,
+                     d_radiance=None, seed=0, stream=0, count_rays=True):
+        """skw_dl_final_gather: the photon integrator's final gather at every sample's camera hit.
+        Returns (camera rays, gather rays) when count_rays, else only enqueues the frame."""
+        prm = FGParams(width, height, spp, gather_samples, seed)
+        rays = np.zeros(2, np.uint64) if count_rays else None
+        check(lib().skw_dl_final_gather(self.scene.handle, self.handle, irr.handle,
+                                        indirect.handle if indirect is not None else None, scene_area,
+                                        ctypes.byref(cam), ctypes.byref(prm), d_rgb, d_radiance,
+                                        rays.ctypes.data if count_rays else None, stream or None),
+              "skw_dl_final_gather")
         return rays
 
     def render_profiled(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0,

@@ -19,6 +19,7 @@ SOURCES = [
     "abi.cpp",
     "abi_dl.cpp",
     "abi_batch.cpp",
+    "abi_fg.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",

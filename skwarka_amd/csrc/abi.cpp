@@ -407,6 +407,7 @@ struct Estimate {
   double kc = 0.0;
   const double *wo = nullptr;
   double *caustic = nullptr;
+  bool ids_tree = false;  // ids hold tree positions instead of caller indices
 };
 
 static skw_status gather(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
@@ -433,6 +434,7 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.kc = est.kc;
   a.wo = est.wo;
   a.caustic = est.caustic;
+  a.ids_tree = est.ids_tree ? 1 : 0;
   if (mode == 0 && k > 0) {
     size_t elems = (size_t)nq * k;
     SKW_HIP(pool_alloc((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
@@ -457,6 +459,20 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   if (e != hipSuccess) return hip_fail(e, "gather launch");
   return SKW_OK;
 }
+
+}  // extern "C"
+
+namespace skw {
+skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                           uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream) {
+  Estimate est;
+  est.ids_tree = true;
+  return gather(map, d_points, d_normals, nq, k, r, 1, 0.0, 1, 0, nullptr, nullptr, nullptr, d_ids, nullptr, d_count,
+                nullptr, stream, est);
+}
+}  // namespace skw
+
+extern "C" {
 
 skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,
                                  uint64_t nq, uint32_t k, double r, uint64_t paths, double rho, float *d_e_ext,

@@ -291,6 +291,13 @@ skw_status check_theta(const skw_dl_scene *dl, const double *theta) {
 
 }  // namespace
 
+namespace skw {
+void dl_tables(const skw_dl_scene *dl, const DevMat **mats, const double **rad) {
+  *mats = dl->d_mats;
+  *rad = dl->d_rad;
+}
+}  // namespace skw
+
 extern "C" {
 
 skw_status skw_dl_scene_create(const skw_scene *scene, const skw_dl_desc *desc, skw_dl_scene **out) {

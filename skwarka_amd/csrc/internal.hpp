@@ -98,4 +98,8 @@ skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, 
 // attached (heap: nq * k slots, slot-major; stack: only when the tree is deeper than 64).
 GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                        uint32_t k, double r, uint64_t paths, double rho, int filter);
+
+// k nearest photons with PhotonFilter within r, ids as tree positions (row-major, k per query).
+skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                           uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream);
 }  // namespace skw

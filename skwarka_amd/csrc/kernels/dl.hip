@@ -640,6 +640,262 @@ __global__ __launch_bounds__(kSampleBlock) void k_dl_bsdf(DlArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ photon final gather
+// PhotonLTEIntegrator::_FinalGather (PhotonLTEIntegrator.cpp:387-577) as a wavefront over a pass's
+// shading points: lookup inputs -> 32 nearest indirect photons (gather kernel) -> BSDF and cone
+// samples with their gather rays -> closest hit -> hit scattering and irradiance lookup inputs ->
+// nearest irradiance photon (gather kernel) -> per-sample ordered sum.
+
+// Counter-based stream of the final gather's stratified samples (dims up to 4096 per sample).
+__device__ __forceinline__ double fg_rnd(uint64_t seed, uint64_t sample, uint32_t dim) {
+  uint64_t h = mix64((seed ^ 0x6A09E667F3BCC909ull) ^ mix64(sample * 4096ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+constexpr double kCosGather = 0.9848;  // PhotonLTEIntegrator.cpp:396 (10 degrees)
+
+__global__ __launch_bounds__(256) void k_fg_lookup_in(FgArgs f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.a.batch) return;
+  Point pt;
+  const bool hit = load_point(f.a, i, pt);
+  const double nan = __builtin_nan("");
+  const D3 p = hit ? pt.p : D3{nan, nan, nan};
+  const D3 n = hit ? pt.g.ng : D3{0.0, 0.0, 1.0};
+  f.s_pts[3 * i] = p.x;
+  f.s_pts[3 * i + 1] = p.y;
+  f.s_pts[3 * i + 2] = p.z;
+  f.s_nrm[3 * i] = n.x;
+  f.s_nrm[3 * i + 1] = n.y;
+  f.s_nrm[3 * i + 2] = n.z;
+}
+
+__device__ __forceinline__ D3 photon_dir(const FgArgs &f, uint64_t i, uint32_t j) {
+  const uint32_t pos = f.near_ids[i * 32 + j];
+  const double *dv = f.cdir + 3 * (size_t)(f.ind_photons[pos].codes & 0xFFFFu);
+  return D3{dv[0], dv[1], dv[2]};
+}
+
+// The photon-sampling pdf of direction ex: a 10-degree cone around each found photon's direction.
+__device__ __forceinline__ double photon_pdf(const FgArgs &f, uint64_t i, uint32_t found, D3 ex) {
+  const double cone_pdf = 1.0 / (2.0 * kPi * (1.0 - kCosGather));  // UniformConePDF
+  double pp = 0.0;
+  for (uint32_t j = 0; j < found; ++j)
+    if (dot3(photon_dir(f, i, j), ex) > kCosGather) pp += cone_pdf;
+  return pp * (found > 0 ? 1.0 / found : 0.0);
+}
+
+// One thread per (slot, sample k): the BSDF sample k (KIND 0, :470-500) or the cone sample k around
+// a found photon's direction (KIND 1, :502-532), and its gather ray.
+template <int KIND>
+__global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
+  const DlArgs &a = f.a;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gs = f.gs;
+  const uint64_t i = t / gs;
+  const uint32_t k = (uint32_t)(t - i * gs);
+  const bool in = i < a.batch;
+  Point pt;
+  bool shade = in && load_point(a, i, pt);
+  const uint32_t found = (shade && f.near_count) ? f.near_count[i] : 0u;
+  if (KIND == 1) shade = shade && found > 0;
+  bool want = false;
+  D3 ex{0, 0, 0};
+  double wv[3] = {0.0, 0.0, 0.0}, mint = 0.0;
+  if (shade) {
+    Bsdf b;
+    point_bsdf(a, i, pt.prim, b);
+    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    const uint32_t gq = f.gs_sqrt;
+    const double inv_n = 1.0 / gs, inv_q = 1.0 / gq;
+    const uint32_t x = k % gq, y = k / gq;
+    if (KIND == 0) {
+      // StratifiedSampling1D / 2D with jitter (SamplingRoutines.h:327-373): dims [0, gs), [gs, 3 gs)
+      const double uc = (k + fg_rnd(a.seed, sidx, k)) * inv_n;
+      const double u0 = (x + fg_rnd(a.seed, sidx, gs + 2 * k)) * inv_q;
+      const double u1 = (y + fg_rnd(a.seed, sidx, gs + 2 * k + 1)) * inv_q;
+      double bp, fv[3];
+      D3 wo_l, wi_l;
+      b.sample(pt.wo, u0, u1, uc, ex, bp, fv, wo_l, wi_l);
+      if (bp > 0.0 && !(fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0)) {
+        const double ppdf = photon_pdf(f, i, found, ex);
+        const double w = power_heuristic((double)gs, bp, (double)gs, ppdf);
+        const double kk = w * fabs(dot3(ex, b.n)) / bp;
+        for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
+        want = true;
+      }
+    } else {
+      // dims [3 gs, 4 gs) and [4 gs, 6 gs)
+      const double us = (k + fg_rnd(a.seed, sidx, 3 * gs + k)) * inv_n;
+      const double u0 = (x + fg_rnd(a.seed, sidx, 4 * gs + 2 * k)) * inv_q;
+      const double u1 = (y + fg_rnd(a.seed, sidx, 4 * gs + 2 * k + 1)) * inv_q;
+      const uint32_t sj = (uint32_t)(us * found);
+      const D3 e1 = photon_dir(f, i, sj);
+      // MathRoutines::CoordinateSystem (MathRoutines.h:298-313)
+      D3 e2;
+      if (fabs(e1.x) > fabs(e1.y)) {
+        const double inv = 1.0 / sqrt(e1.x * e1.x + e1.z * e1.z);
+        e2 = D3{-e1.z * inv, 0.0, e1.x * inv};
+      } else {
+        const double inv = 1.0 / sqrt(e1.y * e1.y + e1.z * e1.z);
+        e2 = D3{0.0, e1.z * inv, -e1.y * inv};
+      }
+      const D3 e3 = cross3(e1, e2);
+      // SamplingRoutines::UniformConeSampling (SamplingRoutines.h:300-310)
+      const double ct = (1.0 - u0) + u0 * kCosGather;
+      const double st = sqrt(1.0 - ct * ct);
+      const double phi = u1 * 2.0 * kPi;
+      const double lx = st * cos(phi), ly = st * sin(phi);
+      ex = add3(add3(mul3(e2, lx), mul3(e3, ly)), mul3(e1, ct));
+      double fv[3];
+      b.evaluate(pt.wo, ex, fv);
+      if (!(fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0)) {
+        const double bp = b.pdf(pt.wo, ex);
+        const double ppdf = photon_pdf(f, i, found, ex);
+        const double w = power_heuristic((double)gs, ppdf, (double)gs, bp);
+        const double kk = w * fabs(dot3(ex, b.n)) / ppdf;
+        for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
+        want = true;
+      }
+    }
+    if (want) mint = next_min_t(a.geom.prims[pt.prim], pt.g, ex);
+  }
+  const int32_t slot = append_slot_block(f.count, want);
+  if (in) f.fg_idx[i * 2 * gs + (uint64_t)KIND * gs + k] = slot;
+  if (want) {
+    put_ray(f.rays + slot, pt.p, ex, mint, __builtin_huge_val());
+    f.w[3 * (uint64_t)slot] = wv[0];
+    f.w[3 * (uint64_t)slot + 1] = wv[1];
+    f.w[3 * (uint64_t)slot + 2] = wv[2];
+  }
+}
+
+// BSDF::TotalScattering(incident, 5x5 stratified samples, BSDF_ALL_REFLECTION) of a matte /
+// plastic surface (BSDF.cpp:209-224): Lambertian::TotalScattering = R (Lambertian.cpp:32-36) plus,
+// for plastic, BxDF::TotalScattering of the microfacet lobe (BxDF.cpp:43-68) through
+// Microfacet::Sample (Microfacet.h:112-128) and BlinnDistribution::Sample (BlinnDistribution.h:
+// 81-98). BSDF::TotalScattering hands the lobes the world-space incident vector, as here.
+__device__ __forceinline__ void total_scattering(const Bsdf &b, D3 incident, double ts[3]) {
+  double r[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 3; ++c) r[c] = r[c] + b.kd[c];
+  if (b.plastic) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int y = 0; y < 5; ++y)
+      for (int x = 0; x < 5; ++x) {
+        const double u0 = (x + 0.5) * (1.0 / 5), u1 = (y + 0.5) * (1.0 / 5);
+        const double ct = pow(u0, 1.0 / (b.e + 1.0));
+        const double st = sqrt(dmax(0.0, 1.0 - ct * ct));
+        const double phi = u1 * 2.0 * kPi;
+        D3 h{st * cos(phi), st * sin(phi), ct};
+        if (incident.z * h.z < 0.0) h.z *= -1.0;
+        const D3 ex = sub3(mul3(h, 2.0 * dot3(incident, h)), incident);
+        double pdf = ((b.e + 1.0) * pow(ct, b.e)) / (2.0 * kPi * 4.0 * fabs(dot3(incident, h)));
+        double fv[3] = {0.0, 0.0, 0.0};
+        if (incident.z * ex.z < 0.0) {
+          pdf = 0.0;
+        } else {
+          double sc, fr, unused;
+          if (b.micro_eval<false>(incident, ex, sc, fr, unused))
+            for (int c = 0; c < 3; ++c) fv[c] = (b.ks[c] * fr) * sc;
+        }
+        if (pdf > 0.0) {
+          const double w = fabs(ex.z) / pdf;
+          for (int c = 0; c < 3; ++c) m[c] += w * fv[c];
+        }
+      }
+    for (int c = 0; c < 3; ++c) {
+      m[c] = m[c] * (1.0 / 25.0);
+      m[c] = dmin(1.0, dmax(0.0, m[c]));
+      r[c] = r[c] + m[c];
+    }
+  }
+  for (int c = 0; c < 3; ++c) ts[c] = dmin(1.0, dmax(0.0, r[c]));
+}
+
+// One thread per gather ray: the hit's geometry, total reflection scattering toward the gather
+// direction, and the irradiance lookup's point and geometric normal (:545-570).
+__global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= f.max_rays) return;
+  const DlArgs &a = f.a;
+  const bool live = g < *f.count;
+  const double nan = __builtin_nan("");
+  D3 qp{nan, nan, nan}, qn{0.0, 0.0, 1.0};
+  double ts[3] = {0.0, 0.0, 0.0};
+  uint8_t side = 0;
+  // Slots past the pass's gather rays get a NaN lookup point too: the irradiance lookup runs over
+  // every slot and a NaN point descends one path and finds nothing.
+  skw_hit h;
+  h.prim = kMiss;
+  if (live) h = f.hits[g];
+  if (h.prim != kMiss) {
+    const skw_ray &r = f.rays[g];
+    const D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
+    const HitGeom g2 = hit_geometry(a.geom, o, d, h.prim, h.tri);
+    const D3 gd = mul3(d, -1.0);
+    Bsdf b2;
+    const DevMat &m = a.mats[h.prim];
+    b2.plastic = m.type == 1;
+    for (int c = 0; c < 3; ++c) {
+      b2.kd[c] = m.kd[c];
+      b2.ks[c] = m.ks[c];
+    }
+    b2.e = m.exponent;
+    total_scattering(b2, gd, ts);
+    side = dot3(gd, g2.ng) > 0.0 ? 1 : 0;
+    qp = g2.p;
+    qn = g2.ng;
+  }
+  if (live)
+    for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
+  f.q_pts[3 * g] = qp.x;
+  f.q_pts[3 * g + 1] = qp.y;
+  f.q_pts[3 * g + 2] = qp.z;
+  f.q_nrm[3 * g] = qn.x;
+  f.q_nrm[3 * g + 1] = qn.y;
+  f.q_nrm[3 * g + 2] = qn.z;
+  if (live) f.side[g] = side;
+}
+
+// One thread per slot: the sample's gather rays in the reference's order (BSDF samples, then
+// cone samples), each adding TotalScattering x irradiance / pi x weight (:561-573); / gather
+// samples (Spectrum::operator/ multiplies by the reciprocal). Film and per-sample output.
+__global__ __launch_bounds__(256) void k_fg_resolve(FgArgs f) {
+  const DlArgs &a = f.a;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    a.totals[0] += a.batch;
+    a.totals[1] += *f.count;
+  }
+  if (i >= a.batch) return;
+  const uint32_t gs = f.gs;
+  double rad[3] = {0.0, 0.0, 0.0};
+  for (uint32_t j = 0; j < 2 * gs; ++j) {
+    const int32_t r = f.fg_idx[i * 2 * gs + j];
+    if (r < 0) continue;
+    if (f.hits[r].prim == kMiss) continue;
+    const uint32_t id = f.irr_id[r];
+    if (id == kMiss) continue;
+    const bool ext = f.side[r] != 0;
+    const float *es = ext ? f.e_ext + 3 * (size_t)id : f.e_int + 3 * (size_t)id;
+    const float *eo = ext ? f.e_int + 3 * (size_t)id : f.e_ext + 3 * (size_t)id;
+    for (int c = 0; c < 3; ++c) {
+      double t = 0.0;
+      t = t + f.ts[3 * (size_t)r + c] * (double)es[c];
+      t = t + 0.0 * (double)eo[c];  // no transmission lobe: its total scattering is 0
+      t = t * kInvPi;
+      rad[c] = rad[c] + (t * 1.0) * f.w[3 * (size_t)r + c];
+    }
+  }
+  const double inv = gs > 0 ? 1.0 / (double)gs : 0.0;
+  const uint64_t pix = pixel_of(a, i);
+  for (int c = 0; c < 3; ++c) {
+    const double L = rad[c] * inv;
+    a.film[3 * pix + c] += L;
+    if (f.radiance) f.radiance[3 * (pix * a.spp + a.pass) + c] = L;
+  }
+}
+
 // Per-sample sum and film accumulation. ADJ: instead of the film, accumulates the sample's
 // contribution to d_grad: for L = Le + (1/nls) sum_k f_k * le_k * W_k + (1/nbs) sum_k f_k * le_k * W_k
 // with the weights W fixed, dL/d(emission l) = base radiance of l times the terms that use l,
@@ -967,6 +1223,13 @@ static void launch_dl_samples(const DlArgs &a, hipStream_t stream) {
   if (nb) hipLaunchKernelGGL((k_dl_bsdf<SPLIT, ADJ>), dim3((uint32_t)((nb + B - 1) / B)), dim3(B), 0, stream, a);
 }
 
+hipError_t launch_dl_point(const DlArgs &a, hipStream_t stream) {
+  uint32_t grid = (uint32_t)((a.batch + 63) / 64);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_point, dim3(grid), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 63) / 64);
   if (grid == 0) return hipSuccess;
@@ -1017,6 +1280,35 @@ hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream) {
   uint32_t grid = (uint32_t)((a.batch + 255) / 256);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dl_finish, dim3(grid), dim3(256), 0, stream, a, rgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream) {
+  const uint32_t grid = (uint32_t)((f.a.batch + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fg_lookup_in, dim3(grid), dim3(256), 0, stream, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream) {
+  const uint64_t n = f.a.batch * f.gs, B = kSampleBlock;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fg_samples<0>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
+  hipLaunchKernelGGL(k_fg_samples<1>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream) {
+  const uint32_t grid = (uint32_t)((f.max_rays + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fg_hit, dim3(grid), dim3(256), 0, stream, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fg_resolve(const FgArgs &f, hipStream_t stream) {
+  const uint32_t grid = (uint32_t)((f.a.batch + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fg_resolve, dim3(grid), dim3(256), 0, stream, f);
   return hipGetLastError();
 }
 

@@ -48,4 +48,21 @@ hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream) {
   return mode == 1 ? launch_mode<1>(args, stream) : launch_mode<0>(args, stream);
 }
 
+namespace {
+// out = (a + b) + c in float (a may be null: read as 0).
+__global__ __launch_bounds__(256) void k_irr_sum(const float *a, const float *b, const float *c, float *out,
+                                                 uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a ? a[i] : 0.0f;
+  out[i] = (x + b[i]) + c[i];
+}
+}  // namespace
+
+hipError_t launch_irr_sum(const float *a, const float *b, const float *c, float *out, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_irr_sum, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, a, b, c, out, n);
+  return hipGetLastError();
+}
+
 }  // namespace skw

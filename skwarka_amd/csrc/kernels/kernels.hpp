@@ -72,7 +72,8 @@ struct GatherArgs {
   double *heap_d2;
   uint64_t q_stride, j_stride;
   // outputs (any may be null)
-  uint32_t *ids;        // k per query, caller ids (row-major)
+  uint32_t *ids;        // k per query, caller ids (row-major); tree positions when ids_tree
+  int ids_tree;
   double *d2;           // k per query
   uint32_t *count;
   float *e_ext, *e_int, *radiance;
@@ -88,6 +89,8 @@ struct GatherArgs {
 };
 // mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
+// out[i] = (a[i] + b[i]) + c[i] in float over n values; a may be null (zeros).
+hipError_t launch_irr_sum(const float *a, const float *b, const float *c, float *out, uint64_t n, hipStream_t stream);
 
 // ------------------------------------------------------------------ direct lighting
 struct DevMat {
@@ -176,8 +179,36 @@ hipError_t launch_dl_tape_adjoint(const DlTapeArgs &t, hipStream_t stream);
 constexpr uint32_t kDlMaxGroups = 64;
 constexpr uint32_t kDlMaxParams = 2 * kDlMaxGroups + 16;
 
+// Photon final gather over the camera hits of a direct-lighting pass (dl.hip).
+struct FgArgs {
+  DlArgs a;                       // camera, scene, materials, shading-point records, film
+  uint32_t gs, gs_sqrt;           // gather samples (a square) and its side
+  const DevPhoton *ind_photons;   // indirect map payload in tree order (incident directions)
+  const double *cdir;             // 65536 x 3 direction table
+  const uint32_t *near_ids;       // 32 per slot: the nearest indirect photons' tree positions
+  const uint32_t *near_count;     // per slot (null: no indirect photons)
+  double *s_pts, *s_nrm;          // 3 each per slot: shading point / geometric normal for the lookup
+  int32_t *fg_idx;                // 2 gs per slot: gather ray index (BSDF samples, then cone samples) or -1
+  skw_ray *rays;                  // gather rays
+  double *w;                      // 3 per gather ray: its weight
+  uint64_t *count;                // gather rays of this batch (device)
+  uint64_t max_rays;              // slots * 2 gs
+  skw_hit *hits;                  // gather hits
+  double *q_pts, *q_nrm;          // 3 each per gather ray: irradiance lookup point / normal (NaN when missed)
+  double *ts;                     // 3 per gather ray: total reflection scattering at the hit
+  uint8_t *side;                  // per gather ray: 1 when the gather direction is on the normal's side
+  const uint32_t *irr_id;         // per gather ray: nearest irradiance photon or 0xFFFFFFFF
+  const float *e_ext, *e_int;     // irradiance photons' irradiance (creation order)
+  double *radiance;               // 3 per sample (pixel * spp + s), optional
+};
+hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream);
+hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream);
+hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream);
+hipError_t launch_fg_resolve(const FgArgs &f, hipStream_t stream);
+
 hipError_t launch_dl_clear(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream);
+hipError_t launch_dl_point(const DlArgs &a, hipStream_t stream);  // shading-point records only
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream);

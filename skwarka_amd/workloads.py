@@ -302,6 +302,42 @@ def photons_on_mesh(mesh, n, seed, power_max=1e-6):
     }
 
 
+def mesh_area(mesh):
+    """TriangleMesh::GetArea (Source/RayLibs/Raytracer/Core/TriangleMesh.cpp:195-205): float
+    triangle areas (Triangle3D<float>::GetArea, Math/Triangle3D.h:82-86) summed in a float, in
+    triangle order."""
+    v = mesh["verts"].astype(np.float32)
+    t = mesh["tris"].astype(np.int64)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    e1, e2 = b - a, c - a
+    cx = e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1]
+    cy = e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2]
+    cz = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    area = np.float32(0.5) * np.sqrt(cx * cx + cy * cy + cz * cz)
+    if len(area) == 0:
+        return np.float32(0.0)
+    return np.cumsum(area, dtype=np.float32)[-1]  # sequential float accumulation
+
+
+def scene_area(meshes, prims=None):
+    """PhotonLTEIntegrator's scene area (PhotonLTEIntegrator.cpp:51-54): the double sum over
+    primitives of their mesh's float area (the untransformed mesh)."""
+    areas = [mesh_area(m) for m in meshes]
+    total = 0.0
+    for p in (range(len(meshes)) if prims is None else [pr[0] for pr in prims]):
+        total += float(areas[p])
+    return total
+
+
+def photon_maps(mesh, n_direct, n_indirect, n_caustic, seed, power_max=1e-6):
+    """Three seeded photon sets on a mesh (photons_on_mesh with seeds seed, seed+1, seed+2): direct,
+    indirect and caustic (None when n_caustic is 0)."""
+    d = photons_on_mesh(mesh, n_direct, seed, power_max)
+    i = photons_on_mesh(mesh, n_indirect, seed + 1, power_max)
+    c = photons_on_mesh(mesh, n_caustic, seed + 2, power_max) if n_caustic > 0 else None
+    return d, i, c
+
+
 # ---------------------------------------------------------------- named workloads
 
 
