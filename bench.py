@@ -13,6 +13,9 @@ timed region starts:
   c1 (component): photon gather, 1,000,000 photons on the c0 scene, 262,143 gather points at the
       c0 primary hits, k=100, r=0.05, PhotonFilter, disc estimate + Lambertian rho=0.5 radiance
       -> M gathers/s.
+  c5 (component): the photon integrator's final gather on the c2 scene at 1024x1024, 4 spp, 8
+      gather samples (rounded to 9), over an irradiance photon map built from 100k direct, 400k
+      indirect and 20k caustic photons -> Mrays/s over camera + gather rays.
 Timing: W untimed warm-up steps, then exactly K steps between HIP events recorded on the stream
 the kernels run on, with device synchronisation on both sides.
 The CPU baseline is the oracle (CPU restatement of the reference path, OpenMP) timed on this host
@@ -265,6 +268,68 @@ def bench_c4(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
             "cam15": cam15, "theta0": th0, "target_theta": tt, "target": target, "img": img}
 
 
+def fg_inputs(sk, wl, s, scene_stream):
+    """Photon maps and irradiance map of the final-gather workload: 100k direct, 400k indirect and
+    20k caustic photons spread uniformly over the c2 room's surfaces (seed 31)."""
+    mesh = wl.merge(s["meshes"])
+    sets = wl.photon_maps(mesh, 100_000, 400_000, 20_000, seed=31)
+    maps = [sk.PhotonMap(p["pos"], p["weight"], p["dir"], p["normal"]) for p in sets]
+    area = wl.scene_area(s["meshes"])
+    paths = (200_000, 150_000, 250_000)
+    t0 = time.time()
+    irr = sk.IrradianceMap(direct=maps[0], indirect=maps[1], caustic=maps[2], caustic_paths=paths[0],
+                           direct_paths=paths[1], indirect_paths=paths[2], scene_area=area, stream=scene_stream)
+    irr_s = time.time() - t0
+    return sets, maps, area, paths, irr, irr_s
+
+
+def bench_c5(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
+    s = spec if spec is not None else wl.scene_room()
+    if scene is None:
+        scene = sk.Scene(s["meshes"])
+    dls = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    sptr = stream.cuda_stream
+    sets, maps, area, paths, irr, irr_s = fg_inputs(sk, wl, s, sptr)
+    w = h = 1024
+    spp, gs = 4, 8
+    cam, cam15 = sk.camera(*s["camera"], w, h)
+    d_rgb = torch.empty(w * h * 12, dtype=torch.uint8, device="cuda")
+
+    def step():
+        dls.final_gather(irr, maps[1], area, cam, w, h, spp, gs, d_rgb.data_ptr(), None, 1234, sptr,
+                         count_rays=False)
+
+    ms = time_steps(torch, step, steps, warmup, stream)
+    rays = dls.final_gather(irr, maps[1], area, cam, w, h, spp, gs, d_rgb.data_ptr(), None, 1234, sptr)
+    total = int(rays.sum())
+    rgb = d_rgb.cpu().numpy().view(np.float32)
+    return {"ms": ms, "value": total / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays],
+            "samples_per_s": w * h * spp / (ms * 1e-3), "irr_n": irr.n, "irr_max_dist": irr.max_lookup_dist,
+            "irr_build_s": irr_s, "mean_rgb": float(rgb.mean()), "spec": s, "cam15": cam15, "sets": sets,
+            "area": area, "paths": paths}
+
+
+def cpu_baseline_c5(c5, threads, rows):
+    import oracle
+    s = c5["spec"]
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    om = [oracle.PhotonMap(p["pos"], p["weight"], p["dir"], p["normal"]) for p in c5["sets"]]
+    oirr = oracle.IrradianceMap(direct=om[0], indirect=om[1], caustic=om[2], paths=c5["paths"], scene_area=c5["area"],
+                                nthreads=threads)
+    w = 1024
+    rows = max(1, min(rows, 1024))
+    r0 = 512 - rows // 2
+    t = time.perf_counter()
+    _, grays = oracle.fg_render(acc, dls, oirr, om[1], c5["area"], c5["cam15"], w, 1024, 4, 8, 1234, r0 * w,
+                                (r0 + rows) * w, nthreads=threads)
+    dt = time.perf_counter() - t
+    n = rows * w * 4 + grays
+    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "image rows %d-%d of the final-gather frame (%d pixels x 4 spp, %d camera + gather rays), "
+                      "one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
+
+
 def cpu_baseline_c3(c3, threads, n_sample):
     import oracle
     import torch
@@ -364,7 +429,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4"], default=None,
+    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
     ap.add_argument("--cpu-rows", type=int, default=1024,
@@ -420,7 +485,12 @@ def main():
         log("c4: direct-lighting gradient, 512x512 x 8 spp, 132 parameters")
         c4 = bench_c4(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
         log("c4: %.1f ms/gradient, %.2f gradients/s" % (c4["ms"], c4["value"]))
-    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = None
+    c5 = None
+    if run("c5"):
+        log("c5: photon final gather, 1024x1024 x 4 spp, 9 gather samples")
+        c5 = bench_c5(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
+        log("c5: %.1f ms/frame, %.1f Mrays/s, rays %s" % (c5["ms"], c5["value"], c5["rays"]))
+    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
         if c2 is not None:
@@ -433,6 +503,8 @@ def main():
             cpu3 = cpu_baseline_c3(c3, threads, 16)
         if c4 is not None:
             cpu4 = cpu_baseline_c4(c4, threads, 64)
+        if c5 is not None:
+            cpu5 = cpu_baseline_c5(c5, threads, 256)
     comps = {}
     if c3 is not None:
         comps["c3_batch"] = {
@@ -456,6 +528,17 @@ def main():
             "workload": "c2 scene at 512x512, 8 spp, 4 light + 4 BSDF samples; target = perturbed parameters "
                         "(seed 4)",
             "cpu_baseline": cpu4,
+        }
+    if c5 is not None:
+        comps["c5_final_gather"] = {
+            "metric": "Mrays/s (photon final gather: camera + gather rays)", "value": round(c5["value"], 3),
+            "ms_per_step": round(c5["ms"], 3), "rays_per_step": c5["rays"],
+            "samples_per_s": round(c5["samples_per_s"], 1),
+            "irradiance_map": {"photons": c5["irr_n"], "max_lookup_dist": c5["irr_max_dist"],
+                               "build_s": round(c5["irr_build_s"], 3)},
+            "workload": "c2 scene at 1024x1024, 4 spp, 8 gather samples (9, a square); 100k direct / 400k "
+                        "indirect / 20k caustic photons uniform over the surfaces (seed 31)",
+            "cpu_baseline": cpu5,
         }
     if c0 is not None:
         comps["c0_closest_hit"] = {
@@ -505,7 +588,7 @@ def main():
         }
     else:
         c = (comps.get("c0_closest_hit") or comps.get("c1_photon_gather") or comps.get("c3_batch")
-             or comps.get("c4_adjoint"))
+             or comps.get("c4_adjoint") or comps.get("c5_final_gather"))
         line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
                 "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

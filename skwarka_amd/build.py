@@ -26,6 +26,7 @@ SOURCES = [
     "kernels/gather.hip",
     "kernels/dl.hip",
     "kernels/batch.hip",
+    "kernels/sort.hip",
 ]
 
 COMMON = [

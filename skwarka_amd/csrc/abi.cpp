@@ -408,6 +408,7 @@ struct Estimate {
   const double *wo = nullptr;
   double *caustic = nullptr;
   bool ids_tree = false;  // ids hold tree positions instead of caller indices
+  const uint32_t *order = nullptr;  // visiting order of the queries (device), optional
 };
 
 static skw_status gather(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
@@ -435,6 +436,7 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.wo = est.wo;
   a.caustic = est.caustic;
   a.ids_tree = est.ids_tree ? 1 : 0;
+  a.order = est.order;
   if (mode == 0 && k > 0) {
     size_t elems = (size_t)nq * k;
     SKW_HIP(pool_alloc((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
@@ -469,6 +471,14 @@ skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, co
   est.ids_tree = true;
   return gather(map, d_points, d_normals, nq, k, r, 1, 0.0, 1, 0, nullptr, nullptr, nullptr, d_ids, nullptr, d_count,
                 nullptr, stream, est);
+}
+
+skw_status photon_nearest_ordered(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                  uint64_t nq, double r, const uint32_t *d_order, uint32_t *d_id, hipStream_t stream) {
+  Estimate est;
+  est.order = d_order;
+  return gather(map, d_points, d_normals, nq, 0, r, 1, 0.0, 1, 1, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, d_id, stream, est);
 }
 }  // namespace skw
 

@@ -50,6 +50,7 @@ constexpr double kInvPi = 0.31830988618379067154;
 constexpr uint32_t kFgK = 32;          // indirect photons per final-gather lookup (PhotonLTEIntegrator.cpp:459)
 constexpr uint32_t kMaxGather = 676;   // 26^2: the sample stream has 4096 dims per sample, 6 per gather sample
 constexpr uint64_t kMaxGatherRays = uint64_t(1) << 23;  // gather rays held per pixel batch
+constexpr int kKeyLo = 12, kKeyHi = 30;  // key bits of the lookup points' 64^3 Morton cells
 
 double inf() { return std::numeric_limits<double>::infinity(); }
 
@@ -323,9 +324,18 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const size_t o_qp = b.take(max_rays * 3 * sizeof(double)), o_qn = b.take(max_rays * 3 * sizeof(double));
   const size_t o_ts = b.take(max_rays * 3 * sizeof(double)), o_side = b.take(max_rays);
   const size_t o_irr = b.take(max_rays * sizeof(uint32_t));
+  // irradiance lookups in spatial order: cell keys of the lookup points, radix-sorted with the
+  // gather-ray indices
+  const size_t o_keys = b.take(max_rays * sizeof(uint32_t)), o_keys2 = b.take(max_rays * sizeof(uint32_t));
+  const size_t o_ord = b.take(max_rays * sizeof(uint32_t)), o_ord2 = b.take(max_rays * sizeof(uint32_t));
+  size_t sort_bytes = 0;
+  SKW_HIP(sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, max_rays, kKeyLo, kKeyHi, st),
+          "sort workspace size");
+  const size_t o_sort = b.take(sort_bytes);
   const size_t o_counts = b.take(2 * sizeof(uint64_t)), o_totals = b.take(9 * sizeof(uint64_t));
   const size_t o_film = b.take(npix * 3 * sizeof(double));
   SKW_HIP(pool_alloc((void **)&b.base, b.off, st), "final gather buffers");
+  skw_status rs0 = SKW_OK;
   a.prim_rays = b.at<skw_ray>(o_prays);
   a.prim_hits = b.at<skw_hit>(o_phits);
   a.le = b.at<double>(o_le);
@@ -350,6 +360,19 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   f.side = b.at<uint8_t>(o_side);
   f.irr_id = b.at<uint32_t>(o_irr);
   uint32_t *irr_id = b.at<uint32_t>(o_irr);
+  f.keys = b.at<uint32_t>(o_keys);
+  f.order = b.at<uint32_t>(o_ord);
+  uint32_t *keys2 = b.at<uint32_t>(o_keys2), *order2 = b.at<uint32_t>(o_ord2);
+  double box[6];
+  if ((rs0 = skw_scene_bounds(scene, box)) != SKW_OK) {
+    pool_free(b.base, st);
+    return rs0;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double ext = box[3 + c] - box[c];
+    f.lo[c] = box[c];
+    f.cell[c] = ext > 0.0 && std::isfinite(ext) ? 64.0 / ext : 0.0;
+  }
   uint32_t *near_ids = b.at<uint32_t>(o_nids), *near_count = b.at<uint32_t>(o_ncnt);
 
   hipError_t e = launch_dl_clear(a, st);
@@ -379,7 +402,11 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
       e = launch_fg_hit(f, st);
       if (e != hipSuccess) break;
       if (irr->map) {
-        rs = skw_photon_nearest(irr->map, f.q_pts, f.q_nrm, a.batch * 2ull * gs, irr->max_dist, 1, irr_id, st);
+        const uint64_t nq = a.batch * 2ull * gs;
+        size_t bytes = sort_bytes;
+        e = sort_pairs_u32(b.base + o_sort, bytes, f.keys, keys2, f.order, order2, nq, kKeyLo, kKeyHi, st);
+        if (e != hipSuccess) break;
+        rs = photon_nearest_ordered(irr->map, f.q_pts, f.q_nrm, nq, irr->max_dist, order2, irr_id, st);
         if (rs != SKW_OK) break;
       } else {
         e = hipMemsetAsync(irr_id, 0xFF, max_rays * sizeof(uint32_t), st);

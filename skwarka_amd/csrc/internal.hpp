@@ -102,4 +102,7 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
 // k nearest photons with PhotonFilter within r, ids as tree positions (row-major, k per query).
 skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                            uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream);
+// Nearest photon with PhotonFilter within r of each query; thread i serves query d_order[i].
+skw_status photon_nearest_ordered(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                  uint64_t nq, double r, const uint32_t *d_order, uint32_t *d_id, hipStream_t stream);
 }  // namespace skw

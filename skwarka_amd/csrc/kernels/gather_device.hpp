@@ -437,7 +437,8 @@ __device__ __forceinline__ bool disc_sums(const GatherArgs &a) { return a.estima
 
 // One query from start to end (one lane, no refill).
 template <int MODE, class Stack>
-__device__ void gather_one(const GatherArgs &a, uint64_t i, const Stack &stk) {
+__device__ void gather_one(const GatherArgs &a, uint64_t t, const Stack &stk) {
+  const uint64_t i = a.order ? (uint64_t)a.order[t] : t;
   const Query q = load_query(a, i);
   double max_d2 = a.r * a.r;
   uint32_t size = 0;

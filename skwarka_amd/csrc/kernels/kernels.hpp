@@ -86,6 +86,9 @@ struct GatherArgs {
   double kc;
   const double *wo;
   double *caustic;
+  // optional visiting order: thread i serves query order[i] (its inputs and outputs); spatially
+  // sorted orders keep a wave's lookups on nearby tree paths
+  const uint32_t *order;
 };
 // mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
@@ -200,11 +203,18 @@ struct FgArgs {
   const uint32_t *irr_id;         // per gather ray: nearest irradiance photon or 0xFFFFFFFF
   const float *e_ext, *e_int;     // irradiance photons' irradiance (creation order)
   double *radiance;               // 3 per sample (pixel * spp + s), optional
+  uint32_t *keys, *order;         // per gather ray: cell key of its lookup point (misses last) and its index
+  double lo[3], cell[3];          // scene box corner and cells per unit length (64 per axis)
 };
 hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_resolve(const FgArgs &f, hipStream_t stream);
+// Sorts n (key, value) pairs by key bits [begin_bit, end_bit) (sort.hip). With temp == nullptr
+// only sets temp_bytes.
+hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                          const uint32_t *vals_in, uint32_t *vals_out, uint64_t n, int begin_bit, int end_bit,
+                          hipStream_t stream);
 
 hipError_t launch_dl_clear(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_camera(const DlArgs &a, hipStream_t stream);
