@@ -337,6 +337,39 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
                                const skw_fg_params *params, float *d_rgb, double *d_radiance, uint64_t *rays_out,
                                void *stream);
 
+/* ---------------------------------------------------------------- photon shooting
+ * PhotonLTEIntegrator::ShootPhotons (PhotonLTEIntegrator.cpp:247-288) with the shooting pipeline
+ * (PhotonShootingPipeline.cpp:48-267) from the area lights of a direct-lighting scene: light
+ * chosen by the lights' power CDF, DiffuseAreaLightSource::SamplePhoton, photons deposited at
+ * every intersection (map 1 direct at the first, map 0 caustic while the path is specular, map 2
+ * indirect otherwise), BSDF sampling and Russian roulette. Samples of path i: radical inverses of
+ * i + 1 as the reference (bases 2, 3, 5, 7, 11, 13 and 17, 19, 23 at the first intersection); the
+ * reference draws later bounces and the roulette from a per-chunk generator, here a counter-based
+ * stream of (seed, path). Paths run in rounds of paths_per_round in path order; a map takes a
+ * round's photons and counts its paths unless it already held max_photons[map] when the round
+ * began (the pipeline's input filter decides per 4096-path chunk); shooting stops at max_paths,
+ * when every map is full, or after 10^7 paths when the maps that are not full are all empty.
+ * Photons of a map are in (path, intersection) order. */
+typedef struct skw_photon_set skw_photon_set;
+typedef struct {
+  uint64_t max_paths;
+  uint64_t max_photons[3]; /* caustic, direct, indirect */
+  uint32_t paths_per_round;
+  uint64_t seed;
+} skw_shoot_params;
+skw_status skw_photon_shoot(const skw_scene *scene, const skw_dl_scene *dl, const skw_shoot_params *params,
+                            skw_photon_set **out, void *stream);
+/* Photons (counts[3]), paths of each map (paths[3]) and paths shot (host outputs, any may be NULL). */
+skw_status skw_photon_set_info(const skw_photon_set *set, uint64_t counts[3], uint64_t paths[3],
+                               uint64_t *total_paths);
+/* Host copies of map `which`: positions (3 floats), weights (3 floats), incident-direction and
+ * normal ids (any may be NULL). */
+skw_status skw_photon_set_export(const skw_photon_set *set, uint32_t which, float *pos, float *weight, uint16_t *dir,
+                                 uint16_t *normal);
+/* The photon map (KDTree<Photon>) of map `which`, as skw_photon_map_create over the exported arrays. */
+skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out);
+skw_status skw_photon_set_destroy(skw_photon_set *set);
+
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in
  * skw_dl_render, sample s of pixel p has index p*spp + s) are traced to their closest hit by one

@@ -86,6 +86,11 @@ def lib():
                                   ctypes.c_int], None),
         "orc_dl_gradient": ([_P, _P, _P, _P, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P, _P,
                              ctypes.c_int], None),
+        "orc_photon_shoot": ([_P, _P, _U64, _P, _U32, _U64, ctypes.c_int], _P),
+        "orc_photon_set_info": ([_P, _P, _P, _P], None),
+        "orc_photon_set_export": ([_P, _U32, _P, _P, _P, _P], None),
+        "orc_photon_set_destroy": ([_P], None),
+        "orc_radical_inverse": ([_U32, _U32], ctypes.c_double),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -362,6 +367,31 @@ def fg_render(accel, dls, irr, indirect, scene_area, cam, width, height, spp, ga
                         cam.ctypes.data, width, height, spp, gather_samples, seed, p0, p1, out.ctypes.data,
                         ctypes.byref(rays), nthreads or threads())
     return out, rays.value
+
+
+def photon_shoot(accel, dls, max_paths, max_photons, paths_per_round=4096, seed=0, nthreads=None):
+    """Photon shooting from the DLScene's area lights: returns {"maps": [caustic, direct, indirect]
+    (each a dict of pos / weight / dir / normal arrays in path order), "paths": [3], "total_paths"}."""
+    mp = np.ascontiguousarray(max_photons, np.uint64)
+    h = lib().orc_photon_shoot(accel.h, dls.h, max_paths, mp.ctypes.data, paths_per_round, seed, nthreads or threads())
+    try:
+        counts, paths, total = np.zeros(3, np.uint64), np.zeros(3, np.uint64), ctypes.c_uint64()
+        lib().orc_photon_set_info(h, counts.ctypes.data, paths.ctypes.data, ctypes.byref(total))
+        maps = []
+        for j in range(3):
+            n = int(counts[j])
+            m = {"pos": np.zeros((n, 3), np.float32), "weight": np.zeros((n, 3), np.float32),
+                 "dir": np.zeros(n, np.uint16), "normal": np.zeros(n, np.uint16)}
+            lib().orc_photon_set_export(h, j, m["pos"].ctypes.data, m["weight"].ctypes.data, m["dir"].ctypes.data,
+                                        m["normal"].ctypes.data)
+            maps.append(m)
+        return {"maps": maps, "paths": [int(x) for x in paths], "total_paths": total.value}
+    finally:
+        lib().orc_photon_set_destroy(h)
+
+
+def radical_inverse(n, base):
+    return lib().orc_radical_inverse(n, base)
 
 
 class DLScene:

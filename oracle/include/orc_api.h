@@ -144,6 +144,17 @@ void orc_irr_map_nearest(const orc_irr_map *m, uint64_t nq, const double *p, con
 
 /* CompressedDirection codec (Source/RayLibs/Math/CompressedDirection.{h,cpp}). */
 uint16_t orc_cdir_encode(double x, double y, double z);
+
+/* Photon shooting (PhotonLTEIntegrator::ShootPhotons with the shooting pipeline): paths in rounds
+ * of paths_per_round, maps 0 caustic / 1 direct / 2 indirect, max_photons[3]. */
+typedef struct orc_photon_set orc_photon_set;
+orc_photon_set *orc_photon_shoot(const orc_accel *accel, const orc_dl_scene *dls, uint64_t max_paths,
+                                 const uint64_t *max_photons, uint32_t paths_per_round, uint64_t seed, int nthreads);
+void orc_photon_set_info(const orc_photon_set *s, uint64_t *counts, uint64_t *paths, uint64_t *total_paths);
+void orc_photon_set_export(const orc_photon_set *s, uint32_t which, float *pos, float *weight, uint16_t *dir,
+                           uint16_t *normal);
+void orc_photon_set_destroy(orc_photon_set *s);
+double orc_radical_inverse(uint32_t n, uint32_t base);
 void orc_cdir_table(double *out_65536x3);
 
 /* Raw 32-bit outputs of std::mt19937 (boost::mt19937 is the same engine). */

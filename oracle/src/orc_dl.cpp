@@ -924,6 +924,152 @@ static uint64_t fg_sample(const FgSpec &F, uint64_t pix, uint32_t si, double L[3
   return gdir.size();
 }
 
+// ------------------------------------------------------------------ photon shooting
+
+// SamplingRoutines::RadicalInverse (SamplingRoutines.h:492-508).
+static double radical_inverse(uint32_t n, uint32_t base) {
+  double ret = 0.0;
+  double inv_base = 1.0 / base, inv = inv_base;
+  while (n > 0) {
+    int d = (int)(n % base);
+    ret += d * inv;
+    n /= base;
+    inv *= inv_base;
+  }
+  return ret;
+}
+
+// Counter-based stream of a photon path's later-bounce BSDF samples and Russian-roulette draws (the
+// reference draws them from a per-chunk RandomGenerator): dims 4k..4k+2 = BSDF sample of
+// intersection k >= 2, 4k+3 = roulette at intersection k; at most 1023 intersections per path.
+static inline double shoot_rnd(uint64_t seed, uint64_t path, uint32_t dim) {
+  uint64_t h = mix64((seed ^ 0xBB67AE8584CAA73Bull) ^ mix64(path * 4096ull + dim));
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+static const uint32_t SHOOT_MAX_ISECT = 1023;
+
+struct ShotPhoton {
+  float p[3], w[3];
+  uint16_t dir, normal;
+};
+
+// PhotonLTEIntegrator::_GetLightsPowerCDF (PhotonLTEIntegrator.cpp:80-124) over area lights:
+// luminance of Power() = (pi * area) * radiance, accumulated, divided by the total.
+static void lights_power_cdf(const DLScene &s, std::vector<double> &cdf) {
+  const size_t nl = s.lights.size();
+  cdf.assign(nl, 0.0);
+  if (nl == 0) return;
+  for (size_t l = 0; l < nl; ++l) {
+    double power[3];
+    for (int c = 0; c < 3; ++c) power[c] = (PI_ * s.lights[l].area) * s.lights[l].radiance[c];
+    cdf[l] = luminance(power);
+  }
+  for (size_t i = 1; i < nl; ++i) cdf[i] += cdf[i - 1];
+  const double total = cdf[nl - 1];
+  if (total > DBL_EPS_)
+    for (size_t i = 0; i < nl; ++i) cdf[i] /= total;
+  else
+    cdf[nl - 1] = 1.0;
+}
+
+struct ShootSpec {
+  const orc_accel *accel;
+  const DLScene *s;
+  const ParamView *v;
+  std::vector<double> light_cdf;
+  uint64_t seed;
+};
+
+// One photon path of PhotonsShootingFilter::operator() (PhotonShootingPipeline.cpp:143-243) with
+// DiffuseAreaLightSource::SamplePhoton (DiffuseAreaLightSource.cpp:118-150). Photons go to out[0]
+// caustic, out[1] direct, out[2] indirect unless that map is done.
+static void shoot_path(const ShootSpec &S, uint64_t path, const bool done[3], std::vector<ShotPhoton> out[3]) {
+  const DLScene &s = *S.s;
+  const uint32_t n1 = (uint32_t)path + 1u;
+  const double ps0 = radical_inverse(n1, 2), ps1 = radical_inverse(n1, 3);
+  const double ds0 = radical_inverse(n1, 5), ds1 = radical_inverse(n1, 7);
+  double light_pdf = 0.0;
+  const size_t li = search_cdf(S.light_cdf.data(), S.light_cdf.size(), radical_inverse(n1, 11), &light_pdf);
+  const Light &L = s.lights[li];
+  if (L.cdf.empty()) return;  // pdf 0
+  const size_t ti = search_cdf(L.cdf.data(), L.cdf.size(), radical_inverse(n1, 13), nullptr);
+  const V3 ln = L.normal[ti];
+  const double tmp = std::sqrt(ps0), b1 = 1.0 - tmp, b2 = ps1 * tmp;
+  const V3 sp = add(add(mul(L.v0[ti], b1), mul(L.v1[ti], b2)), mul(L.v2[ti], 1.0 - b1 - b2));
+  V3 e2, e3;
+  coordinate_system(ln, e2, e3);
+  double dx, dy;
+  concentric_disk(ds0, ds1, dx, dy);
+  const double dz = std::sqrt(smax(0.0, 1.0 - dx * dx - dy * dy));
+  V3 dir = add(add(mul(e2, dx), mul(e3, dy)), mul(ln, dz));
+  const double photon_pdf = (dz * INV_PI_) * (1.0 / L.area);
+  const double cosl = dot(dir, ln);
+  double w[3];
+  for (int c = 0; c < 3; ++c) w[c] = L.radiance[c] * cosl;
+  if (photon_pdf == 0.0 || (w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0)) return;
+  const double inv = 1.0 / (photon_pdf * light_pdf);
+  for (int c = 0; c < 3; ++c) w[c] = w[c] * inv;
+  bool specular_path = true;
+  uint32_t n_isect = 0;
+  double ray[8] = {sp.x, sp.y, sp.z, dir.x, dir.y, dir.z, 1e-4, DBL_INF_};
+  while (true) {
+    orc_hit h;
+    orc_hit_geom g;
+    orc_accel_intersect_one(S.accel, ray, &h, &g);
+    if (h.prim == 0xffffffffu) break;
+    ++n_isect;
+    const V3 d{ray[3], ray[4], ray[5]};
+    const V3 incident = mul(d, -1.0);
+    const V3 p{g.p[0], g.p[1], g.p[2]}, ng{g.ng[0], g.ng[1], g.ng[2]}, ns{g.ns[0], g.ns[1], g.ns[2]};
+    Bsdf b;
+    make_bsdf(s, *S.v, h.prim, ng, ns, b);
+    // Deposit (matte and plastic surfaces always have a non-specular component).
+    ShotPhoton ph;
+    for (int c = 0; c < 3; ++c) {
+      ph.p[c] = (float)g.p[c];
+      ph.w[c] = (float)w[c];
+    }
+    ph.dir = orc_cdir_encode(incident.x, incident.y, incident.z);
+    ph.normal = orc_cdir_encode(ng.x, ng.y, ng.z);
+    if (n_isect == 1) {
+      if (!done[1]) out[1].push_back(ph);
+    } else if (specular_path) {
+      if (!done[0]) out[0].push_back(ph);
+    } else {
+      if (!done[2]) out[2].push_back(ph);
+    }
+    double u0, u1, uc;
+    if (n_isect == 1) {
+      u0 = radical_inverse(n1, 17);
+      u1 = radical_inverse(n1, 19);
+      uc = radical_inverse(n1, 23);
+    } else {
+      u0 = shoot_rnd(S.seed, path, 4 * n_isect);
+      u1 = shoot_rnd(S.seed, path, 4 * n_isect + 1);
+      uc = shoot_rnd(S.seed, path, 4 * n_isect + 2);
+    }
+    V3 ex, wo_l, wi_l;
+    double bp, f[3];
+    b.sample(incident, u0, u1, uc, ex, bp, f, wo_l, wi_l);
+    if (bp == 0.0) break;
+    double wn[3];
+    const double inv_bp = 1.0 / bp;
+    for (int c = 0; c < 3; ++c) wn[c] = (w[c] * f[c]) * inv_bp;
+    const double cs = std::fabs(dot(ex, b.n));  // no specular lobes: always the cosine
+    for (int c = 0; c < 3; ++c) wn[c] = wn[c] * cs;
+    const double cp = smin(1.0, luminance(wn) / luminance(w));
+    if (shoot_rnd(S.seed, path, 4 * n_isect + 3) > cp) break;
+    const double inv_cp = 1.0 / cp;
+    for (int c = 0; c < 3; ++c) w[c] = wn[c] * inv_cp;
+    specular_path = false;  // the sampled component is never specular here
+    if (done[2]) break;
+    if (n_isect >= SHOOT_MAX_ISECT) break;
+    const double mint = orc_next_min_t(S.accel, h.prim, &g, &ex.x);
+    const double r2[8] = {p.x, p.y, p.z, ex.x, ex.y, ex.z, mint, DBL_INF_};
+    std::memcpy(ray, r2, sizeof(ray));
+  }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1161,5 +1307,78 @@ void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_ir
     }
   if (gather_rays) *gather_rays = rays;
 }
+
+// PhotonLTEIntegrator::ShootPhotons (PhotonLTEIntegrator.cpp:247-288) with the pipeline's input
+// and merging filters (PhotonShootingPipeline.cpp:48-92, 254-267) run one round of
+// paths_per_round paths at a time, in path order: a map takes a round's photons (and counts its
+// paths) unless it already held its maximum when the round began.
+struct orc_photon_set {
+  std::vector<ShotPhoton> m[3];
+  uint64_t paths[3] = {0, 0, 0};
+  uint64_t total_paths = 0;
+};
+
+orc_photon_set *orc_photon_shoot(const orc_accel *accel, const orc_dl_scene *dls, uint64_t max_paths,
+                                 const uint64_t *max_photons, uint32_t paths_per_round, uint64_t seed, int nthreads) {
+  orc_photon_set *out = new orc_photon_set();
+  if (dls->s.lights.empty() || max_paths == 0 || paths_per_round == 0) return out;
+  ShootSpec S;
+  S.accel = accel;
+  S.s = &dls->s;
+  S.v = &dls->nominal;
+  S.seed = seed;
+  lights_power_cdf(dls->s, S.light_cdf);
+  const int nt = nthreads > 0 ? nthreads : 1;
+  while (true) {
+    bool done[3];
+    for (int j = 0; j < 3; ++j) done[j] = out->m[j].size() >= max_photons[j];
+    if (out->total_paths >= max_paths || (done[0] && done[1] && done[2])) break;
+    bool stuck = out->total_paths > 10000000;
+    for (int j = 0; j < 3; ++j) stuck = stuck && (done[j] || out->m[j].empty());
+    if (stuck) break;
+    const uint64_t first = out->total_paths;
+    const uint64_t n = std::min<uint64_t>(paths_per_round, max_paths - first);
+    out->total_paths += n;
+    // paths in blocks of 256, merged in block order
+    const uint64_t nb = (n + 255) / 256;
+    std::vector<std::array<std::vector<ShotPhoton>, 3>> part(nb);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+    for (int64_t bi = 0; bi < (int64_t)nb; ++bi) {
+      const uint64_t p0 = first + (uint64_t)bi * 256, p1 = std::min(first + n, p0 + 256);
+      for (uint64_t p = p0; p < p1; ++p) shoot_path(S, p, done, part[bi].data());
+    }
+    for (int j = 0; j < 3; ++j) {
+      if (done[j]) continue;
+      out->paths[j] += n;
+      for (auto &pb : part) out->m[j].insert(out->m[j].end(), pb[j].begin(), pb[j].end());
+    }
+  }
+  return out;
+}
+
+void orc_photon_set_info(const orc_photon_set *s, uint64_t *counts, uint64_t *paths, uint64_t *total_paths) {
+  for (int j = 0; j < 3; ++j) {
+    if (counts) counts[j] = s->m[j].size();
+    if (paths) paths[j] = s->paths[j];
+  }
+  if (total_paths) *total_paths = s->total_paths;
+}
+
+void orc_photon_set_export(const orc_photon_set *s, uint32_t which, float *pos, float *weight, uint16_t *dir,
+                           uint16_t *normal) {
+  const std::vector<ShotPhoton> &v = s->m[which];
+  for (size_t i = 0; i < v.size(); ++i) {
+    for (int c = 0; c < 3; ++c) {
+      if (pos) pos[3 * i + c] = v[i].p[c];
+      if (weight) weight[3 * i + c] = v[i].w[c];
+    }
+    if (dir) dir[i] = v[i].dir;
+    if (normal) normal[i] = v[i].normal;
+  }
+}
+
+void orc_photon_set_destroy(orc_photon_set *s) { delete s; }
+
+double orc_radical_inverse(uint32_t n, uint32_t base) { return radical_inverse(n, base); }
 
 }  // extern "C"

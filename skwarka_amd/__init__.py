@@ -36,6 +36,8 @@ EXPORTS = [
     "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
     "skw_irradiance_map_create", "skw_irradiance_map_create_from", "skw_irradiance_map_destroy",
     "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
+    "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
+    "skw_photon_set_destroy",
 ]
 
 _P = ctypes.c_void_p
@@ -84,6 +86,10 @@ class IrrMapDesc(ctypes.Structure):
 
 class FGParams(ctypes.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("gather_samples", _U32), ("seed", _U64)]
+
+
+class ShootParams(ctypes.Structure):
+    _fields_ = [("max_paths", _U64), ("max_photons", _U64 * 3), ("paths_per_round", _U32), ("seed", _U64)]
 
 
 class BatchView(ctypes.Structure):
@@ -165,6 +171,11 @@ def lib():
         "skw_irradiance_map_destroy": ([_P], ctypes.c_int),
         "skw_irradiance_map_info": ([_P, _P, _P], ctypes.c_int),
         "skw_irradiance_map_export": ([_P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_shoot": ([_P, _P, ctypes.POINTER(ShootParams), ctypes.POINTER(_P), _P], ctypes.c_int),
+        "skw_photon_set_info": ([_P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_set_export": ([_P, _U32, _P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_set_map": ([_P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_photon_set_destroy": ([_P], ctypes.c_int),
         "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
                                  _P, _P, _P, _P], ctypes.c_int),
     }
@@ -414,6 +425,47 @@ class IrradianceMap:
         return pos, nid, ee, ei
 
 
+class PhotonSet:
+    """Photons shot by DLScene.shoot_photons: maps 0 caustic, 1 direct, 2 indirect."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        counts, paths, total = np.zeros(3, np.uint64), np.zeros(3, np.uint64), ctypes.c_uint64()
+        check(lib().skw_photon_set_info(handle, counts.ctypes.data, paths.ctypes.data, ctypes.byref(total)),
+              "skw_photon_set_info")
+        self.counts = [int(x) for x in counts]
+        self.paths = [int(x) for x in paths]
+        self.total_paths = total.value
+
+    def export(self, which):
+        n = self.counts[which]
+        m = {"pos": np.zeros((n, 3), np.float32), "weight": np.zeros((n, 3), np.float32),
+             "dir": np.zeros(n, np.uint16), "normal": np.zeros(n, np.uint16)}
+        check(lib().skw_photon_set_export(self.handle, which, m["pos"].ctypes.data, m["weight"].ctypes.data,
+                                          m["dir"].ctypes.data, m["normal"].ctypes.data), "skw_photon_set_export")
+        return m
+
+    def photon_map(self, which):
+        """The device PhotonMap (KDTree<Photon>) of one map."""
+        h = _P()
+        check(lib().skw_photon_set_map(self.handle, which, ctypes.byref(h)), "skw_photon_set_map")
+        pm = PhotonMap.__new__(PhotonMap)
+        pm.handle = h
+        pm.n = self.counts[which]
+        return pm
+
+    def close(self):
+        if self.handle:
+            lib().skw_photon_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def camera(origin, target_dir, up, x_fov_deg, width, height):
     """skw_camera for a pinhole looking along target_dir (look-at frame of Transform.cpp:138-165,
     tangents of PerspectiveCamera.cpp:20-34). Returns (Camera, cam15) where cam15 is the flat
@@ -478,6 +530,14 @@ class DLScene:
         check(lib().skw_dl_render(self.scene.handle, self.handle, ctypes.byref(cam), ctypes.byref(prm), d_rgb,
                                   rays.ctypes.data if count_rays else None, stream or None), "skw_dl_render")
         return rays
+
+    def shoot_photons(self, max_paths, max_photons, paths_per_round=1 << 20, seed=0, stream=0):
+        """skw_photon_shoot from this scene's area lights; max_photons = (caustic, direct, indirect)."""
+        prm = ShootParams(max_paths, (_U64 * 3)(*max_photons), paths_per_round, seed)
+        h = _P()
+        check(lib().skw_photon_shoot(self.scene.handle, self.handle, ctypes.byref(prm), ctypes.byref(h),
+                                     stream or None), "skw_photon_shoot")
+        return PhotonSet(h)
 
     def final_gather(self, irr, indirect, scene_area, cam, width, height, spp, gather_samples, d_rgb=None,
                      d_radiance=None, seed=0, stream=0, count_rays=True):

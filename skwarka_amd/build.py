@@ -20,6 +20,7 @@ SOURCES = [
     "abi_dl.cpp",
     "abi_batch.cpp",
     "abi_fg.cpp",
+    "abi_shoot.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",
@@ -27,6 +28,7 @@ SOURCES = [
     "kernels/dl.hip",
     "kernels/batch.hip",
     "kernels/sort.hip",
+    "kernels/shoot.hip",
 ]
 
 COMMON = [

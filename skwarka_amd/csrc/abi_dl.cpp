@@ -296,6 +296,12 @@ void dl_tables(const skw_dl_scene *dl, const DevMat **mats, const double **rad) 
   *mats = dl->d_mats;
   *rad = dl->d_rad;
 }
+void dl_lights(const skw_dl_scene *dl, const std::vector<DevLight> **host, const DevLight **d_lights,
+               const DevLightTri **d_ltris) {
+  *host = &dl->lights;
+  *d_lights = dl->d_lights;
+  *d_ltris = dl->d_ltris;
+}
 }  // namespace skw
 
 extern "C" {

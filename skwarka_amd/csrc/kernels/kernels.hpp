@@ -210,6 +210,40 @@ hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_resolve(const FgArgs &f, hipStream_t stream);
+// Photon shooting, one round of paths (shoot.hip). Active paths live in ping-pong lists: bounce
+// kernels read list `cur` (count counts[cur]) and append survivors to the other list.
+struct ShootPath {
+  double w[3];      // photon weight
+  uint32_t path;    // path index
+  uint32_t n_isect; // intersections so far
+};
+struct ShootArgs {
+  GeomArgs geom;
+  const DevMat *mats;
+  const DevLight *lights;
+  const DevLightTri *ltris;
+  const double *light_cdf;  // power CDF over the lights
+  uint32_t n_lights;
+  uint64_t first, n;        // the round's paths [first, first + n)
+  uint64_t seed;
+  uint32_t done[3];         // maps (caustic, direct, indirect) that take no photons this round
+  uint32_t cur;             // list read by the bounce kernel (0 or 1)
+  skw_ray *rays[2];
+  ShootPath *state[2];
+  const skw_hit *hits;      // closest hits of list `cur`
+  uint64_t *counts;         // [0], [1] list sizes; [2 + m] photons of map m this round
+  float *ph_pos[3], *ph_w[3];
+  uint32_t *ph_codes[3];    // incident direction id | normal id << 16
+  uint32_t *ph_key[3];      // path - first (the photons are later ordered by it, stably)
+};
+hipError_t launch_shoot_start(const ShootArgs &a, hipStream_t stream);
+hipError_t launch_shoot_bounce(const ShootArgs &a, uint64_t n_active, hipStream_t stream);
+hipError_t launch_iota_u32(uint32_t *out, uint64_t n, hipStream_t stream);  // out[j] = j
+// out[j] = in[order[j]] for the photon records of one map (n records).
+hipError_t launch_photon_gather_records(const float *pos_in, const float *w_in, const uint32_t *codes_in,
+                                        const uint32_t *order, uint64_t n, float *pos_out, float *w_out,
+                                        uint32_t *codes_out, hipStream_t stream);
+
 // Sorts n (key, value) pairs by key bits [begin_bit, end_bit) (sort.hip). With temp == nullptr
 // only sets temp_bytes.
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,

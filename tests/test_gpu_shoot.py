@@ -1,0 +1,99 @@
+"""GPU parity of photon shooting (skw_photon_shoot, through the C ABI) against the oracle's
+restatement of PhotonLTEIntegrator::ShootPhotons on the same path samples.
+
+Bar: photon counts and paths per map identical, photons in the same (path, intersection) order
+with identical direction / normal ids and positions and weights within 1e-6 relative (float
+records of FP64 values; the GPU's sin / cos / pow may differ from glibc's in the last bits)."""
+import numpy as np
+import pytest
+
+import oracle
+import skwarka_amd as sk
+from skwarka_amd import workloads as wl
+from tests.test_oracle_shoot import _lit_box, BIG
+
+pytestmark = pytest.mark.gpu
+
+
+def both(s, max_paths, max_photons, per_round, seed):
+    acc = oracle.Accel(s["meshes"])
+    odl = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    ref = oracle.photon_shoot(acc, odl, max_paths, max_photons, per_round, seed)
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    got = gdl.shoot_photons(max_paths, max_photons, per_round, seed)
+    return got, ref
+
+
+def check_maps(got, ref):
+    assert got.counts == [len(m["pos"]) for m in ref["maps"]]
+    assert got.paths == ref["paths"] and got.total_paths == ref["total_paths"]
+    for j in range(3):
+        g, r = got.export(j), ref["maps"][j]
+        assert np.array_equal(g["dir"], r["dir"]) and np.array_equal(g["normal"], r["normal"])
+        assert np.allclose(g["pos"], r["pos"], rtol=1e-6, atol=1e-6)
+        assert np.allclose(g["weight"], r["weight"], rtol=1e-6, atol=0)
+
+
+def test_shoot_small_room_matches_oracle(torch_cuda):
+    print("case: photon shooting, small room, 200k paths in rounds of 65536", flush=True)
+    s = wl.scene_room_small()
+    got, ref = both(s, 200000, (BIG, BIG, BIG), 1 << 16, seed=7)
+    assert got.counts[1] > 150000 and got.counts[2] > 100000 and got.counts[0] == 0
+    check_maps(got, ref)
+
+
+def test_shoot_rounds_and_full_maps(torch_cuda):
+    print("case: photon shooting, small room, full maps and a short last round", flush=True)
+    s = wl.scene_room_small(seed=9)
+    got, ref = both(s, 90000, (0, 30000, 70000), 20000, seed=8)
+    # direct full after two rounds, indirect after four: every map full, shooting stops early
+    assert got.paths[1] < got.total_paths < 90000
+    check_maps(got, ref)
+
+
+def test_shoot_lit_box_closed_forms(torch_cuda):
+    print("case: photon shooting, lit box (direct power, geometric indirect count)", flush=True)
+    meshes = [wl.box(1.0), wl.quad((-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25))]
+    mat, kd, ks, rough = np.zeros(2, np.uint8), np.full((2, 3), 0.5), np.zeros((2, 3)), np.full(2, 0.2)
+    scene = sk.Scene(meshes)
+    gdl = sk.DLScene(scene, mat, kd, ks, rough, np.array([1], np.uint32), np.array([[2.0, 2.0, 2.0]]))
+    n = 100000
+    got = gdl.shoot_photons(n, (BIG, BIG, BIG), 1 << 15, seed=2)
+    assert got.counts[1] == n
+    d, ind = got.export(1), got.export(2)
+    assert np.allclose(d["weight"], np.pi * 0.25 * 2.0, rtol=1e-6, atol=0)
+    assert np.allclose(ind["weight"], np.pi * 0.25 * 2.0, rtol=1e-6, atol=0)
+    assert abs(got.counts[2] / n - 1.0) < 0.03
+
+
+def test_photon_set_map_gathers_like_the_oracle_map(torch_cuda):
+    print("case: photon map from a shot set, irradiance against the oracle map", flush=True)
+    s = wl.scene_room_small()
+    got, ref = both(s, 50000, (BIG, BIG, BIG), 1 << 16, seed=3)
+    pm = got.photon_map(2)
+    r = ref["maps"][2]
+    om = oracle.PhotonMap(r["pos"], r["weight"], r["dir"], r["normal"])
+    q = r["pos"][::50].astype(np.float64)
+    qn = wl.cdir_table()[r["normal"][::50]]
+    nq = len(q)
+    from tests.conftest import empty_dev, from_dev, to_dev
+    dp, dn = to_dev(torch_cuda, q), to_dev(torch_cuda, qn)
+    ee, ei = empty_dev(torch_cuda, nq * 12), empty_dev(torch_cuda, nq * 12)
+    pm.irradiance(dp.data_ptr(), dn.data_ptr(), nq, 50, 0.3, got.paths[2], 0.5, ee.data_ptr(), ei.data_ptr())
+    torch_cuda.cuda.synchronize()
+    ge = from_dev(ee, np.float32, nq * 3).reshape(nq, 3)
+    re, _, _, _, _ = om.irradiance(q, qn, 50, 0.3, paths=ref["paths"][2], with_sets=False)
+    assert np.allclose(ge, re, rtol=1e-5, atol=1e-12)
+    assert (ge > 0).any()
+
+
+def test_shoot_rejects_bad_parameters(torch_cuda):
+    print("case: photon shooting argument checks", flush=True)
+    s = wl.scene_room_small()
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    with pytest.raises(sk.SkwError):
+        gdl.shoot_photons(1000, (1, 1, 1), 0)
+    empty = gdl.shoot_photons(0, (1, 1, 1))
+    assert empty.counts == [0, 0, 0] and empty.total_paths == 0
