@@ -129,6 +129,11 @@ skw_status skw_hit_geometry(const skw_scene *scene, const skw_ray *d_rays, const
 skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
                                  const uint16_t *normal, uint32_t flags, skw_photon_map **out);
 skw_status skw_photon_map_destroy(skw_photon_map *map);
+/* KDTree<Photon>::KDTree (KDTree.h:391-473) built on the device from device arrays: the same
+ * tree, node for node, as skw_photon_map_create (both follow a libstdc++ build of the reference:
+ * std::nth_element's introselect, restated exactly). d_weight, d_dir, d_normal may be NULL (0). */
+skw_status skw_photon_map_build(uint64_t n, const float *d_pos, const float *d_weight, const uint16_t *d_dir,
+                                const uint16_t *d_normal, skw_photon_map **out, void *stream);
 
 /* Tree-order permutation and packed nodes (split float, flags/right-child word) on the host. */
 skw_status skw_photon_map_export(const skw_photon_map *map, uint32_t *perm, float *split, uint32_t *word);
@@ -366,8 +371,8 @@ skw_status skw_photon_set_info(const skw_photon_set *set, uint64_t counts[3], ui
  * normal ids (any may be NULL). */
 skw_status skw_photon_set_export(const skw_photon_set *set, uint32_t which, float *pos, float *weight, uint16_t *dir,
                                  uint16_t *normal);
-/* The photon map (KDTree<Photon>) of map `which`, as skw_photon_map_create over the exported arrays. */
-skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out);
+/* The photon map (KDTree<Photon>) of map `which`, built on the device (skw_photon_map_build). */
+skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out, void *stream);
 skw_status skw_photon_set_destroy(skw_photon_set *set);
 
 /* ---------------------------------------------------------------- batched scenes

@@ -37,7 +37,7 @@ EXPORTS = [
     "skw_irradiance_map_create", "skw_irradiance_map_create_from", "skw_irradiance_map_destroy",
     "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
     "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
-    "skw_photon_set_destroy",
+    "skw_photon_set_destroy", "skw_photon_map_build",
 ]
 
 _P = ctypes.c_void_p
@@ -174,7 +174,8 @@ def lib():
         "skw_photon_shoot": ([_P, _P, ctypes.POINTER(ShootParams), ctypes.POINTER(_P), _P], ctypes.c_int),
         "skw_photon_set_info": ([_P, _P, _P, _P], ctypes.c_int),
         "skw_photon_set_export": ([_P, _U32, _P, _P, _P, _P], ctypes.c_int),
-        "skw_photon_set_map": ([_P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_photon_set_map": ([_P, _U32, ctypes.POINTER(_P), _P], ctypes.c_int),
+        "skw_photon_map_build": ([_U64, _P, _P, _P, _P, ctypes.POINTER(_P), _P], ctypes.c_int),
         "skw_photon_set_destroy": ([_P], ctypes.c_int),
         "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
                                  _P, _P, _P, _P], ctypes.c_int),
@@ -330,6 +331,17 @@ class PhotonMap:
                                       SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)), "skw_photon_map_create")
         self.handle = h
 
+    @classmethod
+    def build_on_device(cls, n, d_pos, d_weight=None, d_dir=None, d_normal=None, stream=0):
+        """skw_photon_map_build: the kd-tree built on the device from device arrays."""
+        h = _P()
+        check(lib().skw_photon_map_build(n, d_pos, d_weight, d_dir, d_normal, ctypes.byref(h), stream or None),
+              "skw_photon_map_build")
+        pm = cls.__new__(cls)
+        pm.handle = h
+        pm.n = n
+        return pm
+
     def close(self):
         if self.handle:
             lib().skw_photon_map_destroy(self.handle)
@@ -448,7 +460,7 @@ class PhotonSet:
     def photon_map(self, which):
         """The device PhotonMap (KDTree<Photon>) of one map."""
         h = _P()
-        check(lib().skw_photon_set_map(self.handle, which, ctypes.byref(h)), "skw_photon_set_map")
+        check(lib().skw_photon_set_map(self.handle, which, ctypes.byref(h), None), "skw_photon_set_map")
         pm = PhotonMap.__new__(PhotonMap)
         pm.handle = h
         pm.n = self.counts[which]

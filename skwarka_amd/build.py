@@ -21,6 +21,7 @@ SOURCES = [
     "abi_batch.cpp",
     "abi_fg.cpp",
     "abi_shoot.cpp",
+    "abi_kd.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",
@@ -29,6 +30,7 @@ SOURCES = [
     "kernels/batch.hip",
     "kernels/sort.hip",
     "kernels/shoot.hip",
+    "kernels/kd_build.hip",
 ]
 
 COMMON = [

@@ -330,15 +330,11 @@ skw_status skw_photon_set_export(const skw_photon_set *set, uint32_t which, floa
   return SKW_OK;
 }
 
-skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out) {
+skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out, void *stream) {
   if (!set || !out) return fail(SKW_INVALID_ARG, "set and out must not be NULL");
   if (which > 2) return fail(SKW_INVALID_ARG, "which must be 0, 1 or 2");
-  const uint64_t n = set->m[which].n;
-  std::vector<float> pos(3 * n), w(3 * n);
-  std::vector<uint16_t> dir(n), nrm(n);
-  skw_status s = skw_photon_set_export(set, which, pos.data(), w.data(), dir.data(), nrm.data());
-  if (s != SKW_OK) return s;
-  return skw_photon_map_create(n, pos.data(), w.data(), dir.data(), nrm.data(), 0, out);
+  const PhotonArrays &m = set->m[which];
+  return photon_map_build_device(m.n, m.pos, m.w, m.codes, nullptr, nullptr, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 skw_status skw_photon_set_destroy(skw_photon_set *set) {

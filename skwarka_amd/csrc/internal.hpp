@@ -102,6 +102,12 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
 // k nearest photons with PhotonFilter within r, ids as tree positions (row-major, k per query).
 skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                            uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream);
+// A photon map whose kd-tree is built on the device from device arrays (positions, weights, and
+// either codes = dir | normal << 16 or the two id arrays; weights / ids may be NULL).
+skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *d_weight, const uint32_t *d_codes,
+                                   const uint16_t *d_dir, const uint16_t *d_normal, skw_photon_map **out,
+                                   hipStream_t st);
+
 // Nearest photon with PhotonFilter within r of each query; thread i serves query d_order[i].
 skw_status photon_nearest_ordered(const skw_photon_map *map, const double *d_points, const double *d_normals,
                                   uint64_t nq, double r, const uint32_t *d_order, uint32_t *d_id, hipStream_t stream);

@@ -244,6 +244,28 @@ hipError_t launch_photon_gather_records(const float *pos_in, const float *w_in, 
                                         const uint32_t *order, uint64_t n, float *pos_out, float *w_out,
                                         uint32_t *codes_out, hipStream_t stream);
 
+// Photon kd-tree built on the device (kd_build.hip), element for element the tree of
+// KDTree::_Build with libstdc++'s nth_element.
+struct KdBuildArgs {
+  void *items;           // n kd items (position + caller index, 16 bytes each)
+  uint32_t *word;        // n node words
+  uint32_t *scr_l, *scr_r;  // n each: partition scratch of the large nodes
+  uint64_t n;
+};
+constexpr uint64_t kKdLdsItems = 2048;  // nodes up to this size are built in LDS
+constexpr uint64_t kKdSeqMax = 32;      // subtrees up to this size are built by one thread
+hipError_t launch_kd_init(const KdBuildArgs &a, const float *pos, hipStream_t stream);
+// One level's nodes: ranges as (begin, end) pairs; big = longer than kKdLdsItems.
+hipError_t launch_kd_big(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, hipStream_t stream);
+hipError_t launch_kd_mid(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, uint64_t max_size,
+                         hipStream_t stream);
+hipError_t launch_kd_small(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, hipStream_t stream);
+// Tree-order node records, permutation, payload (weights; codes = dir | normal << 16 by caller
+// index, or the two 16-bit id arrays) and parent splits.
+hipError_t launch_kd_finish(const KdBuildArgs &a, const float *weight, const uint32_t *codes, const uint16_t *dir,
+                            const uint16_t *normal, DevKdNode *nodes, DevPhoton *photons, uint32_t *perm,
+                            float *psplit, hipStream_t stream);
+
 // Sorts n (key, value) pairs by key bits [begin_bit, end_bit) (sort.hip). With temp == nullptr
 // only sets temp_bytes.
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,

@@ -26,6 +26,20 @@ def build(verbose=False):
     return OUT
 
 
+KD_OUT = os.path.join(HERE, "_build", "kd_select_check")
+
+
+def build_kd_select():
+    """tests/cpp/kd_select_check: the restated libstdc++ selection against libstdc++ (host only)."""
+    os.makedirs(os.path.dirname(KD_OUT), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "kd_select_check.cpp"), "-o", KD_OUT + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("kd_select_check build failed:\n%s\n%s" % (" ".join(cmd), res.stderr))
+    os.replace(KD_OUT + ".tmp", KD_OUT)
+    return KD_OUT
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)
