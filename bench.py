@@ -16,6 +16,8 @@ timed region starts:
   c5 (component): the photon integrator's final gather on the c2 scene at 1024x1024, 4 spp, 8
       gather samples (rounded to 9), over an irradiance photon map built from 100k direct, 400k
       indirect and 20k caustic photons -> Mrays/s over camera + gather rays.
+  c6 (component): photon shooting from the c2 scene's area lights, 4M paths -> M paths/s, and the
+      device kd-tree build of the indirect map against the host build.
 Timing: W untimed warm-up steps, then exactly K steps between HIP events recorded on the stream
 the kernels run on, with device synchronisation on both sides.
 The CPU baseline is the oracle (CPU restatement of the reference path, OpenMP) timed on this host
@@ -330,6 +332,65 @@ def cpu_baseline_c5(c5, threads, rows):
                       "one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
 
 
+def bench_c6(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
+    s = spec if spec is not None else wl.scene_room()
+    if scene is None:
+        scene = sk.Scene(s["meshes"])
+    dls = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    sptr = stream.cuda_stream
+    n_paths, per_round, big = 4_000_000, 1 << 20, 10 ** 12
+
+    def shoot():
+        return dls.shoot_photons(n_paths, (big, big, big), per_round, 77, sptr)
+
+    for _ in range(warmup):
+        shoot().close()
+    times, ps = [], None
+    for _ in range(steps):
+        if ps is not None:
+            ps.close()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ps = shoot()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    ms = 1e3 * statistics.median(times)
+    photons = sum(ps.counts)
+    # kd build of the indirect map: on the device (from the set) and on the host (from host arrays)
+    ps.photon_map(2).close()
+    dev_ms, host_ms = [], []
+    m = ps.export(2)
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pm = ps.photon_map(2)
+        torch.cuda.synchronize()
+        dev_ms.append(1e3 * (time.perf_counter() - t))
+        pm.close()
+        t = time.perf_counter()
+        pm = sk.PhotonMap(m["pos"], m["weight"], m["dir"], m["normal"])
+        host_ms.append(1e3 * (time.perf_counter() - t))
+        pm.close()
+    out = {"ms": ms, "value": n_paths / (ms * 1e-3) / 1e6, "photons": photons, "counts": ps.counts,
+           "paths": ps.paths, "photons_per_s": photons / (ms * 1e-3), "spec": s,
+           "kd_n": len(m["pos"]), "kd_device_ms": statistics.median(dev_ms), "kd_host_ms": statistics.median(host_ms)}
+    ps.close()
+    return out
+
+
+def cpu_baseline_c6(c6, threads, n_paths):
+    import oracle
+    s = c6["spec"]
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    t = time.perf_counter()
+    out = oracle.photon_shoot(acc, dls, n_paths, (10 ** 12,) * 3, 1 << 20, 77, nthreads=threads)
+    dt = time.perf_counter() - t
+    photons = sum(len(m["pos"]) for m in out["maps"])
+    return {"value": n_paths / dt / 1e6, "unit": "M paths/s", "cores": threads, "kind": "port",
+            "sample": "the first %d of the 4M paths (%d photons), one timed run of %.1f s" % (n_paths, photons, dt)}
+
+
 def cpu_baseline_c3(c3, threads, n_sample):
     import oracle
     import torch
@@ -429,7 +490,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5"], default=None,
+    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
     ap.add_argument("--cpu-rows", type=int, default=1024,
@@ -490,7 +551,13 @@ def main():
         log("c5: photon final gather, 1024x1024 x 4 spp, 9 gather samples")
         c5 = bench_c5(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
         log("c5: %.1f ms/frame, %.1f Mrays/s, rays %s" % (c5["ms"], c5["value"], c5["rays"]))
-    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = None
+    c6 = None
+    if run("c6"):
+        log("c6: photon shooting, 4M paths from the c2 scene's lights, device kd build")
+        c6 = bench_c6(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
+        log("c6: %.1f ms per 4M paths, %.1f M paths/s, photons %s, kd build %d photons: device %.1f ms, host %.1f ms"
+            % (c6["ms"], c6["value"], c6["counts"], c6["kd_n"], c6["kd_device_ms"], c6["kd_host_ms"]))
+    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
         if c2 is not None:
@@ -505,6 +572,8 @@ def main():
             cpu4 = cpu_baseline_c4(c4, threads, 64)
         if c5 is not None:
             cpu5 = cpu_baseline_c5(c5, threads, 256)
+        if c6 is not None:
+            cpu6 = cpu_baseline_c6(c6, threads, 4_000_000)
     comps = {}
     if c3 is not None:
         comps["c3_batch"] = {
@@ -528,6 +597,17 @@ def main():
             "workload": "c2 scene at 512x512, 8 spp, 4 light + 4 BSDF samples; target = perturbed parameters "
                         "(seed 4)",
             "cpu_baseline": cpu4,
+        }
+    if c6 is not None:
+        comps["c6_photon_shooting"] = {
+            "metric": "M photon paths/s (photon shooting from the area lights)", "value": round(c6["value"], 3),
+            "ms_per_step": round(c6["ms"], 3), "photons_per_step": c6["counts"], "paths_per_map": c6["paths"],
+            "photons_per_s": round(c6["photons_per_s"], 1),
+            "kd_build": {"photons": c6["kd_n"], "device_ms": round(c6["kd_device_ms"], 3),
+                         "host_ms": round(c6["kd_host_ms"], 3)},
+            "workload": "c2 scene, 4,000,000 paths in rounds of 2^20, all maps unbounded (seed 77); kd build "
+                        "of the indirect map on the device and on the host (same tree)",
+            "cpu_baseline": cpu6,
         }
     if c5 is not None:
         comps["c5_final_gather"] = {
@@ -588,7 +668,7 @@ def main():
         }
     else:
         c = (comps.get("c0_closest_hit") or comps.get("c1_photon_gather") or comps.get("c3_batch")
-             or comps.get("c4_adjoint") or comps.get("c5_final_gather"))
+             or comps.get("c4_adjoint") or comps.get("c5_final_gather") or comps.get("c6_photon_shooting"))
         line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
                 "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

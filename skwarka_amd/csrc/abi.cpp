@@ -169,7 +169,7 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
   a.psplit = map->d_psplit;
   a.perm = map->d_perm;
   a.cdir = map->d_cdir;
-  a.n_nodes = map->host.nodes.size();
+  a.n_nodes = map->n;
   a.points = d_points;
   a.normals = d_normals;
   a.nq = nq;
@@ -181,7 +181,7 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
   a.filter = filter;
   a.q_stride = k;  // a query's heap slots are contiguous
   a.j_stride = 1;
-  a.stack_need = map->host.max_depth + 1;
+  a.stack_need = map->max_depth + 1;
   return a;
 }
 
@@ -367,6 +367,8 @@ skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weig
   if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
   std::unique_ptr<skw_photon_map> m(new skw_photon_map());
   build_photon_tree(n, pos, weight, dir, normal, m->host);
+  m->n = n;
+  m->max_depth = m->host.max_depth;
   if (!host_only) {
     SKW_HIP(hipGetDevice(&m->device), "hipGetDevice");
     SKW_HIP(upload(&m->d_nodes, m->host.nodes), "upload kd nodes");
@@ -388,7 +390,8 @@ skw_status skw_photon_map_destroy(skw_photon_map *map) {
 
 skw_status skw_photon_map_export(const skw_photon_map *map, uint32_t *perm, float *split, uint32_t *word) {
   if (!map) return fail(SKW_INVALID_ARG, "map must not be NULL");
-  const KdLayout &h = map->host;
+  const KdLayout &h = photon_map_host(map);
+  if (h.nodes.size() != map->n) return fail(SKW_HIP_ERROR, "photon tree download failed");
   for (size_t i = 0; i < h.nodes.size(); ++i) {
     if (perm) perm[i] = h.perm[i];
     uint32_t w = h.nodes[i].word;

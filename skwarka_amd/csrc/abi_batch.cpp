@@ -183,7 +183,7 @@ skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const s
     if (!g.d_points || !g.d_normals || !g.d_e_ext || !g.d_e_int)
       return fail(SKW_INVALID_ARG, "gather points, normals, e_ext and e_int must not be NULL");
     if (!(g.r >= 0.0)) return fail(SKW_INVALID_ARG, "radius must be >= 0");
-    if (g.map->host.max_depth + 1 > 64) return fail(SKW_UNSUPPORTED, "a photon tree needs a stack deeper than 64");
+    if (g.map->max_depth + 1 > 64) return fail(SKW_UNSUPPORTED, "a photon tree needs a stack deeper than 64");
     GatherArgs a = gather_args(g.map, g.d_points, g.d_normals, g.nq, g.k, g.r, g.paths, g.rho, 1);
     a.e_ext = g.d_e_ext;
     a.e_int = g.d_e_int;

@@ -151,7 +151,8 @@ skw_status skw_irradiance_map_create(const skw_irradiance_map_desc *desc, skw_ir
   if (!(desc->scene_area >= 0.0) || !std::isfinite(desc->scene_area))
     return fail(SKW_INVALID_ARG, "scene_area must be finite and >= 0");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const KdLayout &ind = desc->indirect->host;
+  const KdLayout &ind = photon_map_host(desc->indirect);
+  if (ind.nodes.size() != desc->indirect->n) return fail(SKW_HIP_ERROR, "photon tree download failed");
   std::unique_ptr<skw_irradiance_map> m(new skw_irradiance_map());
   std::vector<double> pts, nrm;
   strided_queries(ind, 10, pts, nrm);
@@ -169,7 +170,7 @@ skw_status skw_irradiance_map_create(const skw_irradiance_map_desc *desc, skw_ir
     return SKW_OK;
   }
   // Radii holding 100 photons of each map on average over the scene area (:187-196).
-  const size_t nd = desc->direct->host.nodes.size(), ni = ind.nodes.size();
+  const size_t nd = desc->direct->n, ni = ind.nodes.size();
   const double r_direct = std::sqrt((nd == 0 ? 0.0 : desc->scene_area / (double)nd) * 100.0 * kInvPi);
   const double r_indirect = std::sqrt((ni == 0 ? 0.0 : desc->scene_area / (double)ni) * 100.0 * kInvPi);
   const double r_caustic = std::max(r_direct, r_indirect);
@@ -275,7 +276,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const uint64_t npix = (uint64_t)prm->width * prm->height;
   const uint32_t gq = (uint32_t)std::ceil(std::sqrt((double)prm->gather_samples));
   const uint32_t gs = gq * gq;
-  const uint64_t n_ind = indirect ? indirect->host.nodes.size() : 0;
+  const uint64_t n_ind = indirect ? indirect->n : 0;
   // PhotonLTEIntegrator.cpp:460-462: the radius holding 32 indirect photons on average.
   const double lookup_r = n_ind > 0 ? std::sqrt((scene_area / (double)n_ind) * kFgK * kInvPi) : 0.0;
   // Pixel batches: every slot holds up to 2 gs gather rays.

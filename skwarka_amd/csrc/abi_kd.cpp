@@ -87,6 +87,53 @@ Plan plan_levels(uint64_t n) {
   return p;
 }
 
+// Levels with fewer large nodes than this run each introselect step with many workgroups per
+// node (chunks of kKdChunk positions); the others with one workgroup per node.
+constexpr size_t kMultiNodes = 64;
+
+// The large nodes of one level, many workgroups per node: introselect steps until every node's
+// subrange fits in LDS (one small read-back of the node states per step), then each node's tail.
+hipError_t multi_level(const KdBuildArgs &a, const Plan &plan, const Plan::Level &L, hipStream_t st) {
+  const uint32_t nn = (uint32_t)L.big_n;
+  std::vector<KdStepNode> nodes(nn);
+  std::vector<uint32_t> chunk_node;
+  for (uint32_t i = 0; i < nn; ++i) {
+    KdStepNode &nd = nodes[i];
+    std::memset(&nd, 0, sizeof(nd));
+    nd.b = (int64_t)plan.ranges[2 * (L.big_off + i)];
+    nd.e = (int64_t)plan.ranges[2 * (L.big_off + i) + 1];
+    nd.chunk0 = (uint32_t)chunk_node.size();
+    nd.nchunks = (uint32_t)((uint64_t)(nd.e - nd.b + kKdChunk - 1) / kKdChunk);
+    for (uint32_t c = 0; c < nd.nchunks; ++c) chunk_node.push_back(i);
+  }
+  const uint32_t nc = (uint32_t)chunk_node.size();
+  KdStepNode *d_nodes = nullptr;
+  uint32_t *d_chunk = nullptr;
+  float *d_work = nullptr;
+  hipError_t e = pool_alloc((void **)&d_nodes, nn * sizeof(KdStepNode), st);
+  if (e == hipSuccess) e = pool_alloc((void **)&d_chunk, nc * sizeof(uint32_t), st);
+  if (e == hipSuccess) e = pool_alloc((void **)&d_work, (size_t)nc * 8 * sizeof(float), st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_nodes, nodes.data(), nn * sizeof(KdStepNode), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_chunk, chunk_node.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_kd_step_init(a, d_nodes, nn, d_chunk, nc, d_work, st);
+  // each step shrinks an active subrange, and the depth budget bounds the steps
+  for (int step = 0; e == hipSuccess && step < 256; ++step) {
+    e = launch_kd_step_begin(a, d_nodes, nn, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(nodes.data(), d_nodes, nn * sizeof(KdStepNode), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) break;
+    bool any = false;
+    for (const KdStepNode &nd : nodes) any = any || nd.active;
+    if (!any) break;
+    e = launch_kd_step_rest(a, d_nodes, nn, d_chunk, nc, d_work, st);
+  }
+  if (e == hipSuccess) e = launch_kd_step_tail(a, d_nodes, nn, st);
+  pool_free(d_nodes, st);
+  pool_free(d_chunk, st);
+  pool_free(d_work, st);
+  return e;
+}
+
 template <class T>
 hipError_t download(std::vector<T> &dst, const T *src, uint64_t n, hipStream_t st) {
   dst.resize(n);
@@ -129,29 +176,49 @@ skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *
     if (e == hipSuccess) e = hipMalloc((void **)&m->d_psplit, n * sizeof(float));
     if (e == hipSuccess) e = launch_kd_init(a, d_pos, st);
     for (const Plan::Level &L : plan.levels) {
-      if (e == hipSuccess) e = launch_kd_big(a, d_ranges + 2 * L.big_off, (uint32_t)L.big_n, st);
+      if (e != hipSuccess) break;
+      if (L.big_n > 0 && L.big_n < kMultiNodes) e = multi_level(a, plan, L, st);
+      else e = launch_kd_big(a, d_ranges + 2 * L.big_off, (uint32_t)L.big_n, st);
       if (e == hipSuccess) e = launch_kd_mid(a, d_ranges + 2 * L.mid_off, (uint32_t)L.mid_n, L.mid_max, st);
     }
     if (e == hipSuccess) e = launch_kd_small(a, d_ranges + 2 * plan.small_off, (uint32_t)plan.small_n, st);
     if (e == hipSuccess)
       e = launch_kd_finish(a, d_weight, d_codes, d_dir, d_normal, m->d_nodes, m->d_photons, m->d_perm, m->d_psplit,
                            st);
-    KdLayout &h = m->host;
-    if (e == hipSuccess) e = download(h.nodes, m->d_nodes, n, st);
-    if (e == hipSuccess) e = download(h.photons, m->d_photons, n, st);
-    if (e == hipSuccess) e = download(h.perm, m->d_perm, n, st);
-    if (e == hipSuccess) e = download(h.psplit, m->d_psplit, n, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     pool_free(work, st);
     pool_free(d_ranges, st);
   }
   if (e != hipSuccess) return hip_fail(e, "device kd build");
+  m->n = n;
+  m->max_depth = plan.max_depth;
   m->host.max_depth = plan.max_depth;
+  m->host_ready = n == 0;
   std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
   SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
   m->on_device = true;
   *out = m.release();
   return SKW_OK;
+}
+
+const KdLayout &photon_map_host(const skw_photon_map *map) {
+  skw_photon_map *m = const_cast<skw_photon_map *>(map);
+  std::lock_guard<std::mutex> lock(m->host_mu);
+  if (!m->host_ready) {
+    KdLayout &h = m->host;
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(m->device);
+    if (e == hipSuccess) e = download(h.nodes, m->d_nodes, m->n, nullptr);
+    if (e == hipSuccess) e = download(h.photons, m->d_photons, m->n, nullptr);
+    if (e == hipSuccess) e = download(h.perm, m->d_perm, m->n, nullptr);
+    if (e == hipSuccess) e = download(h.psplit, m->d_psplit, m->n, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    (void)hipSetDevice(prev);
+    if (e == hipSuccess) m->host_ready = true;
+    else h.nodes.clear();  // callers see the size mismatch
+  }
+  return m->host;
 }
 
 }  // namespace skw

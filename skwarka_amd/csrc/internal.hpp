@@ -74,7 +74,11 @@ struct skw_scene {
 };
 
 struct skw_photon_map {
-  skw::KdLayout host;
+  skw::KdLayout host;       // complete unless the tree was built on the device (see photon_map_host)
+  uint64_t n = 0;           // photons
+  uint32_t max_depth = 0;   // depth of the deepest node (the root is 0)
+  bool host_ready = true;   // `host` holds the tree (device-built maps fill it on first use)
+  std::mutex host_mu;
   bool on_device = false;
   int device = 0;
   skw::DevKdNode *d_nodes = nullptr;
@@ -98,6 +102,9 @@ skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, 
 // attached (heap: nq * k slots, slot-major; stack: only when the tree is deeper than 64).
 GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                        uint32_t k, double r, uint64_t paths, double rho, int filter);
+
+// The map's tree in host memory, downloaded on first use for device-built maps.
+const KdLayout &photon_map_host(const skw_photon_map *map);
 
 // k nearest photons with PhotonFilter within r, ids as tree positions (row-major, k per query).
 skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,

@@ -87,23 +87,42 @@ __device__ int64_t group_partition(Item *a, Pos *sl, Pos *sr, int axis, int64_t 
     s.move_median_to_first(f, f + 1, f + (l - f) / 2, l - 1);
   }
   __syncthreads();
-  const float P = kdsel::key(a[f], axis);
+  // keys of the split axis only: item p's key is float 4 p + axis
+  const float *kp = reinterpret_cast<const float *>(a) + axis;
+  const float P = kp[4 * f];
   const int64_t m = l - f;
   const int64_t seg = (m + G - 1) / G;
   const int64_t s0 = f + (int64_t)t * seg;
   const int64_t s1 = s0 + seg < l ? s0 + seg : l;
+  constexpr int U = G >= 1024 ? 32 : 8;  // keys loaded together (independent loads in flight per thread)
   uint32_t cg = 0, cl = 0;
-  for (int64_t p = s0; p < s1; ++p) {
-    const float k = kdsel::key(a[p], axis);
-    if (p > f && !(k < P)) ++cg;
-    if (!(P < k)) ++cl;
+  for (int64_t p0 = s0; p0 < s1; p0 += U) {
+    float k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = p0 + u < s1 ? kp[4 * (p0 + u)] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t p = p0 + u;
+      if (p < s1) {
+        cg += (p > f && !(k[u] < P)) ? 1u : 0u;
+        cl += !(P < k[u]) ? 1u : 0u;
+      }
+    }
   }
   uint32_t eg, el, ng, nl;
   scan2<G>(cg, cl, eg, el, ng, nl, sh);
-  for (int64_t p = s0; p < s1; ++p) {
-    const float k = kdsel::key(a[p], axis);
-    if (p > f && !(k < P)) sl[eg++] = (Pos)(p - f);
-    if (!(P < k)) sr[el++] = (Pos)(p - f);
+  for (int64_t p0 = s0; p0 < s1; p0 += U) {
+    float k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) k[u] = p0 + u < s1 ? kp[4 * (p0 + u)] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t p = p0 + u;
+      if (p < s1) {
+        if (p > f && !(k[u] < P)) sl[eg++] = (Pos)(p - f);
+        if (!(P < k[u])) sr[el++] = (Pos)(p - f);
+      }
+    }
   }
   __syncthreads();
   if (t == 0) {
@@ -207,25 +226,25 @@ __global__ void k_kd_init(KdBuildArgs a, const float *pos) {
   static_cast<Item *>(a.items)[i] = x;
 }
 
-__global__ __launch_bounds__(kBig) void k_kd_big(KdBuildArgs a, const uint64_t *ranges) {
-  __shared__ Item lds[kLds];
-  __shared__ uint16_t lsl[kLds], lsr[kLds];
-  __shared__ uint32_t sh[2 * (kBig / 64)];
-  __shared__ int64_t sh64[1];
-  __shared__ float shf[6 * (kBig / 64)];
-  __shared__ int sh_axis;
+struct BigShared {
+  Item lds[kLds];
+  uint16_t lsl[kLds], lsr[kLds];
+  uint32_t sh[2 * (kBig / 64)];
+  int64_t sh64[1];
+  float shf[6 * (kBig / 64)];
+  int axis;
+};
+
+// The rest of a large node's introselect from state (f, l, depth), then its node word.
+__device__ void big_node_finish(const KdBuildArgs &a, BigShared &S, int64_t b, int64_t e, int axis, int64_t f,
+                                int64_t l, int64_t depth) {
   Item *items = static_cast<Item *>(a.items);
   const int t = threadIdx.x;
-  const int64_t b = (int64_t)ranges[2 * blockIdx.x], e = (int64_t)ranges[2 * blockIdx.x + 1];
-  const int axis = group_axis<kBig>(items, b, e, shf, &sh_axis);
   const int64_t nth = (b + e) / 2;
-  int64_t f = b, l = e, depth = 2 * kdsel::lg((uint64_t)(e - b));
-  while (l - f > kLds && depth > 0) {
-    --depth;
-    const int64_t cut = group_partition<kBig, uint32_t>(items, a.scr_l + f, a.scr_r + f, axis, f, l, sh, sh64);
-    if (cut <= nth) f = cut;
-    else l = cut;
-  }
+  Item *lds = S.lds;
+  uint16_t *lsl = S.lsl, *lsr = S.lsr;
+  uint32_t *sh = S.sh;
+  int64_t *sh64 = S.sh64;
   if (l - f > kLds) {
     // depth budget spent on a long subrange: libstdc++'s heap selection, sequentially
     if (t == 0) {
@@ -258,6 +277,259 @@ __global__ __launch_bounds__(kBig) void k_kd_big(KdBuildArgs a, const uint64_t *
     items[nth] = x;
     a.word[b] = node_word(axis, (uint64_t)b, (uint64_t)e, (uint64_t)nth);
   }
+}
+
+__global__ __launch_bounds__(kBig) void k_kd_big(KdBuildArgs a, const uint64_t *ranges) {
+  __shared__ BigShared S;
+  Item *items = static_cast<Item *>(a.items);
+  const int64_t b = (int64_t)ranges[2 * blockIdx.x], e = (int64_t)ranges[2 * blockIdx.x + 1];
+  const int axis = group_axis<kBig>(items, b, e, S.shf, &S.axis);
+  const int64_t nth = (b + e) / 2;
+  int64_t f = b, l = e, depth = 2 * kdsel::lg((uint64_t)(e - b));
+  while (l - f > kLds && depth > 0) {
+    --depth;
+    const int64_t cut =
+        group_partition<kBig, uint32_t>(items, a.scr_l + f, a.scr_r + f, axis, f, l, S.sh, S.sh64);
+    if (cut <= nth) f = cut;
+    else l = cut;
+  }
+  big_node_finish(a, S, b, e, axis, f, l, depth);
+}
+
+// ---------------------------------------------------------------- many workgroups per node
+// A level's large nodes are cut into chunks of kKdChunk positions (chunk_node[j] = node of chunk
+// j); per chunk, work holds 8 floats (the bbox part, then counts / offsets as uint32).
+
+__device__ __forceinline__ void chunk_span(const KdStepNode &nd, uint32_t j, int64_t &c0, int64_t &c1) {
+  c0 = nd.b + (int64_t)(j - nd.chunk0) * (int64_t)kKdChunk;
+  c1 = c0 + (int64_t)kKdChunk < nd.e ? c0 + (int64_t)kKdChunk : nd.e;
+}
+
+__global__ __launch_bounds__(kBig) void k_st_bbox(KdBuildArgs a, const KdStepNode *nodes, const uint32_t *chunk_node,
+                                                  float *work) {
+  __shared__ float shf[6 * (kBig / 64)];
+  const uint32_t j = blockIdx.x;
+  const KdStepNode nd = nodes[chunk_node[j]];
+  int64_t c0, c1;
+  chunk_span(nd, j, c0, c1);
+  const Item *items = static_cast<const Item *>(a.items);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+  for (int64_t i = c0 + t; i < c1; i += kBig) {
+    const Item x = items[i];
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fminx(mn[c], x.p[c]);
+      mx[c] = fmaxx(mx[c], x.p[c]);
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1)
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fminx(mn[c], __shfl_xor(mn[c], d));
+      mx[c] = fmaxx(mx[c], __shfl_xor(mx[c], d));
+    }
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) {
+      shf[6 * wave + c] = mn[c];
+      shf[6 * wave + 3 + c] = mx[c];
+    }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < kBig / 64; ++w)
+      for (int c = 0; c < 3; ++c) {
+        shf[c] = fminx(shf[c], shf[6 * w + c]);
+        shf[3 + c] = fmaxx(shf[3 + c], shf[6 * w + 3 + c]);
+      }
+    for (int c = 0; c < 6; ++c) work[8 * (uint64_t)j + c] = shf[c];
+  }
+}
+
+// One thread per node: the node's bbox from its chunks' parts, the split axis, the initial state.
+__global__ void k_st_axis(KdStepNode *nodes, uint32_t n_nodes, const float *work) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  KdStepNode nd = nodes[i];
+  float mn[3], mx[3];
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = work[8 * (uint64_t)nd.chunk0 + c];
+    mx[c] = work[8 * (uint64_t)nd.chunk0 + 3 + c];
+  }
+  for (uint32_t j = nd.chunk0 + 1; j < nd.chunk0 + nd.nchunks; ++j)
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fminx(mn[c], work[8 * (uint64_t)j + c]);
+      mx[c] = fmaxx(mx[c], work[8 * (uint64_t)j + 3 + c]);
+    }
+  nd.axis = longest_axis(mn, mx);
+  nd.f = nd.b;
+  nd.l = nd.e;
+  nd.nth = (nd.b + nd.e) / 2;
+  nd.depth = 2 * kdsel::lg((uint64_t)(nd.e - nd.b));
+  nd.active = 1;
+  nodes[i] = nd;
+}
+
+// Begin a step: nodes whose subrange fits in LDS (or whose depth budget is spent) leave the loop;
+// the others take the median of three to f and record the pivot.
+__global__ void k_st_begin(KdBuildArgs a, KdStepNode *nodes, uint32_t n_nodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  KdStepNode nd = nodes[i];
+  if (!nd.active) return;
+  if (nd.l - nd.f <= kLds || nd.depth == 0) {
+    nd.active = 0;
+  } else {
+    --nd.depth;
+    Seq s{static_cast<Item *>(a.items), nd.axis};
+    s.move_median_to_first(nd.f, nd.f + 1, nd.f + (nd.l - nd.f) / 2, nd.l - 1);
+    nd.pivot = kdsel::key(static_cast<const Item *>(a.items)[nd.f], nd.axis);
+  }
+  nodes[i] = nd;
+}
+
+// Per chunk: this thread's contiguous segment of the chunk within [f, l) (16 positions).
+__device__ __forceinline__ void st_segment(const KdStepNode &nd, uint32_t j, int64_t &s0, int64_t &s1) {
+  int64_t c0, c1;
+  chunk_span(nd, j, c0, c1);
+  if (c0 < nd.f) c0 = nd.f;
+  if (c1 > nd.l) c1 = nd.l;
+  constexpr int64_t seg = (int64_t)kKdChunk / kBig;
+  s0 = c0 + (int64_t)threadIdx.x * seg;
+  s1 = s0 + seg < c1 ? s0 + seg : c1;
+  if (s0 > c1) s0 = c1;
+}
+
+__global__ __launch_bounds__(kBig) void k_st_count(KdBuildArgs a, const KdStepNode *nodes, const uint32_t *chunk_node,
+                                                   uint32_t *work) {
+  __shared__ uint32_t sh[2 * (kBig / 64)];
+  const uint32_t j = blockIdx.x;
+  const KdStepNode nd = nodes[chunk_node[j]];
+  if (!nd.active) return;
+  int64_t s0, s1;
+  st_segment(nd, j, s0, s1);
+  const float *kp = reinterpret_cast<const float *>(a.items) + nd.axis;
+  const float P = nd.pivot;
+  uint32_t cg = 0, cl = 0;
+  for (int64_t p = s0; p < s1; ++p) {
+    const float k = kp[4 * p];
+    cg += (p > nd.f && !(k < P)) ? 1u : 0u;
+    cl += !(P < k) ? 1u : 0u;
+  }
+  uint32_t eg, el, tg, tl;
+  scan2<kBig>(cg, cl, eg, el, tg, tl, sh);
+  if (threadIdx.x == 0) {
+    work[8 * (uint64_t)j] = tg;
+    work[8 * (uint64_t)j + 1] = tl;
+  }
+}
+
+// One workgroup per node: exclusive offsets of the node's chunks, the candidate totals.
+__global__ __launch_bounds__(kBig) void k_st_scan(KdStepNode *nodes, uint32_t *work) {
+  __shared__ uint32_t sh[2 * (kBig / 64)];
+  __shared__ uint32_t carry[2];
+  KdStepNode &nd = nodes[blockIdx.x];
+  if (!nd.active) return;
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nd.nchunks; base += kBig) {
+    const uint32_t j = nd.chunk0 + base + threadIdx.x;
+    const bool in = base + threadIdx.x < nd.nchunks;
+    const uint32_t cg = in ? work[8 * (uint64_t)j] : 0u, cl = in ? work[8 * (uint64_t)j + 1] : 0u;
+    uint32_t eg, el, tg, tl;
+    scan2<kBig>(cg, cl, eg, el, tg, tl, sh);
+    if (in) {
+      work[8 * (uint64_t)j + 2] = carry[0] + eg;
+      work[8 * (uint64_t)j + 3] = carry[1] + el;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      carry[0] += tg;
+      carry[1] += tl;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    nd.nl = carry[0];
+    nd.nr = carry[1];
+  }
+}
+
+__global__ __launch_bounds__(kBig) void k_st_scatter(KdBuildArgs a, const KdStepNode *nodes,
+                                                     const uint32_t *chunk_node, const uint32_t *work) {
+  __shared__ uint32_t sh[2 * (kBig / 64)];
+  const uint32_t j = blockIdx.x;
+  const KdStepNode nd = nodes[chunk_node[j]];
+  if (!nd.active) return;
+  int64_t s0, s1;
+  st_segment(nd, j, s0, s1);
+  const float *kp = reinterpret_cast<const float *>(a.items) + nd.axis;
+  const float P = nd.pivot;
+  uint32_t cg = 0, cl = 0;
+  for (int64_t p = s0; p < s1; ++p) {
+    const float k = kp[4 * p];
+    cg += (p > nd.f && !(k < P)) ? 1u : 0u;
+    cl += !(P < k) ? 1u : 0u;
+  }
+  uint32_t eg, el, tg, tl;
+  scan2<kBig>(cg, cl, eg, el, tg, tl, sh);
+  eg += work[8 * (uint64_t)j + 2];
+  el += work[8 * (uint64_t)j + 3];
+  uint32_t *sl = a.scr_l + nd.f, *sr = a.scr_r + nd.f;
+  for (int64_t p = s0; p < s1; ++p) {
+    const float k = kp[4 * p];
+    if (p > nd.f && !(k < P)) sl[eg++] = (uint32_t)(p - nd.f);
+    if (!(P < k)) sr[el++] = (uint32_t)(p - nd.f);
+  }
+}
+
+// One thread per node: the swap count (binary search), the cut, the next subrange.
+__global__ void k_st_cut(KdBuildArgs a, KdStepNode *nodes, uint32_t n_nodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  KdStepNode nd = nodes[i];
+  if (!nd.active) return;
+  const uint32_t *sl = a.scr_l + nd.f, *sr = a.scr_r + nd.f;
+  uint32_t lo = 0, hi = nd.nl < nd.nr ? nd.nl : nd.nr;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (sl[mid] < sr[nd.nr - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  const int64_t K = lo;
+  int64_t cut;
+  if (K == 0) {
+    cut = nd.f + (int64_t)sl[0];
+  } else {
+    cut = nd.f + (int64_t)sr[nd.nr - K];
+    if ((uint32_t)K < nd.nl && nd.f + (int64_t)sl[K] < cut) cut = nd.f + (int64_t)sl[K];
+  }
+  nd.f_old = nd.f;
+  nd.K = K;
+  if (cut <= nd.nth) nd.f = cut;
+  else nd.l = cut;
+  nodes[i] = nd;
+}
+
+// Swaps of the last step: chunk j of a node swaps pairs [c kKdChunk, (c + 1) kKdChunk) of its K.
+__global__ __launch_bounds__(kBig) void k_st_swap(KdBuildArgs a, const KdStepNode *nodes, const uint32_t *chunk_node) {
+  const uint32_t j = blockIdx.x;
+  const KdStepNode nd = nodes[chunk_node[j]];
+  if (!nd.active) return;
+  const int64_t k0 = (int64_t)(j - nd.chunk0) * (int64_t)kKdChunk;
+  const int64_t k1 = k0 + (int64_t)kKdChunk < nd.K ? k0 + (int64_t)kKdChunk : nd.K;
+  Item *items = static_cast<Item *>(a.items);
+  const uint32_t *sl = a.scr_l + nd.f_old, *sr = a.scr_r + nd.f_old;
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += kBig) {
+    const int64_t p = nd.f_old + (int64_t)sl[k], q = nd.f_old + (int64_t)sr[nd.nr - 1 - k];
+    const Item x = items[p];
+    items[p] = items[q];
+    items[q] = x;
+  }
+}
+
+__global__ __launch_bounds__(kBig) void k_st_tail(KdBuildArgs a, const KdStepNode *nodes) {
+  __shared__ BigShared S;
+  const KdStepNode nd = nodes[blockIdx.x];
+  big_node_finish(a, S, nd.b, nd.e, nd.axis, nd.f, nd.l, nd.depth);
 }
 
 // One wave per node of at most kLds items, the node in LDS (dynamic: items, then two uint16 position
@@ -392,6 +664,38 @@ hipError_t launch_kd_init(const KdBuildArgs &a, const float *pos, hipStream_t st
 hipError_t launch_kd_big(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, hipStream_t stream) {
   if (n_ranges == 0) return hipSuccess;
   hipLaunchKernelGGL(k_kd_big, dim3(n_ranges), dim3(kBig), 0, stream, a, ranges);
+  return hipGetLastError();
+}
+
+hipError_t launch_kd_step_init(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, const uint32_t *chunk_node,
+                               uint32_t n_chunks, float *work, hipStream_t stream) {
+  if (n_nodes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_st_bbox, dim3(n_chunks), dim3(kBig), 0, stream, a, nodes, chunk_node, work);
+  hipLaunchKernelGGL(k_st_axis, dim3((n_nodes + 63) / 64), dim3(64), 0, stream, nodes, n_nodes, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_kd_step_begin(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, hipStream_t stream) {
+  if (n_nodes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_st_begin, dim3((n_nodes + 63) / 64), dim3(64), 0, stream, a, nodes, n_nodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_kd_step_rest(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, const uint32_t *chunk_node,
+                               uint32_t n_chunks, float *work, hipStream_t stream) {
+  if (n_nodes == 0) return hipSuccess;
+  uint32_t *w = reinterpret_cast<uint32_t *>(work);
+  hipLaunchKernelGGL(k_st_count, dim3(n_chunks), dim3(kBig), 0, stream, a, nodes, chunk_node, w);
+  hipLaunchKernelGGL(k_st_scan, dim3(n_nodes), dim3(kBig), 0, stream, nodes, w);
+  hipLaunchKernelGGL(k_st_scatter, dim3(n_chunks), dim3(kBig), 0, stream, a, nodes, chunk_node, w);
+  hipLaunchKernelGGL(k_st_cut, dim3((n_nodes + 63) / 64), dim3(64), 0, stream, a, nodes, n_nodes);
+  hipLaunchKernelGGL(k_st_swap, dim3(n_chunks), dim3(kBig), 0, stream, a, nodes, chunk_node);
+  return hipGetLastError();
+}
+
+hipError_t launch_kd_step_tail(const KdBuildArgs &a, const KdStepNode *nodes, uint32_t n_nodes, hipStream_t stream) {
+  if (n_nodes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_st_tail, dim3(n_nodes), dim3(kBig), 0, stream, a, nodes);
   return hipGetLastError();
 }
 

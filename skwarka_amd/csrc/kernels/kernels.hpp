@@ -260,6 +260,25 @@ hipError_t launch_kd_big(const KdBuildArgs &a, const uint64_t *ranges, uint32_t 
 hipError_t launch_kd_mid(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, uint64_t max_size,
                          hipStream_t stream);
 hipError_t launch_kd_small(const KdBuildArgs &a, const uint64_t *ranges, uint32_t n_ranges, hipStream_t stream);
+// Large nodes of the top levels, many workgroups per node: the state of one node's introselect.
+struct KdStepNode {
+  int64_t b, e, f, l, nth, depth;
+  int64_t f_old, K;        // the last step's start and swap count
+  int32_t axis, active;
+  float pivot;
+  uint32_t nl, nr;         // the last step's candidate counts (>= pivot from the left, <= pivot)
+  uint32_t chunk0, nchunks;  // this node's chunks in the level's chunk list
+};
+constexpr uint64_t kKdChunk = 16384;  // positions per workgroup of the multi-workgroup steps
+// chunk_node[j]: node of chunk j; work: 8 floats + 4 uint32 per chunk.
+hipError_t launch_kd_step_init(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, const uint32_t *chunk_node,
+                               uint32_t n_chunks, float *work, hipStream_t stream);
+// One introselect step: begin (median of three, pivot; nodes leave when short or out of depth).
+hipError_t launch_kd_step_begin(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, hipStream_t stream);
+hipError_t launch_kd_step_rest(const KdBuildArgs &a, KdStepNode *nodes, uint32_t n_nodes, const uint32_t *chunk_node,
+                               uint32_t n_chunks, float *work, hipStream_t stream);
+// The rest of each node in one workgroup (LDS tail, sequential end, node word).
+hipError_t launch_kd_step_tail(const KdBuildArgs &a, const KdStepNode *nodes, uint32_t n_nodes, hipStream_t stream);
 // Tree-order node records, permutation, payload (weights; codes = dir | normal << 16 by caller
 // index, or the two 16-bit id arrays) and parent splits.
 hipError_t launch_kd_finish(const KdBuildArgs &a, const float *weight, const uint32_t *codes, const uint16_t *dir,
