@@ -129,6 +129,8 @@ skw_status skw_hit_geometry(const skw_scene *scene, const skw_ray *d_rays, const
 skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weight, const uint16_t *dir,
                                  const uint16_t *normal, uint32_t flags, skw_photon_map **out);
 skw_status skw_photon_map_destroy(skw_photon_map *map);
+/* Photons in the map and the depth of its deepest node (host outputs, either may be NULL). */
+skw_status skw_photon_map_info(const skw_photon_map *map, uint64_t *n, uint32_t *max_depth);
 /* KDTree<Photon>::KDTree (KDTree.h:391-473) built on the device from device arrays: the same
  * tree, node for node, as skw_photon_map_create (both follow a libstdc++ build of the reference:
  * std::nth_element's introselect, restated exactly). d_weight, d_dir, d_normal may be NULL (0). */
@@ -374,6 +376,20 @@ skw_status skw_photon_set_export(const skw_photon_set *set, uint32_t which, floa
 /* The photon map (KDTree<Photon>) of map `which`, built on the device (skw_photon_map_build). */
 skw_status skw_photon_set_map(const skw_photon_set *set, uint32_t which, skw_photon_map **out, void *stream);
 skw_status skw_photon_set_destroy(skw_photon_set *set);
+
+/* ---------------------------------------------------------------- on-disk formats
+ * Checkpoint / resume of the built structures, which the reference keeps only in memory: the scene
+ * (flattened tree, triangles, instances, meshes, primitives), a photon map (kd-tree, payload) and
+ * an irradiance photon map, each as a versioned little-endian file with an FNV-1a 64 trailer.
+ * Loading skips every build and gives structures identical to the saved ones; files that fail the
+ * magic, version, byte-order, size or checksum checks are rejected with SKW_INVALID_ARG. flags:
+ * SKW_SCENE_HOST_ONLY loads without device copies. */
+skw_status skw_scene_save(const skw_scene *scene, const char *path);
+skw_status skw_scene_load(const char *path, uint32_t flags, skw_scene **out);
+skw_status skw_photon_map_save(const skw_photon_map *map, const char *path);
+skw_status skw_photon_map_load(const char *path, uint32_t flags, skw_photon_map **out);
+skw_status skw_irradiance_map_save(const skw_irradiance_map *map, const char *path);
+skw_status skw_irradiance_map_load(const char *path, skw_irradiance_map **out);
 
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in

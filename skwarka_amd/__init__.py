@@ -26,7 +26,7 @@ SCENE_HOST_ONLY = 1
 EXPORTS = [
     "skw_scene_create", "skw_scene_destroy", "skw_scene_bounds", "skw_scene_info",
     "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
-    "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export",
+    "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export", "skw_photon_map_info",
     "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
     "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
@@ -38,6 +38,8 @@ EXPORTS = [
     "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
     "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
     "skw_photon_set_destroy", "skw_photon_map_build",
+    "skw_scene_save", "skw_scene_load", "skw_photon_map_save", "skw_photon_map_load", "skw_irradiance_map_save",
+    "skw_irradiance_map_load",
 ]
 
 _P = ctypes.c_void_p
@@ -133,6 +135,7 @@ def lib():
         "skw_photon_map_create": ([_U64, _P, _P, _P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_photon_map_destroy": ([_P], ctypes.c_int),
         "skw_photon_map_export": ([_P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_map_info": ([_P, _P, _P], ctypes.c_int),
         "skw_photon_irradiance": ([_P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double,
                                    _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_photon_knn": ([_P, _P, _P, _U64, _U32, ctypes.c_double, ctypes.c_int, _P, _P, _P, _P],
@@ -176,6 +179,12 @@ def lib():
         "skw_photon_set_export": ([_P, _U32, _P, _P, _P, _P], ctypes.c_int),
         "skw_photon_set_map": ([_P, _U32, ctypes.POINTER(_P), _P], ctypes.c_int),
         "skw_photon_map_build": ([_U64, _P, _P, _P, _P, ctypes.POINTER(_P), _P], ctypes.c_int),
+        "skw_scene_save": ([_P, ctypes.c_char_p], ctypes.c_int),
+        "skw_scene_load": ([ctypes.c_char_p, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_photon_map_save": ([_P, ctypes.c_char_p], ctypes.c_int),
+        "skw_photon_map_load": ([ctypes.c_char_p, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_irradiance_map_save": ([_P, ctypes.c_char_p], ctypes.c_int),
+        "skw_irradiance_map_load": ([ctypes.c_char_p, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_photon_set_destroy": ([_P], ctypes.c_int),
         "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
                                  _P, _P, _P, _P], ctypes.c_int),
@@ -268,6 +277,20 @@ class Scene:
               "skw_scene_create")
         self.handle = h
 
+    def save(self, path):
+        """skw_scene_save: the built scene (tree, triangles, meshes, primitives) to a file."""
+        check(lib().skw_scene_save(self.handle, os.fsencode(path)), "skw_scene_save")
+
+    @classmethod
+    def load(cls, path, host_only=False):
+        """skw_scene_load: a saved scene, without rebuilding its tree."""
+        h = _P()
+        check(lib().skw_scene_load(os.fsencode(path), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
+              "skw_scene_load")
+        sc = cls.__new__(cls)
+        sc.handle = h
+        return sc
+
     def close(self):
         if self.handle:
             lib().skw_scene_destroy(self.handle)
@@ -330,6 +353,21 @@ class PhotonMap:
         check(L.skw_photon_map_create(self.n, _ptr(self._pos), _ptr(self._w), _ptr(self._d), _ptr(self._n),
                                       SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)), "skw_photon_map_create")
         self.handle = h
+
+    def save(self, path):
+        check(lib().skw_photon_map_save(self.handle, os.fsencode(path)), "skw_photon_map_save")
+
+    @classmethod
+    def load(cls, path, host_only=False):
+        h = _P()
+        check(lib().skw_photon_map_load(os.fsencode(path), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
+              "skw_photon_map_load")
+        pm = cls.__new__(cls)
+        pm.handle = h
+        n, depth = ctypes.c_uint64(), ctypes.c_uint32()
+        check(lib().skw_photon_map_info(h, ctypes.byref(n), ctypes.byref(depth)), "skw_photon_map_info")
+        pm.n = n.value
+        return pm
 
     @classmethod
     def build_on_device(cls, n, d_pos, d_weight=None, d_dir=None, d_normal=None, stream=0):
@@ -414,6 +452,23 @@ class IrradianceMap:
         check(L.skw_irradiance_map_info(h, ctypes.byref(n), ctypes.byref(md)), "skw_irradiance_map_info")
         self.n = n.value
         self.max_lookup_dist = md.value
+
+    def save(self, path):
+        check(lib().skw_irradiance_map_save(self.handle, os.fsencode(path)), "skw_irradiance_map_save")
+
+    @classmethod
+    def load(cls, path):
+        h = _P()
+        check(lib().skw_irradiance_map_load(os.fsencode(path), ctypes.byref(h)), "skw_irradiance_map_load")
+        m = cls.__new__(cls)
+        m.handle = h
+        m._keep = ()
+        n = ctypes.c_uint64()
+        md = ctypes.c_double()
+        check(lib().skw_irradiance_map_info(h, ctypes.byref(n), ctypes.byref(md)), "skw_irradiance_map_info")
+        m.n = n.value
+        m.max_lookup_dist = md.value
+        return m
 
     def close(self):
         if self.handle:

@@ -22,6 +22,7 @@ SOURCES = [
     "abi_fg.cpp",
     "abi_shoot.cpp",
     "abi_kd.cpp",
+    "io.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",

@@ -207,6 +207,22 @@ GeomArgs skw_scene::geom_args() const {
   return g;
 }
 
+namespace skw {
+skw_status finish_photon_map(skw_photon_map *m) {
+  m->n = m->host.nodes.size();
+  m->max_depth = m->host.max_depth;
+  SKW_HIP(hipGetDevice(&m->device), "hipGetDevice");
+  SKW_HIP(upload(&m->d_nodes, m->host.nodes), "upload kd nodes");
+  SKW_HIP(upload(&m->d_photons, m->host.photons), "upload photons");
+  SKW_HIP(upload(&m->d_psplit, m->host.psplit), "upload parent splits");
+  SKW_HIP(upload(&m->d_perm, m->host.perm), "upload permutation");
+  std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
+  SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
+  m->on_device = true;
+  return SKW_OK;
+}
+}  // namespace skw
+
 skw_photon_map::~skw_photon_map() {
   void *ptrs[] = {d_nodes, d_photons, d_psplit, d_perm, d_cdir};
   for (void *p : ptrs)
@@ -219,14 +235,12 @@ const char *skw_last_error(void) { return skw::last_error(); }
 
 int skw_device_count(void) { return device_count(); }
 
-skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scene **out) {
-  if (!desc || !out) return fail(SKW_INVALID_ARG, "desc and out must not be NULL");
-  *out = nullptr;
-  bool host_only = (flags & SKW_SCENE_HOST_ONLY) != 0;
-  if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
-  std::unique_ptr<skw_scene> s(new skw_scene());
-  std::string err;
-  if (!build_triangle_tree(*desc, s->host, err)) return fail(SKW_INVALID_ARG, err);
+}  // extern "C"
+
+namespace skw {
+// Device tables of a scene whose host layout is complete (built or loaded).
+skw_status finish_scene(std::unique_ptr<skw_scene> &sp, bool host_only) {
+  skw_scene *s = sp.get();
   s->has_inst = !s->host.insts.empty();
   std::vector<uint32_t> users(s->host.meshes.size(), 0);
   for (const HostPrim &p : s->host.prims) users[p.mesh]++;
@@ -242,10 +256,7 @@ skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scen
     prims[i].pad[0] = prims[i].pad[1] = 0;
     if (!prims[i].identity) s->needs_fix = true;
   }
-  if (host_only) {
-    *out = s.release();
-    return SKW_OK;
-  }
+  if (host_only) return SKW_OK;
   SKW_HIP(hipGetDevice(&s->device), "hipGetDevice");
   SKW_HIP(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device), "hipDeviceGetAttribute");
   std::vector<DevMesh> meshes(s->host.meshes.size());
@@ -278,6 +289,22 @@ skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scen
   SKW_HIP(upload(&s->d_mesh_normals, mn), "upload mesh normals");
   SKW_HIP(upload(&s->d_mesh_tris, mt), "upload mesh triangles");
   s->on_device = true;
+  return SKW_OK;
+}
+}  // namespace skw
+
+extern "C" {
+
+skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scene **out) {
+  if (!desc || !out) return fail(SKW_INVALID_ARG, "desc and out must not be NULL");
+  *out = nullptr;
+  bool host_only = (flags & SKW_SCENE_HOST_ONLY) != 0;
+  if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
+  std::unique_ptr<skw_scene> s(new skw_scene());
+  std::string err;
+  if (!build_triangle_tree(*desc, s->host, err)) return fail(SKW_INVALID_ARG, err);
+  skw_status st = finish_scene(s, host_only);
+  if (st != SKW_OK) return st;
   *out = s.release();
   return SKW_OK;
 }
@@ -367,17 +394,12 @@ skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weig
   if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
   std::unique_ptr<skw_photon_map> m(new skw_photon_map());
   build_photon_tree(n, pos, weight, dir, normal, m->host);
-  m->n = n;
-  m->max_depth = m->host.max_depth;
   if (!host_only) {
-    SKW_HIP(hipGetDevice(&m->device), "hipGetDevice");
-    SKW_HIP(upload(&m->d_nodes, m->host.nodes), "upload kd nodes");
-    SKW_HIP(upload(&m->d_photons, m->host.photons), "upload photons");
-    SKW_HIP(upload(&m->d_psplit, m->host.psplit), "upload parent splits");
-    SKW_HIP(upload(&m->d_perm, m->host.perm), "upload permutation");
-    std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
-    SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
-    m->on_device = true;
+    skw_status st = finish_photon_map(m.get());
+    if (st != SKW_OK) return st;
+  } else {
+    m->n = n;
+    m->max_depth = m->host.max_depth;
   }
   *out = m.release();
   return SKW_OK;
@@ -385,6 +407,13 @@ skw_status skw_photon_map_create(uint64_t n, const float *pos, const float *weig
 
 skw_status skw_photon_map_destroy(skw_photon_map *map) {
   delete map;
+  return SKW_OK;
+}
+
+skw_status skw_photon_map_info(const skw_photon_map *map, uint64_t *n, uint32_t *max_depth) {
+  if (!map) return fail(SKW_INVALID_ARG, "map must not be NULL");
+  if (n) *n = map->n;
+  if (max_depth) *max_depth = map->max_depth;
   return SKW_OK;
 }
 

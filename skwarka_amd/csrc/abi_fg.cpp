@@ -31,18 +31,6 @@ namespace skw {
 void dl_tables(const skw_dl_scene *dl, const DevMat **mats, const double **rad);
 }  // namespace skw
 
-struct skw_irradiance_map {
-  skw_photon_map *map = nullptr;  // tree over the irradiance photons (weight: external irradiance)
-  std::vector<float> pos, e_ext, e_int;
-  std::vector<uint16_t> normal;
-  double max_dist = std::numeric_limits<double>::infinity();
-  float *d_e_ext = nullptr, *d_e_int = nullptr;  // creation order
-  ~skw_irradiance_map() {
-    if (map) skw_photon_map_destroy(map);
-    if (d_e_ext) (void)hipFree(d_e_ext);
-    if (d_e_int) (void)hipFree(d_e_int);
-  }
-};
 
 namespace {
 
@@ -85,11 +73,18 @@ struct Block {
   }
 };
 
+}  // namespace
+
+namespace skw {
 skw_status upload_irradiance(skw_irradiance_map *m) {
   SKW_HIP(upload(&m->d_e_ext, m->e_ext), "upload irradiance");
   SKW_HIP(upload(&m->d_e_int, m->e_int), "upload irradiance");
   return SKW_OK;
 }
+
+}  // namespace skw
+
+namespace {
 
 skw_status build_irr_tree(skw_irradiance_map *m) {
   const uint64_t n = m->normal.size();

@@ -5,6 +5,7 @@
 
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -89,6 +90,19 @@ struct skw_photon_map {
   ~skw_photon_map();
 };
 
+struct skw_irradiance_map {
+  skw_photon_map *map = nullptr;  // tree over the irradiance photons (weight: external irradiance)
+  std::vector<float> pos, e_ext, e_int;
+  std::vector<uint16_t> normal;
+  double max_dist = __builtin_inf();
+  float *d_e_ext = nullptr, *d_e_int = nullptr;  // creation order
+  ~skw_irradiance_map() {
+    if (map) skw_photon_map_destroy(map);
+    if (d_e_ext) (void)hipFree(d_e_ext);
+    if (d_e_int) (void)hipFree(d_e_int);
+  }
+};
+
 namespace skw {
 // Enqueues closest-hit (any = false) or occlusion (any = true) for n rays; when n_dev is set the
 // count is read on the device and n only bounds the launch size. fix: rewrite b1/b2 of hits on
@@ -102,6 +116,11 @@ skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, 
 // attached (heap: nq * k slots, slot-major; stack: only when the tree is deeper than 64).
 GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                        uint32_t k, double r, uint64_t paths, double rho, int filter);
+
+// Device tables of a scene / photon map whose host layout is complete (built or loaded).
+skw_status finish_scene(std::unique_ptr<skw_scene> &s, bool host_only);
+skw_status finish_photon_map(skw_photon_map *m);
+skw_status upload_irradiance(skw_irradiance_map *m);  // device copies of the irradiance arrays
 
 // The map's tree in host memory, downloaded on first use for device-built maps.
 const KdLayout &photon_map_host(const skw_photon_map *map);

@@ -1,0 +1,85 @@
+"""Device round trips of the on-disk formats: a loaded scene gives the same hits, a loaded photon
+map (device-built before saving) the same gathers, a loaded irradiance map the same final gather,
+all bit for bit."""
+import numpy as np
+import pytest
+
+import skwarka_amd as sk
+from skwarka_amd import workloads as wl
+from tests import scenes
+from tests.conftest import empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _hits(torch, sc, rays):
+    n = len(rays)
+    dr = to_dev(torch, rays)
+    dh = empty_dev(torch, n * 32)
+    sc.closest_hit(dr.data_ptr(), n, dh.data_ptr())
+    torch.cuda.synchronize()
+    return from_dev(dh, np.uint8, n * 32)
+
+
+def test_loaded_scene_gives_the_same_hits(torch_cuda, tmp_path):
+    print("case: scene save / load, c0 and instanced tetrahedra", flush=True)
+    meshes, prims, _ = scenes.instanced_tetrahedra(30)
+    rng = np.random.default_rng(4)
+    for ms, pr, rays in ((wl.scene_c0(), None, wl.rays_c0(128, 128)),
+                         (meshes, prims, scenes.random_rays(20000, (-10, -10, -10), (110, 110, 110), 5))):
+        sc = sk.Scene(ms, prims=pr)
+        path = str(tmp_path / "s.skw")
+        sc.save(path)
+        back = sk.Scene.load(path)
+        assert np.array_equal(_hits(torch_cuda, sc, rays), _hits(torch_cuda, back, rays))
+
+
+def test_loaded_photon_map_gathers_the_same(torch_cuda, tmp_path):
+    print("case: device-built photon map save / load, gathers", flush=True)
+    mesh = wl.merge(wl.scene_c0())
+    ph = wl.photons_on_mesh(mesh, 200000, seed=8)
+    dp = to_dev(torch_cuda, ph["pos"])
+    dw = to_dev(torch_cuda, ph["weight"])
+    dd, dn = to_dev(torch_cuda, ph["dir"]), to_dev(torch_cuda, ph["normal"])
+    pm = sk.PhotonMap.build_on_device(len(ph["pos"]), dp.data_ptr(), dw.data_ptr(), dd.data_ptr(), dn.data_ptr())
+    path = str(tmp_path / "p.skw")
+    pm.save(path)
+    back = sk.PhotonMap.load(path)
+    assert back.n == pm.n
+    q = ph["pos"][::40].astype(np.float64)
+    qn = wl.cdir_table()[ph["normal"][::40]]
+    nq = len(q)
+    dq, dqn = to_dev(torch_cuda, q), to_dev(torch_cuda, qn)
+    out = []
+    for m in (pm, back):
+        ee, ei = empty_dev(torch_cuda, nq * 12), empty_dev(torch_cuda, nq * 12)
+        m.irradiance(dq.data_ptr(), dqn.data_ptr(), nq, 64, 0.05, 1, 0.5, ee.data_ptr(), ei.data_ptr())
+        torch_cuda.cuda.synchronize()
+        out.append((from_dev(ee, np.uint32, nq * 3), from_dev(ei, np.uint32, nq * 3)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_loaded_irradiance_map_gives_the_same_final_gather(torch_cuda, tmp_path):
+    print("case: irradiance map save / load, final gather", flush=True)
+    s = wl.scene_room_small()
+    mesh = wl.merge(s["meshes"])
+    sets = wl.photon_maps(mesh, 10000, 30000, 0, seed=12)
+    maps = [sk.PhotonMap(p["pos"], p["weight"], p["dir"], p["normal"]) for p in sets[:2]]
+    area = wl.scene_area(s["meshes"])
+    irr = sk.IrradianceMap(direct=maps[0], indirect=maps[1], direct_paths=1000, indirect_paths=2000, scene_area=area)
+    path = str(tmp_path / "i.skw")
+    irr.save(path)
+    back = sk.IrradianceMap.load(path)
+    assert back.n == irr.n and back.max_lookup_dist == irr.max_lookup_dist
+    for x, y in zip(irr.export(), back.export()):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    cam, _ = sk.camera(*s["camera"], 32, 32)
+    res = []
+    for m in (irr, back):
+        d = empty_dev(torch_cuda, 32 * 32 * 24)
+        gdl.final_gather(m, maps[1], area, cam, 32, 32, 1, 9, None, d.data_ptr(), seed=3)
+        torch_cuda.cuda.synchronize()
+        res.append(from_dev(d, np.uint64, 32 * 32 * 3))
+    assert np.array_equal(res[0], res[1])
