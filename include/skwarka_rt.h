@@ -391,6 +391,26 @@ skw_status skw_photon_map_load(const char *path, uint32_t flags, skw_photon_map 
 skw_status skw_irradiance_map_save(const skw_irradiance_map *map, const char *path);
 skw_status skw_irradiance_map_load(const char *path, skw_irradiance_map **out);
 
+/* ---------------------------------------------------------------- PBRT scene import
+ * Replaces PbrtSceneImporter::Import (Source/RayLibs/CADImport/SceneImporters/PbrtSceneImporter.h:
+ * 31-40, PbrtSceneImporter_main.cpp, PbrtSceneImporter_hooks.cpp) for the part of a pbrt-v2 file
+ * the traced path uses: trianglemesh shapes under the transform commands, attribute / transform
+ * blocks, ReverseOrientation, matte and plastic materials (also named), diffuse area lights,
+ * object instancing, Include, the perspective camera and the film resolution. Anything else is
+ * skipped with a warning. Syntax errors (unterminated strings or lists, missing arguments, bad
+ * numbers, unreadable files) fail with SKW_INVALID_ARG. */
+typedef struct skw_pbrt_scene skw_pbrt_scene;
+
+skw_status skw_pbrt_load(const char *path, skw_pbrt_scene **out);
+/* Descriptions for skw_scene_create and skw_dl_scene_create (every primitive is its own material
+ * group), the camera and the film size; any output may be NULL. The arrays stay owned by the
+ * import and live until skw_pbrt_destroy. Fails when `cam` is asked for and no Camera was read. */
+skw_status skw_pbrt_descs(const skw_pbrt_scene *scene, skw_scene_desc *desc, skw_dl_desc *dl, skw_camera *cam,
+                          uint32_t *width, uint32_t *height);
+skw_status skw_pbrt_warnings(const skw_pbrt_scene *scene, uint32_t *count);
+skw_status skw_pbrt_warning(const skw_pbrt_scene *scene, uint32_t i, const char **message);
+skw_status skw_pbrt_destroy(skw_pbrt_scene *scene);
+
 /* ---------------------------------------------------------------- batched scenes
  * Many scenes in one persistent launch: every scene's camera rays (pixel-jittered as in
  * skw_dl_render, sample s of pixel p has index p*spp + s) are traced to their closest hit by one

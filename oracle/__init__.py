@@ -111,7 +111,9 @@ def threads():
 class Accel:
     """TriangleAccelerator restatement over the same mesh/primitive description as the product."""
 
-    def __init__(self, meshes, prims=None):
+    def __init__(self, meshes, prims=None, prim_xform=None):
+        """prim_xform: optional (n_prims, 32) matrices and inverses as skw_scene_desc takes them (the
+        inverses of `prims` are otherwise numpy's)."""
         from skwarka_amd import mesh_arrays
         L = lib()
         if prims is None:
@@ -120,7 +122,9 @@ class Accel:
         self._keep = a
         pm = np.array([p[0] for p in prims], np.uint32)
         xf = None
-        if any(p[1] is not None for p in prims):
+        if prim_xform is not None:
+            xf = np.ascontiguousarray(prim_xform, np.float64).reshape(len(prims), 32)
+        elif any(p[1] is not None for p in prims):
             xf = np.zeros((len(prims), 32), np.float64)
             for i, (_, m) in enumerate(prims):
                 m = np.eye(4) if m is None else np.asarray(m, np.float64)

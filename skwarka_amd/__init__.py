@@ -39,7 +39,8 @@ EXPORTS = [
     "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
     "skw_photon_set_destroy", "skw_photon_map_build",
     "skw_scene_save", "skw_scene_load", "skw_photon_map_save", "skw_photon_map_load", "skw_irradiance_map_save",
-    "skw_irradiance_map_load",
+    "skw_irradiance_map_load", "skw_pbrt_load", "skw_pbrt_descs", "skw_pbrt_warnings", "skw_pbrt_warning",
+    "skw_pbrt_destroy",
 ]
 
 _P = ctypes.c_void_p
@@ -186,6 +187,12 @@ def lib():
         "skw_irradiance_map_save": ([_P, ctypes.c_char_p], ctypes.c_int),
         "skw_irradiance_map_load": ([ctypes.c_char_p, ctypes.POINTER(_P)], ctypes.c_int),
         "skw_photon_set_destroy": ([_P], ctypes.c_int),
+        "skw_pbrt_load": ([ctypes.c_char_p, ctypes.POINTER(_P)], ctypes.c_int),
+        "skw_pbrt_descs": ([_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(DLDesc), ctypes.POINTER(Camera),
+                            ctypes.POINTER(_U32), ctypes.POINTER(_U32)], ctypes.c_int),
+        "skw_pbrt_warnings": ([_P, ctypes.POINTER(_U32)], ctypes.c_int),
+        "skw_pbrt_warning": ([_P, _U32, ctypes.POINTER(ctypes.c_char_p)], ctypes.c_int),
+        "skw_pbrt_destroy": ([_P], ctypes.c_int),
         "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
                                  _P, _P, _P, _P], ctypes.c_int),
     }
@@ -276,6 +283,16 @@ class Scene:
         check(L.skw_scene_create(ctypes.byref(desc), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
               "skw_scene_create")
         self.handle = h
+
+    @classmethod
+    def from_desc(cls, desc, host_only=False):
+        """A scene from a filled SceneDesc whose arrays the caller keeps alive for the call."""
+        h = _P()
+        check(lib().skw_scene_create(ctypes.byref(desc), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
+              "skw_scene_create")
+        sc = cls.__new__(cls)
+        sc.handle = h
+        return sc
 
     def save(self, path):
         """skw_scene_save: the built scene (tree, triangles, meshes, primitives) to a file."""
@@ -531,6 +548,98 @@ class PhotonSet:
             self.close()
         except Exception:
             pass
+
+
+def _np_view(ptr, dtype, n):
+    """Copy of n items of dtype at a host address (empty when n is 0)."""
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype)
+    buf = (ctypes.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype).copy()
+
+
+class PbrtScene:
+    """A pbrt-v2 scene file read by skw_pbrt_load (the subset the traced path uses; everything else
+    is listed in `warnings`). scene() / dl_scene() build the handles straight from the import's
+    descriptions; arrays() returns numpy copies of them."""
+
+    def __init__(self, path):
+        h = _P()
+        check(lib().skw_pbrt_load(os.fsencode(path), ctypes.byref(h)), "skw_pbrt_load")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().skw_pbrt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def warnings(self):
+        n = _U32()
+        check(lib().skw_pbrt_warnings(self.handle, ctypes.byref(n)), "skw_pbrt_warnings")
+        out = []
+        for i in range(n.value):
+            msg = ctypes.c_char_p()
+            check(lib().skw_pbrt_warning(self.handle, i, ctypes.byref(msg)), "skw_pbrt_warning")
+            out.append(msg.value.decode(errors="replace"))
+        return out
+
+    def _descs(self, want_camera=False):
+        sd, dd, cam, w, h = SceneDesc(), DLDesc(), Camera(), _U32(), _U32()
+        check(lib().skw_pbrt_descs(self.handle, ctypes.byref(sd), ctypes.byref(dd),
+                                   ctypes.byref(cam) if want_camera else None, ctypes.byref(w), ctypes.byref(h)),
+              "skw_pbrt_descs")
+        return sd, dd, cam, w.value, h.value
+
+    def camera(self):
+        """(Camera, width, height); raises SkwError when the file has no Camera."""
+        _, _, cam, w, h = self._descs(True)
+        return cam, w, h
+
+    def scene(self, host_only=False):
+        sd, _, _, _, _ = self._descs()
+        return Scene.from_desc(sd, host_only)
+
+    def dl_scene(self, scene):
+        a = self.arrays()
+        return DLScene(scene, a["mat"], a["kd"], a["ks"], a["rough"], a["light_prim"], a["light_radiance"])
+
+    def arrays(self):
+        """meshes (list of dicts as Scene takes), prims [(mesh, 4x4)], prim_xform (n, 32), mat, kd, ks,
+        rough, light_prim, light_radiance, width, height."""
+        sd, dd, _, w, h = self._descs()
+        nm, npr = sd.n_meshes, sd.n_prims
+        vo = _np_view(sd.mesh_vert_offset, np.uint64, nm + 1)
+        to = _np_view(sd.mesh_tri_offset, np.uint64, nm + 1)
+        nv, nt = int(vo[-1]) if nm else 0, int(to[-1]) if nm else 0
+        verts = _np_view(sd.verts, np.float32, 3 * nv).reshape(-1, 3)
+        normals = _np_view(sd.normals, np.float32, 3 * nv).reshape(-1, 3) if sd.normals else None
+        tris = _np_view(sd.tris, np.uint32, 3 * nt).reshape(-1, 3)
+        flags = _np_view(sd.mesh_flags, np.uint8, nm)
+        meshes = []
+        for i in range(nm):
+            m = {"verts": verts[vo[i]:vo[i + 1]], "tris": tris[to[i]:to[i + 1]],
+                 "invert_normals": bool(flags[i] & MESH_INVERT_NORMALS)}
+            if flags[i] & MESH_SHADING_NORMALS:
+                m["normals"] = normals[vo[i]:vo[i + 1]]
+            meshes.append(m)
+        pm = _np_view(sd.prim_mesh, np.uint32, npr)
+        xf = _np_view(sd.prim_xform, np.float64, 32 * npr).reshape(-1, 32)
+        nl = dd.n_lights
+        return {"meshes": meshes, "prims": [(int(pm[i]), xf[i, :16].reshape(4, 4)) for i in range(npr)],
+                "prim_xform": xf, "mat": _np_view(dd.mat_type, np.uint8, npr),
+                "kd": _np_view(dd.kd, np.float64, 3 * npr).reshape(-1, 3),
+                "ks": _np_view(dd.ks, np.float64, 3 * npr).reshape(-1, 3),
+                "rough": _np_view(dd.roughness, np.float64, npr),
+                "light_prim": _np_view(dd.light_prim, np.uint32, nl),
+                "light_radiance": _np_view(dd.light_radiance, np.float64, 3 * nl).reshape(-1, 3),
+                "width": w, "height": h}
 
 
 def camera(origin, target_dir, up, x_fov_deg, width, height):

@@ -23,6 +23,7 @@ SOURCES = [
     "abi_shoot.cpp",
     "abi_kd.cpp",
     "io.cpp",
+    "pbrt_import.cpp",
     "host/builders.cpp",
     "kernels/traverse.hip",
     "kernels/geometry.hip",
