@@ -554,7 +554,7 @@ def main():
     c6 = None
     if run("c6"):
         log("c6: photon shooting, 4M paths from the c2 scene's lights, device kd build")
-        c6 = bench_c6(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
+        c6 = bench_c6(torch, sk, wl, 10, 2, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
         log("c6: %.1f ms per 4M paths, %.1f M paths/s, photons %s, kd build %d photons: device %.1f ms, host %.1f ms"
             % (c6["ms"], c6["value"], c6["counts"], c6["kd_n"], c6["kd_device_ms"], c6["kd_host_ms"]))
     cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = None

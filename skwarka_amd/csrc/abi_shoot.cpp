@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -30,48 +31,25 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kDblEps = 2.220446049250313080847e-16;
 const double kLumW[3] = {0x1.b38293da589a6p-5, 0x1.2b993cc221566p-1, 0x1.725d3400723f6p-2};
 
-// A growable device array of photon records (SoA).
+// A map's photon records (SoA) owned by a photon set, in the library's device memory pool (a
+// set's arrays are reused by the next set instead of being mapped again).
 struct PhotonArrays {
   float *pos = nullptr, *w = nullptr;
   uint32_t *codes = nullptr;
-  uint64_t n = 0, cap = 0;
+  uint64_t n = 0;
   void release() {
-    if (pos) (void)hipFree(pos);
-    if (w) (void)hipFree(w);
-    if (codes) (void)hipFree(codes);
+    for (void *p : {(void *)pos, (void *)w, (void *)codes})
+      if (p) pool_free(p, nullptr);
     pos = w = nullptr;
     codes = nullptr;
-    n = cap = 0;
+    n = 0;
   }
-  // Room for `need` records, keeping the first n.
-  hipError_t reserve(uint64_t need, hipStream_t st, uint32_t **key = nullptr, uint32_t **key_old = nullptr) {
-    if (need <= cap) return hipSuccess;
-    uint64_t c = std::max<uint64_t>(need, cap * 2);
-    float *np = nullptr, *nw = nullptr;
-    uint32_t *nc = nullptr;
-    hipError_t e = hipMalloc((void **)&np, c * 3 * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&nw, c * 3 * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&nc, c * sizeof(uint32_t));
-    if (e == hipSuccess && n > 0) {
-      e = hipMemcpyAsync(np, pos, n * 3 * sizeof(float), hipMemcpyDeviceToDevice, st);
-      if (e == hipSuccess) e = hipMemcpyAsync(nw, w, n * 3 * sizeof(float), hipMemcpyDeviceToDevice, st);
-      if (e == hipSuccess) e = hipMemcpyAsync(nc, codes, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-      if (np) (void)hipFree(np);
-      if (nw) (void)hipFree(nw);
-      if (nc) (void)hipFree(nc);
-      return e;
-    }
-    const uint64_t keep = n;
+  hipError_t alloc(uint64_t count, hipStream_t st) {
     release();
-    pos = np;
-    w = nw;
-    codes = nc;
-    n = keep;
-    cap = c;
-    return hipSuccess;
+    hipError_t e = pool_alloc((void **)&pos, std::max<uint64_t>(count, 1) * 3 * sizeof(float), st);
+    if (e == hipSuccess) e = pool_alloc((void **)&w, std::max<uint64_t>(count, 1) * 3 * sizeof(float), st);
+    if (e == hipSuccess) e = pool_alloc((void **)&codes, std::max<uint64_t>(count, 1) * sizeof(uint32_t), st);
+    return e;
   }
 };
 
@@ -81,49 +59,260 @@ struct skw_photon_set {
   PhotonArrays m[3];
   uint64_t paths[3] = {0, 0, 0};
   uint64_t total_paths = 0;
+  int device = 0;
   ~skw_photon_set() {
+    // as hipFree would: no kernel on the device may still read the arrays
+    int cur = 0;
+    const bool ok = hipGetDevice(&cur) == hipSuccess && hipSetDevice(device) == hipSuccess;
+    if (ok) (void)hipDeviceSynchronize();
     for (auto &a : m) a.release();
+    if (ok) (void)hipSetDevice(cur);
   }
 };
 
 namespace {
 
-// The round's photon buffers of one map: records plus their path keys, growable.
-struct RoundPhotons {
-  PhotonArrays rec;
-  uint32_t *key = nullptr;
-  ~RoundPhotons() {
-    rec.release();
-    if (key) (void)hipFree(key);
-  }
-  hipError_t reserve(uint64_t need, hipStream_t st) {
-    if (need <= rec.cap) return hipSuccess;
-    const uint64_t old_n = rec.n;
-    hipError_t e = rec.reserve(need, st);
+// A device buffer the library keeps between calls: it only grows (keeping its first `keep`
+// items), so repeated calls of one size allocate and free nothing.
+template <class T>
+struct Keep {
+  T *p = nullptr;
+  uint64_t cap = 0;
+  hipError_t grow(uint64_t need, uint64_t keep, hipStream_t s) {
+    if (p && need <= cap) return hipSuccess;
+    const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(need, cap * 2), 1);
+    T *np = nullptr;
+    hipError_t e = hipMalloc((void **)&np, c * sizeof(T));
     if (e != hipSuccess) return e;
-    uint32_t *nk = nullptr;
-    e = hipMalloc((void **)&nk, rec.cap * sizeof(uint32_t));
-    if (e == hipSuccess && old_n > 0 && key)
-      e = hipMemcpyAsync(nk, key, old_n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    keep = std::min(keep, cap);
+    if (keep > 0) e = hipMemcpyAsync(np, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-      if (nk) (void)hipFree(nk);
+      (void)hipFree(np);
       return e;
     }
-    if (key) (void)hipFree(key);
-    key = nk;
+    if (p) (void)hipFree(p);
+    p = np;
+    cap = c;
     return hipSuccess;
   }
 };
 
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+// Photon records (SoA) of one map, with the path keys while they are in deposit order.
+struct Records {
+  Keep<float> pos, w;  // 3 per record
+  Keep<uint32_t> codes, key;
+  hipError_t reserve(uint64_t need, uint64_t keep, hipStream_t s, bool with_key) {
+    hipError_t e = pos.grow(3 * need, 3 * keep, s);
+    if (e == hipSuccess) e = w.grow(3 * need, 3 * keep, s);
+    if (e == hipSuccess) e = codes.grow(need, keep, s);
+    if (e == hipSuccess && with_key) e = key.grow(need, keep, s);
+    return e;
   }
-  hipError_t alloc(uint64_t n) { return hipMalloc((void **)&p, std::max<uint64_t>(n, 1) * sizeof(T)); }
 };
+
+// The working memory of photon shooting on one device, kept for the life of the process; calls
+// on one device take turns on it.
+struct ShootWorkspace {
+  std::mutex mu;
+  Keep<double> cdf;
+  Keep<skw_ray> rays[2];
+  Keep<ShootPath> state[2];
+  Keep<skw_hit> hits;
+  Keep<uint64_t> counts;
+  Keep<uint32_t> iota, order, keys_sorted;
+  Keep<uint8_t> sort_tmp;
+  Records round[3];  // the current round's photons of each map, in deposit order
+  Records out[3];    // every round's photons of each map, in (path, intersection) order
+  uint64_t *h = nullptr;  // two pinned copies of the five counters
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+constexpr int kWorkspaceDevices = 64;
+
+hipError_t shoot_workspace(ShootWorkspace **out) {
+  static ShootWorkspace ws[kWorkspaceDevices];
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kWorkspaceDevices) return hipErrorInvalidDevice;
+  *out = &ws[dev];
+  return hipSuccess;
+}
+
+// The shooting loop; all device work is on `st`, whose pool buffers are released in stream order.
+hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const skw_shoot_params *prm,
+                        const std::vector<DevLight> &lights, const DevLight *d_lights, const DevLightTri *d_ltris,
+                        const DevMat *mats, skw_photon_set *set, hipStream_t st, skw_status *rs) {
+  const uint32_t nl = (uint32_t)lights.size();
+  // _GetLightsPowerCDF: luminance of (pi * area) * radiance, accumulated, divided by the total.
+  std::vector<double> cdf(nl, 0.0);
+  for (uint32_t l = 0; l < nl; ++l) {
+    double y = 0.0;
+    for (int c = 0; c < 3; ++c) y += kLumW[c] * ((kPi * lights[l].area) * lights[l].radiance[c]);
+    cdf[l] = y;
+  }
+  for (uint32_t i = 1; i < nl; ++i) cdf[i] += cdf[i - 1];
+  const double total = cdf[nl - 1];
+  if (total > kDblEps)
+    for (uint32_t i = 0; i < nl; ++i) cdf[i] /= total;
+  else
+    cdf[nl - 1] = 1.0;
+
+  const uint64_t R = std::min<uint64_t>(prm->paths_per_round, prm->max_paths);
+  ShootWorkspace *ws = nullptr;
+  hipError_t e = shoot_workspace(&ws);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  if (!ws->h) {
+    e = hipHostMalloc((void **)&ws->h, 10 * sizeof(uint64_t), hipHostMallocDefault);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ws->ev[i], hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  e = ws->cdf.grow(nl, 0, st);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    e = ws->rays[i].grow(R, 0, st);
+    if (e == hipSuccess) e = ws->state[i].grow(R, 0, st);
+  }
+  if (e == hipSuccess) e = ws->hits.grow(R, 0, st);
+  if (e == hipSuccess) e = ws->counts.grow(5, 0, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ws->cdf.p, cdf.data(), nl * sizeof(double), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  uint64_t *h_counts[2] = {ws->h, ws->h + 5};
+  hipEvent_t *ev = ws->ev;
+  Records *round = ws->round;
+  Keep<uint64_t> &counts = ws->counts;
+  int key_bits = 1;
+  while ((1ull << key_bits) < R) ++key_bits;
+
+  ShootArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.geom = scene->geom_args();
+  a.mats = mats;
+  a.lights = d_lights;
+  a.ltris = d_ltris;
+  a.light_cdf = ws->cdf.p;
+  a.n_lights = nl;
+  a.seed = prm->seed;
+  for (int i = 0; i < 2; ++i) {
+    a.rays[i] = ws->rays[i].p;
+    a.state[i] = ws->state[i].p;
+  }
+  a.hits = ws->hits.p;
+  a.counts = counts.p;
+
+  uint64_t have[3] = {0, 0, 0};  // photons per map so far
+  while (*rs == SKW_OK && e == hipSuccess) {
+    bool done[3];
+    for (int j = 0; j < 3; ++j) done[j] = have[j] >= prm->max_photons[j];
+    if (set->total_paths >= prm->max_paths || (done[0] && done[1] && done[2])) break;
+    bool stuck = set->total_paths > 10000000ull;
+    for (int j = 0; j < 3; ++j) stuck = stuck && (done[j] || have[j] == 0);
+    if (stuck) break;
+    const uint64_t first = set->total_paths;
+    const uint64_t n = std::min<uint64_t>(R, prm->max_paths - first);
+    set->total_paths += n;
+    a.first = first;
+    a.n = n;
+    for (int j = 0; j < 3; ++j) a.done[j] = done[j] ? 1u : 0u;
+    // direct photons: at most one per path
+    if (!done[1] && (e = round[1].reserve(n, 0, st, true)) != hipSuccess) break;
+    if ((e = hipMemsetAsync(counts.p, 0, 5 * sizeof(uint64_t), st)) != hipSuccess) break;
+    a.cur = 0;
+    for (int j = 0; j < 3; ++j) {
+      a.ph_pos[j] = round[j].pos.p;
+      a.ph_w[j] = round[j].w.p;
+      a.ph_codes[j] = round[j].codes.p;
+      a.ph_key[j] = round[j].key.p;
+    }
+    if ((e = launch_shoot_start(a, st)) != hipSuccess) break;
+    // The bounces are enqueued back to back: the host reads the counters of step k (k = 0 the
+    // start, k >= 1 bounce k) from a pinned copy while later bounces run, two steps late. Active
+    // paths never increase and a path deposits at most one photon per bounce, so from exact counts
+    // at step s (A paths active for bounce s + 1, P[j] records) bounce m runs a grid for A paths and
+    // holds at most P[j] + (m - s) A records; the kernels read the exact path count on the device.
+    uint64_t A = n, P[3] = {0, 0, 0};
+    int64_t s_known = -1;
+    if ((e = hipMemcpyAsync(h_counts[0], counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      break;
+    if ((e = hipEventRecord(ev[0], st)) != hipSuccess) break;
+    for (int64_t m = 1; e == hipSuccess && *rs == SKW_OK; ++m) {
+      if (m >= 2) {
+        if ((e = hipEventSynchronize(ev[m & 1])) != hipSuccess) break;
+        const uint64_t *c = h_counts[m & 1];  // step m - 2
+        A = c[(m - 2) & 1];
+        for (int j = 0; j < 3; ++j) P[j] = c[2 + j];
+        s_known = m - 2;
+      }
+      if (A == 0) break;  // no path reaches bounce s_known + 1: every later step is empty
+      const uint64_t steps = (uint64_t)(m - s_known);
+      for (int j = 0; j < 3 && e == hipSuccess; ++j)
+        if (!done[j] && j != 1) e = round[j].reserve(P[j] + steps * A, P[j] + (steps - 1) * A, st, true);
+      if (e != hipSuccess) break;
+      for (int j = 0; j < 3; ++j) {
+        a.ph_pos[j] = round[j].pos.p;
+        a.ph_w[j] = round[j].w.p;
+        a.ph_codes[j] = round[j].codes.p;
+        a.ph_key[j] = round[j].key.p;
+      }
+      *rs = enqueue_trace(scene, ws->rays[a.cur].p, A, counts.p + a.cur, ws->hits.p, nullptr, st, false, nullptr,
+                          nullptr, false);
+      if (*rs != SKW_OK) break;
+      if ((e = hipMemsetAsync(counts.p + (a.cur ^ 1u), 0, sizeof(uint64_t), st)) != hipSuccess) break;
+      if ((e = launch_shoot_bounce(a, A, st)) != hipSuccess) break;
+      a.cur ^= 1u;
+      if ((e = hipMemcpyAsync(h_counts[m & 1], counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st)) !=
+          hipSuccess)
+        break;
+      e = hipEventRecord(ev[m & 1], st);
+    }
+    // the copies still in flight write the pinned counters: let them land first
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (*rs != SKW_OK || e != hipSuccess) break;
+    // the round's photons in (path, intersection) order
+    for (int j = 0; j < 3 && e == hipSuccess; ++j) {
+      if (done[j]) continue;
+      set->paths[j] += n;
+      const uint64_t k = P[j];
+      if (k == 0) continue;
+      size_t sort_bytes = 0;
+      if ((e = sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, k, 0, key_bits, st)) !=
+          hipSuccess)
+        break;
+      if ((e = ws->sort_tmp.grow(sort_bytes, 0, st)) != hipSuccess) break;
+      if ((e = ws->iota.grow(k, 0, st)) != hipSuccess) break;
+      if ((e = ws->order.grow(k, 0, st)) != hipSuccess) break;
+      if ((e = ws->keys_sorted.grow(k, 0, st)) != hipSuccess) break;
+      if ((e = launch_iota_u32(ws->iota.p, k, st)) != hipSuccess) break;
+      if ((e = sort_pairs_u32(ws->sort_tmp.p, sort_bytes, round[j].key.p, ws->keys_sorted.p, ws->iota.p, ws->order.p,
+                              k, 0, key_bits, st)) != hipSuccess)
+        break;
+      Records &o = ws->out[j];
+      if ((e = o.reserve(have[j] + k, have[j], st, false)) != hipSuccess) break;
+      e = launch_photon_gather_records(round[j].pos.p, round[j].w.p, round[j].codes.p, ws->order.p, k,
+                                       o.pos.p + 3 * have[j], o.w.p + 3 * have[j], o.codes.p + have[j], st);
+      have[j] += k;
+    }
+  }
+  if (*rs != SKW_OK || e != hipSuccess) {
+    (void)hipStreamSynchronize(st);  // no copy may still target the pinned counters
+    return e;
+  }
+  // each map's rounds, concatenated into arrays of the exact size
+  for (int j = 0; j < 3 && e == hipSuccess; ++j) {
+    if (have[j] == 0) continue;
+    if ((e = set->m[j].alloc(have[j], st)) != hipSuccess) break;
+    const Records &o = ws->out[j];
+    e = hipMemcpyAsync(set->m[j].pos, o.pos.p, 3 * have[j] * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(set->m[j].w, o.w.p, 3 * have[j] * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(set->m[j].codes, o.codes.p, have[j] * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+    set->m[j].n = have[j];
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
 
 }  // namespace
 
@@ -137,6 +326,7 @@ skw_status skw_photon_shoot(const skw_scene *scene, const skw_dl_scene *dl, cons
   if (prm->paths_per_round == 0) return fail(SKW_INVALID_ARG, "paths_per_round must be > 0");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   std::unique_ptr<skw_photon_set> set(new skw_photon_set());
+  SKW_HIP(hipGetDevice(&set->device), "current device");
   const std::vector<DevLight> *lights = nullptr;
   const DevLight *d_lights = nullptr;
   const DevLightTri *d_ltris = nullptr;
@@ -144,156 +334,12 @@ skw_status skw_photon_shoot(const skw_scene *scene, const skw_dl_scene *dl, cons
   const DevMat *mats = nullptr;
   const double *rad = nullptr;
   dl_tables(dl, &mats, &rad);
-  const uint32_t nl = (uint32_t)lights->size();
-  if (nl == 0 || prm->max_paths == 0) {
+  if (lights->empty() || prm->max_paths == 0) {
     *out = set.release();
     return SKW_OK;
   }
-  // _GetLightsPowerCDF: luminance of (pi * area) * radiance, accumulated, divided by the total.
-  std::vector<double> cdf(nl, 0.0);
-  for (uint32_t l = 0; l < nl; ++l) {
-    double y = 0.0;
-    for (int c = 0; c < 3; ++c) y += kLumW[c] * ((kPi * (*lights)[l].area) * (*lights)[l].radiance[c]);
-    cdf[l] = y;
-  }
-  for (uint32_t i = 1; i < nl; ++i) cdf[i] += cdf[i - 1];
-  const double total = cdf[nl - 1];
-  if (total > kDblEps)
-    for (uint32_t i = 0; i < nl; ++i) cdf[i] /= total;
-  else
-    cdf[nl - 1] = 1.0;
-
-  const uint64_t R = std::min<uint64_t>(prm->paths_per_round, prm->max_paths);
-  DevBuf<double> d_cdf;
-  DevBuf<skw_ray> rays[2];
-  DevBuf<ShootPath> state[2];
-  DevBuf<skw_hit> hits;
-  DevBuf<uint64_t> counts;
-  DevBuf<uint32_t> iota, order;
-  DevBuf<uint32_t> keys_sorted;
-  SKW_HIP(d_cdf.alloc(nl), "shooting buffers");
-  for (int i = 0; i < 2; ++i) {
-    SKW_HIP(rays[i].alloc(R), "shooting buffers");
-    SKW_HIP(state[i].alloc(R), "shooting buffers");
-  }
-  SKW_HIP(hits.alloc(R), "shooting buffers");
-  SKW_HIP(counts.alloc(5), "shooting buffers");
-  SKW_HIP(hipMemcpyAsync(d_cdf.p, cdf.data(), nl * sizeof(double), hipMemcpyHostToDevice, st), "upload light CDF");
-  RoundPhotons round[3];
-  void *sort_tmp = nullptr;
-  size_t sort_bytes = 0;
-  uint64_t sort_cap = 0;
-  int key_bits = 1;
-  while ((1ull << key_bits) < R) ++key_bits;
-
-  ShootArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.geom = scene->geom_args();
-  a.mats = mats;
-  a.lights = d_lights;
-  a.ltris = d_ltris;
-  a.light_cdf = d_cdf.p;
-  a.n_lights = nl;
-  a.seed = prm->seed;
-  for (int i = 0; i < 2; ++i) {
-    a.rays[i] = rays[i].p;
-    a.state[i] = state[i].p;
-  }
-  a.hits = hits.p;
-  a.counts = counts.p;
-
   skw_status rs = SKW_OK;
-  hipError_t e = hipSuccess;
-  while (rs == SKW_OK && e == hipSuccess) {
-    bool done[3];
-    for (int j = 0; j < 3; ++j) done[j] = set->m[j].n >= prm->max_photons[j];
-    if (set->total_paths >= prm->max_paths || (done[0] && done[1] && done[2])) break;
-    bool stuck = set->total_paths > 10000000ull;
-    for (int j = 0; j < 3; ++j) stuck = stuck && (done[j] || set->m[j].n == 0);
-    if (stuck) break;
-    const uint64_t first = set->total_paths;
-    const uint64_t n = std::min<uint64_t>(R, prm->max_paths - first);
-    set->total_paths += n;
-    a.first = first;
-    a.n = n;
-    for (int j = 0; j < 3; ++j) {
-      a.done[j] = done[j] ? 1u : 0u;
-      round[j].rec.n = 0;
-    }
-    // direct photons: at most one per path
-    if (!done[1] && (e = round[1].reserve(n, st)) != hipSuccess) break;
-    if ((e = hipMemsetAsync(counts.p, 0, 5 * sizeof(uint64_t), st)) != hipSuccess) break;
-    a.cur = 0;
-    for (int j = 0; j < 3; ++j) {
-      a.ph_pos[j] = round[j].rec.pos;
-      a.ph_w[j] = round[j].rec.w;
-      a.ph_codes[j] = round[j].rec.codes;
-      a.ph_key[j] = round[j].key;
-    }
-    if ((e = launch_shoot_start(a, st)) != hipSuccess) break;
-    uint64_t c[5];
-    while (true) {
-      if ((e = hipMemcpyAsync(c, counts.p, sizeof(c), hipMemcpyDeviceToHost, st)) != hipSuccess) break;
-      if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
-      const uint64_t n_active = c[a.cur];
-      if (n_active == 0) break;
-      // every active path may deposit one photon: room in the maps that take photons
-      for (int j = 0; j < 3 && e == hipSuccess; ++j) {
-        if (done[j] || j == 1) continue;
-        round[j].rec.n = c[2 + j];  // records a reallocation keeps
-        e = round[j].reserve(c[2 + j] + n_active, st);
-      }
-      if (e != hipSuccess) break;
-      for (int j = 0; j < 3; ++j) {
-        a.ph_pos[j] = round[j].rec.pos;
-        a.ph_w[j] = round[j].rec.w;
-        a.ph_codes[j] = round[j].rec.codes;
-        a.ph_key[j] = round[j].key;
-      }
-      rs = enqueue_trace(scene, rays[a.cur].p, n_active, nullptr, hits.p, nullptr, st, false, nullptr, nullptr, false);
-      if (rs != SKW_OK) break;
-      if ((e = hipMemsetAsync(counts.p + (a.cur ^ 1u), 0, sizeof(uint64_t), st)) != hipSuccess) break;
-      if ((e = launch_shoot_bounce(a, n_active, st)) != hipSuccess) break;
-      a.cur ^= 1u;
-    }
-    if (rs != SKW_OK || e != hipSuccess) break;
-    // the round's photons in (path, intersection) order, appended to their maps
-    for (int j = 0; j < 3 && e == hipSuccess; ++j) {
-      if (done[j]) continue;
-      set->paths[j] += n;
-      const uint64_t k = c[2 + j];
-      if (k == 0) continue;
-      if (k > sort_cap) {
-        if (sort_tmp) (void)hipFree(sort_tmp);
-        sort_tmp = nullptr;
-        if (iota.p) (void)hipFree(iota.p);
-        if (order.p) (void)hipFree(order.p);
-        if (keys_sorted.p) (void)hipFree(keys_sorted.p);
-        iota.p = order.p = keys_sorted.p = nullptr;
-        sort_cap = std::max<uint64_t>(k, 2 * sort_cap);
-        sort_bytes = 0;
-        if ((e = sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, sort_cap, 0, key_bits, st)) !=
-            hipSuccess)
-          break;
-        if ((e = hipMalloc(&sort_tmp, std::max<size_t>(sort_bytes, 1))) != hipSuccess) break;
-        if ((e = iota.alloc(sort_cap)) != hipSuccess) break;
-        if ((e = order.alloc(sort_cap)) != hipSuccess) break;
-        if ((e = keys_sorted.alloc(sort_cap)) != hipSuccess) break;
-      }
-      if ((e = launch_iota_u32(iota.p, k, st)) != hipSuccess) break;
-      size_t bytes = sort_bytes;
-      if ((e = sort_pairs_u32(sort_tmp, bytes, round[j].key, keys_sorted.p, iota.p, order.p, k, 0, key_bits, st)) !=
-          hipSuccess)
-        break;
-      PhotonArrays &dst = set->m[j];
-      if ((e = dst.reserve(dst.n + k, st)) != hipSuccess) break;
-      e = launch_photon_gather_records(round[j].rec.pos, round[j].rec.w, round[j].rec.codes, order.p, k,
-                                       dst.pos + 3 * dst.n, dst.w + 3 * dst.n, dst.codes + dst.n, st);
-      dst.n += k;
-    }
-  }
-  if (e == hipSuccess && rs == SKW_OK) e = hipStreamSynchronize(st);
-  if (sort_tmp) (void)hipFree(sort_tmp);
+  const hipError_t e = shoot_rounds(scene, dl, prm, *lights, d_lights, d_ltris, mats, set.get(), st, &rs);
   if (rs != SKW_OK) return rs;
   if (e != hipSuccess) return hip_fail(e, "photon shooting");
   *out = set.release();

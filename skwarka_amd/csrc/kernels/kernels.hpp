@@ -237,7 +237,8 @@ struct ShootArgs {
   uint32_t *ph_key[3];      // path - first (the photons are later ordered by it, stably)
 };
 hipError_t launch_shoot_start(const ShootArgs &a, hipStream_t stream);
-hipError_t launch_shoot_bounce(const ShootArgs &a, uint64_t n_active, hipStream_t stream);
+// Grid for n_bound paths; the kernel reads the exact count of list `cur` from counts[cur].
+hipError_t launch_shoot_bounce(const ShootArgs &a, uint64_t n_bound, hipStream_t stream);
 hipError_t launch_iota_u32(uint32_t *out, uint64_t n, hipStream_t stream);  // out[j] = j
 // out[j] = in[order[j]] for the photon records of one map (n records).
 hipError_t launch_photon_gather_records(const float *pos_in, const float *w_in, const uint32_t *codes_in,

@@ -155,9 +155,10 @@ __global__ __launch_bounds__(kSampleBlock) void k_shoot_start(ShootArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kSampleBlock) void k_shoot_bounce(ShootArgs a, uint64_t n_active) {
+__global__ __launch_bounds__(kSampleBlock) void k_shoot_bounce(ShootArgs a) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t cur = a.cur;
+  const uint64_t n_active = a.counts[cur];  // the grid covers an upper bound of it
   bool alive = false, next = false;
   int map = -1;
   uint32_t codes = 0, key = 0, path = 0, n_isect = 0;
@@ -287,10 +288,10 @@ hipError_t launch_shoot_start(const ShootArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_shoot_bounce(const ShootArgs &a, uint64_t n_active, hipStream_t stream) {
-  if (n_active == 0) return hipSuccess;
+hipError_t launch_shoot_bounce(const ShootArgs &a, uint64_t n_bound, hipStream_t stream) {
+  if (n_bound == 0) return hipSuccess;
   const uint64_t B = kSampleBlock;
-  hipLaunchKernelGGL(k_shoot_bounce, dim3((uint32_t)((n_active + B - 1) / B)), dim3(B), 0, stream, a, n_active);
+  hipLaunchKernelGGL(k_shoot_bounce, dim3((uint32_t)((n_bound + B - 1) / B)), dim3(B), 0, stream, a);
   return hipGetLastError();
 }
 
