@@ -216,17 +216,27 @@ skw_status finish_photon_map(skw_photon_map *m) {
   SKW_HIP(upload(&m->d_photons, m->host.photons), "upload photons");
   SKW_HIP(upload(&m->d_psplit, m->host.psplit), "upload parent splits");
   SKW_HIP(upload(&m->d_perm, m->host.perm), "upload permutation");
-  std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
-  SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
+  SKW_HIP(shared_cdir(&m->d_cdir), "direction table");
+  m->cdir_shared = true;
   m->on_device = true;
   return SKW_OK;
 }
 }  // namespace skw
 
 skw_photon_map::~skw_photon_map() {
-  void *ptrs[] = {d_nodes, d_photons, d_psplit, d_perm, d_cdir};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  if (pooled) {
+    // as hipFree would: no kernel on the device may still read the arrays
+    int cur = 0;
+    const bool ok = hipGetDevice(&cur) == hipSuccess && hipSetDevice(device) == hipSuccess;
+    if (ok) (void)hipDeviceSynchronize();
+    for (void *p : {(void *)d_nodes, (void *)d_photons, (void *)d_psplit, (void *)d_perm})
+      if (p) skw::pool_free(p, nullptr);
+    if (ok) (void)hipSetDevice(cur);
+  } else {
+    for (void *p : {(void *)d_nodes, (void *)d_photons, (void *)d_psplit, (void *)d_perm})
+      if (p) (void)hipFree(p);
+  }
+  if (d_cdir && !cdir_shared) (void)hipFree(d_cdir);
 }
 
 extern "C" {

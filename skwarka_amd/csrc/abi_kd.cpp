@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -170,10 +171,11 @@ skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *
     a.word = reinterpret_cast<uint32_t *>(work + items_bytes);
     a.scr_l = reinterpret_cast<uint32_t *>(work + items_bytes + words_bytes);
     a.scr_r = reinterpret_cast<uint32_t *>(work + items_bytes + 2 * words_bytes);
-    if (e == hipSuccess) e = hipMalloc((void **)&m->d_nodes, n * sizeof(DevKdNode));
-    if (e == hipSuccess) e = hipMalloc((void **)&m->d_photons, n * sizeof(DevPhoton));
-    if (e == hipSuccess) e = hipMalloc((void **)&m->d_perm, n * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void **)&m->d_psplit, n * sizeof(float));
+    m->pooled = true;
+    if (e == hipSuccess) e = pool_alloc((void **)&m->d_nodes, n * sizeof(DevKdNode), st);
+    if (e == hipSuccess) e = pool_alloc((void **)&m->d_photons, n * sizeof(DevPhoton), st);
+    if (e == hipSuccess) e = pool_alloc((void **)&m->d_perm, n * sizeof(uint32_t), st);
+    if (e == hipSuccess) e = pool_alloc((void **)&m->d_psplit, n * sizeof(float), st);
     if (e == hipSuccess) e = launch_kd_init(a, d_pos, st);
     for (const Plan::Level &L : plan.levels) {
       if (e != hipSuccess) break;
@@ -194,11 +196,34 @@ skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *
   m->max_depth = plan.max_depth;
   m->host.max_depth = plan.max_depth;
   m->host_ready = n == 0;
-  std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
-  SKW_HIP(upload(&m->d_cdir, table), "upload direction table");
+  m->pooled = true;
+  SKW_HIP(shared_cdir(&m->d_cdir), "direction table");
+  m->cdir_shared = true;
   m->on_device = true;
   *out = m.release();
   return SKW_OK;
+}
+
+hipError_t shared_cdir(double **out) {
+  static std::mutex mu;
+  static double *tables[64] = {nullptr};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!tables[dev]) {
+    std::vector<double> table(cdir_table(), cdir_table() + 3 * 65536);
+    double *d = nullptr;
+    e = upload(&d, table);
+    if (e != hipSuccess) {
+      if (d) (void)hipFree(d);
+      return e;
+    }
+    tables[dev] = d;  // kept for the life of the process
+  }
+  *out = tables[dev];
+  return hipSuccess;
 }
 
 const KdLayout &photon_map_host(const skw_photon_map *map) {

@@ -87,6 +87,8 @@ struct skw_photon_map {
   float *d_psplit = nullptr;
   uint32_t *d_perm = nullptr;
   double *d_cdir = nullptr;
+  bool pooled = false;      // arrays from the device pool (device builds)
+  bool cdir_shared = false; // d_cdir is the device's shared direction table
   ~skw_photon_map();
 };
 
@@ -130,6 +132,8 @@ skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, co
                            uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream);
 // A photon map whose kd-tree is built on the device from device arrays (positions, weights, and
 // either codes = dir | normal << 16 or the two id arrays; weights / ids may be NULL).
+// The direction table of CompressedDirection on the current device, uploaded once and shared.
+hipError_t shared_cdir(double **out);
 skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *d_weight, const uint32_t *d_codes,
                                    const uint16_t *d_dir, const uint16_t *d_normal, skw_photon_map **out,
                                    hipStream_t st);
