@@ -17,12 +17,22 @@ using namespace skw;
 namespace {
 
 // Height of the subtree of a range of s items (the tree's shape depends on s only).
-uint32_t subtree_height(uint64_t s) {
+uint32_t subtree_height_rec(uint64_t s) {
   if (s <= 1) return 0;
   const uint64_t left = s / 2, right = s - s / 2 - 1;
-  uint32_t h = subtree_height(left);
-  if (right > 0) h = std::max(h, subtree_height(right));
+  uint32_t h = subtree_height_rec(left);
+  if (right > 0) h = std::max(h, subtree_height_rec(right));
   return h + 1;
+}
+
+// Height of the subtree over s items (the plan asks it for every node of at most kKdSeqMax items).
+uint32_t subtree_height(uint64_t s) {
+  static const std::vector<uint32_t> table = [] {
+    std::vector<uint32_t> t(kKdSeqMax + 1);
+    for (uint64_t i = 0; i <= kKdSeqMax; ++i) t[i] = subtree_height_rec(i);
+    return t;
+  }();
+  return s <= kKdSeqMax ? table[s] : subtree_height_rec(s);
 }
 
 struct Plan {
@@ -43,6 +53,7 @@ Plan plan_levels(uint64_t n) {
   std::vector<uint32_t> small_level;
   for (uint32_t level = 0; !cur.empty(); ++level) {
     std::vector<uint64_t> big, mid, next;
+    next.reserve(2 * cur.size());
     uint64_t mmax = 0;
     for (size_t i = 0; i < cur.size(); i += 2) {
       const uint64_t b = cur[i], e = cur[i + 1], s = e - b;
@@ -117,9 +128,16 @@ hipError_t multi_level(const KdBuildArgs &a, const Plan &plan, const Plan::Level
   if (e == hipSuccess) e = hipMemcpyAsync(d_nodes, nodes.data(), nn * sizeof(KdStepNode), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(d_chunk, chunk_node.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_kd_step_init(a, d_nodes, nn, d_chunk, nc, d_work, st);
-  // each step shrinks an active subrange, and the depth budget bounds the steps
-  for (int step = 0; e == hipSuccess && step < 256; ++step) {
-    e = launch_kd_step_begin(a, d_nodes, nn, st);
+  // Each step shrinks an active subrange, and the depth budget bounds the steps. Steps are
+  // enqueued kStepGroup at a time between read-backs: a node that has left the loop stays out and
+  // every step kernel skips it, so the steps past the last active one change nothing.
+  constexpr int kStepGroup = 8;
+  for (int step = 0; e == hipSuccess && step < 256; step += kStepGroup) {
+    for (int g = 0; g < kStepGroup && e == hipSuccess; ++g) {
+      e = launch_kd_step_begin(a, d_nodes, nn, st);
+      if (e == hipSuccess) e = launch_kd_step_rest(a, d_nodes, nn, d_chunk, nc, d_work, st);
+    }
+    if (e == hipSuccess) e = launch_kd_step_begin(a, d_nodes, nn, st);
     if (e == hipSuccess) e = hipMemcpyAsync(nodes.data(), d_nodes, nn * sizeof(KdStepNode), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) break;
