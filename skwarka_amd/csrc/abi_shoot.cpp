@@ -111,21 +111,44 @@ struct Records {
   }
 };
 
-// The working memory of photon shooting on one device, kept for the life of the process; calls
-// on one device take turns on it.
-struct ShootWorkspace {
-  std::mutex mu;
-  Keep<double> cdf;
+// One round of paths in flight: its stream, path lists, counters and photon records.
+struct Lane {
+  hipStream_t st = nullptr;
   Keep<skw_ray> rays[2];
   Keep<ShootPath> state[2];
   Keep<skw_hit> hits;
   Keep<uint64_t> counts;
-  Keep<uint32_t> iota, order, keys_sorted;
-  Keep<uint8_t> sort_tmp;
-  Records round[3];  // the current round's photons of each map, in deposit order
-  Records out[3];    // every round's photons of each map, in (path, intersection) order
+  Records round[3];       // the round's photons of each map, in deposit order
   uint64_t *h = nullptr;  // two pinned copies of the five counters
   hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t fin = nullptr;  // the lane's last round has been sorted into the staging arrays
+};
+
+// The working memory of photon shooting on one device, kept for the life of the process; calls
+// on one device take turns on it. Two lanes let a round start while the previous round's last
+// few paths still bounce.
+struct ShootWorkspace {
+  std::mutex mu;
+  bool ready = false;
+  hipStream_t aux = nullptr;  // the second lane's stream (default priority)
+  hipEvent_t start = nullptr;
+  Keep<double> cdf;
+  Lane lane[2];
+  Keep<uint32_t> iota, order, keys_sorted;
+  Keep<uint8_t> sort_tmp;
+  Records out[3];  // every round's photons of each map, in (path, intersection) order
+  hipError_t init() {
+    if (ready) return hipSuccess;
+    hipError_t e = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&start, hipEventDisableTiming);
+    for (Lane &L : lane) {
+      if (e == hipSuccess) e = hipHostMalloc((void **)&L.h, 10 * sizeof(uint64_t), hipHostMallocDefault);
+      for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&L.ev[i], hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&L.fin, hipEventDisableTiming);
+    }
+    ready = e == hipSuccess;
+    return e;
+  }
 };
 
 constexpr int kWorkspaceDevices = 64;
@@ -140,7 +163,76 @@ hipError_t shoot_workspace(ShootWorkspace **out) {
   return hipSuccess;
 }
 
-// The shooting loop; all device work is on `st`, whose pool buffers are released in stream order.
+// Host state of the round a lane runs.
+struct RoundState {
+  bool busy = false;      // paths still bounce
+  bool finished = false;  // every path ended; waiting to be sorted in round order
+  uint64_t index = 0, n = 0;
+  bool done[3] = {false, false, false};
+  ShootArgs a;
+  uint64_t A = 0, P[3] = {0, 0, 0};  // exact counts at step s_known (see step_lane)
+  int64_t s_known = -1, m = 1;
+};
+
+// At most this many intersections per path (the bounce kernel's limit) bound a round's photons.
+constexpr uint64_t kMaxIsectBound = 1024;
+
+// The lane's next bounce. The host reads the counters of step k (k = 0 the start, k >= 1 bounce k)
+// from a pinned copy two steps late: active paths never increase and a path deposits at most one
+// photon per bounce, so from exact counts at step s (A paths active for bounce s + 1, P[j]
+// records) bounce m runs a grid for A paths and holds at most P[j] + (m - s) A records; the kernels
+// read the exact path count on the device. Without `block` it returns false when the counters it
+// needs have not arrived.
+bool step_lane(const skw_scene *scene, Lane &L, RoundState &R, bool block, hipError_t &e, skw_status &rs) {
+  if (R.m >= 2) {
+    hipEvent_t ev = L.ev[R.m & 1];
+    if (!block) {
+      const hipError_t q = hipEventQuery(ev);
+      if (q == hipErrorNotReady) return false;
+      if (q != hipSuccess) {
+        e = q;
+        return true;
+      }
+    } else if ((e = hipEventSynchronize(ev)) != hipSuccess) {
+      return true;
+    }
+    const uint64_t *c = L.h + 5 * (R.m & 1);  // step m - 2
+    R.A = c[(R.m - 2) & 1];
+    for (int j = 0; j < 3; ++j) R.P[j] = c[2 + j];
+    R.s_known = R.m - 2;
+  }
+  if (R.A == 0) {  // no path reaches bounce s_known + 1: every later step is empty
+    e = hipStreamSynchronize(L.st);  // the copies still in flight write the pinned counters
+    R.busy = false;
+    R.finished = true;
+    return true;
+  }
+  const uint64_t steps = (uint64_t)(R.m - R.s_known);
+  for (int j = 0; j < 3 && e == hipSuccess; ++j)
+    if (!R.done[j] && j != 1) e = L.round[j].reserve(R.P[j] + steps * R.A, R.P[j] + (steps - 1) * R.A, L.st, true);
+  if (e != hipSuccess) return true;
+  ShootArgs &a = R.a;
+  for (int j = 0; j < 3; ++j) {
+    a.ph_pos[j] = L.round[j].pos.p;
+    a.ph_w[j] = L.round[j].w.p;
+    a.ph_codes[j] = L.round[j].codes.p;
+    a.ph_key[j] = L.round[j].key.p;
+  }
+  rs = enqueue_trace(scene, L.rays[a.cur].p, R.A, L.counts.p + a.cur, L.hits.p, nullptr, L.st, false, nullptr,
+                     nullptr, false);
+  if (rs != SKW_OK) return true;
+  if ((e = hipMemsetAsync(L.counts.p + (a.cur ^ 1u), 0, sizeof(uint64_t), L.st)) != hipSuccess) return true;
+  if ((e = launch_shoot_bounce(a, R.A, L.st)) != hipSuccess) return true;
+  a.cur ^= 1u;
+  if ((e = hipMemcpyAsync(L.h + 5 * (R.m & 1), L.counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, L.st)) !=
+      hipSuccess)
+    return true;
+  e = hipEventRecord(L.ev[R.m & 1], L.st);
+  ++R.m;
+  return true;
+}
+
+// The shooting loop over rounds, two rounds in flight where the maps' limits allow it.
 hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const skw_shoot_params *prm,
                         const std::vector<DevLight> &lights, const DevLight *d_lights, const DevLightTri *d_ltris,
                         const DevMat *mats, skw_photon_set *set, hipStream_t st, skw_status *rs) {
@@ -164,141 +256,178 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   hipError_t e = shoot_workspace(&ws);
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lock(ws->mu);
-  if (!ws->h) {
-    e = hipHostMalloc((void **)&ws->h, 10 * sizeof(uint64_t), hipHostMallocDefault);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ws->ev[i], hipEventDisableTiming);
-    if (e != hipSuccess) return e;
-  }
+  if ((e = ws->init()) != hipSuccess) return e;
+  ws->lane[0].st = st;
+  ws->lane[1].st = ws->aux;
   e = ws->cdf.grow(nl, 0, st);
-  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-    e = ws->rays[i].grow(R, 0, st);
-    if (e == hipSuccess) e = ws->state[i].grow(R, 0, st);
+  for (Lane &L : ws->lane) {
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = L.rays[i].grow(R, 0, L.st);
+      if (e == hipSuccess) e = L.state[i].grow(R, 0, L.st);
+    }
+    if (e == hipSuccess) e = L.hits.grow(R, 0, L.st);
+    if (e == hipSuccess) e = L.counts.grow(5, 0, L.st);
   }
-  if (e == hipSuccess) e = ws->hits.grow(R, 0, st);
-  if (e == hipSuccess) e = ws->counts.grow(5, 0, st);
   if (e == hipSuccess) e = hipMemcpyAsync(ws->cdf.p, cdf.data(), nl * sizeof(double), hipMemcpyHostToDevice, st);
+  // the second lane starts after everything the caller's stream holds
+  if (e == hipSuccess) e = hipEventRecord(ws->start, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ws->aux, ws->start, 0);
   if (e != hipSuccess) return e;
-  uint64_t *h_counts[2] = {ws->h, ws->h + 5};
-  hipEvent_t *ev = ws->ev;
-  Records *round = ws->round;
-  Keep<uint64_t> &counts = ws->counts;
   int key_bits = 1;
   while ((1ull << key_bits) < R) ++key_bits;
 
-  ShootArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.geom = scene->geom_args();
-  a.mats = mats;
-  a.lights = d_lights;
-  a.ltris = d_ltris;
-  a.light_cdf = ws->cdf.p;
-  a.n_lights = nl;
-  a.seed = prm->seed;
-  for (int i = 0; i < 2; ++i) {
-    a.rays[i] = ws->rays[i].p;
-    a.state[i] = ws->state[i].p;
-  }
-  a.hits = ws->hits.p;
-  a.counts = counts.p;
+  ShootArgs base;
+  std::memset(&base, 0, sizeof(base));
+  base.geom = scene->geom_args();
+  base.mats = mats;
+  base.lights = d_lights;
+  base.ltris = d_ltris;
+  base.light_cdf = ws->cdf.p;
+  base.n_lights = nl;
+  base.seed = prm->seed;
 
-  uint64_t have[3] = {0, 0, 0};  // photons per map so far
-  while (*rs == SKW_OK && e == hipSuccess) {
-    bool done[3];
-    for (int j = 0; j < 3; ++j) done[j] = have[j] >= prm->max_photons[j];
-    if (set->total_paths >= prm->max_paths || (done[0] && done[1] && done[2])) break;
-    bool stuck = set->total_paths > 10000000ull;
-    for (int j = 0; j < 3; ++j) stuck = stuck && (done[j] || have[j] == 0);
-    if (stuck) break;
-    const uint64_t first = set->total_paths;
-    const uint64_t n = std::min<uint64_t>(R, prm->max_paths - first);
-    set->total_paths += n;
-    a.first = first;
-    a.n = n;
-    for (int j = 0; j < 3; ++j) a.done[j] = done[j] ? 1u : 0u;
-    // direct photons: at most one per path
-    if (!done[1] && (e = round[1].reserve(n, 0, st, true)) != hipSuccess) break;
-    if ((e = hipMemsetAsync(counts.p, 0, 5 * sizeof(uint64_t), st)) != hipSuccess) break;
-    a.cur = 0;
+  RoundState rst[2];
+  uint64_t have[3] = {0, 0, 0};  // photons of the rounds sorted so far
+  uint64_t started = 0, sorted = 0;
+  bool stop = false;
+  hipEvent_t last_fin = nullptr;
+
+  // Whether a round may start now, and with which maps done: the rule (a map is done once it
+  // holds max_photons) on exact counts, decided early only when the round in flight cannot change
+  // the outcome. Returns 1 start, 0 wait, -1 no more rounds.
+  auto decide = [&](bool done[3]) -> int {
+    const RoundState *other = nullptr;
+    for (const RoundState &r : rst)
+      if (r.busy || r.finished) other = &r;
+    bool all_done = true, any_unknown = false;
     for (int j = 0; j < 3; ++j) {
-      a.ph_pos[j] = round[j].pos.p;
-      a.ph_w[j] = round[j].w.p;
-      a.ph_codes[j] = round[j].codes.p;
-      a.ph_key[j] = round[j].key.p;
-    }
-    if ((e = launch_shoot_start(a, st)) != hipSuccess) break;
-    // The bounces are enqueued back to back: the host reads the counters of step k (k = 0 the
-    // start, k >= 1 bounce k) from a pinned copy while later bounces run, two steps late. Active
-    // paths never increase and a path deposits at most one photon per bounce, so from exact counts
-    // at step s (A paths active for bounce s + 1, P[j] records) bounce m runs a grid for A paths and
-    // holds at most P[j] + (m - s) A records; the kernels read the exact path count on the device.
-    uint64_t A = n, P[3] = {0, 0, 0};
-    int64_t s_known = -1;
-    if ((e = hipMemcpyAsync(h_counts[0], counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
-      break;
-    if ((e = hipEventRecord(ev[0], st)) != hipSuccess) break;
-    for (int64_t m = 1; e == hipSuccess && *rs == SKW_OK; ++m) {
-      if (m >= 2) {
-        if ((e = hipEventSynchronize(ev[m & 1])) != hipSuccess) break;
-        const uint64_t *c = h_counts[m & 1];  // step m - 2
-        A = c[(m - 2) & 1];
-        for (int j = 0; j < 3; ++j) P[j] = c[2 + j];
-        s_known = m - 2;
+      if (have[j] >= prm->max_photons[j]) {
+        done[j] = true;
+      } else if (!other) {
+        done[j] = false;
+      } else {
+        const uint64_t bound = other->finished ? other->P[j] : other->P[j] + other->A * (kMaxIsectBound + 2);
+        if (have[j] + bound < prm->max_photons[j]) done[j] = false;
+        else any_unknown = true;
       }
-      if (A == 0) break;  // no path reaches bounce s_known + 1: every later step is empty
-      const uint64_t steps = (uint64_t)(m - s_known);
-      for (int j = 0; j < 3 && e == hipSuccess; ++j)
-        if (!done[j] && j != 1) e = round[j].reserve(P[j] + steps * A, P[j] + (steps - 1) * A, st, true);
-      if (e != hipSuccess) break;
+      all_done = all_done && done[j];
+    }
+    if (set->total_paths >= prm->max_paths) return -1;
+    if (any_unknown) return 0;
+    if (all_done) return -1;
+    if (set->total_paths > 10000000ull) {
+      if (other) return 0;  // the stop rule below needs exact counts
+      bool stuck = true;
+      for (int j = 0; j < 3; ++j) stuck = stuck && (done[j] || have[j] == 0);
+      if (stuck) return -1;
+    }
+    return 1;
+  };
+
+  while (e == hipSuccess && *rs == SKW_OK) {
+    bool progress = false;
+    // sort the finished rounds into the staging arrays, in round order
+    for (int l = 0; l < 2 && e == hipSuccess; ++l) {
+      RoundState &Rs = rst[l];
+      if (!Rs.finished || Rs.index != sorted) continue;
+      Lane &L = ws->lane[l];
+      if (last_fin && (e = hipStreamWaitEvent(L.st, last_fin, 0)) != hipSuccess) break;
+      for (int j = 0; j < 3 && e == hipSuccess; ++j) {
+        if (Rs.done[j]) continue;
+        set->paths[j] += Rs.n;
+        const uint64_t k = Rs.P[j];
+        if (k == 0) continue;
+        size_t sort_bytes = 0;
+        if ((e = sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, k, 0, key_bits, L.st)) !=
+            hipSuccess)
+          break;
+        if ((e = ws->sort_tmp.grow(sort_bytes, 0, L.st)) != hipSuccess) break;
+        if ((e = ws->iota.grow(k, 0, L.st)) != hipSuccess) break;
+        if ((e = ws->order.grow(k, 0, L.st)) != hipSuccess) break;
+        if ((e = ws->keys_sorted.grow(k, 0, L.st)) != hipSuccess) break;
+        if ((e = launch_iota_u32(ws->iota.p, k, L.st)) != hipSuccess) break;
+        if ((e = sort_pairs_u32(ws->sort_tmp.p, sort_bytes, L.round[j].key.p, ws->keys_sorted.p, ws->iota.p,
+                                ws->order.p, k, 0, key_bits, L.st)) != hipSuccess)
+          break;
+        Records &o = ws->out[j];
+        if ((e = o.reserve(have[j] + k, have[j], L.st, false)) != hipSuccess) break;
+        e = launch_photon_gather_records(L.round[j].pos.p, L.round[j].w.p, L.round[j].codes.p, ws->order.p, k,
+                                         o.pos.p + 3 * have[j], o.w.p + 3 * have[j], o.codes.p + have[j], L.st);
+        have[j] += k;
+      }
+      if (e == hipSuccess) e = hipEventRecord(L.fin, L.st);
+      last_fin = L.fin;
+      Rs.finished = false;
+      ++sorted;
+      progress = true;
+    }
+    if (e != hipSuccess) break;
+    // start rounds on free lanes
+    for (int l = 0; l < 2 && !stop && e == hipSuccess; ++l) {
+      RoundState &Rs = rst[l];
+      if (Rs.busy || Rs.finished) continue;
+      bool done[3];
+      const int d = decide(done);
+      if (d < 0) stop = true;
+      if (d <= 0) break;
+      Lane &L = ws->lane[l];
+      Rs = RoundState();
+      Rs.busy = true;
+      Rs.index = started++;
+      Rs.n = std::min<uint64_t>(R, prm->max_paths - set->total_paths);
+      ShootArgs &a = Rs.a;
+      a = base;
+      a.first = set->total_paths;
+      a.n = Rs.n;
+      set->total_paths += Rs.n;
       for (int j = 0; j < 3; ++j) {
-        a.ph_pos[j] = round[j].pos.p;
-        a.ph_w[j] = round[j].w.p;
-        a.ph_codes[j] = round[j].codes.p;
-        a.ph_key[j] = round[j].key.p;
+        Rs.done[j] = done[j];
+        a.done[j] = done[j] ? 1u : 0u;
       }
-      *rs = enqueue_trace(scene, ws->rays[a.cur].p, A, counts.p + a.cur, ws->hits.p, nullptr, st, false, nullptr,
-                          nullptr, false);
-      if (*rs != SKW_OK) break;
-      if ((e = hipMemsetAsync(counts.p + (a.cur ^ 1u), 0, sizeof(uint64_t), st)) != hipSuccess) break;
-      if ((e = launch_shoot_bounce(a, A, st)) != hipSuccess) break;
-      a.cur ^= 1u;
-      if ((e = hipMemcpyAsync(h_counts[m & 1], counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st)) !=
-          hipSuccess)
+      for (int i = 0; i < 2; ++i) {
+        a.rays[i] = L.rays[i].p;
+        a.state[i] = L.state[i].p;
+      }
+      a.hits = L.hits.p;
+      a.counts = L.counts.p;
+      a.cur = 0;
+      Rs.A = Rs.n;
+      // direct photons: at most one per path
+      if (!done[1] && (e = L.round[1].reserve(Rs.n, 0, L.st, true)) != hipSuccess) break;
+      for (int j = 0; j < 3; ++j) {
+        a.ph_pos[j] = L.round[j].pos.p;
+        a.ph_w[j] = L.round[j].w.p;
+        a.ph_codes[j] = L.round[j].codes.p;
+        a.ph_key[j] = L.round[j].key.p;
+      }
+      if ((e = hipMemsetAsync(L.counts.p, 0, 5 * sizeof(uint64_t), L.st)) != hipSuccess) break;
+      if ((e = launch_shoot_start(a, L.st)) != hipSuccess) break;
+      if ((e = hipMemcpyAsync(L.h, L.counts.p, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, L.st)) != hipSuccess)
         break;
-      e = hipEventRecord(ev[m & 1], st);
+      if ((e = hipEventRecord(L.ev[0], L.st)) != hipSuccess) break;
+      progress = true;
     }
-    // the copies still in flight write the pinned counters: let them land first
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (*rs != SKW_OK || e != hipSuccess) break;
-    // the round's photons in (path, intersection) order
-    for (int j = 0; j < 3 && e == hipSuccess; ++j) {
-      if (done[j]) continue;
-      set->paths[j] += n;
-      const uint64_t k = P[j];
-      if (k == 0) continue;
-      size_t sort_bytes = 0;
-      if ((e = sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, k, 0, key_bits, st)) !=
-          hipSuccess)
-        break;
-      if ((e = ws->sort_tmp.grow(sort_bytes, 0, st)) != hipSuccess) break;
-      if ((e = ws->iota.grow(k, 0, st)) != hipSuccess) break;
-      if ((e = ws->order.grow(k, 0, st)) != hipSuccess) break;
-      if ((e = ws->keys_sorted.grow(k, 0, st)) != hipSuccess) break;
-      if ((e = launch_iota_u32(ws->iota.p, k, st)) != hipSuccess) break;
-      if ((e = sort_pairs_u32(ws->sort_tmp.p, sort_bytes, round[j].key.p, ws->keys_sorted.p, ws->iota.p, ws->order.p,
-                              k, 0, key_bits, st)) != hipSuccess)
-        break;
-      Records &o = ws->out[j];
-      if ((e = o.reserve(have[j] + k, have[j], st, false)) != hipSuccess) break;
-      e = launch_photon_gather_records(round[j].pos.p, round[j].w.p, round[j].codes.p, ws->order.p, k,
-                                       o.pos.p + 3 * have[j], o.w.p + 3 * have[j], o.codes.p + have[j], st);
-      have[j] += k;
+    if (e != hipSuccess) break;
+    // the lanes' next bounces
+    int oldest = -1;
+    for (int l = 0; l < 2 && e == hipSuccess && *rs == SKW_OK; ++l) {
+      if (!rst[l].busy) continue;
+      if (oldest < 0 || rst[l].index < rst[oldest].index) oldest = l;
+      progress = step_lane(scene, ws->lane[l], rst[l], false, e, *rs) || progress;
     }
+    if (e != hipSuccess || *rs != SKW_OK) break;
+    const bool idle = !rst[0].busy && !rst[0].finished && !rst[1].busy && !rst[1].finished;
+    if (stop && idle) break;
+    if (!progress && oldest >= 0) step_lane(scene, ws->lane[oldest], rst[oldest], true, e, *rs);
   }
   if (*rs != SKW_OK || e != hipSuccess) {
-    (void)hipStreamSynchronize(st);  // no copy may still target the pinned counters
+    // no copy may still target the pinned counters
+    (void)hipStreamSynchronize(ws->aux);
+    (void)hipStreamSynchronize(st);
     return e;
   }
-  // each map's rounds, concatenated into arrays of the exact size
+  // each map's rounds, concatenated into arrays of the exact size, on the caller's stream
+  if (last_fin && (e = hipStreamWaitEvent(st, last_fin, 0)) != hipSuccess) return e;
   for (int j = 0; j < 3 && e == hipSuccess; ++j) {
     if (have[j] == 0) continue;
     if ((e = set->m[j].alloc(have[j], st)) != hipSuccess) break;
@@ -310,6 +439,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
       e = hipMemcpyAsync(set->m[j].codes, o.codes.p, have[j] * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
     set->m[j].n = have[j];
   }
+  if (e == hipSuccess) e = hipStreamSynchronize(ws->aux);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e;
 }
