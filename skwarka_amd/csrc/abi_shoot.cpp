@@ -3,8 +3,9 @@
 //
 // PhotonLTEIntegrator::ShootPhotons (Raytracer/LTEIntegrators/PhotonLTEIntegrator/
 // PhotonLTEIntegrator.cpp:247-288) with the input filter's stop rules and the merging filter's
-// per-map bookkeeping (PhotonShootingPipeline.cpp:48-92, 254-310), one round of paths at a time;
-// the lights' power CDF of _GetLightsPowerCDF (PhotonLTEIntegrator.cpp:80-124).
+// per-map bookkeeping (PhotonShootingPipeline.cpp:48-92, 254-310) over rounds of paths (up to two
+// in flight when that cannot change which maps take photons); the lights' power CDF of
+// _GetLightsPowerCDF (PhotonLTEIntegrator.cpp:80-124).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
