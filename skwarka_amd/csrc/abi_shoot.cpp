@@ -126,21 +126,24 @@ struct Lane {
 };
 
 // The working memory of photon shooting on one device, kept for the life of the process; calls
-// on one device take turns on it. Two lanes let a round start while the previous round's last
-// few paths still bounce.
+// on one device take turns on it. Several lanes let a round start while earlier rounds' last few
+// paths still bounce.
+constexpr int kLanes = 3;
+
 struct ShootWorkspace {
   std::mutex mu;
   bool ready = false;
-  hipStream_t aux = nullptr;  // the second lane's stream (default priority)
+  hipStream_t aux[kLanes - 1] = {};  // the other lanes' streams (default priority)
   hipEvent_t start = nullptr;
   Keep<double> cdf;
-  Lane lane[2];
+  Lane lane[kLanes];
   Keep<uint32_t> iota, order, keys_sorted;
   Keep<uint8_t> sort_tmp;
   Records out[3];  // every round's photons of each map, in (path, intersection) order
   hipError_t init() {
     if (ready) return hipSuccess;
-    hipError_t e = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < kLanes - 1 && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&start, hipEventDisableTiming);
     for (Lane &L : lane) {
       if (e == hipSuccess) e = hipHostMalloc((void **)&L.h, 10 * sizeof(uint64_t), hipHostMallocDefault);
@@ -259,7 +262,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   std::lock_guard<std::mutex> lock(ws->mu);
   if ((e = ws->init()) != hipSuccess) return e;
   ws->lane[0].st = st;
-  ws->lane[1].st = ws->aux;
+  for (int i = 1; i < kLanes; ++i) ws->lane[i].st = ws->aux[i - 1];
   e = ws->cdf.grow(nl, 0, st);
   for (Lane &L : ws->lane) {
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
@@ -270,9 +273,9 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
     if (e == hipSuccess) e = L.counts.grow(5, 0, L.st);
   }
   if (e == hipSuccess) e = hipMemcpyAsync(ws->cdf.p, cdf.data(), nl * sizeof(double), hipMemcpyHostToDevice, st);
-  // the second lane starts after everything the caller's stream holds
+  // the other lanes start after everything the caller's stream holds
   if (e == hipSuccess) e = hipEventRecord(ws->start, st);
-  if (e == hipSuccess) e = hipStreamWaitEvent(ws->aux, ws->start, 0);
+  for (int i = 0; i < kLanes - 1 && e == hipSuccess; ++i) e = hipStreamWaitEvent(ws->aux[i], ws->start, 0);
   if (e != hipSuccess) return e;
   int key_bits = 1;
   while ((1ull << key_bits) < R) ++key_bits;
@@ -287,7 +290,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   base.n_lights = nl;
   base.seed = prm->seed;
 
-  RoundState rst[2];
+  RoundState rst[kLanes];
   uint64_t have[3] = {0, 0, 0};  // photons of the rounds sorted so far
   uint64_t started = 0, sorted = 0;
   bool stop = false;
@@ -297,19 +300,23 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   // holds max_photons) on exact counts, decided early only when the round in flight cannot change
   // the outcome. Returns 1 start, 0 wait, -1 no more rounds.
   auto decide = [&](bool done[3]) -> int {
-    const RoundState *other = nullptr;
-    for (const RoundState &r : rst)
-      if (r.busy || r.finished) other = &r;
+    bool other = false;
+    uint64_t bound[3] = {0, 0, 0};  // photons the rounds in flight may still add
+    for (const RoundState &r : rst) {
+      if (!(r.busy || r.finished)) continue;
+      other = true;
+      for (int j = 0; j < 3; ++j) bound[j] += r.finished ? r.P[j] : r.P[j] + r.A * (kMaxIsectBound + 2);
+    }
     bool all_done = true, any_unknown = false;
     for (int j = 0; j < 3; ++j) {
       if (have[j] >= prm->max_photons[j]) {
         done[j] = true;
       } else if (!other) {
         done[j] = false;
+      } else if (have[j] + bound[j] < prm->max_photons[j]) {
+        done[j] = false;
       } else {
-        const uint64_t bound = other->finished ? other->P[j] : other->P[j] + other->A * (kMaxIsectBound + 2);
-        if (have[j] + bound < prm->max_photons[j]) done[j] = false;
-        else any_unknown = true;
+        any_unknown = true;
       }
       all_done = all_done && done[j];
     }
@@ -328,7 +335,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   while (e == hipSuccess && *rs == SKW_OK) {
     bool progress = false;
     // sort the finished rounds into the staging arrays, in round order
-    for (int l = 0; l < 2 && e == hipSuccess; ++l) {
+    for (int l = 0; l < kLanes && e == hipSuccess; ++l) {
       RoundState &Rs = rst[l];
       if (!Rs.finished || Rs.index != sorted) continue;
       Lane &L = ws->lane[l];
@@ -364,7 +371,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
     }
     if (e != hipSuccess) break;
     // start rounds on free lanes
-    for (int l = 0; l < 2 && !stop && e == hipSuccess; ++l) {
+    for (int l = 0; l < kLanes && !stop && e == hipSuccess; ++l) {
       RoundState &Rs = rst[l];
       if (Rs.busy || Rs.finished) continue;
       bool done[3];
@@ -411,19 +418,20 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
     if (e != hipSuccess) break;
     // the lanes' next bounces
     int oldest = -1;
-    for (int l = 0; l < 2 && e == hipSuccess && *rs == SKW_OK; ++l) {
+    for (int l = 0; l < kLanes && e == hipSuccess && *rs == SKW_OK; ++l) {
       if (!rst[l].busy) continue;
       if (oldest < 0 || rst[l].index < rst[oldest].index) oldest = l;
       progress = step_lane(scene, ws->lane[l], rst[l], false, e, *rs) || progress;
     }
     if (e != hipSuccess || *rs != SKW_OK) break;
-    const bool idle = !rst[0].busy && !rst[0].finished && !rst[1].busy && !rst[1].finished;
+    bool idle = true;
+    for (const RoundState &r : rst) idle = idle && !r.busy && !r.finished;
     if (stop && idle) break;
     if (!progress && oldest >= 0) step_lane(scene, ws->lane[oldest], rst[oldest], true, e, *rs);
   }
   if (*rs != SKW_OK || e != hipSuccess) {
     // no copy may still target the pinned counters
-    (void)hipStreamSynchronize(ws->aux);
+    for (hipStream_t x : ws->aux) (void)hipStreamSynchronize(x);
     (void)hipStreamSynchronize(st);
     return e;
   }
@@ -440,7 +448,7 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
       e = hipMemcpyAsync(set->m[j].codes, o.codes.p, have[j] * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
     set->m[j].n = have[j];
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(ws->aux);
+  for (int i = 0; i < kLanes - 1 && e == hipSuccess; ++i) e = hipStreamSynchronize(ws->aux[i]);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e;
 }
