@@ -150,6 +150,14 @@ skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_poin
                                  float *d_e_int, float *d_radiance, uint32_t *d_ids, double *d_d2,
                                  uint32_t *d_count, void *stream);
 
+/* skw_photon_irradiance's lookups and outputs (no set returned) with the work of each lookup
+ * counted: d_stats gets 3 uint32 per query = nodes visited (KDTree.h:545-575), photons inside the
+ * radius that reach the PhotonFilter (KDTree.h:549-551, PhotonInternalTypes.h:128-134) and the
+ * final set size. A measurement variant: the same walk as the plain call, slower. */
+skw_status skw_photon_irradiance_stats(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                       uint64_t nq, uint32_t k, double r, uint64_t paths, double rho, float *d_e_ext,
+                                       float *d_e_int, float *d_radiance, uint32_t *d_stats, void *stream);
+
 /* PhotonLTEIntegrator::_LookupCausticRadiance + _PhotonKernel (PhotonLTEIntegrator.cpp:579-636) for a
  * Lambertian surface of albedo rho seen from direction d_wo (3 doubles per query, unit): the
  * PhotonFilter'd k-nearest set weighted by Simpson's kernel with the running maximum of d^2 in the
@@ -470,6 +478,11 @@ skw_status skw_batch_hits(const skw_batch *batch, uint32_t scene, const skw_hit 
 /* Completion time of each item's last chunk (scenes, then gather jobs), in nanoseconds after the
  * frame started (waits for the last frame). */
 skw_status skw_batch_latency(skw_batch *batch, double *ns_per_scene);
+/* Waits for the last frame and reports whether every wave of it ran to its end: SKW_HIP_ERROR if
+ * a wave found an iteration in which none of its lanes could move and left the loop (a broken
+ * invariant; the frame's hits are then incomplete). skw_batch_latency makes the same check.
+ * No counterpart in the reference (its renderer has no persistent kernel). */
+skw_status skw_batch_check(skw_batch *batch);
 /* Writes scene `scene`'s camera rays (as the frame generates them) to d_rays; for checking. */
 skw_status skw_batch_rays(const skw_batch *batch, uint32_t scene, skw_ray *d_rays, void *stream);
 /* Total rays per frame. */

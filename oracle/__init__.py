@@ -56,6 +56,7 @@ def lib():
         "orc_photon_map_create": ([_U64, _P, _P, _P, _P], _P),
         "orc_photon_map_destroy": ([_P], None),
         "orc_photon_map_dump": ([_P, _P, _P, _P], None),
+        "orc_photon_lookup_counts": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _P, ctypes.c_int], None),
         "orc_photon_irradiance": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, _P, _P, _P, _P, _P, ctypes.c_int],
                                   None),
         "orc_photon_nearest": ([_P, _U64, _P, _P, ctypes.c_double, _P, ctypes.c_int], None),
@@ -280,6 +281,15 @@ class PhotonMap:
         lib().orc_photon_irradiance(self.h, nq, p.ctypes.data, n.ctypes.data, k, r, paths, e_ext.ctypes.data,
                                     e_int.ctypes.data, _ptr(ids), _ptr(d2), cnt.ctypes.data, nthreads or threads())
         return e_ext, e_int, ids, d2, cnt
+
+    def lookup_counts(self, p, n, k, r, nthreads=None):
+        """Per query of irradiance(): (nodes visited, photons inside the radius, set size)."""
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        out = np.zeros((len(p), 3), np.uint32)
+        lib().orc_photon_lookup_counts(self.h, len(p), p.ctypes.data, n.ctypes.data, k, r, out.ctypes.data,
+                                       nthreads or threads())
+        return out
 
     def caustic(self, p, n, wo, k, r, paths, rho, nthreads=None):
         p = np.ascontiguousarray(p, np.float64)

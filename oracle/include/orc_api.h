@@ -112,6 +112,9 @@ void orc_photon_map_dump(const orc_photon_map *h, uint32_t *perm, float *split, 
 void orc_photon_irradiance(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
                            double r, uint64_t paths, float *e_ext, float *e_int, uint32_t *ids, double *d2,
                            uint32_t *count, int nthreads);
+/* Per query of orc_photon_irradiance: nodes visited, photons inside the radius, set size. */
+void orc_photon_lookup_counts(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
+                              double r, uint32_t *counts, int nthreads);
 /* _LookupCausticRadiance with a Lambertian BRDF of albedo rho (reflection side only). */
 void orc_photon_caustic(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, const double *wo,
                         uint32_t k, double r, uint64_t paths, double rho, double *radiance, int nthreads);

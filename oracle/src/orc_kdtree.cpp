@@ -120,24 +120,30 @@ struct KD {
     }
   }
 
+  // cnt (optional): [0] calls of _Lookup (nodes visited), [1] points inside the radius (those the
+  // processor is called for, KDTree.h:549-551).
   template <class Proc>
-  void lookup(size_t i, V3 q, Proc &proc, double &max_d2) const {
+  void lookup(size_t i, V3 q, Proc &proc, double &max_d2, uint32_t *cnt = nullptr) const {
     if (nodes.empty()) return;
     const NodeK &n = nodes[i];
     const P &p = pts[i];
     double dx = p[0] - q.x, dy = p[1] - q.y, dz = p[2] - q.z;
     double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 < max_d2) proc(i, d2, max_d2);
+    if (cnt) ++cnt[0];
+    if (d2 < max_d2) {
+      if (cnt) ++cnt[1];
+      proc(i, d2, max_d2);
+    }
     if (!n.leaf()) {
       unsigned a = n.axis();
       double qa = q[(int)a];
       double pd2 = (qa - n.split) * (qa - n.split);
       if (qa <= n.split) {
-        if (n.has_left()) lookup(i + 1, q, proc, max_d2);
-        if (pd2 < max_d2 && n.right() < nodes.size()) lookup(n.right(), q, proc, max_d2);
+        if (n.has_left()) lookup(i + 1, q, proc, max_d2, cnt);
+        if (pd2 < max_d2 && n.right() < nodes.size()) lookup(n.right(), q, proc, max_d2, cnt);
       } else {
-        if (n.right() < nodes.size()) lookup(n.right(), q, proc, max_d2);
-        if (pd2 < max_d2 && n.has_left()) lookup(i + 1, q, proc, max_d2);
+        if (n.right() < nodes.size()) lookup(n.right(), q, proc, max_d2, cnt);
+        if (pd2 < max_d2 && n.has_left()) lookup(i + 1, q, proc, max_d2, cnt);
       }
     }
   }
@@ -245,12 +251,12 @@ struct PhotonMap {
     return proc.best;
   }
 
-  // Returns photons found; arr holds them in the processor's final order.
-  size_t knn(V3 p, V3 n, uint32_t k, double r, NP *arr) const {
+  // Returns photons found; arr holds them in the processor's final order. cnt: see KD::lookup.
+  size_t knn(V3 p, V3 n, uint32_t k, double r, NP *arr, uint32_t *cnt = nullptr) const {
     if (k == 0 || tree.pts.empty()) return 0;
     KnnProc<PhotonFilter> proc{arr, k, 0, PhotonFilter(&tree, p, n, MAX_NORMAL_DEVIATION_COS)};
     double max_d2 = r * r;
-    tree.lookup(0, p, proc, max_d2);
+    tree.lookup(0, p, proc, max_d2, cnt);
     return proc.size;
   }
 };
@@ -438,6 +444,26 @@ void orc_photon_irradiance(const orc_photon_map *h, uint64_t nq, const double *p
         e_ext[3 * i + c] = (float)(ext[c] * inv);
         e_int[3 * i + c] = (float)(in[c] * inv);
       }
+    }
+  }
+}
+
+// The work of each lookup of orc_photon_irradiance: nodes visited (calls of KDTree::_Lookup,
+// KDTree.h:535-576), photons inside the radius (KDTree.h:549) and the final set size.
+void orc_photon_lookup_counts(const orc_photon_map *h, uint64_t nq, const double *p, const double *n, uint32_t k,
+                              double r, uint32_t *counts, int nthreads) {
+  const PhotonMap &m = *h->m;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    std::vector<NP> arr(k > 0 ? k : 1);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)nq; ++i) {
+      V3 qp{p[3 * i], p[3 * i + 1], p[3 * i + 2]}, qn{n[3 * i], n[3 * i + 1], n[3 * i + 2]};
+      uint32_t cnt[2] = {0, 0};
+      size_t found = m.knn(qp, qn, k, r, arr.data(), cnt);
+      counts[3 * i] = cnt[0];
+      counts[3 * i + 1] = cnt[1];
+      counts[3 * i + 2] = (uint32_t)found;
     }
   }
 }

@@ -27,13 +27,13 @@ EXPORTS = [
     "skw_scene_create", "skw_scene_destroy", "skw_scene_bounds", "skw_scene_info",
     "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export", "skw_photon_map_info",
-    "skw_photon_irradiance", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
+    "skw_photon_irradiance", "skw_photon_irradiance_stats", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
     "skw_photon_caustic_radiance", "skw_last_error",
     "skw_device_count", "skw_dl_scene_create", "skw_dl_scene_destroy", "skw_dl_render",
     "skw_dl_render_profiled", "skw_dl_param_count", "skw_dl_render_replay", "skw_dl_adjoint",
     "skw_dl_render_tape", "skw_dl_tape_adjoint", "skw_dl_tape_destroy",
     "skw_batch_create", "skw_batch_create_with_gathers", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
-    "skw_batch_hits", "skw_batch_latency", "skw_batch_rays", "skw_batch_ray_count",
+    "skw_batch_hits", "skw_batch_latency", "skw_batch_check", "skw_batch_rays", "skw_batch_ray_count",
     "skw_irradiance_map_create", "skw_irradiance_map_create_from", "skw_irradiance_map_destroy",
     "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
     "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
@@ -139,6 +139,8 @@ def lib():
         "skw_photon_map_info": ([_P, _P, _P], ctypes.c_int),
         "skw_photon_irradiance": ([_P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double,
                                    _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "skw_photon_irradiance_stats": ([_P, _P, _P, _U64, _U32, ctypes.c_double, _U64, ctypes.c_double,
+                                         _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_photon_knn": ([_P, _P, _P, _U64, _U32, ctypes.c_double, ctypes.c_int, _P, _P, _P, _P],
                            ctypes.c_int),
         "skw_photon_nearest": ([_P, _P, _P, _U64, ctypes.c_double, ctypes.c_int, _P, _P], ctypes.c_int),
@@ -167,6 +169,7 @@ def lib():
         "skw_batch_frame": ([_P, _P], ctypes.c_int),
         "skw_batch_hits": ([_P, _U32, ctypes.POINTER(_P), ctypes.POINTER(_U64)], ctypes.c_int),
         "skw_batch_latency": ([_P, _P], ctypes.c_int),
+        "skw_batch_check": ([_P], ctypes.c_int),
         "skw_batch_rays": ([_P, _U32, _P, _P], ctypes.c_int),
         "skw_batch_ray_count": ([_P, ctypes.POINTER(_U64)], ctypes.c_int),
         "skw_irradiance_map_create": ([ctypes.POINTER(IrrMapDesc), ctypes.POINTER(_P), _P], ctypes.c_int),
@@ -420,6 +423,12 @@ class PhotonMap:
                    d_count=None, stream=0):
         check(lib().skw_photon_irradiance(self.handle, d_p, d_n, nq, k, r, paths, rho, d_eext, d_eint, d_rad, d_ids,
                                           d_d2, d_count, stream or None), "skw_photon_irradiance")
+
+    def irradiance_stats(self, d_p, d_n, nq, k, r, paths, rho, d_eext, d_eint, d_rad, d_stats, stream=0):
+        """irradiance() with each lookup's work counted: d_stats gets (visits, photons inside the
+        radius, set size) per query as uint32."""
+        check(lib().skw_photon_irradiance_stats(self.handle, d_p, d_n, nq, k, r, paths, rho, d_eext, d_eint, d_rad,
+                                                d_stats, stream or None), "skw_photon_irradiance_stats")
 
     def knn(self, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream=0):
         check(lib().skw_photon_knn(self.handle, d_p, d_n, nq, k, r, filt, d_ids, d_d2, d_count, stream or None),
@@ -867,6 +876,10 @@ class Batch:
         out = np.zeros(self.n, np.float64)
         check(lib().skw_batch_latency(self.handle, out.ctypes.data), "skw_batch_latency")
         return out
+
+    def check(self):
+        """Waits for the last frame; raises if a wave of it stalled and left early."""
+        check(lib().skw_batch_check(self.handle), "skw_batch_check")
 
     def rays(self, scene, d_rays, stream=0):
         check(lib().skw_batch_rays(self.handle, scene, d_rays, stream or None), "skw_batch_rays")

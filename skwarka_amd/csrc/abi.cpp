@@ -114,7 +114,7 @@ hipError_t CounterPool::get(void *stream, uint64_t **out) {
 
 skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev,
                          skw_hit *d_hits, uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops,
-                         uint32_t *d_tests, bool fix, uint64_t *d_sums) {
+                         uint32_t *d_tests, bool fix, uint64_t *d_sums, const TraceStorage *fixed) {
   skw_scene *s = const_cast<skw_scene *>(scene);
   const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
   uint64_t waves = (n + 63) / 64;
@@ -140,15 +140,26 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.leaf_min = (uint32_t)clamp_int(env_int("SKW_TRACE_LEAF_MIN", kTraceLeafMin), 1, 64);
   a.claim = (uint32_t)clamp_int(env_int("SKW_TRACE_CLAIM", kTraceClaim), 64, 4096);
   const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
-  SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
   uint32_t need = s->host.stack_need;
-  if (need > 128) {
-    SKW_HIP(pool_alloc((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
+  if (fixed) {
+    a.counter = fixed->counter;
+    if (need > 128) {
+      const uint64_t per_wave = (uint64_t)64 * need * sizeof(uint32_t);
+      const uint64_t fit = fixed->gstack_bytes / per_wave;
+      if (fit == 0) return fail(SKW_UNSUPPORTED, "the tree's traversal stack does not fit the working memory");
+      if (grid > fit) grid = (uint32_t)fit;
+      a.gstack = fixed->gstack;
+    }
+  } else {
+    SKW_HIP(s->counters.get(st, &a.counter), "counter allocation");
+    if (need > 128) {
+      SKW_HIP(pool_alloc((void **)&a.gstack, (size_t)grid * 64 * need * sizeof(uint32_t), st), "stack allocation");
+    }
   }
   SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
   hipError_t e = any ? launch_any_hit(a, s->has_inst, stats, need, grid, st)
                      : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
-  pool_free(a.gstack, st);
+  if (!fixed) pool_free(a.gstack, st);
   if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
   if (!any && fix && s->needs_fix) {
     GeomArgs g = s->geom_args();
@@ -451,6 +462,7 @@ struct Estimate {
   double *caustic = nullptr;
   bool ids_tree = false;  // ids hold tree positions instead of caller indices
   const uint32_t *order = nullptr;  // visiting order of the queries (device), optional
+  uint32_t *stats = nullptr;        // per query (visits, photons inside the radius, set size)
 };
 
 static skw_status gather(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
@@ -479,6 +491,9 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.caustic = est.caustic;
   a.ids_tree = est.ids_tree ? 1 : 0;
   a.order = est.order;
+  a.stats = est.stats;
+  // A disc estimate whose set is not returned writes the set's slots only if the set fills.
+  a.defer = (mode == 0 && est.kind == 0 && e_ext && !ids && !d2) ? 1 : 0;
   if (mode == 0 && k > 0) {
     size_t elems = (size_t)nq * k;
     SKW_HIP(pool_alloc((void **)&a.heap_pos, elems * sizeof(uint32_t), st), "heap allocation");
@@ -534,6 +549,17 @@ skw_status skw_photon_irradiance(const skw_photon_map *map, const double *d_poin
   if (nq > 0 && !d_normals) return fail(SKW_INVALID_ARG, "normals must not be NULL");
   return gather(map, d_points, d_normals, nq, k, r, paths, rho, 1, 0, d_e_ext, d_e_int, d_radiance, d_ids, d_d2,
                 d_count, nullptr, stream);
+}
+
+skw_status skw_photon_irradiance_stats(const skw_photon_map *map, const double *d_points, const double *d_normals,
+                                       uint64_t nq, uint32_t k, double r, uint64_t paths, double rho, float *d_e_ext,
+                                       float *d_e_int, float *d_radiance, uint32_t *d_stats, void *stream) {
+  if (nq > 0 && (!d_e_ext || !d_e_int || !d_stats)) return fail(SKW_INVALID_ARG, "e_ext, e_int and stats must not be NULL");
+  if (nq > 0 && !d_normals) return fail(SKW_INVALID_ARG, "normals must not be NULL");
+  Estimate est;
+  est.stats = d_stats;
+  return gather(map, d_points, d_normals, nq, k, r, paths, rho, 1, 0, d_e_ext, d_e_int, d_radiance, nullptr, nullptr,
+                nullptr, nullptr, stream, est);
 }
 
 skw_status skw_photon_cone_irradiance(const skw_photon_map *map, const double *d_points, const double *d_normals,

@@ -28,7 +28,7 @@ struct skw_batch {
   DevBatchScene *d_table = nullptr;
   DevChunk *d_chunks = nullptr;
   uint64_t n_chunks = 0;
-  uint64_t *d_state = nullptr;  // queue head, frame start, then per scene: done, end
+  uint64_t *d_state = nullptr;  // queue head, frame start, stall flag, then per item: done, end
   skw_hit *d_hits = nullptr;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -44,8 +44,9 @@ struct skw_batch {
     a.n_chunks = n_chunks;
     a.queue_head = d_state;
     a.frame_start = d_state + 1;
-    a.item_done = d_state + 2;
-    a.item_end = d_state + 2 + n_items();
+    a.stalled = d_state + 2;
+    a.item_done = d_state + 3;
+    a.item_end = d_state + 3 + n_items();
     a.n_scenes = (uint32_t)table.size();
     a.n_jobs = (uint32_t)jobs.size();
     a.hits = d_hits;
@@ -188,6 +189,7 @@ skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const s
     a.e_ext = g.d_e_ext;
     a.e_int = g.d_e_int;
     a.radiance = g.d_radiance;
+    a.defer = 1;  // no ids or d^2 are returned: slots are written only for sets that fill
     const size_t slots = (size_t)g.nq * g.k;
     void *mem = nullptr;
     SKW_HIP(hipMalloc(&mem, slots * (sizeof(uint32_t) + sizeof(double)) + 256), "gather heap allocation");
@@ -201,7 +203,7 @@ skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const s
   b->cfg.stack_entries = b->max_stack <= 32 ? 32 : 64;
   SKW_HIP(upload(&b->d_table, b->table), "upload scene table");
   SKW_HIP(upload(&b->d_jobs, b->jobs), "upload gather jobs");
-  SKW_HIP(hipMalloc((void **)&b->d_state, (2 + 2 * b->n_items()) * sizeof(uint64_t)), "state allocation");
+  SKW_HIP(hipMalloc((void **)&b->d_state, (3 + 2 * b->n_items()) * sizeof(uint64_t)), "state allocation");
   SKW_HIP(hipMalloc((void **)&b->d_hits, std::max<uint64_t>(1, off) * sizeof(skw_hit)), "hit allocation");
   SKW_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream creation");
   SKW_HIP(build_queue(b.get()), "queue upload");
@@ -295,13 +297,23 @@ skw_status skw_batch_hits(const skw_batch *b, uint32_t scene, const skw_hit **d_
 skw_status skw_batch_latency(skw_batch *b, double *ns) {
   if (!b || !ns) return fail(SKW_INVALID_ARG, "arguments must not be NULL");
   SKW_HIP(hipDeviceSynchronize(), "synchronize");
-  std::vector<uint64_t> st(2 + 2 * b->n_items());
+  std::vector<uint64_t> st(3 + 2 * b->n_items());
   SKW_HIP(hipMemcpy(st.data(), b->d_state, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost), "state copy");
   const size_t n = b->n_items();
   for (size_t i = 0; i < n; ++i) {
-    uint64_t end = st[2 + n + i];
+    uint64_t end = st[3 + n + i];
     ns[i] = end >= st[1] ? (double)(end - st[1]) * 1e6 / b->wall_khz : 0.0;
   }
+  if (st[2] != 0) return fail(SKW_HIP_ERROR, "a wave of the last batch frame stalled (no lane could move) and left early");
+  return SKW_OK;
+}
+
+skw_status skw_batch_check(skw_batch *b) {
+  if (!b) return fail(SKW_INVALID_ARG, "batch must not be NULL");
+  SKW_HIP(hipDeviceSynchronize(), "synchronize");
+  uint64_t stalled = 0;
+  SKW_HIP(hipMemcpy(&stalled, b->d_state + 2, sizeof(uint64_t), hipMemcpyDeviceToHost), "state copy");
+  if (stalled != 0) return fail(SKW_HIP_ERROR, "a wave of the last batch frame stalled (no lane could move) and left early");
   return SKW_OK;
 }
 

@@ -110,9 +110,17 @@ namespace skw {
 // count is read on the device and n only bounds the launch size. fix: rewrite b1/b2 of hits on
 // transformed primitives in mesh space. d_sums (optional, device): [0] += node boxes tested,
 // [1] += triangles tested over all rays (selects the counting variant of the walk).
+// fixed (optional): caller-owned working memory instead of the scene's counter and the pool: the
+// launch counter, and for trees deeper than the LDS stacks a global stack area of gstack_bytes,
+// to which the launch's grid is then limited (the kernels are persistent, so any grid completes).
+struct TraceStorage {
+  uint64_t *counter;
+  uint32_t *gstack;
+  uint64_t gstack_bytes;
+};
 skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, const uint64_t *n_dev, skw_hit *d_hits,
                          uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops, uint32_t *d_tests, bool fix,
-                         uint64_t *d_sums = nullptr);
+                         uint64_t *d_sums = nullptr, const TraceStorage *fixed = nullptr);
 
 // Gather arguments for one photon map and query set, outputs unset and no heap or stack storage
 // attached (heap: nq * k slots, slot-major; stack: only when the tree is deeper than 64).

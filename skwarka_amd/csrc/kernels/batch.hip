@@ -51,6 +51,7 @@ __global__ void k_batch_reset(BatchArgs a) {
   if (i == 0) {
     *a.queue_head = 0;
     *a.frame_start = wall_clock64();
+    *a.stalled = 0;
   }
   if (i < a.n_scenes + a.n_jobs) {
     a.item_done[i] = 0;
@@ -88,9 +89,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
   uint64_t idx = 0, cursor = 0, end = 0, chunk_len = 0;
   uint32_t sid = 0xFFFFFFFFu;
   const DevBatchScene *sc = nullptr;
+  // Every iteration moves the wave forward: with no lane busy it takes a chunk or starts rays;
+  // otherwise a busy lane between leaves advances, or, when every busy lane holds a leaf
+  // (moving == 0), the leaves are tested. The conditions below are ballots, the same in every
+  // lane, so an iteration that moves nothing can only mean that lanes and ballots disagree (a
+  // lane missing from the loop): the wave then records the stall and leaves instead of spinning.
   while (true) {
+    bool moved = false;
     const uint64_t idle = __ballot(!busy);
     if (idle == ~0ull && cursor >= end) {
+      moved = true;
       // The wave's chunk is done: account for it, then take the next one.
       if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lane);
       if (GATHER) sid = 0xFFFFFFFFu;  // a gather chunk taken next must not account this scene again
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
           // ray walk uses (no lane of the wave is walking a ray now).
           const GatherArgs &ga = a.jobs[item - a.n_scenes];
           const gath::LdsStack<S> gstk{stk.base};
-          for (uint64_t qi = c.begin + lane; qi < c.end; qi += 64) gath::gather_one<0>(ga, qi, gstk);
+          for (uint64_t qi = c.begin + lane; qi < c.end; qi += 64) gath::gather_one<0, false>(ga, qi, gstk);
           account(a, item, c.end - c.begin, ga.nq, lane);
           cursor = 0;
           end = 0;
@@ -125,6 +133,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     // leaf tests wait until kRefillMin / kLeafMin lanes need them, unless nothing else can move
     // or the chunk has no rays left to start.
     if (idle != 0 && cursor < end && (idle == ~0ull || (uint32_t)__popcll(idle) >= kRefillMin)) {
+      moved = true;
       const uint64_t avail = end - cursor;
       const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
       if (!busy && rank < avail) {
@@ -141,10 +150,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     if (held) {
       const uint64_t moving = __ballot(busy && !w.at_leaf);
       if ((uint32_t)__popcll(held) >= kLeafMin || moving == 0 || cursor >= end) {
+        moved = true;
         if (busy && w.at_leaf) w.leaf();
       }
     }
     const bool nan_free = __ballot(busy && !w.nan_free) == 0;
+    moved = moved || __ballot(busy && !w.at_leaf) != 0;
+    if (!moved) {
+      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(a.stalled), 1ull);
+      break;
+    }
     if (busy && !w.at_leaf) {
       const bool done = nan_free ? w.template advance<true>() : w.template advance<false>();
       if (done) {

@@ -12,32 +12,32 @@ namespace {
 
 using namespace gath;
 
-template <int MODE, int S>
+template <int MODE, int S, bool STATS>
 __global__ __launch_bounds__(64) void k_gather_lds(GatherArgs a) {
   __shared__ uint32_t lds[S * kWave];
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   LdsStack<S> stk{lds + (threadIdx.x & (kWave - 1))};
-  gather_one<MODE>(a, i, stk);
+  gather_one<MODE, STATS>(a, i, stk);
 }
 
-template <int MODE>
+template <int MODE, bool STATS>
 __global__ __launch_bounds__(64) void k_gather_global(GatherArgs a) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   GlobalStack stk{a.gstack + i, a.nq};
-  gather_one<MODE>(a, i, stk);
+  gather_one<MODE, STATS>(a, i, stk);
 }
 
-template <int MODE>
+template <int MODE, bool STATS>
 hipError_t launch_mode(const GatherArgs &a, hipStream_t stream) {
   const uint32_t grid = (uint32_t)((a.nq + kWave - 1) / kWave);
   if (a.stack_need <= 32)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 32>), dim3(grid), dim3(kWave), 0, stream, a);
+    hipLaunchKernelGGL((k_gather_lds<MODE, 32, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   else if (a.stack_need <= 64)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 64>), dim3(grid), dim3(kWave), 0, stream, a);
+    hipLaunchKernelGGL((k_gather_lds<MODE, 64, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   else
-    hipLaunchKernelGGL((k_gather_global<MODE>), dim3(grid), dim3(kWave), 0, stream, a);
+    hipLaunchKernelGGL((k_gather_global<MODE, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -45,7 +45,8 @@ hipError_t launch_mode(const GatherArgs &a, hipStream_t stream) {
 
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream) {
   if (args.nq == 0) return hipSuccess;
-  return mode == 1 ? launch_mode<1>(args, stream) : launch_mode<0>(args, stream);
+  if (args.stats) return mode == 1 ? launch_mode<1, true>(args, stream) : launch_mode<0, true>(args, stream);
+  return mode == 1 ? launch_mode<1, false>(args, stream) : launch_mode<0, false>(args, stream);
 }
 
 namespace {

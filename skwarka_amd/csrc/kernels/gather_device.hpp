@@ -165,29 +165,43 @@ struct DiscSums {
   bool full;
 };
 
+// Work counters of one lookup (the statistics instance only): nodes visited, photons inside the
+// radius (those reach the PhotonFilter).
+struct Counts {
+  uint32_t visits, filtered;
+};
+
 // Visits node `cur` of a lookup (KDTree.h:545-575): tests its photon (strict d^2 < max_d2, then
 // the PhotonFilter, then the bounded set), then picks the children. Returns true when the node
 // has a near child (set in `cur`); `far` / `axis` / `split` describe a far child to visit later
 // when `far_exists`.
-template <int MODE, bool SUMS>
+// STORE = false: the set's slots are not written (the disc estimate's sums are kept instead); the
+// visit that would fill the set sets `filled` and the caller restarts the lookup with STORE.
+// STATS: counts visited nodes and photons that reach the PhotonFilter (inside the radius).
+template <int MODE, bool SUMS, bool STORE, bool STATS>
 __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint32_t &cur, double &max_d2,
                                       uint32_t &size, int64_t &nearest, const Heap &heap, DiscSums &sums,
-                                      bool &far_exists, uint32_t &far, uint32_t &axis, float &split) {
+                                      bool &far_exists, uint32_t &far, uint32_t &axis, float &split, bool &filled,
+                                      Counts &cnt) {
   const uint32_t k = a.k;
   // The node and its photon record are independent loads, issued together.
   const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
   const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
   double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
   double d2 = dx * dx + dy * dy + dz * dz;
+  if (STATS) ++cnt.visits;
   if (d2 < max_d2) {
     const uint32_t codes = __float_as_uint(ph.w);
+    if (STATS) ++cnt.filtered;
     bool pass = a.filter == 0 || photon_filter(a, q, codes, dx, dy, dz);
     if (pass) {
       if (MODE == 1) {
         nearest = cur;
         max_d2 = d2;
+      } else if (!STORE && size + 1 == k) {
+        filled = true;
       } else if (size < k) {
-        heap.set(size, cur, d2);
+        if (STORE) heap.set(size, cur, d2);
         ++size;
         if (SUMS) {
           if (d2 > sums.mx) sums.mx = d2;
@@ -195,12 +209,12 @@ __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint3
           const bool ext = dv[0] * q.n0 + dv[1] * q.n1 + dv[2] * q.n2 > 0.0;
           add_split(ext, ph, sums.ext0, sums.ext1, sums.ext2, sums.in0, sums.in1, sums.in2);
         }
-        if (size == k) {
+        if (STORE && size == k) {
           if (SUMS) sums.full = true;
           make_heap(heap, size);
           max_d2 = heap.d(0);
         }
-      } else {
+      } else if (STORE) {
         pop_heap(heap, size);
         heap.set(k - 1, cur, d2);
         push_heap(heap, size);
@@ -234,10 +248,13 @@ __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint3
 // push time fails at pop time too, and while the radius is unchanged the push-time test stands.
 // This visits the nodes the recursion visits (KDTree.h:558-575), in its order, without
 // re-reading the parent node at each pop.
-template <int MODE, bool SUMS, class Stack>
-__device__ __forceinline__ void lookup(const GatherArgs &a, const Query &q, double &max_d2, const Stack &stk,
-                                       uint32_t &size, int64_t &nearest, const Heap &heap, DiscSums &sums) {
-  if (a.n_nodes == 0) return;
+// Without STORE the lookup returns true, with its state incomplete, at the insertion that would
+// fill the set.
+template <int MODE, bool SUMS, bool STORE, bool STATS, class Stack>
+__device__ __forceinline__ bool lookup(const GatherArgs &a, const Query &q, double &max_d2, const Stack &stk,
+                                       uint32_t &size, int64_t &nearest, const Heap &heap, DiscSums &sums,
+                                       Counts &cnt) {
+  if (a.n_nodes == 0) return false;
   const double r2 = max_d2;
   uint32_t sp = 0;
   uint32_t cur = 0;
@@ -264,10 +281,12 @@ __device__ __forceinline__ void lookup(const GatherArgs &a, const Query &q, doub
       }
       if (!next) break;
     }
-    bool far_exists;
+    bool far_exists, filled = false;
     uint32_t far, axis;
     float split;
-    have = visit<MODE, SUMS>(a, q, cur, max_d2, size, nearest, heap, sums, far_exists, far, axis, split);
+    have = visit<MODE, SUMS, STORE, STATS>(a, q, cur, max_d2, size, nearest, heap, sums, far_exists, far, axis,
+                                          split, filled, cnt);
+    if (!STORE && filled) return true;
     if (far_exists) {
       const double qa = axis_coord(q, axis);
       const double sd = (double)split;
@@ -275,6 +294,7 @@ __device__ __forceinline__ void lookup(const GatherArgs &a, const Query &q, doub
       if (pd2 < max_d2) stk.set(sp++, (far << 2) | axis);
     }
   }
+  return false;
 }
 
 // Cone-filtered irradiance over the set (Jensen's cone filter; not an estimator of the
@@ -436,7 +456,14 @@ __device__ __forceinline__ void finish_query(const GatherArgs &a, const Query q,
 __device__ __forceinline__ bool disc_sums(const GatherArgs &a) { return a.estimate == 0 && a.e_ext != nullptr; }
 
 // One query from start to end (one lane, no refill).
-template <int MODE, class Stack>
+// With a.defer (a disc estimate whose set is not returned) the lookup first runs without writing
+// the set's slots: a set that never fills needs only the sums kept during insertion, which are the
+// estimate's sums bit for bit. Only a lookup whose set fills is run again from the start with its
+// slots written (the same visits in the same order, then make_heap and the bounded heap), so the
+// slots cost memory traffic only for sets that use them.
+// STATS writes the lookup's work counts (visits, photons inside the radius, final set size) to
+// a.stats[3 i ..]; that instance always writes the slots.
+template <int MODE, bool STATS, class Stack>
 __device__ void gather_one(const GatherArgs &a, uint64_t t, const Stack &stk) {
   const uint64_t i = a.order ? (uint64_t)a.order[t] : t;
   const Query q = load_query(a, i);
@@ -445,13 +472,27 @@ __device__ void gather_one(const GatherArgs &a, uint64_t t, const Stack &stk) {
   int64_t nearest = -1;
   const Heap heap = query_heap(a, i);
   DiscSums sums{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false};
+  Counts cnt{0u, 0u};
   if (MODE == 1) {
-    lookup<1, false>(a, q, max_d2, stk, size, nearest, heap, sums);
+    lookup<1, false, true, STATS>(a, q, max_d2, stk, size, nearest, heap, sums, cnt);
   } else if (a.k > 0) {
-    if (disc_sums(a))
-      lookup<0, true>(a, q, max_d2, stk, size, nearest, heap, sums);
-    else
-      lookup<0, false>(a, q, max_d2, stk, size, nearest, heap, sums);
+    if (disc_sums(a)) {
+      bool filled = false;
+      if (!STATS && a.defer) filled = lookup<0, true, false, false>(a, q, max_d2, stk, size, nearest, heap, sums, cnt);
+      if (STATS || !a.defer || filled) {
+        max_d2 = a.r * a.r;
+        size = 0;
+        sums = DiscSums{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false};
+        lookup<0, true, true, STATS>(a, q, max_d2, stk, size, nearest, heap, sums, cnt);
+      }
+    } else {
+      lookup<0, false, true, STATS>(a, q, max_d2, stk, size, nearest, heap, sums, cnt);
+    }
+  }
+  if (STATS) {
+    a.stats[3 * i] = cnt.visits;
+    a.stats[3 * i + 1] = cnt.filtered;
+    a.stats[3 * i + 2] = MODE == 1 ? (nearest < 0 ? 0u : 1u) : size;
   }
   finish_query<MODE>(a, q, i, heap, size, nearest, sums);
 }

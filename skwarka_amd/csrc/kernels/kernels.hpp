@@ -89,8 +89,14 @@ struct GatherArgs {
   // optional visiting order: thread i serves query order[i] (its inputs and outputs); spatially
   // sorted orders keep a wave's lookups on nearby tree paths
   const uint32_t *order;
+  // 1: a disc estimate runs its lookup first without writing the set's slots and repeats it with
+  // them only when the set fills (set by the host when neither ids nor d^2 are returned)
+  int defer;
+  // statistics instance: per query (visits, photons inside the radius, final set size)
+  uint32_t *stats;
 };
-// mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point.
+// mode 0: k nearest (+ estimate when e_ext != null), mode 1: nearest point. With args.stats the
+// counting instance runs (same lookups, same outputs).
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
 // out[i] = (a[i] + b[i]) + c[i] in float over n values; a may be null (zeros).
 hipError_t launch_irr_sum(const float *a, const float *b, const float *c, float *out, uint64_t n, hipStream_t stream);
@@ -344,6 +350,7 @@ struct BatchArgs {
   uint64_t *item_done;   // rays / queries finished per item (scenes, then jobs)
   uint64_t *item_end;    // wall clock of each item's completion
   uint64_t *frame_start; // wall clock at frame start
+  uint64_t *stalled;     // set to 1 by a wave that found an iteration in which no lane could move
   uint32_t n_scenes;
   uint32_t n_jobs;
   skw_hit *hits;

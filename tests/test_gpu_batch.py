@@ -39,6 +39,7 @@ def test_batch_hits_equal_single_scene_traversal(torch_cuda):
     scenes, views = _small_batch()
     batch = sk.Batch(scenes, views)
     batch.frame()
+    batch.check()
     torch_cuda.cuda.synchronize()
     for i, sc in enumerate(scenes):
         got = _batch_hits(torch_cuda, batch, i)
@@ -64,6 +65,7 @@ def test_batch_graph_replays_and_configs(torch_cuda):
         batch.configure(*shape)
         for _ in range(3):
             batch.frame()
+            batch.check()
             x = torch_cuda.ones(1 << 16, device="cuda")  # eager work between replays
             assert x.sum().item() == float(1 << 16)
             torch_cuda.cuda.synchronize()
@@ -114,6 +116,7 @@ def test_batch_with_photon_gathers(torch_cuda):
         batch.configure(*shape)
         for _ in range(2):
             batch.frame()
+            batch.check()
             torch.cuda.synchronize()
             for outs, ref, _, nq in refs:
                 for o, r in zip(outs, ref):

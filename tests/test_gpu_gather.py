@@ -152,3 +152,37 @@ def test_cone_and_caustic_estimates_match_oracle(torch_cuda, c1_data):
         ref = om.caustic(p, n, wo, k, r, 3, 0.5)
         assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), np.abs(got - ref).max()
         assert ref.max() > 0
+
+
+@pytest.mark.parametrize("k,r", [(100, 0.05), (8, 0.05), (32, 0.2)])
+def test_irradiance_without_sets_and_lookup_counts(torch_cuda, c1_data, k, r):
+    """The disc estimate without returned sets (slots written only for sets that fill, the lookup
+    repeated for those) equals the oracle bit for bit; the counting variant reports the oracle's
+    visits, photons inside the radius and set sizes."""
+    print("case: irradiance without sets + lookup counts, 1M photons k=%d r=%g" % (k, r), flush=True)
+    torch = torch_cuda
+    ph, om, p, n = c1_data
+    pm = sk.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
+    nq = len(p)
+    dp, dn = to_dev(torch, p), to_dev(torch, n)
+    ee, ei, rad = empty_dev(torch, nq * 12), empty_dev(torch, nq * 12), empty_dev(torch, nq * 12)
+    cnt = empty_dev(torch, nq * 4)
+    pm.irradiance(dp.data_ptr(), dn.data_ptr(), nq, k, r, 1, 0.5, ee.data_ptr(), ei.data_ptr(), rad.data_ptr(),
+                  None, None, cnt.data_ptr())
+    torch.cuda.synchronize()
+    ree, rei, _, _, rc = om.irradiance(p, n, k, r, paths=1, with_sets=False)
+    assert np.array_equal(from_dev(ee, np.uint32, nq * 3), ree.view(np.uint32).reshape(-1))
+    assert np.array_equal(from_dev(ei, np.uint32, nq * 3), rei.view(np.uint32).reshape(-1))
+    assert np.array_equal(from_dev(cnt, np.uint32, nq), rc)
+    if k == 8:
+        assert (rc == k).mean() > 0.5  # the repeated lookups of filled sets were exercised
+    if k == 100:
+        assert (rc < k).all()  # none fills: no slot written at all
+    stats = empty_dev(torch, nq * 12)
+    ee2 = empty_dev(torch, nq * 12)
+    pm.irradiance_stats(dp.data_ptr(), dn.data_ptr(), nq, k, r, 1, 0.5, ee2.data_ptr(), ei.data_ptr(), None,
+                        stats.data_ptr())
+    torch.cuda.synchronize()
+    got = from_dev(stats, np.uint32, nq * 3).reshape(nq, 3)
+    assert np.array_equal(got, om.lookup_counts(p, n, k, r))
+    assert np.array_equal(from_dev(ee2, np.uint32, nq * 3), ree.view(np.uint32).reshape(-1))

@@ -396,3 +396,33 @@ def test_caustic_simpson_kernel_on_uniform_density():
     expect = 0.5 / np.pi * e * (99.5 / 100.0)
     assert abs(rad.mean() / expect - 1.0) < 0.03, rad.mean() / expect
     assert np.all(pm.caustic(q, qn, -wo, 100, 0.2, 1, 0.5) == 0.0)
+
+
+def test_photon_lookup_counts_closed_forms():
+    """The oracle's per-lookup work counts (the C1 roofline's N_visit / N_filter / N_found): with a
+    set that never fills the radius never shrinks, so the photons inside it are exactly the
+    brute-force count within r; an unbounded radius visits every node; a zero radius visits one
+    root-to-leaf path and finds nothing."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    pos = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    nrm = wl.cdir_encode(np.asarray([v / np.linalg.norm(v) for v in rng.normal(size=(n, 3))]))
+    dirs = wl.cdir_encode(np.asarray([v / np.linalg.norm(v) for v in rng.normal(size=(n, 3))]))
+    w = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    om = oracle.PhotonMap(pos, w, dirs, nrm)
+    q = rng.uniform(-1, 1, (200, 3))
+    qn = rng.normal(size=(200, 3))
+    qn /= np.linalg.norm(qn, axis=1)[:, None]
+    r = 0.3
+    c = om.lookup_counts(q, qn, n + 1, r)
+    d2 = ((pos.astype(np.float64)[None, :, :] - q[:, None, :]) ** 2).sum(-1)
+    assert np.array_equal(c[:, 1], (d2 < r * r).sum(1))
+    _, _, _, _, found = om.irradiance(q, qn, n + 1, r, with_sets=False)
+    assert np.array_equal(c[:, 2], found) and (found <= c[:, 1]).all()
+    assert (c[:, 0] >= c[:, 1]).all()
+    big = om.lookup_counts(q[:5], qn[:5], n + 1, 100.0)
+    assert (big[:, 0] == n).all() and (big[:, 1] == n).all()
+    zero = om.lookup_counts(q[:20], qn[:20], 4, 0.0)
+    assert (zero[:, 1] == 0).all() and (zero[:, 2] == 0).all()
+    depth = int(np.ceil(np.log2(n + 1)))
+    assert (zero[:, 0] >= depth - 1).all() and (zero[:, 0] <= depth).all()
