@@ -41,6 +41,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Per-ray algorithmic bytes: ray in, hit out, 40 B per node box tested (the reference's node: float
 # bbox + child/range words) and 36 B per triangle tested (three float vertices).
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 64, 32, 40, 36
+# Per-gather algorithmic bytes: query point and normal in (48), E_ext / E_int / radiance out (36),
+# 20 B per node visited (8 B node + 12 B point), 2 B per photon reaching the PhotonFilter (its
+# normal code), 18 B per photon in the final set (12 B weight + 2 B direction code + 4 B id).
+GATHER_IN_BYTES, GATHER_OUT_BYTES, GATHER_VISIT_BYTES, GATHER_FILTER_BYTES, GATHER_FOUND_BYTES = 48, 36, 20, 2, 18
 
 
 def log(*a):
@@ -75,6 +79,33 @@ def pmc_summary(name):
 def pmc_traffic(name):
     """HBM bytes per launch from a committed rocprofv3 --pmc summary (profiles/<name>.json)."""
     return pmc_summary(name).get("hbm_bytes_per_launch")
+
+
+def source_sha(files):
+    """sha256 (first 16 hex digits) of the given kernel sources, as the PMC summaries record it."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in files:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def pmc_fields(name):
+    """The measured figures of a committed PMC summary, and whether the kernel sources it was taken
+    from are the ones this run executes (a summary of older kernels is reported as stale)."""
+    p = pmc_summary(name)
+    if not p:
+        return {"traffic": None, "pmc": None}
+    files = p.get("source_files") or []
+    current = source_sha(files) if files else None
+    return {"traffic": p.get("hbm_bytes_per_launch"),
+            "pmc": {"summary": "profiles/%s.json" % name, "l2_hit_rate": p.get("l2_hit_rate"),
+                    "valu_issue_utilisation": p.get("valu_issue_utilisation"),
+                    "valu_active_lanes_of_64": p.get("valu_active_lanes_of_64"),
+                    "hbm_write_bytes_per_launch": p.get("hbm_write_bytes_per_launch"),
+                    "profiled_source_sha": p.get("source_sha"), "current_source_sha": current,
+                    "matches_current_source": bool(files) and p.get("source_sha") == current}}
 
 
 def time_steps(torch, fn, steps, warmup, stream):
@@ -192,8 +223,21 @@ def bench_c1(torch, sk, wl, steps, warmup, stream, c0):
 
     ms = time_steps(torch, step, steps, warmup, stream)
     counts = cnt.cpu().numpy().view(np.uint32)
+    # untimed counting pass of the same lookups: nodes visited, photons inside the radius (they
+    # reach the PhotonFilter), set size per gather
+    st = torch.empty(nq * 12, dtype=torch.uint8, device="cuda")
+    pm.irradiance_stats(dp.data_ptr(), dn.data_ptr(), nq, k, r, 1, 0.5, ee.data_ptr(), ei.data_ptr(), rad.data_ptr(),
+                        st.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    c = st.cpu().numpy().view(np.uint32).reshape(nq, 3).astype(np.int64)
+    n_visit, n_filter, n_found = int(c[:, 0].sum()), int(c[:, 1].sum()), int(c[:, 2].sum())
+    assert n_found == int(counts.astype(np.int64).sum())
+    alg_bytes = (GATHER_IN_BYTES + GATHER_OUT_BYTES) * nq + GATHER_VISIT_BYTES * n_visit + \
+        GATHER_FILTER_BYTES * n_filter + GATHER_FOUND_BYTES * n_found
     return {"nq": nq, "ms": ms, "value": nq / (ms * 1e-3) / 1e6, "build_s": build_s,
-            "mean_found": float(counts.mean()), "photons": ph, "p": p, "n": nrm}
+            "mean_found": float(counts.mean()), "photons": ph, "p": p, "n": nrm, "alg_bytes": alg_bytes,
+            "achieved_gbs": alg_bytes / (ms * 1e-3) / 1e9, "mean_visits": n_visit / nq,
+            "mean_inside": n_filter / nq}
 
 
 def bench_c3(torch, sk, wl, steps, warmup, stream):
@@ -624,16 +668,31 @@ def main():
         comps["c0_closest_hit"] = {
             "metric": "Mrays/s closest-hit", "value": round(c0["value"], 3), "ms_per_step": round(c0["ms"], 5),
             "rays_per_step": c0["n"], "cpu_baseline": cpu0,
-            "roofline": {"bound": "hbm", "achieved": round(c0["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(c0["achieved_gbs"] / HBM_PEAK_GBS, 5),
-                         "traffic": pmc_traffic("closest_hit_c0_pmc"),
-                         "bytes_per_ray": round(c0["alg_bytes"] / c0["n"], 2),
-                         "mean_node_tests": round(c0["mean_pops"], 3), "mean_tri_tests": round(c0["mean_tests"], 3)},
+            "roofline": dict({"bound": "hbm", "achieved": round(c0["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(c0["achieved_gbs"] / HBM_PEAK_GBS, 5)},
+                             **pmc_fields("closest_hit_c0_pmc"),
+                             **{"limiter": "launch latency and the tail of the slowest walks (4,096 waves, one "
+                                           "round of the chip): VALU issue and active lanes in 'pmc'",
+                                "kernel": "k_trace_lds<false,false,false,32> (one launch per step)",
+                                "alg_bytes_per_launch": c0["alg_bytes"], "avg_launch_ms": round(c0["ms"], 5),
+                                "bytes_per_ray": round(c0["alg_bytes"] / c0["n"], 2),
+                                "mean_node_tests": round(c0["mean_pops"], 3),
+                                "mean_tri_tests": round(c0["mean_tests"], 3)}),
         }
     if c1 is not None:
         comps["c1_photon_gather"] = {
             "metric": "M photon-gathers/s", "value": round(c1["value"], 4), "ms_per_step": round(c1["ms"], 4),
             "gathers_per_step": c1["nq"], "mean_photons_found": round(c1["mean_found"], 3), "cpu_baseline": cpu1,
+            "roofline": dict({"bound": "hbm", "achieved": round(c1["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(c1["achieved_gbs"] / HBM_PEAK_GBS, 5)},
+                             **pmc_fields("photon_gather_c1_pmc"),
+                             **{"limiter": "latency of the lookups' dependent node loads with divergent lanes: "
+                                           "VALU issue and active lanes in 'pmc'; HBM traffic is a few percent "
+                                           "of the algorithmic bytes (L2 hits)",
+                                "kernel": "k_gather_lds<0,32,false> (one launch per step)",
+                                "alg_bytes_per_launch": c1["alg_bytes"], "avg_launch_ms": round(c1["ms"], 5),
+                                "mean_visits": round(c1["mean_visits"], 3),
+                                "mean_inside_radius": round(c1["mean_inside"], 3)}),
         }
     if c2 is not None:
         line = {
@@ -648,20 +707,21 @@ def main():
                        "nodes": c2["info"]["nodes"]},
             # The frame's dominant kernel is the traversal (primary, occlusion and BSDF launches):
             # algorithmic bytes from the counting frame over the traversal stages' event time.
-            "roofline": {"bound": "hbm", "achieved": round(c2["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(c2["achieved_gbs"] / HBM_PEAK_GBS, 5),
-                         "traffic": pmc_traffic("trace_c2_pmc"),
-                         "kernel": "k_trace_lds (closest + any hit), the 48 trace launches of a frame",
-                         "launches_per_frame": 48,
-                         "alg_bytes_per_launch": c2["alg_bytes"] // 48,
-                         "avg_launch_ms": round(c2["trace_ms"] / 48, 4),
-                         "trace_ms_per_frame": round(c2["trace_ms"], 3),
-                         "alg_bytes_per_frame": c2["alg_bytes"],
-                         "l2_hit_rate": pmc_summary("trace_c2_pmc").get("l2_hit_rate"),
-                         "valu_issue_utilisation": (pmc_summary("trace_c2_pmc").get("closest_hit") or {}).get(
-                             "valu_issue_utilisation"),
-                         "mean_node_tests": [round(x, 3) for x in c2["mean_node_tests"]],
-                         "mean_tri_tests": [round(x, 3) for x in c2["mean_tri_tests"]]},
+            "roofline": dict({"bound": "hbm", "achieved": round(c2["achieved_gbs"], 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(c2["achieved_gbs"] / HBM_PEAK_GBS, 5)},
+                             **pmc_fields("trace_c2_pmc"),
+                             **{"limiter": "FP64 VALU issue of divergent walks: VALU issue utilisation and active "
+                                           "lanes per instruction in 'pmc'; most node and triangle reads hit L2, so "
+                                           "HBM traffic ('traffic') is a fraction of the algorithmic bytes and "
+                                           "'frac' is not a DRAM figure",
+                                "kernel": "k_trace_lds (closest + any hit), the 48 trace launches of a frame",
+                                "launches_per_frame": 48,
+                                "alg_bytes_per_launch": c2["alg_bytes"] // 48,
+                                "avg_launch_ms": round(c2["trace_ms"] / 48, 4),
+                                "trace_ms_per_frame": round(c2["trace_ms"], 3),
+                                "alg_bytes_per_frame": c2["alg_bytes"],
+                                "mean_node_tests": [round(x, 3) for x in c2["mean_node_tests"]],
+                                "mean_tri_tests": [round(x, 3) for x in c2["mean_tri_tests"]]}),
             "stage_ms_per_frame": {k: round(v, 3) for k, v in c2["stage_ms"].items()},
             "cpu_baseline": cpu2,
             "components": comps,

@@ -68,3 +68,25 @@ def random_rays(n, lo, hi, seed, tmin=0.0, tmax=np.inf):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return wl.make_rays(o, d, tmin, tmax)
+
+
+def staircase(n=400, ratio=1.1):
+    """Triangles at exponentially shrinking x: the split search peels them off one at a time, so the
+    tree reaches the builder's depth limit of 200 (TriangleAccelerator.cpp:632) and the traversal
+    needs more than the 128 stack entries the LDS variants hold."""
+    xs = ratio ** (-np.arange(n, dtype=np.float64))
+    verts, tris = [], []
+    for i, x in enumerate(xs):
+        s = x * 0.05
+        verts += [[x, -1, -1], [x + s, 1, -1], [x, 1, 1]]
+        tris.append([3 * i, 3 * i + 1, 3 * i + 2])
+    return [{"verts": np.array(verts, np.float32), "tris": np.array(tris, np.uint32)}]
+
+
+def staircase_rays(n, seed):
+    """Rays along +x through the staircase, slightly spread."""
+    rng = np.random.default_rng(seed)
+    o = np.c_[np.full(n, -0.5), rng.uniform(-0.9, 0.9, (n, 2))]
+    d = np.c_[np.ones(n), rng.normal(0, 0.02, (n, 2))]
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return wl.make_rays(o, d)

@@ -178,3 +178,21 @@ def test_empty_scene_and_empty_batch(torch_cuda):
     sc2.closest_hit(0, 0, 0)  # n = 0 is a no-op even with NULL pointers
     with pytest.raises(sk.SkwError):
         sc2.closest_hit(0, 5, 0)
+
+
+def test_deepest_tree_uses_global_stack(torch_cuda):
+    """A tree at the builder's depth limit (200 levels): closest and any hit through the
+    global-memory stack variant, bit-exact against the oracle and brute force."""
+    print("case: 200-level tree, global-stack traversal, 20000 rays", flush=True)
+    meshes = scenes.staircase()
+    scene = sk.Scene(meshes)
+    info = scene.info()
+    assert info["max_depth"] == 200 and info["stack_need"] > 128
+    rays = scenes.staircase_rays(20000, 3)
+    acc = oracle.Accel(meshes)
+    ref = acc.closest_hit(rays)
+    assert_hits_equal(gpu_trace(torch_cuda, scene, rays), ref)
+    assert np.array_equal(gpu_trace(torch_cuda, scene, rays, any_hit=True), acc.any_hit(rays))
+    brute = acc.brute_closest(rays)
+    assert np.array_equal(ref["prim"], brute["prim"]) and np.array_equal(ref["tri"], brute["tri"])
+    assert (ref["prim"] != MISS).mean() > 0.5
