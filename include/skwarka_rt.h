@@ -114,6 +114,18 @@ skw_status skw_closest_hit(const skw_scene *scene, const skw_ray *d_rays, uint64
 skw_status skw_closest_hit_stats(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
                                  uint32_t *d_pops, uint32_t *d_tests, void *stream);
 
+/* skw_closest_hit that allocates nothing: no counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same kernels and bit-identical hit records as skw_closest_hit. Its
+ * working memory (the ray-claim counter and, for trees deeper than 128 levels, the traversal
+ * stack) is a fixed 16 MiB device array per GPU, held until the process ends; a tree whose
+ * stack does not fit it (more than 65,535 entries per ray, beyond any tree the builder makes) is
+ * rejected with SKW_UNSUPPORTED. Every call on one GPU shares that array: calls that overlap on one
+ * GPU (on different streams, or from different threads without ordering) race and must not be
+ * issued. */
+skw_status skw_closest_hit_noalloc(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, skw_hit *d_hits,
+                                   void *stream);
+
 /* Scene::IntersectTest / _NodeIntersectTest (TriangleAccelerator.cpp:354-468): 1 if occluded. */
 skw_status skw_any_hit(const skw_scene *scene, const skw_ray *d_rays, uint64_t n, uint8_t *d_occluded,
                        void *stream);

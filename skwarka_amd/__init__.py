@@ -25,7 +25,7 @@ SCENE_HOST_ONLY = 1
 # Every symbol include/skwarka_rt.h declares.
 EXPORTS = [
     "skw_scene_create", "skw_scene_destroy", "skw_scene_bounds", "skw_scene_info",
-    "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_any_hit", "skw_hit_geometry",
+    "skw_scene_export_tree", "skw_closest_hit", "skw_closest_hit_stats", "skw_closest_hit_noalloc", "skw_any_hit", "skw_hit_geometry",
     "skw_photon_map_create", "skw_photon_map_destroy", "skw_photon_map_export", "skw_photon_map_info",
     "skw_photon_irradiance", "skw_photon_irradiance_stats", "skw_photon_knn", "skw_photon_nearest", "skw_photon_cone_irradiance",
     "skw_photon_caustic_radiance", "skw_last_error",
@@ -131,6 +131,7 @@ def lib():
         "skw_scene_export_tree": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
         "skw_closest_hit": ([_P, _P, _U64, _P, _P], ctypes.c_int),
         "skw_closest_hit_stats": ([_P, _P, _U64, _P, _P, _P, _P], ctypes.c_int),
+        "skw_closest_hit_noalloc": ([_P, _P, _U64, _P, _P], ctypes.c_int),
         "skw_any_hit": ([_P, _P, _U64, _P, _P], ctypes.c_int),
         "skw_hit_geometry": ([_P, _P, _P, _U64, _P, _P], ctypes.c_int),
         "skw_photon_map_create": ([_U64, _P, _P, _P, _P, _U32, ctypes.POINTER(_P)], ctypes.c_int),
@@ -351,6 +352,9 @@ class Scene:
     def closest_hit_stats(self, d_rays, n, d_hits, d_pops, d_tests, stream=0):
         check(lib().skw_closest_hit_stats(self.handle, d_rays, n, d_hits, d_pops, d_tests, stream or None),
               "skw_closest_hit_stats")
+
+    def closest_hit_noalloc(self, d_rays, n, d_hits, stream=0):
+        check(lib().skw_closest_hit_noalloc(self.handle, d_rays, n, d_hits, stream or None), "skw_closest_hit_noalloc")
 
     def any_hit(self, d_rays, n, d_occ, stream=0):
         check(lib().skw_any_hit(self.handle, d_rays, n, d_occ, stream or None), "skw_any_hit")

@@ -22,6 +22,7 @@ SOURCES = [
     "abi_fg.cpp",
     "abi_shoot.cpp",
     "abi_kd.cpp",
+    "abi_noalloc.hip",
     "io.cpp",
     "pbrt_import.cpp",
     "host/builders.cpp",
