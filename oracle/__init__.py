@@ -57,6 +57,18 @@ def lib():
         "orc_photon_map_destroy": ([_P], None),
         "orc_photon_map_dump": ([_P, _P, _P, _P], None),
         "orc_photon_lookup_counts": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _P, ctypes.c_int], None),
+        "orc_blinn_d": ([ctypes.c_double, _P], ctypes.c_double),
+        "orc_fresnel_dielectric": ([ctypes.c_double], ctypes.c_double),
+        "orc_microfacet_eval": ([ctypes.c_double, _P, _P, _P, _P], ctypes.c_int),
+        "orc_microfacet_pdf": ([ctypes.c_double, _P, _P], ctypes.c_double),
+        "orc_bsdf_local_eval": ([ctypes.c_int, _P, _P, ctypes.c_double, _P, _P, _P, _P], None),
+        "orc_bsdf_local_sample": ([ctypes.c_int, _P, _P, ctypes.c_double, _P, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, _P, _P], ctypes.c_double),
+        "orc_dl_lights_cdf": ([_P, _P, _P, _P], None),
+        "orc_dl_light_sample": ([_P, _U32, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P],
+                                _U32),
+        "orc_dl_light_pdf": ([_P, _U32, _U32, _P, ctypes.c_double], ctypes.c_double),
+        "orc_dl_light_info": ([_P, _U32, _P, _P], None),
         "orc_photon_irradiance": ([_P, _U64, _P, _P, _U32, ctypes.c_double, _U64, _P, _P, _P, _P, _P, ctypes.c_int],
                                   None),
         "orc_photon_nearest": ([_P, _U64, _P, _P, ctypes.c_double, _P, ctypes.c_int], None),
@@ -408,6 +420,53 @@ def radical_inverse(n, base):
     return lib().orc_radical_inverse(n, base)
 
 
+def _v3(v):
+    return np.ascontiguousarray(v, np.float64).reshape(3)
+
+
+def blinn_d(e, h):
+    """BlinnDistribution::Evaluate: (e + 2) / (2 pi) |h_z|^e."""
+    return lib().orc_blinn_d(e, _v3(h).ctypes.data)
+
+
+def fresnel_dielectric(cos_i):
+    """FresnelDielectric(1.5, 1.0) at the cosine of incidence."""
+    return lib().orc_fresnel_dielectric(cos_i)
+
+
+def microfacet_eval(e, wo, wi):
+    """Microfacet<FresnelDielectric, Blinn>::Evaluate without the reflectance: (D G / (4 cos cos), F),
+    or None when the pair is rejected."""
+    a, b = _v3(wo), _v3(wi)
+    sc, fr = ctypes.c_double(), ctypes.c_double()
+    ok = lib().orc_microfacet_eval(e, a.ctypes.data, b.ctypes.data, ctypes.byref(sc), ctypes.byref(fr))
+    return (sc.value, fr.value) if ok else None
+
+
+def microfacet_pdf(e, wo, wi):
+    a, b = _v3(wo), _v3(wi)
+    return lib().orc_microfacet_pdf(e, a.ctypes.data, b.ctypes.data)
+
+
+def bsdf_local(plastic, kd, ks, e, wo, wi):
+    """BSDF::Evaluate and BSDF::PDF of a matte (plastic=0) or plastic surface in its local frame."""
+    kd, ks, a, b = _v3(kd), _v3(ks), _v3(wo), _v3(wi)
+    f = np.zeros(3)
+    pdf = ctypes.c_double()
+    lib().orc_bsdf_local_eval(int(plastic), kd.ctypes.data, ks.ctypes.data, e, a.ctypes.data, b.ctypes.data,
+                              f.ctypes.data, ctypes.byref(pdf))
+    return f, pdf.value
+
+
+def bsdf_local_sample(plastic, kd, ks, e, wo, u0, u1, uc):
+    """BSDF::Sample in the local frame: (wi, pdf, f)."""
+    kd, ks, a = _v3(kd), _v3(ks), _v3(wo)
+    wi, f = np.zeros(3), np.zeros(3)
+    pdf = lib().orc_bsdf_local_sample(int(plastic), kd.ctypes.data, ks.ctypes.data, e, a.ctypes.data, u0, u1, uc,
+                                      wi.ctypes.data, f.ctypes.data)
+    return wi, pdf, f
+
+
 class DLScene:
     """Direct-lighting restatement over an Accel (materials / lights as skwarka_amd.DLScene)."""
 
@@ -424,6 +483,33 @@ class DLScene:
                                                   k[3].ctypes.data, len(k[4]), k[4].ctypes.data, k[5].ctypes.data,
                                                   n_groups if grp is not None else 0, _ptr(grp))
         self.n_params = int(lib().orc_dl_param_count(self.h))
+
+    def lights_cdf(self, p, n, n_lights):
+        """IrradianceLightsSamplingStrategy::GetLightsCDF(point, normal) over the scene's lights."""
+        cdf = np.zeros(n_lights)
+        pp, nn = _v3(p), _v3(n)
+        lib().orc_dl_lights_cdf(self.h, pp.ctypes.data, nn.ctypes.data, cdf.ctypes.data)
+        return cdf
+
+    def light_sample(self, l, p, tri_u, u0, u1):
+        """DiffuseAreaLightSource::SampleLighting: (triangle, wi, distance, pdf, lit)."""
+        pp = _v3(p)
+        wi = np.zeros(3)
+        dist, pdf, lit = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        t = lib().orc_dl_light_sample(self.h, l, pp.ctypes.data, tri_u, u0, u1, wi.ctypes.data, ctypes.byref(dist),
+                                      ctypes.byref(pdf), ctypes.byref(lit))
+        return int(t), wi, dist.value, pdf.value, bool(lit.value)
+
+    def light_pdf(self, l, tri, wi, dist):
+        """DiffuseAreaLightSource::LightingPDF of a ray reaching triangle tri after dist."""
+        w = _v3(wi)
+        return lib().orc_dl_light_pdf(self.h, l, tri, w.ctypes.data, dist)
+
+    def light_info(self, l):
+        """(area, intensity) of light l."""
+        a, i = ctypes.c_double(), ctypes.c_double()
+        lib().orc_dl_light_info(self.h, l, ctypes.byref(a), ctypes.byref(i))
+        return a.value, i.value
 
     def render_replay(self, theta_eval, theta_sample, cam15, width, height, spp, light_samples=4, bsdf_samples=4,
                       seed=0, p0=0, p1=None, nthreads=None):

@@ -79,6 +79,22 @@ void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const doub
                      uint64_t seed, uint64_t p0, uint64_t p1, const double *adj, double *grad, double *img,
                      int nthreads);
 
+/* Single-function hooks for the oracle's own checks: BSDF pieces in the local frame (normal = z),
+ * FresnelDielectric(1.5, 1), the lights CDF and area-light sampling of a DL scene. */
+double orc_blinn_d(double e, const double *h);
+double orc_fresnel_dielectric(double cos_i);
+int orc_microfacet_eval(double e, const double *wo, const double *wi, double *scalar, double *fres);
+double orc_microfacet_pdf(double e, const double *wo, const double *wi);
+void orc_bsdf_local_eval(int plastic, const double *kd, const double *ks, double e, const double *wo,
+                         const double *wi, double *f, double *pdf);
+double orc_bsdf_local_sample(int plastic, const double *kd, const double *ks, double e, const double *wo, double u0,
+                             double u1, double uc, double *wi, double *f);
+void orc_dl_lights_cdf(const orc_dl_scene *h, const double *p, const double *n, double *cdf);
+uint32_t orc_dl_light_sample(const orc_dl_scene *h, uint32_t l, const double *p, double tri_u, double u0, double u1,
+                             double *wi, double *dist, double *pdf, int *lit);
+double orc_dl_light_pdf(const orc_dl_scene *h, uint32_t l, uint32_t tri, const double *wi, double dist);
+void orc_dl_light_info(const orc_dl_scene *h, uint32_t l, double *area, double *intensity);
+
 typedef struct orc_irr_map orc_irr_map;
 typedef struct orc_photon_map orc_photon_map;
 /* PhotonLTEIntegrator::_FinalGather (PhotonLTEIntegrator.cpp:387-577) at the camera hit of every sample

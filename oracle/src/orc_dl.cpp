@@ -189,6 +189,30 @@ static void lights_cdf(const DLScene &s, const double *intensity, V3 p, V3 n, do
   }
 }
 
+// DiffuseAreaLightSource::SampleLighting (DiffuseAreaLightSource.cpp:73-102): triangle by the area
+// CDF, point by UniformTriangleSampling (SamplingRoutines.h:318-325), pdf with respect to solid
+// angle = d^2 / (area |cos|); radiance only toward the lit side (normal . wi < 0).
+static size_t search_cdf(const double *b, size_t n, double u, double *pdf);
+static void light_sample(const Light &Lt, V3 p, double tri_u, double u0, double u1, size_t &ti, V3 &wi, double &len,
+                         double &lpdf, bool &lit) {
+  ti = search_cdf(Lt.cdf.data(), Lt.cdf.size(), tri_u, nullptr);
+  double tmp = std::sqrt(u0);
+  double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
+  V3 sp = add(add(mul(Lt.v0[ti], bb1), mul(Lt.v1[ti], bb2)), mul(Lt.v2[ti], 1.0 - bb1 - bb2));
+  V3 lv = sub(sp, p);
+  len = std::sqrt(len_sqr(lv));
+  wi = mul(lv, 1.0 / len);
+  double ac = std::fabs(dot(wi, Lt.normal[ti]));
+  lpdf = ac > DBL_EPS_ ? (len * len) / (Lt.area * ac) : 0.0;
+  lit = dot(Lt.normal[ti], wi) < 0.0;
+}
+// DiffuseAreaLightSource::LightingPDF (DiffuseAreaLightSource.cpp:104-116) of a ray that reaches
+// triangle tri after distance dist.
+static double light_pdf(const Light &Lt, size_t tri, V3 wi, double dist) {
+  double ac = std::fabs(dot(wi, Lt.normal[tri]));
+  return ac > DBL_EPS_ ? (dist * dist) / (Lt.area * ac) : 0.0;
+}
+
 // MathRoutines::BinarySearchCDF
 static size_t search_cdf(const double *b, size_t n, double u, double *pdf) {
   size_t lo = 0, hi = n - 1;
@@ -603,16 +627,11 @@ static bool dl_sample(const FrameSpec &F, uint64_t pix, uint32_t si, T L[3], uin
         const Light &Lt = s.lights[l];
         double tri_u = (l == 0 ? uc : uc - cdf[l - 1]) / comp_pdf;
         if (Lt.cdf.empty()) continue;
-        size_t ti = search_cdf(Lt.cdf.data(), Lt.cdf.size(), tri_u, nullptr);
-        double tmp = std::sqrt(u0);
-        double bb1 = 1.0 - tmp, bb2 = u1 * tmp;
-        V3 sp = add(add(mul(Lt.v0[ti], bb1), mul(Lt.v1[ti], bb2)), mul(Lt.v2[ti], 1.0 - bb1 - bb2));
-        V3 lv = sub(sp, p);
-        double len = std::sqrt(len_sqr(lv));
-        V3 wi = mul(lv, 1.0 / len);
-        double ac = std::fabs(dot(wi, Lt.normal[ti]));
-        double lpdf = ac > DBL_EPS_ ? (len * len) / (Lt.area * ac) : 0.0;
-        bool lit = dot(Lt.normal[ti], wi) < 0.0;
+        size_t ti;
+        V3 wi;
+        double len, lpdf;
+        bool lit;
+        light_sample(Lt, p, tri_u, u0, u1, ti, wi, len, lpdf, lit);
         const double *rs = &vs.rad[3 * l];
         double le[3] = {lit ? rs[0] : 0.0, lit ? rs[1] : 0.0, lit ? rs[2] : 0.0};
         if (!(lpdf > 0.0) || (le[0] == 0.0 && le[1] == 0.0 && le[2] == 0.0)) continue;
@@ -662,10 +681,7 @@ static bool dl_sample(const FrameSpec &F, uint64_t pix, uint32_t si, T L[3], uin
         const Light &Lt = s.lights[l2];
         const double *rs = &vs.rad[3 * l2];
         if (rs[0] == 0.0 && rs[1] == 0.0 && rs[2] == 0.0) continue;
-        V3 ln = Lt.normal[h2.tri];
-        double dist = h2.t - mint;
-        double ac = std::fabs(dot(wi, ln));
-        double lpdf = ac > DBL_EPS_ ? (dist * dist) / (Lt.area * ac) : 0.0;
+        double lpdf = light_pdf(Lt, h2.tri, wi, h2.t - mint);
         double cp = l2 == 0 ? cdf[0] : cdf[l2] - cdf[l2 - 1];
         double w = power_heuristic((double)F.nbs, bp, (double)F.nls, lpdf * cp);
         w *= std::fabs(dot(wi, ns)) / bp;
@@ -1145,6 +1161,92 @@ orc_dl_scene *orc_dl_scene_create_groups(const orc_accel *accel, const uint8_t *
 }
 
 void orc_dl_scene_destroy(orc_dl_scene *h) { delete h; }
+
+// ---- single-function hooks for the oracle's own checks (tests only) ----
+
+// A BSDF in its local frame (normal = z, geometric normal = shading normal).
+static Bsdf local_bsdf(int plastic, const double *kd, const double *ks, double e) {
+  Bsdf b;
+  b.n = V3{0.0, 0.0, 1.0};
+  b.ng = b.n;
+  b.e1 = V3{1.0, 0.0, 0.0};
+  b.e2 = V3{0.0, 1.0, 0.0};
+  b.plastic = plastic != 0;
+  for (int c = 0; c < 3; ++c) {
+    b.kd[c] = kd ? kd[c] : 0.0;
+    b.ks[c] = ks ? ks[c] : 0.0;
+  }
+  b.e = e;
+  return b;
+}
+
+double orc_blinn_d(double e, const double *h) {
+  Bsdf b = local_bsdf(1, nullptr, nullptr, e);
+  return b.blinn_d(V3{h[0], h[1], h[2]});
+}
+
+double orc_fresnel_dielectric(double cos_i) { return fresnel_dielectric(cos_i); }
+
+int orc_microfacet_eval(double e, const double *wo, const double *wi, double *scalar, double *fres) {
+  Bsdf b = local_bsdf(1, nullptr, nullptr, e);
+  return b.micro_eval(V3{wo[0], wo[1], wo[2]}, V3{wi[0], wi[1], wi[2]}, *scalar, *fres) ? 1 : 0;
+}
+
+double orc_microfacet_pdf(double e, const double *wo, const double *wi) {
+  Bsdf b = local_bsdf(1, nullptr, nullptr, e);
+  return b.micro_pdf(V3{wo[0], wo[1], wo[2]}, V3{wi[0], wi[1], wi[2]});
+}
+
+void orc_bsdf_local_eval(int plastic, const double *kd, const double *ks, double e, const double *wo,
+                         const double *wi, double *f, double *pdf) {
+  Bsdf b = local_bsdf(plastic, kd, ks, e);
+  V3 o{wo[0], wo[1], wo[2]}, i{wi[0], wi[1], wi[2]};
+  b.evaluate(o, i, f);
+  *pdf = b.pdf(o, i);
+}
+
+double orc_bsdf_local_sample(int plastic, const double *kd, const double *ks, double e, const double *wo, double u0,
+                             double u1, double uc, double *wi, double *f) {
+  Bsdf b = local_bsdf(plastic, kd, ks, e);
+  V3 w, wo_l, wi_l;
+  double p = 0.0;
+  b.sample(V3{wo[0], wo[1], wo[2]}, u0, u1, uc, w, p, f, wo_l, wi_l);
+  wi[0] = w.x;
+  wi[1] = w.y;
+  wi[2] = w.z;
+  return p;
+}
+
+// IrradianceLightsSamplingStrategy::GetLightsCDF(point, normal) with the scene's own lights.
+void orc_dl_lights_cdf(const orc_dl_scene *h, const double *p, const double *n, double *cdf) {
+  if (h->s.lights.empty()) return;
+  lights_cdf(h->s, h->nominal.intensity.data(), V3{p[0], p[1], p[2]}, V3{n[0], n[1], n[2]}, cdf);
+}
+
+// DiffuseAreaLightSource::SampleLighting of light l from point p: returns the triangle, writes the
+// direction, distance, solid-angle pdf and whether the lit side faces p.
+uint32_t orc_dl_light_sample(const orc_dl_scene *h, uint32_t l, const double *p, double tri_u, double u0, double u1,
+                             double *wi, double *dist, double *pdf, int *lit) {
+  size_t ti;
+  V3 w;
+  bool lt;
+  light_sample(h->s.lights[l], V3{p[0], p[1], p[2]}, tri_u, u0, u1, ti, w, *dist, *pdf, lt);
+  wi[0] = w.x;
+  wi[1] = w.y;
+  wi[2] = w.z;
+  *lit = lt ? 1 : 0;
+  return (uint32_t)ti;
+}
+
+double orc_dl_light_pdf(const orc_dl_scene *h, uint32_t l, uint32_t tri, const double *wi, double dist) {
+  return light_pdf(h->s.lights[l], tri, V3{wi[0], wi[1], wi[2]}, dist);
+}
+
+// Area (sum of float triangle areas), Power luminance weight (pi * area) and intensity of light l.
+void orc_dl_light_info(const orc_dl_scene *h, uint32_t l, double *area, double *intensity) {
+  *area = h->s.lights[l].area;
+  *intensity = h->s.lights[l].intensity;
+}
 
 uint32_t orc_dl_param_count(const orc_dl_scene *h) { return 2 * h->s.n_groups + (uint32_t)h->s.lights.size(); }
 
