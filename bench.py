@@ -400,6 +400,18 @@ def bench_c6(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
         times.append(time.perf_counter() - t)
     ms = 1e3 * statistics.median(times)
     photons = sum(ps.counts)
+    # the same shooting with the limits a drop-in caller passes by default: the integrator clamps
+    # every map to MAX_PHOTONS_IN_MAP = 40M (never reached by 4M paths, but finite)
+    lim, lim_times = (40_000_000,) * 3, []
+    for _ in range(max(3, steps // 2)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        q = dls.shoot_photons(n_paths, lim, per_round, 77, sptr)
+        torch.cuda.synchronize()
+        lim_times.append(time.perf_counter() - t)
+        assert q.counts == ps.counts
+        q.close()
+    ms_lim = 1e3 * statistics.median(lim_times)
     # kd build of the indirect map: on the device (from the set) and on the host (from host arrays)
     ps.photon_map(2).close()
     dev_ms, host_ms = [], []
@@ -417,7 +429,8 @@ def bench_c6(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
         pm.close()
     out = {"ms": ms, "value": n_paths / (ms * 1e-3) / 1e6, "photons": photons, "counts": ps.counts,
            "paths": ps.paths, "photons_per_s": photons / (ms * 1e-3), "spec": s,
-           "kd_n": len(m["pos"]), "kd_device_ms": statistics.median(dev_ms), "kd_host_ms": statistics.median(host_ms)}
+           "kd_n": len(m["pos"]), "kd_device_ms": statistics.median(dev_ms), "kd_host_ms": statistics.median(host_ms),
+           "ms_limits_40M": ms_lim}
     ps.close()
     return out
 
@@ -647,6 +660,7 @@ def main():
             "metric": "M photon paths/s (photon shooting from the area lights)", "value": round(c6["value"], 3),
             "ms_per_step": round(c6["ms"], 3), "photons_per_step": c6["counts"], "paths_per_map": c6["paths"],
             "photons_per_s": round(c6["photons_per_s"], 1),
+            "ms_per_step_limits_40M": round(c6["ms_limits_40M"], 3),
             "kd_build": {"photons": c6["kd_n"], "device_ms": round(c6["kd_device_ms"], 3),
                          "host_ms": round(c6["kd_host_ms"], 3)},
             "workload": "c2 scene, 4,000,000 paths in rounds of 2^20, all maps unbounded (seed 77); kd build "

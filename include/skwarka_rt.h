@@ -376,7 +376,10 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
  * round's photons and counts its paths unless it already held max_photons[map] when the round
  * began (the pipeline's input filter decides per 4096-path chunk); shooting stops at max_paths,
  * when every map is full, or after 10^7 paths when the maps that are not full are all empty.
- * Photons of a map are in (path, intersection) order. */
+ * Photons of a map are in (path, intersection) order. max_photons are taken as given: pass the
+ * integrator's clamped values, as the reference's constructor leaves them (PhotonLTEIntegrator.cpp:
+ * 42-49: a limit of 0 becomes MAX_PHOTONS_IN_MAP = 40,000,000 and larger limits are clamped to it);
+ * a limit of 0 here means the map takes no photons. */
 typedef struct skw_photon_set skw_photon_set;
 typedef struct {
   uint64_t max_paths;

@@ -3,9 +3,9 @@
 //
 // PhotonLTEIntegrator::ShootPhotons (Raytracer/LTEIntegrators/PhotonLTEIntegrator/
 // PhotonLTEIntegrator.cpp:247-288) with the input filter's stop rules and the merging filter's
-// per-map bookkeeping (PhotonShootingPipeline.cpp:48-92, 254-310) over rounds of paths (up to two
-// in flight when that cannot change which maps take photons); the lights' power CDF of
-// _GetLightsPowerCDF (PhotonLTEIntegrator.cpp:80-124).
+// per-map bookkeeping (PhotonShootingPipeline.cpp:48-92, 254-310) over rounds of paths (up to three
+// in flight; a round's done flags are checked against exact counts when it is sorted, in round
+// order); the lights' power CDF of _GetLightsPowerCDF (PhotonLTEIntegrator.cpp:80-124).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -178,8 +178,6 @@ struct RoundState {
   int64_t s_known = -1, m = 1;
 };
 
-// At most this many intersections per path (the bounce kernel's limit) bound a round's photons.
-constexpr uint64_t kMaxIsectBound = 1024;
 
 // The lane's next bounce. The host reads the counters of step k (k = 0 the start, k >= 1 bounce k)
 // from a pinned copy two steps late: active paths never increase and a path deposits at most one
@@ -236,7 +234,7 @@ bool step_lane(const skw_scene *scene, Lane &L, RoundState &R, bool block, hipEr
   return true;
 }
 
-// The shooting loop over rounds, two rounds in flight where the maps' limits allow it.
+// The shooting loop over rounds, up to three in flight.
 hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const skw_shoot_params *prm,
                         const std::vector<DevLight> &lights, const DevLight *d_lights, const DevLightTri *d_ltris,
                         const DevMat *mats, skw_photon_set *set, hipStream_t st, skw_status *rs) {
@@ -296,32 +294,25 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
   bool stop = false;
   hipEvent_t last_fin = nullptr;
 
-  // Whether a round may start now, and with which maps done: the rule (a map is done once it
-  // holds max_photons) on exact counts, decided early only when the round in flight cannot change
-  // the outcome. Returns 1 start, 0 wait, -1 no more rounds.
+  // Whether a round may start now, and with which maps done. The rule (a map is done for a round
+  // once the rounds before it hold max_photons) needs the exact counts of every earlier round; a
+  // round starts before the rounds in flight are counted, with a map marked done only when the
+  // sorted rounds already fill it (counts only grow, so such a mark is final). The prediction is
+  // checked when the round is sorted, in round order, with exact counts: a map that turned out full
+  // drops the round's photons and does not count its paths. That equals shooting the round with
+  // the map marked done, because a done map only suppresses its own deposits (and, for the
+  // indirect map, the bounces after the first intersection, which deposit into that map alone);
+  // the other maps' photons are the same. A round that finds every map full is dropped whole.
+  // Returns 1 start, 0 wait, -1 no more rounds.
   auto decide = [&](bool done[3]) -> int {
     bool other = false;
-    uint64_t bound[3] = {0, 0, 0};  // photons the rounds in flight may still add
-    for (const RoundState &r : rst) {
-      if (!(r.busy || r.finished)) continue;
-      other = true;
-      for (int j = 0; j < 3; ++j) bound[j] += r.finished ? r.P[j] : r.P[j] + r.A * (kMaxIsectBound + 2);
-    }
-    bool all_done = true, any_unknown = false;
+    for (const RoundState &r : rst) other = other || r.busy || r.finished;
+    bool all_done = true;
     for (int j = 0; j < 3; ++j) {
-      if (have[j] >= prm->max_photons[j]) {
-        done[j] = true;
-      } else if (!other) {
-        done[j] = false;
-      } else if (have[j] + bound[j] < prm->max_photons[j]) {
-        done[j] = false;
-      } else {
-        any_unknown = true;
-      }
+      done[j] = have[j] >= prm->max_photons[j];
       all_done = all_done && done[j];
     }
     if (set->total_paths >= prm->max_paths) return -1;
-    if (any_unknown) return 0;
     if (all_done) return -1;
     if (set->total_paths > 10000000ull) {
       if (other) return 0;  // the stop rule below needs exact counts
@@ -340,8 +331,15 @@ hipError_t shoot_rounds(const skw_scene *scene, const skw_dl_scene *dl, const sk
       if (!Rs.finished || Rs.index != sorted) continue;
       Lane &L = ws->lane[l];
       if (last_fin && (e = hipStreamWaitEvent(L.st, last_fin, 0)) != hipSuccess) break;
-      for (int j = 0; j < 3 && e == hipSuccess; ++j) {
-        if (Rs.done[j]) continue;
+      // the done flags the round should have had: exact counts of every earlier round
+      bool done_now[3], all_now = true;
+      for (int j = 0; j < 3; ++j) {
+        done_now[j] = have[j] >= prm->max_photons[j];
+        all_now = all_now && done_now[j];
+      }
+      if (all_now) set->total_paths -= Rs.n;  // no round would have started: its paths do not count
+      for (int j = 0; j < 3 && e == hipSuccess && !all_now; ++j) {
+        if (done_now[j]) continue;
         set->paths[j] += Rs.n;
         const uint64_t k = Rs.P[j];
         if (k == 0) continue;

@@ -9,6 +9,7 @@
 // counts it implies, and the hash; a file that fails any check is rejected with SKW_INVALID_ARG.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -149,17 +150,104 @@ void write_kd(Writer &w, const KdLayout &h) {
   w.section(4, h.psplit);
 }
 
+// The implicit kd-tree of a loaded photon map (KDTree.h:204-213): node i's left child is i + 1,
+// its right child the word's index, both after i (which rules out cycles). Every node must be
+// reached exactly once from the root and the stored depth must be the tree's: the lookups size
+// their stacks by it.
+bool check_kd(const KdLayout &h) {
+  const uint64_t n = h.nodes.size();
+  if (n == 0) return h.max_depth == 0;
+  std::vector<uint32_t> depth(n, 0);
+  std::vector<unsigned char> seen(n, 0);
+  seen[0] = 1;
+  uint32_t max_depth = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (!seen[i] || h.perm[i] >= n) return false;
+    const uint32_t word = h.nodes[i].word, right = word >> 3;
+    max_depth = std::max(max_depth, depth[i]);
+    if ((word & 3u) == 3u) continue;  // a leaf: the lookups read none of its child fields
+    if (word & 4u) {
+      if (i + 1 >= n || seen[i + 1]) return false;
+      seen[i + 1] = 1;
+      depth[i + 1] = depth[i] + 1;
+    }
+    if (right != kNoChild) {
+      if (right < i + ((word & 4u) ? 2 : 1) || right >= n || seen[right]) return false;
+      seen[right] = 1;
+      depth[right] = depth[i] + 1;
+    }
+  }
+  return max_depth == h.max_depth;
+}
+
 bool read_kd(Reader &r, KdLayout &h) {
   uint64_t n = 0;
   if (!r.pod(n) || !r.pod(h.max_depth) || n >= (uint64_t)kNoChild) return false;
   if (!r.section(1, h.nodes, n) || !r.section(2, h.photons, n) || !r.section(3, h.perm, n) ||
       !r.section(4, h.psplit, n))
     return false;
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t word = h.nodes[i].word, right = word >> 3;
-    if (h.perm[i] >= n || ((word & 3u) != 3u && right != kNoChild && right >= n)) return false;
+  return check_kd(h);
+}
+
+// Depth of the subtree at `root` (root = 0), checking every index the traversal kernels follow:
+// children blocks inside the node array, leaf triangle runs inside the triangle array, instance
+// ranges inside the instance array (and only where allow_inst), each node reached once.
+bool check_subtree(const TreeLayout &t, uint32_t root, bool allow_inst, uint32_t &depth_out) {
+  const uint64_t nn = t.nodes.size(), nt = t.tris.size(), ni = t.insts.size();
+  if (root >= nn) return false;
+  std::vector<unsigned char> seen(nn, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{root, 0u}};
+  seen[root] = 1;
+  uint32_t md = 0;
+  while (!stack.empty()) {
+    const auto [i, d] = stack.back();
+    stack.pop_back();
+    md = std::max(md, d);
+    const DevNode &nd = t.nodes[i];
+    if ((nd.b & 3u) != 0) {
+      const uint32_t mask = (nd.b >> 2) & 7u;
+      const uint32_t nch = (uint32_t)__builtin_popcount(mask);
+      if (mask == 0 || (uint64_t)nd.a + nch > nn) return false;
+      for (uint32_t c = 0; c < nch; ++c) {
+        const uint32_t k = nd.a + c;
+        if (seen[k]) return false;
+        seen[k] = 1;
+        stack.push_back({k, d + 1});
+      }
+    } else {
+      if ((uint64_t)nd.a + (nd.b >> 3) > nt) return false;
+      if (nd.b & 4u) {
+        if (!allow_inst || t.inst_range.size() != 2 * nn) return false;
+        const uint32_t ib = t.inst_range[2 * (size_t)i], ie = t.inst_range[2 * (size_t)i + 1];
+        if (ib > ie || ie > ni) return false;
+      }
+    }
   }
+  depth_out = md;
   return true;
+}
+
+// Recomputes the depths the builder records and rejects a tree that would send a traversal out of
+// its arrays or past its stack (the kernels trust these values).
+bool check_tree(const TreeLayout &t) {
+  if (t.nodes.empty()) return t.tris.empty() && t.insts.empty();
+  uint32_t top = 0;
+  if (!check_subtree(t, t.root, true, top)) return false;
+  uint32_t inst_depth = 0;
+  std::vector<uint32_t> roots;
+  for (const DevInst &in : t.insts) {
+    if (in.prim >= t.prims.size()) return false;
+    roots.push_back(in.root);
+  }
+  std::sort(roots.begin(), roots.end());
+  roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+  for (uint32_t r : roots) {
+    uint32_t d = 0;
+    if (!check_subtree(t, r, false, d)) return false;
+    inst_depth = std::max(inst_depth, d);
+  }
+  const uint32_t need = top + 1 + (t.insts.empty() ? 0 : inst_depth + 1);
+  return top == t.max_depth && need == t.stack_need;
 }
 
 }  // namespace
@@ -251,6 +339,8 @@ skw_status skw_scene_load(const char *path, uint32_t flags, skw_scene **out) {
   ok = ok && (nn == 0 || t.root < nn);
   for (uint64_t i = 0; i < nt && ok; ++i) ok = t.tris[i].prim < np || t.tris[i].prim == kMiss;  // kMiss: instanced
   if (!ok) return fail(SKW_INVALID_ARG, "scene file: inconsistent sections");
+  if (!check_tree(t))
+    return fail(SKW_INVALID_ARG, "scene file: the tree's indices or recorded depths are inconsistent");
   st = finish_scene(s, host_only);
   if (st != SKW_OK) return st;
   *out = s.release();

@@ -52,6 +52,22 @@ def test_shoot_rounds_and_full_maps(torch_cuda):
     check_maps(got, ref)
 
 
+@pytest.mark.parametrize("max_paths,limits,per_round", [
+    (300000, (0, 60000, 100000), 8192),          # both maps fill mid-run: shooting stops early
+    (150000, (BIG, 40000, BIG), 4096),           # the direct map fills, the indirect map keeps taking
+    (120000, (40_000_000,) * 3, 16384),          # the reference's clamped limits, never reached
+])
+def test_shoot_rounds_in_flight_under_finite_limits(torch_cuda, max_paths, limits, per_round):
+    """Up to three rounds run at once and each starts before the ones in flight are counted; maps
+    that those rounds fill must still drop the later rounds' photons, in round order."""
+    print("case: photon shooting, rounds in flight, limits %s, rounds of %d" % (limits, per_round), flush=True)
+    s = wl.scene_room_small(seed=5)
+    got, ref = both(s, max_paths, limits, per_round, seed=21)
+    check_maps(got, ref)
+    if limits[1] < max_paths:  # the direct map (one photon per path at most) filled mid-run
+        assert got.counts[1] >= limits[1] and got.paths[1] < got.total_paths
+
+
 def test_shoot_lit_box_closed_forms(torch_cuda):
     print("case: photon shooting, lit box (direct power, geometric indirect count)", flush=True)
     meshes = [wl.box(1.0), wl.quad((-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25))]
