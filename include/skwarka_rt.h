@@ -364,6 +364,27 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
                                const skw_fg_params *params, float *d_rgb, double *d_radiance, uint64_t *rays_out,
                                void *stream);
 
+/* The photon-mapping render: PhotonLTEIntegrator::_SurfaceRadiance (PhotonLTEIntegrator.cpp:320-385)
+ * per camera sample = ((Le + ComputeDirectLighting) + _LookupCausticRadiance) + _FinalGather, then
+ * the renderer's sample rule (NaN, negative luminance, infinite: dropped) on the sum and the pixel
+ * mean into d_rgb (3 floats per pixel). The direct-lighting part is skw_dl_render's (its sample
+ * stream, light_samples + bsdf_samples per hit), the final gather skw_dl_final_gather's over irr and
+ * the indirect map (its own stream; irr NULL: no final gather, as the reference without an
+ * irradiance map). Matte and plastic surfaces scatter no photon specularly, so the caustic map is
+ * empty and its lookup adds a zero spectrum (PhotonLTEIntegrator.cpp:585-586): caustic must be NULL
+ * or hold no photons (SKW_UNSUPPORTED otherwise). rays_out (optional, 4): camera, shadow, BSDF and
+ * gather rays. */
+typedef struct {
+  uint32_t width, height, spp;
+  uint32_t light_samples, bsdf_samples; /* per hit, 1..8 each */
+  uint32_t gather_samples;              /* 1..676, rounded up to a square */
+  uint64_t seed;
+} skw_pm_params;
+skw_status skw_pm_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_irradiance_map *irr,
+                         const skw_photon_map *caustic, const skw_photon_map *indirect, double scene_area,
+                         const skw_camera *cam, const skw_pm_params *params, float *d_rgb, uint64_t *rays_out,
+                         void *stream);
+
 /* ---------------------------------------------------------------- photon shooting
  * PhotonLTEIntegrator::ShootPhotons (PhotonLTEIntegrator.cpp:247-288) with the shooting pipeline
  * (PhotonShootingPipeline.cpp:48-267) from the area lights of a direct-lighting scene: light

@@ -80,6 +80,8 @@ def lib():
         "orc_irr_map_info": ([_P, _P, _P], None),
         "orc_irr_map_export": ([_P, _P, _P, _P, _P], None),
         "orc_irr_map_nearest": ([_P, _U64, _P, _P, ctypes.c_double, _P, ctypes.c_int], None),
+        "orc_pm_render": ([_P, _P, _P, _P, ctypes.c_double, _P, _U32, _U32, _U32, _U32, _U32, _U32, _U64, _U64, _U64,
+                           _P, _P, ctypes.c_int], None),
         "orc_fg_render": ([_P, _P, _P, _P, ctypes.c_double, _P, _U32, _U32, _U32, _U32, _U64, _U64, _U64, _P, _P,
                            ctypes.c_int], None),
         "orc_photon_caustic": ([_P, _U64, _P, _P, _P, _U32, ctypes.c_double, _U64, ctypes.c_double, _P,
@@ -393,6 +395,21 @@ def fg_render(accel, dls, irr, indirect, scene_area, cam, width, height, spp, ga
                         cam.ctypes.data, width, height, spp, gather_samples, seed, p0, p1, out.ctypes.data,
                         ctypes.byref(rays), nthreads or threads())
     return out, rays.value
+
+
+def pm_render(accel, dls, irr, indirect, scene_area, cam, width, height, spp, light_samples, bsdf_samples,
+              gather_samples, seed, p0=0, p1=None, nthreads=None):
+    """Photon-mapping render (direct lighting + caustic + final gather per sample): (rgb (P, 3) float32
+    for pixels [p0, p1), counters [primary, shadow, BSDF, gather rays]). irr may be None."""
+    p1 = width * height if p1 is None else p1
+    rgb = np.zeros((p1 - p0, 3), np.float32)
+    cnt = np.zeros(4, np.uint64)
+    cam = np.ascontiguousarray(cam, np.float64)
+    lib().orc_pm_render(accel.h, dls.h, irr.h if irr is not None else None,
+                        indirect.h if indirect is not None else None, scene_area, cam.ctypes.data, width, height,
+                        spp, light_samples, bsdf_samples, gather_samples, seed, p0, p1, rgb.ctypes.data,
+                        cnt.ctypes.data, nthreads or threads())
+    return rgb, cnt
 
 
 def photon_shoot(accel, dls, max_paths, max_photons, paths_per_round=4096, seed=0, nthreads=None):

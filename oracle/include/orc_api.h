@@ -105,6 +105,16 @@ void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_ir
                    uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1,
                    double *radiance, uint64_t *gather_rays, int nthreads);
 
+/* PhotonLTEIntegrator::_SurfaceRadiance for matte / plastic scenes: ((Le + direct) + caustic (zero:
+ * no specular surfaces)) + final gather per sample, the renderer's sample rule on the sum, pixel
+ * means as float rgb over pixels [p0, p1). irr may be NULL (no final gather). counters: primary,
+ * shadow, BSDF, gather rays. */
+void orc_pm_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                   uint32_t height, uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples,
+                   uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1, float *rgb, uint64_t *counters,
+                   int nthreads);
+
 /* Generic point kd-tree (KDTree<Point3D_d> / KDTree<Point3D_f>) for the reference's own tests. */
 orc_kdtree *orc_kdtree_create_d(uint64_t n, const double *pts);
 orc_kdtree *orc_kdtree_create_f(uint64_t n, const float *pts);

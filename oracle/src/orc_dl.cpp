@@ -1357,13 +1357,22 @@ void orc_dl_gradient(const orc_accel *accel, const orc_dl_scene *dls, const doub
   for (uint32_t j = 0; j < n_params; ++j) grad[j] = total[j];
 }
 
-void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
-                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
-                   uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1,
-                   double *radiance, uint64_t *gather_rays, int nthreads) {
-  std::vector<double> table(3 * 65536);
-  orc_cdir_table(table.data());
+}  // extern "C"
+
+// The tables a final gather reads (direction decode table, indirect photons' direction codes by
+// caller index, irradiance photons' estimates) and the FgSpec over them.
+struct FgData {
+  std::vector<double> table;
   std::vector<uint16_t> dir_by_id;
+  std::vector<float> e_ext, e_int;
+  FgSpec F;
+};
+
+static void init_fg(FgData &D, const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                    const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                    uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed) {
+  D.table.assign(3 * 65536, 0.0);
+  orc_cdir_table(D.table.data());
   uint64_t n_ind = indirect ? orc_photon_map_size(indirect) : 0;
   if (n_ind > 0) {
     std::vector<float> pos(3 * n_ind);
@@ -1371,24 +1380,25 @@ void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_ir
     std::vector<uint32_t> perm(n_ind);
     orc_photon_map_points(indirect, pos.data(), dir.data(), nullptr);
     orc_photon_map_dump(indirect, perm.data(), nullptr, nullptr);
-    dir_by_id.assign(n_ind, 0);
-    for (uint64_t i = 0; i < n_ind; ++i) dir_by_id[perm[i]] = dir[i];
+    D.dir_by_id.assign(n_ind, 0);
+    for (uint64_t i = 0; i < n_ind; ++i) D.dir_by_id[perm[i]] = dir[i];
   }
   uint64_t n_irr = 0;
   double irr_max = 0.0;
   orc_irr_map_info(irr, &n_irr, &irr_max);
-  std::vector<float> e_ext(3 * n_irr), e_int(3 * n_irr);
-  orc_irr_map_export(irr, nullptr, nullptr, e_ext.data(), e_int.data());
-  FgSpec F;
+  D.e_ext.assign(3 * n_irr, 0.0f);
+  D.e_int.assign(3 * n_irr, 0.0f);
+  orc_irr_map_export(irr, nullptr, nullptr, D.e_ext.data(), D.e_int.data());
+  FgSpec &F = D.F;
   F.accel = accel;
   F.s = &dls->s;
   F.v = &dls->nominal;
   F.irr = irr;
   F.indirect = indirect;
-  F.dir_by_id = &dir_by_id;
-  F.cdir = table.data();
-  F.irr_ext = &e_ext;
-  F.irr_int = &e_int;
+  F.dir_by_id = &D.dir_by_id;
+  F.cdir = D.table.data();
+  F.irr_ext = &D.e_ext;
+  F.irr_int = &D.e_int;
   F.irr_max_dist = irr_max;
   // PhotonLTEIntegrator.cpp:460-462: the radius holding 32 indirect photons on average
   F.lookup_r = n_ind > 0 ? std::sqrt((scene_area / (double)n_ind) * 32 * INV_PI_) : 0.0;
@@ -1399,6 +1409,17 @@ void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_ir
   F.gs_sqrt = (uint32_t)std::ceil(std::sqrt((double)gather_samples));
   F.gs = F.gs_sqrt * F.gs_sqrt;
   F.seed = seed;
+}
+
+extern "C" {
+
+void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                   uint32_t height, uint32_t spp, uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1,
+                   double *radiance, uint64_t *gather_rays, int nthreads) {
+  FgData D;
+  init_fg(D, accel, dls, irr, indirect, scene_area, cam, width, height, spp, gather_samples, seed);
+  const FgSpec &F = D.F;
   uint64_t rays = 0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : rays)
   for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix)
@@ -1408,6 +1429,49 @@ void orc_fg_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_ir
       for (int c = 0; c < 3; ++c) radiance[3 * (((uint64_t)pix - p0) * spp + si) + c] = L[c];
     }
   if (gather_rays) *gather_rays = rays;
+}
+
+// PhotonLTEIntegrator::_SurfaceRadiance (PhotonLTEIntegrator.cpp:320-385) for matte and plastic
+// surfaces, one camera sample at a time: ((Le + ComputeDirectLighting) + _LookupCausticRadiance)
+// + _FinalGather, then the renderer's NaN / negative-luminance / infinite rule on the sum
+// (SamplerBasedRenderer.cpp:458-478) and the pixel mean. Matte and plastic surfaces scatter no
+// photon specularly, so the caustic map holds no photons and its lookup adds a zero spectrum
+// (PhotonLTEIntegrator.cpp:585-586). irr may be NULL (no final gather). The direct-lighting part
+// draws from the direct-lighting stream, the final gather from its own.
+void orc_pm_render(const orc_accel *accel, const orc_dl_scene *dls, const orc_irr_map *irr,
+                   const orc_photon_map *indirect, double scene_area, const double *cam, uint32_t width,
+                   uint32_t height, uint32_t spp, uint32_t n_light_samples, uint32_t n_bsdf_samples,
+                   uint32_t gather_samples, uint64_t seed, uint64_t p0, uint64_t p1, float *rgb, uint64_t *counters,
+                   int nthreads) {
+  FrameSpec F{accel, &dls->s, cam, width, height, spp, n_light_samples, n_bsdf_samples, seed, &dls->nominal,
+              &dls->nominal};
+  FgData D;
+  if (irr) init_fg(D, accel, dls, irr, indirect, scene_area, cam, width, height, spp, gather_samples, seed);
+  uint64_t c_prim = 0, c_shadow = 0, c_bsdf = 0, c_gather = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : c_prim, c_shadow, c_bsdf, c_gather)
+  for (int64_t pix = (int64_t)p0; pix < (int64_t)p1; ++pix) {
+    double acc[3] = {0, 0, 0};
+    for (uint32_t si = 0; si < spp; ++si) {
+      double L[3], Lg[3] = {0.0, 0.0, 0.0};
+      uint32_t hp, ht;
+      (void)dl_sample<double>(F, (uint64_t)pix, si, L, hp, ht, c_shadow, c_bsdf);
+      ++c_prim;
+      if (irr) c_gather += fg_sample(D.F, (uint64_t)pix, si, Lg);
+      double T[3];
+      for (int c = 0; c < 3; ++c) T[c] = hp != 0xffffffffu ? (L[c] + 0.0) + Lg[c] : L[c];
+      const bool bad = std::isnan(T[0]) || std::isnan(T[1]) || std::isnan(T[2]) || luminance(T) < -DBL_EPS_ ||
+                       std::isinf(T[0]) || std::isinf(T[1]) || std::isinf(T[2]);
+      if (!bad)
+        for (int c = 0; c < 3; ++c) acc[c] += T[c];
+    }
+    for (int c = 0; c < 3; ++c) rgb[3 * (pix - p0) + c] = (float)(acc[c] / (double)spp);
+  }
+  if (counters) {
+    counters[0] = c_prim;
+    counters[1] = c_shadow;
+    counters[2] = c_bsdf;
+    counters[3] = c_gather;
+  }
 }
 
 // PhotonLTEIntegrator::ShootPhotons (PhotonLTEIntegrator.cpp:247-288) with the pipeline's input

@@ -35,7 +35,7 @@ EXPORTS = [
     "skw_batch_create", "skw_batch_create_with_gathers", "skw_batch_destroy", "skw_batch_configure", "skw_batch_autotune", "skw_batch_frame",
     "skw_batch_hits", "skw_batch_latency", "skw_batch_check", "skw_batch_rays", "skw_batch_ray_count",
     "skw_irradiance_map_create", "skw_irradiance_map_create_from", "skw_irradiance_map_destroy",
-    "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather",
+    "skw_irradiance_map_info", "skw_irradiance_map_export", "skw_dl_final_gather", "skw_pm_render",
     "skw_photon_shoot", "skw_photon_set_info", "skw_photon_set_export", "skw_photon_set_map",
     "skw_photon_set_destroy", "skw_photon_map_build",
     "skw_scene_save", "skw_scene_load", "skw_photon_map_save", "skw_photon_map_load", "skw_irradiance_map_save",
@@ -89,6 +89,11 @@ class IrrMapDesc(ctypes.Structure):
 
 class FGParams(ctypes.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("gather_samples", _U32), ("seed", _U64)]
+
+
+class PMParams(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("spp", _U32), ("light_samples", _U32), ("bsdf_samples", _U32),
+                ("gather_samples", _U32), ("seed", _U64)]
 
 
 class ShootParams(ctypes.Structure):
@@ -199,6 +204,8 @@ def lib():
         "skw_pbrt_destroy": ([_P], ctypes.c_int),
         "skw_dl_final_gather": ([_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(FGParams),
                                  _P, _P, _P, _P], ctypes.c_int),
+        "skw_pm_render": ([_P, _P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(Camera), ctypes.POINTER(PMParams),
+                           _P, _P, _P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -739,6 +746,19 @@ class DLScene:
                                         ctypes.byref(cam), ctypes.byref(prm), d_rgb, d_radiance,
                                         rays.ctypes.data if count_rays else None, stream or None),
               "skw_dl_final_gather")
+        return rays
+
+    def pm_render(self, irr, caustic, indirect, scene_area, cam, width, height, spp, d_rgb, light_samples=4,
+                  bsdf_samples=4, gather_samples=8, seed=0, stream=0, count_rays=True):
+        """skw_pm_render: Le + direct lighting + caustic + final gather per sample. Returns (camera,
+        shadow, BSDF, gather rays) when count_rays, else only enqueues the frame. irr may be None."""
+        prm = PMParams(width, height, spp, light_samples, bsdf_samples, gather_samples, seed)
+        rays = np.zeros(4, np.uint64) if count_rays else None
+        check(lib().skw_pm_render(self.scene.handle, self.handle, irr.handle if irr is not None else None,
+                                  caustic.handle if caustic is not None else None,
+                                  indirect.handle if indirect is not None else None, scene_area, ctypes.byref(cam),
+                                  ctypes.byref(prm), d_rgb, rays.ctypes.data if count_rays else None, stream or None),
+              "skw_pm_render")
         return rays
 
     def render_profiled(self, cam, width, height, spp, d_rgb, light_samples=4, bsdf_samples=4, seed=0,

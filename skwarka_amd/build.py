@@ -20,6 +20,7 @@ SOURCES = [
     "abi_dl.cpp",
     "abi_batch.cpp",
     "abi_fg.cpp",
+    "abi_pm.cpp",
     "abi_shoot.cpp",
     "abi_kd.cpp",
     "abi_noalloc.hip",

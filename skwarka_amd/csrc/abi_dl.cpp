@@ -63,6 +63,7 @@ namespace {
 // What a frame produces: a float image, a double image, (adj set) a gradient, or (tape set) a
 // double image and the tape.
 struct FrameOut {
+  double *sample_L = nullptr;  // per-sample Le + direct lighting (no film, no image)
   float *rgb = nullptr;
   double *img = nullptr;
   const double *adj = nullptr;
@@ -198,6 +199,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   a.counts = (uint64_t *)(base + o_counts);
   a.totals = (uint64_t *)(base + o_totals);
   a.film = (double *)(base + o_film);
+  a.sample_L = out.sample_L;
   if (adj || out.tape) {
     a.adj = out.adj;
     a.grad = out.grad;
@@ -430,6 +432,23 @@ skw_status skw_dl_scene_destroy(skw_dl_scene *dl) {
   delete dl;
   return SKW_OK;
 }
+
+}  // extern "C"
+
+namespace skw {
+// Per-sample Le + direct lighting (3 doubles per sample, pixel * spp + pass) of a plain frame,
+// before the renderer's sample rule; rays_out: primary, shadow, BSDF rays.
+skw_status dl_sample_radiance(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
+                              const skw_dl_params *prm, double *d_sample_L, uint64_t *rays_out, hipStream_t st) {
+  skw_status s = check_frame_args(scene, dl, cam, prm);
+  if (s != SKW_OK) return s;
+  FrameOut out;
+  out.sample_L = d_sample_L;
+  return run_frame(scene, dl, ParamPlan(), cam, prm, out, rays_out, st);
+}
+}  // namespace skw
+
+extern "C" {
 
 skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const skw_camera *cam,
                          const skw_dl_params *prm, float *d_rgb, uint64_t *rays_out, void *stream) {

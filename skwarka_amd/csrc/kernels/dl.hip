@@ -673,7 +673,13 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
   for (int c = 0; c < 3; ++c) lum += kLumW[c] * L[c];
   bool bad = isnan(L[0]) || isnan(L[1]) || isnan(L[2]) || lum < -kEps || isinf(L[0]) || isinf(L[1]) || isinf(L[2]);
   if (!ADJ) {
-    if (in && !bad) {
+    if (a.sample_L) {
+      // the sum before the renderer's rule: the composition adds the other terms first
+      if (in) {
+        double *o = a.sample_L + 3 * (pixel_of(a, i) * a.spp + a.pass);
+        for (int c = 0; c < 3; ++c) o[c] = L[c];
+      }
+    } else if (in && !bad) {
       double *f = a.film + 3 * pixel_of(a, i);
       for (int c = 0; c < 3; ++c) f[c] += L[c];
     }
@@ -838,6 +844,30 @@ __global__ __launch_bounds__(256) void k_dl_finish(DlArgs a, float *rgb) {
   for (int c = 0; c < 3; ++c) rgb[3 * pix + c] = (float)(a.film[3 * pix + c] / (double)a.spp);
 }
 
+// PhotonLTEIntegrator::_SurfaceRadiance (PhotonLTEIntegrator.cpp:320-385) per sample: ((Le + direct)
+// + caustic) + final gather, where the caustic lookup of a photon-less caustic map is a zero
+// spectrum (matte and plastic surfaces scatter nothing specularly); a missed camera ray has 0 in
+// both inputs. Then the renderer's rule (SamplerBasedRenderer.cpp:458-478: NaN, negative
+// luminance or infinite samples are dropped), the passes summed in order, and the pixel mean.
+__global__ __launch_bounds__(256) void k_pm_combine(const double *sample_dl, const double *sample_fg, uint64_t npix,
+                                                    uint32_t spp, float *rgb) {
+  const uint64_t pix = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= npix) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (uint32_t s = 0; s < spp; ++s) {
+    const uint64_t k = 3 * (pix * spp + s);
+    double L[3];
+    for (int c = 0; c < 3; ++c) L[c] = (sample_dl[k + c] + 0.0) + (sample_fg ? sample_fg[k + c] : 0.0);
+    double lum = 0.0;
+    for (int c = 0; c < 3; ++c) lum += kLumW[c] * L[c];
+    const bool bad = isnan(L[0]) || isnan(L[1]) || isnan(L[2]) || lum < -kEps || isinf(L[0]) || isinf(L[1]) ||
+                     isinf(L[2]);
+    if (!bad)
+      for (int c = 0; c < 3; ++c) acc[c] += L[c];
+  }
+  for (int c = 0; c < 3; ++c) rgb[3 * pix + c] = (float)(acc[c] / (double)spp);
+}
+
 __global__ __launch_bounds__(256) void k_dl_finish_f64(DlArgs a, double *img) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.batch) return;
@@ -971,6 +1001,14 @@ hipError_t launch_dl_params(const DlParamArgs &a, hipStream_t stream) {
   uint32_t grid = (n + 255) / 256;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dl_params, dim3(grid), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_combine(const double *sample_dl, const double *sample_fg, uint64_t npix, uint32_t spp,
+                             float *rgb, hipStream_t stream) {
+  const uint32_t grid = (uint32_t)((npix + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pm_combine, dim3(grid), dim3(256), 0, stream, sample_dl, sample_fg, npix, spp, rgb);
   return hipGetLastError();
 }
 

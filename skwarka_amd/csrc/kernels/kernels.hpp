@@ -99,6 +99,10 @@ struct GatherArgs {
 // counting instance runs (same lookups, same outputs).
 hipError_t launch_gather(const GatherArgs &args, int mode, hipStream_t stream);
 // out[i] = (a[i] + b[i]) + c[i] in float over n values; a may be null (zeros).
+// Photon-mapping composition per pixel: for every pass, ((direct + 0) + gather) with the renderer's
+// sample rule, summed in pass order; rgb = float(sum / spp).
+hipError_t launch_pm_combine(const double *sample_dl, const double *sample_fg, uint64_t npix, uint32_t spp,
+                             float *rgb, hipStream_t stream);
 hipError_t launch_irr_sum(const float *a, const float *b, const float *c, float *out, uint64_t n, hipStream_t stream);
 
 // ------------------------------------------------------------------ direct lighting
@@ -162,6 +166,8 @@ struct DlArgs {
   uint8_t *occluded;
   skw_hit *bsdf_hits;
   double *film;          // 3 per pixel, running sum over passes
+  double *sample_L;      // when set: 3 per sample (pixel * spp + pass), Le + direct lighting before the
+                         // renderer's sample rule, instead of the film (photon-mapping composition)
   uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device); [3..8]
                          // node / triangle tests of the three ray classes (counting frames)
   // adjoint (when adj is set): dLoss/dImage per pixel, gradient output, per-sample records
