@@ -258,11 +258,36 @@ def bench_c3(torch, sk, wl, steps, warmup, stream):
     ms = time_steps(torch, lambda: batch.frame(sptr), steps, warmup, stream)
     lat = batch.latency_ns() / 1e6  # completion of each scene in the last frame, ms after its start
     tris = [int(s.info()["triangles"]) for s in scenes]
+    # The same 64 scenes with 8 photon-gather jobs in the same persistent, graph-replayed launch.
+    jobs = wl.batch_gather_jobs(specs)
+    gathers, keep = [], []
+    for jb in jobs:
+        pm = sk.PhotonMap(jb["ph"]["pos"], jb["ph"]["weight"], jb["ph"]["dir"], jb["ph"]["normal"])
+        nq = len(jb["p"])
+        dp, dn = dev_bytes(torch, jb["p"]), dev_bytes(torch, jb["n"])
+        outs = [torch.empty(nq * 12, dtype=torch.uint8, device="cuda") for _ in range(3)]
+        gathers.append((pm, dp.data_ptr(), dn.data_ptr(), nq, jb["k"], jb["r"], jb["paths"], jb["rho"],
+                        outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr()))
+        keep.append((pm, dp, dn, outs))
+    bg = sk.Batch(scenes, views, gathers)
+    cfg_g, _ = bg.autotune(frames=1)
+    ms_g = time_steps(torch, lambda: bg.frame(sptr), steps, warmup, stream)
+    lat_g = bg.latency_ns() / 1e6
+    bg.check()
+    nq_total = sum(len(jb["p"]) for jb in jobs)
+    with_g = {"ms": ms_g, "rays_per_s": n / (ms_g * 1e-3) / 1e6, "gathers_per_s": nq_total / (ms_g * 1e-3) / 1e6,
+              "config": cfg_g, "jobs": len(jobs), "queries": nq_total,
+              "p50_ms": float(np.percentile(lat_g, 50)), "p99_ms": float(np.percentile(lat_g, 99)),
+              "job_p50_ms": float(np.percentile(lat_g[len(scenes):], 50)),
+              "job_max_ms": float(lat_g[len(scenes):].max())}
+    bg.close()
+    for pm, *_ in keep:
+        pm.close()
     return {"ms": ms, "value": n / (ms * 1e-3) / 1e6, "rays": n, "config": cfg, "tune_ms": tune_ms,
             "tune_s": tune_s, "build_s": build_s, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "n_scenes": len(scenes),
             "tri_range": [min(tris), max(tris)], "tris_total": sum(tris), "specs": specs, "batch": batch,
-            "scenes": scenes}
+            "scenes": scenes, "with_gathers": with_g}
 
 
 def bench_c4(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
@@ -435,6 +460,71 @@ def bench_c6(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
     return out
 
 
+def bench_c7(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
+    """C2's photon-mapping render: direct lighting + caustic + final gather per sample at 1024x1024,
+    16 spp, 4 light + 4 BSDF samples, 8 (9) gather samples, over maps shot from the room's lights
+    (2^20 paths), built on the device, and their irradiance map."""
+    s = spec if spec is not None else wl.scene_room()
+    if scene is None:
+        scene = sk.Scene(s["meshes"])
+    dls = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    sptr = stream.cuda_stream
+    area = wl.scene_area(s["meshes"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ps = dls.shoot_photons(1 << 20, (40_000_000,) * 3, 1 << 20, 77, sptr)
+    maps = [ps.photon_map(j) if ps.counts[j] > 0 else None for j in range(3)]
+    paths = [int(x) for x in ps.paths]
+    irr = sk.IrradianceMap(direct=maps[1], indirect=maps[2], caustic=maps[0], caustic_paths=paths[0],
+                           direct_paths=paths[1], indirect_paths=paths[2], scene_area=area, stream=sptr)
+    torch.cuda.synchronize()
+    maps_ms = 1e3 * (time.perf_counter() - t0)
+    w = h = 1024
+    spp, gs = 16, 8
+    cam, cam15 = sk.camera(*s["camera"], w, h)
+    d_rgb = torch.empty(w * h * 12, dtype=torch.uint8, device="cuda")
+
+    def step():
+        dls.pm_render(irr, maps[0], maps[2], area, cam, w, h, spp, d_rgb.data_ptr(), 4, 4, gs, 1234, sptr,
+                      count_rays=False)
+
+    ms = time_steps(torch, step, steps, warmup, stream)
+    rays = dls.pm_render(irr, maps[0], maps[2], area, cam, w, h, spp, d_rgb.data_ptr(), 4, 4, gs, 1234, sptr)
+    rgb = d_rgb.cpu().numpy().view(np.float32)
+    exported = [ps.export(j) if ps.counts[j] > 0 else None for j in range(3)]
+    out = {"ms": ms, "value": int(rays.sum()) / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays],
+           "samples_per_s": w * h * spp / (ms * 1e-3), "mean_rgb": float(rgb.mean()), "maps_ms": maps_ms,
+           "photons": [int(x) for x in ps.counts], "paths": paths, "irr_n": irr.n, "spec": s, "cam15": cam15,
+           "area": area, "exported": exported}
+    for m in maps:
+        if m is not None:
+            m.close()
+    irr.close()
+    ps.close()
+    return out
+
+
+def cpu_baseline_c7(c7, threads, rows):
+    import oracle
+    s = c7["spec"]
+    acc = oracle.Accel(s["meshes"])
+    dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    om = [oracle.PhotonMap(m["pos"], m["weight"], m["dir"], m["normal"]) if m is not None else None
+          for m in c7["exported"]]
+    oirr = oracle.IrradianceMap(direct=om[1], indirect=om[2], caustic=om[0], paths=tuple(c7["paths"]),
+                                scene_area=c7["area"], nthreads=threads)
+    w = 1024
+    r0 = 512 - rows // 2
+    t = time.perf_counter()
+    _, cnt = oracle.pm_render(acc, dls, oirr, om[2], c7["area"], c7["cam15"], w, 1024, 16, 4, 4, 8, 1234, r0 * w,
+                              (r0 + rows) * w, nthreads=threads)
+    dt = time.perf_counter() - t
+    n = int(cnt.sum())
+    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "image rows %d-%d of the photon-mapping frame (%d pixels x 16 spp, %d camera + shadow + BSDF + "
+                      "gather rays), one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
+
+
 def cpu_baseline_c6(c6, threads, n_paths):
     import oracle
     s = c6["spec"]
@@ -547,7 +637,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6"], default=None,
+    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6", "c7"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
     ap.add_argument("--cpu-rows", type=int, default=1024,
@@ -599,6 +689,9 @@ def main():
         c3 = bench_c3(torch, sk, wl, 5, 1, stream)
         log("c3: %.2f ms/frame, %.1f Mrays/s, p50 %.2f ms p99 %.2f ms, config %s" %
             (c3["ms"], c3["value"], c3["p50_ms"], c3["p99_ms"], c3["config"]))
+        g = c3["with_gathers"]
+        log("c3 + gathers: %.2f ms/frame, %.1f Mrays/s + %.1f M gathers/s, item p50 %.2f ms p99 %.2f ms" %
+            (g["ms"], g["rays_per_s"], g["gathers_per_s"], g["p50_ms"], g["p99_ms"]))
     if run("c4"):
         log("c4: direct-lighting gradient, 512x512 x 8 spp, 132 parameters")
         c4 = bench_c4(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
@@ -614,7 +707,13 @@ def main():
         c6 = bench_c6(torch, sk, wl, 10, 2, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
         log("c6: %.1f ms per 4M paths, %.1f M paths/s, photons %s, kd build %d photons: device %.1f ms, host %.1f ms"
             % (c6["ms"], c6["value"], c6["counts"], c6["kd_n"], c6["kd_device_ms"], c6["kd_host_ms"]))
-    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = None
+    c7 = None
+    if run("c7"):
+        log("c7: photon-mapping render (direct + caustic + final gather), 1024x1024 x 16 spp, shot maps")
+        c7 = bench_c7(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
+        log("c7: %.1f ms/frame, %.1f Mrays/s, rays %s, maps %.1f ms (photons %s)"
+            % (c7["ms"], c7["value"], c7["rays"], c7["maps_ms"], c7["photons"]))
+    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = cpu7 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
         if c2 is not None:
@@ -631,6 +730,8 @@ def main():
             cpu5 = cpu_baseline_c5(c5, threads, 256)
         if c6 is not None:
             cpu6 = cpu_baseline_c6(c6, threads, 4_000_000)
+        if c7 is not None:
+            cpu7 = cpu_baseline_c7(c7, threads, 64)
     comps = {}
     if c3 is not None:
         comps["c3_batch"] = {
@@ -639,6 +740,18 @@ def main():
             "scene_latency_ms": {"p50": round(c3["p50_ms"], 4), "p99": round(c3["p99_ms"], 4),
                                  "max": round(c3["max_ms"], 4)},
             "autotuned": c3["config"], "autotune_s": round(c3["tune_s"], 2),
+            "with_photon_gathers": {
+                "ms_per_step": round(c3["with_gathers"]["ms"], 4),
+                "Mrays_per_s": round(c3["with_gathers"]["rays_per_s"], 3),
+                "M_gathers_per_s": round(c3["with_gathers"]["gathers_per_s"], 3),
+                "item_latency_ms": {"p50": round(c3["with_gathers"]["p50_ms"], 4),
+                                    "p99": round(c3["with_gathers"]["p99_ms"], 4),
+                                    "gather_jobs_p50": round(c3["with_gathers"]["job_p50_ms"], 4),
+                                    "gather_jobs_max": round(c3["with_gathers"]["job_max_ms"], 4)},
+                "autotuned": c3["with_gathers"]["config"],
+                "workload": "the 64 scenes' camera rays plus %d photon-gather jobs (%d queries; 250k photons "
+                            "each, k=100, radius holding 40 photons on average, disc estimate) in one persistent "
+                            "launch per frame" % (c3["with_gathers"]["jobs"], c3["with_gathers"]["queries"])},
             "workload": "%d scenes, %d-%d triangles (%d total), 128/256/512 square frames, 4 spp" %
                         (c3["n_scenes"], c3["tri_range"][0], c3["tri_range"][1], c3["tris_total"]),
             "cpu_baseline": cpu3,
@@ -666,6 +779,18 @@ def main():
             "workload": "c2 scene, 4,000,000 paths in rounds of 2^20, all maps unbounded (seed 77); kd build "
                         "of the indirect map on the device and on the host (same tree)",
             "cpu_baseline": cpu6,
+        }
+    if c7 is not None:
+        comps["c7_photon_mapping"] = {
+            "metric": "Mrays/s (photon-mapping render: camera + shadow + BSDF + gather rays)",
+            "value": round(c7["value"], 3), "ms_per_step": round(c7["ms"], 3), "rays_per_step": c7["rays"],
+            "samples_per_s": round(c7["samples_per_s"], 1),
+            "maps": {"paths": 1 << 20, "photons": c7["photons"], "irradiance_photons": c7["irr_n"],
+                     "shoot_build_ms": round(c7["maps_ms"], 2)},
+            "workload": "c2 scene at 1024x1024, 16 spp, 4 light + 4 BSDF samples, 8 (9) gather samples; "
+                        "Le + direct + caustic + final gather per sample over maps of 2^20 paths shot from the "
+                        "room's lights (seed 77), device-built kd-trees and irradiance map",
+            "cpu_baseline": cpu7,
         }
     if c5 is not None:
         comps["c5_final_gather"] = {
@@ -742,7 +867,8 @@ def main():
         }
     else:
         c = (comps.get("c0_closest_hit") or comps.get("c1_photon_gather") or comps.get("c3_batch")
-             or comps.get("c4_adjoint") or comps.get("c5_final_gather") or comps.get("c6_photon_shooting"))
+             or comps.get("c4_adjoint") or comps.get("c5_final_gather") or comps.get("c6_photon_shooting")
+             or comps.get("c7_photon_mapping"))
         line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
                 "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

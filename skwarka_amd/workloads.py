@@ -444,3 +444,26 @@ def batch_scenes(n=64, seed=3, tri_lo=1e3, tri_hi=1e6, sizes=(128, 256, 512), sp
         size = int(sizes[rng.integers(0, len(sizes))])
         out.append((meshes, ((0.0, 0.0, -2.9), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0), size, spp, 1000 + i))
     return out
+
+
+def batch_gather_jobs(specs, n_jobs=8, photons=250_000, queries=65_536, k=100, seed=5):
+    """Photon-gather jobs for the batched workload: job j gathers on scene j * len(specs) / n_jobs,
+    whose surfaces carry `photons` photons (uniform by area), at `queries` points taken from those
+    photons' positions with their normals; k nearest with the PhotonFilter in the radius that holds
+    40 photons on average (r^2 = 40 A / (pi N)), disc estimate, albedo 0.5. Returns a list of dicts
+    with the photon arrays (ph), query points (p), normals (n), k, r, paths and rho."""
+    out = []
+    table = cdir_table()
+    for j in range(n_jobs):
+        i = j * len(specs) // n_jobs
+        meshes = specs[i][0]
+        mesh = merge(meshes)
+        ph = photons_on_mesh(mesh, photons, seed=seed + j)
+        rng = np.random.default_rng(seed + 100 + j)
+        sel = rng.choice(photons, queries, replace=False)
+        p = ph["pos"][sel].astype(np.float64)
+        n = table[ph["normal"][sel]]
+        r = float(np.sqrt(40.0 * scene_area(meshes) / (np.pi * photons)))
+        out.append({"scene": i, "ph": ph, "p": np.ascontiguousarray(p), "n": np.ascontiguousarray(n), "k": k, "r": r,
+                    "paths": 1, "rho": 0.5})
+    return out

@@ -183,3 +183,60 @@ def test_c4_gradient_at_config_size(torch_cuda, room):
     fscale = np.abs(fd).max()
     assert np.all(np.abs(g - fd) <= 1e-3 * np.abs(fd) + 1e-6 * fscale), np.max(np.abs(g - fd) / np.abs(fd))
     assert np.count_nonzero(g[:64]) > 30 and np.count_nonzero(g[64:128]) > 10 and np.all(g[128:] != 0)
+
+
+def test_c3_batch_with_photon_gathers_matches_oracle(torch_cuda):
+    """The 64 scenes and 8 photon-gather jobs (250k photons each, 65,536 queries, k = 100) in one
+    persistent launch, replayed: every job's disc estimates bit-exact against the oracle on every
+    7th query, and each scene's hits against the oracle on every 127th ray."""
+    print("case: C3 64 scenes + 8 photon-gather jobs in one launch vs oracle", flush=True)
+    torch = torch_cuda
+    specs = wl.batch_scenes()
+    scenes, views = [], []
+    for meshes, camspec, size, spp, seed in specs:
+        scenes.append(sk.Scene(meshes))
+        cam, _ = sk.camera(*camspec, size, size)
+        views.append((cam, size, size, spp, seed))
+    jobs = wl.batch_gather_jobs(specs)
+    gathers, keep = [], []
+    for jb in jobs:
+        pm = sk.PhotonMap(jb["ph"]["pos"], jb["ph"]["weight"], jb["ph"]["dir"], jb["ph"]["normal"])
+        nq = len(jb["p"])
+        dp = torch.from_numpy(jb["p"].view(np.uint8).reshape(-1).copy()).cuda()
+        dn = torch.from_numpy(jb["n"].view(np.uint8).reshape(-1).copy()).cuda()
+        outs = [empty_dev(torch, nq * 12) for _ in range(3)]
+        gathers.append((pm, dp.data_ptr(), dn.data_ptr(), nq, jb["k"], jb["r"], jb["paths"], jb["rho"],
+                        outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr()))
+        keep.append((pm, dp, dn, outs))
+    batch = sk.Batch(scenes, views, gathers)
+    for shape in ((1, 64, 256, 16), (2, 64, 1024, 8)):
+        batch.configure(*shape)
+        for _ in range(2):
+            batch.frame()
+            batch.check()
+    torch.cuda.synchronize()
+    for jb, (pm, dp, dn, outs) in zip(jobs, keep):
+        om = oracle.PhotonMap(jb["ph"]["pos"], jb["ph"]["weight"], jb["ph"]["dir"], jb["ph"]["normal"])
+        nq = len(jb["p"])
+        sel = np.arange(0, nq, 7)
+        ree, rei, _, _, rc = om.irradiance(jb["p"][sel], jb["n"][sel], jb["k"], jb["r"], paths=1, with_sets=False)
+        ee = from_dev(outs[0], np.float32, nq * 3).reshape(nq, 3)[sel]
+        ei = from_dev(outs[1], np.float32, nq * 3).reshape(nq, 3)[sel]
+        assert np.array_equal(ee.view(np.uint32), ree.view(np.uint32))
+        assert np.array_equal(ei.view(np.uint32), rei.view(np.uint32))
+        assert rc.mean() > 20
+    with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
+        accs = list(ex.map(lambda sp: oracle.Accel(sp[0]), specs))
+    for i, acc in enumerate(accs):
+        got = _batch_hits(torch, batch, i)
+        n = len(got)
+        d_rays = empty_dev(torch, n * 64)
+        batch.rays(i, d_rays.data_ptr())
+        torch.cuda.synchronize()
+        rays = from_dev(d_rays, wl.RAY_DTYPE(), n)
+        sel = np.arange(i % 127, n, 127)
+        ref = acc.closest_hit(np.ascontiguousarray(rays[sel]))
+        assert np.array_equal(got[sel].view(np.uint8), ref.view(np.uint8)), "scene %d" % i
+    lat = batch.latency_ns()
+    assert len(lat) == 72 and np.all(lat > 0)
+    batch.close()
