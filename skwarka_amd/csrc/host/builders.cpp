@@ -348,7 +348,9 @@ void place(const BNode *n, uint32_t idx, TreeLayout &out, bool with_inst) {
 
 }  // namespace
 
-bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::string &err) {
+// Checks the description, cleans the meshes (TriangleMesh.cpp:174-229: out-of-range and repeated
+// vertex indices dropped, the rest renumbered in order) and reads the primitives.
+static bool read_scene(const skw_scene_desc &desc, TreeLayout &out, std::string &err) {
   if (desc.n_meshes > 0 && (!desc.mesh_vert_offset || !desc.mesh_tri_offset)) {
     err = "mesh offsets are NULL";
     return false;
@@ -425,6 +427,11 @@ bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::strin
     }
   }
 
+  return true;
+}
+
+bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::string &err) {
+  if (!read_scene(desc, out, err)) return false;
   TriTreeBuilder b;
   b.max_threads = std::max(1, (int)std::thread::hardware_concurrency());
   std::vector<std::vector<uint32_t>> users(desc.n_meshes);
@@ -542,6 +549,50 @@ bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::strin
     di.root = inst_tree_root[inst_tree_of[inst]];
     di.prim = b.inst_prim[inst];
     di.pad[0] = di.pad[1] = 0;
+  }
+  return true;
+}
+
+bool prepare_plain_triangles(const skw_scene_desc &desc, TreeLayout &out, PlainTriangles &pt, bool &instanced,
+                             std::string &err) {
+  instanced = false;
+  if (!read_scene(desc, out, err)) return false;
+  std::vector<uint32_t> users(desc.n_meshes, 0), user(desc.n_meshes, 0);
+  for (uint32_t p = 0; p < desc.n_prims; ++p) {
+    users[desc.prim_mesh[p]]++;
+    user[desc.prim_mesh[p]] = p;
+  }
+  for (uint32_t m = 0; m < desc.n_meshes; ++m)
+    if (users[m] > 1) {
+      instanced = true;
+      return true;
+    }
+  pt = PlainTriangles();
+  // the builder's order: meshes with one primitive, in mesh order, their triangles in order,
+  // moved to world space as TriangleAccelerator.cpp:104-129 does
+  for (uint32_t m = 0; m < desc.n_meshes; ++m) {
+    if (users[m] != 1) continue;
+    const HostMesh &mesh = out.meshes[m];
+    const uint32_t p = user[m];
+    const HostPrim &hp = out.prims[p];
+    for (size_t j = 0; j < mesh.tris.size() / 3; ++j) {
+      float w[3][3];
+      for (int k = 0; k < 3; ++k) xform_point_f(hp.m2w, &mesh.verts[3 * mesh.tris[3 * j + k]], w[k]);
+      BoxF bb;
+      bb.clear();
+      for (int k = 0; k < 3; ++k) {
+        bb.add_point(w[k]);
+        pt.tv.insert(pt.tv.end(), w[k], w[k] + 3);
+      }
+      pt.box.insert(pt.box.end(), bb.mn, bb.mn + 3);
+      pt.box.insert(pt.box.end(), bb.mx, bb.mx + 3);
+      pt.tprim.push_back(p);
+      pt.ttri.push_back((uint32_t)j);
+    }
+  }
+  if (pt.tprim.size() >= (1ull << 29)) {
+    err = "too many triangles for the 29-bit leaf count";
+    return false;
   }
   return true;
 }

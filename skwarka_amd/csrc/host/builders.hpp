@@ -42,6 +42,20 @@ struct TreeLayout {
 // Reads and cleans the description, builds the tree. Returns false with a message on bad input.
 bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::string &err);
 
+// World-space triangles of a scene without instanced meshes, in the order the tree builder
+// partitions them: vertices (9 floats), float bounding boxes (min xyz, max xyz), primitive and
+// in-mesh triangle ids.
+struct PlainTriangles {
+  std::vector<float> tv;
+  std::vector<float> box;
+  std::vector<uint32_t> tprim, ttri;
+};
+
+// The tree builder's input preparation without the build: meshes, primitives and the triangles.
+// Sets instanced (and prepares nothing else) when a mesh is used by several primitives.
+bool prepare_plain_triangles(const skw_scene_desc &desc, TreeLayout &out, PlainTriangles &pt, bool &instanced,
+                             std::string &err);
+
 // Depth-first pre-order export (children left, middle, right) in the form of skw_scene_export_tree.
 uint64_t export_tree(const TreeLayout &t, float *bbox, uint64_t *meta, uint32_t *tri_prim, uint32_t *tri_tri);
 
