@@ -82,10 +82,13 @@ typedef struct skw_photon_map skw_photon_map;
 
 /* Scene flags. */
 #define SKW_SCENE_HOST_ONLY 1u /* build the tree on the host only (no device upload) */
+#define SKW_SCENE_DEVICE_BUILD 2u /* build the tree on the current HIP device */
 
-/* TriangleAccelerator::TriangleAccelerator (TriangleAccelerator.cpp:22-147): builds the ternary
- * tree on the host (same split search, partition and limits) and uploads the flattened arrays to
- * the current HIP device. */
+/* TriangleAccelerator::TriangleAccelerator (TriangleAccelerator.cpp:22-147, Node::Node and
+ * _DetermineBestSplit :505-729): builds the ternary tree (same split search, partition and limits)
+ * and uploads the flattened arrays to the current HIP device. By default the tree is built on the
+ * host; with SKW_SCENE_DEVICE_BUILD it is built on the device, level by level, node for node the
+ * host's tree (scenes with instanced meshes or non-finite vertices: SKW_UNSUPPORTED). */
 skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scene **out);
 skw_status skw_scene_destroy(skw_scene *scene);
 

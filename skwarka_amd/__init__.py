@@ -21,6 +21,7 @@ assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 32 and GEOM_DTYPE.item
 
 MISS = 0xFFFFFFFF
 SCENE_HOST_ONLY = 1
+SCENE_DEVICE_BUILD = 2
 
 # Every symbol include/skwarka_rt.h declares.
 EXPORTS = [
@@ -268,9 +269,11 @@ class Scene:
     meshes: list of dicts with 'verts' (V,3) float32, 'tris' (T,3) uint32, optional 'normals'
             (V,3) float32 (shading normals, used when present) and 'invert_normals' bool.
     prims:  list of (mesh index, 4x4 mesh_to_world or None). None for all prims = identity.
+    device_build: build the tree on the device (SKW_SCENE_DEVICE_BUILD), node for node the host
+            builder's tree; scenes with instanced meshes raise (SKW_UNSUPPORTED).
     """
 
-    def __init__(self, meshes, prims=None, host_only=False):
+    def __init__(self, meshes, prims=None, host_only=False, device_build=False):
         L = lib()
         if prims is None:
             prims = [(i, None) for i in range(len(meshes))]
@@ -291,16 +294,16 @@ class Scene:
         desc = SceneDesc(len(meshes), _ptr(vo), _ptr(self._verts), _ptr(self._normals), _ptr(to),
                          _ptr(self._tris), _ptr(self._flags), len(prims), _ptr(self._pm), _ptr(self._xf))
         h = _P()
-        check(L.skw_scene_create(ctypes.byref(desc), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
-              "skw_scene_create")
+        flags = (SCENE_HOST_ONLY if host_only else 0) | (SCENE_DEVICE_BUILD if device_build else 0)
+        check(L.skw_scene_create(ctypes.byref(desc), flags, ctypes.byref(h)), "skw_scene_create")
         self.handle = h
 
     @classmethod
-    def from_desc(cls, desc, host_only=False):
+    def from_desc(cls, desc, host_only=False, device_build=False):
         """A scene from a filled SceneDesc whose arrays the caller keeps alive for the call."""
         h = _P()
-        check(lib().skw_scene_create(ctypes.byref(desc), SCENE_HOST_ONLY if host_only else 0, ctypes.byref(h)),
-              "skw_scene_create")
+        flags = (SCENE_HOST_ONLY if host_only else 0) | (SCENE_DEVICE_BUILD if device_build else 0)
+        check(lib().skw_scene_create(ctypes.byref(desc), flags, ctypes.byref(h)), "skw_scene_create")
         sc = cls.__new__(cls)
         sc.handle = h
         return sc

@@ -24,6 +24,7 @@ SOURCES = [
     "abi_shoot.cpp",
     "abi_kd.cpp",
     "abi_noalloc.hip",
+    "abi_tribuild.cpp",
     "io.cpp",
     "pbrt_import.cpp",
     "host/builders.cpp",
@@ -35,6 +36,7 @@ SOURCES = [
     "kernels/sort.hip",
     "kernels/shoot.hip",
     "kernels/kd_build.hip",
+    "kernels/tri_build.hip",
 ]
 
 COMMON = [

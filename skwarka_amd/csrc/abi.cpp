@@ -321,9 +321,16 @@ skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scen
   *out = nullptr;
   bool host_only = (flags & SKW_SCENE_HOST_ONLY) != 0;
   if (!host_only && device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
+  const bool device_build = (flags & SKW_SCENE_DEVICE_BUILD) != 0;
+  if (host_only && device_build) return fail(SKW_INVALID_ARG, "SKW_SCENE_DEVICE_BUILD needs a device");
   std::unique_ptr<skw_scene> s(new skw_scene());
   std::string err;
-  if (!build_triangle_tree(*desc, s->host, err)) return fail(SKW_INVALID_ARG, err);
+  if (device_build) {
+    skw_status bs = build_triangle_tree_device(*desc, s->host);
+    if (bs != SKW_OK) return bs;
+  } else if (!build_triangle_tree(*desc, s->host, err)) {
+    return fail(SKW_INVALID_ARG, err);
+  }
   skw_status st = finish_scene(s, host_only);
   if (st != SKW_OK) return st;
   *out = s.release();

@@ -127,6 +127,10 @@ skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, 
 GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                        uint32_t k, double r, uint64_t paths, double rho, int filter);
 
+// The host layout of a scene without instanced meshes, its tree built on the current device
+// (node for node the host builder's tree); SKW_UNSUPPORTED for instanced or non-finite scenes.
+skw_status build_triangle_tree_device(const skw_scene_desc &desc, TreeLayout &out);
+
 // Device tables of a scene / photon map whose host layout is complete (built or loaded).
 skw_status finish_scene(std::unique_ptr<skw_scene> &s, bool host_only);
 skw_status finish_photon_map(skw_photon_map *m);

@@ -366,4 +366,32 @@ hipError_t launch_batch_reset(const BatchArgs &a, hipStream_t stream);
 hipError_t launch_batch_trace(const BatchArgs &a, uint32_t waves, uint32_t stack, uint32_t grid, hipStream_t stream);
 hipError_t launch_batch_rays(const DevBatchScene *scene, skw_ray *rays, uint64_t n, hipStream_t stream);
 
+// Device build of the triangle tree (one level of nodes per launch pair). Node records are
+// allocated in blocks of siblings; info = flags (0 leaf, axis + 1) | child mask << 2 |
+// split-axis mask << 5 | depth << 16.
+struct TriBuildNode {
+  float bmin[3], bmax[3];
+  uint32_t tb, te;        // triangle range (positions in the partition arrays)
+  uint32_t first_child;   // id of the first present child, 0xFFFFFFFF for a leaf
+  uint32_t info;
+};
+struct TriBuildArgs {
+  float *box;     // 6 per position: triangle box min xyz, max xyz
+  uint32_t *ref;  // per position: triangle id
+  float *box2;    // scatter targets of the partition (same sizes)
+  uint32_t *ref2;
+  // per-position partition scratch (nine arrays of the triangle count)
+  uint32_t *s_cls, *s_rf, *s_nb, *s_G, *s_Rpos, *s_order, *s_elem, *s_ptr, *s_dest;
+  TriBuildNode *nodes;
+  uint32_t *node_count;  // ids handed out
+  const uint32_t *list;  // nodes of this launch
+  uint32_t list_n;
+  uint32_t *next_big, *next_big_n;      // children larger than small_max
+  uint32_t *next_small, *next_small_n;  // children of at most small_max triangles
+  uint32_t small_max;
+};
+// k_tb_level: one 256-thread workgroup per listed node; k_tb_small: one thread per listed node.
+hipError_t launch_tri_build_level(const TriBuildArgs &a, hipStream_t stream);
+hipError_t launch_tri_build_small(const TriBuildArgs &a, hipStream_t stream);
+
 }  // namespace skw

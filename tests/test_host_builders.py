@@ -35,6 +35,11 @@ def test_no_device_is_reported_not_faked():
         sk.Scene([wl.box(1.0)])
 
 
+def test_device_build_needs_a_device():
+    with pytest.raises(sk.SkwError, match="needs a device"):
+        sk.Scene([wl.box(1.0)], host_only=True, device_build=True)
+
+
 def _compare_trees(meshes, prims=None):
     acc = oracle.Accel(meshes, prims)
     sc = sk.Scene(meshes, prims, host_only=True)
