@@ -16,8 +16,15 @@ timed region starts:
   c5 (component): the photon integrator's final gather on the c2 scene at 1024x1024, 4 spp, 8
       gather samples (rounded to 9), over an irradiance photon map built from 100k direct, 400k
       indirect and 20k caustic photons -> Mrays/s over camera + gather rays.
+  c3 (component): 64 batched scenes (1.3k-874k triangles) closest hit in one persistent launch
+      per frame, alone and with 8 photon-gather jobs -> Mrays/s, per-item latency.
+  c4 (component): the 132-parameter direct-lighting gradient at 512x512, 8 spp -> gradients/s.
   c6 (component): photon shooting from the c2 scene's area lights, 4M paths -> M paths/s, and the
       device kd-tree build of the indirect map against the host build.
+  c7 (component): the photon-mapping render of the c2 scene (direct + caustic + final gather) at
+      1024x1024, 16 spp, over shot maps -> Mrays/s.
+  c8 (component): the triangle-tree build of the c2 room on the device against the host build
+      -> M triangles/s.
 Timing: W untimed warm-up steps, then exactly K steps between HIP events recorded on the stream
 the kernels run on, with device synchronisation on both sides.
 The CPU baseline is the oracle (CPU restatement of the reference path, OpenMP) timed on this host
@@ -242,13 +249,22 @@ def bench_c1(torch, sk, wl, steps, warmup, stream, c0):
 
 def bench_c3(torch, sk, wl, steps, warmup, stream):
     specs = wl.batch_scenes()
+    # the 64 scenes' trees built on the device, one timed scene creation each (descriptions made
+    # beforehand; one untimed build first)
+    descs = [sk.scene_desc(sp[0]) for sp in specs]
+    sk.Scene.from_desc(descs[0][0], device_build=True).close()
     t0 = time.time()
-    scenes, views = [], []
-    for meshes, camspec, size, spp, seed in specs:
-        scenes.append(sk.Scene(meshes))
+    scenes, views, build_ms = [], [], []
+    for (desc, _), (meshes, camspec, size, spp, seed) in zip(descs, specs):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        scenes.append(sk.Scene.from_desc(desc, device_build=True))
+        torch.cuda.synchronize()
+        build_ms.append(1e3 * (time.perf_counter() - t))
         cam, _ = sk.camera(*camspec, size, size)
         views.append((cam, size, size, spp, seed))
     build_s = time.time() - t0
+    del descs
     batch = sk.Batch(scenes, views)
     t0 = time.time()
     cfg, tune_ms = batch.autotune(frames=1)
@@ -287,7 +303,7 @@ def bench_c3(torch, sk, wl, steps, warmup, stream):
             "tune_s": tune_s, "build_s": build_s, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "n_scenes": len(scenes),
             "tri_range": [min(tris), max(tris)], "tris_total": sum(tris), "specs": specs, "batch": batch,
-            "scenes": scenes, "with_gathers": with_g}
+            "scenes": scenes, "with_gathers": with_g, "build_ms": build_ms, "tris": tris}
 
 
 def bench_c4(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
@@ -525,6 +541,62 @@ def cpu_baseline_c7(c7, threads, rows):
                       "gather rays), one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
 
 
+def bench_c8(torch, sk, wl, steps, warmup, stream, spec=None):
+    """Triangle-tree build (scene creation: tree + device tables) of the 2,026,260-triangle room on
+    the device (SKW_SCENE_DEVICE_BUILD) and on the host (the same tree), and of the 64 batched
+    scenes both ways; scene descriptions are prepared before the timed region."""
+    s = spec if spec is not None else wl.scene_room()
+    desc, keep = sk.scene_desc(s["meshes"])
+
+    def build(device):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sc = sk.Scene.from_desc(desc, device_build=device)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t), sc
+
+    for _ in range(warmup):
+        build(True)[1].close()
+    dev_ms = []
+    for _ in range(steps):
+        ms, sc = build(True)
+        dev_ms.append(ms)
+        info = sc.info()
+        sc.close()
+    host_ms = []
+    for _ in range(2):
+        ms, sc = build(False)
+        host_ms.append(ms)
+        assert sc.info() == info
+        sc.close()
+    specs = wl.batch_scenes()
+    descs = [sk.scene_desc(sp[0]) for sp in specs]
+    batch = {}
+    for device in (True, False):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for d, _ in descs:
+            sk.Scene.from_desc(d, device_build=device).close()
+        torch.cuda.synchronize()
+        batch[device] = 1e3 * (time.perf_counter() - t)
+    ms = statistics.median(dev_ms)
+    tris = info["triangles"]
+    return {"ms": ms, "value": tris / (ms * 1e-3) / 1e6, "host_ms": statistics.median(host_ms), "info": info,
+            "batch_device_ms": batch[True], "batch_host_ms": batch[False],
+            "batch_tris": int(sum(sum(len(m["tris"]) for m in sp[0]) for sp in specs)), "spec": s}
+
+
+def cpu_baseline_c8(c8):
+    import oracle
+    t = time.perf_counter()
+    acc = oracle.Accel(c8["spec"]["meshes"])
+    dt = time.perf_counter() - t
+    assert acc.info()["nodes"] == c8["info"]["nodes"]
+    return {"value": c8["info"]["triangles"] / dt / 1e6, "unit": "M triangles/s", "cores": 1, "kind": "port",
+            "sample": "the whole room's tree built once by the oracle's serial restatement of the reference "
+                      "builder (%.1f s)" % dt}
+
+
 def cpu_baseline_c6(c6, threads, n_paths):
     import oracle
     s = c6["spec"]
@@ -637,7 +709,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5, help="timed c2 frames")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6", "c7"], default=None,
+    ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
     ap.add_argument("--cpu-rows", type=int, default=1024,
@@ -713,7 +785,13 @@ def main():
         c7 = bench_c7(torch, sk, wl, 3, 1, stream, c2["spec"] if c2 else None, c2["scene"] if c2 else None)
         log("c7: %.1f ms/frame, %.1f Mrays/s, rays %s, maps %.1f ms (photons %s)"
             % (c7["ms"], c7["value"], c7["rays"], c7["maps_ms"], c7["photons"]))
-    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = cpu7 = None
+    c8 = None
+    if run("c8"):
+        log("c8: triangle-tree build, 2,026,260 triangles (and the 64 batched scenes), device and host")
+        c8 = bench_c8(torch, sk, wl, 5, 1, stream, c2["spec"] if c2 else None)
+        log("c8: device %.1f ms (%.1f M triangles/s), host %.1f ms; 64 batched scenes: device %.1f ms, host %.1f ms"
+            % (c8["ms"], c8["value"], c8["host_ms"], c8["batch_device_ms"], c8["batch_host_ms"]))
+    cpu0 = cpu1 = cpu2 = cpu3 = cpu4 = cpu5 = cpu6 = cpu7 = cpu8 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % threads)
         if c2 is not None:
@@ -732,6 +810,8 @@ def main():
             cpu6 = cpu_baseline_c6(c6, threads, 4_000_000)
         if c7 is not None:
             cpu7 = cpu_baseline_c7(c7, threads, 64)
+        if c8 is not None:
+            cpu8 = cpu_baseline_c8(c8)
     comps = {}
     if c3 is not None:
         comps["c3_batch"] = {
@@ -740,6 +820,10 @@ def main():
             "scene_latency_ms": {"p50": round(c3["p50_ms"], 4), "p99": round(c3["p99_ms"], 4),
                                  "max": round(c3["max_ms"], 4)},
             "autotuned": c3["config"], "autotune_s": round(c3["tune_s"], 2),
+            "scene_build": {"mode": "tree built on the device (SKW_SCENE_DEVICE_BUILD), scene creation timed",
+                            "total_ms": round(sum(c3["build_ms"]), 2),
+                            "ms_per_scene": [round(x, 3) for x in c3["build_ms"]],
+                            "triangles_per_scene": c3["tris"]},
             "with_photon_gathers": {
                 "ms_per_step": round(c3["with_gathers"]["ms"], 4),
                 "Mrays_per_s": round(c3["with_gathers"]["rays_per_s"], 3),
@@ -791,6 +875,17 @@ def main():
                         "Le + direct + caustic + final gather per sample over maps of 2^20 paths shot from the "
                         "room's lights (seed 77), device-built kd-trees and irradiance map",
             "cpu_baseline": cpu7,
+        }
+    if c8 is not None:
+        comps["c8_tree_build"] = {
+            "metric": "M triangles/s (triangle-tree build: scene creation with the tree and device tables)",
+            "value": round(c8["value"], 3), "ms_per_step": round(c8["ms"], 3), "host_build_ms": round(c8["host_ms"], 3),
+            "nodes": c8["info"]["nodes"], "max_depth": c8["info"]["max_depth"],
+            "batch_64_scenes": {"triangles": c8["batch_tris"], "device_ms": round(c8["batch_device_ms"], 2),
+                                "host_ms": round(c8["batch_host_ms"], 2)},
+            "workload": "the c2 room (2,026,260 triangles) built on the device (the same tree as the host "
+                        "builder, node for node), host builder timed beside it; the 64 batched scenes both ways",
+            "cpu_baseline": cpu8,
         }
     if c5 is not None:
         comps["c5_final_gather"] = {
@@ -868,7 +963,7 @@ def main():
     else:
         c = (comps.get("c0_closest_hit") or comps.get("c1_photon_gather") or comps.get("c3_batch")
              or comps.get("c4_adjoint") or comps.get("c5_final_gather") or comps.get("c6_photon_shooting")
-             or comps.get("c7_photon_mapping"))
+             or comps.get("c7_photon_mapping") or comps.get("c8_tree_build"))
         line = {"metric": c["metric"], "value": c["value"], "unit": c["metric"].split(" ")[0], "n_gpus": 1,
                 "steps": 20, "warmup": 3, "ms_per_step": c["ms_per_step"], "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -262,6 +262,26 @@ def mesh_arrays(meshes):
     return {"vert_off": vo, "tri_off": to, "verts": verts, "tris": tris, "normals": normals, "flags": flags}
 
 
+def scene_desc(meshes, prims=None):
+    """(SceneDesc, arrays it points into) for a list of mesh dicts and (mesh, 4x4 or None) primitives
+    (None: one identity primitive per mesh); keep the arrays alive while the description is used."""
+    if prims is None:
+        prims = [(i, None) for i in range(len(meshes))]
+    arrays = mesh_arrays(meshes)
+    pm = np.array([p[0] for p in prims], np.uint32)
+    xf = None
+    if any(p[1] is not None for p in prims):
+        xf = np.zeros((len(prims), 32), np.float64)
+        for i, (_, m) in enumerate(prims):
+            m = np.eye(4) if m is None else np.asarray(m, np.float64)
+            xf[i, :16] = m.reshape(-1)
+            xf[i, 16:] = np.linalg.inv(m).reshape(-1)
+    desc = SceneDesc(len(meshes), _ptr(arrays["vert_off"]), _ptr(arrays["verts"]), _ptr(arrays["normals"]),
+                     _ptr(arrays["tri_off"]), _ptr(arrays["tris"]), _ptr(arrays["flags"]), len(prims), _ptr(pm),
+                     _ptr(xf))
+    return desc, (arrays, pm, xf)
+
+
 class Scene:
     """A scene handle: the ternary triangle tree of TriangleAccelerator, built on the host and
     (unless host_only) uploaded to the current HIP device.
@@ -275,24 +295,7 @@ class Scene:
 
     def __init__(self, meshes, prims=None, host_only=False, device_build=False):
         L = lib()
-        if prims is None:
-            prims = [(i, None) for i in range(len(meshes))]
-        arrays = mesh_arrays(meshes)
-        vo, to = arrays["vert_off"], arrays["tri_off"]
-        self._verts, self._tris, self._normals, self._flags = (arrays["verts"], arrays["tris"], arrays["normals"],
-                                                               arrays["flags"])
-        self._vo, self._to = vo, to
-        self._pm = np.array([p[0] for p in prims], np.uint32)
-        self._xf = None
-        if any(p[1] is not None for p in prims):
-            xf = np.zeros((len(prims), 32), np.float64)
-            for i, (_, m) in enumerate(prims):
-                m = np.eye(4) if m is None else np.asarray(m, np.float64)
-                xf[i, :16] = m.reshape(-1)
-                xf[i, 16:] = np.linalg.inv(m).reshape(-1)
-            self._xf = xf
-        desc = SceneDesc(len(meshes), _ptr(vo), _ptr(self._verts), _ptr(self._normals), _ptr(to),
-                         _ptr(self._tris), _ptr(self._flags), len(prims), _ptr(self._pm), _ptr(self._xf))
+        desc, self._keep = scene_desc(meshes, prims)
         h = _P()
         flags = (SCENE_HOST_ONLY if host_only else 0) | (SCENE_DEVICE_BUILD if device_build else 0)
         check(L.skw_scene_create(ctypes.byref(desc), flags, ctypes.byref(h)), "skw_scene_create")

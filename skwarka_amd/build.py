@@ -37,6 +37,7 @@ SOURCES = [
     "kernels/shoot.hip",
     "kernels/kd_build.hip",
     "kernels/tri_build.hip",
+    "kernels/tri_build_wide.hip",
 ]
 
 COMMON = [

@@ -3,6 +3,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -49,7 +50,11 @@ struct PlainTriangles {
   std::vector<float> tv;
   std::vector<float> box;
   std::vector<uint32_t> tprim, ttri;
+  bool finite_boxes = true;  // every box coordinate is finite
 };
+
+// f(begin, end) over [0, n) split into contiguous ranges on the host's threads.
+void parallel_ranges(uint64_t n, const std::function<void(uint64_t, uint64_t)> &f);
 
 // The tree builder's input preparation without the build: meshes, primitives and the triangles.
 // Sets instanced (and prepares nothing else) when a mesh is used by several primitives.

@@ -388,10 +388,41 @@ struct TriBuildArgs {
   uint32_t list_n;
   uint32_t *next_big, *next_big_n;      // children larger than small_max
   uint32_t *next_small, *next_small_n;  // children of at most small_max triangles
-  uint32_t small_max;
+  uint32_t *next_wide, *next_wide_n;    // children larger than wide_min
+  uint32_t small_max;                   // at most 64
+  uint32_t wide_min;
+  uint32_t node_cap;                    // node records allocated (subtree builds check it)
+  uint32_t *overflow;                   // set when a subtree build ran out of node records
 };
-// k_tb_level: one 256-thread workgroup per listed node; k_tb_small: one thread per listed node.
+// k_tb_level: one 256-thread workgroup per listed node, children listed by size;
+// launch_tri_build_small: one wave per listed node builds its whole subtree.
 hipError_t launch_tri_build_level(const TriBuildArgs &a, hipStream_t stream);
 hipError_t launch_tri_build_small(const TriBuildArgs &a, hipStream_t stream);
+
+// The largest nodes of a level, each split over chunks of `chunk` triangles (one workgroup each).
+struct TriWide {
+  uint32_t id, tb, n, info;   // node id, range start, triangles, info bits of the node record
+  uint32_t c_begin, c_end;    // its chunks
+  uint32_t leaf, axis;
+  double split;
+  unsigned long long key[6];  // node box keys (first extremes)
+  uint32_t n_right, n_nonr, K, n_left, m, r_total, pad[2];
+};
+struct TriWideArgs {
+  TriBuildArgs b;          // list / list_n: the wide nodes
+  TriWide *w;
+  void *bins;              // 3 bin sets per wide node (tri_build_bins_bytes() each)
+  uint32_t *chunk_w;       // per chunk: wide node index
+  uint32_t *ccnt, *cbase;  // per chunk: 8 counters, 8 prefix bases
+  uint32_t *info_out;      // [0] chunks, [1] largest node
+  uint32_t chunk_cap;      // chunk table entries allocated
+  uint32_t n_chunks;
+  uint32_t chunk;          // triangles per chunk (multiple of 256)
+};
+size_t tri_build_bins_bytes();
+// setup: the chunk table and per-node state from the listed nodes (info_out read by the host);
+// then the build of the listed nodes' children with jump_rounds pointer-jumping rounds.
+hipError_t launch_tri_wide_setup(const TriWideArgs &a, hipStream_t stream);
+hipError_t launch_tri_wide_level(const TriWideArgs &a, uint32_t jump_rounds, hipStream_t stream);
 
 }  // namespace skw

@@ -2,8 +2,9 @@
 // and _DetermineBestSplit; Source/RayLibs/Raytracer/Core/TriangleAccelerator.cpp:22-147, 505-729),
 // node for node the tree of the host builder (host/builders.cpp), level by level:
 //
-//  * k_tb_level: one workgroup per node larger than kSmallMax triangles. The node box is the fold
-//    of its triangles' boxes in array order (min / max keep the first of equal values, which
+//  * k_tb_level: one workgroup per node between the small-node size (at most 64) and the wide-node
+//    size (larger nodes take the multi-workgroup kernels of tri_build_wide.hip, the same steps).
+//    The node box is the fold of its triangles' boxes in array order (min / max keep the first of equal values, which
 //    decides the sign of a zero), found with 64-bit (ordered value, position) atomics. The split
 //    search bins the triangles per axis exactly as the host (float bin coefficient, double
 //    positions, bins 0..num_tries), left / right bins in LDS, straddling ranges as a difference
@@ -16,258 +17,263 @@
 //    left and right elements land in examination order from the two ends; the middle block is a
 //    queue that every later left element rotates by one, whose final order is the last m entries
 //    of the append sequence, resolved by pointer jumping.
-//  * k_tb_small: one thread per node of at most kSmallMax triangles: the same box fold, split
-//    search (every candidate evaluated directly) and the sequential partition loop.
+//  * k_tb_subtree: one wave per node of at most 64 triangles builds that node's whole subtree
+//    depth first with the triangles' boxes in LDS: the same box fold, split search (a lane per
+//    candidate bin position, every candidate evaluated directly, the first least cost over the
+//    wave) and the sequential partition loop on a permutation of the node's positions.
 // Children are allocated as contiguous id blocks; the host lays the tree out depth first as the
 // host builder does.
 // Every translation unit is compiled with -ffp-contract=off; f32 divisions are correctly rounded.
 #include <hip/hip_runtime.h>
 
-#include "kernels.hpp"
+#include "tri_build_device.hpp"
 
 namespace skw {
 namespace {
 
-constexpr uint32_t kMaxTries = 200;  // TriangleAccelerator.h:131
-constexpr uint32_t kMaxLeaf = 4;     // TriangleAccelerator.h:128
-constexpr uint32_t kMaxDepth = 200;  // TriangleAccelerator.h:134
-constexpr uint32_t kBlock = 256;
-constexpr uint32_t kSeg = 256;       // segment-tree leaves (>= kMaxTries)
+using namespace tb;
 
-__device__ __forceinline__ float fmin_std(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float fmax_std(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double dmin_std(double a, double b) { return (b < a) ? b : a; }
+// ------------------------------------------------------------------ one wave per small subtree
 
-// Float order as unsigned order (no NaN here).
-__device__ __forceinline__ uint32_t ord32(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float unord32(uint32_t k) {
-  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
-}
-constexpr uint32_t kOrdPosInf = 0xFF800000u;  // ord32(+inf)
-constexpr uint32_t kOrdNegInf = 0x007FFFFFu;  // ord32(-inf)
+constexpr uint32_t kWaveMax = 64;    // triangles a subtree wave holds (one per lane)
+constexpr uint32_t kStackMax = 416;  // DFS entries: at most 2 per level below the subtree's root, depth <= 200
 
-// BoxF::area (float) and BoxD::area (double) of host/builders.cpp.
-__device__ __forceinline__ float area_f(const float mn[3], const float mx[3]) {
-  const float x = fabsf(mx[0] - mn[0]), y = fabsf(mx[1] - mn[1]), z = fabsf(mx[2] - mn[2]);
-  return 2.0f * (x * y + x * z + y * z);
-}
-__device__ __forceinline__ double area_d(const double mn[3], const double mx[3]) {
-  const double x = fabs(mx[0] - mn[0]), y = fabs(mx[1] - mn[1]), z = fabs(mx[2] - mn[2]);
-  return 2.0 * (x * y + x * z + y * z);
+struct WaveLds {
+  float bx[kWaveMax][6];
+  uint32_t ref[kWaveMax];
+  uint32_t lbin[kWaveMax], rbin[kWaveMax];
+  uint32_t cls[kWaveMax], perm[kWaveMax];
+  uint32_t st_id[kStackMax], st_rng[kStackMax];
+  uint32_t misc[4];  // stack size, stop flag, partition mid, partition right
+};
+
+// stack entry: local range [lo, hi), split-axis mask, depth
+__device__ __forceinline__ uint32_t pack_rng(uint32_t lo, uint32_t hi, uint32_t smask, uint32_t depth) {
+  return lo | (hi << 7) | (smask << 14) | (depth << 17);
 }
 
-__device__ __forceinline__ uint32_t bin_index(double v, uint32_t num_tries) {
-  if (!(v < 18446744073709551616.0)) return num_tries;
-  const uint64_t b = (uint64_t)v;
-  return b < num_tries ? (uint32_t)b : num_tries;
-}
-
-// Bins of a triangle box on `axis`: lefts_begin from its max, rights_end from its min.
-__device__ __forceinline__ void bins_of(const float *bx, int axis, float lo, double coef, uint32_t T, uint32_t &lb,
-                                        uint32_t &re) {
-  lb = bin_index((double)(bx[3 + axis] - lo) * coef + 1.0, T);
-  re = bin_index((double)(bx[axis] - lo) * coef + 1.0, T);
-}
-
-__device__ __forceinline__ uint32_t info_depth(uint32_t info) { return info >> 16; }
-__device__ __forceinline__ uint32_t info_smask(uint32_t info) { return (info >> 5) & 7u; }
-
-// Records the children of node `id` (ranges left / middle / right) and queues them by size.
-__device__ void emit_children(const TriBuildArgs &a, uint32_t id, uint32_t axis, uint32_t tb, uint32_t mid,
-                              uint32_t right, uint32_t te, uint32_t smask, uint32_t depth) {
-  const uint32_t rb[3] = {tb, mid, right}, re[3] = {mid, right, te};
-  uint32_t cmask = 0, nch = 0;
-  for (int c = 0; c < 3; ++c)
-    if (re[c] > rb[c]) {
-      cmask |= 1u << c;
-      ++nch;
-    }
-  const uint32_t first = atomicAdd(a.node_count, nch);
-  TriBuildNode &n = a.nodes[id];
-  n.first_child = first;
-  n.info = (axis + 1u) | (cmask << 2) | (smask << 5) | (depth << 16);
-  uint32_t k = 0;
-  for (int c = 0; c < 3; ++c) {
-    if (re[c] <= rb[c]) continue;
-    const uint32_t cid = first + k++;
-    TriBuildNode &ch = a.nodes[cid];
-    ch.tb = rb[c];
-    ch.te = re[c];
-    ch.first_child = 0xFFFFFFFFu;
-    const uint32_t cm = c == 1 ? (smask | (1u << axis)) : smask;
-    ch.info = (cm << 5) | ((depth + 1u) << 16);
-    if (re[c] - rb[c] <= a.small_max)
-      a.next_small[atomicAdd(a.next_small_n, 1u)] = cid;
-    else
-      a.next_big[atomicAdd(a.next_big_n, 1u)] = cid;
-  }
-}
-
-__device__ __forceinline__ void make_leaf(const TriBuildArgs &a, uint32_t id, uint32_t smask, uint32_t depth) {
-  TriBuildNode &n = a.nodes[id];
-  n.first_child = 0xFFFFFFFFu;
-  n.info = (smask << 5) | (depth << 16);
-}
-
-// ------------------------------------------------------------------ one thread per small node
-
-__global__ __launch_bounds__(64) void k_tb_small(TriBuildArgs a) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= a.list_n) return;
-  const uint32_t id = a.list[w];
-  TriBuildNode &nd = a.nodes[id];
-  const uint32_t tb = nd.tb, te = nd.te, n = te - tb;
-  const uint32_t smask = info_smask(nd.info), depth = info_depth(nd.info);
-  float mn[3] = {__int_as_float(0x7F800000), __int_as_float(0x7F800000), __int_as_float(0x7F800000)};
-  float mx[3] = {-mn[0], -mn[1], -mn[2]};
-  for (uint32_t p = tb; p < te; ++p) {
-    const float *bx = a.box + 6 * (size_t)p;
-    for (int c = 0; c < 3; ++c) {
-      mn[c] = fmin_std(mn[c], bx[c]);
-      mx[c] = fmax_std(mx[c], bx[3 + c]);
-    }
-  }
-  for (int c = 0; c < 3; ++c) {
-    nd.bmin[c] = mn[c];
-    nd.bmax[c] = mx[c];
-  }
-  if (n <= kMaxLeaf || smask == 7u || depth >= kMaxDepth) {
-    make_leaf(a, id, smask, depth);
+__global__ __launch_bounds__(64) void k_tb_subtree(TriBuildArgs a) {
+  __shared__ WaveLds W;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t root = a.list[blockIdx.x];
+  const uint32_t base = a.nodes[root].tb, n0 = a.nodes[root].te - base, rinfo = a.nodes[root].info;
+  if (n0 > kWaveMax) {  // listed by mistake: reported, nothing written
+    if (lane == 0) atomicOr(a.overflow, 2u);
     return;
   }
-  const double inv_area = 1.0 / (double)area_f(mn, mx);
-  double axis_cost[3] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val()};
-  double axis_pos[3] = {0.0, 0.0, 0.0};
-  for (int axis = 0; axis < 3; ++axis) {
-    if (smask & (1u << axis)) continue;
-    const uint32_t T = kMaxTries < 2 * n ? kMaxTries : 2 * n;
-    const double coef = (double)((float)T / (mx[axis] - mn[axis]));
-    const float lo = mn[axis];
-    double best = __builtin_huge_val();
-    for (uint32_t i = 0; i < T; ++i) {
-      // candidate i: left = bins 0..i, right = bins i..T-1, middle = straddling bin boundary i
-      double lmn[3], lmx[3], mmn[3], mmx[3], rmn[3], rmx[3];
+  if (lane < n0) {
+    for (int c = 0; c < 6; ++c) W.bx[lane][c] = a.box[6 * (size_t)(base + lane) + c];
+    W.ref[lane] = a.ref[base + lane];
+  }
+  if (lane == 0) {
+    W.st_id[0] = root;
+    W.st_rng[0] = pack_rng(0, n0, info_smask(rinfo), info_depth(rinfo));
+    W.misc[0] = 1;
+    W.misc[1] = 0;
+  }
+  __syncthreads();
+  while (true) {
+    const uint32_t sp = W.misc[0];
+    if (sp == 0 || W.misc[1] != 0) break;
+    const uint32_t id = W.st_id[sp - 1], rng = W.st_rng[sp - 1];
+    __syncthreads();
+    if (lane == 0) W.misc[0] = sp - 1;
+    const uint32_t lo = rng & 127u, hi = (rng >> 7) & 127u, smask = (rng >> 14) & 7u, depth = rng >> 17;
+    const uint32_t nn = hi - lo;
+    // node box: the fold in array order (every lane)
+    float mn[3] = {__int_as_float(0x7F800000), __int_as_float(0x7F800000), __int_as_float(0x7F800000)};
+    float mx[3] = {-mn[0], -mn[1], -mn[2]};
+    for (uint32_t p = lo; p < hi; ++p)
       for (int c = 0; c < 3; ++c) {
-        lmn[c] = mmn[c] = rmn[c] = __builtin_huge_val();
-        lmx[c] = mmx[c] = rmx[c] = -__builtin_huge_val();
+        mn[c] = fmin_std(mn[c], W.bx[p][c]);
+        mx[c] = fmax_std(mx[c], W.bx[p][3 + c]);
       }
-      uint32_t ln = 0, mc = 0, rn = 0;
-      for (uint32_t p = tb; p < te; ++p) {
-        const float *bx = a.box + 6 * (size_t)p;
+    TriBuildNode &nd = a.nodes[id];
+    if (lane == 0) {
+      nd.tb = base + lo;
+      nd.te = base + hi;
+      for (int c = 0; c < 3; ++c) {
+        nd.bmin[c] = mn[c];
+        nd.bmax[c] = mx[c];
+      }
+    }
+    if (leaf_by_rule(nn, smask, depth)) {
+      if (lane == 0) make_leaf(a, id, smask, depth);
+      __syncthreads();
+      continue;
+    }
+    const double inv_area = 1.0 / (double)area_f(mn, mx);
+    const uint32_t T = num_tries(nn);
+    double axis_cost[3], axis_pos[3];
+    for (int axis = 0; axis < 3; ++axis) {
+      axis_cost[axis] = __builtin_huge_val();
+      axis_pos[axis] = 0.0;
+      if (smask & (1u << axis)) continue;
+      const double coef = bin_coef(T, mn[axis], mx[axis]);
+      const float flo = mn[axis];
+      if (lane < nn) {
         uint32_t lbg, re;
-        bins_of(bx, axis, lo, coef, T, lbg, re);
-        if (lbg <= i && lbg < T) {
-          ++ln;
+        bins_of(W.bx[lo + lane], axis, flo, coef, T, lbg, re);
+        W.lbin[lane] = lbg;
+        W.rbin[lane] = re;
+      }
+      __syncthreads();
+      // candidates lane and lane + 64: left = bins 0..i, right = bins i..T-1, middle = straddling i
+      double my_cost = __builtin_huge_val();
+      uint32_t my_i = 0xFFFFFFFFu;
+      for (uint32_t i = lane; i < T; i += kWaveMax) {
+        float lmn[3], lmx[3], mmn[3], mmx[3], rmn[3], rmx[3];
+        for (int c = 0; c < 3; ++c) {
+          lmn[c] = mmn[c] = rmn[c] = __int_as_float(0x7F800000);
+          lmx[c] = mmx[c] = rmx[c] = __int_as_float(0xFF800000);
+        }
+        uint32_t ln = 0, mc = 0, rn = 0;
+        for (uint32_t k = 0; k < nn; ++k) {
+          const uint32_t lbg = W.lbin[k], re = W.rbin[k];
+          const float *bx = W.bx[lo + k];
+          const bool isl = lbg <= i && lbg < T, isr = re > i, ism = re <= i && i < lbg;
+          ln += isl ? 1u : 0u;
+          rn += isr ? 1u : 0u;
+          mc += ism ? 1u : 0u;
           for (int c = 0; c < 3; ++c) {
-            lmn[c] = dmin_std(lmn[c], (double)bx[c]);
-            lmx[c] = lmx[c] < (double)bx[3 + c] ? (double)bx[3 + c] : lmx[c];
+            if (isl) {
+              lmn[c] = fmin_std(lmn[c], bx[c]);
+              lmx[c] = fmax_std(lmx[c], bx[3 + c]);
+            }
+            if (isr) {
+              rmn[c] = fmin_std(rmn[c], bx[c]);
+              rmx[c] = fmax_std(rmx[c], bx[3 + c]);
+            }
+            if (ism) {
+              mmn[c] = fmin_std(mmn[c], bx[c]);
+              mmx[c] = fmax_std(mmx[c], bx[3 + c]);
+            }
           }
         }
-        if (re > 0 && re - 1 >= i) {
-          ++rn;
-          for (int c = 0; c < 3; ++c) {
-            rmn[c] = dmin_std(rmn[c], (double)bx[c]);
-            rmx[c] = rmx[c] < (double)bx[3 + c] ? (double)bx[3 + c] : rmx[c];
-          }
+        double dl[3], dL[3], dm[3], dM[3], dr[3], dR[3];
+        for (int c = 0; c < 3; ++c) {
+          dl[c] = lmn[c];
+          dL[c] = lmx[c];
+          dm[c] = mmn[c];
+          dM[c] = mmx[c];
+          dr[c] = rmn[c];
+          dR[c] = rmx[c];
         }
-        if (re <= i && i < lbg) {
-          ++mc;
-          for (int c = 0; c < 3; ++c) {
-            mmn[c] = dmin_std(mmn[c], (double)bx[c]);
-            mmx[c] = mmx[c] < (double)bx[3 + c] ? (double)bx[3 + c] : mmx[c];
-          }
+        const double p1 = dmin_std(1.0, area_d(dl, dL) * inv_area);
+        const double p2 = dmin_std(1.0, area_d(dm, dM) * inv_area);
+        const double p3 = dmin_std(1.0, area_d(dr, dR) * inv_area);
+        double cost = (double)ln * p1 + (double)mc * p2 + (double)rn * p3;
+        if (ln == 0 || rn == 0) cost *= 0.8;
+        if (cost < my_cost) {
+          my_cost = cost;
+          my_i = i;
         }
       }
-      const double p1 = dmin_std(1.0, area_d(lmn, lmx) * inv_area);
-      const double p2 = dmin_std(1.0, area_d(mmn, mmx) * inv_area);
-      const double p3 = dmin_std(1.0, area_d(rmn, rmx) * inv_area);
-      double cost = (double)ln * p1 + (double)mc * p2 + (double)rn * p3;
-      if (ln == 0 || rn == 0) cost *= 0.8;
-      if (cost < best) {
-        best = cost;
-        const float ext = mx[axis] - mn[axis];
-        const float step = ((float)i * ext) / (float)T;
-        axis_pos[axis] = (double)(mn[axis] + step);
+      // the first candidate of least cost
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double oc = __shfl_xor(my_cost, off);
+        const uint32_t oi = __shfl_xor(my_i, off);
+        if (oc < my_cost || (oc == my_cost && oi < my_i)) {
+          my_cost = oc;
+          my_i = oi;
+        }
+      }
+      axis_cost[axis] = my_cost;
+      if (my_i != kNone) axis_pos[axis] = split_pos(my_i, T, mn[axis], mx[axis]);
+      __syncthreads();
+    }
+    const uint32_t axis = pick_axis(axis_cost[0], axis_cost[1], axis_cost[2]);
+    const double split = axis_pos[axis];
+    // the sequential three-way partition (TriangleAccelerator.cpp:650-701) on a permutation of the
+    // node's positions, then applied to the boxes
+    if (lane < nn) {
+      const float *bx = W.bx[lo + lane];
+      W.cls[lane] = class_of(bx, axis, split);
+      W.perm[lane] = lane;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      uint32_t mid = 0, right = nn, i = 0;
+      while (i < right) {
+        const uint32_t c = W.cls[W.perm[i]];
+        if (c == 0u) {
+          const uint32_t t = W.perm[mid];
+          W.perm[mid] = W.perm[i];
+          W.perm[i] = t;
+          ++mid;
+          ++i;
+        } else if (c == 2u) {
+          --right;
+          const uint32_t t = W.perm[i];
+          W.perm[i] = W.perm[right];
+          W.perm[right] = t;
+        } else {
+          ++i;
+        }
+      }
+      W.misc[2] = mid;
+      W.misc[3] = right;
+    }
+    __syncthreads();
+    float v[6];
+    uint32_t r = 0;
+    if (lane < nn) {
+      const uint32_t src = lo + W.perm[lane];
+      for (int c = 0; c < 6; ++c) v[c] = W.bx[src][c];
+      r = W.ref[src];
+    }
+    __syncthreads();
+    if (lane < nn) {
+      for (int c = 0; c < 6; ++c) W.bx[lo + lane][c] = v[c];
+      W.ref[lo + lane] = r;
+    }
+    if (lane == 0) {
+      const uint32_t mid = lo + W.misc[2], right = lo + W.misc[3];
+      const uint32_t rb[3] = {lo, mid, right}, re[3] = {mid, right, hi};
+      uint32_t cmask = 0, nch = 0;
+      for (int c = 0; c < 3; ++c)
+        if (re[c] > rb[c]) {
+          cmask |= 1u << c;
+          ++nch;
+        }
+      const uint32_t first = atomicAdd(a.node_count, nch);
+      uint32_t top = W.misc[0];
+      if (first + nch > a.node_cap || top + nch > kStackMax) {
+        atomicOr(a.overflow, 1u);
+        W.misc[1] = 1;
+      } else {
+        nd.first_child = first;
+        nd.info = (axis + 1u) | (cmask << 2) | (smask << 5) | (depth << 16);
+        uint32_t k = 0;
+        for (int c = 0; c < 3; ++c) {
+          if (re[c] <= rb[c]) continue;
+          const uint32_t cm = c == 1 ? (smask | (1u << axis)) : smask;
+          W.st_id[top] = first + k++;
+          W.st_rng[top] = pack_rng(rb[c], re[c], cm, depth + 1u);
+          ++top;
+        }
+        W.misc[0] = top;
       }
     }
-    axis_cost[axis] = best;
+    __syncthreads();
   }
-  const uint32_t axis = (axis_cost[0] <= axis_cost[1] && axis_cost[0] <= axis_cost[2])
-                            ? 0u
-                            : ((axis_cost[1] <= axis_cost[0] && axis_cost[1] <= axis_cost[2]) ? 1u : 2u);
-  const double split = axis_pos[axis];
-  // the sequential three-way partition (TriangleAccelerator.cpp:650-701)
-  uint32_t mid = tb, right = te, i = tb;
-  auto swap_el = [&](uint32_t x, uint32_t y) {
-    for (int c = 0; c < 6; ++c) {
-      const float t = a.box[6 * (size_t)x + c];
-      a.box[6 * (size_t)x + c] = a.box[6 * (size_t)y + c];
-      a.box[6 * (size_t)y + c] = t;
-    }
-    const uint32_t t = a.ref[x];
-    a.ref[x] = a.ref[y];
-    a.ref[y] = t;
-  };
-  while (i < right) {
-    const float *bx = a.box + 6 * (size_t)i;
-    if ((double)bx[3 + axis] < split) {
-      swap_el(mid, i);
-      ++mid;
-      ++i;
-    } else if ((double)bx[axis] > split) {
-      --right;
-      swap_el(i, right);
-    } else {
-      ++i;
-    }
+  __syncthreads();
+  if (lane < n0) {
+    for (int c = 0; c < 6; ++c) a.box[6 * (size_t)(base + lane) + c] = W.bx[lane][c];
+    a.ref[base + lane] = W.ref[lane];
   }
-  emit_children(a, id, axis, tb, mid, right, te, smask, depth);
 }
 
 // ------------------------------------------------------------------ one workgroup per larger node
 
 struct Lds {
-  uint32_t lbmin[3][kMaxTries], lbmax[3][kMaxTries], rbmin[3][kMaxTries], rbmax[3][kMaxTries];
-  uint32_t smin[3][2 * kSeg], smax[3][2 * kSeg];
-  uint32_t lcnt[kMaxTries], rcnt[kMaxTries];
-  int32_t mdiff[kMaxTries + 1];
+  Bins B;
   double cost[kMaxTries];
   unsigned long long key[6];
   unsigned long long wsum[kBlock / 64];
   double axis_cost[3], axis_pos[3];
   uint32_t misc[8];
 };
-
-// Exclusive block scan of one 64-bit value per thread; returns the prefix, total in *total.
-__device__ __forceinline__ unsigned long long block_scan(unsigned long long v, unsigned long long *wsum,
-                                                         unsigned long long *total) {
-  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-  unsigned long long x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned long long y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
-  }
-  if (lane == 63) wsum[wv] = x;
-  __syncthreads();
-  unsigned long long before = 0, all = 0;
-  for (uint32_t k = 0; k < kBlock / 64; ++k) {
-    if (k < wv) before += wsum[k];
-    all += wsum[k];
-  }
-  __syncthreads();
-  *total = all;
-  return before + x - v;
-}
-
-__device__ __forceinline__ void seg_update(Lds &L, uint32_t node, const float *bx) {
-  for (int c = 0; c < 3; ++c) {
-    atomicMin(&L.smin[c][node], ord32(bx[c]));
-    atomicMax(&L.smax[c][node], ord32(bx[3 + c]));
-  }
-}
 
 __global__ __launch_bounds__(kBlock) void k_tb_level(TriBuildArgs a) {
   __shared__ Lds L;
@@ -281,42 +287,25 @@ __global__ __launch_bounds__(kBlock) void k_tb_level(TriBuildArgs a) {
   if (tid < 6) L.key[tid] = tid < 3 ? ~0ull : 0ull;
   __syncthreads();
   {
-    unsigned long long k[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
-    for (uint32_t p = tb + tid; p < te; p += kBlock) {
-      const float *bx = a.box + 6 * (size_t)p;
-      for (int c = 0; c < 3; ++c) {
-        const float vmn = bx[c] == 0.0f ? 0.0f : bx[c];
-        const float vmx = bx[3 + c] == 0.0f ? 0.0f : bx[3 + c];
-        const unsigned long long kmn = ((unsigned long long)ord32(vmn) << 32) | p;
-        const unsigned long long kmx = ((unsigned long long)ord32(vmx) << 32) | (0xFFFFFFFFu - p);
-        k[c] = kmn < k[c] ? kmn : k[c];
-        k[3 + c] = kmx > k[3 + c] ? kmx : k[3 + c];
-      }
-    }
-    for (int c = 0; c < 3; ++c) {
-      atomicMin(&L.key[c], k[c]);
-      atomicMax(&L.key[3 + c], k[3 + c]);
-    }
+    BoxKeys k;
+    k.clear();
+    for (uint32_t p = tb + tid; p < te; p += kBlock) k.add(a.box + 6 * (size_t)p, p);
+    keys_merge(L.key, k);
   }
   __syncthreads();
   float mn[3], mx[3];
-  for (int c = 0; c < 3; ++c) {
-    const uint32_t pmn = (uint32_t)(L.key[c] & 0xFFFFFFFFull);
-    const uint32_t pmx = 0xFFFFFFFFu - (uint32_t)(L.key[3 + c] & 0xFFFFFFFFull);
-    mn[c] = a.box[6 * (size_t)pmn + c];
-    mx[c] = a.box[6 * (size_t)pmx + 3 + c];
-  }
+  keys_decode(L.key, a.box, mn, mx);
   if (tid == 0)
     for (int c = 0; c < 3; ++c) {
       nd.bmin[c] = mn[c];
       nd.bmax[c] = mx[c];
     }
-  if (n <= kMaxLeaf || smask == 7u || depth >= kMaxDepth) {
+  if (leaf_by_rule(n, smask, depth)) {
     if (tid == 0) make_leaf(a, id, smask, depth);
     return;
   }
   const double inv_area = 1.0 / (double)area_f(mn, mx);
-  const uint32_t T = kMaxTries < 2 * n ? kMaxTries : 2 * n;
+  const uint32_t T = num_tries(n);
 
   // ---- split search per axis
   for (int axis = 0; axis < 3; ++axis) {
@@ -327,137 +316,36 @@ __global__ __launch_bounds__(kBlock) void k_tb_level(TriBuildArgs a) {
       }
       continue;
     }
-    for (uint32_t i = tid; i < kMaxTries; i += kBlock) {
-      for (int c = 0; c < 3; ++c) {
-        L.lbmin[c][i] = L.rbmin[c][i] = kOrdPosInf;
-        L.lbmax[c][i] = L.rbmax[c][i] = kOrdNegInf;
-      }
-      L.lcnt[i] = L.rcnt[i] = 0;
-    }
-    for (uint32_t i = tid; i < 2 * kSeg; i += kBlock)
-      for (int c = 0; c < 3; ++c) {
-        L.smin[c][i] = kOrdPosInf;
-        L.smax[c][i] = kOrdNegInf;
-      }
-    for (uint32_t i = tid; i <= kMaxTries; i += kBlock) L.mdiff[i] = 0;
+    bins_clear(L.B, tid, kBlock);
     __syncthreads();
-    const double coef = (double)((float)T / (mx[axis] - mn[axis]));
+    const double coef = bin_coef(T, mn[axis], mx[axis]);
     const float lo = mn[axis];
     for (uint32_t p = tb + tid; p < te; p += kBlock) {
       const float *bx = a.box + 6 * (size_t)p;
       uint32_t lbg, re;
       bins_of(bx, axis, lo, coef, T, lbg, re);
-      if (re > 0) {
-        for (int c = 0; c < 3; ++c) {
-          atomicMin(&L.rbmin[c][re - 1], ord32(bx[c]));
-          atomicMax(&L.rbmax[c][re - 1], ord32(bx[3 + c]));
-        }
-        atomicAdd(&L.rcnt[re - 1], 1u);
-      }
-      if (lbg < T) {
-        for (int c = 0; c < 3; ++c) {
-          atomicMin(&L.lbmin[c][lbg], ord32(bx[c]));
-          atomicMax(&L.lbmax[c][lbg], ord32(bx[3 + c]));
-        }
-        atomicAdd(&L.lcnt[lbg], 1u);
-      }
-      if (re < lbg) {
-        atomicAdd(&L.mdiff[re], 1);
-        atomicAdd(&L.mdiff[lbg], -1);
-        // the boxes of bins [re, lbg): canonical segments of the segment tree
-        uint32_t l = re + kSeg, r = lbg + kSeg;
-        while (l < r) {
-          if (l & 1u) seg_update(L, l++, bx);
-          if (r & 1u) seg_update(L, --r, bx);
-          l >>= 1;
-          r >>= 1;
-        }
-      }
+      bins_add(L.B, bx, lbg, re, T);
     }
     __syncthreads();
-    // candidate costs: cumulative left / right unions, middle = union over the leaf's ancestors
-    for (uint32_t i = tid; i < T; i += kBlock) {
-      double lmn[3], lmx[3], rmn[3], rmx[3], mmn[3], mmx[3];
-      uint64_t ln = 0, rn = 0;
-      int64_t mc = 0;
-      for (int c = 0; c < 3; ++c) {
-        uint32_t a0 = kOrdPosInf, a1 = kOrdNegInf;
-        for (uint32_t j = 0; j <= i; ++j) {
-          a0 = L.lbmin[c][j] < a0 ? L.lbmin[c][j] : a0;
-          a1 = L.lbmax[c][j] > a1 ? L.lbmax[c][j] : a1;
-        }
-        lmn[c] = (double)unord32(a0);
-        lmx[c] = (double)unord32(a1);
-        a0 = kOrdPosInf;
-        a1 = kOrdNegInf;
-        for (uint32_t j = i; j < T; ++j) {
-          a0 = L.rbmin[c][j] < a0 ? L.rbmin[c][j] : a0;
-          a1 = L.rbmax[c][j] > a1 ? L.rbmax[c][j] : a1;
-        }
-        rmn[c] = (double)unord32(a0);
-        rmx[c] = (double)unord32(a1);
-        a0 = kOrdPosInf;
-        a1 = kOrdNegInf;
-        for (uint32_t s = i + kSeg; s >= 1; s >>= 1) {
-          a0 = L.smin[c][s] < a0 ? L.smin[c][s] : a0;
-          a1 = L.smax[c][s] > a1 ? L.smax[c][s] : a1;
-        }
-        mmn[c] = (double)unord32(a0);
-        mmx[c] = (double)unord32(a1);
-      }
-      for (uint32_t j = 0; j <= i; ++j) ln += L.lcnt[j];
-      for (uint32_t j = i; j < T; ++j) rn += L.rcnt[j];
-      for (uint32_t j = 0; j <= i; ++j) mc += L.mdiff[j];
-      const double p1 = dmin_std(1.0, area_d(lmn, lmx) * inv_area);
-      const double p2 = dmin_std(1.0, area_d(mmn, mmx) * inv_area);
-      const double p3 = dmin_std(1.0, area_d(rmn, rmx) * inv_area);
-      double cost = (double)ln * p1 + (double)mc * p2 + (double)rn * p3;
-      if (ln == 0 || rn == 0) cost *= 0.8;
-      L.cost[i] = cost;
-    }
+    for (uint32_t i = tid; i < T; i += kBlock) L.cost[i] = bins_cost(L.B, i, T, inv_area);
     __syncthreads();
-    if (tid == 0) {
-      double best = __builtin_huge_val();
-      uint32_t bi = 0;
-      bool any = false;
-      for (uint32_t i = 0; i < T; ++i)
-        if (L.cost[i] < best) {
-          best = L.cost[i];
-          bi = i;
-          any = true;
-        }
-      L.axis_cost[axis] = best;
-      double pos = 0.0;
-      if (any) {
-        const float ext = mx[axis] - mn[axis];
-        const float step = ((float)bi * ext) / (float)T;
-        pos = (double)(mn[axis] + step);
-      }
-      L.axis_pos[axis] = pos;
-    }
+    if (tid == 0) first_min(L.cost, T, mn[axis], mx[axis], L.axis_cost[axis], L.axis_pos[axis]);
     __syncthreads();
   }
   __syncthreads();  // a masked axis's entries are written without a barrier of their own
-  const double c0 = L.axis_cost[0], c1 = L.axis_cost[1], c2 = L.axis_cost[2];
-  const uint32_t axis = (c0 <= c1 && c0 <= c2) ? 0u : ((c1 <= c0 && c1 <= c2) ? 1u : 2u);
+  const uint32_t axis = pick_axis(L.axis_cost[0], L.axis_cost[1], L.axis_cost[2]);
   const double split = L.axis_pos[axis];
 
   // ---- exact parallel form of the three-way partition (positions relative to tb)
   uint32_t *cls = a.s_cls + tb, *rf = a.s_rf + tb, *nb = a.s_nb + tb, *G = a.s_G + tb, *Rpos = a.s_Rpos + tb;
   uint32_t *order = a.s_order + tb, *elem = a.s_elem + tb, *ptr = a.s_ptr + tb, *dest = a.s_dest + tb;
-  auto class_of = [&](uint32_t k) -> uint32_t {
-    const float *bx = a.box + 6 * (size_t)(tb + k);
-    if ((double)bx[3 + axis] < split) return 0u;  // left
-    if ((double)bx[axis] > split) return 2u;      // right
-    return 1u;                                    // middle
-  };
   // classes; right elements before each front position; non-right ones before each back position
   unsigned long long carry_r = 0, carry_b = 0, tot = 0;
   for (uint32_t base = 0; base < n; base += kBlock) {
     const uint32_t k = base + tid;
     uint32_t ck = 3u;
     if (k < n) {
-      ck = class_of(k);
+      ck = class_of(a.box + 6 * (size_t)(tb + k), axis, split);
       cls[k] = ck;
     }
     const unsigned long long pre = block_scan(k < n && ck == 2u ? 1ull : 0ull, L.wsum, &tot);
@@ -599,7 +487,8 @@ hipError_t launch_tri_build_level(const TriBuildArgs &a, hipStream_t stream) {
 
 hipError_t launch_tri_build_small(const TriBuildArgs &a, hipStream_t stream) {
   if (a.list_n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tb_small, dim3((a.list_n + 63) / 64), dim3(64), 0, stream, a);
+  if (a.small_max > kWaveMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tb_subtree, dim3(a.list_n), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 

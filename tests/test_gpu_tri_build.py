@@ -5,7 +5,8 @@ axes, child masks and triangle ranges in the same depth-first layout, the same t
 counts. Checked against the oracle's restatement of the builder on the C0 scene, the reference's
 tetra-sphere fixture, seeded spheres, the 200-deep staircase, coplanar and duplicated triangles and
 signed zeros (whose node boxes keep the sign of the first extreme in array order), at several
-small-node thresholds (which move nodes between the workgroup kernel and the per-thread kernel);
+small- and wide-node thresholds (which move nodes between the one-wave subtree kernel, the
+workgroup-per-node kernel and the chunked multi-workgroup kernels);
 and against the product's host builder (itself pinned to the oracle) on the 2.03M-triangle room
 and the batched scenes. The traversal of a device-built scene returns the host-built scene's hit
 records bit for bit."""
@@ -59,17 +60,21 @@ def _vs_host(meshes, prims=None):
 
 
 @pytest.fixture
-def small_max():
-    old = os.environ.get("SKW_TRIBUILD_SMALL_MAX")
+def build_env():
+    """Sets the builder's size thresholds for one test (restored afterwards)."""
+    names = ("SKW_TRIBUILD_SMALL_MAX", "SKW_TRIBUILD_WIDE_MIN", "SKW_TRIBUILD_CHUNK")
+    old = {k: os.environ.get(k) for k in names}
 
-    def set_(v):
-        os.environ["SKW_TRIBUILD_SMALL_MAX"] = str(v)
+    def set_(**kw):
+        for k, v in kw.items():
+            os.environ["SKW_TRIBUILD_" + k.upper()] = str(v)
 
     yield set_
-    if old is None:
-        os.environ.pop("SKW_TRIBUILD_SMALL_MAX", None)
-    else:
-        os.environ["SKW_TRIBUILD_SMALL_MAX"] = old
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
 
 
 def coplanar_plane(n=40, seed=1):
@@ -121,10 +126,20 @@ def test_reference_fixture():
 
 
 @pytest.mark.parametrize("sm", [1, 4, 32, 256])
-def test_small_node_threshold_does_not_change_the_tree(small_max, sm):
-    small_max(sm)
+def test_small_node_threshold_does_not_change_the_tree(build_env, sm):
+    build_env(small_max=sm)
     _vs_oracle(scenes.small_spheres(seed=4, n=5, subdiv=3))
     _vs_oracle(signed_zero_soup())
+
+
+@pytest.mark.parametrize("wide_min,chunk", [(256, 256), (1000, 512), (3000, 4096)])
+def test_wide_node_threshold_does_not_change_the_tree(build_env, wide_min, chunk):
+    # nodes above wide_min triangles are spread over chunks (one workgroup each)
+    build_env(wide_min=wide_min, chunk=chunk)
+    _vs_oracle(scenes.small_spheres(seed=4, n=5, subdiv=3))
+    _vs_oracle(signed_zero_soup())
+    _vs_oracle(coplanar_plane())
+    _vs_oracle(scenes.staircase())
 
 
 def test_staircase_reaches_the_depth_limit():
@@ -133,8 +148,8 @@ def test_staircase_reaches_the_depth_limit():
 
 
 @pytest.mark.parametrize("sm", [1, 32])
-def test_coplanar_and_duplicated_triangles(small_max, sm):
-    small_max(sm)
+def test_coplanar_and_duplicated_triangles(build_env, sm):
+    build_env(small_max=sm)
     _vs_oracle(coplanar_plane())
 
 
