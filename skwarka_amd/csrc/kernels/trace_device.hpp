@@ -537,6 +537,81 @@ struct Walker {
     if (ANY && stop) sp = 0;  // the walk ends at its first accepted hit
   }
 
+  // The held leaves of the wave tested together; every lane of the wave calls it (hold: this lane
+  // holds an accepted leaf). Leaves with instances or with more than kPoolMax triangles are tested
+  // by their own lane as in leaf(). The other leaves' triangles are spread over the wave's lanes,
+  // 64 per round: each is tested with its owner's ray and the owner's tmax at the round's start,
+  // then every owner takes its own results in its leaf's order and accepts t <= its current tmax.
+  // That accepts exactly the triangles leaf() accepts: a test depends on tmax only through
+  // t <= tmax, and tmax only shrinks along the leaf (so a rejected t stays rejected).
+  static constexpr uint32_t kPoolMax = 4;
+  __device__ __forceinline__ void leaves_pooled(bool hold) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t n_all = hold ? (cb >> 3) : 0u;
+    const bool solo = hold && ((INST && (cb & 4u)) || n_all > kPoolMax);
+    if (solo) leaf();
+    const bool pooled = hold && !solo;
+    const uint32_t cnt = pooled ? n_all : 0u;
+    uint32_t incl = cnt;  // inclusive prefix of the pooled triangle counts over the wave
+    for (uint32_t d = 1; d < (uint32_t)kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d);
+      if (lane >= d) incl += y;
+    }
+    const uint32_t off = incl - cnt;
+    const uint32_t total = __shfl(incl, kWave - 1);
+    if (pooled) {
+      at_leaf = false;
+      have = false;
+    }
+    bool stop = false;
+    for (uint32_t base = 0; base < total; base += kWave) {
+      const uint32_t j = base + lane;
+      // the owner of triangle j: the last lane whose offset is <= j
+      uint32_t o = 0;
+      for (uint32_t step = kWave / 2; step > 0; step >>= 1) {
+        const uint32_t om = __shfl(off, o + step);
+        if (om <= j) o += step;
+      }
+      RayState rr;
+      rr.o0 = __shfl(r.o0, o);
+      rr.o1 = __shfl(r.o1, o);
+      rr.o2 = __shfl(r.o2, o);
+      rr.d0 = __shfl(r.d0, o);
+      rr.d1 = __shfl(r.d1, o);
+      rr.d2 = __shfl(r.d2, o);
+      rr.tmin = __shfl(r.tmin, o);
+      const double otmax = __shfl(tmax, o);
+      const uint32_t opos = __shfl(ca, o) + (j - __shfl(off, o));
+      double t = 0.0, b1 = 0.0, b2 = 0.0;
+      bool h = false;
+      if (j < total) h = tri_test(tris, opos, rr, otmax, t, b1, b2);
+      for (uint32_t k = 0; k < kPoolMax; ++k) {
+        const uint32_t jj = off + k;
+        const bool mine = k < cnt && jj >= base && jj < base + kWave;
+        const uint32_t src = mine ? jj - base : lane;
+        const double tt = __shfl(t, src), bb1 = __shfl(b1, src), bb2 = __shfl(b2, src);
+        const bool hh = __shfl(h ? 1 : 0, src) != 0;
+        if (mine && !stop) {
+          if (STATS) ++tests;
+          if (hh && tt <= tmax) {
+            found = true;
+            if (ANY) {
+              stop = true;
+            } else {
+              tmax = tt;
+              best.hit = true;
+              best.inst = false;
+              best.pos = ca + k;
+              best.b1 = bb1;
+              best.b2 = bb2;
+            }
+          }
+        }
+      }
+    }
+    if (ANY && stop) sp = 0;  // the walk ends at its first accepted hit
+  }
+
   // One move of either kind (for callers that do not group leaves); true when the walk has ended.
   __device__ __forceinline__ bool step() {
     if (at_leaf) {

@@ -2,9 +2,9 @@
 // trace_device.hpp).
 //
 // Scheduling: one ray per lane, 64-lane workgroups, a persistent grid sized to the chip. A wave
-// iteration has three phases, each run only for the lanes that need it: refill (idle lanes load
-// a new ray and test the root), leaf (lanes holding an accepted leaf test its triangles) and
-// advance (every other busy lane pops or descends one internal node). Refill and leaf phases are
+// iteration has three phases: refill (idle lanes load a new ray and test the root), leaf (the
+// triangles of the leaves the lanes hold, spread over all 64 lanes) and advance (every other busy
+// lane pops or descends one internal node). Refill and leaf phases are
 // the expensive, rarely needed ones, so a wave defers them until refill_min / leaf_min lanes wait
 // for them (or no lane could move otherwise): each then runs for many lanes at once instead of
 // for one or two lanes per iteration. Leaves are no longer deferred once the rays run out.
@@ -41,11 +41,13 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
   // The wave's pool of claimed, unstarted rays [pool, pool_end): the same value in every lane.
   uint64_t pool = 0, pool_end = 0;
   bool drained = false;  // some lane found no ray left (the same value in every lane)
+  // Every lane stays in the loop until the wave is done (the pooled leaf tests read other lanes'
+  // registers); a lane without a ray left stays idle.
   while (true) {
     // Refill: idle lanes take new rays once at least refill_min of them wait (or nothing else can
     // move), so the ray load and root test run for many lanes at a time.
     const uint64_t idle = __ballot(!busy);
-    if (idle) {
+    if (idle && !drained) {
       const uint64_t others = __ballot(busy);
       if (first || (uint64_t)__popcll(idle) >= args.refill_min || others == 0) {
         if (!first) {
@@ -68,9 +70,8 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
           }
         }
         // Once the launch's rays run out, deferring leaves only delays the last walks.
-        drained = drained || __ballot(!busy && idx >= n) != 0;
-        if (!busy) {
-          if (idx >= n) break;
+        drained = __ballot(!busy && idx >= n) != 0;
+        if (!busy && idx < n) {
           RayState r;
           double tmax;
           load_ray(args.rays, idx, r, tmax);
@@ -79,15 +80,14 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
         }
       }
     }
+    if (__ballot(busy) == 0) break;  // nothing to walk and (the refill ran) no ray left
     first = false;
     // Leaves: lanes holding an accepted leaf test it together once at least leaf_min of them
     // hold one (or no other lane can move).
     const uint64_t held = __ballot(busy && w.at_leaf);
     if (held) {
       const uint64_t moving = __ballot(busy && !w.at_leaf);
-      if ((uint64_t)__popcll(held) >= args.leaf_min || moving == 0 || drained) {
-        if (busy && w.at_leaf) w.leaf();
-      }
+      if ((uint64_t)__popcll(held) >= args.leaf_min || moving == 0 || drained) w.leaves_pooled(busy && w.at_leaf);
     }
     // Between leaves: one move per lane; the NaN-free slab test when every busy lane allows it.
     const bool nan_free = __ballot(busy && !w.nan_free) == 0;
