@@ -452,6 +452,41 @@ __device__ __forceinline__ void finish_query(const GatherArgs &a, const Query q,
   }
 }
 
+// An upper bound for a nearest lookup (MODE 1) from the photons on the query's near path (the
+// first nodes the lookup visits): the least d^2 below r2 among them that passes the filter. The
+// nearest is the first photon, in the lookup's fixed visiting order, whose d^2 is the least of
+// those below the start radius (a pruned side holds only photons at or beyond the radius, which
+// cannot replace the current one under the strict comparison); every photon at or below the bound
+// c is below the next double after c, so a lookup started there returns the same photon and d^2,
+// and prunes far sides from its first node on. Returns r2 when no photon on the path qualifies.
+__device__ __forceinline__ double near_path_bound(const GatherArgs &a, const Query &q, double r2) {
+  double best = r2;
+  bool any = false;
+  uint32_t cur = 0;
+  for (uint32_t depth = 0; depth < 64; ++depth) {
+    const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
+    const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
+    const double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < best && (a.filter == 0 || photon_filter(a, q, __float_as_uint(ph.w), dx, dy, dz))) {
+      best = d2;
+      any = true;
+    }
+    const uint32_t w = __float_as_uint(nd.w);
+    const uint32_t axis = w & 3u;
+    if (axis == 3u) break;
+    const float sx = nd.x, sy = nd.y, sz = nd.z;
+    const float split = axis == 0 ? sx : (axis == 1 ? sy : sz);
+    const bool left_near = axis_coord(q, axis) <= (double)split;
+    const uint32_t right = w >> 3;
+    const bool near_exists = left_near ? (w & 4u) != 0 : right < a.n_nodes;
+    if (!near_exists) break;
+    cur = left_near ? cur + 1 : right;
+  }
+  // the next double above best (best is a finite non-negative d^2 here)
+  return any ? __longlong_as_double(__double_as_longlong(best) + 1) : r2;
+}
+
 // The sums are kept during the lookup only for the disc estimate.
 __device__ __forceinline__ bool disc_sums(const GatherArgs &a) { return a.estimate == 0 && a.e_ext != nullptr; }
 
@@ -474,6 +509,8 @@ __device__ void gather_one(const GatherArgs &a, uint64_t t, const Stack &stk) {
   DiscSums sums{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false};
   Counts cnt{0u, 0u};
   if (MODE == 1) {
+    // (the statistics instance counts the visits of the lookup from the full radius)
+    if (!STATS && a.n_nodes > 0) max_d2 = near_path_bound(a, q, max_d2);
     lookup<1, false, true, STATS>(a, q, max_d2, stk, size, nearest, heap, sums, cnt);
   } else if (a.k > 0) {
     if (disc_sums(a)) {
