@@ -86,10 +86,13 @@ skw_status upload_irradiance(skw_irradiance_map *m) {
 
 namespace {
 
+// The irradiance photons' tree, with the normal boxes its nearest lookups prune by.
 skw_status build_irr_tree(skw_irradiance_map *m) {
   const uint64_t n = m->normal.size();
   std::vector<uint16_t> dir(n, 0);
-  return skw_photon_map_create(n, m->pos.data(), m->e_ext.data(), dir.data(), m->normal.data(), 0, &m->map);
+  skw_status st = skw_photon_map_create(n, m->pos.data(), m->e_ext.data(), dir.data(), m->normal.data(), 0, &m->map);
+  if (st != SKW_OK) return st;
+  return photon_map_nbox(m->map, nullptr);
 }
 
 // The lookup distance: 0.1 % percentile (index max(size / 1000, 1) of the descending order) of the

@@ -87,6 +87,7 @@ struct skw_photon_map {
   float *d_psplit = nullptr;
   uint32_t *d_perm = nullptr;
   double *d_cdir = nullptr;
+  uint64_t *d_nbox = nullptr;  // per node: its subtree's normal box (irradiance maps; launch_kd_nbox)
   bool pooled = false;      // arrays from the device pool (device builds)
   bool cdir_shared = false; // d_cdir is the device's shared direction table
   ~skw_photon_map();
@@ -130,6 +131,9 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
 // The host layout of a scene without instanced meshes, its tree built on the current device
 // (node for node the host builder's tree); SKW_UNSUPPORTED for instanced or non-finite scenes.
 skw_status build_triangle_tree_device(const skw_scene_desc &desc, TreeLayout &out);
+
+// The per-node normal boxes of a map whose device arrays are complete (synchronises `st`).
+skw_status photon_map_nbox(skw_photon_map *m, hipStream_t st);
 
 // Device tables of a scene / photon map whose host layout is complete (built or loaded).
 skw_status finish_scene(std::unique_ptr<skw_scene> &s, bool host_only);

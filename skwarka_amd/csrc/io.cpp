@@ -414,6 +414,7 @@ skw_status skw_irradiance_map_load(const char *path, skw_irradiance_map **out) {
     ok = read_kd(r, pm->host) && pm->host.nodes.size() == n;
     if (ok) {
       if ((st = finish_photon_map(pm.get())) != SKW_OK) return st;
+      if ((st = photon_map_nbox(pm.get(), nullptr)) != SKW_OK) return st;
       m->map = pm.release();
     }
   }

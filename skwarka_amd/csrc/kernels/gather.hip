@@ -60,6 +60,62 @@ __global__ __launch_bounds__(256) void k_irr_sum(const float *a, const float *b,
 }
 }  // namespace
 
+namespace {
+constexpr uint64_t kNboxMax = 2048;  // larger subtrees get the whole cube (they are rarely prunable)
+
+// Node b's subtree holds the positions [b, e): e - 1 is its last node in pre-order, reached by
+// following right children (or the left child where there is no right one) down to a leaf.
+__global__ __launch_bounds__(256) void k_kd_nbox(const DevKdNode *__restrict__ nodes,
+                                                 const DevPhoton *__restrict__ photons, const double *__restrict__ cdir,
+                                                 uint64_t n, uint64_t *__restrict__ nbox) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  uint64_t cur = b;
+  for (int d = 0; d < 64; ++d) {
+    const uint32_t w = __float_as_uint(reinterpret_cast<const float4 *>(nodes)[cur].w);
+    if ((w & 3u) == 3u) break;
+    const uint64_t right = w >> 3;
+    if (right < n)
+      cur = right;
+    else if (w & 4u)
+      cur = cur + 1;
+    else
+      break;
+  }
+  const uint64_t e = cur + 1;
+  double lo[3] = {1.0, 1.0, 1.0}, hi[3] = {-1.0, -1.0, -1.0};
+  if (e - b > kNboxMax) {
+    lo[0] = lo[1] = lo[2] = -1.0;
+    hi[0] = hi[1] = hi[2] = 1.0;
+  } else {
+    for (uint64_t p = b; p < e; ++p) {
+      const double *nv = cdir + 3 * (size_t)(photons[p].codes >> 16);
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = fmin(lo[c], nv[c]);
+        hi[c] = fmax(hi[c], nv[c]);
+      }
+    }
+  }
+  uint64_t box = 0;
+  for (int c = 0; c < 3; ++c) {
+    // outward to multiples of 1/127 (a margin keeps the rounding of x * 127 outward too)
+    const int ql = (int)fmax(-127.0, floor(lo[c] * 127.0 - 1e-6));
+    const int qh = (int)fmin(127.0, ceil(hi[c] * 127.0 + 1e-6));
+    box |= (uint64_t)(uint8_t)(int8_t)ql << (8 * c);
+    box |= (uint64_t)(uint8_t)(int8_t)qh << (8 * (c + 3));
+  }
+  nbox[b] = box;
+}
+}  // namespace
+
+hipError_t launch_kd_nbox(const DevKdNode *nodes, const DevPhoton *photons, const double *cdir, uint64_t n,
+                          uint64_t *nbox, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_kd_nbox, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, nodes, photons, cdir, n,
+                     nbox);
+  return hipGetLastError();
+}
+
 hipError_t launch_irr_sum(const float *a, const float *b, const float *c, float *out, uint64_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_irr_sum, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, a, b, c, out, n);

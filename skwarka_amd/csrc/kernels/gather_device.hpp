@@ -141,6 +141,19 @@ __device__ __forceinline__ bool photon_filter(const GatherArgs &a, const Query &
   return fabs(along) < q.sin_t * sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// True when every normal in the box fails the PhotonFilter's test (decoded normal . query normal
+// > 0.87): the dot product's upper bound over the box is below 0.87 by a margin far above the
+// rounding of either computation.
+__device__ __forceinline__ bool normals_fail(uint64_t box, const Query &q) {
+  const double s = 1.0 / 127.0;
+  const double lo0 = (double)(int8_t)(box & 0xFF) * s, lo1 = (double)(int8_t)((box >> 8) & 0xFF) * s,
+               lo2 = (double)(int8_t)((box >> 16) & 0xFF) * s;
+  const double hi0 = (double)(int8_t)((box >> 24) & 0xFF) * s, hi1 = (double)(int8_t)((box >> 32) & 0xFF) * s,
+               hi2 = (double)(int8_t)((box >> 40) & 0xFF) * s;
+  const double bound = fmax(q.n0 * lo0, q.n0 * hi0) + fmax(q.n1 * lo1, q.n1 * hi1) + fmax(q.n2 * lo2, q.n2 * hi2);
+  return bound < kCosThreshold - 1e-9;
+}
+
 // Adds a photon's weight to the exitant (ext) or the incoming-side sums (PhotonLTEIntegrator.cpp:
 // 150-153). Both sums are updated with value selects (the other one gains +0.0, which leaves a
 // sum of non-negative weights unchanged bit for bit); an if/else over the sums would let the
@@ -184,9 +197,15 @@ __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint3
                                       bool &far_exists, uint32_t &far, uint32_t &axis, float &split, bool &filled,
                                       Counts &cnt) {
   const uint32_t k = a.k;
-  // The node and its photon record are independent loads, issued together.
+  // The node, its photon record and its subtree's normal box are independent loads, issued together.
   const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
   const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
+  if (MODE == 1 && !STATS && a.nbox && a.filter != 0 && normals_fail(a.nbox[cur], q)) {
+    // no photon of this subtree passes the PhotonFilter's normal test: none can enter the result
+    // (the statistics instance walks it, counting the reference's visits)
+    far_exists = false;
+    return false;
+  }
   double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
   double d2 = dx * dx + dy * dy + dz * dz;
   if (STATS) ++cnt.visits;

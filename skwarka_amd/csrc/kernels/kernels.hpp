@@ -51,6 +51,12 @@ struct GeomArgs {
 hipError_t launch_fix_bary(const GeomArgs &args, hipStream_t stream);
 hipError_t launch_hit_geometry(const GeomArgs &args, hipStream_t stream);
 
+// Per node: an outward-rounded box of the decoded normals of the photons in its subtree, as six
+// int8 components (min x y z, max x y z) over 127, in one uint64 (subtrees of more than 2,048
+// photons: the whole cube). Computed for the irradiance photon maps (built or loaded).
+hipError_t launch_kd_nbox(const DevKdNode *nodes, const DevPhoton *photons, const double *cdir, uint64_t n,
+                          uint64_t *nbox, hipStream_t stream);
+
 struct GatherArgs {
   const DevKdNode *nodes;
   const DevPhoton *photons;
@@ -92,6 +98,9 @@ struct GatherArgs {
   // 1: a disc estimate runs its lookup first without writing the set's slots and repeats it with
   // them only when the set fills (set by the host when neither ids nor d^2 are returned)
   int defer;
+  // optional per-node bounds of the subtree's photon normals (launch_kd_nbox); a nearest lookup
+  // with the PhotonFilter does not walk a subtree none of whose photons can pass its normal test
+  const uint64_t *nbox;
   // statistics instance: per query (visits, photons inside the radius, final set size)
   uint32_t *stats;
 };
