@@ -225,6 +225,16 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     if (prof && e == hipSuccess) e = hipEventRecord(ev[ev_next++], st);
   };
   uint64_t *sums = count ? a.totals + 3 : nullptr;
+  // The occlusion and BSDF traces of a pass are independent: the BSDF trace runs on a second
+  // stream beside the occlusion trace, so one launch's tail (its last long walks) overlaps the
+  // other. A profiled frame keeps one stream, so that its events time each stage alone.
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  if (!prof) {
+    if (e == hipSuccess) e = side_stream(&st2);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming);
+  }
   if (e == hipSuccess) e = launch_dl_clear(a, st);
   if (e == hipSuccess && adj) e = launch_dl_clear_grad(out.grad, 2 * dl->n_groups + nl, st);
   mark();
@@ -240,13 +250,28 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     e = launch_dl_shade(a, st);
     if (e != hipSuccess) break;
     mark();
+    if (st2) {
+      e = hipEventRecord(ev_fork, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st2, ev_fork, 0);
+      if (e != hipSuccess) break;
+      rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st2, false, nullptr,
+                         nullptr, false, sums ? sums + 4 : nullptr);
+      if (rs != SKW_OK) break;
+      e = hipEventRecord(ev_join, st2);
+      if (e != hipSuccess) break;
+    }
     rs = enqueue_trace(scene, a.shadow_rays, batch * nls, a.counts, nullptr, a.occluded, st, true, nullptr, nullptr,
                        false, sums ? sums + 2 : nullptr);
     if (rs != SKW_OK) break;
     mark();
-    rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
-                       nullptr, false, sums ? sums + 4 : nullptr);
-    if (rs != SKW_OK) break;
+    if (st2) {
+      e = hipStreamWaitEvent(st, ev_join, 0);
+      if (e != hipSuccess) break;
+    } else {
+      rs = enqueue_trace(scene, a.bsdf_rays, batch * nbs, a.counts + 1, a.bsdf_hits, nullptr, st, false, nullptr,
+                         nullptr, false, sums ? sums + 4 : nullptr);
+      if (rs != SKW_OK) break;
+    }
     mark();
     e = launch_dl_resolve(a, st);
     mark();
@@ -275,6 +300,9 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);
+  if (st2 && (rs != SKW_OK || e != hipSuccess)) (void)hipStreamSynchronize(st2);  // nothing left reading `base`
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
   pool_free(base, st);
   if (rs != SKW_OK) return rs;
   if (e != hipSuccess) return hip_fail(e, "direct lighting frame");

@@ -244,6 +244,22 @@ hipError_t shared_cdir(double **out) {
   return hipSuccess;
 }
 
+hipError_t side_stream(hipStream_t *out) {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {nullptr};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!streams[dev]) {
+    e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+  }
+  *out = streams[dev];
+  return hipSuccess;
+}
+
 const KdLayout &photon_map_host(const skw_photon_map *map) {
   skw_photon_map *m = const_cast<skw_photon_map *>(map);
   std::lock_guard<std::mutex> lock(m->host_mu);

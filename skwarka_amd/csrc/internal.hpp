@@ -150,6 +150,9 @@ skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, co
 // either codes = dir | normal << 16 or the two id arrays; weights / ids may be NULL).
 // The direction table of CompressedDirection on the current device, uploaded once and shared.
 hipError_t shared_cdir(double **out);
+// A second stream of the current device (default priority, non-blocking), kept for the life of the
+// process, on which a wavefront runs one of two independent stages beside the caller's stream.
+hipError_t side_stream(hipStream_t *out);
 skw_status photon_map_build_device(uint64_t n, const float *d_pos, const float *d_weight, const uint32_t *d_codes,
                                    const uint16_t *d_dir, const uint16_t *d_normal, skw_photon_map **out,
                                    hipStream_t st);
