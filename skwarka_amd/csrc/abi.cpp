@@ -269,10 +269,8 @@ int skw_device_count(void) { return device_count(); }
 }  // extern "C"
 
 namespace skw {
-// Device tables of a scene whose host layout is complete (built or loaded).
-skw_status finish_scene(std::unique_ptr<skw_scene> &sp, bool host_only) {
-  skw_scene *s = sp.get();
-  s->has_inst = !s->host.insts.empty();
+skw_status scene_tables(skw_scene *s, bool host_only) {
+  if (s->tables_done) return SKW_OK;
   std::vector<uint32_t> users(s->host.meshes.size(), 0);
   for (const HostPrim &p : s->host.prims) users[p.mesh]++;
   std::vector<DevPrim> prims(s->host.prims.size());
@@ -287,7 +285,10 @@ skw_status finish_scene(std::unique_ptr<skw_scene> &sp, bool host_only) {
     prims[i].pad[0] = prims[i].pad[1] = 0;
     if (!prims[i].identity) s->needs_fix = true;
   }
-  if (host_only) return SKW_OK;
+  if (host_only) {
+    s->tables_done = true;
+    return SKW_OK;
+  }
   SKW_HIP(hipGetDevice(&s->device), "hipGetDevice");
   SKW_HIP(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device), "hipDeviceGetAttribute");
   std::vector<DevMesh> meshes(s->host.meshes.size());
@@ -310,15 +311,26 @@ skw_status finish_scene(std::unique_ptr<skw_scene> &sp, bool host_only) {
     }
     mt.insert(mt.end(), m.tris.begin(), m.tris.end());
   }
-  SKW_HIP(upload(&s->d_nodes, s->host.nodes), "upload nodes");
-  SKW_HIP(upload(&s->d_tris, s->host.tris), "upload triangles");
-  SKW_HIP(upload(&s->d_insts, s->host.insts), "upload instances");
-  SKW_HIP(upload(&s->d_inst_range, s->host.inst_range), "upload instance ranges");
   SKW_HIP(upload(&s->d_prims, prims), "upload primitives");
   SKW_HIP(upload(&s->d_meshes, meshes), "upload meshes");
   SKW_HIP(upload(&s->d_mesh_verts, mv), "upload mesh vertices");
   SKW_HIP(upload(&s->d_mesh_normals, mn), "upload mesh normals");
   SKW_HIP(upload(&s->d_mesh_tris, mt), "upload mesh triangles");
+  s->tables_done = true;
+  return SKW_OK;
+}
+
+// Device tables of a scene whose tree is complete (built on the host or the device, or loaded).
+skw_status finish_scene(std::unique_ptr<skw_scene> &sp, bool host_only) {
+  skw_scene *s = sp.get();
+  s->has_inst = !s->host.insts.empty();
+  skw_status st = scene_tables(s, host_only);
+  if (st != SKW_OK || host_only) return st;
+  // a device-built tree is already in place
+  if (!s->d_nodes) SKW_HIP(upload(&s->d_nodes, s->host.nodes), "upload nodes");
+  if (!s->d_tris) SKW_HIP(upload(&s->d_tris, s->host.tris), "upload triangles");
+  SKW_HIP(upload(&s->d_insts, s->host.insts), "upload instances");
+  SKW_HIP(upload(&s->d_inst_range, s->host.inst_range), "upload instance ranges");
   s->on_device = true;
   return SKW_OK;
 }
@@ -335,12 +347,12 @@ skw_status skw_scene_create(const skw_scene_desc *desc, uint32_t flags, skw_scen
   if (host_only && device_build) return fail(SKW_INVALID_ARG, "SKW_SCENE_DEVICE_BUILD needs a device");
   std::unique_ptr<skw_scene> s(new skw_scene());
   std::string err;
+  bool built = false;
   if (device_build) {
-    skw_status bs = build_triangle_tree_device(*desc, s->host);
+    skw_status bs = build_triangle_tree_device(*desc, s.get(), built);
     if (bs != SKW_OK) return bs;
-  } else if (!build_triangle_tree(*desc, s->host, err)) {
-    return fail(SKW_INVALID_ARG, err);
   }
+  if (!built && !build_triangle_tree(*desc, s->host, err)) return fail(SKW_INVALID_ARG, err);
   skw_status st = finish_scene(s, host_only);
   if (st != SKW_OK) return st;
   *out = s.release();
@@ -360,8 +372,8 @@ skw_status skw_scene_bounds(const skw_scene *scene, double out[6]) {
 
 skw_status skw_scene_info(const skw_scene *scene, uint64_t out[6]) {
   if (!scene || !out) return fail(SKW_INVALID_ARG, "scene and out must not be NULL");
-  out[0] = scene->host.nodes.size();
-  out[1] = scene->host.tris.size();
+  out[0] = scene->host_tree ? scene->host.nodes.size() : scene->n_nodes;
+  out[1] = scene->host_tree ? scene->host.tris.size() : scene->n_tris;
   out[2] = scene->host.max_depth;
   out[3] = scene->host.insts.size();
   out[4] = scene->host.leaves;
@@ -373,7 +385,10 @@ skw_status skw_scene_export_tree(const skw_scene *scene, float *bbox, uint64_t *
                                  uint32_t *tri_tri, uint64_t *n_nodes) {
   if (!scene) return fail(SKW_INVALID_ARG, "scene must not be NULL");
   if ((tri_prim == nullptr) != (tri_tri == nullptr)) return fail(SKW_INVALID_ARG, "tri_prim and tri_tri go together");
-  uint64_t k = export_tree(scene->host, bbox, meta, tri_prim, tri_tri);
+  const TreeLayout *t = nullptr;
+  skw_status st = scene_host(scene, &t);
+  if (st != SKW_OK) return st;
+  uint64_t k = export_tree(*t, bbox, meta, tri_prim, tri_tri);
   if (n_nodes) *n_nodes = k;
   return SKW_OK;
 }

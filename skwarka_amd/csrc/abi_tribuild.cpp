@@ -1,14 +1,17 @@
-// Triangle tree built on the device (skw_scene_create with SKW_SCENE_DEVICE_BUILD): the host
-// prepares the world-space triangles and their boxes exactly as the host builder does, the
-// kernels of tri_build.hip build the tree level by level (node boxes, split search, the in-place
-// three-way partition), and the host lays the nodes out depth first as the host builder's
-// place() does, so the scene is node for node the one the host builder makes
-// (TriangleAccelerator.cpp:22-147, 505-729).
+// Triangle tree built on the device (skw_scene_create with SKW_SCENE_DEVICE_BUILD), node for node
+// the tree of the host builder (TriangleAccelerator.cpp:22-147, 505-729):
+//   1. the host reads and cleans the meshes as the host builder does and uploads the scene's
+//      primitive and mesh tables (the same tables every scene uploads);
+//   2. k_tb_prep moves the triangles to world space in the builder's order with the builder's
+//      arithmetic and forms their float boxes;
+//   3. the kernels of tri_build.hip / tri_build_wide.hip build the tree level by level (node
+//      boxes, split search, the in-place three-way partition);
+//   4. the k_tl_* kernels lay the nodes out depth first as the host builder's place() does and
+//      gather the triangle array in its final order, straight into the scene's device arrays.
+// The host copy of the tree (for export and save) is downloaded on first use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
-#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -39,49 +42,7 @@ struct Buffers {
     if (e == hipSuccess) held.push_back((void *)*p);
     return e;
   }
-  template <class T>
-  void release(T *p) {
-    auto it = std::find(held.begin(), held.end(), (void *)p);
-    if (it != held.end()) {
-      pool_free(p, st);
-      held.erase(it);
-    }
-  }
 };
-
-// Lays the nodes out as the host builder's place() does: node by node depth first (children left,
-// middle, right), each internal node's children as one block appended when the node is placed.
-void place_all(const std::vector<TriBuildNode> &hn, TreeLayout &out) {
-  out.nodes.clear();
-  out.nodes.reserve(hn.size());
-  out.nodes.resize(1);
-  out.leaves = 0;
-  std::vector<std::pair<uint32_t, uint32_t>> st;  // (node id, layout index)
-  st.reserve(1024);
-  st.push_back({0u, 0u});
-  while (!st.empty()) {
-    const uint32_t id = st.back().first, idx = st.back().second;
-    st.pop_back();
-    const TriBuildNode &n = hn[id];
-    DevNode &d = out.nodes[idx];
-    for (int i = 0; i < 3; ++i) {
-      d.bmin[i] = n.bmin[i];
-      d.bmax[i] = n.bmax[i];
-    }
-    if (n.first_child == kNoChild) {
-      d.a = n.tb;
-      d.b = (n.te - n.tb) << 3;
-      out.leaves++;
-      continue;
-    }
-    const uint32_t flags = n.info & 3u, mask = (n.info >> 2) & 7u, nch = (uint32_t)__builtin_popcount(mask);
-    const uint32_t first = (uint32_t)out.nodes.size();
-    d.a = first;
-    d.b = flags | (mask << 2);
-    out.nodes.resize(out.nodes.size() + nch);
-    for (uint32_t k = nch; k-- > 0;) st.push_back({n.first_child + k, first + k});
-  }
-}
 
 struct BuildConfig {
   uint32_t small_max;  // nodes of at most this many triangles: one wave builds the subtree
@@ -95,43 +56,17 @@ uint32_t ceil_log2(uint32_t v) {
   return r;
 }
 
-// One build on the device with `cap` node records: the nodes (ids as allocated) and the final
-// triangle order; overflow when a subtree build ran out of records.
-skw_status build_levels(const PlainTriangles &pt, uint32_t n, const BuildConfig &cfg,
-                        const std::vector<uint32_t> &iota, uint32_t cap, std::vector<TriBuildNode> &hn,
-                        std::vector<uint32_t> &order, bool &overflow) {
-  Buffers B;
-  SKW_HIP(hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking), "hipStreamCreate");
-  TriWideArgs wa;
-  std::memset(&wa, 0, sizeof(wa));
+// One build over the boxes and order that k_tb_prep leaves in a.box / a.ref, with `cap` node
+// records (d_nodes_out, ids as allocated; count set); overflow when a subtree build ran out of
+// records.
+skw_status build_levels(Buffers &B, TriWideArgs &wa, uint32_t n, const BuildConfig &cfg, uint32_t cap,
+                        TriBuildNode **d_nodes_out, uint32_t &count, bool &overflow) {
   TriBuildArgs &a = wa.b;
-  SKW_HIP(B.alloc(&a.box, 6 * (size_t)n), "tree build allocation");
-  SKW_HIP(B.alloc(&a.box2, 6 * (size_t)n), "tree build allocation");
-  SKW_HIP(B.alloc(&a.ref, n), "tree build allocation");
-  SKW_HIP(B.alloc(&a.ref2, n), "tree build allocation");
-  uint32_t **scratch[9] = {&a.s_cls, &a.s_rf, &a.s_nb, &a.s_G, &a.s_Rpos, &a.s_order, &a.s_elem, &a.s_ptr, &a.s_dest};
-  for (uint32_t **s : scratch) SKW_HIP(B.alloc(s, n), "tree build allocation");
-  // counts: node count, next big, next small, overflow, next wide
-  uint32_t *d_counts = nullptr;
+  uint32_t *d_counts = nullptr;  // node count, next big, next small, overflow, next wide
   SKW_HIP(B.alloc(&d_counts, 8), "tree build allocation");
-  // wide-node state: at most n / wide_min nodes per level, their chunks
   const uint32_t wcap = n / cfg.wide_min + 2;
-  wa.chunk_cap = n / cfg.chunk + wcap + 2;
-  wa.chunk = cfg.chunk;
-  SKW_HIP(B.alloc(&wa.w, wcap), "tree build allocation");
-  unsigned char *d_bins = nullptr;
-  SKW_HIP(B.alloc(&d_bins, 3 * (size_t)wcap * tri_build_bins_bytes()), "tree build allocation");
-  wa.bins = d_bins;
-  SKW_HIP(B.alloc(&wa.chunk_w, wa.chunk_cap), "tree build allocation");
-  SKW_HIP(B.alloc(&wa.ccnt, 8 * (size_t)wa.chunk_cap), "tree build allocation");
-  SKW_HIP(B.alloc(&wa.cbase, 8 * (size_t)wa.chunk_cap), "tree build allocation");
-  SKW_HIP(B.alloc(&wa.info_out, 2), "tree build allocation");
-  SKW_HIP(hipMemcpyAsync(a.box, pt.box.data(), pt.box.size() * sizeof(float), hipMemcpyHostToDevice, B.st),
-          "upload boxes");
-  SKW_HIP(hipMemcpyAsync(a.ref, iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, B.st), "upload ids");
   TriBuildNode *d_nodes = nullptr;
-  // current / next lists: wide, big, small
-  uint32_t *cur[3] = {nullptr, nullptr, nullptr}, *nxt[3] = {nullptr, nullptr, nullptr};
+  uint32_t *cur[3] = {nullptr, nullptr, nullptr}, *nxt[3] = {nullptr, nullptr, nullptr};  // wide, big, small
   SKW_HIP(B.alloc(&d_nodes, cap), "tree build allocation");
   for (int l = 0; l < 3; ++l) {
     SKW_HIP(B.alloc(&cur[l], cap), "tree build allocation");
@@ -142,13 +77,13 @@ skw_status build_levels(const PlainTriangles &pt, uint32_t n, const BuildConfig 
   root.te = n;
   root.first_child = kNoChild;
   const int root_list = n > cfg.wide_min ? 0 : (n > cfg.small_max ? 1 : 2);
-  uint32_t counts[3] = {0, 0, 0};  // wide, big, small
+  uint32_t counts[3] = {0, 0, 0};
   counts[root_list] = 1;
   uint32_t h_counts[8] = {1, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t zero = 0;
   SKW_HIP(hipMemcpyAsync(d_nodes, &root, sizeof(root), hipMemcpyHostToDevice, B.st), "upload root");
   SKW_HIP(hipMemcpyAsync(cur[root_list], &zero, sizeof(zero), hipMemcpyHostToDevice, B.st), "upload root list");
-  uint32_t count = 1;
+  count = 1;
   overflow = false;
   a.nodes = d_nodes;
   a.node_count = d_counts;
@@ -164,7 +99,7 @@ skw_status build_levels(const PlainTriangles &pt, uint32_t n, const BuildConfig 
     if (need > cap || counts[0] > wcap) {
       // the level's children would not fit: a larger capacity is the caller's retry
       overflow = true;
-      break;
+      return SKW_OK;
     }
     h_counts[0] = count;
     for (int i = 1; i < 8; ++i) h_counts[i] = 0;
@@ -192,7 +127,7 @@ skw_status build_levels(const PlainTriangles &pt, uint32_t n, const BuildConfig 
     SKW_HIP(hipStreamSynchronize(B.st), "tree build level");
     if (h_counts[3] != 0 || h_counts[0] > cap) {
       overflow = true;
-      break;
+      return SKW_OK;
     }
     count = h_counts[0];
     counts[0] = h_counts[4];
@@ -200,57 +135,83 @@ skw_status build_levels(const PlainTriangles &pt, uint32_t n, const BuildConfig 
     counts[2] = h_counts[2];
     for (int l = 0; l < 3; ++l) std::swap(cur[l], nxt[l]);
   }
-  if (overflow) return SKW_OK;
-  hn.resize(count);
-  order.resize(n);
-  SKW_HIP(hipMemcpyAsync(hn.data(), d_nodes, (size_t)count * sizeof(TriBuildNode), hipMemcpyDeviceToHost, B.st),
-          "download nodes");
-  SKW_HIP(hipMemcpyAsync(order.data(), a.ref, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, B.st),
-          "download triangle order");
-  SKW_HIP(hipStreamSynchronize(B.st), "download tree");
+  *d_nodes_out = d_nodes;
   return SKW_OK;
 }
 
-// The host layout of a device-built tree: depth-first placement, depths, bounds, triangles.
-skw_status layout(const PlainTriangles &pt, uint32_t n, const std::vector<TriBuildNode> &hn,
-                  const std::vector<uint32_t> &order, TreeLayout &out) {
-  // children are allocated after their parent: ids grow along every path, so the placement ends
-  for (size_t i = 0; i < hn.size(); ++i) {
-    const TriBuildNode &nd = hn[i];
-    if (nd.first_child == kNoChild) continue;
-    const uint64_t nch = (uint64_t)__builtin_popcount((nd.info >> 2) & 7u);
-    if (nd.first_child <= i || nch == 0 || nd.first_child + nch > hn.size())
-      return fail(SKW_HIP_ERROR, "device tree build: child index out of range");
+// The depth-first layout of the built records into the scene's node and triangle arrays.
+skw_status layout_on_device(Buffers &B, const TriBuildNode *d_hn, uint32_t count, uint32_t n, const float *d_tv,
+                            const uint32_t *d_tprim, const uint32_t *d_ttri, const uint32_t *d_ref, skw_scene *s) {
+  TriLayoutArgs L;
+  std::memset(&L, 0, sizeof(L));
+  L.hn = d_hn;
+  L.count = count;
+  L.n = n;
+  L.tv = d_tv;
+  L.tprim = d_tprim;
+  L.ttri = d_ttri;
+  L.ref = d_ref;
+  // hist, fill (kTriLayoutDepths each), flags, leaves: one block, read back at once
+  uint32_t *d_small = nullptr, *d_off = nullptr;
+  SKW_HIP(B.alloc(&d_small, 2 * kTriLayoutDepths + 2), "layout allocation");
+  SKW_HIP(B.alloc(&d_off, kTriLayoutDepths + 1), "layout allocation");
+  L.hist = d_small;
+  L.fill = d_small + kTriLayoutDepths;
+  L.flags = d_small + 2 * kTriLayoutDepths;
+  L.leaves = d_small + 2 * kTriLayoutDepths + 1;
+  L.level_off = d_off;
+  SKW_HIP(B.alloc(&L.by_depth, count), "layout allocation");
+  SKW_HIP(B.alloc(&L.size, count), "layout allocation");
+  SKW_HIP(B.alloc(&L.cval, count), "layout allocation");
+  SKW_HIP(B.alloc(&L.idx, count), "layout allocation");
+  SKW_HIP(hipMalloc((void **)&s->d_nodes, (size_t)count * sizeof(DevNode)), "scene nodes");
+  SKW_HIP(hipMalloc((void **)&s->d_tris, (size_t)n * sizeof(DevTri)), "scene triangles");
+  L.nodes = s->d_nodes;
+  L.tris = s->d_tris;
+  SKW_HIP(hipMemsetAsync(d_small, 0, (2 * kTriLayoutDepths + 2) * sizeof(uint32_t), B.st), "layout clear");
+  SKW_HIP(launch_tri_layout_hist(L, B.st), "layout histogram");
+  std::vector<uint32_t> small(2 * kTriLayoutDepths + 2);
+  SKW_HIP(hipMemcpyAsync(small.data(), d_small, small.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, B.st),
+          "layout histogram");
+  SKW_HIP(hipStreamSynchronize(B.st), "layout histogram");
+  if (small[2 * kTriLayoutDepths] != 0) return fail(SKW_HIP_ERROR, "device tree build: a node record is out of range");
+  std::vector<uint32_t> off(kTriLayoutDepths + 1, 0);
+  uint32_t max_depth = 0;
+  for (uint32_t d = 0; d < kTriLayoutDepths; ++d) {
+    off[d + 1] = off[d] + small[d];
+    if (small[d]) max_depth = d;
   }
-  place_all(hn, out);
-  out.root = 0;
-  uint32_t depth = 0;
-  for (const TriBuildNode &nd : hn) depth = std::max(depth, nd.info >> 16);
-  out.max_depth = depth;
-  out.stack_need = depth + 1;
+  if (off[kTriLayoutDepths] != count || small[0] != 1)
+    return fail(SKW_HIP_ERROR, "device tree build: the records do not form one tree");
+  SKW_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, B.st),
+          "layout offsets");
+  SKW_HIP(launch_tri_layout_scatter(L, B.st), "layout scatter");
+  for (uint32_t d = max_depth + 1; d-- > 0;) {
+    L.level_begin = off[d];
+    L.level_end = off[d + 1];
+    SKW_HIP(launch_tri_layout_size(L, B.st), "layout sizes");
+  }
+  for (uint32_t d = 0; d <= max_depth; ++d) {
+    L.level_begin = off[d];
+    L.level_end = off[d + 1];
+    SKW_HIP(launch_tri_layout_place(L, B.st), "layout placement");
+  }
+  SKW_HIP(launch_tri_layout_tris(L, B.st), "layout triangles");
+  TriBuildNode root;
+  SKW_HIP(hipMemcpyAsync(&root, d_hn, sizeof(root), hipMemcpyDeviceToHost, B.st), "root record");
+  SKW_HIP(hipStreamSynchronize(B.st), "layout");
+  TreeLayout &t = s->host;
+  t.root = 0;
+  t.max_depth = max_depth;
+  t.stack_need = max_depth + 1;
+  t.leaves = small[2 * kTriLayoutDepths + 1];
   for (int i = 0; i < 3; ++i) {
-    out.bounds[i] = hn[0].bmin[i];
-    out.bounds[3 + i] = hn[0].bmax[i];
+    t.bounds[i] = root.bmin[i];
+    t.bounds[3 + i] = root.bmax[i];
   }
-  out.tris.resize(n);
-  std::atomic<bool> bad{false};
-  parallel_ranges(n, [&](uint64_t lo, uint64_t hi) {
-    for (uint64_t pos = lo; pos < hi; ++pos) {
-      const uint32_t id = order[pos];
-      if (id >= n) {
-        bad = true;
-        return;
-      }
-      DevTri &t = out.tris[pos];
-      std::memcpy(t.v0, &pt.tv[9 * (size_t)id], 3 * sizeof(float));
-      std::memcpy(t.v1, &pt.tv[9 * (size_t)id + 3], 3 * sizeof(float));
-      std::memcpy(t.v2, &pt.tv[9 * (size_t)id + 6], 3 * sizeof(float));
-      t.prim = pt.tprim[id];
-      t.tri = pt.ttri[id];
-      t.pad = 0;
-    }
-  });
-  if (bad) return fail(SKW_HIP_ERROR, "device tree build: triangle order out of range");
+  s->n_nodes = count;
+  s->n_tris = n;
+  s->host_tree = false;
   return SKW_OK;
 }
 
@@ -258,47 +219,122 @@ skw_status layout(const PlainTriangles &pt, uint32_t n, const std::vector<TriBui
 
 namespace skw {
 
-skw_status build_triangle_tree_device(const skw_scene_desc &desc, TreeLayout &out) {
+skw_status build_triangle_tree_device(const skw_scene_desc &desc, skw_scene *s, bool &built) {
+  built = false;
   std::string err;
-  PlainTriangles pt;
+  std::vector<uint64_t> goff;
+  std::vector<uint32_t> user;
   bool instanced = false;
-  if (!prepare_plain_triangles(desc, out, pt, instanced, err)) return fail(SKW_INVALID_ARG, err);
+  if (!read_plain_scene(desc, s->host, goff, user, instanced, err)) return fail(SKW_INVALID_ARG, err);
   if (instanced)
     return fail(SKW_UNSUPPORTED, "the device tree build takes scenes without instanced meshes (a mesh used by "
                                  "several primitives); build such scenes on the host");
-  const uint64_t n64 = pt.tprim.size();
-  if (n64 == 0) {
-    // the empty tree: one leaf whose box stays (+inf, -inf)
-    if (!build_triangle_tree(desc, out, err)) return fail(SKW_INVALID_ARG, err);
-    return SKW_OK;
-  }
-  if (!pt.finite_boxes) return fail(SKW_UNSUPPORTED, "the device tree build needs finite vertices");
+  const uint64_t n64 = goff.back();
+  if (n64 == 0) return SKW_OK;  // the empty tree: the caller builds it on the host
   const uint32_t n = (uint32_t)n64;
   BuildConfig cfg;
   cfg.small_max = (uint32_t)std::min(64, std::max(1, env_int("SKW_TRIBUILD_SMALL_MAX", 64)));
   cfg.wide_min = (uint32_t)std::max<int>((int)std::max<uint32_t>(cfg.small_max, 256),
                                          env_int("SKW_TRIBUILD_WIDE_MIN", 16384));
   cfg.chunk = 256u * (uint32_t)std::min(64, std::max(1, env_int("SKW_TRIBUILD_CHUNK", 4096) / 256));
-  std::vector<uint32_t> iota(n);
-  parallel_ranges(n, [&](uint64_t lo, uint64_t hi) {
-    for (uint64_t i = lo; i < hi; ++i) iota[i] = (uint32_t)i;
-  });
+  skw_status st = scene_tables(s, false);
+  if (st != SKW_OK) return st;
+
+  Buffers B;
+  SKW_HIP(hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking), "hipStreamCreate");
+  TriPrepArgs p;
+  std::memset(&p, 0, sizeof(p));
+  p.meshes = s->d_meshes;
+  p.verts = s->d_mesh_verts;
+  p.tris = s->d_mesh_tris;
+  p.prims = s->d_prims;
+  p.n_meshes = (uint32_t)s->host.meshes.size();
+  p.n = n;
+  uint64_t *d_goff = nullptr;
+  uint32_t *d_user = nullptr;
+  SKW_HIP(B.alloc(&d_goff, goff.size()), "tree build allocation");
+  SKW_HIP(B.alloc(&d_user, user.size()), "tree build allocation");
+  SKW_HIP(hipMemcpyAsync(d_goff, goff.data(), goff.size() * sizeof(uint64_t), hipMemcpyHostToDevice, B.st),
+          "upload mesh offsets");
+  SKW_HIP(hipMemcpyAsync(d_user, user.data(), user.size() * sizeof(uint32_t), hipMemcpyHostToDevice, B.st),
+          "upload mesh primitives");
+  p.goff = d_goff;
+  p.user = d_user;
+  SKW_HIP(B.alloc(&p.tv, 9 * (size_t)n), "tree build allocation");
+  SKW_HIP(B.alloc(&p.box, 6 * (size_t)n), "tree build allocation");
+  SKW_HIP(B.alloc(&p.tprim, n), "tree build allocation");
+  SKW_HIP(B.alloc(&p.ttri, n), "tree build allocation");
+  SKW_HIP(B.alloc(&p.ref, n), "tree build allocation");
+  SKW_HIP(B.alloc(&p.flags, 4), "tree build allocation");
+
+  TriWideArgs wa;
+  std::memset(&wa, 0, sizeof(wa));
+  TriBuildArgs &a = wa.b;
+  a.box = p.box;
+  a.ref = p.ref;
+  SKW_HIP(B.alloc(&a.box2, 6 * (size_t)n), "tree build allocation");
+  SKW_HIP(B.alloc(&a.ref2, n), "tree build allocation");
+  uint32_t **scratch[9] = {&a.s_cls, &a.s_rf, &a.s_nb, &a.s_G, &a.s_Rpos, &a.s_order, &a.s_elem, &a.s_ptr, &a.s_dest};
+  for (uint32_t **sc : scratch) SKW_HIP(B.alloc(sc, n), "tree build allocation");
+  const uint32_t wcap = n / cfg.wide_min + 2;
+  wa.chunk_cap = n / cfg.chunk + wcap + 2;
+  wa.chunk = cfg.chunk;
+  SKW_HIP(B.alloc(&wa.w, wcap), "tree build allocation");
+  unsigned char *d_bins = nullptr;
+  SKW_HIP(B.alloc(&d_bins, 3 * (size_t)wcap * tri_build_bins_bytes()), "tree build allocation");
+  wa.bins = d_bins;
+  SKW_HIP(B.alloc(&wa.chunk_w, wa.chunk_cap), "tree build allocation");
+  SKW_HIP(B.alloc(&wa.ccnt, 8 * (size_t)wa.chunk_cap), "tree build allocation");
+  SKW_HIP(B.alloc(&wa.cbase, 8 * (size_t)wa.chunk_cap), "tree build allocation");
+  SKW_HIP(B.alloc(&wa.info_out, 2), "tree build allocation");
+
   // node records: the level kernel's children are counted before each level; subtree builds check
-  // the capacity themselves and the build restarts with more records when they run out
+  // the capacity themselves and the build restarts (from the prepared triangles) with more records
+  // when they run out
   uint32_t cap = std::max<uint32_t>(1024, n + n / 2 + 64);
   for (int attempt = 0;; ++attempt) {
-    std::vector<TriBuildNode> hn;
-    std::vector<uint32_t> order;
+    uint32_t flags = 0;
+    SKW_HIP(hipMemsetAsync(p.flags, 0, 4 * sizeof(uint32_t), B.st), "tree build clear");
+    SKW_HIP(launch_tri_prep(p, B.st), "world-space triangles");
+    SKW_HIP(hipMemcpyAsync(&flags, p.flags, sizeof(flags), hipMemcpyDeviceToHost, B.st), "read flags");
+    SKW_HIP(hipStreamSynchronize(B.st), "world-space triangles");
+    if (flags & 1u) return fail(SKW_UNSUPPORTED, "the device tree build needs finite vertices");
+    TriBuildNode *d_hn = nullptr;
+    uint32_t count = 0;
     bool overflow = false;
-    skw_status st = build_levels(pt, n, cfg, iota, cap, hn, order, overflow);
+    st = build_levels(B, wa, n, cfg, cap, &d_hn, count, overflow);
     if (st != SKW_OK) return st;
     if (overflow) {
       if (attempt >= 3 || cap >= 0x40000000u) return fail(SKW_OUT_OF_MEMORY, "device tree build: too many nodes");
       cap *= 4;
       continue;
     }
-    return layout(pt, n, hn, order, out);
+    st = layout_on_device(B, d_hn, count, n, p.tv, p.tprim, p.ttri, a.ref, s);
+    if (st != SKW_OK) return st;
+    built = true;
+    return SKW_OK;
   }
+}
+
+skw_status scene_host(const skw_scene *scene, const TreeLayout **out) {
+  skw_scene *s = const_cast<skw_scene *>(scene);
+  std::lock_guard<std::mutex> lock(s->host_mu);
+  if (!s->host_tree) {
+    int cur = 0;
+    SKW_HIP(hipGetDevice(&cur), "hipGetDevice");
+    SKW_HIP(hipSetDevice(s->device), "hipSetDevice");
+    std::vector<DevNode> nodes(s->n_nodes);
+    std::vector<DevTri> tris(s->n_tris);
+    hipError_t e = hipMemcpy(nodes.data(), s->d_nodes, nodes.size() * sizeof(DevNode), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(tris.data(), s->d_tris, tris.size() * sizeof(DevTri), hipMemcpyDeviceToHost);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "download tree");
+    s->host.nodes.swap(nodes);
+    s->host.tris.swap(tris);
+    s->host_tree = true;
+  }
+  *out = &s->host;
+  return SKW_OK;
 }
 
 }  // namespace skw

@@ -20,7 +20,6 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
-#include <functional>
 #include <future>
 #include <limits>
 #include <memory>
@@ -554,24 +553,14 @@ bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::strin
   return true;
 }
 
-void parallel_ranges(uint64_t n, const std::function<void(uint64_t, uint64_t)> &f) {
-  const uint64_t min_per = 1 << 15;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const uint64_t parts = std::max<uint64_t>(1, std::min<uint64_t>(hw, n / min_per));
-  if (parts <= 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (uint64_t t = 0; t < parts; ++t) th.emplace_back(f, n * t / parts, n * (t + 1) / parts);
-  for (std::thread &x : th) x.join();
-}
-
-bool prepare_plain_triangles(const skw_scene_desc &desc, TreeLayout &out, PlainTriangles &pt, bool &instanced,
-                             std::string &err) {
+bool read_plain_scene(const skw_scene_desc &desc, TreeLayout &out, std::vector<uint64_t> &goff,
+                      std::vector<uint32_t> &user, bool &instanced, std::string &err) {
   instanced = false;
+  goff.clear();
+  user.clear();
   if (!read_scene(desc, out, err)) return false;
-  std::vector<uint32_t> users(desc.n_meshes, 0), user(desc.n_meshes, 0);
+  std::vector<uint32_t> users(desc.n_meshes, 0);
+  user.assign(desc.n_meshes, 0);
   for (uint32_t p = 0; p < desc.n_prims; ++p) {
     users[desc.prim_mesh[p]]++;
     user[desc.prim_mesh[p]] = p;
@@ -579,49 +568,17 @@ bool prepare_plain_triangles(const skw_scene_desc &desc, TreeLayout &out, PlainT
   for (uint32_t m = 0; m < desc.n_meshes; ++m)
     if (users[m] > 1) {
       instanced = true;
+      user.clear();
       return true;
     }
-  // the builder's order: meshes with one primitive, in mesh order, their triangles in order,
-  // moved to world space as TriangleAccelerator.cpp:104-129 does
-  std::vector<uint64_t> off(desc.n_meshes + 1, 0);
+  // the builder's order: meshes with one primitive, in mesh order, their triangles in order
+  goff.assign(desc.n_meshes + 1, 0);
   for (uint32_t m = 0; m < desc.n_meshes; ++m)
-    off[m + 1] = off[m] + (users[m] == 1 ? out.meshes[m].tris.size() / 3 : 0);
-  const uint64_t n = off.back();
-  if (n >= (1ull << 29)) {
+    goff[m + 1] = goff[m] + (users[m] == 1 ? out.meshes[m].tris.size() / 3 : 0);
+  if (goff.back() >= (1ull << 29)) {
     err = "too many triangles for the 29-bit leaf count";
     return false;
   }
-  pt = PlainTriangles();
-  pt.tv.resize(9 * n);
-  pt.box.resize(6 * n);
-  pt.tprim.resize(n);
-  pt.ttri.resize(n);
-  std::atomic<bool> finite{true};
-  parallel_ranges(n, [&](uint64_t lo, uint64_t hi) {
-    if (lo >= hi) return;
-    uint32_t m = (uint32_t)(std::upper_bound(off.begin(), off.end(), lo) - off.begin()) - 1;
-    bool fin = true;
-    for (uint64_t i = lo; i < hi; ++i) {
-      while (i >= off[m + 1]) ++m;
-      const HostMesh &mesh = out.meshes[m];
-      const HostPrim &hp = out.prims[user[m]];
-      const size_t j = (size_t)(i - off[m]);
-      BoxF bb;
-      bb.clear();
-      for (int k = 0; k < 3; ++k) {
-        float *w = &pt.tv[9 * i + 3 * k];
-        xform_point_f(hp.m2w, &mesh.verts[3 * mesh.tris[3 * j + k]], w);
-        bb.add_point(w);
-      }
-      std::memcpy(&pt.box[6 * i], bb.mn, 3 * sizeof(float));
-      std::memcpy(&pt.box[6 * i + 3], bb.mx, 3 * sizeof(float));
-      for (int c = 0; c < 3; ++c) fin = fin && std::isfinite(bb.mn[c]) && std::isfinite(bb.mx[c]);
-      pt.tprim[i] = user[m];
-      pt.ttri[i] = (uint32_t)j;
-    }
-    if (!fin) finite = false;
-  });
-  pt.finite_boxes = finite.load();
   return true;
 }
 

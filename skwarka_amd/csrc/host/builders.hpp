@@ -3,7 +3,6 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
-#include <functional>
 #include <string>
 #include <vector>
 
@@ -43,23 +42,12 @@ struct TreeLayout {
 // Reads and cleans the description, builds the tree. Returns false with a message on bad input.
 bool build_triangle_tree(const skw_scene_desc &desc, TreeLayout &out, std::string &err);
 
-// World-space triangles of a scene without instanced meshes, in the order the tree builder
-// partitions them: vertices (9 floats), float bounding boxes (min xyz, max xyz), primitive and
-// in-mesh triangle ids.
-struct PlainTriangles {
-  std::vector<float> tv;
-  std::vector<float> box;
-  std::vector<uint32_t> tprim, ttri;
-  bool finite_boxes = true;  // every box coordinate is finite
-};
-
-// f(begin, end) over [0, n) split into contiguous ranges on the host's threads.
-void parallel_ranges(uint64_t n, const std::function<void(uint64_t, uint64_t)> &f);
-
-// The tree builder's input preparation without the build: meshes, primitives and the triangles.
-// Sets instanced (and prepares nothing else) when a mesh is used by several primitives.
-bool prepare_plain_triangles(const skw_scene_desc &desc, TreeLayout &out, PlainTriangles &pt, bool &instanced,
-                             std::string &err);
+// The tree builder's input without the build, for scenes without instanced meshes: meshes and
+// primitives read and cleaned; goff: per mesh the first triangle of the builder's order (meshes
+// used by one primitive, in mesh order; n_meshes + 1 entries); user: the primitive using each
+// mesh. Sets instanced (offsets left empty) when a mesh is used by several primitives.
+bool read_plain_scene(const skw_scene_desc &desc, TreeLayout &out, std::vector<uint64_t> &goff,
+                      std::vector<uint32_t> &user, bool &instanced, std::string &err);
 
 // Depth-first pre-order export (children left, middle, right) in the form of skw_scene_export_tree.
 uint64_t export_tree(const TreeLayout &t, float *bbox, uint64_t *meta, uint32_t *tri_prim, uint32_t *tri_tri);

@@ -54,7 +54,11 @@ struct CounterPool {
 }  // namespace skw
 
 struct skw_scene {
-  skw::TreeLayout host;
+  skw::TreeLayout host;     // nodes / tris pending (host_tree false) for device-built trees
+  bool host_tree = true;    // host.nodes / host.tris hold the tree (device builds: on first use)
+  uint64_t n_nodes = 0, n_tris = 0;  // counts while the host copy is pending
+  std::mutex host_mu;
+  bool tables_done = false; // primitive / mesh tables computed (and uploaded unless host-only)
   bool on_device = false;
   bool has_inst = false;
   bool needs_fix = false;  // some primitive's traversal barycentrics are not mesh-space values
@@ -128,9 +132,15 @@ skw_status enqueue_trace(const skw_scene *s, const skw_ray *d_rays, uint64_t n, 
 GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
                        uint32_t k, double r, uint64_t paths, double rho, int filter);
 
-// The host layout of a scene without instanced meshes, its tree built on the current device
-// (node for node the host builder's tree); SKW_UNSUPPORTED for instanced or non-finite scenes.
-skw_status build_triangle_tree_device(const skw_scene_desc &desc, TreeLayout &out);
+// A scene without instanced meshes with its tree built on the current device, node for node the
+// host builder's tree, straight into the scene's device arrays (built set; an empty scene is left
+// to the host builder); SKW_UNSUPPORTED for instanced scenes or non-finite triangle boxes.
+skw_status build_triangle_tree_device(const skw_scene_desc &desc, skw_scene *s, bool &built);
+// The scene's host tree (downloaded on first use for device-built trees).
+skw_status scene_host(const skw_scene *s, const TreeLayout **out);
+// The primitive and mesh tables of a scene whose meshes and primitives are read (uploaded unless
+// host_only); once per scene.
+skw_status scene_tables(skw_scene *s, bool host_only);
 
 // The per-node normal boxes of a map whose device arrays are complete (synchronises `st`).
 skw_status photon_map_nbox(skw_photon_map *m, hipStream_t st);

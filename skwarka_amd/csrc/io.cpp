@@ -256,7 +256,10 @@ extern "C" {
 
 skw_status skw_scene_save(const skw_scene *scene, const char *path) {
   if (!scene || !path) return fail(SKW_INVALID_ARG, "scene and path must not be NULL");
-  const TreeLayout &t = scene->host;
+  const TreeLayout *tp = nullptr;
+  skw_status hs = scene_host(scene, &tp);
+  if (hs != SKW_OK) return hs;
+  const TreeLayout &t = *tp;
   Writer w(kSceneMagic);
   w.pod(t.root);
   w.pod(t.max_depth);

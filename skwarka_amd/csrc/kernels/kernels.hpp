@@ -408,6 +408,54 @@ struct TriBuildArgs {
 hipError_t launch_tri_build_level(const TriBuildArgs &a, hipStream_t stream);
 hipError_t launch_tri_build_small(const TriBuildArgs &a, hipStream_t stream);
 
+// World-space triangles of a scene without instancing, in the tree builder's order (meshes used by
+// one primitive, in mesh order; their cleaned triangles in order), moved by the primitive's
+// mesh-to-world matrix as the host builder does (Transform::operator() on float points), with
+// their float boxes and the identity order.
+struct TriPrepArgs {
+  const DevMesh *meshes;
+  const float *verts;
+  const uint32_t *tris;
+  const DevPrim *prims;
+  const uint64_t *goff;  // n_meshes + 1: first triangle of each mesh in the order (0 triangles if unused)
+  const uint32_t *user;  // per mesh: its primitive
+  uint32_t n_meshes;
+  uint64_t n;
+  float *tv;             // 9 per triangle
+  float *box;            // 6 per triangle
+  uint32_t *tprim, *ttri, *ref;
+  uint32_t *flags;       // bit 0: a box with a non-finite coordinate
+};
+hipError_t launch_tri_prep(const TriPrepArgs &a, hipStream_t stream);
+
+// The host builder's depth-first layout of a device-built tree (children blocks appended as nodes
+// are placed), computed level by level from subtree sizes, and the triangle array in final order.
+struct TriLayoutArgs {
+  const TriBuildNode *hn;
+  uint32_t count;        // node records
+  uint32_t n;            // triangles
+  uint32_t *hist;        // per depth (kTriLayoutDepths): nodes
+  uint32_t *fill;        // per depth: scatter cursor
+  const uint32_t *level_off;  // per depth + 1: first entry of the depth in by_depth
+  uint32_t *by_depth;    // node ids grouped by depth
+  uint32_t *size;        // subtree node counts
+  uint32_t *cval;        // children of the nodes before it in depth-first order
+  uint32_t *idx;         // layout index
+  uint32_t *flags;       // bit 1: a record out of range
+  uint32_t *leaves;
+  const float *tv;
+  const uint32_t *tprim, *ttri, *ref;
+  DevNode *nodes;
+  DevTri *tris;
+  uint32_t level_begin, level_end;  // slice of by_depth of one level launch
+};
+constexpr uint32_t kTriLayoutDepths = 256;
+hipError_t launch_tri_layout_hist(const TriLayoutArgs &a, hipStream_t stream);     // hist, leaves, flags
+hipError_t launch_tri_layout_scatter(const TriLayoutArgs &a, hipStream_t stream);  // by_depth, root's idx / cval
+hipError_t launch_tri_layout_size(const TriLayoutArgs &a, hipStream_t stream);     // one level, bottom-up
+hipError_t launch_tri_layout_place(const TriLayoutArgs &a, hipStream_t stream);    // one level, top-down
+hipError_t launch_tri_layout_tris(const TriLayoutArgs &a, hipStream_t stream);
+
 // The largest nodes of a level, each split over chunks of `chunk` triangles (one workgroup each).
 struct TriWide {
   uint32_t id, tb, n, info;   // node id, range start, triangles, info bits of the node record

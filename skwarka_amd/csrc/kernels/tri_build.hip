@@ -477,7 +477,200 @@ __global__ __launch_bounds__(kBlock) void k_tb_level(TriBuildArgs a) {
   if (tid == 0) emit_children(a, id, axis, tb, tb + n_left, te - n_right, te, smask, depth);
 }
 
+// ------------------------------------------------------------------ world-space triangles
+
+// Transform::operator()(Point3D<float>) (Source/RayLibs/Math/Transform.h:223-238) as the host
+// builder computes it: double products of the float point, float results, the w division in
+// double only when w != 1.
+__device__ __forceinline__ void xform_point_f(const double m[16], const float *p, float *out) {
+  const double x = p[0], y = p[1], z = p[2];
+  const float xp = (float)(m[0] * x + m[1] * y + m[2] * z + m[3]);
+  const float yp = (float)(m[4] * x + m[5] * y + m[6] * z + m[7]);
+  const float zp = (float)(m[8] * x + m[9] * y + m[10] * z + m[11]);
+  const float wp = (float)(m[12] * x + m[13] * y + m[14] * z + m[15]);
+  if (wp == 1.0f) {
+    out[0] = xp;
+    out[1] = yp;
+    out[2] = zp;
+    return;
+  }
+  const double inv = 1.0 / (double)wp;
+  out[0] = (float)(xp * inv);
+  out[1] = (float)(yp * inv);
+  out[2] = (float)(zp * inv);
+}
+
+__global__ __launch_bounds__(256) void k_tb_prep(TriPrepArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  // the mesh holding triangle i: the last mesh whose first triangle is <= i
+  uint32_t lo = 0, hi = a.n_meshes;  // goff[lo] <= i < goff[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (a.goff[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const uint32_t m = lo;
+  const uint64_t j = i - a.goff[m];
+  const DevMesh mesh = a.meshes[m];
+  const double *m2w = a.prims[a.user[m]].m2w;
+  float mn[3] = {__int_as_float(0x7F800000), __int_as_float(0x7F800000), __int_as_float(0x7F800000)};
+  float mx[3] = {-mn[0], -mn[1], -mn[2]};
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t vi = a.tris[3 * (mesh.tri_off + j) + k];
+    float w[3];
+    xform_point_f(m2w, a.verts + 3 * (mesh.vert_off + vi), w);
+    for (int c = 0; c < 3; ++c) {
+      a.tv[9 * i + 3 * k + c] = w[c];
+      mn[c] = fmin_std(mn[c], w[c]);
+      mx[c] = fmax_std(mx[c], w[c]);
+    }
+  }
+  bool fin = true;
+  for (int c = 0; c < 3; ++c) {
+    a.box[6 * i + c] = mn[c];
+    a.box[6 * i + 3 + c] = mx[c];
+    fin = fin && isfinite(mn[c]) && isfinite(mx[c]);
+  }
+  a.tprim[i] = a.user[m];
+  a.ttri[i] = (uint32_t)j;
+  a.ref[i] = (uint32_t)i;
+  if (!fin) atomicOr(a.flags, 1u);
+}
+
+// ------------------------------------------------------------------ depth-first layout
+
+__global__ __launch_bounds__(256) void k_tl_hist(TriLayoutArgs a) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.count) return;
+  const TriBuildNode nd = a.hn[v];
+  const uint32_t depth = info_depth(nd.info);
+  bool bad = depth >= kTriLayoutDepths;
+  if (nd.first_child == kNone) {
+    atomicAdd(a.leaves, 1u);
+    bad = bad || nd.te < nd.tb || nd.te > a.n;
+  } else {
+    // children are allocated after their parent, as one block inside the records
+    const uint32_t nch = __builtin_popcount((nd.info >> 2) & 7u);
+    bad = bad || nch == 0 || nd.first_child <= v || (uint64_t)nd.first_child + nch > a.count;
+  }
+  if (bad) {
+    atomicOr(a.flags, 2u);
+    return;
+  }
+  atomicAdd(&a.hist[depth], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_tl_scatter(TriLayoutArgs a) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.count) return;
+  const uint32_t depth = info_depth(a.hn[v].info);
+  a.by_depth[a.level_off[depth] + atomicAdd(&a.fill[depth], 1u)] = v;
+  if (v == 0) {
+    a.idx[0] = 0;
+    a.cval[0] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tl_size(TriLayoutArgs a) {
+  const uint32_t t = a.level_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.level_end) return;
+  const uint32_t v = a.by_depth[t];
+  const TriBuildNode nd = a.hn[v];
+  uint32_t sz = 1;
+  if (nd.first_child != kNone) {
+    const uint32_t nch = __builtin_popcount((nd.info >> 2) & 7u);
+    for (uint32_t k = 0; k < nch; ++k) sz += a.size[nd.first_child + k];
+  }
+  a.size[v] = sz;
+}
+
+// Node v's children block starts at 1 + cval(v), cval(v) = children of the nodes placed before v;
+// the first child follows v (cval + v's children), each next child follows the previous child's
+// subtree (cval + its size - 1: every node of a subtree but its top is a child inside it).
+__global__ __launch_bounds__(256) void k_tl_place(TriLayoutArgs a) {
+  const uint32_t t = a.level_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.level_end) return;
+  const uint32_t v = a.by_depth[t];
+  const TriBuildNode nd = a.hn[v];
+  DevNode out;
+  for (int c = 0; c < 3; ++c) {
+    out.bmin[c] = nd.bmin[c];
+    out.bmax[c] = nd.bmax[c];
+  }
+  if (nd.first_child == kNone) {
+    out.a = nd.tb;
+    out.b = (nd.te - nd.tb) << 3;
+  } else {
+    const uint32_t mask = (nd.info >> 2) & 7u, nch = __builtin_popcount(mask);
+    const uint32_t bs = 1u + a.cval[v];
+    out.a = bs;
+    out.b = (nd.info & 3u) | (mask << 2);
+    uint32_t cv = a.cval[v] + nch;
+    for (uint32_t k = 0; k < nch; ++k) {
+      const uint32_t c = nd.first_child + k;
+      a.idx[c] = bs + k;
+      a.cval[c] = cv;
+      cv += a.size[c] - 1u;
+    }
+  }
+  a.nodes[a.idx[v]] = out;
+}
+
+__global__ __launch_bounds__(256) void k_tl_tris(TriLayoutArgs a) {
+  const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= a.n) return;
+  const uint32_t id = a.ref[pos];
+  DevTri t;
+  const float *v = a.tv + 9 * (size_t)id;
+  for (int c = 0; c < 3; ++c) {
+    t.v0[c] = v[c];
+    t.v1[c] = v[3 + c];
+    t.v2[c] = v[6 + c];
+  }
+  t.prim = a.tprim[id];
+  t.tri = a.ttri[id];
+  t.pad = 0;
+  a.tris[pos] = t;
+}
+
 }  // namespace
+
+hipError_t launch_tri_prep(const TriPrepArgs &a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tb_prep, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_layout_hist(const TriLayoutArgs &a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tl_hist, dim3((a.count + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_layout_scatter(const TriLayoutArgs &a, hipStream_t stream) {
+  hipLaunchKernelGGL(k_tl_scatter, dim3((a.count + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_layout_size(const TriLayoutArgs &a, hipStream_t stream) {
+  if (a.level_end <= a.level_begin) return hipSuccess;
+  hipLaunchKernelGGL(k_tl_size, dim3((a.level_end - a.level_begin + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_layout_place(const TriLayoutArgs &a, hipStream_t stream) {
+  if (a.level_end <= a.level_begin) return hipSuccess;
+  hipLaunchKernelGGL(k_tl_place, dim3((a.level_end - a.level_begin + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_layout_tris(const TriLayoutArgs &a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tl_tris, dim3((a.n + 255) / 256), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_tri_build_level(const TriBuildArgs &a, hipStream_t stream) {
   if (a.list_n == 0) return hipSuccess;

@@ -211,3 +211,15 @@ def test_batched_scenes():
     pick = sorted(set(list(range(0, len(specs), 8)) + [int(np.argmax(sizes))]))
     for i in pick:
         _vs_host(specs[i][0])
+
+
+def test_device_built_scene_saves_and_loads_as_the_host_built(tmp_path):
+    # the host copy of a device-built tree is made on first use (save, export)
+    meshes = scenes.small_spheres(seed=8, n=4, subdiv=3)
+    dev = sk.Scene(meshes, device_build=True)
+    path = str(tmp_path / "dev.skws")
+    dev.save(path)
+    loaded = sk.Scene.load(path)
+    host = sk.Scene(meshes, host_only=True)
+    _same_tree((host.export_tree(), host.info()), loaded)
+    _same_tree((host.export_tree(), host.info()), dev)
