@@ -139,6 +139,7 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.refill_min = (uint32_t)clamp_int(env_int("SKW_TRACE_REFILL_MIN", kTraceRefillMin), 1, 64);
   a.leaf_min = (uint32_t)clamp_int(env_int("SKW_TRACE_LEAF_MIN", kTraceLeafMin), 1, 64);
   a.claim = (uint32_t)clamp_int(env_int("SKW_TRACE_CLAIM", kTraceClaim), 64, 4096);
+  a.nested = s->host.nested ? 1u : 0u;
   const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
   uint32_t need = s->host.stack_need;
   if (fixed) {

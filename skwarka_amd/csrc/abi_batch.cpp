@@ -171,6 +171,7 @@ skw_status skw_batch_create_with_gathers(const skw_scene *const *scenes, const s
     t.seed = v.seed;
     t.n_rays = (uint64_t)v.width * v.height * v.spp;
     t.hit_offset = off;
+    t.nested = s->host.nested ? 1u : 0u;
     off += t.n_rays;
     b->table.push_back(t);
     b->scenes.push_back(s);

@@ -35,6 +35,10 @@ struct TreeLayout {
   uint32_t stack_need = 0;  // stack entries a traversal needs (outer + instanced subtree)
   uint64_t leaves = 0;
   float bounds[6] = {0, 0, 0, 0, 0, 0};
+  // Every child's box lies inside its parent's box and no box coordinate is NaN. True for every
+  // tree the builders make (a node's box is the fold of its triangles' boxes, which start at
+  // (+inf, -inf) and never take a NaN); checked for trees read from files.
+  bool nested = true;
   std::vector<HostMesh> meshes;
   std::vector<HostPrim> prims;
 };

@@ -227,6 +227,24 @@ bool check_subtree(const TreeLayout &t, uint32_t root, bool allow_inst, uint32_t
   return true;
 }
 
+// True when every child's box lies inside its parent's box and no box coordinate is NaN, as in
+// every tree the builders make (the traversal's fast paths rely on it; other trees take the
+// reference's exact slab test at every node).
+bool boxes_nest(const TreeLayout &t) {
+  for (const DevNode &nd : t.nodes) {
+    for (int k = 0; k < 3; ++k)
+      if (!(nd.bmin[k] == nd.bmin[k]) || !(nd.bmax[k] == nd.bmax[k])) return false;
+    if ((nd.b & 3u) == 0) continue;
+    const uint32_t nch = (uint32_t)__builtin_popcount((nd.b >> 2) & 7u);
+    for (uint32_t c = 0; c < nch; ++c) {
+      const DevNode &ch = t.nodes[nd.a + c];
+      for (int k = 0; k < 3; ++k)
+        if (!(ch.bmin[k] >= nd.bmin[k]) || !(ch.bmax[k] <= nd.bmax[k])) return false;
+    }
+  }
+  return true;
+}
+
 // Recomputes the depths the builder records and rejects a tree that would send a traversal out of
 // its arrays or past its stack (the kernels trust these values).
 bool check_tree(const TreeLayout &t) {
@@ -344,6 +362,7 @@ skw_status skw_scene_load(const char *path, uint32_t flags, skw_scene **out) {
   if (!ok) return fail(SKW_INVALID_ARG, "scene file: inconsistent sections");
   if (!check_tree(t))
     return fail(SKW_INVALID_ARG, "scene file: the tree's indices or recorded depths are inconsistent");
+  t.nested = boxes_nest(t);
   st = finish_scene(s, host_only);
   if (st != SKW_OK) return st;
   *out = s.release();

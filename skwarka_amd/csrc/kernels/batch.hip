@@ -154,14 +154,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
         if (busy && w.at_leaf) w.leaf();
       }
     }
-    const bool nan_free = __ballot(busy && !w.nan_free) == 0;
+    const bool fast = sc->nested && __ballot(busy && !w.nan_free) == 0;
     moved = moved || __ballot(busy && !w.at_leaf) != 0;
     if (!moved) {
       if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(a.stalled), 1ull);
       break;
     }
     if (busy && !w.at_leaf) {
-      const bool done = nan_free ? w.template advance<true>() : w.template advance<false>();
+      const bool done = fast ? w.template advance<true>() : w.template advance<false>();
       if (done) {
         store_hit(a.hits + sc->hit_offset + idx, w.tris, w.found, w.best, w.tmax, w.r);
         busy = false;

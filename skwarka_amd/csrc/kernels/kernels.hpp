@@ -28,6 +28,7 @@ struct TraceArgs {
   uint32_t refill_min;  // idle lanes a wave gathers before taking new rays (1..64)
   uint32_t leaf_min;    // lanes holding a leaf a wave gathers before testing leaves (1..64)
   uint32_t claim;       // rays a wave claims from the counter at once (64..4096)
+  uint32_t nested;      // 1: the tree's child boxes lie inside their parents' (no NaN coordinate)
 };
 
 // Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).
@@ -346,6 +347,8 @@ struct DevBatchScene {
   uint64_t seed;
   uint64_t n_rays;      // width * height * spp
   uint64_t hit_offset;  // first record of this scene in the hit buffer
+  uint32_t nested;      // 1: the tree's child boxes lie inside their parents' (no NaN coordinate)
+  uint32_t pad;
 };
 
 // A queue chunk: a ray range of a scene, or (item >= n_scenes) a query range of gather job

@@ -95,10 +95,12 @@ __device__ __forceinline__ TriVerts load_tri(const DevTri *__restrict__ tris, ui
   return v;
 }
 
-// The same decision as slab_cull for a ray whose slab products cannot be NaN. The products
-// (coord - o) * inv are NaN only for 0 * inf (a zero or tiny direction component with the origin
-// on the slab plane) or a non-finite origin; for a ray with finite origin and finite inverse
-// direction every product is a number (or +-inf), so the swap of :183-185 is min / max and the 12
+// The same decision as slab_cull for a ray whose slab products cannot be NaN. Box coordinates are
+// never NaN (they may be +-inf), so a product (coord - o) * inv is NaN only for 0 * inf: a zero or
+// tiny direction component with the origin on the slab plane, an infinite direction component
+// (inv = 0) against an infinite coordinate, or a non-finite origin. For a ray with finite origin,
+// direction and inverse direction every product is a number (or +-inf), so the swap of :183-185
+// is min / max and the 12
 // comparisons of :188-192 are max(tmin, a1, a2, a3) > min(b1, b2, b3) or max(a1, a2, a3) > tmax
 // (the pairs a_k > b_k they add are false since a_k <= b_k, and tmin > tmax is not one of them).
 __device__ __forceinline__ bool slab_cull_nan_free(const float4 &lo, const float4 &hi, const RayState &r,
@@ -112,8 +114,13 @@ __device__ __forceinline__ bool slab_cull_nan_free(const float4 &lo, const float
 }
 
 __device__ __forceinline__ bool ray_nan_free(const RayState &r) {
-  return isfinite(r.o0) && isfinite(r.o1) && isfinite(r.o2) && isfinite(r.i0) && isfinite(r.i1) &&
-         isfinite(r.i2);
+  return isfinite(r.o0) && isfinite(r.o1) && isfinite(r.o2) && isfinite(r.d0) && isfinite(r.d1) &&
+         isfinite(r.d2) && isfinite(r.i0) && isfinite(r.i1) && isfinite(r.i2);
+}
+
+// The child words (first child index, kind / mask / count) of a node: its last 8 bytes.
+__device__ __forceinline__ uint2 load_child_words(const DevNode *__restrict__ nodes, uint32_t idx) {
+  return reinterpret_cast<const uint2 *>(nodes + idx)[3];
 }
 
 // TriangleAccelerator.cpp:238-274. Returns true with t, b1, b2 when accepted in [tmin, tmax].
@@ -408,7 +415,8 @@ struct Walker {
     at_leaf = have && (cb & 3u) == 0;
   }
 
-  // FAST: every busy lane of the wave has nan_free set (a wave-uniform choice of the slab test).
+  // FAST: every busy lane of the wave has nan_free set and the tree's boxes nest (a wave-uniform
+  // choice of the slab test and of the pop below).
   template <bool FAST>
   __device__ __forceinline__ bool cull(const float4 &lo, const float4 &hi) const {
     return FAST ? slab_cull_nan_free(lo, hi, r, tmax) : slab_cull(lo, hi, r, tmax);
@@ -416,6 +424,8 @@ struct Walker {
 
   template <bool FAST>
   __device__ __forceinline__ bool advance() {
+    // A popped leaf in the FAST closest-hit walk: its box is the only one tested below.
+    bool single = false;
     if (!have) {
       if (sp == 0) return true;
       uint32_t e = stk.get(sp - 1);
@@ -427,28 +437,41 @@ struct Walker {
       } else {
         --sp;
       }
-      float4 lo, hi;
-      load_node(nodes, cur, lo, hi);
-      keep_live(hi);
-      // An occlusion walk never changes tmax (it ends at its first hit), so a sibling that passed
-      // its test when pushed passes it again now: only the closest-hit walk re-tests.
-      if (!ANY && cull<FAST>(lo, hi)) return false;
-      ca = __float_as_uint(hi.z);
-      cb = __float_as_uint(hi.w);
-      if ((cb & 3u) == 0) {
-        have = true;
-        at_leaf = true;
-        return false;
+      if (FAST && !ANY && !STATS) {
+        // The popped node's own re-test is left out when it has children. Its children's boxes
+        // lie inside its box, and for a NaN-free ray every slab value is monotone in the box
+        // coordinates (rounding is monotone), so each child's entry and exit values lie between
+        // the parent's: a parent culled with the current tmax has every child culled with it.
+        // Testing the children directly (as below) therefore reaches the same nodes as testing
+        // the parent first. The counting walk (STATS) keeps the reference's test of every node.
+        const uint2 cw = load_child_words(nodes, cur);
+        ca = cw.x;
+        cb = cw.y;
+        single = (cb & 3u) == 0;
+      } else {
+        float4 lo, hi;
+        load_node(nodes, cur, lo, hi);
+        keep_live(hi);
+        // An occlusion walk never changes tmax (it ends at its first hit), so a sibling that
+        // passed its test when pushed passes it again now: only the closest-hit walk re-tests.
+        if (!ANY && cull<FAST>(lo, hi)) return false;
+        ca = __float_as_uint(hi.z);
+        cb = __float_as_uint(hi.w);
+        if ((cb & 3u) == 0) {
+          have = true;
+          at_leaf = true;
+          return false;
+        }
       }
     }
     have = false;
     const uint32_t a = ca, b = cb;
-    const uint32_t kind = b & 3u;
-    const uint32_t nch = __builtin_popcount((b >> 2) & 7u);
-    const bool fwd = (dpos >> (kind - 1u)) & 1u;
+    const uint32_t kind = single ? 1u : b & 3u;
+    const uint32_t nch = single ? 1u : __builtin_popcount((b >> 2) & 7u);
+    const bool fwd = single || ((dpos >> (kind - 1u)) & 1u);
     // Children in visiting order (scalars, not arrays: an indexed array would live in scratch).
     // Absent children alias c0, so the three loads are issued together and wait once.
-    const uint32_t c0 = fwd ? a : a + nch - 1u;
+    const uint32_t c0 = single ? cur : (fwd ? a : a + nch - 1u);
     const uint32_t c1 = nch > 1 ? (fwd ? a + 1u : a + nch - 2u) : c0;
     const uint32_t c2 = nch > 2 ? a + (fwd ? 2u : 0u) : c0;
     float4 lo0, hi0, lo1, hi1, lo2, hi2;
@@ -541,37 +564,39 @@ struct Walker {
   // holds an accepted leaf). Leaves with instances or with more than kPoolMax triangles are tested
   // by their own lane as in leaf(). The other leaves' triangles are spread over the wave's lanes,
   // 64 per round: each is tested with its owner's ray and the owner's tmax at the round's start,
-  // then every owner takes its own results in its leaf's order and accepts t <= its current tmax.
+  // then every owner takes its own hits in its leaf's order and accepts t <= its current tmax.
   // That accepts exactly the triangles leaf() accepts: a test depends on tmax only through
   // t <= tmax, and tmax only shrinks along the leaf (so a rejected t stays rejected).
+  // slots: kWave * kPoolMax bytes of LDS for this wave; slot j of a phase holds the owner lane of
+  // the phase's j-th pooled triangle and the triangle's place in the owner's leaf.
   static constexpr uint32_t kPoolMax = 4;
-  __device__ __forceinline__ void leaves_pooled(bool hold) {
+  __device__ __forceinline__ void leaves_pooled(bool hold, uint8_t *slots) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t n_all = hold ? (cb >> 3) : 0u;
     const bool solo = hold && ((INST && (cb & 4u)) || n_all > kPoolMax);
     if (solo) leaf();
     const bool pooled = hold && !solo;
-    const uint32_t cnt = pooled ? n_all : 0u;
-    uint32_t incl = cnt;  // inclusive prefix of the pooled triangle counts over the wave
-    for (uint32_t d = 1; d < (uint32_t)kWave; d <<= 1) {
-      const uint32_t y = __shfl_up(incl, d);
-      if (lane >= d) incl += y;
-    }
-    const uint32_t off = incl - cnt;
-    const uint32_t total = __shfl(incl, kWave - 1);
+    const uint32_t cnt = pooled ? n_all : 0u;  // 0..kPoolMax: three bits
+    // Exclusive prefix of the counts over the wave, one ballot per bit of the count.
+    const uint64_t m1 = __ballot(cnt & 1u), m2 = __ballot(cnt & 2u), m4 = __ballot(cnt & 4u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t off = (uint32_t)(__popcll(m1 & below) + 2 * __popcll(m2 & below) + 4 * __popcll(m4 & below));
+    const uint32_t total = (uint32_t)(__popcll(m1) + 2 * __popcll(m2) + 4 * __popcll(m4));
     if (pooled) {
       at_leaf = false;
       have = false;
     }
+    for (uint32_t k = 0; k < kPoolMax; ++k)
+      if (k < cnt) slots[off + k] = (uint8_t)(lane | (k << 6));
+    // The slots are read by other lanes of this wave: keep the writes before the reads.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     bool stop = false;
     for (uint32_t base = 0; base < total; base += kWave) {
       const uint32_t j = base + lane;
-      // the owner of triangle j: the last lane whose offset is <= j
-      uint32_t o = 0;
-      for (uint32_t step = kWave / 2; step > 0; step >>= 1) {
-        const uint32_t om = __shfl(off, o + step);
-        if (om <= j) o += step;
-      }
+      const uint32_t slot = j < total ? slots[j] : 0u;
+      const uint32_t o = slot & (kWave - 1);
       RayState rr;
       rr.o0 = __shfl(r.o0, o);
       rr.o1 = __shfl(r.o1, o);
@@ -581,30 +606,40 @@ struct Walker {
       rr.d2 = __shfl(r.d2, o);
       rr.tmin = __shfl(r.tmin, o);
       const double otmax = __shfl(tmax, o);
-      const uint32_t opos = __shfl(ca, o) + (j - __shfl(off, o));
+      const uint32_t opos = __shfl(ca, o) + (slot >> 6);
       double t = 0.0, b1 = 0.0, b2 = 0.0;
       bool h = false;
       if (j < total) h = tri_test(tris, opos, rr, otmax, t, b1, b2);
-      for (uint32_t k = 0; k < kPoolMax; ++k) {
-        const uint32_t jj = off + k;
-        const bool mine = k < cnt && jj >= base && jj < base + kWave;
-        const uint32_t src = mine ? jj - base : lane;
-        const double tt = __shfl(t, src), bb1 = __shfl(b1, src), bb2 = __shfl(b2, src);
-        const bool hh = __shfl(h ? 1 : 0, src) != 0;
-        if (mine && !stop) {
-          if (STATS) ++tests;
-          if (hh && tt <= tmax) {
-            found = true;
-            if (ANY) {
-              stop = true;
-            } else {
+      // This lane's pooled triangles in the round: bits [lo, hi) of the round's lanes.
+      const int32_t s = (int32_t)off - (int32_t)base, e = s + (int32_t)cnt;
+      const int32_t lo = s < 0 ? 0 : s, hi = e > (int32_t)kWave ? (int32_t)kWave : e;
+      const uint64_t seg = hi > lo ? ((1ull << (hi - lo)) - 1ull) << lo : 0ull;
+      const uint64_t hits = __ballot(h);  // every lane votes: a stopped walk still tests for others
+      uint64_t mine = stop ? 0ull : hits & seg;
+      if (ANY) {
+        // the walk ends at its first accepted triangle
+        if (STATS) tests += (uint32_t)__popcll(mine ? seg & ((mine & (0ull - mine)) * 2ull - 1ull) : (stop ? 0ull : seg));
+        if (mine) {
+          found = true;
+          stop = true;
+        }
+      } else {
+        if (STATS) tests += (uint32_t)__popcll(seg);
+        // in leaf order, each accepted t becomes the bound for the next
+        while (__ballot(mine != 0ull)) {
+          const uint32_t src = mine ? (uint32_t)__ffsll((unsigned long long)mine) - 1u : lane;
+          const double tt = __shfl(t, src), bb1 = __shfl(b1, src), bb2 = __shfl(b2, src);
+          if (mine) {
+            if (tt <= tmax) {
+              found = true;
               tmax = tt;
               best.hit = true;
               best.inst = false;
-              best.pos = ca + k;
+              best.pos = ca + (base + src - off);
               best.b1 = bb1;
               best.b2 = bb2;
             }
+            mine &= mine - 1ull;
           }
         }
       }

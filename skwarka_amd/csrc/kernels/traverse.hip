@@ -25,7 +25,7 @@ using namespace trace;
 
 // STATS: count node boxes and triangles tested per ray (the oracle counts the same events).
 template <bool ANY, bool INST, bool STATS, class Stack>
-__device__ void run_lane(const TraceArgs &args, const Stack &stk) {
+__device__ void run_lane(const TraceArgs &args, const Stack &stk, uint8_t *slots) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t n = args.n_dev ? *args.n_dev : args.n;
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,12 +87,13 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
     const uint64_t held = __ballot(busy && w.at_leaf);
     if (held) {
       const uint64_t moving = __ballot(busy && !w.at_leaf);
-      if ((uint64_t)__popcll(held) >= args.leaf_min || moving == 0 || drained) w.leaves_pooled(busy && w.at_leaf);
+      if ((uint64_t)__popcll(held) >= args.leaf_min || moving == 0 || drained) w.leaves_pooled(busy && w.at_leaf, slots);
     }
-    // Between leaves: one move per lane; the NaN-free slab test when every busy lane allows it.
-    const bool nan_free = __ballot(busy && !w.nan_free) == 0;
+    // Between leaves: one move per lane; the NaN-free slab test (and pops without the re-test of
+    // an internal node) when the tree's boxes nest and every busy lane's ray allows it.
+    const bool fast = args.nested && __ballot(busy && !w.nan_free) == 0;
     if (busy && !w.at_leaf) {
-      const bool done = nan_free ? w.template advance<true>() : w.template advance<false>();
+      const bool done = fast ? w.template advance<true>() : w.template advance<false>();
       if (done) {
         if (ANY)
           args.occluded[idx] = w.found ? 1 : 0;
@@ -124,15 +125,17 @@ __device__ void run_lane(const TraceArgs &args, const Stack &stk) {
 template <bool ANY, bool INST, bool STATS, int S>
 __global__ __launch_bounds__(64) void k_trace_lds(TraceArgs args) {
   __shared__ uint32_t lds[S * kWave];
+  __shared__ uint8_t slots[kWave * Walker<ANY, INST, STATS, LdsStack<S>>::kPoolMax];
   LdsStack<S> stk{lds + (threadIdx.x & (kWave - 1))};
-  run_lane<ANY, INST, STATS>(args, stk);
+  run_lane<ANY, INST, STATS>(args, stk, slots);
 }
 
 template <bool ANY, bool INST, bool STATS>
 __global__ __launch_bounds__(64) void k_trace_global(TraceArgs args) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   GlobalStack stk{args.gstack + g, (uint64_t)gridDim.x * blockDim.x};
-  run_lane<ANY, INST, STATS>(args, stk);
+  __shared__ uint8_t slots[kWave * Walker<ANY, INST, STATS, GlobalStack>::kPoolMax];
+  run_lane<ANY, INST, STATS>(args, stk, slots);
 }
 
 __global__ void k_set_counter(uint64_t *counter, uint64_t value) { *counter = value; }

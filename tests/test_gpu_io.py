@@ -83,3 +83,45 @@ def test_loaded_irradiance_map_gives_the_same_final_gather(torch_cuda, tmp_path)
         torch_cuda.cuda.synchronize()
         res.append(from_dev(d, np.uint64, 32 * 32 * 3))
     assert np.array_equal(res[0], res[1])
+
+
+def test_loaded_tree_whose_boxes_do_not_nest(torch_cuda, tmp_path):
+    """A hand-edited tree whose internal boxes no longer hold their children's: the traversal's
+    fast path, which relies on child boxes lying inside their parent's, is switched off on load,
+    and the hits are those of the reference's node-by-node box test (the counting walk tests every
+    node's box, as TriangleAccelerator.cpp:165-192 does)."""
+    import struct
+    from tests.test_io import _rehash
+    print("case: loaded c0 tree with shrunk internal boxes", flush=True)
+    torch = torch_cuda
+    sc = sk.Scene(wl.scene_c0())
+    path = str(tmp_path / "s.skw")
+    sc.save(path)
+    body = bytearray(open(path, "rb").read()[:-8])
+    nodes0 = 16 + 12 + 8 + 24 + 48 + 16  # header, then section 1 of 32-byte nodes
+    root = struct.unpack_from("<I", body, 16)[0]
+    nn = struct.unpack_from("<Q", body, 16 + 12 + 8 + 24)[0]
+    shrunk = 0
+    for i in range(nn):
+        if i == root or struct.unpack_from("<I", body, nodes0 + 32 * i + 28)[0] & 3 == 0:
+            continue
+        box = np.array(struct.unpack_from("<6f", body, nodes0 + 32 * i), np.float32)
+        box[3:] = box[:3] + np.float32(0.5) * (box[3:] - box[:3])  # the lower half on every axis
+        struct.pack_into("<6f", body, nodes0 + 32 * i, *box.tolist())
+        shrunk += 1
+    assert shrunk > 100
+    bad = str(tmp_path / "bad.skw")
+    open(bad, "wb").write(_rehash(body))
+    back = sk.Scene.load(bad)
+    rays = wl.rays_c0(128, 128)
+    n = len(rays)
+    dr = to_dev(torch, rays)
+    fast, ref = empty_dev(torch, n * 32), empty_dev(torch, n * 32)
+    pops, tests = empty_dev(torch, n * 4), empty_dev(torch, n * 4)
+    back.closest_hit(dr.data_ptr(), n, fast.data_ptr())
+    back.closest_hit_stats(dr.data_ptr(), n, ref.data_ptr(), pops.data_ptr(), tests.data_ptr())
+    torch.cuda.synchronize()
+    h_fast, h_ref = from_dev(fast, np.uint8, n * 32), from_dev(ref, np.uint8, n * 32)
+    assert np.array_equal(h_fast, h_ref)
+    # the edit changes what the reference's walk finds (so the test can tell the two walks apart)
+    assert not np.array_equal(h_ref, _hits(torch, sc, rays))
