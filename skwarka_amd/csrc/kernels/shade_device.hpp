@@ -45,15 +45,9 @@ __device__ double solid_angle3(D3 d1, D3 d2, D3 d3) {
   return 2.0 * kPi;
 }
 
-__device__ double subtended_solid_angle(D3 p, const DevLight &L) {
-  D3 dirs[8];
-  for (int i = 0; i < 8; ++i) {
-    D3 v{(i & 1) == 0 ? L.bmin[0] : L.bmax[0], (i & 2) == 0 ? L.bmin[1] : L.bmax[1],
-         (i & 4) == 0 ? L.bmin[2] : L.bmax[2]};
-    D3 d = sub3(v, p);
-    normalize3(d);
-    dirs[i] = d;
-  }
+// Solid angle of the light's box seen from p, from the unit directions toward its corners
+// (dirs[i]: corner i takes bmin on axis k where bit k of i is clear, bmax where it is set).
+__device__ double subtended_solid_angle(D3 p, const DevLight &L, const D3 *dirs) {
   bool inside = true;
   double sum = 0.0;
   if (p.x < L.bmin[0] - kEps) {
@@ -80,6 +74,8 @@ __device__ double subtended_solid_angle(D3 p, const DevLight &L) {
   return inside ? 4.0 * kPi : sum;
 }
 
+// The corner directions are normalized once per light and serve both the cosine average and the
+// solid angle (the same values the two loops would each compute, summed in the same order).
 __device__ void lights_cdf(const DlArgs &a, D3 p, D3 n, double *cdf) {
   const uint32_t nl = a.n_lights;
   for (uint32_t i = 0; i < nl; ++i) {
@@ -89,17 +85,20 @@ __device__ void lights_cdf(const DlArgs &a, D3 p, D3 n, double *cdf) {
     if (inside) {
       cdf[i] = kPi * L.intensity;
     } else {
-      double avg = 0.0;
-      for (int j = 0; j < 8; ++j) {
-        D3 v{(j & 1) ? L.bmin[0] : L.bmax[0], (j & 2) ? L.bmin[1] : L.bmax[1], (j & 4) ? L.bmin[2] : L.bmax[2]};
+      D3 dirs[8];
+      for (int c = 0; c < 8; ++c) {
+        D3 v{(c & 1) ? L.bmax[0] : L.bmin[0], (c & 2) ? L.bmax[1] : L.bmin[1], (c & 4) ? L.bmax[2] : L.bmin[2]};
         D3 d = sub3(v, p);
         normalize3(d);
-        avg += dmax(0.0, dot3(d, n));
+        dirs[c] = d;
       }
+      // corners in the order bmax first on every axis (corner j of the average is dirs[j ^ 7])
+      double avg = 0.0;
+      for (int j = 0; j < 8; ++j) avg += dmax(0.0, dot3(dirs[j ^ 7], n));
       avg *= 0.125;
       if (avg > 0.0) {
         if (avg < 0.1) avg = 0.1;
-        cdf[i] = subtended_solid_angle(p, L) * L.intensity * avg;
+        cdf[i] = subtended_solid_angle(p, L, dirs) * L.intensity * avg;
       } else {
         cdf[i] = 0.0;
       }
