@@ -10,6 +10,7 @@
 // MatteMaterial / PlasticMaterial do (MatteMaterial.cpp:33-34, PlasticMaterial.cpp:37-45).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -22,6 +23,9 @@ using namespace skw;
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+// Sample passes per launch of a direct-lighting frame (SKW_DL_PASSES_PER_LAUNCH overrides).
+constexpr int kDlPassesPerLaunch = 16;
+constexpr int kDlMaxPassesPerLaunch = 16;
 constexpr uint32_t kMaxLights = 16;
 const double kLumW[3] = {0x1.b38293da589a6p-5, 0x1.2b993cc221566p-1, 0x1.725d3400723f6p-2};
 
@@ -85,22 +89,47 @@ skw_status check_frame_args(const skw_scene *scene, const skw_dl_scene *dl, cons
   if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(SKW_INVALID_ARG, "empty image");
   if (prm->light_samples == 0 || prm->light_samples > 8 || prm->bsdf_samples == 0 || prm->bsdf_samples > 8)
     return fail(SKW_INVALID_ARG, "light_samples and bsdf_samples must be in [1, 8]");
+  // shadow / BSDF ray slots of a launch are 32-bit indices
+  if ((uint64_t)prm->width * prm->height * std::max(prm->light_samples, prm->bsdf_samples) > (uint64_t)INT32_MAX)
+    return fail(SKW_INVALID_ARG, "image too large: width x height x samples per hit must stay below 2^31");
   return SKW_OK;
 }
 
-// The wavefront of one frame: per sample pass, camera -> closest hit -> shade -> occlusion and
-// BSDF rays (ray counts read on the device) -> resolve; then the image (or nothing, for the
-// adjoint, whose resolve accumulates the gradient).
+// The wavefront of one frame: per launch of P sample passes, camera -> closest hit -> shade ->
+// occlusion and BSDF rays (ray counts read on the device) -> resolve; then the image (or nothing,
+// for the adjoint, whose resolve accumulates the gradient). A launch covers P passes of every
+// pixel (P a power of two dividing spp, at most kDlMaxPassesPerLaunch): fewer, larger trace
+// launches with shorter drains, and the P samples of a pixel in adjacent slots (coherent waves).
+// Every sample's values are those of its own one-pass launch, and the film adds a pixel's samples
+// in pass order, so the image does not depend on P.
 // prof (optional): HIP-event time per stage summed over the passes; count: also count node and
 // triangle tests per ray class (the walk's counting variant).
 skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const ParamPlan &pp, const skw_camera *cam,
                      const skw_dl_params *prm, const FrameOut &out, uint64_t *rays_out, hipStream_t st,
                      skw_dl_profile *prof = nullptr, bool count = false) {
   const uint64_t npix = (uint64_t)prm->width * prm->height;
-  const uint64_t batch = npix;
   const uint32_t nls = prm->light_samples, nbs = prm->bsdf_samples, nl = (uint32_t)dl->lights.size();
   const uint32_t np = (uint32_t)dl->mats.size();
   const bool adj = out.adj != nullptr;
+  // Passes per launch: the largest power of two up to the setting that divides spp and keeps the
+  // frame buffer (per_slot bytes per sample slot, the per-slot arrays laid out below) within a
+  // quarter of the device's free memory.
+  uint32_t ppl = (uint32_t)std::min(kDlMaxPassesPerLaunch, std::max(1, env_int("SKW_DL_PASSES_PER_LAUNCH",
+                                                                                  kDlPassesPerLaunch)));
+  uint32_t ppl_log2 = 0;
+  while ((2u << ppl_log2) <= ppl) ++ppl_log2;
+  while (ppl_log2 > 0 && prm->spp % (1u << ppl_log2) != 0) --ppl_log2;
+  const uint64_t per_slot = sizeof(skw_ray) + sizeof(skw_hit) + (3 + (nl ? nl : 1) + 3 + 24 + 4) * sizeof(double) +
+                            nls * (sizeof(skw_ray) + sizeof(int32_t) + 3 * sizeof(double) + 1) +
+                            nbs * (sizeof(skw_ray) + sizeof(int32_t) + 8 * sizeof(double) + sizeof(skw_hit)) +
+                            ((adj || out.tape) ? nls * 6 * sizeof(double) + sizeof(int32_t) : 0);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  while (ppl_log2 > 0 && (npix * (1ull << ppl_log2) * per_slot > free_b / 4 ||
+                          npix * (1ull << ppl_log2) * std::max(nls, nbs) > (uint64_t)INT32_MAX))
+    --ppl_log2;
+  const uint32_t ppl_eff = 1u << ppl_log2;
+  const uint64_t batch = npix * ppl_eff;
   DlArgs a;
   std::memset(&a, 0, sizeof(a));
   a.geom = scene->geom_args();
@@ -119,6 +148,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   a.seed = prm->seed;
   a.pix0 = 0;
   a.batch = batch;
+  a.ppl_log2 = ppl_log2;
   // One device block holds every per-frame buffer.
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -143,7 +173,9 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   size_t o_brec = take(batch * nbs * 8 * sizeof(double)), o_bhits = take(batch * nbs * sizeof(skw_hit));
   size_t o_counts = take(2 * sizeof(uint64_t)), o_totals = take(9 * sizeof(uint64_t));
   size_t o_film = take(npix * 3 * sizeof(double));
-  size_t o_saux = 0, o_caml = 0;
+  size_t o_saux = 0, o_caml = 0, o_slot_l = 0;
+  const bool slot_film = ppl_eff > 1 && !adj && !out.sample_L;
+  if (slot_film) o_slot_l = take(batch * 4 * sizeof(double));
   if (adj || out.tape) {
     o_saux = take(batch * nls * 6 * sizeof(double));
     o_caml = take(batch * sizeof(int32_t));
@@ -199,6 +231,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   a.counts = (uint64_t *)(base + o_counts);
   a.totals = (uint64_t *)(base + o_totals);
   a.film = (double *)(base + o_film);
+  if (slot_film) a.slot_L = (double *)(base + o_slot_l);
   a.sample_L = out.sample_L;
   if (adj || out.tape) {
     a.adj = out.adj;
@@ -216,7 +249,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   // occlusion trace / BSDF trace / resolve
   std::vector<hipEvent_t> ev;
   if (prof) {
-    ev.assign(6 * (size_t)prm->spp + 1, nullptr);
+    ev.assign(6 * (size_t)(prm->spp / ppl_eff) + 1, nullptr);
     for (auto &x : ev)
       if (e == hipSuccess) e = hipEventCreate(&x);
   }
@@ -238,7 +271,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
   if (e == hipSuccess) e = launch_dl_clear(a, st);
   if (e == hipSuccess && adj) e = launch_dl_clear_grad(out.grad, 2 * dl->n_groups + nl, st);
   mark();
-  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
+  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; s += ppl_eff) {
     a.pass = s;
     e = launch_dl_camera(a, st);
     if (e != hipSuccess) break;
@@ -274,6 +307,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     }
     mark();
     e = launch_dl_resolve(a, st);
+    if (e == hipSuccess && a.slot_L) e = launch_dl_accum(a, st);
     mark();
   }
   if (e == hipSuccess && rs == SKW_OK) {

@@ -35,8 +35,18 @@ __device__ __forceinline__ uint64_t pixel_of(uint32_t width, uint32_t height, ui
   const uint64_t x = (tile % tiles_x) * 8 + (in & 7u), y = (tile / tiles_x) * 8 + (in >> 3);
   return y * width + x;
 }
+// Sample slot i of a launch: pixel slot i >> ppl_log2, pass pass + (i & (passes - 1)). A pixel's
+// samples of the launch are adjacent slots, so a wave's 64 camera rays cover 64 >> ppl_log2 pixels
+// of one tile (with one pass per launch, 64 pixels of one pass).
 __device__ __forceinline__ uint64_t pixel_of(const DlArgs &a, uint64_t i) {
-  return pixel_of(a.width, a.height, a.pix0 + i);
+  return pixel_of(a.width, a.height, a.pix0 + (i >> a.ppl_log2));
+}
+__device__ __forceinline__ uint32_t pass_of(const DlArgs &a, uint64_t i) {
+  return a.pass + (uint32_t)(i & ((1ull << a.ppl_log2) - 1ull));
+}
+// The sample's row of the frame's tape: pass-major over the batch's pixel slots.
+__device__ __forceinline__ uint64_t tape_row(const DlArgs &a, uint64_t i) {
+  return (uint64_t)pass_of(a, i) * (a.batch >> a.ppl_log2) + (i >> a.ppl_log2);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -50,7 +60,7 @@ __global__ __launch_bounds__(256) void k_dl_camera(DlArgs a) {
   if (i >= a.batch) return;
   const uint64_t pix = pixel_of(a, i);
   const uint32_t px = (uint32_t)(pix % a.width), py = (uint32_t)(pix / a.width);
-  const uint64_t sidx = pix * a.spp + a.pass;
+  const uint64_t sidx = pix * a.spp + pass_of(a, i);
   double ix = (double)px + rnd(a.seed, sidx, 0), iy = (double)py + rnd(a.seed, sidx, 1);
   double cx = 2.0 * a.x_tan * (ix - a.width * 0.5) / a.width;
   double cy = 2.0 * a.y_tan * (iy - a.height * 0.5) / a.height;
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_dl_light(DlArgs a) {
   if (shade) {
     Bsdf b;
     point_bsdf(a, i, pt.prim, b);
-    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    const uint64_t sidx = pixel_of(a, i) * a.spp + pass_of(a, i);
     const double *cdf = a.cdf + i * nl;
     double uc = rnd(a.seed, sidx, 2 + k);
     double u0 = rnd(a.seed, sidx, 6 + 2 * k), u1 = rnd(a.seed, sidx, 7 + 2 * k);
@@ -275,7 +285,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_dl_bsdf(DlArgs a) {
   if (shade) {
     Bsdf b;
     point_bsdf(a, i, pt.prim, b);
-    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    const uint64_t sidx = pixel_of(a, i) * a.spp + pass_of(a, i);
     double uc = rnd(a.seed, sidx, 14 + k);
     double u0 = rnd(a.seed, sidx, 18 + 2 * k), u1 = rnd(a.seed, sidx, 19 + 2 * k);
     D3 wo_l, wi_l;
@@ -379,7 +389,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
   if (shade) {
     Bsdf b;
     point_bsdf(a, i, pt.prim, b);
-    const uint64_t sidx = pixel_of(a, i) * a.spp + a.pass;
+    const uint64_t sidx = pixel_of(a, i) * a.spp + pass_of(a, i);
     const uint32_t gq = f.gs_sqrt;
     const double inv_n = 1.0 / gs, inv_q = 1.0 / gq;
     const uint32_t x = k % gq, y = k / gq;
@@ -590,7 +600,7 @@ __global__ __launch_bounds__(256) void k_fg_resolve(FgArgs f) {
   for (int c = 0; c < 3; ++c) {
     const double L = rad[c] * inv;
     a.film[3 * pix + c] += L;
-    if (f.radiance) f.radiance[3 * (pix * a.spp + a.pass) + c] = L;
+    if (f.radiance) f.radiance[3 * (pix * a.spp + pass_of(a, i)) + c] = L;
   }
 }
 
@@ -676,8 +686,15 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
     if (a.sample_L) {
       // the sum before the renderer's rule: the composition adds the other terms first
       if (in) {
-        double *o = a.sample_L + 3 * (pixel_of(a, i) * a.spp + a.pass);
+        double *o = a.sample_L + 3 * (pixel_of(a, i) * a.spp + pass_of(a, i));
         for (int c = 0; c < 3; ++c) o[c] = L[c];
+      }
+    } else if (a.slot_L) {
+      // several passes per launch: k_dl_accum adds the launch's samples of a pixel in pass order
+      if (in) {
+        double *o = a.slot_L + 4 * i;
+        for (int c = 0; c < 3; ++c) o[c] = L[c];
+        o[3] = bad ? 1.0 : 0.0;
       }
     } else if (in && !bad) {
       double *f = a.film + 3 * pixel_of(a, i);
@@ -685,7 +702,7 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
     }
     if (TAPE && in) {
       // row: [0..2] reflectance, [3..5] roughness, [6 + 3l + c] emission scale of light l
-      double *J = a.tape_j + (a.pass * a.batch + i) * (uint64_t)a.tape_stride;
+      double *J = a.tape_j + tape_row(a, i) * (uint64_t)a.tape_stride;
       for (uint32_t t = 0; t < a.tape_stride; ++t) J[t] = 0.0;
       uint32_t grp = kMiss;
       if (h.prim != kMiss && !bad) {
@@ -730,7 +747,7 @@ __global__ __launch_bounds__(256) void k_dl_resolve(DlArgs a) {
           J[3 + c] = js[c];
         }
       }
-      a.tape_group[a.pass * a.batch + i] = grp;
+      a.tape_group[tape_row(a, i)] = grp;
     }
     return;
   }
@@ -837,9 +854,25 @@ __global__ __launch_bounds__(256) void k_dl_clear(DlArgs a) {
   if (i < n) a.film[i] = 0.0;
 }
 
+// Several passes per launch: the launch's samples of each pixel added to the film in pass order,
+// as the one-pass launches add them (dropped samples skipped), one thread per pixel slot.
+__global__ __launch_bounds__(256) void k_dl_accum(DlArgs a) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (a.batch >> a.ppl_log2)) return;
+  const uint64_t first = q << a.ppl_log2;
+  double *f = a.film + 3 * pixel_of(a, first);
+  double acc[3] = {f[0], f[1], f[2]};
+  for (uint64_t k = 0; k < (1ull << a.ppl_log2); ++k) {
+    const double *o = a.slot_L + 4 * (first + k);
+    if (o[3] == 0.0)
+      for (int c = 0; c < 3; ++c) acc[c] += o[c];
+  }
+  for (int c = 0; c < 3; ++c) f[c] = acc[c];
+}
+
 __global__ __launch_bounds__(256) void k_dl_finish(DlArgs a, float *rgb) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.batch) return;
+  if (i >= (a.batch >> a.ppl_log2)) return;
   const uint64_t pix = a.pix0 + i;
   for (int c = 0; c < 3; ++c) rgb[3 * pix + c] = (float)(a.film[3 * pix + c] / (double)a.spp);
 }
@@ -870,7 +903,7 @@ __global__ __launch_bounds__(256) void k_pm_combine(const double *sample_dl, con
 
 __global__ __launch_bounds__(256) void k_dl_finish_f64(DlArgs a, double *img) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.batch) return;
+  if (i >= (a.batch >> a.ppl_log2)) return;
   const uint64_t pix = a.pix0 + i;
   for (int c = 0; c < 3; ++c) img[3 * pix + c] = a.film[3 * pix + c] / (double)a.spp;
 }
@@ -985,7 +1018,7 @@ hipError_t launch_dl_tape_adjoint(const DlTapeArgs &t, hipStream_t stream) {
 }
 
 hipError_t launch_dl_finish_f64(const DlArgs &a, double *img, hipStream_t stream) {
-  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  uint32_t grid = (uint32_t)(((a.batch >> a.ppl_log2) + 255) / 256);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dl_finish_f64, dim3(grid), dim3(256), 0, stream, a, img);
   return hipGetLastError();
@@ -1013,9 +1046,16 @@ hipError_t launch_pm_combine(const double *sample_dl, const double *sample_fg, u
 }
 
 hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream) {
-  uint32_t grid = (uint32_t)((a.batch + 255) / 256);
+  uint32_t grid = (uint32_t)(((a.batch >> a.ppl_log2) + 255) / 256);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dl_finish, dim3(grid), dim3(256), 0, stream, a, rgb);
+  return hipGetLastError();
+}
+
+hipError_t launch_dl_accum(const DlArgs &a, hipStream_t stream) {
+  uint32_t grid = (uint32_t)(((a.batch >> a.ppl_log2) + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dl_accum, dim3(grid), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

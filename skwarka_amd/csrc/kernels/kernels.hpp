@@ -157,9 +157,11 @@ struct DlArgs {
   double cam_o[3], cam_r[9], x_tan, y_tan;
   uint32_t width, height, spp, nls, nbs;
   uint64_t seed;
-  uint32_t pass;      // sample index s of this pass
+  uint32_t pass;      // sample index s of the launch's first pass
+  uint32_t ppl_log2;  // passes per launch = 1 << ppl_log2: slot i is pixel slot i >> ppl_log2 of
+                      // pass pass + (i & (passes - 1)), so a pixel's samples of the launch are adjacent
   uint64_t pix0;      // first pixel of this batch
-  uint64_t batch;     // pixels in this batch
+  uint64_t batch;     // sample slots in this launch (pixels x passes per launch)
   skw_ray *prim_rays;
   skw_hit *prim_hits;
   double *le;         // 3 per sample: emitted radiance seen by the camera ray
@@ -176,6 +178,8 @@ struct DlArgs {
   uint8_t *occluded;
   skw_hit *bsdf_hits;
   double *film;          // 3 per pixel, running sum over passes
+  double *slot_L;        // several passes per launch: 4 per slot (L rgb, 1 when the sample is dropped),
+                         // added to the film in pass order by k_dl_accum
   double *sample_L;      // when set: 3 per sample (pixel * spp + pass), Le + direct lighting before the
                          // renderer's sample rule, instead of the film (photon-mapping composition)
   uint64_t *totals;      // [0] primary, [1] shadow, [2] BSDF rays over the frame (device); [3..8]
@@ -185,7 +189,7 @@ struct DlArgs {
   double *grad;          // 2 * n_groups + n_lights
   double *shadow_aux;    // 6 per (sample, k): f rgb (evaluation), weight, d f / d roughness, light
   int32_t *cam_light;    // per sample: light seen by the camera ray, or -1
-  // tape (when tape_j is set): per sample of the frame (pass * batch + slot) a row of tape_stride
+  // tape (when tape_j is set): per sample of the frame (pass * pixels + pixel slot) a row of tape_stride
   // derivatives and the hit primitive's group
   double *tape_j;
   uint32_t *tape_group;
@@ -320,6 +324,7 @@ hipError_t launch_dl_point(const DlArgs &a, hipStream_t stream);  // shading-poi
 hipError_t launch_dl_shade(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_resolve(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_finish(const DlArgs &a, float *rgb, hipStream_t stream);
+hipError_t launch_dl_accum(const DlArgs &a, hipStream_t stream);
 hipError_t launch_dl_finish_f64(const DlArgs &a, double *img, hipStream_t stream);
 hipError_t launch_dl_clear_grad(double *grad, uint32_t n, hipStream_t stream);
 

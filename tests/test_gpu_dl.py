@@ -99,3 +99,51 @@ def test_profiled_frame_is_the_render_and_counts_the_walk(torch_cuda):
     torch.cuda.synchronize()
     assert p2["node_tests"][0] == int(from_dev(d_pops, np.uint32, n).astype(np.int64).sum())
     assert p2["tri_tests"][0] == int(from_dev(d_tests, np.uint32, n).astype(np.int64).sum())
+
+
+def test_passes_per_launch_do_not_change_the_frame(torch_cuda, monkeypatch):
+    """A frame's launches cover P sample passes of every pixel (P a power of two dividing spp, the
+    largest such not above SKW_DL_PASSES_PER_LAUNCH): each sample keeps its own values and the film
+    adds a pixel's samples in pass order, so the image, the ray counts, the replayed image and the
+    tape are those of one pass per launch, bit for bit; adjoints (whose FP64 atomics add in any
+    order) agree to rounding. The default P renders as the oracle does."""
+    print("case: direct lighting, 1 / 2 / 4 / 8 passes per launch, small room 48x40, 8 and 6 spp", flush=True)
+    torch = torch_cuda
+    s = wl.scene_room_small()
+    w, h = 48, 40
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"],
+                     prim_group=s["prim_group"], n_groups=s["n_groups"])
+    cam, cam15 = sk.camera(*s["camera"], w, h)
+    th0 = np.asarray(s["theta0"], np.float64)
+    rng = np.random.default_rng(3)
+    adj = torch.from_numpy(rng.normal(size=(w * h, 3))).cuda()
+    runs = {}
+    for spp in (8, 6):
+        for P in (1, 2, 4, 8):
+            monkeypatch.setenv("SKW_DL_PASSES_PER_LAUNCH", str(P))
+            d_rgb = empty_dev(torch, w * h * 12)
+            cnt = gdl.render(cam, w, h, spp, d_rgb.data_ptr(), 2, 3, 21)
+            img = torch.empty((w * h, 3), dtype=torch.float64, device="cuda")
+            rcnt = gdl.render_replay(th0, th0, cam, w, h, spp, img.data_ptr(), 2, 3, 21)
+            timg = torch.empty_like(img)
+            tape = gdl.render_tape(th0, cam, w, h, spp, timg.data_ptr(), 1 << 30, 2, 3, 21)
+            g_tape = torch.empty(gdl.n_params, dtype=torch.float64, device="cuda")
+            tape.adjoint(adj.data_ptr(), g_tape.data_ptr())
+            g_adj = torch.empty_like(g_tape)
+            gdl.adjoint(th0, cam, w, h, spp, adj.data_ptr(), g_adj.data_ptr(), 2, 3, 21)
+            torch.cuda.synchronize()
+            tape.close()
+            runs[(spp, P)] = (from_dev(d_rgb, np.uint8, w * h * 12), cnt, rcnt, img.cpu().numpy(), timg.cpu().numpy(),
+                              g_tape.cpu().numpy(), g_adj.cpu().numpy())
+        base = runs[(spp, 1)]
+        for P in (2, 4, 8):
+            r = runs[(spp, P)]
+            for k in range(5):
+                assert np.array_equal(base[k], r[k]), (spp, P, k)
+            for k in (5, 6):
+                assert np.all(np.abs(base[k] - r[k]) <= 1e-12 * np.abs(base[k]).max()), (spp, P, k)
+    monkeypatch.delenv("SKW_DL_PASSES_PER_LAUNCH")
+    got, cnt, ref, rcnt = render_both(torch, s, w, h, 4, 2, 3, 21)
+    assert np.array_equal(cnt, rcnt)
+    assert_close_rel(got, ref)
