@@ -167,7 +167,7 @@ def bench_c2(torch, sk, wl, steps, warmup, stream):
     rgb = d_rgb.cpu().numpy().view(np.float32)
     return {"ms": ms, "value": total / (ms * 1e-3) / 1e6, "rays": [int(x) for x in rays], "build_s": build_s,
             "info": scene.info(), "spec": s, "cam15": cam15, "mean_rgb": float(rgb.mean()), "scene": scene,
-            "stage_ms": stage_ms, "trace_ms": trace_ms, "alg_bytes": alg_bytes,
+            "stage_ms": stage_ms, "trace_ms": trace_ms, "alg_bytes": alg_bytes, "launches": timed[-1]["launches"],
             "achieved_gbs": alg_bytes / (trace_ms * 1e-3) / 1e9,
             "mean_node_tests": (nodes / np.maximum(rays, 1)).tolist(),
             "mean_tri_tests": (tris / np.maximum(rays, 1)).tolist()}
@@ -948,10 +948,11 @@ def main():
                                            "lanes per instruction in 'pmc'; most node and triangle reads hit L2, so "
                                            "HBM traffic ('traffic') is a fraction of the algorithmic bytes and "
                                            "'frac' is not a DRAM figure",
-                                "kernel": "k_trace_lds (closest + any hit), the 48 trace launches of a frame",
-                                "launches_per_frame": 48,
-                                "alg_bytes_per_launch": c2["alg_bytes"] // 48,
-                                "avg_launch_ms": round(c2["trace_ms"] / 48, 4),
+                                "kernel": "k_trace_lds (closest + any hit): per launch of %d sample passes, "
+                                          "one primary, one occlusion and one BSDF trace" % (16 // c2["launches"]),
+                                "launches_per_frame": 3 * c2["launches"],
+                                "alg_bytes_per_launch": c2["alg_bytes"] // (3 * c2["launches"]),
+                                "avg_launch_ms": round(c2["trace_ms"] / (3 * c2["launches"]), 4),
                                 "trace_ms_per_frame": round(c2["trace_ms"], 3),
                                 "alg_bytes_per_frame": c2["alg_bytes"],
                                 "mean_node_tests": [round(x, 3) for x in c2["mean_node_tests"]],

@@ -254,13 +254,15 @@ skw_status skw_dl_render(const skw_scene *scene, const skw_dl_scene *dl, const s
                          const skw_dl_params *params, float *d_rgb, uint64_t *rays_out, void *stream);
 
 /* Per-frame profile of skw_dl_render's wavefront: HIP-event time of each stage summed over the
- * sample passes, the rays cast per class, and (when counted) the node boxes and triangles the
- * traversal tested per class. Classes: 0 primary, 1 occlusion, 2 BSDF. */
+ * sample passes, the rays cast per class, (when counted) the node boxes and triangles the
+ * traversal tested per class, and the frame's launches: each launch covers a group of sample
+ * passes of every pixel and traces each ray class once. Classes: 0 primary, 1 occlusion, 2 BSDF. */
 typedef struct {
   double stage_ms[6]; /* camera, primary trace, shade, occlusion trace, BSDF trace, resolve */
   uint64_t rays[3];
   uint64_t node_tests[3];
   uint64_t tri_tests[3];
+  uint64_t launches; /* pass groups of the frame (spp / passes per launch) */
 } skw_dl_profile;
 
 /* skw_dl_render with a profile (no counterpart in the reference; for measurement). count_tests

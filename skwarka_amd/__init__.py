@@ -74,13 +74,14 @@ class DLParams(ctypes.Structure):
 
 class DLProfile(ctypes.Structure):
     _fields_ = [("stage_ms", ctypes.c_double * 6), ("rays", _U64 * 3), ("node_tests", _U64 * 3),
-                ("tri_tests", _U64 * 3)]
+                ("tri_tests", _U64 * 3), ("launches", _U64)]
 
     STAGES = ("camera", "primary_trace", "shade", "occlusion_trace", "bsdf_trace", "resolve")
 
     def as_dict(self):
         return {"stage_ms": dict(zip(self.STAGES, list(self.stage_ms))), "rays": list(self.rays),
-                "node_tests": list(self.node_tests), "tri_tests": list(self.tri_tests)}
+                "node_tests": list(self.node_tests), "tri_tests": list(self.tri_tests),
+                "launches": int(self.launches)}
 
 
 class IrrMapDesc(ctypes.Structure):

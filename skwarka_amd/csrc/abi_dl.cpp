@@ -258,12 +258,15 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
     if (prof && e == hipSuccess) e = hipEventRecord(ev[ev_next++], st);
   };
   uint64_t *sums = count ? a.totals + 3 : nullptr;
-  // The occlusion and BSDF traces of a pass are independent: the BSDF trace runs on a second
-  // stream beside the occlusion trace, so one launch's tail (its last long walks) overlaps the
-  // other. A profiled frame keeps one stream, so that its events time each stage alone.
+  // The occlusion and BSDF traces of a launch are independent. When a launch covers one pass, the
+  // BSDF trace runs on a second stream beside the occlusion trace, so one trace's tail (its last
+  // long walks) overlaps the other. Launches of several passes have tails short against their
+  // length, and two persistent grids sharing the chip cost more than the overlap saves (c2 frame
+  // 64.2 ms with the second stream, 63.0 ms without). A profiled frame keeps one stream, so that
+  // its events time each stage alone. SKW_DL_SIDE_STREAM=0 / 1 forces the choice.
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  if (!prof) {
+  if (!prof && env_int("SKW_DL_SIDE_STREAM", ppl_eff == 1 ? 1 : 0) != 0) {
     if (e == hipSuccess) e = side_stream(&st2);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming);
@@ -331,6 +334,7 @@ skw_status run_frame(const skw_scene *scene, const skw_dl_scene *dl, const Param
       prof->node_tests[c] = totals[3 + 2 * c];
       prof->tri_tests[c] = totals[4 + 2 * c];
     }
+    prof->launches = prm->spp / ppl_eff;
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);

@@ -37,7 +37,10 @@ namespace {
 constexpr double kInvPi = 0.31830988618379067154;
 constexpr uint32_t kFgK = 32;          // indirect photons per final-gather lookup (PhotonLTEIntegrator.cpp:459)
 constexpr uint32_t kMaxGather = 676;   // 26^2: the sample stream has 4096 dims per sample, 6 per gather sample
-constexpr uint64_t kMaxGatherRays = uint64_t(1) << 23;  // gather rays held per pixel batch
+// Gather rays held per pixel batch, at most (SKW_FG_GATHER_RAYS_LOG2 overrides); fewer when the
+// batch's buffers would exceed a quarter of the device's free memory. Ray indices within a batch
+// are int32.
+constexpr int kMaxGatherRaysLog2 = 26;
 constexpr int kKeyLo = 12, kKeyHi = 30;  // key bits of the lookup points' 64^3 Morton cells
 
 double inf() { return std::numeric_limits<double>::infinity(); }
@@ -277,9 +280,20 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const uint64_t n_ind = indirect ? indirect->n : 0;
   // PhotonLTEIntegrator.cpp:460-462: the radius holding 32 indirect photons on average.
   const double lookup_r = n_ind > 0 ? std::sqrt((scene_area / (double)n_ind) * kFgK * kInvPi) : 0.0;
-  // Pixel batches: every slot holds up to 2 gs gather rays.
-  uint64_t batch = std::max<uint64_t>(64, kMaxGatherRays / (2ull * gs));
+  // Pixel batches: every slot holds up to 2 gs gather rays. The batch does not change any value
+  // (the sample stream depends on the pixel and the sample only, and the film adds a pixel's
+  // samples in pass order); larger batches mean fewer launches and fewer launch tails.
+  // Bytes held per pixel slot and per gather ray (the buffers laid out below).
+  const uint64_t per_pixel = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 24 + 3 + 3) * sizeof(double) +
+                             kFgK * sizeof(uint32_t) + sizeof(uint32_t) + 2ull * gs * sizeof(int32_t);
+  const uint64_t per_ray = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 3 + 3) * sizeof(double) + 1 +
+                           5 * sizeof(uint32_t) + 16;  // + radix-sort workspace, about 8 B per key
+  const int rays_log2 = std::min(28, std::max(10, env_int("SKW_FG_GATHER_RAYS_LOG2", kMaxGatherRaysLog2)));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  uint64_t batch = std::max<uint64_t>(64, (uint64_t(1) << rays_log2) / (2ull * gs));
   batch = std::min(batch, npix);
+  while (batch > 64 && batch * (per_pixel + 2ull * gs * per_ray) > free_b / 4) batch = (batch + 1) / 2;
   const uint64_t max_rays = batch * 2ull * gs;
 
   const DevMat *mats = nullptr;
