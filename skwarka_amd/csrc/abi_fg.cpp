@@ -37,10 +37,11 @@ namespace {
 constexpr double kInvPi = 0.31830988618379067154;
 constexpr uint32_t kFgK = 32;          // indirect photons per final-gather lookup (PhotonLTEIntegrator.cpp:459)
 constexpr uint32_t kMaxGather = 676;   // 26^2: the sample stream has 4096 dims per sample, 6 per gather sample
-// Gather rays held per pixel batch, at most (SKW_FG_GATHER_RAYS_LOG2 overrides); fewer when the
-// batch's buffers would exceed a quarter of the device's free memory. Ray indices within a batch
+// Gather rays held per launch, at most (SKW_FG_GATHER_RAYS_LOG2 overrides); fewer when the
+// launch's buffers would exceed a quarter of the device's free memory. Ray indices within a launch
 // are int32.
-constexpr int kMaxGatherRaysLog2 = 26;
+constexpr int kMaxGatherRaysLog2 = 28;
+constexpr uint32_t kMaxPassesPerLaunchLog2 = 4;
 constexpr int kKeyLo = 12, kKeyHi = 30;  // key bits of the lookup points' 64^3 Morton cells
 
 double inf() { return std::numeric_limits<double>::infinity(); }
@@ -280,21 +281,31 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const uint64_t n_ind = indirect ? indirect->n : 0;
   // PhotonLTEIntegrator.cpp:460-462: the radius holding 32 indirect photons on average.
   const double lookup_r = n_ind > 0 ? std::sqrt((scene_area / (double)n_ind) * kFgK * kInvPi) : 0.0;
-  // Pixel batches: every slot holds up to 2 gs gather rays. The batch does not change any value
-  // (the sample stream depends on the pixel and the sample only, and the film adds a pixel's
-  // samples in pass order); larger batches mean fewer launches and fewer launch tails.
-  // Bytes held per pixel slot and per gather ray (the buffers laid out below).
-  const uint64_t per_pixel = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 24 + 3 + 3) * sizeof(double) +
-                             kFgK * sizeof(uint32_t) + sizeof(uint32_t) + 2ull * gs * sizeof(int32_t);
+  // Launches: a pixel batch (every slot holds up to 2 gs gather rays) and, when one batch holds every
+  // pixel, a power-of-two group of sample passes (at most 16, dividing spp): slot i is pixel
+  // i >> ppl_log2 of pass pass + (i & (passes - 1)), as in the direct-lighting frame. Neither
+  // changes any value (the sample stream depends on the pixel and the sample only, and the film
+  // adds a pixel's samples in pass order); larger launches mean fewer launch tails.
+  // Bytes held per sample slot and per gather ray (the buffers laid out below).
+  const uint64_t per_slot = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 24 + 3 + 3 + 4) * sizeof(double) +
+                            kFgK * sizeof(uint32_t) + sizeof(uint32_t) + 2ull * gs * sizeof(int32_t);
   const uint64_t per_ray = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 3 + 3) * sizeof(double) + 1 +
                            5 * sizeof(uint32_t) + 16;  // + radix-sort workspace, about 8 B per key
+  const uint64_t slot_bytes = per_slot + 2ull * gs * per_ray;
   const int rays_log2 = std::min(28, std::max(10, env_int("SKW_FG_GATHER_RAYS_LOG2", kMaxGatherRaysLog2)));
+  const uint64_t rays_cap = uint64_t(1) << rays_log2;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-  uint64_t batch = std::max<uint64_t>(64, (uint64_t(1) << rays_log2) / (2ull * gs));
+  uint64_t batch = std::max<uint64_t>(64, rays_cap / (2ull * gs));
   batch = std::min(batch, npix);
-  while (batch > 64 && batch * (per_pixel + 2ull * gs * per_ray) > free_b / 4) batch = (batch + 1) / 2;
-  const uint64_t max_rays = batch * 2ull * gs;
+  while (batch > 64 && batch * slot_bytes > free_b / 4) batch = (batch + 1) / 2;
+  uint32_t ppl_log2 = 0;
+  if (batch == npix)
+    while (ppl_log2 < kMaxPassesPerLaunchLog2 && prm->spp % (2u << ppl_log2) == 0 &&
+           (npix << (ppl_log2 + 1)) * 2ull * gs <= rays_cap && (npix << (ppl_log2 + 1)) * slot_bytes <= free_b / 4)
+      ++ppl_log2;
+  const uint64_t slots = batch << ppl_log2;
+  const uint64_t max_rays = slots * 2ull * gs;
 
   const DevMat *mats = nullptr;
   const double *rad = nullptr;
@@ -326,12 +337,13 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   f.radiance = d_radiance;
 
   Block b;
-  const size_t o_prays = b.take(batch * sizeof(skw_ray)), o_phits = b.take(batch * sizeof(skw_hit));
-  const size_t o_le = b.take(batch * 3 * sizeof(double)), o_ns = b.take(batch * 3 * sizeof(double));
-  const size_t o_cdf = b.take(sizeof(double)), o_point = b.take(batch * 24 * sizeof(double));
-  const size_t o_spts = b.take(batch * 3 * sizeof(double)), o_snrm = b.take(batch * 3 * sizeof(double));
-  const size_t o_nids = b.take(batch * kFgK * sizeof(uint32_t)), o_ncnt = b.take(batch * sizeof(uint32_t));
-  const size_t o_idx = b.take(batch * 2 * gs * sizeof(int32_t));
+  const size_t o_prays = b.take(slots * sizeof(skw_ray)), o_phits = b.take(slots * sizeof(skw_hit));
+  const size_t o_le = b.take(slots * 3 * sizeof(double)), o_ns = b.take(slots * 3 * sizeof(double));
+  const size_t o_cdf = b.take(sizeof(double)), o_point = b.take(slots * 24 * sizeof(double));
+  const size_t o_spts = b.take(slots * 3 * sizeof(double)), o_snrm = b.take(slots * 3 * sizeof(double));
+  const size_t o_nids = b.take(slots * kFgK * sizeof(uint32_t)), o_ncnt = b.take(slots * sizeof(uint32_t));
+  const size_t o_idx = b.take(slots * 2 * gs * sizeof(int32_t));
+  const size_t o_slot_l = ppl_log2 > 0 ? b.take(slots * 4 * sizeof(double)) : 0;
   const size_t o_rays = b.take(max_rays * sizeof(skw_ray)), o_w = b.take(max_rays * 3 * sizeof(double));
   const size_t o_hits = b.take(max_rays * sizeof(skw_hit));
   const size_t o_qp = b.take(max_rays * 3 * sizeof(double)), o_qn = b.take(max_rays * 3 * sizeof(double));
@@ -358,6 +370,8 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   a.counts = b.at<uint64_t>(o_counts);
   a.totals = b.at<uint64_t>(o_totals);
   a.film = b.at<double>(o_film);
+  a.ppl_log2 = ppl_log2;
+  a.slot_L = ppl_log2 > 0 ? b.at<double>(o_slot_l) : nullptr;
   f.s_pts = b.at<double>(o_spts);
   f.s_nrm = b.at<double>(o_snrm);
   f.near_ids = b.at<uint32_t>(o_nids);
@@ -390,11 +404,11 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
 
   hipError_t e = launch_dl_clear(a, st);
   skw_status rs = SKW_OK;
-  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; ++s) {
+  for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; s += 1u << ppl_log2) {
     for (uint64_t p0 = 0; p0 < npix && e == hipSuccess && rs == SKW_OK; p0 += batch) {
       a.pass = s;
       a.pix0 = p0;
-      a.batch = std::min(batch, npix - p0);
+      a.batch = std::min(batch, npix - p0) << ppl_log2;
       e = launch_dl_camera(a, st);  // also zeroes the gather-ray counter
       if (e != hipSuccess) break;
       rs = enqueue_trace(scene, a.prim_rays, a.batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr,
@@ -426,10 +440,12 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
         if (e != hipSuccess) break;
       }
       e = launch_fg_resolve(f, st);
+      if (e == hipSuccess && a.slot_L) e = launch_dl_accum(a, st);
     }
   }
   if (e == hipSuccess && rs == SKW_OK && d_rgb) {
     a.pix0 = 0;
+    a.ppl_log2 = 0;
     a.batch = npix;
     e = launch_dl_finish(a, d_rgb, st);
   }

@@ -599,9 +599,12 @@ __global__ __launch_bounds__(256) void k_fg_resolve(FgArgs f) {
   const uint64_t pix = pixel_of(a, i);
   for (int c = 0; c < 3; ++c) {
     const double L = rad[c] * inv;
-    a.film[3 * pix + c] += L;
+    // several passes per launch: the slot's value, added to the film in pass order by k_dl_accum
+    if (a.slot_L) a.slot_L[4 * i + c] = L;
+    else a.film[3 * pix + c] += L;
     if (f.radiance) f.radiance[3 * (pix * a.spp + pass_of(a, i)) + c] = L;
   }
+  if (a.slot_L) a.slot_L[4 * i + 3] = 0.0;  // the final gather drops no sample
 }
 
 // Per-sample sum and film accumulation. ADJ: instead of the film, accumulates the sample's

@@ -89,6 +89,29 @@ def test_final_gather_small_room_matches_oracle(torch_cuda, room):
     assert_close_rel(rgb, mean.astype(np.float32), rel=2e-6)
 
 
+def test_final_gather_launch_shape_changes_no_value(torch_cuda, room, monkeypatch):
+    # The same frame as pixel batches of 28 pixels with one pass per launch (2^10 gather rays) and
+    # as every pixel with all 4 passes in one launch: per-sample values and the image bit for bit.
+    print("case: final gather, small room 48x32, 4 spp, 2^10 gather rays per launch vs 4 passes per launch",
+          flush=True)
+    s, gpu, ref, area, irr, oirr = room
+    w, h, spp = 48, 32, 4
+    scene = sk.Scene(s["meshes"])
+    gdl = sk.DLScene(scene, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
+    cam, _ = sk.camera(*s["camera"], w, h)
+    out = []
+    for log2 in ("10", "27"):
+        monkeypatch.setenv("SKW_FG_GATHER_RAYS_LOG2", log2)
+        d_rad = empty_dev(torch_cuda, w * h * spp * 24)
+        d_rgb = empty_dev(torch_cuda, w * h * 12)
+        rays = gdl.final_gather(irr, gpu[1], area, cam, w, h, spp, 9, d_rgb.data_ptr(), d_rad.data_ptr(), seed=7)
+        torch_cuda.cuda.synchronize()
+        out.append((from_dev(d_rad, np.uint8, w * h * spp * 24), from_dev(d_rgb, np.uint8, w * h * 12), list(rays)))
+    assert out[0][2] == out[1][2] and out[0][2][1] > w * h * spp * 9
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_final_gather_without_indirect_photons(torch_cuda, room):
     print("case: final gather, no indirect map (BSDF samples only), 16 gather samples", flush=True)
     s, gpu, ref, area, irr, oirr = room
@@ -101,8 +124,10 @@ def _box(inward):
     return [wl.box(1.0, inward=inward)], np.zeros(1, np.uint8), np.full((1, 3), 0.6), np.zeros((1, 3)), np.full(1, 0.2)
 
 
-def test_final_gather_closed_box_over_pixel_batches(torch_cuda):
-    # 676 gather samples hold 6204 pixels per batch: an 80 x 80 image takes two batches.
+def test_final_gather_closed_box_over_pixel_batches(torch_cuda, monkeypatch):
+    # At most 2^23 gather rays per launch: 676 gather samples hold 6204 pixels per batch, so an
+    # 80 x 80 image takes two batches.
+    monkeypatch.setenv("SKW_FG_GATHER_RAYS_LOG2", "23")
     print("case: final gather, closed box 80x80, 676 gather samples (two pixel batches)", flush=True)
     e_ext, e_int = np.array([1.0, 0.5, 0.25], np.float32), np.array([0.3, 0.2, 0.1], np.float32)
     for inward, e in ((True, e_ext), (False, e_int)):
