@@ -529,6 +529,14 @@ skw_status skw_batch_rays(const skw_batch *batch, uint32_t scene, skw_ray *d_ray
 /* Total rays per frame. */
 skw_status skw_batch_ray_count(const skw_batch *batch, uint64_t *n);
 
+/* Stable sort of n (key, value) pairs of device uint32 arrays by key bits [begin_bit, end_bit):
+ * the library's own radix sort, which orders the final gather's irradiance lookups and a shooting
+ * round's photons. The inputs are not written; temporary memory comes from the library's pool.
+ * No counterpart in the reference (its lookups run in pixel order on the host); exported so that
+ * callers and tests can check it. */
+skw_status skw_sort_pairs_u32(const uint32_t *d_keys_in, uint32_t *d_keys_out, const uint32_t *d_vals_in,
+                              uint32_t *d_vals_out, uint64_t n, int begin_bit, int end_bit, void *stream);
+
 /* Thread-local text of the last error. */
 const char *skw_last_error(void);
 

@@ -41,7 +41,7 @@ EXPORTS = [
     "skw_photon_set_destroy", "skw_photon_map_build",
     "skw_scene_save", "skw_scene_load", "skw_photon_map_save", "skw_photon_map_load", "skw_irradiance_map_save",
     "skw_irradiance_map_load", "skw_pbrt_load", "skw_pbrt_descs", "skw_pbrt_warnings", "skw_pbrt_warning",
-    "skw_pbrt_destroy",
+    "skw_pbrt_destroy", "skw_sort_pairs_u32",
 ]
 
 _P = ctypes.c_void_p
@@ -158,6 +158,7 @@ def lib():
                                          _P], ctypes.c_int),
         "skw_last_error": ([], ctypes.c_char_p),
         "skw_device_count": ([], ctypes.c_int),
+        "skw_sort_pairs_u32": ([_P, _P, _P, _P, _U64, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
         "skw_dl_scene_create": ([_P, ctypes.POINTER(DLDesc), ctypes.POINTER(_P)], ctypes.c_int),
         "skw_dl_scene_destroy": ([_P], ctypes.c_int),
         "skw_dl_param_count": ([_P, _P], ctypes.c_int),
@@ -225,6 +226,12 @@ def check(status, what=""):
     if status != 0:
         msg = lib().skw_last_error().decode(errors="replace")
         raise SkwError("%s failed (status %d): %s" % (what, status, msg))
+
+
+def sort_pairs_u32(d_keys_in, d_keys_out, d_vals_in, d_vals_out, n, begin_bit=0, end_bit=32, stream=0):
+    """Stable sort of n (key, value) uint32 pairs (device pointers) by key bits [begin_bit, end_bit)."""
+    check(lib().skw_sort_pairs_u32(d_keys_in, d_keys_out, d_vals_in, d_vals_out, n, begin_bit, end_bit,
+                                   stream or None), "skw_sort_pairs_u32")
 
 
 def _ptr(a):

@@ -263,6 +263,27 @@ skw_photon_map::~skw_photon_map() {
 
 extern "C" {
 
+skw_status skw_sort_pairs_u32(const uint32_t *d_keys_in, uint32_t *d_keys_out, const uint32_t *d_vals_in,
+                              uint32_t *d_vals_out, uint64_t n, int begin_bit, int end_bit, void *stream) {
+  if (n > 0xFFFFFFFFull) return fail(SKW_INVALID_ARG, "at most 2^32 - 1 pairs");
+  if (begin_bit < 0 || end_bit > 32 || begin_bit > end_bit) return fail(SKW_INVALID_ARG, "bad key bit range");
+  if (n > 0 && (!d_keys_in || !d_keys_out || !d_vals_in || !d_vals_out))
+    return fail(SKW_INVALID_ARG, "arrays must not be NULL");
+  if (device_count() == 0) return fail(SKW_NO_DEVICE, "no HIP device is visible");
+  if (n == 0) return SKW_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  size_t bytes = 0;
+  SKW_HIP(sort_pairs_u32(nullptr, bytes, nullptr, nullptr, nullptr, nullptr, n, begin_bit, end_bit, st),
+          "sort workspace size");
+  void *tmp = nullptr;
+  SKW_HIP(pool_alloc(&tmp, bytes, st), "sort workspace");
+  const hipError_t e = sort_pairs_u32(tmp, bytes, d_keys_in, d_keys_out, d_vals_in, d_vals_out, n, begin_bit,
+                                      end_bit, st);
+  pool_free(tmp, st);
+  if (e != hipSuccess) return hip_fail(e, "radix sort");
+  return SKW_OK;
+}
+
 const char *skw_last_error(void) { return skw::last_error(); }
 
 int skw_device_count(void) { return device_count(); }
