@@ -64,10 +64,10 @@ __global__ void k_batch_reset(BatchArgs a) {
 constexpr uint32_t kRefillMin = 8;
 constexpr uint32_t kLeafMin = 16;
 
-// Marks len units of item done; the wave that completes the item records the wall clock.
-__device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint64_t len, uint64_t total,
-                                        uint32_t lane) {
-  if (lane == 0) {
+// Marks len units of item done (lead: the one lane of the wave that does it); the wave that
+// completes the item records the wall clock.
+__device__ __forceinline__ void account(const BatchArgs &a, uint32_t item, uint64_t len, uint64_t total, bool lead) {
+  if (lead) {
     unsigned long long old =
         atomicAdd(reinterpret_cast<unsigned long long *>(a.item_done + item), (unsigned long long)len);
     if (old + len == total) a.item_end[item] = wall_clock64();
@@ -92,19 +92,28 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
   // Every iteration moves the wave forward: with no lane busy it takes a chunk or starts rays;
   // otherwise a busy lane between leaves advances, or, when every busy lane holds a leaf
   // (moving == 0), the leaves are tested. The conditions below are ballots, the same in every
-  // lane, so an iteration that moves nothing can only mean that lanes and ballots disagree (a
-  // lane missing from the loop): the wave then records the stall and leaves instead of spinning.
+  // lane, so an iteration that moves nothing can only mean that lanes and ballots disagree: the
+  // wave then records the stall and leaves instead of spinning.
+  // Nothing here assumes that all 64 lanes run the iteration together: the compiler may let a
+  // subset of the lanes go round the loop on its own (a loop back edge threaded past a branch
+  // that only lane 0 takes did exactly that to an earlier form of this kernel, which then tested
+  // `idle == ~0ull` among 63 lanes forever). "Every lane idle" means every lane of `active`, the
+  // lanes running this iteration, and the claim's leader and broadcast come from that set, so
+  // such a subset takes and finishes chunks of its own.
   while (true) {
     bool moved = false;
+    const uint64_t active = __ballot(1);
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const bool lead = (int)lane == leader;
     const uint64_t idle = __ballot(!busy);
-    if (idle == ~0ull && cursor >= end) {
+    if (idle == active && cursor >= end) {
       moved = true;
       // The wave's chunk is done: account for it, then take the next one.
-      if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lane);
+      if (sid != 0xFFFFFFFFu) account(a, sid, chunk_len, sc->n_rays, lead);
       if (GATHER) sid = 0xFFFFFFFFu;  // a gather chunk taken next must not account this scene again
       unsigned long long q = 0;
-      if (lane == 0) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
-      q = __shfl(q, 0);
+      if (lead) q = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue_head), 1ull);
+      q = __shfl(q, leader);
       if (q >= a.n_chunks) break;
       const DevChunk c = a.chunks[q];
       if (GATHER) {
@@ -115,7 +124,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
           const GatherArgs &ga = a.jobs[item - a.n_scenes];
           const gath::LdsStack<S> gstk{stk.base};
           for (uint64_t qi = c.begin + lane; qi < c.end; qi += 64) gath::gather_one<0, false>(ga, qi, gstk);
-          account(a, item, c.end - c.begin, ga.nq, lane);
+          account(a, item, c.end - c.begin, ga.nq, lead);
           cursor = 0;
           end = 0;
           continue;
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     // Refill, leaf and advance phases as in the single-scene kernel (traverse.hip): refills and
     // leaf tests wait until kRefillMin / kLeafMin lanes need them, unless nothing else can move
     // or the chunk has no rays left to start.
-    if (idle != 0 && cursor < end && (idle == ~0ull || (uint32_t)__popcll(idle) >= kRefillMin)) {
+    if (idle != 0 && cursor < end && (idle == active || (uint32_t)__popcll(idle) >= kRefillMin)) {
       moved = true;
       const uint64_t avail = end - cursor;
       const uint64_t rank = (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
@@ -157,7 +166,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_trace(BatchArgs a) {
     const bool fast = sc->nested && __ballot(busy && !w.nan_free) == 0;
     moved = moved || __ballot(busy && !w.at_leaf) != 0;
     if (!moved) {
-      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(a.stalled), 1ull);
+      if (lead) atomicOr(reinterpret_cast<unsigned long long *>(a.stalled), 1ull);
       break;
     }
     if (busy && !w.at_leaf) {

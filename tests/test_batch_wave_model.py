@@ -13,8 +13,16 @@ at_leaf). The model runs the kernel's control flow over a chunk list shaped like
 
 It models both the current single loop and the earlier nested form (a per-chunk inner loop inside
 a lane-0 atomicAdd + shuffle claim loop) that was removed after a GPU run hung. Both terminate
-here for every chunk list and walk shape drawn, so the hang is not explained by the loop's control
-flow under wave-uniform ballots.
+here for every chunk list and walk shape drawn when all 64 lanes run every iteration together.
+
+The hang came from the compiled form of the nested loop, not from its source-level control flow:
+in the gfx950 code of that kernel the outer loop's back edge was threaded past the loop-invariant
+branch `lane == 0` (the claim's atomicAdd), so after the first chunk lanes 1..63 went straight
+back to the shuffle, without a claim, and ran a chunk's ray loop by themselves while lane 0 waited
+at the loop's exit. `test_compiled_nested_form_spins_without_lane_0` models that structure: the
+63-lane subset never sees lane 0's bit in `__ballot(!busy)`, so `idle == ~0ull` never holds and
+the loop never ends. The current kernel tests idle lanes against the active mask (`__ballot(1)`)
+and takes its leader from it, which the model shows ends the same subset's loop.
 """
 import random
 
@@ -241,3 +249,67 @@ def test_missing_lane_would_spin_and_the_guard_catches_it():
     claimed = idle == FULL and cursor >= end
     refill = idle != 0 and cursor < end
     assert not claimed and not refill
+
+
+def run_chunk_subset(lanes, b, e, walk_rng, exit_mask, max_iter=20000):
+    """The ray loop of one chunk [b, e) run by `lanes` only (the others are masked off: they are
+    missing from every ballot). exit_mask: the mask `idle` is compared with to leave the loop.
+    Returns the iterations taken, or None when the loop did not end within max_iter."""
+    busy = {l: False for l in lanes}
+    walk = {}
+    cursor = b
+    for it in range(max_iter):
+        idle = ballot([l in busy and not busy[l] for l in range(64)])
+        if idle == exit_mask and cursor >= e:
+            return it
+        if idle != 0 and cursor < e:
+            avail = e - cursor
+            for l in lanes:
+                rank = popc(idle & ((1 << l) - 1))
+                if not busy[l] and rank < avail:
+                    walk[l] = Walk(walk_rng)
+                    busy[l] = True
+            cursor += min(popc(idle), avail)
+        for l in lanes:
+            if busy[l]:
+                w = walk[l]
+                if w.at_leaf:
+                    w.leaf()
+                elif w.advance():
+                    busy[l] = False
+    return None
+
+
+@pytest.mark.parametrize("stale_q", [0, 1])
+def test_compiled_nested_form_spins_without_lane_0(stale_q):
+    """The removed kernel as compiled: all 64 lanes run the first chunk; then lanes 1..63 re-enter
+    the ray loop on a chunk index they did not claim (0, or the stale value of lane 0's register)
+    while lane 0 waits at the loop exit. With the `idle == ~0ull` exit the subset spins forever;
+    with the exit taken against the subset's active mask it ends."""
+    walk_rng = random.Random(7)
+    chunks = [(0, 0, 256), (0, 256, 512)]
+    assert run_chunk_subset(list(range(64)), 0, 256, walk_rng, FULL) is not None  # first chunk: whole wave
+    b, e = chunks[stale_q][1:]
+    subset = list(range(1, 64))
+    assert run_chunk_subset(subset, b, e, walk_rng, FULL) is None  # the hang
+    active = ballot([l in subset for l in range(64)])
+    assert run_chunk_subset(subset, b, e, walk_rng, active) is not None
+
+
+def test_subset_with_its_own_leader_claims_fresh_chunks():
+    """In the current loop the claim's leader is the first lane of `__ballot(1)`: a subset of the
+    wave that goes round the loop alone still claims its own chunks and finishes, so every chunk is
+    claimed and the queue drains (duplicated rays would be traced twice with identical results)."""
+    sh = Shared(*make_queue(random.Random(3), 4, 64))
+    walk_rng = random.Random(9)
+    for subset in (list(range(64)), list(range(1, 64)), [0]):
+        active = ballot([l in subset for l in range(64)])
+        leader = (active & -active).bit_length() - 1
+        assert leader == subset[0]
+        while True:
+            q = sh.claim()  # done by `leader`, broadcast from it
+            if q >= len(sh.chunks):
+                break
+            it, b, e = sh.chunks[q]
+            assert run_chunk_subset(subset, b, e, walk_rng, active) is not None
+    assert sh.head >= len(sh.chunks)
