@@ -364,6 +364,7 @@ void dl_tables(const skw_dl_scene *dl, const DevMat **mats, const double **rad) 
   *mats = dl->d_mats;
   *rad = dl->d_rad;
 }
+uint32_t dl_material_count(const skw_dl_scene *dl) { return (uint32_t)dl->mats.size(); }
 void dl_lights(const skw_dl_scene *dl, const std::vector<DevLight> **host, const DevLight **d_lights,
                const DevLightTri **d_ltris) {
   *host = &dl->lights;

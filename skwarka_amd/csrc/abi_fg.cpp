@@ -29,6 +29,8 @@ struct skw_dl_scene;  // defined in abi_dl.cpp; only its device tables are read,
 namespace skw {
 // Device tables of a direct-lighting scene (abi_dl.cpp).
 void dl_tables(const skw_dl_scene *dl, const DevMat **mats, const double **rad);
+// Entries of a direct-lighting scene's material table (one per primitive; abi_dl.cpp).
+uint32_t dl_material_count(const skw_dl_scene *dl);
 }  // namespace skw
 
 
@@ -357,6 +359,8 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   SKW_HIP(sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, max_rays, kKeyLo, kKeyHi, st),
           "sort workspace size");
   const size_t o_sort = b.take(sort_bytes);
+  const uint32_t n_mats = dl_material_count(dl);
+  const size_t o_quad = b.take((10 + 10 * (size_t)n_mats) * sizeof(double));
   const size_t o_counts = b.take(2 * sizeof(uint64_t)), o_totals = b.take(9 * sizeof(uint64_t));
   const size_t o_film = b.take(npix * 3 * sizeof(double));
   SKW_HIP(pool_alloc((void **)&b.base, b.off, st), "final gather buffers");
@@ -372,6 +376,8 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   a.film = b.at<double>(o_film);
   a.ppl_log2 = ppl_log2;
   a.slot_L = ppl_log2 > 0 ? b.at<double>(o_slot_l) : nullptr;
+  f.quad = b.at<double>(o_quad);
+  f.n_mats = n_mats;
   f.s_pts = b.at<double>(o_spts);
   f.s_nrm = b.at<double>(o_snrm);
   f.near_ids = b.at<uint32_t>(o_nids);
@@ -403,6 +409,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   uint32_t *near_ids = b.at<uint32_t>(o_nids), *near_count = b.at<uint32_t>(o_ncnt);
 
   hipError_t e = launch_dl_clear(a, st);
+  if (e == hipSuccess) e = launch_fg_quad(f, st);
   skw_status rs = SKW_OK;
   for (uint32_t s = 0; s < prm->spp && e == hipSuccess && rs == SKW_OK; s += 1u << ppl_log2) {
     for (uint64_t p0 = 0; p0 < npix && e == hipSuccess && rs == SKW_OK; p0 += batch) {

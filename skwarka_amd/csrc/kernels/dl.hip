@@ -466,16 +466,34 @@ __global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
 // ct^e cancels (ct >= 0.1^(1 / (e + 1)), so it never underflows), leaving
 // ks Fr(|c|) G |c| (e + 2) / ((e + 1) |cos_i|): the same sum, term by term, up to rounding, with
 // five pow and five sincos per hit instead of a hundred transcendentals.
-__device__ __forceinline__ void total_scattering(const Bsdf &b, D3 incident, double ts[3]) {
+// The quadrature's angles depend only on the sample index (azimuth) and the material's exponent
+// (cos_h), so k_fg_quad computes them once per call with the same expressions, and
+// total_scattering reads them: q = the azimuth table, qm = the material's cos_h / sin_h.
+__device__ __forceinline__ void quad_azimuths(double *q) {
+  for (int x = 0; x < 5; ++x) {
+    const double u = (x + 0.5) * (1.0 / 5);
+    sincos(u * 2.0 * kPi, &q[5 + x], &q[x]);
+  }
+}
+__device__ __forceinline__ void quad_heights(double e, double *qm) {
+  for (int x = 0; x < 5; ++x) {
+    const double u = (x + 0.5) * (1.0 / 5);
+    qm[x] = pow(u, 1.0 / (e + 1.0));
+    qm[5 + x] = sqrt(dmax(0.0, 1.0 - qm[x] * qm[x]));
+  }
+}
+
+__device__ __forceinline__ void total_scattering(const Bsdf &b, D3 incident, const double *__restrict__ q,
+                                                 const double *__restrict__ qm, double ts[3]) {
   double r[3] = {0.0, 0.0, 0.0};
   for (int c = 0; c < 3; ++c) r[c] = r[c] + b.kd[c];
   if (b.plastic) {
     double ct[5], st[5], cp[5], sp[5];
     for (int x = 0; x < 5; ++x) {
-      const double u = (x + 0.5) * (1.0 / 5);
-      ct[x] = pow(u, 1.0 / (b.e + 1.0));
-      st[x] = sqrt(dmax(0.0, 1.0 - ct[x] * ct[x]));
-      sincos(u * 2.0 * kPi, &sp[x], &cp[x]);
+      cp[x] = q[x];
+      sp[x] = q[5 + x];
+      ct[x] = qm[x];
+      st[x] = qm[5 + x];
     }
     const double ci = fabs(incident.z);
     const double kfac = (b.e + 2.0) / (b.e + 1.0);
@@ -535,7 +553,7 @@ __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
       b2.ks[c] = m.ks[c];
     }
     b2.e = m.exponent;
-    total_scattering(b2, gd, ts);
+    total_scattering(b2, gd, f.quad, f.quad + 10 + 10 * (size_t)h.prim, ts);
     side = dot3(gd, g2.ng) > 0.0 ? 1 : 0;
     qp = g2.p;
     qn = g2.ng;
@@ -565,6 +583,16 @@ __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
   }
 }
 
+// The quadrature table of total_scattering: one thread per material (plastic ones), thread
+// n_mats the azimuths.
+__global__ __launch_bounds__(256) void k_fg_quad(FgArgs f) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == f.n_mats) quad_azimuths(f.quad);
+  if (i >= f.n_mats) return;
+  const DevMat &m = f.a.mats[i];
+  if (m.type == 1) quad_heights(m.exponent, f.quad + 10 + 10 * (size_t)i);
+}
+
 // One thread per slot: the sample's gather rays in the reference's order (BSDF samples, then
 // cone samples), each adding TotalScattering x irradiance / pi x weight (:561-573); / gather
 // samples (Spectrum::operator/ multiplies by the reciprocal). Film and per-sample output.
@@ -581,7 +609,8 @@ __global__ __launch_bounds__(256) void k_fg_resolve(FgArgs f) {
   for (uint32_t j = 0; j < 2 * gs; ++j) {
     const int32_t r = f.fg_idx[i * 2 * gs + j];
     if (r < 0) continue;
-    if (f.hits[r].prim == kMiss) continue;
+    // A gather ray that missed has a NaN lookup point (k_fg_hit), which finds no irradiance
+    // photon: its id is kMiss, so the hit record need not be read.
     const uint32_t id = f.irr_id[r];
     if (id == kMiss) continue;
     const bool ext = f.side[r] != 0;
@@ -1074,6 +1103,11 @@ hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fg_samples<0>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
   hipLaunchKernelGGL(k_fg_samples<1>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fg_quad(const FgArgs &f, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fg_quad, dim3(f.n_mats / 256 + 1), dim3(256), 0, stream, f);
   return hipGetLastError();
 }
 

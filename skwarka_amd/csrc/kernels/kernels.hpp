@@ -231,10 +231,16 @@ struct FgArgs {
   double *radiance;               // 3 per sample (pixel * spp + s), optional
   uint32_t *keys, *order;         // per gather ray: cell key of its lookup point (misses last) and its index
   double lo[3], cell[3];          // scene box corner and cells per unit length (64 per axis)
+  // The plastic lobe's 5 x 5 quadrature (k_fg_quad): quad[0..4] cos and quad[5..9] sin of the
+  // five azimuths, then per material m quad[10 + 10 m ..] the five cos_h and quad[15 + 10 m ..]
+  // the five sin_h of its Blinn exponent.
+  double *quad;
+  uint32_t n_mats;
 };
 hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream);
+hipError_t launch_fg_quad(const FgArgs &f, hipStream_t stream);
 hipError_t launch_fg_resolve(const FgArgs &f, hipStream_t stream);
 // Photon shooting, one round of paths (shoot.hip). Active paths live in ping-pong lists: bounce
 // kernels read list `cur` (count counts[cur]) and append survivors to the other list.
