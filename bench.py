@@ -70,6 +70,18 @@ def cpu_info():
     return model
 
 
+def median_runs(fn, runs=5):
+    """CPU baseline protocol: one warm-up call, then `runs` timed calls; returns the median time
+    and the last call's result."""
+    fn()
+    times, out = [], None
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t)
+    return statistics.median(times), out
+
+
 def pmc_summary(name):
     """The committed rocprofv3 --pmc summary profiles/<name>.json (HBM bytes per launch, L2 hit
     rate, VALU figures), or {} when it is absent."""
@@ -407,14 +419,12 @@ def cpu_baseline_c5(c5, threads, rows):
     w = 1024
     rows = max(1, min(rows, 1024))
     r0 = 512 - rows // 2
-    t = time.perf_counter()
-    _, grays = oracle.fg_render(acc, dls, oirr, om[1], c5["area"], c5["cam15"], w, 1024, 4, 8, 1234, r0 * w,
-                                (r0 + rows) * w, nthreads=threads)
-    dt = time.perf_counter() - t
+    dt, (_, grays) = median_runs(lambda: oracle.fg_render(acc, dls, oirr, om[1], c5["area"], c5["cam15"], w, 1024, 4,
+                                                          8, 1234, r0 * w, (r0 + rows) * w, nthreads=threads))
     n = rows * w * 4 + grays
     return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "image rows %d-%d of the final-gather frame (%d pixels x 4 spp, %d camera + gather rays), "
-                      "one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
+                      "1 warm-up, median of 5 runs (%.2f s)" % (r0, r0 + rows - 1, rows * w, n, dt)}
 
 
 def bench_c6(torch, sk, wl, steps, warmup, stream, spec=None, scene=None):
@@ -531,14 +541,12 @@ def cpu_baseline_c7(c7, threads, rows):
                                 scene_area=c7["area"], nthreads=threads)
     w = 1024
     r0 = 512 - rows // 2
-    t = time.perf_counter()
-    _, cnt = oracle.pm_render(acc, dls, oirr, om[2], c7["area"], c7["cam15"], w, 1024, 16, 4, 4, 8, 1234, r0 * w,
-                              (r0 + rows) * w, nthreads=threads)
-    dt = time.perf_counter() - t
+    dt, (_, cnt) = median_runs(lambda: oracle.pm_render(acc, dls, oirr, om[2], c7["area"], c7["cam15"], w, 1024, 16,
+                                                        4, 4, 8, 1234, r0 * w, (r0 + rows) * w, nthreads=threads))
     n = int(cnt.sum())
     return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "image rows %d-%d of the photon-mapping frame (%d pixels x 16 spp, %d camera + shadow + BSDF + "
-                      "gather rays), one timed run of %.1f s" % (r0, r0 + rows - 1, rows * w, n, dt)}
+                      "gather rays), 1 warm-up, median of 5 runs (%.2f s)" % (r0, r0 + rows - 1, rows * w, n, dt)}
 
 
 def bench_c8(torch, sk, wl, steps, warmup, stream, spec=None):
@@ -602,12 +610,12 @@ def cpu_baseline_c6(c6, threads, n_paths):
     s = c6["spec"]
     acc = oracle.Accel(s["meshes"])
     dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
-    t = time.perf_counter()
-    out = oracle.photon_shoot(acc, dls, n_paths, (10 ** 12,) * 3, 1 << 20, 77, nthreads=threads)
-    dt = time.perf_counter() - t
+    dt, out = median_runs(lambda: oracle.photon_shoot(acc, dls, n_paths, (10 ** 12,) * 3, 1 << 20, 77,
+                                                      nthreads=threads))
     photons = sum(len(m["pos"]) for m in out["maps"])
     return {"value": n_paths / dt / 1e6, "unit": "M paths/s", "cores": threads, "kind": "port",
-            "sample": "the first %d of the 4M paths (%d photons), one timed run of %.1f s" % (n_paths, photons, dt)}
+            "sample": "the first %d of the 4M paths (%d photons), 1 warm-up, median of 5 runs (%.2f s)"
+                      % (n_paths, photons, dt)}
 
 
 def cpu_baseline_c3(c3, threads, n_sample):
@@ -616,21 +624,18 @@ def cpu_baseline_c3(c3, threads, n_sample):
     batch, scenes, specs = c3["batch"], c3["scenes"], c3["specs"]
     order = sorted(range(len(scenes)), key=lambda i: c3["specs"][i][2] ** 2 * c3["specs"][i][3])
     pick = [order[int(round(j * (len(order) - 1) / max(n_sample - 1, 1)))] for j in range(n_sample)]
-    rays_total, secs = 0, 0.0
+    work = []
     for i in pick:
         _, n = batch.hits(i)
         d = torch.empty(n * 64, dtype=torch.uint8, device="cuda")
         batch.rays(i, d.data_ptr())
         torch.cuda.synchronize()
-        rays = d.cpu().numpy().view(np.float64).reshape(n, 8)
-        acc = oracle.Accel(specs[i][0])
-        t = time.perf_counter()
-        acc.closest_hit(rays, nthreads=threads)
-        secs += time.perf_counter() - t
-        rays_total += n
+        work.append((oracle.Accel(specs[i][0]), d.cpu().numpy().view(np.float64).reshape(n, 8)))
+    rays_total = sum(len(r) for _, r in work)
+    secs, _ = median_runs(lambda: [acc.closest_hit(r, nthreads=threads) for acc, r in work])
     return {"value": rays_total / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "every camera ray of %d of the %d scenes (spread over frame sizes), %d rays, one timed run"
-                      % (len(pick), len(scenes), rays_total)}
+            "sample": "every camera ray of %d of the %d scenes (spread over frame sizes), %d rays; 1 warm-up, "
+                      "median of 5 runs" % (len(pick), len(scenes), rays_total)}
 
 
 def cpu_baseline_c4(c4, threads, rows):
@@ -644,13 +649,15 @@ def cpu_baseline_c4(c4, threads, rows):
     p0, p1 = r0 * w, (r0 + rows) * w
     th0 = c4["theta0"]
     target = c4["target"].cpu().numpy()[p0:p1]
-    t = time.perf_counter()
-    img = dls.render_replay(th0, th0, c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
-    dls.gradient(th0, 2.0 * (img - target), c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
-    dt = time.perf_counter() - t
+    def run():
+        img = dls.render_replay(th0, th0, c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
+        dls.gradient(th0, 2.0 * (img - target), c4["cam15"], w, 512, spp, 4, 4, 1234, p0, p1, nthreads=threads)
+
+    dt, _ = median_runs(run)
     return {"value": (rows / 512.0) / dt, "unit": "gradients/s", "cores": threads, "kind": "port",
             "sample": "image rows %d-%d of the 512x512 x 8 spp frame (forward image + FP64 forward-mode "
-                      "gradient of those rows), time scaled by 512/%d rows" % (r0, r0 + rows - 1, rows)}
+                      "gradient of those rows), 1 warm-up, median of 5 runs, time scaled by 512/%d rows"
+                      % (r0, r0 + rows - 1, rows)}
 
 
 def cpu_baseline_c2(c2, threads, rows):
@@ -659,16 +666,14 @@ def cpu_baseline_c2(c2, threads, rows):
     acc = oracle.Accel(s["meshes"])
     dls = oracle.DLScene(acc, s["mat"], s["kd"], s["ks"], s["rough"], s["light_prim"], s["light_radiance"])
     w, spp = 1024, 16
-    # rows around the image centre (all 1024 rows by default: the whole frame)
+    # rows around the image centre (the middle half of the frame by default)
     rows = max(1, min(rows, 1024))
     r0 = 512 - rows // 2
-    dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, 512 * w, 513 * w, nthreads=threads)  # warm-up
-    t = time.perf_counter()
-    _, cnt = dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + rows) * w, nthreads=threads)
-    dt = time.perf_counter() - t
+    dt, (_, cnt) = median_runs(lambda: dls.render(c2["cam15"], w, 1024, spp, 4, 4, 1234, r0 * w, (r0 + rows) * w,
+                                                  nthreads=threads))
     return {"value": float(cnt.sum()) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "image rows %d-%d of the c2 frame (%d pixels x 16 spp, %d rays), one timed run of %.1f s (%s)"
-                      % (r0, r0 + rows - 1, rows * w, int(cnt.sum()), dt, cpu_info())}
+            "sample": "image rows %d-%d of the c2 frame (%d pixels x 16 spp, %d rays), 1 warm-up, median of 5 runs "
+                      "(%.2f s; %s)" % (r0, r0 + rows - 1, rows * w, int(cnt.sum()), dt, cpu_info())}
 
 
 def cpu_baseline_c0(c0, threads):
@@ -712,8 +717,9 @@ def main():
     ap.add_argument("--only", choices=["c0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"], default=None,
                     help="run one workload (profiling runs); the printed line still names c2's metric only "
                          "when c2 runs")
-    ap.add_argument("--cpu-rows", type=int, default=1024,
-                    help="c2 image rows in the CPU baseline sample (1024: the whole frame)")
+    ap.add_argument("--cpu-rows", type=int, default=512,
+                    help="c2 image rows in the CPU baseline sample (1024: the whole frame; 1 warm-up + median of "
+                         "5 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (3 times): the reference "
                          "point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -803,13 +809,13 @@ def main():
         if c3 is not None:
             cpu3 = cpu_baseline_c3(c3, threads, 16)
         if c4 is not None:
-            cpu4 = cpu_baseline_c4(c4, threads, 64)
+            cpu4 = cpu_baseline_c4(c4, threads, 128)
         if c5 is not None:
-            cpu5 = cpu_baseline_c5(c5, threads, 256)
+            cpu5 = cpu_baseline_c5(c5, threads, 128)
         if c6 is not None:
             cpu6 = cpu_baseline_c6(c6, threads, 4_000_000)
         if c7 is not None:
-            cpu7 = cpu_baseline_c7(c7, threads, 64)
+            cpu7 = cpu_baseline_c7(c7, threads, 32)
         if c8 is not None:
             cpu8 = cpu_baseline_c8(c8)
     comps = {}
