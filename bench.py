@@ -596,13 +596,11 @@ def bench_c8(torch, sk, wl, steps, warmup, stream, spec=None):
 
 def cpu_baseline_c8(c8):
     import oracle
-    t = time.perf_counter()
-    acc = oracle.Accel(c8["spec"]["meshes"])
-    dt = time.perf_counter() - t
+    dt, acc = median_runs(lambda: oracle.Accel(c8["spec"]["meshes"]), runs=3)
     assert acc.info()["nodes"] == c8["info"]["nodes"]
     return {"value": c8["info"]["triangles"] / dt / 1e6, "unit": "M triangles/s", "cores": 1, "kind": "port",
-            "sample": "the whole room's tree built once by the oracle's serial restatement of the reference "
-                      "builder (%.1f s)" % dt}
+            "sample": "the whole room's tree built by the oracle's serial restatement of the reference builder; "
+                      "1 warm-up, median of 3 runs (%.1f s each)" % dt}
 
 
 def cpu_baseline_c6(c6, threads, n_paths):
@@ -697,15 +695,9 @@ def cpu_baseline_c1(c1, threads, sample):
     om = oracle.PhotonMap(ph["pos"], ph["weight"], ph["dir"], ph["normal"])
     sel = slice(0, min(sample, len(c1["p"])))
     p, n = c1["p"][sel], c1["n"][sel]
-    om.irradiance(p, n, 100, 0.05, with_sets=False, nthreads=threads)
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        om.irradiance(p, n, 100, 0.05, with_sets=False, nthreads=threads)
-        times.append(time.perf_counter() - t)
-    med = statistics.median(times)
+    med, _ = median_runs(lambda: om.irradiance(p, n, 100, 0.05, with_sets=False, nthreads=threads))
     return {"value": len(p) / med / 1e6, "unit": "M gathers/s", "cores": threads, "kind": "port",
-            "sample": "first %d of the c1 gather points; 1 warm-up, median of 3" % len(p)}
+            "sample": "first %d of the c1 gather points; 1 warm-up, median of 5 runs" % len(p)}
 
 
 def main():
