@@ -514,7 +514,7 @@ struct Estimate {
   double kc = 0.0;
   const double *wo = nullptr;
   double *caustic = nullptr;
-  bool ids_tree = false;  // ids hold tree positions instead of caller indices
+  int ids_tree = 0;  // ids hold caller indices (0), tree positions (1) or direction codes (2)
   const uint32_t *order = nullptr;  // visiting order of the queries (device), optional
   uint32_t *stats = nullptr;        // per query (visits, photons inside the radius, set size)
 };
@@ -543,7 +543,7 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
   a.kc = est.kc;
   a.wo = est.wo;
   a.caustic = est.caustic;
-  a.ids_tree = est.ids_tree ? 1 : 0;
+  a.ids_tree = est.ids_tree;
   a.order = est.order;
   a.stats = est.stats;
   // A disc estimate whose set is not returned writes the set's slots only if the set fills.
@@ -576,11 +576,11 @@ static skw_status gather(const skw_photon_map *map, const double *d_points, cons
 }  // extern "C"
 
 namespace skw {
-skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
-                           uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream) {
+skw_status photon_knn_codes(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                            uint32_t k, double r, uint32_t *d_codes, uint32_t *d_count, hipStream_t stream) {
   Estimate est;
-  est.ids_tree = true;
-  return gather(map, d_points, d_normals, nq, k, r, 1, 0.0, 1, 0, nullptr, nullptr, nullptr, d_ids, nullptr, d_count,
+  est.ids_tree = 2;
+  return gather(map, d_points, d_normals, nq, k, r, 1, 0.0, 1, 0, nullptr, nullptr, nullptr, d_codes, nullptr, d_count,
                 nullptr, stream, est);
 }
 

@@ -332,7 +332,6 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   f.gs = gs;
   f.gs_sqrt = gq;
   f.max_rays = max_rays;
-  f.ind_photons = n_ind > 0 ? indirect->d_photons : nullptr;
   f.cdir = n_ind > 0 ? indirect->d_cdir : nullptr;
   f.e_ext = irr->d_e_ext;
   f.e_int = irr->d_e_int;
@@ -425,7 +424,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
       if (e == hipSuccess) e = launch_fg_lookup_in(f, st);
       if (e != hipSuccess) break;
       if (n_ind > 0) {
-        rs = photon_knn_tree(indirect, f.s_pts, f.s_nrm, a.batch, kFgK, lookup_r, near_ids, near_count, st);
+        rs = photon_knn_codes(indirect, f.s_pts, f.s_nrm, a.batch, kFgK, lookup_r, near_ids, near_count, st);
         if (rs != SKW_OK) break;
       }
       e = launch_fg_samples(f, st);

@@ -153,9 +153,10 @@ skw_status upload_irradiance(skw_irradiance_map *m);  // device copies of the ir
 // The map's tree in host memory, downloaded on first use for device-built maps.
 const KdLayout &photon_map_host(const skw_photon_map *map);
 
-// k nearest photons with PhotonFilter within r, ids as tree positions (row-major, k per query).
-skw_status photon_knn_tree(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
-                           uint32_t k, double r, uint32_t *d_ids, uint32_t *d_count, hipStream_t stream);
+// k nearest photons with PhotonFilter within r, each given by its direction code (the low 16 bits
+// of its packed ids; row-major, k per query, in the bounded heap's final order).
+skw_status photon_knn_codes(const skw_photon_map *map, const double *d_points, const double *d_normals, uint64_t nq,
+                            uint32_t k, double r, uint32_t *d_codes, uint32_t *d_count, hipStream_t stream);
 // A photon map whose kd-tree is built on the device from device arrays (positions, weights, and
 // either codes = dir | normal << 16 or the two id arrays; weights / ids may be NULL).
 // The direction table of CompressedDirection on the current device, uploaded once and shared.

@@ -355,8 +355,7 @@ __global__ __launch_bounds__(256) void k_fg_lookup_in(FgArgs f) {
 }
 
 __device__ __forceinline__ D3 photon_dir(const FgArgs &f, uint64_t i, uint32_t j) {
-  const uint32_t pos = f.near_ids[i * 32 + j];
-  const double *dv = f.cdir + 3 * (size_t)(f.ind_photons[pos].codes & 0xFFFFu);
+  const double *dv = f.cdir + 3 * (size_t)f.near_ids[i * 32 + j];
   return D3{dv[0], dv[1], dv[2]};
 }
 

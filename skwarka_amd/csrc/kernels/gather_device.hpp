@@ -423,7 +423,10 @@ __device__ __forceinline__ void finish_query(const GatherArgs &a, const Query q,
   if (a.ids || a.d2) {
     for (uint32_t j = 0; j < size; ++j) {
       uint32_t pos = heap.p(j);
-      if (a.ids) a.ids[i * a.k + j] = a.ids_tree ? pos : a.perm[pos];
+      if (a.ids)
+        a.ids[i * a.k + j] = a.ids_tree == 2   ? (a.photons[pos].codes & 0xFFFFu)
+                             : a.ids_tree == 1 ? pos
+                                               : a.perm[pos];
       if (a.d2) a.d2[i * a.k + j] = heap.d(j);
     }
   }

@@ -79,8 +79,8 @@ struct GatherArgs {
   double *heap_d2;
   uint64_t q_stride, j_stride;
   // outputs (any may be null)
-  uint32_t *ids;        // k per query, caller ids (row-major); tree positions when ids_tree
-  int ids_tree;
+  uint32_t *ids;        // k per query (row-major): caller ids (ids_tree 0), tree positions (1) or the
+  int ids_tree;         // photons' direction codes, the low 16 bits of their packed ids (2)
   double *d2;           // k per query
   uint32_t *count;
   float *e_ext, *e_int, *radiance;
@@ -212,9 +212,8 @@ constexpr uint32_t kDlMaxParams = 2 * kDlMaxGroups + 16;
 struct FgArgs {
   DlArgs a;                       // camera, scene, materials, shading-point records, film
   uint32_t gs, gs_sqrt;           // gather samples (a square) and its side
-  const DevPhoton *ind_photons;   // indirect map payload in tree order (incident directions)
   const double *cdir;             // 65536 x 3 direction table
-  const uint32_t *near_ids;       // 32 per slot: the nearest indirect photons' tree positions
+  const uint32_t *near_ids;       // 32 per slot: the nearest indirect photons' direction codes
   const uint32_t *near_count;     // per slot (null: no indirect photons)
   double *s_pts, *s_nrm;          // 3 each per slot: shading point / geometric normal for the lookup
   int32_t *fg_idx;                // 2 gs per slot: gather ray index (BSDF samples, then cone samples) or -1
