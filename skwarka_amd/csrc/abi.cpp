@@ -182,6 +182,7 @@ GatherArgs gather_args(const skw_photon_map *map, const double *d_points, const 
   a.perm = map->d_perm;
   a.cdir = map->d_cdir;
   a.nbox = map->d_nbox;
+  a.near = map->d_near;
   a.n_nodes = map->n;
   a.points = d_points;
   a.normals = d_normals;
@@ -238,7 +239,8 @@ skw_status finish_photon_map(skw_photon_map *m) {
 skw_status photon_map_nbox(skw_photon_map *m, hipStream_t st) {
   if (m->n == 0) return SKW_OK;
   SKW_HIP(hipMalloc((void **)&m->d_nbox, m->n * sizeof(uint64_t)), "normal boxes");
-  SKW_HIP(launch_kd_nbox(m->d_nodes, m->d_photons, m->d_cdir, m->n, m->d_nbox, st), "normal boxes");
+  SKW_HIP(hipMalloc((void **)&m->d_near, m->n * sizeof(DevNearRec)), "nearest-lookup records");
+  SKW_HIP(launch_kd_nbox(m->d_nodes, m->d_photons, m->d_cdir, m->n, m->d_nbox, m->d_near, st), "normal boxes");
   SKW_HIP(hipStreamSynchronize(st), "normal boxes");
   return SKW_OK;
 }
@@ -259,6 +261,7 @@ skw_photon_map::~skw_photon_map() {
   }
   if (d_cdir && !cdir_shared) (void)hipFree(d_cdir);
   if (d_nbox) (void)hipFree(d_nbox);
+  if (d_near) (void)hipFree(d_near);
 }
 
 extern "C" {

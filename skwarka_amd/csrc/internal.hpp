@@ -92,6 +92,7 @@ struct skw_photon_map {
   uint32_t *d_perm = nullptr;
   double *d_cdir = nullptr;
   uint64_t *d_nbox = nullptr;  // per node: its subtree's normal box (irradiance maps; launch_kd_nbox)
+  skw::DevNearRec *d_near = nullptr;  // per node: the nearest-lookup record (irradiance maps)
   bool pooled = false;      // arrays from the device pool (device builds)
   bool cdir_shared = false; // d_cdir is the device's shared direction table
   ~skw_photon_map();

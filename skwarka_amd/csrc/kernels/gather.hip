@@ -73,7 +73,8 @@ constexpr uint64_t kNboxMax = 2048;  // larger subtrees get the whole cube (they
 // following right children (or the left child where there is no right one) down to a leaf.
 __global__ __launch_bounds__(256) void k_kd_nbox(const DevKdNode *__restrict__ nodes,
                                                  const DevPhoton *__restrict__ photons, const double *__restrict__ cdir,
-                                                 uint64_t n, uint64_t *__restrict__ nbox) {
+                                                 uint64_t n, uint64_t *__restrict__ nbox,
+                                                 DevNearRec *__restrict__ near) {
   const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   uint64_t cur = b;
@@ -111,14 +112,26 @@ __global__ __launch_bounds__(256) void k_kd_nbox(const DevKdNode *__restrict__ n
     box |= (uint64_t)(uint8_t)(int8_t)qh << (8 * (c + 3));
   }
   nbox[b] = box;
+  if (near) {
+    DevNearRec r;
+    r.node = reinterpret_cast<const float4 *>(nodes)[b];
+    const double *nv = cdir + 3 * (size_t)(photons[b].codes >> 16);
+    r.nv[0] = nv[0];
+    r.nv[1] = nv[1];
+    r.nv[2] = nv[2];
+    r.nbox = box;
+    r.pad[0] = 0;
+    r.pad[1] = 0;
+    near[b] = r;
+  }
 }
 }  // namespace
 
 hipError_t launch_kd_nbox(const DevKdNode *nodes, const DevPhoton *photons, const double *cdir, uint64_t n,
-                          uint64_t *nbox, hipStream_t stream) {
+                          uint64_t *nbox, DevNearRec *near, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_kd_nbox, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, nodes, photons, cdir, n,
-                     nbox);
+                     nbox, near);
   return hipGetLastError();
 }
 

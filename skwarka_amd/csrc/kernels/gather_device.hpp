@@ -141,6 +141,16 @@ __device__ __forceinline__ bool photon_filter(const GatherArgs &a, const Query &
   return fabs(along) < q.sin_t * sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// photon_filter with the photon's decoded normal given (a nearest-lookup record's copy of the
+// direction table's values): the same arithmetic.
+__device__ __forceinline__ bool photon_filter_nv(const Query &q, double nv0, double nv1, double nv2, double dx,
+                                                 double dy, double dz) {
+  double c = nv0 * q.n0 + nv1 * q.n1 + nv2 * q.n2;
+  if (!(c > kCosThreshold)) return false;
+  double along = dx * q.n0 + dy * q.n1 + dz * q.n2;
+  return fabs(along) < q.sin_t * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
 // True when every normal in the box fails the PhotonFilter's test (decoded normal . query normal
 // > 0.87): the dot product's upper bound over the box is below 0.87 by a margin far above the
 // rounding of either computation.
@@ -197,10 +207,26 @@ __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint3
                                       bool &far_exists, uint32_t &far, uint32_t &axis, float &split, bool &filled,
                                       Counts &cnt) {
   const uint32_t k = a.k;
-  // The node, its photon record and its subtree's normal box are independent loads, issued together.
-  const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
-  const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
-  if (MODE == 1 && !STATS && a.nbox && a.filter != 0 && normals_fail(a.nbox[cur], q)) {
+  // A nearest lookup over a map with records reads the node, the photon's normal and the subtree's
+  // normal box from one 64-byte record; otherwise the node, its photon record and the normal box
+  // are independent loads, issued together.
+  const bool rec = MODE == 1 && !STATS && a.near != nullptr;
+  float4 nd, ph = make_float4(0.f, 0.f, 0.f, 0.f);
+  double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
+  uint64_t box = 0;
+  if (rec) {
+    const DevNearRec &r = a.near[cur];
+    nd = r.node;
+    nv0 = r.nv[0];
+    nv1 = r.nv[1];
+    nv2 = r.nv[2];
+    box = r.nbox;
+  } else {
+    nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
+    ph = reinterpret_cast<const float4 *>(a.photons)[cur];
+    if (MODE == 1 && !STATS && a.nbox) box = a.nbox[cur];
+  }
+  if (MODE == 1 && !STATS && (rec || a.nbox) && a.filter != 0 && normals_fail(box, q)) {
     // no photon of this subtree passes the PhotonFilter's normal test: none can enter the result
     // (the statistics instance walks it, counting the reference's visits)
     far_exists = false;
@@ -212,7 +238,8 @@ __device__ __forceinline__ bool visit(const GatherArgs &a, const Query &q, uint3
   if (d2 < max_d2) {
     const uint32_t codes = __float_as_uint(ph.w);
     if (STATS) ++cnt.filtered;
-    bool pass = a.filter == 0 || photon_filter(a, q, codes, dx, dy, dz);
+    bool pass = a.filter == 0 || (rec ? photon_filter_nv(q, nv0, nv1, nv2, dx, dy, dz)
+                                      : photon_filter(a, q, codes, dx, dy, dz));
     if (pass) {
       if (MODE == 1) {
         nearest = cur;
@@ -486,11 +513,25 @@ __device__ __forceinline__ double near_path_bound(const GatherArgs &a, const Que
   bool any = false;
   uint32_t cur = 0;
   for (uint32_t depth = 0; depth < 64; ++depth) {
-    const float4 nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
-    const float4 ph = reinterpret_cast<const float4 *>(a.photons)[cur];
+    float4 nd;
+    bool from_rec = false;
+    double nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
+    uint32_t codes = 0;
+    if (a.near) {
+      const DevNearRec &r = a.near[cur];
+      nd = r.node;
+      nv0 = r.nv[0];
+      nv1 = r.nv[1];
+      nv2 = r.nv[2];
+      from_rec = true;
+    } else {
+      nd = reinterpret_cast<const float4 *>(a.nodes)[cur];
+      codes = __float_as_uint(reinterpret_cast<const float4 *>(a.photons)[cur].w);
+    }
     const double dx = (double)nd.x - q.q0, dy = (double)nd.y - q.q1, dz = (double)nd.z - q.q2;
     const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 < best && (a.filter == 0 || photon_filter(a, q, __float_as_uint(ph.w), dx, dy, dz))) {
+    if (d2 < best && (a.filter == 0 || (from_rec ? photon_filter_nv(q, nv0, nv1, nv2, dx, dy, dz)
+                                                        : photon_filter(a, q, codes, dx, dy, dz)))) {
       best = d2;
       any = true;
     }

@@ -55,8 +55,21 @@ hipError_t launch_hit_geometry(const GeomArgs &args, hipStream_t stream);
 // Per node: an outward-rounded box of the decoded normals of the photons in its subtree, as six
 // int8 components (min x y z, max x y z) over 127, in one uint64 (subtrees of more than 2,048
 // photons: the whole cube). Computed for the irradiance photon maps (built or loaded).
+// With `near` (not null) it also writes each tree position's nearest-lookup record.
+struct DevNearRec;
 hipError_t launch_kd_nbox(const DevKdNode *nodes, const DevPhoton *photons, const double *cdir, uint64_t n,
-                          uint64_t *nbox, hipStream_t stream);
+                          uint64_t *nbox, DevNearRec *near, hipStream_t stream);
+
+// A nearest lookup's view of one tree position of an irradiance map: the node (point, packed
+// word), the photon's decoded normal (the direction table's doubles) and its subtree's normal box,
+// in one 64-byte line, so that a visit makes one memory request and no dependent table read.
+struct DevNearRec {
+  float4 node;
+  double nv[3];
+  uint64_t nbox;
+  uint64_t pad[2];
+};
+static_assert(sizeof(DevNearRec) == 64, "one 64-byte line per record");
 
 struct GatherArgs {
   const DevKdNode *nodes;
@@ -102,6 +115,9 @@ struct GatherArgs {
   // optional per-node bounds of the subtree's photon normals (launch_kd_nbox); a nearest lookup
   // with the PhotonFilter does not walk a subtree none of whose photons can pass its normal test
   const uint64_t *nbox;
+  // optional per-node records of the same map (launch_kd_nbox): nearest lookups read them instead
+  // of the node, photon, normal-box and direction-table arrays
+  const DevNearRec *near;
   // statistics instance: per query (visits, photons inside the radius, final set size)
   uint32_t *stats;
 };
