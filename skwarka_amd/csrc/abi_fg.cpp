@@ -290,7 +290,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   // adds a pixel's samples in pass order); larger launches mean fewer launch tails.
   // Bytes held per sample slot and per gather ray (the buffers laid out below).
   const uint64_t per_slot = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 24 + 3 + 3 + 4) * sizeof(double) +
-                            kFgK * sizeof(uint32_t) + sizeof(uint32_t) + 2ull * gs * sizeof(int32_t);
+                            kFgK * sizeof(uint32_t) + sizeof(uint32_t);
   const uint64_t per_ray = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 3 + 3) * sizeof(double) + 1 +
                            5 * sizeof(uint32_t) + 16;  // + radix-sort workspace, about 8 B per key
   const uint64_t slot_bytes = per_slot + 2ull * gs * per_ray;
@@ -307,7 +307,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
            (npix << (ppl_log2 + 1)) * 2ull * gs <= rays_cap && (npix << (ppl_log2 + 1)) * slot_bytes <= free_b / 4)
       ++ppl_log2;
   const uint64_t slots = batch << ppl_log2;
-  const uint64_t max_rays = slots * 2ull * gs;
+  const uint64_t max_rays = (slots + kFgBlock - 1) / kFgBlock * kFgBlock * 2ull * gs;
 
   const DevMat *mats = nullptr;
   const double *rad = nullptr;
@@ -343,7 +343,6 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const size_t o_cdf = b.take(sizeof(double)), o_point = b.take(slots * 24 * sizeof(double));
   const size_t o_spts = b.take(slots * 3 * sizeof(double)), o_snrm = b.take(slots * 3 * sizeof(double));
   const size_t o_nids = b.take(slots * kFgK * sizeof(uint32_t)), o_ncnt = b.take(slots * sizeof(uint32_t));
-  const size_t o_idx = b.take(slots * 2 * gs * sizeof(int32_t));
   const size_t o_slot_l = ppl_log2 > 0 ? b.take(slots * 4 * sizeof(double)) : 0;
   const size_t o_rays = b.take(max_rays * sizeof(skw_ray)), o_w = b.take(max_rays * 3 * sizeof(double));
   const size_t o_hits = b.take(max_rays * sizeof(skw_hit));
@@ -381,7 +380,6 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   f.s_nrm = b.at<double>(o_snrm);
   f.near_ids = b.at<uint32_t>(o_nids);
   f.near_count = n_ind > 0 ? b.at<uint32_t>(o_ncnt) : nullptr;
-  f.fg_idx = b.at<int32_t>(o_idx);
   f.rays = b.at<skw_ray>(o_rays);
   f.w = b.at<double>(o_w);
   f.count = a.counts;
@@ -400,10 +398,19 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
     pool_free(b.base, st);
     return rs0;
   }
+  bool box_finite = true;
   for (int c = 0; c < 3; ++c) {
     const double ext = box[3 + c] - box[c];
     f.lo[c] = box[c];
     f.cell[c] = ext > 0.0 && std::isfinite(ext) ? 64.0 / ext : 0.0;
+    box_finite = box_finite && std::isfinite(box[c]);
+  }
+  // The ray of a sample that casts none: from below the scene box's lower corner, pointing away
+  // (finite inverse direction components, so its wave keeps the NaN-free slab test); with
+  // tmax < tmin it can accept no triangle wherever it starts.
+  for (int c = 0; c < 3; ++c) {
+    f.hole_o[c] = box_finite ? box[c] - 1.0 : 0.0;
+    f.hole_d[c] = -0.57735026918962584;
   }
   uint32_t *near_ids = b.at<uint32_t>(o_nids), *near_count = b.at<uint32_t>(o_ncnt);
 
@@ -415,6 +422,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
       a.pass = s;
       a.pix0 = p0;
       a.batch = std::min(batch, npix - p0) << ppl_log2;
+      f.slots_padded = (a.batch + kFgBlock - 1) / kFgBlock * kFgBlock;
       e = launch_dl_camera(a, st);  // also zeroes the gather-ray counter
       if (e != hipSuccess) break;
       rs = enqueue_trace(scene, a.prim_rays, a.batch, nullptr, a.prim_hits, nullptr, st, false, nullptr, nullptr,
@@ -429,13 +437,13 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
       }
       e = launch_fg_samples(f, st);
       if (e != hipSuccess) break;
-      rs = enqueue_trace(scene, f.rays, a.batch * 2ull * gs, f.count, f.hits, nullptr, st, false, nullptr, nullptr,
+      rs = enqueue_trace(scene, f.rays, f.slots_padded * 2ull * gs, nullptr, f.hits, nullptr, st, false, nullptr, nullptr,
                          false);
       if (rs != SKW_OK) break;
       e = launch_fg_hit(f, st);
       if (e != hipSuccess) break;
       if (irr->map) {
-        const uint64_t nq = a.batch * 2ull * gs;
+        const uint64_t nq = f.slots_padded * 2ull * gs;
         size_t bytes = sort_bytes;
         e = sort_pairs_u32(b.base + o_sort, bytes, f.keys, keys2, f.order, order2, nq, kKeyLo, kKeyHi, st);
         if (e != hipSuccess) break;

@@ -338,6 +338,15 @@ __device__ __forceinline__ double fg_rnd(uint64_t seed, uint64_t sample, uint32_
 
 constexpr double kCosGather = 0.9848;  // PhotonLTEIntegrator.cpp:396 (10 degrees)
 
+// Gather rays are stored sample-major within blocks of kFgBlock slots: sample j (0 .. 2 gs: BSDF
+// samples, then cone samples) of slot i is ray (i / B) B 2gs + j B + i % B. A wave of consecutive
+// rays then holds one sample stratum of 64 neighbouring slots, and the per-slot reads of the
+// resolve are consecutive across a wave's lanes. Slots past the batch (up to the block) hold rays
+// that cast nothing.
+__device__ __forceinline__ uint64_t fg_ray(uint64_t i, uint32_t j, uint32_t gs) {
+  return (i / kFgBlock) * (kFgBlock * 2ull * gs) + (uint64_t)j * kFgBlock + (i % kFgBlock);
+}
+
 __global__ __launch_bounds__(256) void k_fg_lookup_in(FgArgs f) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.a.batch) return;
@@ -375,9 +384,13 @@ __global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
   const DlArgs &a = f.a;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t gs = f.gs;
-  const uint64_t i = t / gs;
-  const uint32_t k = (uint32_t)(t - i * gs);
+  // thread t: sample k of slot i, consecutive threads on consecutive slots of a slot block
+  const uint64_t per_blk = (uint64_t)kFgBlock * gs;
+  const uint64_t blk = t / per_blk, rem = t - blk * per_blk;
+  const uint32_t k = (uint32_t)(rem / kFgBlock);
+  const uint64_t i = blk * kFgBlock + (rem % kFgBlock);
   const bool in = i < a.batch;
+  const bool slot_ok = t < f.slots_padded * gs;  // slots past the batch, up to the block, get holes
   Point pt;
   bool shade = in && load_point(a, i, pt);
   const uint32_t found = (shade && f.near_count) ? f.near_count[i] : 0u;
@@ -443,13 +456,19 @@ __global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
     }
     if (want) mint = next_min_t(a.geom.prims[pt.prim], pt.g, ex);
   }
-  const int32_t slot = append_slot_block(f.count, want);
-  if (in) f.fg_idx[i * 2 * gs + (uint64_t)KIND * gs + k] = slot;
-  if (want) {
-    put_ray(f.rays + slot, pt.p, ex, mint, __builtin_huge_val());
-    f.w[3 * (uint64_t)slot] = wv[0];
-    f.w[3 * (uint64_t)slot + 1] = wv[1];
-    f.w[3 * (uint64_t)slot + 2] = wv[2];
+  (void)append_slot_block(f.count, want);  // counts the gather rays cast
+  if (slot_ok) {
+    // A sample that casts no ray gets a ray that starts outside the scene box, points away from
+    // it and has tmax < tmin: its root test culls it, its record is a miss, its weight is 0.
+    const uint64_t g = fg_ray(i, KIND * gs + k, gs);
+    if (want)
+      put_ray(f.rays + g, pt.p, ex, mint, __builtin_huge_val());
+    else
+      put_ray(f.rays + g, D3{f.hole_o[0], f.hole_o[1], f.hole_o[2]}, D3{f.hole_d[0], f.hole_d[1], f.hole_d[2]}, 0.0,
+              -1.0);
+    f.w[3 * g] = wv[0];
+    f.w[3 * g + 1] = wv[1];
+    f.w[3 * g + 2] = wv[2];
   }
 }
 
@@ -527,18 +546,15 @@ __device__ __forceinline__ void total_scattering(const Bsdf &b, D3 incident, con
 // direction, and the irradiance lookup's point and geometric normal (:545-570).
 __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= f.max_rays) return;
+  if (g >= f.slots_padded * 2ull * f.gs) return;
   const DlArgs &a = f.a;
-  const bool live = g < *f.count;
   const double nan = __builtin_nan("");
   D3 qp{nan, nan, nan}, qn{0.0, 0.0, 1.0};
   double ts[3] = {0.0, 0.0, 0.0};
   uint8_t side = 0;
-  // Slots past the pass's gather rays get a NaN lookup point too: the irradiance lookup runs over
-  // every slot and a NaN point descends one path and finds nothing.
-  skw_hit h;
-  h.prim = kMiss;
-  if (live) h = f.hits[g];
+  // Misses (samples that cast no ray included) get a NaN lookup point: the irradiance lookup runs
+  // over every slot and a NaN point descends one path and finds nothing.
+  const skw_hit h = f.hits[g];
   if (h.prim != kMiss) {
     const skw_ray &r = f.rays[g];
     const D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
@@ -557,15 +573,14 @@ __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
     qp = g2.p;
     qn = g2.ng;
   }
-  if (live)
-    for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
+  for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
   f.q_pts[3 * g] = qp.x;
   f.q_pts[3 * g + 1] = qp.y;
   f.q_pts[3 * g + 2] = qp.z;
   f.q_nrm[3 * g] = qn.x;
   f.q_nrm[3 * g + 1] = qn.y;
   f.q_nrm[3 * g + 2] = qn.z;
-  if (live) f.side[g] = side;
+  f.side[g] = side;
   if (f.keys) {
     // 64^3 cells over the scene box, Morton-interleaved into bits 12-29; misses sort last
     uint32_t key = 0xFFFFFFFFu;
@@ -606,10 +621,9 @@ __global__ __launch_bounds__(256) void k_fg_resolve(FgArgs f) {
   const uint32_t gs = f.gs;
   double rad[3] = {0.0, 0.0, 0.0};
   for (uint32_t j = 0; j < 2 * gs; ++j) {
-    const int32_t r = f.fg_idx[i * 2 * gs + j];
-    if (r < 0) continue;
-    // A gather ray that missed has a NaN lookup point (k_fg_hit), which finds no irradiance
-    // photon: its id is kMiss, so the hit record need not be read.
+    const uint64_t r = fg_ray(i, j, gs);
+    // A gather ray that missed (or a sample that cast none) has a NaN lookup point (k_fg_hit),
+    // which finds no irradiance photon: its id is kMiss.
     const uint32_t id = f.irr_id[r];
     if (id == kMiss) continue;
     const bool ext = f.side[r] != 0;
@@ -1098,7 +1112,7 @@ hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream) {
 }
 
 hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream) {
-  const uint64_t n = f.a.batch * f.gs, B = kSampleBlock;
+  const uint64_t n = f.slots_padded * f.gs, B = kSampleBlock;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fg_samples<0>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
   hipLaunchKernelGGL(k_fg_samples<1>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
@@ -1111,7 +1125,7 @@ hipError_t launch_fg_quad(const FgArgs &f, hipStream_t stream) {
 }
 
 hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream) {
-  const uint32_t grid = (uint32_t)((f.max_rays + 255) / 256);
+  const uint32_t grid = (uint32_t)((f.slots_padded * 2ull * f.gs + 255) / 256);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fg_hit, dim3(grid), dim3(256), 0, stream, f);
   return hipGetLastError();

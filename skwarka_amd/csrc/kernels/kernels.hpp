@@ -225,6 +225,7 @@ constexpr uint32_t kDlMaxGroups = 64;
 constexpr uint32_t kDlMaxParams = 2 * kDlMaxGroups + 16;
 
 // Photon final gather over the camera hits of a direct-lighting pass (dl.hip).
+constexpr uint32_t kFgBlock = 64;  // slots per block of the gather rays' layout
 struct FgArgs {
   DlArgs a;                       // camera, scene, materials, shading-point records, film
   uint32_t gs, gs_sqrt;           // gather samples (a square) and its side
@@ -232,10 +233,12 @@ struct FgArgs {
   const uint32_t *near_ids;       // 32 per slot: the nearest indirect photons' direction codes
   const uint32_t *near_count;     // per slot (null: no indirect photons)
   double *s_pts, *s_nrm;          // 3 each per slot: shading point / geometric normal for the lookup
-  int32_t *fg_idx;                // 2 gs per slot: gather ray index (BSDF samples, then cone samples) or -1
-  skw_ray *rays;                  // gather rays
+  skw_ray *rays;                  // gather rays: 2 gs per slot (BSDF samples, then cone samples), sample-major
+                                  // within blocks of kFgBlock slots (dl.hip fg_ray)
+  uint64_t slots_padded;          // slots of this batch rounded up to kFgBlock
+  double hole_o[3], hole_d[3];    // a ray outside the scene box pointing away (samples that cast none)
   double *w;                      // 3 per gather ray: its weight
-  uint64_t *count;                // gather rays of this batch (device)
+  uint64_t *count;                // gather rays cast in this batch (device)
   uint64_t max_rays;              // slots * 2 gs
   skw_hit *hits;                  // gather hits
   double *q_pts, *q_nrm;          // 3 each per gather ray: irradiance lookup point / normal (NaN when missed)
