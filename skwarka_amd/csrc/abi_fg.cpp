@@ -292,7 +292,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const uint64_t per_slot = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 24 + 3 + 3 + 4) * sizeof(double) +
                             kFgK * sizeof(uint32_t) + sizeof(uint32_t);
   const uint64_t per_ray = sizeof(skw_ray) + sizeof(skw_hit) + (3 + 3 + 3 + 3) * sizeof(double) + 1 +
-                           5 * sizeof(uint32_t) + 16;  // + radix-sort workspace, about 8 B per key
+                           6 * sizeof(uint32_t) + 16;  // + radix-sort workspace, about 8 B per key
   const uint64_t slot_bytes = per_slot + 2ull * gs * per_ray;
   const int rays_log2 = std::min(28, std::max(10, env_int("SKW_FG_GATHER_RAYS_LOG2", kMaxGatherRaysLog2)));
   const uint64_t rays_cap = uint64_t(1) << rays_log2;
@@ -349,6 +349,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const size_t o_qp = b.take(max_rays * 3 * sizeof(double)), o_qn = b.take(max_rays * 3 * sizeof(double));
   const size_t o_ts = b.take(max_rays * 3 * sizeof(double)), o_side = b.take(max_rays);
   const size_t o_irr = b.take(max_rays * sizeof(uint32_t));
+  const size_t o_plastic = b.take(max_rays * sizeof(uint32_t)), o_pcount = b.take(sizeof(uint64_t));
   // irradiance lookups in spatial order: cell keys of the lookup points, radix-sorted with the
   // gather-ray indices
   const size_t o_keys = b.take(max_rays * sizeof(uint32_t)), o_keys2 = b.take(max_rays * sizeof(uint32_t));
@@ -375,6 +376,8 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   a.ppl_log2 = ppl_log2;
   a.slot_L = ppl_log2 > 0 ? b.at<double>(o_slot_l) : nullptr;
   f.quad = b.at<double>(o_quad);
+  f.plastic = b.at<uint32_t>(o_plastic);
+  f.plastic_count = b.at<uint64_t>(o_pcount);
   f.n_mats = n_mats;
   f.s_pts = b.at<double>(o_spts);
   f.s_nrm = b.at<double>(o_snrm);

@@ -18,6 +18,8 @@
 // Rays go through the scene's own traversal kernels. Compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "shade_device.hpp"
 
 namespace skw {
@@ -544,36 +546,55 @@ __device__ __forceinline__ void total_scattering(const Bsdf &b, D3 incident, con
 
 // One thread per gather ray: the hit's geometry, total reflection scattering toward the gather
 // direction, and the irradiance lookup's point and geometric normal (:545-570).
+// The total scattering of a gather hit toward the gather direction (k_fg_hit for matte surfaces,
+// k_fg_plastic for plastic ones).
+__device__ __forceinline__ void gather_scattering(const FgArgs &f, uint64_t g, uint32_t prim, double ts[3]) {
+  const skw_ray &r = f.rays[g];
+  const D3 gd{-r.d[0], -r.d[1], -r.d[2]};
+  Bsdf b2;
+  const DevMat &m = f.a.mats[prim];
+  b2.plastic = m.type == 1;
+  for (int c = 0; c < 3; ++c) {
+    b2.kd[c] = m.kd[c];
+    b2.ks[c] = m.ks[c];
+  }
+  b2.e = m.exponent;
+  total_scattering(b2, gd, f.quad, f.quad + 10 + 10 * (size_t)prim, ts);
+}
+
+// Plastic gather hits are listed (one atomic per block) and their 25-point quadrature runs in
+// k_fg_plastic with one listed hit per lane: in k_fg_hit the quadrature would run with only the
+// plastic lanes of each wave active.
 __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= f.slots_padded * 2ull * f.gs) return;
+  const bool in = g < f.slots_padded * 2ull * f.gs;
   const DlArgs &a = f.a;
   const double nan = __builtin_nan("");
   D3 qp{nan, nan, nan}, qn{0.0, 0.0, 1.0};
   double ts[3] = {0.0, 0.0, 0.0};
   uint8_t side = 0;
+  bool plastic = false;
   // Misses (samples that cast no ray included) get a NaN lookup point: the irradiance lookup runs
   // over every slot and a NaN point descends one path and finds nothing.
-  const skw_hit h = f.hits[g];
+  skw_hit h;
+  h.prim = kMiss;
+  if (in) h = f.hits[g];
   if (h.prim != kMiss) {
     const skw_ray &r = f.rays[g];
     const D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
     const HitGeom g2 = hit_geometry(a.geom, o, d, h.prim, h.tri);
     const D3 gd = mul3(d, -1.0);
-    Bsdf b2;
-    const DevMat &m = a.mats[h.prim];
-    b2.plastic = m.type == 1;
-    for (int c = 0; c < 3; ++c) {
-      b2.kd[c] = m.kd[c];
-      b2.ks[c] = m.ks[c];
-    }
-    b2.e = m.exponent;
-    total_scattering(b2, gd, f.quad, f.quad + 10 + 10 * (size_t)h.prim, ts);
+    plastic = a.mats[h.prim].type == 1;
+    if (!plastic) gather_scattering(f, g, h.prim, ts);
     side = dot3(gd, g2.ng) > 0.0 ? 1 : 0;
     qp = g2.p;
     qn = g2.ng;
   }
-  for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
+  const int32_t at = append_slot_block(f.plastic_count, plastic);
+  if (plastic) f.plastic[at] = (uint32_t)g;
+  if (!in) return;
+  if (!plastic)
+    for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
   f.q_pts[3 * g] = qp.x;
   f.q_pts[3 * g + 1] = qp.y;
   f.q_pts[3 * g + 2] = qp.z;
@@ -594,6 +615,17 @@ __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
     }
     f.keys[g] = key;
     f.order[g] = (uint32_t)g;
+  }
+}
+
+// The total scattering of the listed plastic gather hits (a grid-stride loop over the list).
+__global__ __launch_bounds__(256) void k_fg_plastic(FgArgs f) {
+  const uint64_t n = *f.plastic_count;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = f.plastic[t];
+    double ts[3];
+    gather_scattering(f, g, f.hits[g].prim, ts);
+    for (int c = 0; c < 3; ++c) f.ts[3 * g + c] = ts[c];
   }
 }
 
@@ -1127,7 +1159,13 @@ hipError_t launch_fg_quad(const FgArgs &f, hipStream_t stream) {
 hipError_t launch_fg_hit(const FgArgs &f, hipStream_t stream) {
   const uint32_t grid = (uint32_t)((f.slots_padded * 2ull * f.gs + 255) / 256);
   if (grid == 0) return hipSuccess;
+  hipError_t e = launch_set_counter(f.plastic_count, 0, stream);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fg_hit, dim3(grid), dim3(256), 0, stream, f);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // the list's length is read on the device: a grid of a few blocks per CU strides over it
+  hipLaunchKernelGGL(k_fg_plastic, dim3(std::min<uint32_t>(grid, 4096)), dim3(256), 0, stream, f);
   return hipGetLastError();
 }
 

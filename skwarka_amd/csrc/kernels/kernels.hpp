@@ -236,6 +236,8 @@ struct FgArgs {
   skw_ray *rays;                  // gather rays: 2 gs per slot (BSDF samples, then cone samples), sample-major
                                   // within blocks of kFgBlock slots (dl.hip fg_ray)
   uint64_t slots_padded;          // slots of this batch rounded up to kFgBlock
+  uint32_t *plastic;              // gather rays whose hit is plastic (k_fg_hit lists them for k_fg_plastic)
+  uint64_t *plastic_count;        // their number (device)
   double hole_o[3], hole_d[3];    // a ray outside the scene box pointing away (samples that cast none)
   double *w;                      // 3 per gather ray: its weight
   uint64_t *count;                // gather rays cast in this batch (device)
