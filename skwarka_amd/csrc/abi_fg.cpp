@@ -41,10 +41,15 @@ constexpr uint32_t kFgK = 32;          // indirect photons per final-gather look
 constexpr uint32_t kMaxGather = 676;   // 26^2: the sample stream has 4096 dims per sample, 6 per gather sample
 // Gather rays held per launch, at most (SKW_FG_GATHER_RAYS_LOG2 overrides); fewer when the
 // launch's buffers would exceed a quarter of the device's free memory. Ray indices within a launch
-// are int32.
-constexpr int kMaxGatherRaysLog2 = 28;
+// are int32. 2^27 rays hold about 21 GB; at 2^28 (about 41 GB) a c7 frame took 366 ms in the
+// first processes of a GPU session but 750 ms in later ones (the device's free memory unchanged),
+// while 2^27 and 2^26 stayed steady (375 ms, 388 ms) over repeated processes.
+constexpr int kMaxGatherRaysLog2 = 27;
 constexpr uint32_t kMaxPassesPerLaunchLog2 = 4;
-constexpr int kKeyLo = 12, kKeyHi = 30;  // key bits of the lookup points' 64^3 Morton cells
+// Bits per axis of the lookup points' Morton cells: 256^3 cells over the scene box, a 24-bit key
+// (three 8-bit sort passes). Finer cells keep a wave's 64 lookups closer together than 64^3 did
+// (c7 374.9 -> 361.7 ms); 512^3 gains nothing more.
+constexpr int kMortonBits = 8;
 
 double inf() { return std::numeric_limits<double>::infinity(); }
 
@@ -355,6 +360,8 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   const size_t o_keys = b.take(max_rays * sizeof(uint32_t)), o_keys2 = b.take(max_rays * sizeof(uint32_t));
   const size_t o_ord = b.take(max_rays * sizeof(uint32_t)), o_ord2 = b.take(max_rays * sizeof(uint32_t));
   size_t sort_bytes = 0;
+  const int mbits = kMortonBits;
+  const int kKeyLo = 32 - 3 * mbits, kKeyHi = 32;  // the key bits of the Morton cells
   SKW_HIP(sort_pairs_u32(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, max_rays, kKeyLo, kKeyHi, st),
           "sort workspace size");
   const size_t o_sort = b.take(sort_bytes);
@@ -376,6 +383,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   a.ppl_log2 = ppl_log2;
   a.slot_L = ppl_log2 > 0 ? b.at<double>(o_slot_l) : nullptr;
   f.quad = b.at<double>(o_quad);
+  f.morton_bits = (uint32_t)mbits;
   f.plastic = b.at<uint32_t>(o_plastic);
   f.plastic_count = b.at<uint64_t>(o_pcount);
   f.n_mats = n_mats;
@@ -405,7 +413,7 @@ skw_status skw_dl_final_gather(const skw_scene *scene, const skw_dl_scene *dl, c
   for (int c = 0; c < 3; ++c) {
     const double ext = box[3 + c] - box[c];
     f.lo[c] = box[c];
-    f.cell[c] = ext > 0.0 && std::isfinite(ext) ? 64.0 / ext : 0.0;
+    f.cell[c] = ext > 0.0 && std::isfinite(ext) ? (double)(1u << mbits) / ext : 0.0;
     box_finite = box_finite && std::isfinite(box[c]);
   }
   // The ray of a sample that casts none: from below the scene box's lower corner, pointing away

@@ -603,14 +603,17 @@ __global__ __launch_bounds__(256) void k_fg_hit(FgArgs f) {
   f.q_nrm[3 * g + 2] = qn.z;
   f.side[g] = side;
   if (f.keys) {
-    // 64^3 cells over the scene box, Morton-interleaved into bits 12-29; misses sort last
+    // 2^B cells per axis over the scene box, Morton-interleaved into the key's top 3B bits;
+    // misses sort last
     uint32_t key = 0xFFFFFFFFu;
     if (h.prim != kMiss) {
+      const uint32_t B = f.morton_bits, lo_bit = 32u - 3u * B;
+      const double top = (double)((1u << B) - 1u);
       const double c[3] = {(qp.x - f.lo[0]) * f.cell[0], (qp.y - f.lo[1]) * f.cell[1], (qp.z - f.lo[2]) * f.cell[2]};
       key = 0;
       for (int ax = 0; ax < 3; ++ax) {
-        const uint32_t v = (uint32_t)dmin(63.0, dmax(0.0, c[ax]));
-        for (int bit = 0; bit < 6; ++bit) key |= ((v >> bit) & 1u) << (12 + 3 * bit + ax);
+        const uint32_t v = (uint32_t)dmin(top, dmax(0.0, c[ax]));
+        for (uint32_t bit = 0; bit < B; ++bit) key |= ((v >> bit) & 1u) << (lo_bit + 3 * bit + ax);
       }
     }
     f.keys[g] = key;

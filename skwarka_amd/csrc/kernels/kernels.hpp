@@ -250,7 +250,8 @@ struct FgArgs {
   const float *e_ext, *e_int;     // irradiance photons' irradiance (creation order)
   double *radiance;               // 3 per sample (pixel * spp + s), optional
   uint32_t *keys, *order;         // per gather ray: cell key of its lookup point (misses last) and its index
-  double lo[3], cell[3];          // scene box corner and cells per unit length (64 per axis)
+  double lo[3], cell[3];          // scene box corner and cells per unit length (2^morton_bits per axis)
+  uint32_t morton_bits;           // bits per axis of the lookup points' Morton cells
   // The plastic lobe's 5 x 5 quadrature (k_fg_quad): quad[0..4] cos and quad[5..9] sin of the
   // five azimuths, then per material m quad[10 + 10 m ..] the five cos_h and quad[15 + 10 m ..]
   // the five sin_h of its Blinn exponent.
