@@ -365,104 +365,117 @@ __global__ __launch_bounds__(256) void k_fg_lookup_in(FgArgs f) {
   f.s_nrm[3 * i + 2] = n.z;
 }
 
-__device__ __forceinline__ D3 photon_dir(const FgArgs &f, uint64_t i, uint32_t j) {
-  const double *dv = f.cdir + 3 * (size_t)f.near_ids[i * 32 + j];
-  return D3{dv[0], dv[1], dv[2]};
+// The found photons' directions of a block's kFgBlock slots, staged in LDS once per block and read
+// by every sample of the slot: entry (j, c) of slot lane is dirs[(3 j + c) kFgBlock + lane], so a
+// wave's lanes (consecutive slots, one j) read consecutive doubles.
+constexpr uint32_t kFgSampleThreads = 256;
+struct FgDirs {
+  double v[32 * 3 * kFgBlock];
+};
+
+__device__ __forceinline__ D3 photon_dir(const FgDirs &d, uint32_t lane, uint32_t j) {
+  const double *p = d.v + 3 * j * kFgBlock + lane;
+  return D3{p[0], p[kFgBlock], p[2 * kFgBlock]};
 }
 
 // The photon-sampling pdf of direction ex: a 10-degree cone around each found photon's direction.
-__device__ __forceinline__ double photon_pdf(const FgArgs &f, uint64_t i, uint32_t found, D3 ex) {
+__device__ __forceinline__ double photon_pdf(const FgDirs &d, uint32_t lane, uint32_t found, D3 ex) {
   const double cone_pdf = 1.0 / (2.0 * kPi * (1.0 - kCosGather));  // UniformConePDF
   double pp = 0.0;
   for (uint32_t j = 0; j < found; ++j)
-    if (dot3(photon_dir(f, i, j), ex) > kCosGather) pp += cone_pdf;
+    if (dot3(photon_dir(d, lane, j), ex) > kCosGather) pp += cone_pdf;
   return pp * (found > 0 ? 1.0 / found : 0.0);
 }
 
-// One thread per (slot, sample k): the BSDF sample k (KIND 0, :470-500) or the cone sample k around
-// a found photon's direction (KIND 1, :502-532), and its gather ray.
-template <int KIND>
-__global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
+// The BSDF sample k (s < gs, :470-500) or the cone sample k = s - gs around a found photon's
+// direction (s >= gs, :502-532) of a shaded slot; returns whether it casts a gather ray.
+__device__ __forceinline__ bool fg_sample(const FgArgs &f, uint64_t i, uint32_t s, const Point &pt, const Bsdf &b,
+                                          uint32_t found, const FgDirs &d, uint32_t lane, D3 &ex, double wv[3]) {
   const DlArgs &a = f.a;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t gs = f.gs;
-  // thread t: sample k of slot i, consecutive threads on consecutive slots of a slot block
-  const uint64_t per_blk = (uint64_t)kFgBlock * gs;
-  const uint64_t blk = t / per_blk, rem = t - blk * per_blk;
-  const uint32_t k = (uint32_t)(rem / kFgBlock);
-  const uint64_t i = blk * kFgBlock + (rem % kFgBlock);
-  const bool in = i < a.batch;
-  const bool slot_ok = t < f.slots_padded * gs;  // slots past the batch, up to the block, get holes
-  Point pt;
-  bool shade = in && load_point(a, i, pt);
-  const uint32_t found = (shade && f.near_count) ? f.near_count[i] : 0u;
-  if (KIND == 1) shade = shade && found > 0;
-  bool want = false;
-  D3 ex{0, 0, 0};
-  double wv[3] = {0.0, 0.0, 0.0}, mint = 0.0;
-  if (shade) {
-    Bsdf b;
-    point_bsdf(a, i, pt.prim, b);
-    const uint64_t sidx = pixel_of(a, i) * a.spp + pass_of(a, i);
-    const uint32_t gq = f.gs_sqrt;
-    const double inv_n = 1.0 / gs, inv_q = 1.0 / gq;
-    const uint32_t x = k % gq, y = k / gq;
-    if (KIND == 0) {
-      // StratifiedSampling1D / 2D with jitter (SamplingRoutines.h:327-373): dims [0, gs), [gs, 3 gs)
-      const double uc = (k + fg_rnd(a.seed, sidx, k)) * inv_n;
-      const double u0 = (x + fg_rnd(a.seed, sidx, gs + 2 * k)) * inv_q;
-      const double u1 = (y + fg_rnd(a.seed, sidx, gs + 2 * k + 1)) * inv_q;
-      double bp, fv[3];
-      D3 wo_l, wi_l;
-      b.sample(pt.wo, u0, u1, uc, ex, bp, fv, wo_l, wi_l);
-      if (bp > 0.0 && !(fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0)) {
-        const double ppdf = photon_pdf(f, i, found, ex);
-        const double w = power_heuristic((double)gs, bp, (double)gs, ppdf);
-        const double kk = w * fabs(dot3(ex, b.n)) / bp;
-        for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
-        want = true;
-      }
-    } else {
-      // dims [3 gs, 4 gs) and [4 gs, 6 gs)
-      const double us = (k + fg_rnd(a.seed, sidx, 3 * gs + k)) * inv_n;
-      const double u0 = (x + fg_rnd(a.seed, sidx, 4 * gs + 2 * k)) * inv_q;
-      const double u1 = (y + fg_rnd(a.seed, sidx, 4 * gs + 2 * k + 1)) * inv_q;
-      const uint32_t sj = (uint32_t)(us * found);
-      const D3 e1 = photon_dir(f, i, sj);
-      // MathRoutines::CoordinateSystem (MathRoutines.h:298-313)
-      D3 e2;
-      if (fabs(e1.x) > fabs(e1.y)) {
-        const double inv = 1.0 / sqrt(e1.x * e1.x + e1.z * e1.z);
-        e2 = D3{-e1.z * inv, 0.0, e1.x * inv};
-      } else {
-        const double inv = 1.0 / sqrt(e1.y * e1.y + e1.z * e1.z);
-        e2 = D3{0.0, e1.z * inv, -e1.y * inv};
-      }
-      const D3 e3 = cross3(e1, e2);
-      // SamplingRoutines::UniformConeSampling (SamplingRoutines.h:300-310)
-      const double ct = (1.0 - u0) + u0 * kCosGather;
-      const double st = sqrt(1.0 - ct * ct);
-      const double phi = u1 * 2.0 * kPi;
-      const double lx = st * cos(phi), ly = st * sin(phi);
-      ex = add3(add3(mul3(e2, lx), mul3(e3, ly)), mul3(e1, ct));
-      double fv[3];
-      b.evaluate(pt.wo, ex, fv);
-      if (!(fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0)) {
-        const double bp = b.pdf(pt.wo, ex);
-        const double ppdf = photon_pdf(f, i, found, ex);
-        const double w = power_heuristic((double)gs, ppdf, (double)gs, bp);
-        const double kk = w * fabs(dot3(ex, b.n)) / ppdf;
-        for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
-        want = true;
-      }
-    }
-    if (want) mint = next_min_t(a.geom.prims[pt.prim], pt.g, ex);
+  const uint32_t gs = f.gs, gq = f.gs_sqrt;
+  const bool cone = s >= gs;
+  const uint32_t k = cone ? s - gs : s;
+  const uint64_t sidx = pixel_of(a, i) * a.spp + pass_of(a, i);
+  const double inv_n = 1.0 / gs, inv_q = 1.0 / gq;
+  const uint32_t x = k % gq, y = k / gq;
+  if (!cone) {
+    // StratifiedSampling1D / 2D with jitter (SamplingRoutines.h:327-373): dims [0, gs), [gs, 3 gs)
+    const double uc = (k + fg_rnd(a.seed, sidx, k)) * inv_n;
+    const double u0 = (x + fg_rnd(a.seed, sidx, gs + 2 * k)) * inv_q;
+    const double u1 = (y + fg_rnd(a.seed, sidx, gs + 2 * k + 1)) * inv_q;
+    double bp, fv[3];
+    D3 wo_l, wi_l;
+    b.sample(pt.wo, u0, u1, uc, ex, bp, fv, wo_l, wi_l);
+    if (!(bp > 0.0 && !(fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0))) return false;
+    const double ppdf = photon_pdf(d, lane, found, ex);
+    const double w = power_heuristic((double)gs, bp, (double)gs, ppdf);
+    const double kk = w * fabs(dot3(ex, b.n)) / bp;
+    for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
+    return true;
   }
-  (void)append_slot_block(f.count, want);  // counts the gather rays cast
-  if (slot_ok) {
+  if (found == 0) return false;
+  // dims [3 gs, 4 gs) and [4 gs, 6 gs)
+  const double us = (k + fg_rnd(a.seed, sidx, 3 * gs + k)) * inv_n;
+  const double u0 = (x + fg_rnd(a.seed, sidx, 4 * gs + 2 * k)) * inv_q;
+  const double u1 = (y + fg_rnd(a.seed, sidx, 4 * gs + 2 * k + 1)) * inv_q;
+  const uint32_t sj = (uint32_t)(us * found);
+  const D3 e1 = photon_dir(d, lane, sj);
+  // MathRoutines::CoordinateSystem (MathRoutines.h:298-313)
+  D3 e2;
+  if (fabs(e1.x) > fabs(e1.y)) {
+    const double inv = 1.0 / sqrt(e1.x * e1.x + e1.z * e1.z);
+    e2 = D3{-e1.z * inv, 0.0, e1.x * inv};
+  } else {
+    const double inv = 1.0 / sqrt(e1.y * e1.y + e1.z * e1.z);
+    e2 = D3{0.0, e1.z * inv, -e1.y * inv};
+  }
+  const D3 e3 = cross3(e1, e2);
+  // SamplingRoutines::UniformConeSampling (SamplingRoutines.h:300-310)
+  const double ct = (1.0 - u0) + u0 * kCosGather;
+  const double st = sqrt(1.0 - ct * ct);
+  const double phi = u1 * 2.0 * kPi;
+  const double lx = st * cos(phi), ly = st * sin(phi);
+  ex = add3(add3(mul3(e2, lx), mul3(e3, ly)), mul3(e1, ct));
+  double fv[3];
+  b.evaluate(pt.wo, ex, fv);
+  if (fv[0] == 0.0 && fv[1] == 0.0 && fv[2] == 0.0) return false;
+  const double bp = b.pdf(pt.wo, ex);
+  const double ppdf = photon_pdf(d, lane, found, ex);
+  const double w = power_heuristic((double)gs, ppdf, (double)gs, bp);
+  const double kk = w * fabs(dot3(ex, b.n)) / ppdf;
+  for (int c = 0; c < 3; ++c) wv[c] = fv[c] * kk;
+  return true;
+}
+
+// One block per kFgBlock slots, one lane per slot: the slot's point, BSDF and found photon
+// directions are loaded once, then the block's waves take the 2 gs samples in turn (wave w takes
+// samples w, w + 4, ...) and write their gather rays.
+__global__ __launch_bounds__(kFgSampleThreads) void k_fg_samples(FgArgs f) {
+  __shared__ FgDirs dirs;
+  __shared__ uint32_t wave_rays[kFgSampleThreads / 64];
+  const DlArgs &a = f.a;
+  const uint32_t gs = f.gs, lane = threadIdx.x % kFgBlock, wave = threadIdx.x / kFgBlock;
+  const uint64_t i = (uint64_t)blockIdx.x * kFgBlock + lane;
+  Point pt;
+  const bool shade = i < a.batch && load_point(a, i, pt);
+  const uint32_t found = (shade && f.near_count) ? f.near_count[i] : 0u;
+  for (uint32_t j = wave; j < found; j += kFgSampleThreads / kFgBlock) {
+    const double *dv = f.cdir + 3 * (size_t)f.near_ids[i * 32 + j];
+    for (int c = 0; c < 3; ++c) dirs.v[(3 * j + c) * kFgBlock + lane] = dv[c];
+  }
+  __syncthreads();
+  Bsdf b;
+  if (shade) point_bsdf(a, i, pt.prim, b);
+  uint32_t rays = 0;
+  for (uint32_t s = wave; s < 2 * gs; s += kFgSampleThreads / kFgBlock) {
+    D3 ex{0, 0, 0};
+    double wv[3] = {0.0, 0.0, 0.0}, mint = 0.0;
+    const bool want = shade && fg_sample(f, i, s, pt, b, found, dirs, lane, ex, wv);
+    if (want) mint = next_min_t(a.geom.prims[pt.prim], pt.g, ex);
+    rays += (uint32_t)__popcll(__ballot(want));
     // A sample that casts no ray gets a ray that starts outside the scene box, points away from
     // it and has tmax < tmin: its root test culls it, its record is a miss, its weight is 0.
-    const uint64_t g = fg_ray(i, KIND * gs + k, gs);
+    const uint64_t g = fg_ray(i, s, gs);
     if (want)
       put_ray(f.rays + g, pt.p, ex, mint, __builtin_huge_val());
     else
@@ -471,6 +484,14 @@ __global__ __launch_bounds__(kSampleBlock) void k_fg_samples(FgArgs f) {
     f.w[3 * g] = wv[0];
     f.w[3 * g + 1] = wv[1];
     f.w[3 * g + 2] = wv[2];
+  }
+  // the gather rays cast, one atomic per block
+  if (lane == 0) wave_rays[wave] = rays;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < kFgSampleThreads / kFgBlock; ++w) total += wave_rays[w];
+    if (total) atomicAdd(reinterpret_cast<unsigned long long *>(f.count), (unsigned long long)total);
   }
 }
 
@@ -1147,10 +1168,10 @@ hipError_t launch_fg_lookup_in(const FgArgs &f, hipStream_t stream) {
 }
 
 hipError_t launch_fg_samples(const FgArgs &f, hipStream_t stream) {
-  const uint64_t n = f.slots_padded * f.gs, B = kSampleBlock;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fg_samples<0>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
-  hipLaunchKernelGGL(k_fg_samples<1>, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, stream, f);
+  // slots_padded is a multiple of kFgBlock; at most 32 found photons per slot
+  const uint64_t blocks = f.slots_padded / kFgBlock;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fg_samples, dim3((uint32_t)blocks), dim3(kFgSampleThreads), 0, stream, f);
   return hipGetLastError();
 }
 
