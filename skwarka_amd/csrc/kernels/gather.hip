@@ -32,10 +32,15 @@ __global__ __launch_bounds__(64) void k_gather_global(GatherArgs a) {
 template <int MODE, bool STATS>
 hipError_t launch_mode(const GatherArgs &a, hipStream_t stream) {
   const uint32_t grid = (uint32_t)((a.nq + kWave - 1) / kWave);
-  // Nearest lookups keep few registers (54 VGPRs), so the LDS stack bounds their occupancy
-  // (32 entries: 8 KB per wave, 20 waves per CU): shallow trees get a shorter stack.
+  // Nearest lookups keep few registers (56 VGPRs: 36 waves per CU), so the LDS stack bounds their
+  // occupancy (32 entries: 8 KB per wave, 20 waves per CU; 24: 26; 20: 32; 18: 35): shallow trees
+  // get a shorter stack. k-nearest lookups hold 96 VGPRs (20 waves per CU) whatever the stack.
   if (MODE == 1 && a.stack_need <= 16)
     hipLaunchKernelGGL((k_gather_lds<MODE, 16, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
+  else if (MODE == 1 && a.stack_need <= 18)
+    hipLaunchKernelGGL((k_gather_lds<MODE, 18, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
+  else if (MODE == 1 && a.stack_need <= 20)
+    hipLaunchKernelGGL((k_gather_lds<MODE, 20, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   else if (MODE == 1 && a.stack_need <= 24)
     hipLaunchKernelGGL((k_gather_lds<MODE, 24, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   else if (a.stack_need <= 32)
