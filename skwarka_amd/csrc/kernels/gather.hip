@@ -29,26 +29,31 @@ __global__ __launch_bounds__(64) void k_gather_global(GatherArgs a) {
   gather_one<MODE, STATS>(a, i, stk);
 }
 
+template <int MODE, int S, bool STATS>
+hipError_t launch_lds(const GatherArgs &a, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_gather_lds<MODE, S, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
+  return hipGetLastError();
+}
+
+// The LDS stack holds stack_need entries per lane (4 bytes each), so it bounds occupancy together
+// with the registers: nearest lookups (MODE 1) hold 56 VGPRs (36 waves per CU), so shallow trees
+// get a short stack (32 entries: 8 KB per wave, 20 waves per CU; 24: 26; 20: 32; 18: 35; 16: 36).
+// k-nearest lookups (MODE 0) hold 96 VGPRs (20 waves per CU) whatever the stack. (An instance for
+// sets without an estimate holds 80 VGPRs; with a 24-entry stack it ran the final gather's 32-nearest
+// lookups at 24 waves per CU no faster.)
 template <int MODE, bool STATS>
 hipError_t launch_mode(const GatherArgs &a, hipStream_t stream) {
   const uint32_t grid = (uint32_t)((a.nq + kWave - 1) / kWave);
-  // Nearest lookups keep few registers (56 VGPRs: 36 waves per CU), so the LDS stack bounds their
-  // occupancy (32 entries: 8 KB per wave, 20 waves per CU; 24: 26; 20: 32; 18: 35): shallow trees
-  // get a shorter stack. k-nearest lookups hold 96 VGPRs (20 waves per CU) whatever the stack.
-  if (MODE == 1 && a.stack_need <= 16)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 16, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else if (MODE == 1 && a.stack_need <= 18)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 18, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else if (MODE == 1 && a.stack_need <= 20)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 20, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else if (MODE == 1 && a.stack_need <= 24)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 24, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else if (a.stack_need <= 32)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 32, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else if (a.stack_need <= 64)
-    hipLaunchKernelGGL((k_gather_lds<MODE, 64, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
-  else
-    hipLaunchKernelGGL((k_gather_global<MODE, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
+  const uint32_t need = a.stack_need;
+  if constexpr (MODE == 1) {
+    if (need <= 16) return launch_lds<MODE, 16, STATS>(a, grid, stream);
+    if (need <= 18) return launch_lds<MODE, 18, STATS>(a, grid, stream);
+    if (need <= 20) return launch_lds<MODE, 20, STATS>(a, grid, stream);
+    if (need <= 24) return launch_lds<MODE, 24, STATS>(a, grid, stream);
+  }
+  if (need <= 32) return launch_lds<MODE, 32, STATS>(a, grid, stream);
+  if (need <= 64) return launch_lds<MODE, 64, STATS>(a, grid, stream);
+  hipLaunchKernelGGL((k_gather_global<MODE, STATS>), dim3(grid), dim3(kWave), 0, stream, a);
   return hipGetLastError();
 }
 
