@@ -89,6 +89,21 @@ def test_final_gather_small_room_matches_oracle(torch_cuda, room):
     assert_close_rel(rgb, mean.astype(np.float32), rel=2e-6)
 
 
+def test_final_gather_odd_size_25_samples_matches_oracle(torch_cuda, room):
+    # 37 x 23 pixels at 3 spp (2553 slots: the last block of 64 slots is partly empty) with 25
+    # gather samples: each wave of the sample kernel takes 12 or 13 of the 50 BSDF and cone samples.
+    print("case: final gather, small room 37x23, 3 spp, 25 gather samples", flush=True)
+    s, gpu, ref, area, irr, oirr = room
+    w, h, spp = 37, 23, 3
+    got, rgb, rays, want, rrays = fg_both(torch_cuda, s, irr, oirr, gpu[1], ref[1], area, w, h, spp, 25, seed=11)
+    assert rays[0] == w * h * spp
+    assert rays[1] == rrays and rrays > w * h * spp * 25
+    assert want.max() > 0
+    assert_close_rel(got, want)
+    mean = want.reshape(w * h, spp, 3).mean(axis=1)
+    assert_close_rel(rgb, mean.astype(np.float32), rel=2e-6)
+
+
 def test_final_gather_launch_shape_changes_no_value(torch_cuda, room, monkeypatch):
     # The same frame as pixel batches of 28 pixels with one pass per launch (2^10 gather rays) and
     # as every pixel with all 4 passes in one launch: per-sample values and the image bit for bit.
