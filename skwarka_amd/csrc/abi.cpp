@@ -92,6 +92,7 @@ static int clamp_int(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi 
 constexpr int kTraceRefillMin = 8;
 constexpr int kTraceLeafMin = 16;
 constexpr int kTraceClaim = 64;  // rays per claim on the launch counter
+constexpr int kTraceWideWavesPerCu = 19;
 
 CounterPool::~CounterPool() {
   for (auto &kv : by_stream) (void)hipFree(kv.second);
@@ -116,9 +117,20 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
                          skw_hit *d_hits, uint8_t *d_occ, hipStream_t st, bool any, uint32_t *d_pops,
                          uint32_t *d_tests, bool fix, uint64_t *d_sums, const TraceStorage *fixed) {
   skw_scene *s = const_cast<skw_scene *>(scene);
-  const int waves_per_cu = env_int("SKW_TRACE_WAVES_PER_CU", 16);
+  const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
+  uint32_t need = s->host.stack_need;
   uint64_t waves = (n + 63) / 64;
-  uint64_t cap = (uint64_t)s->cus * (uint64_t)(waves_per_cu > 0 ? waves_per_cu : 16);
+  // A launch with work for kTraceWideWavesPerCu waves per CU or more, on a tree whose stack fits 32
+  // LDS entries, runs the walk compiled for five waves per SIMD at that many waves per CU (the LDS
+  // of 8,448 bytes per wave holds 19 per CU); smaller launches (one round of waves) run the
+  // instance whose registers do not spill, 16 waves per CU. SKW_TRACE_WAVES_PER_CU above 16 selects
+  // the five-wave instance for any launch whose stack fits.
+  const bool wide_fits = need <= 32 && !stats;
+  const bool big = waves >= (uint64_t)s->cus * kTraceWideWavesPerCu;
+  const int env_w = env_int("SKW_TRACE_WAVES_PER_CU", 0);
+  const int waves_per_cu = env_w > 0 ? env_w : (wide_fits && big ? kTraceWideWavesPerCu : 16);
+  const bool wide = wide_fits && waves_per_cu > 16;
+  uint64_t cap = (uint64_t)s->cus * (uint64_t)waves_per_cu;
   uint32_t grid = (uint32_t)(waves < cap ? waves : cap);
   if (grid == 0) grid = 1;
   TraceArgs a;
@@ -140,8 +152,6 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   a.leaf_min = (uint32_t)clamp_int(env_int("SKW_TRACE_LEAF_MIN", kTraceLeafMin), 1, 64);
   a.claim = (uint32_t)clamp_int(env_int("SKW_TRACE_CLAIM", kTraceClaim), 64, 4096);
   a.nested = s->host.nested ? 1u : 0u;
-  const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
-  uint32_t need = s->host.stack_need;
   if (fixed) {
     a.counter = fixed->counter;
     if (need > 128) {
@@ -158,8 +168,8 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
     }
   }
   SKW_HIP(launch_set_counter(a.counter, (uint64_t)grid * 64, st), "counter init");
-  hipError_t e = any ? launch_any_hit(a, s->has_inst, stats, need, grid, st)
-                     : launch_closest_hit(a, s->has_inst, stats, need, grid, st);
+  hipError_t e = any ? launch_any_hit(a, s->has_inst, stats, need, grid, wide, st)
+                     : launch_closest_hit(a, s->has_inst, stats, need, grid, wide, st);
   if (!fixed) pool_free(a.gstack, st);
   if (e != hipSuccess) return hip_fail(e, any ? "any_hit launch" : "closest_hit launch");
   if (!any && fix && s->needs_fix) {

@@ -31,10 +31,12 @@ struct TraceArgs {
   uint32_t nested;      // 1: the tree's child boxes lie inside their parents' (no NaN coordinate)
 };
 
-// Chooses the LDS stack size from stack_need (global-memory stack above 128 entries).
+// Chooses the LDS stack size from stack_need (global-memory stack above 128 entries). wide: a
+// walk without statistics on a stack of up to 32 entries runs the instance compiled for five waves
+// per SIMD (the grid then holds more than 16 waves per CU).
 hipError_t launch_closest_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
-                              hipStream_t stream);
-hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
+                              bool wide, hipStream_t stream);
+hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid, bool wide,
                           hipStream_t stream);
 hipError_t launch_set_counter(uint64_t *counter, uint64_t value, hipStream_t stream);
 

@@ -130,6 +130,17 @@ __global__ __launch_bounds__(64) void k_trace_lds(TraceArgs args) {
   run_lane<ANY, INST, STATS>(args, stk, slots);
 }
 
+// The same walk compiled for five resident waves per SIMD (96 VGPRs, 16 of them spilled to scratch,
+// where k_trace_lds holds 116 and four waves): for launches with work for many rounds of waves,
+// where the extra resident waves hide more of the walk's latency than the spills cost.
+template <bool ANY, bool INST>
+__global__ __launch_bounds__(64, 5) void k_trace_lds_wide(TraceArgs args) {
+  __shared__ uint32_t lds[32 * kWave];
+  __shared__ uint8_t slots[kWave * Walker<ANY, INST, false, LdsStack<32>>::kPoolMax];
+  LdsStack<32> stk{lds + (threadIdx.x & (kWave - 1))};
+  run_lane<ANY, INST, false>(args, stk, slots);
+}
+
 template <bool ANY, bool INST, bool STATS>
 __global__ __launch_bounds__(64) void k_trace_global(TraceArgs args) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -143,10 +154,14 @@ __global__ void k_set_counter(uint64_t *counter, uint64_t value) { *counter = va
 }  // namespace
 
 template <bool ANY, bool INST, bool STATS>
-static hipError_t launch_trace(const TraceArgs &args, uint32_t stack_need, uint32_t grid, hipStream_t stream) {
+static hipError_t launch_trace(const TraceArgs &args, uint32_t stack_need, uint32_t grid, bool wide,
+                               hipStream_t stream) {
   dim3 blk(kWave);
   if (stack_need <= 32) {
-    hipLaunchKernelGGL((k_trace_lds<ANY, INST, STATS, 32>), dim3(grid), blk, 0, stream, args);
+    if (!STATS && wide)
+      hipLaunchKernelGGL((k_trace_lds_wide<ANY, INST>), dim3(grid), blk, 0, stream, args);
+    else
+      hipLaunchKernelGGL((k_trace_lds<ANY, INST, STATS, 32>), dim3(grid), blk, 0, stream, args);
   } else if (stack_need <= 64) {
     hipLaunchKernelGGL((k_trace_lds<ANY, INST, STATS, 64>), dim3(grid), blk, 0, stream, args);
   } else if (stack_need <= 128) {
@@ -158,21 +173,21 @@ static hipError_t launch_trace(const TraceArgs &args, uint32_t stack_need, uint3
 }
 
 hipError_t launch_closest_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
-                              hipStream_t stream) {
+                              bool wide, hipStream_t stream) {
   if (stats)
-    return inst ? launch_trace<false, true, true>(args, stack_need, grid, stream)
-                : launch_trace<false, false, true>(args, stack_need, grid, stream);
-  return inst ? launch_trace<false, true, false>(args, stack_need, grid, stream)
-              : launch_trace<false, false, false>(args, stack_need, grid, stream);
+    return inst ? launch_trace<false, true, true>(args, stack_need, grid, false, stream)
+                : launch_trace<false, false, true>(args, stack_need, grid, false, stream);
+  return inst ? launch_trace<false, true, false>(args, stack_need, grid, wide, stream)
+              : launch_trace<false, false, false>(args, stack_need, grid, wide, stream);
 }
 
-hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid,
+hipError_t launch_any_hit(const TraceArgs &args, bool inst, bool stats, uint32_t stack_need, uint32_t grid, bool wide,
                           hipStream_t stream) {
   if (stats)
-    return inst ? launch_trace<true, true, true>(args, stack_need, grid, stream)
-                : launch_trace<true, false, true>(args, stack_need, grid, stream);
-  return inst ? launch_trace<true, true, false>(args, stack_need, grid, stream)
-              : launch_trace<true, false, false>(args, stack_need, grid, stream);
+    return inst ? launch_trace<true, true, true>(args, stack_need, grid, false, stream)
+                : launch_trace<true, false, true>(args, stack_need, grid, false, stream);
+  return inst ? launch_trace<true, true, false>(args, stack_need, grid, wide, stream)
+              : launch_trace<true, false, false>(args, stack_need, grid, wide, stream);
 }
 
 hipError_t launch_set_counter(uint64_t *counter, uint64_t value, hipStream_t stream) {

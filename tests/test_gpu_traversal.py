@@ -196,3 +196,15 @@ def test_deepest_tree_uses_global_stack(torch_cuda):
     brute = acc.brute_closest(rays)
     assert np.array_equal(ref["prim"], brute["prim"]) and np.array_equal(ref["tri"], brute["tri"])
     assert (ref["prim"] != MISS).mean() > 0.5
+
+
+def test_five_wave_instance_matches_oracle(torch_cuda, monkeypatch):
+    # SKW_TRACE_WAVES_PER_CU above 16 runs every launch whose stack fits 32 entries on the walk
+    # compiled for five waves per SIMD (the instance large launches take by default): the same
+    # closest hits and occlusion, bit for bit, on the cases above.
+    print("case: five-wave trace instance (C0, random windows, degenerate, instanced rays)", flush=True)
+    monkeypatch.setenv("SKW_TRACE_WAVES_PER_CU", "19")
+    test_closest_hit_c0_full(torch_cuda)
+    test_random_rays_many_spheres(torch_cuda)
+    test_axis_aligned_and_degenerate_rays(torch_cuda)
+    test_transformed_and_instanced_primitives(torch_cuda)
