@@ -946,7 +946,8 @@ def main():
                                            "lanes per instruction in 'pmc'; most node and triangle reads hit L2, so "
                                            "HBM traffic ('traffic') is a fraction of the algorithmic bytes and "
                                            "'frac' is not a DRAM figure",
-                                "kernel": "k_trace_lds (closest + any hit): per launch of %d sample passes, "
+                                "kernel": "k_trace_lds_wide (closest + any hit; the walk compiled for five waves "
+                                          "per SIMD): per launch of %d sample passes, "
                                           "one primary, one occlusion and one BSDF trace" % (16 // c2["launches"]),
                                 "launches_per_frame": 3 * c2["launches"],
                                 "alg_bytes_per_launch": c2["alg_bytes"] // (3 * c2["launches"]),
