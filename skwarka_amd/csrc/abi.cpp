@@ -92,7 +92,7 @@ static int clamp_int(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi 
 constexpr int kTraceRefillMin = 8;
 constexpr int kTraceLeafMin = 16;
 constexpr int kTraceClaim = 64;  // rays per claim on the launch counter
-constexpr int kTraceWideWavesPerCu = 19;
+constexpr int kTraceWideWavesPerCu = 19;  // five-wave instance, 32-entry stacks (20 for 24)
 
 CounterPool::~CounterPool() {
   for (auto &kv : by_stream) (void)hipFree(kv.second);
@@ -120,15 +120,17 @@ skw_status enqueue_trace(const skw_scene *scene, const skw_ray *d_rays, uint64_t
   const bool stats = d_pops != nullptr || d_tests != nullptr || d_sums != nullptr;
   uint32_t need = s->host.stack_need;
   uint64_t waves = (n + 63) / 64;
-  // A launch with work for kTraceWideWavesPerCu waves per CU or more, on a tree whose stack fits 32
-  // LDS entries, runs the walk compiled for five waves per SIMD at that many waves per CU (the LDS
-  // of 8,448 bytes per wave holds 19 per CU); smaller launches (one round of waves) run the
-  // instance whose registers do not spill, 16 waves per CU. SKW_TRACE_WAVES_PER_CU above 16 selects
-  // the five-wave instance for any launch whose stack fits.
+  // A launch with work for that many waves per CU or more, on a tree whose stack fits 32 LDS
+  // entries, runs the walk compiled for five waves per SIMD at 20 waves per CU (stacks of up to 24
+  // entries, 6,400 bytes of LDS per wave) or 19 (up to 32 entries: 8,448 bytes per wave, of which
+  // the CU holds 19); smaller launches (one round of waves) run the instance whose registers do not
+  // spill, 16 waves per CU. SKW_TRACE_WAVES_PER_CU above 16 selects the five-wave instance for any
+  // launch whose stack fits.
   const bool wide_fits = need <= 32 && !stats;
-  const bool big = waves >= (uint64_t)s->cus * kTraceWideWavesPerCu;
+  const int wide_waves = need <= 24 ? kTraceWideWavesPerCu + 1 : kTraceWideWavesPerCu;
+  const bool big = waves >= (uint64_t)s->cus * (uint64_t)wide_waves;
   const int env_w = env_int("SKW_TRACE_WAVES_PER_CU", 0);
-  const int waves_per_cu = env_w > 0 ? env_w : (wide_fits && big ? kTraceWideWavesPerCu : 16);
+  const int waves_per_cu = env_w > 0 ? env_w : (wide_fits && big ? wide_waves : 16);
   const bool wide = wide_fits && waves_per_cu > 16;
   uint64_t cap = (uint64_t)s->cus * (uint64_t)waves_per_cu;
   uint32_t grid = (uint32_t)(waves < cap ? waves : cap);

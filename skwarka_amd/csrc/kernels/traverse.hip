@@ -133,11 +133,13 @@ __global__ __launch_bounds__(64) void k_trace_lds(TraceArgs args) {
 // The same walk compiled for five resident waves per SIMD (96 VGPRs, 16 of them spilled to scratch,
 // where k_trace_lds holds 116 and four waves): for launches with work for many rounds of waves,
 // where the extra resident waves hide more of the walk's latency than the spills cost.
-template <bool ANY, bool INST>
+// S = 24 (6,400 bytes of LDS per wave) lets all 20 such waves of a CU be resident; S = 32 (8,448
+// bytes) holds 19.
+template <bool ANY, bool INST, int S>
 __global__ __launch_bounds__(64, 5) void k_trace_lds_wide(TraceArgs args) {
-  __shared__ uint32_t lds[32 * kWave];
-  __shared__ uint8_t slots[kWave * Walker<ANY, INST, false, LdsStack<32>>::kPoolMax];
-  LdsStack<32> stk{lds + (threadIdx.x & (kWave - 1))};
+  __shared__ uint32_t lds[S * kWave];
+  __shared__ uint8_t slots[kWave * Walker<ANY, INST, false, LdsStack<S>>::kPoolMax];
+  LdsStack<S> stk{lds + (threadIdx.x & (kWave - 1))};
   run_lane<ANY, INST, false>(args, stk, slots);
 }
 
@@ -158,8 +160,10 @@ static hipError_t launch_trace(const TraceArgs &args, uint32_t stack_need, uint3
                                hipStream_t stream) {
   dim3 blk(kWave);
   if (stack_need <= 32) {
-    if (!STATS && wide)
-      hipLaunchKernelGGL((k_trace_lds_wide<ANY, INST>), dim3(grid), blk, 0, stream, args);
+    if (!STATS && wide && stack_need <= 24)
+      hipLaunchKernelGGL((k_trace_lds_wide<ANY, INST, 24>), dim3(grid), blk, 0, stream, args);
+    else if (!STATS && wide)
+      hipLaunchKernelGGL((k_trace_lds_wide<ANY, INST, 32>), dim3(grid), blk, 0, stream, args);
     else
       hipLaunchKernelGGL((k_trace_lds<ANY, INST, STATS, 32>), dim3(grid), blk, 0, stream, args);
   } else if (stack_need <= 64) {
