@@ -946,7 +946,7 @@ def main():
                                            "lanes per instruction in 'pmc'; most node and triangle reads hit L2, so "
                                            "HBM traffic ('traffic') is a fraction of the algorithmic bytes and "
                                            "'frac' is not a DRAM figure",
-                                "kernel": "k_trace_lds_wide (closest + any hit; the walk compiled for five waves "
+                                "kernel": "k_trace_lds_wide<.., 24> (closest + any hit; the walk compiled for five waves "
                                           "per SIMD): per launch of %d sample passes, "
                                           "one primary, one occlusion and one BSDF trace" % (16 // c2["launches"]),
                                 "launches_per_frame": 3 * c2["launches"],
